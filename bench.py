@@ -1,0 +1,231 @@
+"""Benchmark: SPH<DFSPH> dam break, 1,048,576 particles, one MI355X.
+
+Prints one JSON line (rank 0). A step is one SPH<DFSPH>::forward_step of the
+whole scene (grid build, density, 4 divergence + 6 density iterations) with
+every input resident in HBM. `value` is particle-steps/s over the timed steps;
+`roofline` is the dominant pass, its algorithmic bytes per launch over its
+average launch time (HIP events on the solver stream during the timed steps);
+`cpu_baseline` is the FP32 oracle (OpenMP over particles, the same
+reference-order arithmetic) timed on this host over one step of the same scene.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+HBM_PEAK_GBS = 8000.0
+METRIC = "particle-steps/s (1M DFSPH, 1 GPU, %HBM peak); fwd+adjoint/fwd time ratio"
+
+
+def parse_args(argv=None):
+    p = argparse.ArgumentParser(description=__doc__)
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=20)
+    p.add_argument("--warmup", type=int, default=3)
+    p.add_argument("--workload", choices=["dfsph", "pbf"], default="dfsph")
+    p.add_argument("--no-cpu-baseline", action="store_true", help="skip the CPU oracle timing")
+    p.add_argument("--no-pass-timing", action="store_true", help="time without per-pass events")
+    return p.parse_args(argv)
+
+
+def dfsph_bytes_per_step(iterations, particles, cells):
+    """Minimal-round-trip algorithmic bytes per particle per DFSPH step:
+    384 + 124 I_div + 140 I_den + 8 C / N."""
+    nd, ne = iterations
+    return 384.0 + 124.0 * nd + 140.0 * ne + 8.0 * cells / particles
+
+
+def pbf_bytes_per_step(iterations, particles, cells):
+    """PBF: 252 + 108 I + 8 C / N."""
+    return 252.0 + 108.0 * iterations + 8.0 * cells / particles
+
+
+def traffic_for(kernel: str):
+    """Measured HBM bytes per launch of `kernel` from the committed rocprofv3
+    PMC summary (FETCH_SIZE + WRITE_SIZE, copy-kernel calibrated), if any."""
+    path = ROOT / "profiles" / "traffic.json"
+    if not path.exists():
+        return None
+    try:
+        data = json.loads(path.read_text())
+    except ValueError:
+        return None
+    entry = data.get(kernel)
+    return None if entry is None else float(entry["bytes_per_launch"])
+
+
+def setup_dfsph(device):
+    import torch
+
+    from physica_amd import scenes
+    from physica_amd.particle import DFSPH, SPH
+
+    sc = scenes.dfsph_dam_break()
+    cfg = DFSPH.Configuration(divergence_iterations=4, density_iterations=6)
+    solver = SPH(sc.domain, cfg, device=device)
+    a, b = solver.allocate_state(), solver.allocate_state()
+    control = solver.allocate_control()
+    params = solver.allocate_parameters()
+    cache = solver.allocate_step_cache()
+    a.particles.positions.upload(sc.positions)
+    a.particles.velocities.upload(sc.velocities)
+    for name in ("masses", "rest_densities", "viscosities", "surface_tensions"):
+        getattr(params.particles, name).copy_(torch.from_numpy(sc.parameters[name]))
+    params.method.divergence_relaxation.copy_(torch.from_numpy(sc.parameters["divergence_relaxation"]))
+    params.method.density_relaxation.copy_(torch.from_numpy(sc.parameters["density_relaxation"]))
+    return sc, solver, [a, b], control, params, cache
+
+
+def setup_pbf(device):
+    import torch
+
+    from physica_amd import scenes
+    from physica_amd.particle import PBF
+
+    sc = scenes.pbf_dam_break()
+    solver = PBF(sc.domain, PBF.Configuration(pressure_iterations=4), device=device)
+    a, b = solver.allocate_state(), solver.allocate_state()
+    control = solver.allocate_control()
+    params = solver.allocate_parameters()
+    cache = solver.allocate_step_cache()
+    a.particles.positions.upload(sc.positions)
+    a.particles.velocities.upload(sc.velocities)
+    for name in ("masses", "rest_densities", "viscosities", "surface_tensions"):
+        getattr(params.particles, name).copy_(torch.from_numpy(sc.parameters[name]))
+    for name in ("relaxation", "artificial_pressure_strength", "artificial_pressure_exponent",
+                 "artificial_pressure_radius", "xsph_viscosity", "vorticity_confinement"):
+        getattr(params, name).copy_(torch.from_numpy(sc.parameters[name]))
+    return sc, solver, [a, b], control, params, cache
+
+
+def cpu_baseline(workload, sc):
+    """The FP32 oracle on this host: one step of the same scene, all threads."""
+    import oracle
+
+    d = oracle.make_domain(sc.domain)
+    zeros = np.zeros_like(sc.positions)
+    t0 = time.perf_counter()
+    if workload == "dfsph":
+        z = np.zeros(sc.count, np.float32)
+        oracle.dfsph_step(d, 0, sc.positions, sc.velocities, zeros, sc.parameters, z, z, 4, 6)
+        sample = "1 step of the 1,048,576-particle DFSPH dam break (FP32 oracle, OpenMP over particles)"
+    else:
+        oracle.pbf_step(d, 0, sc.positions, sc.velocities, zeros, sc.parameters, 4)
+        sample = "1 step of the 32,768-particle PBF dam break (FP32 oracle, OpenMP over particles)"
+    elapsed = time.perf_counter() - t0
+    return {"value": sc.count / elapsed, "unit": "particle-steps/s", "cores": oracle.threads(), "kind": "port",
+            "sample": sample, "seconds": elapsed}
+
+
+def main(argv=None) -> int:
+    args = parse_args(argv)
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
+              "(multi-GPU slab decomposition is a design, not part of this build)", file=sys.stderr)
+        return 2
+    import torch
+
+    if not torch.cuda.is_available():
+        print("bench.py needs a HIP device", file=sys.stderr)
+        return 2
+    device = torch.device("cuda:0")
+    torch.cuda.set_device(device)
+    if args.workload == "dfsph":
+        sc, solver, states, control, params, cache = setup_dfsph(device)
+    else:
+        sc, solver, states, control, params, cache = setup_pbf(device)
+    n = sc.count
+    cells = math.prod(sc.domain.grid_resolution())
+
+    def step(k):
+        solver.forward_step(states[k % 2], control, params, states[(k + 1) % 2], cache)
+
+    for k in range(args.warmup):
+        step(k)
+    torch.cuda.synchronize()
+    if not args.no_pass_timing:
+        solver.set_profiling(True)
+    stream = torch.cuda.current_stream(device)
+    start = torch.cuda.Event(enable_timing=True)
+    stop = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    start.record(stream)
+    for k in range(args.warmup, args.warmup + args.steps):
+        step(k)
+    stop.record(stream)
+    torch.cuda.synchronize()
+    elapsed_ms = start.elapsed_time(stop)
+    passes = [] if args.no_pass_timing else solver.read_profile()
+    solver.set_profiling(False)
+    ms_per_step = elapsed_ms / args.steps
+    value = n * args.steps / (elapsed_ms * 1e-3)
+
+    final = states[(args.warmup + args.steps) % 2]
+    x = final.particles.positions.numpy()
+    if not np.isfinite(x).all():
+        print("non-finite positions after the timed steps", file=sys.stderr)
+        return 1
+
+    if args.workload == "dfsph":
+        iterations = solver.last_iterations()
+        bytes_step = dfsph_bytes_per_step(iterations, n, cells)
+        workload = ("DFSPH 3D dam break: 1,048,576 particles (128x64x128 block, spacing 0.01, jitter 0.001), "
+                    "h 0.02 cubic spline, dt 1e-3, 4 divergence + 6 density iterations (fixed counts, the "
+                    "reference's DFSPH), viscosity 0.01, static box 2.56x1.28x1.28")
+        config = {"workload": workload, "particles": n, "grid_cells": cells,
+                  "divergence_iterations": iterations[0], "density_iterations": iterations[1]}
+    else:
+        bytes_step = pbf_bytes_per_step(4, n, cells)
+        workload = "PBF dam break: 32,768 particles (32^3, spacing 0.05), h 0.1, dt 1/120, 4 iterations"
+        config = {"workload": workload, "particles": n, "grid_cells": cells, "pressure_iterations": 4}
+
+    roofline = None
+    if passes:
+        dom = max(passes, key=lambda p: p["total_ms"])
+        avg_ms = dom["total_ms"] / dom["launches"]
+        achieved = dom["bytes_per_particle"] * n / (avg_ms * 1e-3) / 1e9
+        roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic_for(dom["name"]),
+                    "kernel": dom["name"], "avg_launch_us": round(avg_ms * 1e3, 2),
+                    "algorithmic_bytes_per_launch": dom["bytes_per_particle"] * n}
+    step_achieved = bytes_step * n * args.steps / (elapsed_ms * 1e-3) / 1e9
+
+    baseline = None
+    if not args.no_cpu_baseline:
+        baseline = cpu_baseline(args.workload, sc)
+
+    line = {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": config,
+        "roofline": roofline,
+        "cpu_baseline": baseline,
+        "hbm_fraction_step": round(step_achieved / HBM_PEAK_GBS, 5),
+        "algorithmic_bytes_per_particle_step": round(bytes_step, 2),
+        "fwd_adjoint_ratio": None,
+        "passes": [{k: (round(v, 4) if isinstance(v, float) else v) for k, v in p.items()} for p in passes],
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

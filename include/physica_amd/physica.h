@@ -1,0 +1,290 @@
+/*
+ * physica_amd: MI355X (gfx950) particle-liquid engine, C ABI.
+ *
+ * Drop-in boundary for the particle-liquid path of Xayah-Graphics/physica
+ * (fluids/liquid/particle/). Plain C types only: device pointers, sizes,
+ * PODs, and an opaque handle per solver. All device buffers named in the
+ * argument lists are caller-owned and live on the device of the stream the
+ * solver was created with. Every entry point returns a phx_status; nothing
+ * throws across the ABI. Work is enqueued on the solver's stream and is
+ * asynchronous unless stated otherwise.
+ *
+ * Reference interfaces replaced (paths relative to fluids/liquid/particle/):
+ *   phx_domain_configuration   DomainConfiguration      domain/domain.ixx:12-44
+ *   phx_build_neighborhood     NeighborhoodSearch::build neighborhood/neighborhood.cpp:44-89
+ *                              cuda_detail::build_neighborhood neighborhood/neighborhood.cu:54-70
+ *   phx_density_forward        density::sph_forward/pbf_forward density/density.cpp:12-34,
+ *                              cuda_detail::density_forward density/density.cu:196-198
+ *   phx_dfsph_*                SPH<DFSPH> sph/sph.ixx:273-372, DFSPH sph/sph.ixx:195-268,
+ *                              DFSPH::forward sph/sph.cpp:572-610
+ *   phx_pbf_*                  PBF pbf/pbf.ixx:14-141, PBF::forward_step pbf/pbf.cpp:167-191
+ */
+#ifndef PHYSICA_AMD_PHYSICA_H
+#define PHYSICA_AMD_PHYSICA_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* A hipStream_t; 0 is the null stream. */
+typedef void* phx_stream;
+
+typedef enum phx_status {
+    PHX_SUCCESS = 0,
+    PHX_ERROR_INVALID_ARGUMENT = 1,
+    PHX_ERROR_UNSUPPORTED = 2,
+    PHX_ERROR_WRONG_MODE = 3,
+    PHX_ERROR_WORKSPACE_TOO_SMALL = 4,
+    PHX_ERROR_OUT_OF_MEMORY = 5,
+    PHX_ERROR_DEVICE = 6,
+    PHX_ERROR_CAPACITY = 7
+} phx_status;
+
+/* Message of the last failing call on this host thread ("" if none). */
+const char* phx_last_error(void);
+const char* phx_version(void);
+
+/* ---------------------------------------------------------------- domain */
+/* Vector3, BoxBoundary, BoundaryParticle, DomainConfiguration, ExecutionMode:
+ * domain/domain.ixx:12-44 */
+typedef struct phx_vector3 {
+    float x, y, z;
+} phx_vector3;
+
+typedef struct phx_box_boundary {
+    phx_vector3 minimum;
+    phx_vector3 maximum;
+    phx_vector3 velocity;
+    uint32_t no_slip;
+} phx_box_boundary;
+
+typedef struct phx_boundary_particle {
+    phx_vector3 position;
+    phx_vector3 velocity;
+    float volume;
+} phx_boundary_particle;
+
+typedef struct phx_domain_configuration {
+    uint32_t particle_count;
+    float time_step;
+    float support_radius;
+    float particle_radius;
+    phx_vector3 gravity;
+    phx_box_boundary boundary;
+    const phx_boundary_particle* boundary_particles; /* host array */
+    uint32_t boundary_particle_count;
+} phx_domain_configuration;
+
+typedef enum phx_execution_mode { PHX_MODE_FORWARD = 0, PHX_MODE_DIFFERENTIABLE = 1 } phx_execution_mode;
+
+/* Device SoA views, domain/device.h:20-36 */
+typedef struct phx_vector_view {
+    float* x;
+    float* y;
+    float* z;
+} phx_vector_view;
+
+typedef struct phx_const_vector_view {
+    const float* x;
+    const float* y;
+    const float* z;
+} phx_const_vector_view;
+
+/* ParticleState, Control, ParticleParameters: domain/domain.ixx:70-107.
+ * step_index is host-side, as in the reference. */
+typedef struct phx_particle_state {
+    phx_vector_view positions;
+    phx_vector_view velocities;
+    uint64_t step_index;
+} phx_particle_state;
+
+typedef struct phx_control {
+    phx_const_vector_view external_accelerations;
+} phx_control;
+
+typedef struct phx_particle_parameters {
+    const float* masses;
+    const float* rest_densities;
+    const float* viscosities;
+    const float* surface_tensions;
+} phx_particle_parameters;
+
+/* Grid resolution ceil((max - min) / h) per axis (neighborhood.cpp:25-30). */
+phx_status phx_grid_resolution(const phx_domain_configuration* domain, uint32_t resolution[3]);
+
+/* Per-pass timing (no reference counterpart: the reference has no timing in
+ * the liquid module). With profiling on, each pass of a step is bracketed by
+ * HIP events on the solver stream; read_profile waits for them and returns,
+ * per pass, the summed milliseconds, the launch count, and the pass's
+ * algorithmic bytes per particle per launch (its unique per-particle inputs
+ * and outputs read or written once, plus the cell-offset table). */
+typedef struct phx_pass_profile {
+    const char* name;
+    double total_ms;
+    uint64_t launches;
+    double bytes_per_particle;
+} phx_pass_profile;
+
+/* ----------------------------------------------------------- neighborhood */
+/* NeighborhoodView, neighborhood/device.h:8-23 (reference layout: arrays in
+ * original particle order are indexed through sorted_particle_indices). */
+typedef struct phx_neighborhood_view {
+    const uint64_t* sorted_keys;
+    const uint32_t* sorted_particle_indices;
+    const uint32_t* cell_offsets;
+    const uint64_t* sorted_boundary_keys;
+    const uint32_t* sorted_boundary_indices;
+    const uint32_t* boundary_cell_offsets;
+    uint32_t particle_count;
+    uint32_t boundary_count;
+    uint32_t cells_x, cells_y, cells_z;
+    float origin_x, origin_y, origin_z;
+    float cell_size;
+} phx_neighborhood_view;
+
+/* Workspace bytes phx_build_neighborhood needs (radix_sort_storage_size,
+ * neighborhood.cu:48-52). */
+phx_status phx_neighborhood_workspace_size(const phx_domain_configuration* domain, size_t* bytes);
+
+/* NeighborhoodSearch::build(step_index, positions, neighborhood):
+ * key = (cz*Cy + cy)*Cx + cx with cell = clamp(floor((x - min - t v_box)/h)),
+ * t = step_index * time_step; stable ascending sort of (key, index);
+ * cell_offsets[c] = first sorted position with key >= c for c in [0, C].
+ * Outputs (device): sorted_keys[N], sorted_particle_indices[N],
+ * cell_offsets[C+1], boundary_cell_offsets[C+1] (all zero when the domain
+ * has no boundary particles). Writes the lookup origin min + t v_box into
+ * *view if view is non-null. Boundary particles: PHX_ERROR_UNSUPPORTED. */
+phx_status phx_build_neighborhood(phx_stream stream, const phx_domain_configuration* domain, uint64_t step_index,
+                                  phx_const_vector_view positions, void* workspace, size_t workspace_bytes,
+                                  uint64_t* sorted_keys, uint32_t* sorted_particle_indices, uint32_t* cell_offsets,
+                                  uint32_t* boundary_cell_offsets, phx_neighborhood_view* view);
+
+/* density_forward (density.cu:18-40): rho_i = sum_j m_j W(x_i - x_j) over the
+ * 27 cells of topology_positions[i], j = i included; W = poly6 when
+ * pbf_kernel != 0, else the cubic spline. */
+phx_status phx_density_forward(phx_stream stream, uint32_t particle_count, float support_radius, int pbf_kernel,
+                               phx_const_vector_view topology_positions, phx_const_vector_view positions,
+                               phx_particle_parameters parameters, phx_neighborhood_view neighborhood,
+                               float* densities);
+
+/* ----------------------------------------------------------------- DFSPH */
+/* DFSPH::Configuration, sph/sph.ixx:196-201. density_error_tolerance > 0
+ * selects tolerance mode (an extension: iterate each phase until the mean
+ * relative density error, reduced in FP64, is below the tolerance, at most
+ * max_iterations times); 0 gives the reference's fixed iteration counts. */
+typedef struct phx_dfsph_configuration {
+    uint32_t divergence_iterations; /* reference default 4 */
+    uint32_t density_iterations;    /* reference default 6 */
+    uint32_t checkpoint_interval;   /* reference default 2 */
+    uint32_t pressure_warm_start;   /* reference default 1 */
+    float density_error_tolerance;  /* 0: fixed iterations */
+    uint32_t max_iterations;
+} phx_dfsph_configuration;
+
+/* DFSPH::Parameters and DFSPH::State, sph/sph.ixx:202-217 */
+typedef struct phx_dfsph_parameters {
+    const float* divergence_relaxation;
+    const float* density_relaxation;
+} phx_dfsph_parameters;
+
+typedef struct phx_dfsph_state {
+    float* warm_divergence_pressure;
+    float* warm_density_pressure;
+} phx_dfsph_state;
+
+/* SPH<DFSPH>::State / Parameters, sph/sph.ixx:275-298 */
+typedef struct phx_sph_dfsph_state {
+    phx_particle_state particles;
+    phx_dfsph_state method;
+} phx_sph_dfsph_state;
+
+typedef struct phx_sph_dfsph_parameters {
+    phx_particle_parameters particles;
+    phx_dfsph_parameters method;
+} phx_sph_dfsph_parameters;
+
+typedef struct phx_dfsph phx_dfsph;
+typedef struct phx_dfsph_step_cache phx_dfsph_step_cache;
+
+/* SPH<DFSPH>(domain, configuration, mode, stream), sph/sph.ixx:309-312 */
+phx_status phx_dfsph_create(const phx_domain_configuration* domain, const phx_dfsph_configuration* configuration,
+                            phx_execution_mode mode, phx_stream stream, phx_dfsph** solver);
+phx_status phx_dfsph_destroy(phx_dfsph* solver);
+/* allocate_step_cache, sph/sph.ixx:322 */
+phx_status phx_dfsph_allocate_step_cache(phx_dfsph* solver, phx_dfsph_step_cache** cache);
+phx_status phx_dfsph_free_step_cache(phx_dfsph_step_cache* cache);
+/* forward_step(state, control, parameters, next_state, cache), sph/sph.ixx:350-354 */
+phx_status phx_dfsph_forward_step(phx_dfsph* solver, const phx_sph_dfsph_state* state, const phx_control* control,
+                                  const phx_sph_dfsph_parameters* parameters, phx_sph_dfsph_state* next_state,
+                                  phx_dfsph_step_cache* cache);
+/* The reference's DFSPH::reference_gradient_norm (sph/sph.cpp:37-58). */
+phx_status phx_dfsph_reference_gradient_norm(const phx_dfsph* solver, float* value);
+/* Iterations executed by the last forward_step (divergence, density). */
+phx_status phx_dfsph_last_iterations(const phx_dfsph* solver, uint32_t iterations[2]);
+
+phx_status phx_dfsph_set_profiling(phx_dfsph* solver, int enabled);
+/* Fills up to capacity entries (passes never launched are omitted), sets
+ * *count, and resets the totals. Synchronises with the recorded events. */
+phx_status phx_dfsph_read_profile(phx_dfsph* solver, phx_pass_profile* profile, uint32_t capacity, uint32_t* count);
+
+/* Cache inspection (StepCache fields, sph/sph.ixx:299-303), copied out in
+ * original particle order into caller device buffers; enqueued on the solver
+ * stream. Neighborhood in the reference layout; any output may be null. */
+phx_status phx_dfsph_cache_neighborhood(const phx_dfsph_step_cache* cache, uint64_t* sorted_keys,
+                                        uint32_t* sorted_particle_indices, uint32_t* cell_offsets);
+phx_status phx_dfsph_cache_densities(const phx_dfsph_step_cache* cache, float* densities);
+phx_status phx_dfsph_cache_non_pressure_accelerations(const phx_dfsph_step_cache* cache, phx_vector_view out);
+
+/* ------------------------------------------------------------------- PBF */
+/* PBF::Configuration, pbf/pbf.ixx:15-18 */
+typedef struct phx_pbf_configuration {
+    uint32_t pressure_iterations; /* reference default 5 */
+    uint32_t checkpoint_interval; /* reference default 2 */
+} phx_pbf_configuration;
+
+/* PBF::State / Parameters, pbf/pbf.ixx:20-32 */
+typedef struct phx_pbf_state {
+    phx_particle_state particles;
+} phx_pbf_state;
+
+typedef struct phx_pbf_parameters {
+    phx_particle_parameters particles;
+    const float* relaxation;
+    const float* artificial_pressure_strength;
+    const float* artificial_pressure_exponent;
+    const float* artificial_pressure_radius;
+    const float* xsph_viscosity;
+    const float* vorticity_confinement;
+} phx_pbf_parameters;
+
+typedef struct phx_pbf phx_pbf;
+typedef struct phx_pbf_step_cache phx_pbf_step_cache;
+
+/* PBF(domain, configuration, mode, stream), pbf/pbf.cpp:39-72 */
+phx_status phx_pbf_create(const phx_domain_configuration* domain, const phx_pbf_configuration* configuration,
+                          phx_execution_mode mode, phx_stream stream, phx_pbf** solver);
+phx_status phx_pbf_destroy(phx_pbf* solver);
+phx_status phx_pbf_allocate_step_cache(phx_pbf* solver, phx_pbf_step_cache** cache);
+phx_status phx_pbf_free_step_cache(phx_pbf_step_cache* cache);
+/* forward_step(state, control, parameters, next_state, cache), pbf/pbf.cpp:167-191 */
+phx_status phx_pbf_forward_step(phx_pbf* solver, const phx_pbf_state* state, const phx_control* control,
+                                const phx_pbf_parameters* parameters, phx_pbf_state* next_state,
+                                phx_pbf_step_cache* cache);
+phx_status phx_pbf_set_profiling(phx_pbf* solver, int enabled);
+phx_status phx_pbf_read_profile(phx_pbf* solver, phx_pass_profile* profile, uint32_t capacity, uint32_t* count);
+/* StepCache fields (pbf/pbf.ixx:54-70), original particle order. */
+phx_status phx_pbf_cache_neighborhood(const phx_pbf_step_cache* cache, uint64_t* sorted_keys,
+                                      uint32_t* sorted_particle_indices, uint32_t* cell_offsets);
+phx_status phx_pbf_cache_predicted_positions(const phx_pbf_step_cache* cache, phx_vector_view out);
+/* Densities and lambdas of the last constraint iteration (IterationPrimal,
+ * pbf/pbf.ixx:98-105). */
+phx_status phx_pbf_cache_last_iteration(const phx_pbf_step_cache* cache, float* densities, float* lambdas);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

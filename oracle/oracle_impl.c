@@ -1,0 +1,874 @@
+/*
+ * CPU oracle implementation: TEST INFRASTRUCTURE ONLY (see oracle.h).
+ *
+ * Compiled twice: with ORACLE_F64 undefined (FP32 reference semantics) and
+ * defined (FP64 oracle). Every expression keeps the reference's operand order;
+ * the build uses -ffp-contract=off so no FMA is formed, and float/double
+ * arithmetic is IEEE (x86-64 SSE2, FLT_EVAL_METHOD == 0).
+ *
+ * Each function cites the reference file:line it restates; paths are relative
+ * to fluids/liquid/particle/ of Xayah-Graphics/physica.
+ */
+#include "oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#ifdef ORACLE_F64
+typedef double real;
+#define SUF(name) name##_f64
+#define R_SQRT sqrt
+#define R_FLOOR floor
+#define R_POW pow
+#define R_LOG log
+#define R_FMIN fmin
+#define R_FMAX fmax
+#define R_PI 3.14159265358979323846
+#else
+typedef float real;
+#define SUF(name) name##_f32
+#define R_SQRT sqrtf
+#define R_FLOOR floorf
+#define R_POW powf
+#define R_LOG logf
+#define R_FMIN fminf
+#define R_FMAX fmaxf
+#define R_PI 3.14159265358979323846F
+#endif
+#define K(c) ((real)(c))
+
+/* ---------------------------------------------------------------- vectors */
+/* domain/device.cuh:8-82 */
+typedef struct {
+    real x, y, z;
+} v3;
+
+static inline v3 v3_make(real x, real y, real z) {
+    v3 r = {x, y, z};
+    return r;
+}
+static inline v3 v3_load(const real* const f[3], uint32_t i) { return v3_make(f[0][i], f[1][i], f[2][i]); }
+static inline void v3_store(real* const f[3], uint32_t i, v3 v) {
+    f[0][i] = v.x;
+    f[1][i] = v.y;
+    f[2][i] = v.z;
+}
+static inline v3 v3_add(v3 a, v3 b) { return v3_make(a.x + b.x, a.y + b.y, a.z + b.z); }
+static inline v3 v3_sub(v3 a, v3 b) { return v3_make(a.x - b.x, a.y - b.y, a.z - b.z); }
+static inline v3 v3_scale(v3 v, real f) { return v3_make(f * v.x, f * v.y, f * v.z); }
+static inline real v3_dot(v3 a, v3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static inline v3 v3_cross(v3 a, v3 b) {
+    return v3_make(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+static inline real v3_length(v3 a) { return R_SQRT(v3_dot(a, a)); }
+
+/* ------------------------------------------------------------ SPH kernels */
+/* density/device.cuh:10-16 */
+static real poly6(v3 d, real h) {
+    const real sq = v3_dot(d, d);
+    const real hr = h * h;
+    if (sq >= hr) return K(0.0);
+    const real diff = hr - sq;
+    return K(315.0) * diff * diff * diff / (K(64.0) * R_PI * hr * hr * hr * h * h * h);
+}
+
+/* density/device.cuh:18-24 */
+static v3 poly6_gradient(v3 d, real h) {
+    const real sq = v3_dot(d, d);
+    const real hr = h * h;
+    if (sq >= hr) return v3_make(0, 0, 0);
+    const real diff = hr - sq;
+    return v3_scale(d, K(-945.0) * diff * diff / (K(32.0) * R_PI * hr * hr * hr * h * h * h));
+}
+
+/* density/device.cuh:26-34 */
+static real cubic(v3 d, real h) {
+    const real dist = v3_length(d);
+    if (dist >= h) return K(0.0);
+    const real q = dist / h;
+    const real coef = K(8.0) / (R_PI * h * h * h);
+    if (q <= K(0.5)) return coef * (K(6.0) * q * q * q - K(6.0) * q * q + K(1.0));
+    const real diff = K(1.0) - q;
+    return coef * K(2.0) * diff * diff * diff;
+}
+
+/* density/device.cuh:36-43 */
+static v3 cubic_gradient(v3 d, real h) {
+    const real dist = v3_length(d);
+    if (dist == K(0.0) || dist >= h) return v3_make(0, 0, 0);
+    const real q = dist / h;
+    const real coef = K(8.0) / (R_PI * h * h * h);
+    const real deriv = q <= K(0.5) ? K(18.0) * q * q - K(12.0) * q : K(-6.0) * (K(1.0) - q) * (K(1.0) - q);
+    return v3_scale(d, coef * deriv / (h * dist));
+}
+
+/* density/device.cuh:72-76 */
+static real viscosity_laplacian(v3 d, real h) {
+    const real dist = v3_length(d);
+    if (dist >= h) return K(0.0);
+    return K(45.0) * (h - dist) / (R_PI * h * h * h * h * h * h);
+}
+
+/* pbf.cu:26-32 (the coefficient -45/(pi h^6) is a per-call constant) */
+static real spiky_coefficient(real h) { return K(-45.0) / (R_PI * R_POW(h, K(6.0))); }
+
+static v3 spiky_gradient(v3 d, real h, real coefficient) {
+    const real dist = v3_length(d);
+    if (dist == K(0.0) || dist >= h) return v3_make(0, 0, 0);
+    const real diff = h - dist;
+    return v3_scale(d, coefficient * diff * diff / dist);
+}
+
+/* pbf.cu:56-71, value only (the forward correction uses nothing else) */
+static real artificial_value(v3 d, real h, real strength, real exponent, real radius) {
+    const real reference = poly6(v3_make(radius, K(0.0), K(0.0)), h);
+    const real kernel = poly6(d, h);
+    if (kernel == K(0.0)) return K(0.0);
+    const real ratio = kernel / reference;
+    const real power = R_POW(ratio, exponent);
+    return -strength * power;
+}
+
+/* ------------------------------------------------------------------- grid */
+#ifndef ORACLE_F64
+void oracle_grid_resolution(const oracle_domain* d, uint32_t res[3]) {
+    /* neighborhood.cpp:25-30 and neighborhood.cu:31-33 (FP32 in both) */
+    for (int a = 0; a < 3; ++a) res[a] = (uint32_t)ceilf((d->box_maximum[a] - d->box_minimum[a]) / d->support_radius);
+}
+#endif
+
+typedef struct {
+    uint32_t res[3];
+    uint32_t cells;
+    real origin[3];
+    real cell_size;
+    real time;
+    uint32_t n, nb;
+    uint64_t* keys;
+    uint32_t* idx;
+    uint32_t* off;
+    uint64_t* bkeys;
+    uint32_t* bidx;
+    uint32_t* boff;
+    const oracle_domain* d;
+} grid_t;
+
+/* min(C-1, max(0, __float2int_rd(v))) of neighborhood.cu:34-36 and
+ * neighborhood/device.cuh:44-48 */
+static int cell_clamp(real v, uint32_t count) {
+    const real f = R_FLOOR(v);
+    if (!(f > K(0.0))) return 0;
+    if (f >= (real)(count - 1u)) return (int)count - 1;
+    return (int)f;
+}
+
+/* key_kernel, neighborhood.cu:20-39 */
+static uint64_t grid_key(const oracle_domain* d, const uint32_t res[3], real time, real x, real y, real z) {
+    const real h = (real)d->support_radius;
+    const int cx = cell_clamp((x - (real)d->box_minimum[0] - time * (real)d->box_velocity[0]) / h, res[0]);
+    const int cy = cell_clamp((y - (real)d->box_minimum[1] - time * (real)d->box_velocity[1]) / h, res[1]);
+    const int cz = cell_clamp((z - (real)d->box_minimum[2] - time * (real)d->box_velocity[2]) / h, res[2]);
+    return ((uint64_t)cz * res[1] + (uint64_t)cy) * res[0] + (uint64_t)cx;
+}
+
+/* lower_bound, neighborhood.cu:9-18 */
+static uint32_t lower_bound(const uint64_t* keys, uint32_t count, uint64_t key) {
+    uint32_t first = 0u, last = count;
+    while (first < last) {
+        const uint32_t middle = first + (last - first) / 2u;
+        if (keys[middle] < key)
+            first = middle + 1u;
+        else
+            last = middle;
+    }
+    return first;
+}
+
+/* stable sort of (key, index) pairs by key; keys < cells. A sequential
+ * counting sort places equal keys in input order, which is exactly the
+ * stable-sort semantics of cub::DeviceRadixSort::SortPairs used at
+ * neighborhood.cu:57,68. */
+static void stable_sort(const uint64_t* keys, uint32_t n, uint32_t cells, uint64_t* sorted_keys, uint32_t* sorted_idx) {
+    uint32_t* start = (uint32_t*)calloc((size_t)cells + 1u, sizeof(uint32_t));
+    for (uint32_t i = 0; i < n; ++i) start[keys[i] + 1u]++;
+    for (uint32_t c = 0; c < cells; ++c) start[c + 1u] += start[c];
+    for (uint32_t i = 0; i < n; ++i) {
+        const uint32_t p = start[keys[i]]++;
+        sorted_keys[p] = keys[i];
+        sorted_idx[p] = i;
+    }
+    free(start);
+}
+
+/* build_neighborhood, neighborhood.cu:54-70, plus the host-side origin of
+ * neighborhood.cpp:83-88 */
+static void grid_build(grid_t* g, const oracle_domain* d, uint64_t step, const real* px, const real* py, const real* pz) {
+    memset(g, 0, sizeof(*g));
+    g->d = d;
+    oracle_grid_resolution(d, g->res);
+    g->cells = g->res[0] * g->res[1] * g->res[2];
+    g->n = d->particle_count;
+    g->nb = d->boundary_count;
+    const real time = (real)step * (real)d->time_step;
+    g->time = time;
+    for (int a = 0; a < 3; ++a) g->origin[a] = (real)d->box_minimum[a] + time * (real)d->box_velocity[a];
+    g->cell_size = (real)d->support_radius;
+    uint64_t* unsorted = (uint64_t*)malloc(sizeof(uint64_t) * (g->n > g->nb ? g->n : g->nb) + 8u);
+    g->keys = (uint64_t*)malloc(sizeof(uint64_t) * g->n + 8u);
+    g->idx = (uint32_t*)malloc(sizeof(uint32_t) * g->n + 4u);
+    g->off = (uint32_t*)malloc(sizeof(uint32_t) * ((size_t)g->cells + 1u));
+    g->bkeys = (uint64_t*)malloc(sizeof(uint64_t) * g->nb + 8u);
+    g->bidx = (uint32_t*)malloc(sizeof(uint32_t) * g->nb + 4u);
+    g->boff = (uint32_t*)malloc(sizeof(uint32_t) * ((size_t)g->cells + 1u));
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < (int64_t)g->n; ++i) unsorted[i] = grid_key(d, g->res, time, px[i], py[i], pz[i]);
+    stable_sort(unsorted, g->n, g->cells, g->keys, g->idx);
+#pragma omp parallel for schedule(static)
+    for (int64_t c = 0; c <= (int64_t)g->cells; ++c) g->off[c] = lower_bound(g->keys, g->n, (uint64_t)c);
+    if (g->nb == 0u) {
+        memset(g->boff, 0, sizeof(uint32_t) * ((size_t)g->cells + 1u));
+    } else {
+        for (uint32_t i = 0; i < g->nb; ++i) {
+            /* key_kernel with moving_positions = true: x += t v first */
+            const real x = (real)d->boundary_position[0][i] + time * (real)d->boundary_velocity[0][i];
+            const real y = (real)d->boundary_position[1][i] + time * (real)d->boundary_velocity[1][i];
+            const real z = (real)d->boundary_position[2][i] + time * (real)d->boundary_velocity[2][i];
+            unsorted[i] = grid_key(d, g->res, time, x, y, z);
+        }
+        stable_sort(unsorted, g->nb, g->cells, g->bkeys, g->bidx);
+        for (uint32_t c = 0; c <= g->cells; ++c) g->boff[c] = lower_bound(g->bkeys, g->nb, (uint64_t)c);
+    }
+    free(unsorted);
+}
+
+static void grid_free(grid_t* g) {
+    free(g->keys);
+    free(g->idx);
+    free(g->off);
+    free(g->bkeys);
+    free(g->bidx);
+    free(g->boff);
+}
+
+typedef struct {
+    uint32_t first, last, bfirst, blast;
+} cell_range;
+
+/* 27-cell walk of every neighbour kernel: particle_cell + cell_range,
+ * neighborhood/device.cuh:28-48, visited offset_x outer, offset_z inner
+ * (e.g. density.cu:25-27). Returns the valid ranges in visiting order. */
+static int stencil(const grid_t* g, v3 topo, cell_range out[27]) {
+    const int cx = cell_clamp((topo.x - g->origin[0]) / g->cell_size, g->res[0]);
+    const int cy = cell_clamp((topo.y - g->origin[1]) / g->cell_size, g->res[1]);
+    const int cz = cell_clamp((topo.z - g->origin[2]) / g->cell_size, g->res[2]);
+    int n = 0;
+    for (int ox = -1; ox <= 1; ++ox)
+        for (int oy = -1; oy <= 1; ++oy)
+            for (int oz = -1; oz <= 1; ++oz) {
+                const int x = cx + ox, y = cy + oy, z = cz + oz;
+                if (x < 0 || x >= (int)g->res[0] || y < 0 || y >= (int)g->res[1] || z < 0 || z >= (int)g->res[2]) continue;
+                const uint32_t c = ((uint32_t)z * g->res[1] + (uint32_t)y) * g->res[0] + (uint32_t)x;
+                out[n].first = g->off[c];
+                out[n].last = g->off[c + 1u];
+                out[n].bfirst = g->boff[c];
+                out[n].blast = g->boff[c + 1u];
+                ++n;
+            }
+    return n;
+}
+
+/* boundary_position / boundary_velocity, neighborhood/device.cuh:16-26 */
+static v3 boundary_position(const grid_t* g, uint32_t b) {
+    const oracle_domain* d = g->d;
+    return v3_make((real)d->boundary_position[0][b] + g->time * (real)d->boundary_velocity[0][b],
+                   (real)d->boundary_position[1][b] + g->time * (real)d->boundary_velocity[1][b],
+                   (real)d->boundary_position[2][b] + g->time * (real)d->boundary_velocity[2][b]);
+}
+static v3 boundary_velocity(const grid_t* g, uint32_t b) {
+    const oracle_domain* d = g->d;
+    return v3_make((real)d->boundary_velocity[0][b], (real)d->boundary_velocity[1][b], (real)d->boundary_velocity[2][b]);
+}
+static real boundary_volume(const grid_t* g, uint32_t b) { return (real)g->d->boundary_volume[b]; }
+
+/* ---------------------------------------------------------------- helpers */
+static real* alloc_real(uint32_t n) { return (real*)malloc(sizeof(real) * (size_t)n + 16u); }
+typedef struct {
+    real* c[3];
+} vfield;
+static vfield alloc_vfield(uint32_t n) {
+    vfield f;
+    for (int a = 0; a < 3; ++a) f.c[a] = alloc_real(n);
+    return f;
+}
+static void free_vfield(vfield* f) {
+    for (int a = 0; a < 3; ++a) free(f->c[a]);
+}
+#define CV(f) ((const real* const*)(f).c)
+
+/* Box of the solvers' collision_box, sph.cpp:21-35 and pbf.cpp:17-31 */
+typedef struct {
+    real lo[3], hi[3], vel[3];
+    int no_slip;
+} cbox;
+static cbox collision_box(const oracle_domain* d, uint64_t step) {
+    cbox b;
+    const real time = (real)step * (real)d->time_step;
+    const real rp = (real)d->particle_radius;
+    for (int a = 0; a < 3; ++a) {
+        b.lo[a] = (real)d->box_minimum[a] + time * (real)d->box_velocity[a] + rp;
+        b.hi[a] = (real)d->box_maximum[a] + time * (real)d->box_velocity[a] - rp;
+        b.vel[a] = (real)d->box_velocity[a];
+    }
+    b.no_slip = d->no_slip != 0u;
+    return b;
+}
+
+/* ------------------------------------------------------------- density */
+/* density_forward_kernel, density.cu:18-40 */
+static void density_kernel(const grid_t* g, int pbf_kernel, const real* const topo[3], const real* const pos[3],
+                           const real* masses, const real* rest_densities, real* out) {
+    const real h = (real)g->d->support_radius;
+#pragma omp parallel for schedule(static)
+    for (int64_t ii = 0; ii < (int64_t)g->n; ++ii) {
+        const uint32_t i = (uint32_t)ii;
+        const v3 p = v3_load(pos, i);
+        cell_range r[27];
+        const int nr = stencil(g, v3_load(topo, i), r);
+        real result = K(0.0);
+        for (int k = 0; k < nr; ++k) {
+            for (uint32_t s = r[k].first; s < r[k].last; ++s) {
+                const uint32_t j = g->idx[s];
+                const v3 d = v3_sub(p, v3_load(pos, j));
+                result += masses[j] * (pbf_kernel ? poly6(d, h) : cubic(d, h));
+            }
+            for (uint32_t s = r[k].bfirst; s < r[k].blast; ++s) {
+                const uint32_t b = g->bidx[s];
+                const v3 d = v3_sub(p, boundary_position(g, b));
+                result += rest_densities[i] * boundary_volume(g, b) * (pbf_kernel ? poly6(d, h) : cubic(d, h));
+            }
+        }
+        out[i] = result;
+    }
+}
+
+/* ------------------------------------------------------------ SPH common */
+/* non_pressure_forward_kernel, sph/common.cu:14-47 */
+static void non_pressure_kernel(const grid_t* g, const real* const pos[3], const real* const vel[3],
+                                const real* const ctl[3], const real* masses, const real* viscosities,
+                                const real* densities, real* const out[3]) {
+    const oracle_domain* dm = g->d;
+    const real h = (real)dm->support_radius;
+    const v3 gravity = v3_make((real)dm->gravity[0], (real)dm->gravity[1], (real)dm->gravity[2]);
+#pragma omp parallel for schedule(static)
+    for (int64_t ii = 0; ii < (int64_t)g->n; ++ii) {
+        const uint32_t i = (uint32_t)ii;
+        const v3 p = v3_load(pos, i);
+        const v3 v = v3_load(vel, i);
+        v3 acc = v3_add(gravity, v3_load(ctl, i));
+        cell_range r[27];
+        const int nr = stencil(g, p, r);
+        for (int k = 0; k < nr; ++k) {
+            for (uint32_t s = r[k].first; s < r[k].last; ++s) {
+                const uint32_t j = g->idx[s];
+                if (j == i) continue;
+                const v3 d = v3_sub(p, v3_load(pos, j));
+                const real visc = K(0.5) * (viscosities[i] + viscosities[j]);
+                const real w = visc * masses[j] * viscosity_laplacian(d, h) / densities[j];
+                acc = v3_add(acc, v3_scale(v3_sub(v3_load(vel, j), v), w));
+            }
+            for (uint32_t s = r[k].bfirst; s < r[k].blast; ++s) {
+                const uint32_t b = g->bidx[s];
+                const v3 d = v3_sub(p, boundary_position(g, b));
+                const real w = viscosities[i] * boundary_volume(g, b) * viscosity_laplacian(d, h);
+                acc = v3_add(acc, v3_scale(v3_sub(boundary_velocity(g, b), v), w));
+            }
+        }
+        v3_store(out, i, acc);
+    }
+}
+
+/* pressure_forward_kernel, sph/common.cu:165-194 */
+static void pressure_kernel(const grid_t* g, const real* const pos[3], const real* masses, const real* rest_densities,
+                            const real* densities, const real* pressures, real* const out[3]) {
+    const real h = (real)g->d->support_radius;
+#pragma omp parallel for schedule(static)
+    for (int64_t ii = 0; ii < (int64_t)g->n; ++ii) {
+        const uint32_t i = (uint32_t)ii;
+        const v3 p = v3_load(pos, i);
+        v3 acc = v3_make(0, 0, 0);
+        const real first = pressures[i] / (densities[i] * densities[i]);
+        cell_range r[27];
+        const int nr = stencil(g, p, r);
+        for (int k = 0; k < nr; ++k) {
+            for (uint32_t s = r[k].first; s < r[k].last; ++s) {
+                const uint32_t j = g->idx[s];
+                if (j == i) continue;
+                const real second = pressures[j] / (densities[j] * densities[j]);
+                acc = v3_add(acc, v3_scale(cubic_gradient(v3_sub(p, v3_load(pos, j)), h), -masses[j] * (first + second)));
+            }
+            for (uint32_t s = r[k].bfirst; s < r[k].blast; ++s) {
+                const uint32_t b = g->bidx[s];
+                acc = v3_add(acc, v3_scale(cubic_gradient(v3_sub(p, boundary_position(g, b)), h),
+                                           -rest_densities[i] * boundary_volume(g, b) * first));
+            }
+        }
+        v3_store(out, i, acc);
+    }
+}
+
+/* add_kernel, sph/common.cu:368-374 */
+static void add_kernel(uint32_t n, const real* const a[3], const real* const b[3], real* const out[3]) {
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < (int64_t)n; ++i)
+        for (int c = 0; c < 3; ++c) out[c][i] = a[c][i] + b[c][i];
+}
+
+/* predict_forward_kernel, sph/dfsph.cu:26-44 (the same body as
+ * integrate_forward_kernel, sph/common.cu:302-320, with a = a1 + a2) */
+static void sph_predict_kernel(uint32_t n, real dt, cbox box, const real* const pos[3], const real* const vel[3],
+                               const real* const a1[3], const real* const a2[3], real* const out_pos[3],
+                               real* const out_vel[3]) {
+#pragma omp parallel for schedule(static)
+    for (int64_t ii = 0; ii < (int64_t)n; ++ii) {
+        const uint32_t i = (uint32_t)ii;
+        v3 v = v3_add(v3_load(vel, i), v3_scale(v3_add(v3_load(a1, i), v3_load(a2, i)), dt));
+        v3 x = v3_add(v3_load(pos, i), v3_scale(v, dt));
+        const int cx = x.x < box.lo[0] || x.x > box.hi[0];
+        const int cy = x.y < box.lo[1] || x.y > box.hi[1];
+        const int cz = x.z < box.lo[2] || x.z > box.hi[2];
+        x.x = R_FMIN(box.hi[0], R_FMAX(box.lo[0], x.x));
+        x.y = R_FMIN(box.hi[1], R_FMAX(box.lo[1], x.y));
+        x.z = R_FMIN(box.hi[2], R_FMAX(box.lo[2], x.z));
+        if (box.no_slip && (cx || cy || cz)) {
+            v = v3_make(box.vel[0], box.vel[1], box.vel[2]);
+        } else {
+            if (cx) v.x = box.vel[0];
+            if (cy) v.y = box.vel[1];
+            if (cz) v.z = box.vel[2];
+        }
+        if (out_pos) v3_store(out_pos, i, x);
+        if (out_vel) v3_store(out_vel, i, v);
+    }
+}
+
+/* projection_update_forward_kernel, sph/dfsph.cu:104-110 */
+static void projection_update_kernel(uint32_t n, real dt, real G, const real* masses, const real* densities,
+                                     const real* targets, const real* predicted, const real* relaxation,
+                                     real* pressures) {
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < (int64_t)n; ++i) {
+        const real volume = masses[i] / densities[i];
+        const real delta = K(1.0) / (K(2.0) * dt * dt * volume * volume * G);
+        pressures[i] = R_FMAX(K(0.0), pressures[i] + relaxation[i] * delta * (predicted[i] - targets[i]));
+    }
+}
+
+/* compute_reference_gradient_norm, sph/sph.cpp:37-58 */
+real SUF(oracle_reference_gradient_norm)(const oracle_domain* d) {
+    const real h = (real)d->support_radius;
+    const real diameter = K(2.0) * (real)d->particle_radius;
+    const real coef = K(8.0) / (R_PI * h * h * h);
+    v3 gs = v3_make(0, 0, 0);
+    real sq = K(0.0);
+    for (real x = -h; x <= h; x += diameter)
+        for (real y = -h; y <= h; y += diameter)
+            for (real z = -h; z <= h; z += diameter) {
+                const real dist = R_SQRT(x * x + y * y + z * z);
+                if (dist == K(0.0) || dist >= h) continue;
+                const real q = dist / h;
+                const real deriv = q <= K(0.5) ? K(18.0) * q * q - K(12.0) * q : K(-6.0) * (K(1.0) - q) * (K(1.0) - q);
+                const real scale = coef * deriv / (h * dist);
+                gs.x += scale * x;
+                gs.y += scale * y;
+                gs.z += scale * z;
+                sq += scale * scale * (x * x + y * y + z * z);
+            }
+    return gs.x * gs.x + gs.y * gs.y + gs.z * gs.z + sq;
+}
+
+/* -------------------------------------------------------------- exports */
+void SUF(oracle_build_neighborhood)(const oracle_domain* d, uint64_t step, const real* px, const real* py,
+                                    const real* pz, uint64_t* sorted_keys, uint32_t* sorted_indices,
+                                    uint32_t* cell_offsets, uint64_t* sorted_boundary_keys,
+                                    uint32_t* sorted_boundary_indices, uint32_t* boundary_cell_offsets) {
+    grid_t g;
+    grid_build(&g, d, step, px, py, pz);
+    if (sorted_keys) memcpy(sorted_keys, g.keys, sizeof(uint64_t) * g.n);
+    if (sorted_indices) memcpy(sorted_indices, g.idx, sizeof(uint32_t) * g.n);
+    if (cell_offsets) memcpy(cell_offsets, g.off, sizeof(uint32_t) * ((size_t)g.cells + 1u));
+    if (sorted_boundary_keys) memcpy(sorted_boundary_keys, g.bkeys, sizeof(uint64_t) * g.nb);
+    if (sorted_boundary_indices) memcpy(sorted_boundary_indices, g.bidx, sizeof(uint32_t) * g.nb);
+    if (boundary_cell_offsets) memcpy(boundary_cell_offsets, g.boff, sizeof(uint32_t) * ((size_t)g.cells + 1u));
+    grid_free(&g);
+}
+
+void SUF(oracle_kernel_values)(int kind, float support_radius, uint32_t count, const real* dx, const real* dy,
+                               const real* dz, real* value, real* gx, real* gy, real* gz) {
+    const real h = (real)support_radius;
+    const real sc = spiky_coefficient(h);
+    for (uint32_t i = 0; i < count; ++i) {
+        const v3 d = v3_make(dx[i], dy[i], dz[i]);
+        v3 gr = v3_make(0, 0, 0);
+        real w = K(0.0);
+        switch (kind) {
+        case ORACLE_POLY6:
+            w = poly6(d, h);
+            gr = poly6_gradient(d, h);
+            break;
+        case ORACLE_CUBIC:
+            w = cubic(d, h);
+            gr = cubic_gradient(d, h);
+            break;
+        case ORACLE_SPIKY:
+            gr = spiky_gradient(d, h, sc);
+            break;
+        case ORACLE_VISCOSITY_LAPLACIAN:
+            w = viscosity_laplacian(d, h);
+            break;
+        default:
+            break;
+        }
+        if (value) value[i] = w;
+        if (gx) gx[i] = gr.x;
+        if (gy) gy[i] = gr.y;
+        if (gz) gz[i] = gr.z;
+    }
+}
+
+void SUF(oracle_density)(const oracle_domain* d, uint64_t grid_step, int pbf_kernel, const real* topology[3],
+                         const real* positions[3], const real* masses, const real* rest_densities, real* densities) {
+    grid_t g;
+    grid_build(&g, d, grid_step, topology[0], topology[1], topology[2]);
+    density_kernel(&g, pbf_kernel, topology, positions, masses, rest_densities, densities);
+    grid_free(&g);
+}
+
+void SUF(oracle_non_pressure)(const oracle_domain* d, uint64_t grid_step, const real* positions[3],
+                              const real* velocities[3], const real* controls[3], const real* masses,
+                              const real* viscosities, const real* densities, real* out[3]) {
+    grid_t g;
+    grid_build(&g, d, grid_step, positions[0], positions[1], positions[2]);
+    non_pressure_kernel(&g, positions, velocities, controls, masses, viscosities, densities, out);
+    grid_free(&g);
+}
+
+void SUF(oracle_pressure)(const oracle_domain* d, uint64_t grid_step, const real* positions[3], const real* masses,
+                          const real* rest_densities, const real* densities, const real* pressures, real* out[3]) {
+    grid_t g;
+    grid_build(&g, d, grid_step, positions[0], positions[1], positions[2]);
+    pressure_kernel(&g, positions, masses, rest_densities, densities, pressures, out);
+    grid_free(&g);
+}
+
+/* SPH<DFSPH>::forward_step (sph.ixx:350-354) = build(step, x) -> density(x, x)
+ * -> DFSPH::forward (sph.cpp:572-610), fixed iteration counts. */
+void SUF(oracle_dfsph_step)(const oracle_domain* d, uint32_t nd, uint32_t ne, int warm_start, uint64_t step,
+                            const real* pos[3], const real* vel[3], const real* ctl[3], const real* masses,
+                            const real* rest_densities, const real* viscosities, const real* div_relax,
+                            const real* den_relax, const real* warm_div, const real* warm_den, real* next_pos[3],
+                            real* next_vel[3], real* next_warm_div, real* next_warm_den, real* densities_out,
+                            real* non_pressure_out[3]) {
+    const uint32_t n = d->particle_count;
+    const real dt = (real)d->time_step;
+    const real G = SUF(oracle_reference_gradient_norm)(d);
+    const cbox box = collision_box(d, step);
+    grid_t g;
+    grid_build(&g, d, step, pos[0], pos[1], pos[2]);
+    real* rho = alloc_real(n);
+    real* rho_star = alloc_real(n);
+    real* p = alloc_real(n);
+    vfield anp = alloc_vfield(n), ap = alloc_vfield(n), adiv = alloc_vfield(n), total = alloc_vfield(n);
+    vfield xs = alloc_vfield(n), vs = alloc_vfield(n);
+    density_kernel(&g, 0, pos, pos, masses, rest_densities, rho);
+    non_pressure_kernel(&g, pos, vel, ctl, masses, viscosities, rho, anp.c);
+    /* divergence phase, sph.cpp:574-589 */
+    for (uint32_t i = 0; i < n; ++i) p[i] = warm_start ? warm_div[i] : K(0.0);
+    pressure_kernel(&g, pos, masses, rest_densities, rho, p, ap.c);
+    for (uint32_t it = 1; it <= nd; ++it) {
+        sph_predict_kernel(n, dt, box, pos, vel, CV(anp), CV(ap), xs.c, vs.c);
+        density_kernel(&g, 0, pos, CV(xs), masses, rest_densities, rho_star);
+        projection_update_kernel(n, dt, G, masses, rho, rho, rho_star, div_relax, p);
+        pressure_kernel(&g, pos, masses, rest_densities, rho, p, ap.c);
+    }
+    for (int c = 0; c < 3; ++c) memcpy(adiv.c[c], ap.c[c], sizeof(real) * n);
+    memcpy(next_warm_div, p, sizeof(real) * n);
+    /* density phase, sph.cpp:591-607 */
+    for (uint32_t i = 0; i < n; ++i) p[i] = warm_start ? warm_den[i] : K(0.0);
+    pressure_kernel(&g, pos, masses, rest_densities, rho, p, ap.c);
+    for (uint32_t it = 1; it <= ne; ++it) {
+        add_kernel(n, CV(adiv), CV(ap), total.c);
+        sph_predict_kernel(n, dt, box, pos, vel, CV(anp), CV(total), xs.c, vs.c);
+        density_kernel(&g, 0, pos, CV(xs), masses, rest_densities, rho_star);
+        projection_update_kernel(n, dt, G, masses, rho, rest_densities, rho_star, den_relax, p);
+        pressure_kernel(&g, pos, masses, rest_densities, rho, p, ap.c);
+    }
+    add_kernel(n, CV(adiv), CV(ap), total.c);
+    memcpy(next_warm_den, p, sizeof(real) * n);
+    sph_predict_kernel(n, dt, box, pos, vel, CV(anp), CV(total), next_pos, next_vel);
+    if (densities_out) memcpy(densities_out, rho, sizeof(real) * n);
+    if (non_pressure_out)
+        for (int c = 0; c < 3; ++c)
+            if (non_pressure_out[c]) memcpy(non_pressure_out[c], anp.c[c], sizeof(real) * n);
+    free(rho);
+    free(rho_star);
+    free(p);
+    free_vfield(&anp);
+    free_vfield(&ap);
+    free_vfield(&adiv);
+    free_vfield(&total);
+    free_vfield(&xs);
+    free_vfield(&vs);
+    grid_free(&g);
+}
+
+/* ------------------------------------------------------------------ PBF */
+/* lambda_forward_kernel, pbf.cu:95-127 */
+static void lambda_kernel(const grid_t* g, const real* const topo[3], const real* const pos[3], const real* masses,
+                          const real* rest_densities, const real* densities, const real* relaxation,
+                          real* const gsum[3], real* denominators, real* lambdas) {
+    const real h = (real)g->d->support_radius;
+    const real sc = spiky_coefficient(h);
+#pragma omp parallel for schedule(static)
+    for (int64_t ii = 0; ii < (int64_t)g->n; ++ii) {
+        const uint32_t i = (uint32_t)ii;
+        const v3 p = v3_load(pos, i);
+        const real rho0 = rest_densities[i];
+        v3 sg = v3_make(0, 0, 0);
+        real den = relaxation[i];
+        cell_range r[27];
+        const int nr = stencil(g, v3_load(topo, i), r);
+        for (int k = 0; k < nr; ++k) {
+            for (uint32_t s = r[k].first; s < r[k].last; ++s) {
+                const uint32_t j = g->idx[s];
+                if (j == i) continue;
+                const v3 gr = spiky_gradient(v3_sub(p, v3_load(pos, j)), h, sc);
+                const real c = masses[j] / rho0;
+                sg = v3_add(sg, v3_scale(gr, c));
+                den += c * c * v3_dot(gr, gr);
+            }
+            for (uint32_t s = r[k].bfirst; s < r[k].blast; ++s) {
+                const uint32_t b = g->bidx[s];
+                sg = v3_add(sg, v3_scale(spiky_gradient(v3_sub(p, boundary_position(g, b)), h, sc), boundary_volume(g, b)));
+            }
+        }
+        den += v3_dot(sg, sg);
+        if (gsum) v3_store(gsum, i, sg);
+        if (denominators) denominators[i] = den;
+        lambdas[i] = -(densities[i] / rho0 - K(1.0)) / den;
+    }
+}
+
+/* correction_forward_kernel, pbf.cu:254-288 */
+static void correction_kernel(const grid_t* g, const real* const topo[3], const real* const pos[3], const real* masses,
+                              const real* rest_densities, const real* lambdas, const real* strength,
+                              const real* exponent, const real* radius, real* const out[3]) {
+    const real h = (real)g->d->support_radius;
+    const real sc = spiky_coefficient(h);
+#pragma omp parallel for schedule(static)
+    for (int64_t ii = 0; ii < (int64_t)g->n; ++ii) {
+        const uint32_t i = (uint32_t)ii;
+        const v3 p = v3_load(pos, i);
+        const real rho0 = rest_densities[i];
+        v3 corr = v3_make(0, 0, 0);
+        cell_range r[27];
+        const int nr = stencil(g, v3_load(topo, i), r);
+        for (int k = 0; k < nr; ++k) {
+            for (uint32_t s = r[k].first; s < r[k].last; ++s) {
+                const uint32_t j = g->idx[s];
+                if (j == i) continue;
+                const v3 d = v3_sub(p, v3_load(pos, j));
+                const real art = artificial_value(d, h, strength[i], exponent[i], radius[i]);
+                const real c = masses[j] / rho0;
+                corr = v3_add(corr, v3_scale(spiky_gradient(d, h, sc), c * (lambdas[i] + lambdas[j] + art)));
+            }
+            for (uint32_t s = r[k].bfirst; s < r[k].blast; ++s) {
+                const uint32_t b = g->bidx[s];
+                const v3 d = v3_sub(p, boundary_position(g, b));
+                const real art = artificial_value(d, h, strength[i], exponent[i], radius[i]);
+                corr = v3_add(corr, v3_scale(spiky_gradient(d, h, sc), boundary_volume(g, b) * (lambdas[i] + art)));
+            }
+        }
+        v3_store(out, i, corr);
+    }
+}
+
+/* vorticity_forward_kernel, pbf.cu:476-497 */
+static void vorticity_kernel(const grid_t* g, const real* const topo[3], const real* const pos[3],
+                             const real* const vel[3], real* const out[3]) {
+    const real h = (real)g->d->support_radius;
+    const real sc = spiky_coefficient(h);
+#pragma omp parallel for schedule(static)
+    for (int64_t ii = 0; ii < (int64_t)g->n; ++ii) {
+        const uint32_t i = (uint32_t)ii;
+        const v3 p = v3_load(pos, i);
+        const v3 v = v3_load(vel, i);
+        v3 w = v3_make(0, 0, 0);
+        cell_range r[27];
+        const int nr = stencil(g, v3_load(topo, i), r);
+        for (int k = 0; k < nr; ++k)
+            for (uint32_t s = r[k].first; s < r[k].last; ++s) {
+                const uint32_t j = g->idx[s];
+                if (j == i) continue;
+                w = v3_add(w, v3_cross(v3_sub(v3_load(vel, j), v), spiky_gradient(v3_sub(p, v3_load(pos, j)), h, sc)));
+            }
+        v3_store(out, i, w);
+    }
+}
+
+/* magnitude_forward_kernel + normal_forward_kernel, pbf.cu:565-593 */
+static void normal_kernel(const grid_t* g, const real* const topo[3], const real* const pos[3],
+                          const real* magnitudes, real* const normals[3], real* normalizers) {
+    const real h = (real)g->d->support_radius;
+    const real sc = spiky_coefficient(h);
+#pragma omp parallel for schedule(static)
+    for (int64_t ii = 0; ii < (int64_t)g->n; ++ii) {
+        const uint32_t i = (uint32_t)ii;
+        const v3 p = v3_load(pos, i);
+        v3 gr = v3_make(0, 0, 0);
+        cell_range r[27];
+        const int nr = stencil(g, v3_load(topo, i), r);
+        for (int k = 0; k < nr; ++k)
+            for (uint32_t s = r[k].first; s < r[k].last; ++s) {
+                const uint32_t j = g->idx[s];
+                if (j == i) continue;
+                gr = v3_add(gr, v3_scale(spiky_gradient(v3_sub(p, v3_load(pos, j)), h, sc), magnitudes[j] - magnitudes[i]));
+            }
+        const real nrm = v3_length(gr);
+        normalizers[i] = nrm;
+        v3_store(normals, i, nrm > K(0.0) ? v3_scale(gr, K(1.0) / nrm) : v3_make(0, 0, 0));
+    }
+}
+
+/* xsph_forward_kernel, pbf.cu:700-721 */
+static void xsph_kernel(const grid_t* g, const real* const topo[3], const real* const pos[3], const real* const vel[3],
+                        const real* viscosity, real* const out[3]) {
+    const real h = (real)g->d->support_radius;
+#pragma omp parallel for schedule(static)
+    for (int64_t ii = 0; ii < (int64_t)g->n; ++ii) {
+        const uint32_t i = (uint32_t)ii;
+        const v3 p = v3_load(pos, i);
+        const v3 v = v3_load(vel, i);
+        v3 sm = v3_make(0, 0, 0);
+        cell_range r[27];
+        const int nr = stencil(g, v3_load(topo, i), r);
+        for (int k = 0; k < nr; ++k)
+            for (uint32_t s = r[k].first; s < r[k].last; ++s) {
+                const uint32_t j = g->idx[s];
+                if (j == i) continue;
+                sm = v3_add(sm, v3_scale(v3_sub(v3_load(vel, j), v), poly6(v3_sub(p, v3_load(pos, j)), h)));
+            }
+        v3_store(out, i, v3_add(v, v3_scale(sm, viscosity[i])));
+    }
+}
+
+void SUF(oracle_pbf_lambda)(const oracle_domain* d, uint64_t grid_step, const real* topology[3],
+                            const real* positions[3], const real* masses, const real* rest_densities,
+                            const real* densities, const real* relaxation, real* gradient_sums[3],
+                            real* denominators, real* lambdas) {
+    grid_t g;
+    grid_build(&g, d, grid_step, topology[0], topology[1], topology[2]);
+    lambda_kernel(&g, topology, positions, masses, rest_densities, densities, relaxation, gradient_sums, denominators,
+                  lambdas);
+    grid_free(&g);
+}
+
+/* PBF::forward_step, pbf.cpp:167-191 */
+void SUF(oracle_pbf_step)(const oracle_domain* d, uint32_t iterations, uint64_t step, const real* pos[3],
+                          const real* vel[3], const real* ctl[3], const real* masses, const real* rest_densities,
+                          const real* relaxation, const real* strength, const real* exponent, const real* radius,
+                          const real* xsph, const real* confinement, real* next_pos[3], real* next_vel[3],
+                          real* densities_out, real* lambdas_out, real* predicted_out[3]) {
+    const uint32_t n = d->particle_count;
+    const real dt = (real)d->time_step;
+    const v3 gravity = v3_make((real)d->gravity[0], (real)d->gravity[1], (real)d->gravity[2]);
+    vfield xs = alloc_vfield(n), pc = alloc_vfield(n), corr = alloc_vfield(n), v = alloc_vfield(n);
+    vfield w = alloc_vfield(n), nrm = alloc_vfield(n), vc = alloc_vfield(n);
+    real* rho = alloc_real(n);
+    real* lam = alloc_real(n);
+    real* mag = alloc_real(n);
+    real* nz = alloc_real(n);
+    /* predict_forward_kernel, pbf.cu:73-78 */
+#pragma omp parallel for schedule(static)
+    for (int64_t ii = 0; ii < (int64_t)n; ++ii) {
+        const uint32_t i = (uint32_t)ii;
+        const v3 pv = v3_add(v3_load(vel, i), v3_scale(v3_add(gravity, v3_load(ctl, i)), dt));
+        v3_store(xs.c, i, v3_add(v3_load(pos, i), v3_scale(pv, dt)));
+    }
+    grid_t g;
+    grid_build(&g, d, step + 1u, xs.c[0], xs.c[1], xs.c[2]);
+    for (int c = 0; c < 3; ++c) memcpy(pc.c[c], xs.c[c], sizeof(real) * n);
+    const cbox box = collision_box(d, step + 1u);
+    for (uint32_t it = 0; it < iterations; ++it) {
+        density_kernel(&g, 1, CV(xs), CV(pc), masses, rest_densities, rho);
+        lambda_kernel(&g, CV(xs), CV(pc), masses, rest_densities, rho, relaxation, NULL, NULL, lam);
+        correction_kernel(&g, CV(xs), CV(pc), masses, rest_densities, lam, strength, exponent, radius, corr.c);
+        /* project_forward_kernel, pbf.cu:417-431 */
+#pragma omp parallel for schedule(static)
+        for (int64_t ii = 0; ii < (int64_t)n; ++ii) {
+            const uint32_t i = (uint32_t)ii;
+            const v3 cand = v3_add(v3_load(CV(pc), i), v3_load(CV(corr), i));
+            v3_store(pc.c, i,
+                     v3_make(R_FMIN(box.hi[0], R_FMAX(box.lo[0], cand.x)), R_FMIN(box.hi[1], R_FMAX(box.lo[1], cand.y)),
+                             R_FMIN(box.hi[2], R_FMAX(box.lo[2], cand.z))));
+        }
+    }
+    /* reconstruct_forward_kernel, pbf.cu:456-460 */
+    const real inv_dt = K(1.0) / dt;
+#pragma omp parallel for schedule(static)
+    for (int64_t ii = 0; ii < (int64_t)n; ++ii) {
+        const uint32_t i = (uint32_t)ii;
+        v3_store(v.c, i, v3_scale(v3_sub(v3_load(CV(pc), i), v3_load(pos, i)), inv_dt));
+    }
+    vorticity_kernel(&g, CV(xs), CV(pc), CV(v), w.c);
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < (int64_t)n; ++i) mag[i] = v3_length(v3_load(CV(w), (uint32_t)i));
+    normal_kernel(&g, CV(xs), CV(pc), mag, nrm.c, nz);
+    /* confinement_forward_kernel, pbf.cu:674-678 */
+#pragma omp parallel for schedule(static)
+    for (int64_t ii = 0; ii < (int64_t)n; ++ii) {
+        const uint32_t i = (uint32_t)ii;
+        v3_store(vc.c, i,
+                 v3_add(v3_load(CV(v), i),
+                        v3_scale(v3_cross(v3_load(CV(nrm), i), v3_load(CV(w), i)), dt * confinement[i])));
+    }
+    xsph_kernel(&g, CV(xs), CV(pc), CV(vc), xsph, next_vel);
+    for (int c = 0; c < 3; ++c) memcpy(next_pos[c], pc.c[c], sizeof(real) * n);
+    if (densities_out) memcpy(densities_out, rho, sizeof(real) * n);
+    if (lambdas_out) memcpy(lambdas_out, lam, sizeof(real) * n);
+    if (predicted_out)
+        for (int c = 0; c < 3; ++c)
+            if (predicted_out[c]) memcpy(predicted_out[c], xs.c[c], sizeof(real) * n);
+    free_vfield(&xs);
+    free_vfield(&pc);
+    free_vfield(&corr);
+    free_vfield(&v);
+    free_vfield(&w);
+    free_vfield(&nrm);
+    free_vfield(&vc);
+    free(rho);
+    free(lam);
+    free(mag);
+    free(nz);
+    grid_free(&g);
+}
+
+#ifndef ORACLE_F64
+int oracle_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+void oracle_set_threads(int threads) {
+#ifdef _OPENMP
+    if (threads > 0) omp_set_num_threads(threads);
+#else
+    (void)threads;
+#endif
+}
+#endif

@@ -1,0 +1,428 @@
+// extern "C" entry points of libphysica_amd (include/physica_amd/physica.h).
+// Exceptions never cross the ABI: each call maps them to a phx_status and a
+// per-thread message.
+#include <cstring>
+#include <new>
+#include <string>
+
+#include "dfsph.hpp"
+#include "pbf.hpp"
+
+struct phx_dfsph {
+    phx::DfsphSolver impl;
+    phx_dfsph(const phx_domain_configuration* d, const phx_dfsph_configuration* c, phx_execution_mode m, hipStream_t s)
+        : impl(d, c, m, s) {}
+};
+struct phx_dfsph_step_cache {
+    const phx_dfsph* owner = nullptr;
+    phx::DfsphCache impl;
+    bool valid = false;
+};
+struct phx_pbf {
+    phx::PbfSolver impl;
+    phx_pbf(const phx_domain_configuration* d, const phx_pbf_configuration* c, phx_execution_mode m, hipStream_t s)
+        : impl(d, c, m, s) {}
+};
+struct phx_pbf_step_cache {
+    const phx_pbf* owner = nullptr;
+    phx::PbfCache impl;
+    bool valid = false;
+};
+
+namespace {
+
+thread_local std::string g_last_error;
+
+template <class F>
+phx_status guarded(F&& body) {
+    try {
+        body();
+        g_last_error.clear();
+        return PHX_SUCCESS;
+    } catch (const phx::Error& e) {
+        g_last_error = e.what();
+        return e.status;
+    } catch (const std::bad_alloc&) {
+        g_last_error = "host allocation failed";
+        return PHX_ERROR_OUT_OF_MEMORY;
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+        return PHX_ERROR_DEVICE;
+    } catch (...) {
+        g_last_error = "unknown error";
+        return PHX_ERROR_DEVICE;
+    }
+}
+
+inline hipStream_t stream_of(phx_stream s) { return reinterpret_cast<hipStream_t>(s); }
+
+constexpr uint32_t kBlock = 256;
+
+// Reference-layout view (neighborhood/device.h:8-23).
+struct RefGrid {
+    const uint32_t* indices;
+    const uint32_t* offsets;
+    uint32_t cx, cy, cz;
+    float ox, oy, oz, cell;
+};
+
+// density_forward_kernel (density/density.cu:18-40) in original particle order
+// over a reference-layout neighbourhood; boundary particles are not supported.
+__global__ void __launch_bounds__(kBlock)
+    reference_density_kernel(uint32_t n, float h, int pbf_kernel, phx::CVec topo, phx::CVec pos, const float* masses,
+                             RefGrid g, float* out) {
+    using namespace phx;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const Float3 p = load(pos, i);
+    const Float3 t = load(topo, i);
+    const int cx = cell_clamp((t.x - g.ox) / g.cell, g.cx);
+    const int cy = cell_clamp((t.y - g.oy) / g.cell, g.cy);
+    const int cz = cell_clamp((t.z - g.oz) / g.cell, g.cz);
+    float result = 0.0F;
+    for (int ox = -1; ox <= 1; ++ox)
+        for (int oy = -1; oy <= 1; ++oy)
+            for (int oz = -1; oz <= 1; ++oz) {
+                const int X = cx + ox, Y = cy + oy, Z = cz + oz;
+                if (X < 0 || X >= static_cast<int>(g.cx) || Y < 0 || Y >= static_cast<int>(g.cy) || Z < 0 ||
+                    Z >= static_cast<int>(g.cz))
+                    continue;
+                const uint32_t c = (static_cast<uint32_t>(Z) * g.cy + static_cast<uint32_t>(Y)) * g.cx +
+                                   static_cast<uint32_t>(X);
+                for (uint32_t s = g.offsets[c]; s < g.offsets[c + 1u]; ++s) {
+                    const uint32_t j = g.indices[s];
+                    const Float3 d = sub(p, load(pos, j));
+                    result += masses[j] * (pbf_kernel != 0 ? poly6(d, h) : cubic(d, h));
+                }
+            }
+    out[i] = result;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* phx_last_error(void) { return g_last_error.c_str(); }
+
+const char* phx_version(void) { return "physica_amd 0.1.0 (gfx950)"; }
+
+phx_status phx_grid_resolution(const phx_domain_configuration* domain, uint32_t resolution[3]) {
+    return guarded([&] {
+        phx::require(domain != nullptr && resolution != nullptr, PHX_ERROR_INVALID_ARGUMENT, "null argument");
+        phx::grid_resolution(*domain, resolution);
+    });
+}
+
+phx_status phx_neighborhood_workspace_size(const phx_domain_configuration* domain, size_t* bytes) {
+    return guarded([&] {
+        phx::require(bytes != nullptr, PHX_ERROR_INVALID_ARGUMENT, "null bytes");
+        const phx::Domain d = phx::make_domain(domain);
+        *bytes = phx::grid_scratch_bytes(d.config.particle_count, d.cells);
+    });
+}
+
+phx_status phx_build_neighborhood(phx_stream stream, const phx_domain_configuration* domain, uint64_t step_index,
+                                  phx_const_vector_view positions, void* workspace, size_t workspace_bytes,
+                                  uint64_t* sorted_keys, uint32_t* sorted_particle_indices, uint32_t* cell_offsets,
+                                  uint32_t* boundary_cell_offsets, phx_neighborhood_view* view) {
+    return guarded([&] {
+        const phx::Domain d = phx::make_domain(domain);
+        const uint32_t n = d.config.particle_count;
+        const hipStream_t s = stream_of(stream);
+        phx::require(cell_offsets != nullptr, PHX_ERROR_INVALID_ARGUMENT, "null cell_offsets");
+        if (n > 0) {
+            phx::require_vec(positions, "positions");
+            phx::require(sorted_particle_indices != nullptr, PHX_ERROR_INVALID_ARGUMENT, "null sorted_particle_indices");
+        }
+        phx::require(workspace != nullptr, PHX_ERROR_INVALID_ARGUMENT, "null workspace");
+        const phx::GridScratch scratch = phx::carve_grid_scratch(workspace, workspace_bytes, n, d.cells);
+        phx::ChannelList none;
+        const phx::GridGeometry geo = phx::build_grid(s, d, step_index, phx::cvec(positions), nullptr, none.get(),
+                                                      scratch, sorted_particle_indices, cell_offsets);
+        if (sorted_keys != nullptr && n > 0)
+            phx::export_sorted_keys(s, sorted_particle_indices, scratch.keys, n, sorted_keys);
+        if (boundary_cell_offsets != nullptr)
+            phx::fill_u32(s, boundary_cell_offsets, static_cast<size_t>(d.cells) + 1u, 0u);
+        if (view != nullptr) {
+            view->sorted_keys = sorted_keys;
+            view->sorted_particle_indices = sorted_particle_indices;
+            view->cell_offsets = cell_offsets;
+            view->sorted_boundary_keys = nullptr;
+            view->sorted_boundary_indices = nullptr;
+            view->boundary_cell_offsets = boundary_cell_offsets;
+            view->particle_count = n;
+            view->boundary_count = 0u;
+            view->cells_x = geo.resolution[0];
+            view->cells_y = geo.resolution[1];
+            view->cells_z = geo.resolution[2];
+            view->origin_x = geo.origin[0];
+            view->origin_y = geo.origin[1];
+            view->origin_z = geo.origin[2];
+            view->cell_size = geo.cell_size;
+        }
+    });
+}
+
+phx_status phx_density_forward(phx_stream stream, uint32_t particle_count, float support_radius, int pbf_kernel,
+                               phx_const_vector_view topology_positions, phx_const_vector_view positions,
+                               phx_particle_parameters parameters, phx_neighborhood_view neighborhood,
+                               float* densities) {
+    return guarded([&] {
+        if (particle_count == 0) return;
+        phx::require_vec(topology_positions, "topology positions");
+        phx::require_vec(positions, "positions");
+        phx::require(parameters.masses != nullptr && densities != nullptr, PHX_ERROR_INVALID_ARGUMENT,
+                     "null masses or densities");
+        phx::require(neighborhood.sorted_particle_indices != nullptr && neighborhood.cell_offsets != nullptr,
+                     PHX_ERROR_INVALID_ARGUMENT, "null neighbourhood arrays");
+        phx::require(neighborhood.boundary_count == 0u, PHX_ERROR_UNSUPPORTED, "boundary particles are not supported yet");
+        phx::require(neighborhood.particle_count == particle_count, PHX_ERROR_INVALID_ARGUMENT,
+                     "neighbourhood particle_count does not match");
+        phx::require(neighborhood.cells_x > 0 && neighborhood.cells_y > 0 && neighborhood.cells_z > 0 &&
+                         neighborhood.cell_size > 0.0F,
+                     PHX_ERROR_INVALID_ARGUMENT, "empty neighbourhood grid");
+        RefGrid g{neighborhood.sorted_particle_indices, neighborhood.cell_offsets, neighborhood.cells_x,
+                  neighborhood.cells_y, neighborhood.cells_z, neighborhood.origin_x, neighborhood.origin_y,
+                  neighborhood.origin_z, neighborhood.cell_size};
+        reference_density_kernel<<<phx::blocks_for(particle_count, kBlock), kBlock, 0, stream_of(stream)>>>(
+            particle_count, support_radius, pbf_kernel, phx::cvec(topology_positions), phx::cvec(positions),
+            parameters.masses, g, densities);
+        PHX_LAUNCH_CHECK();
+    });
+}
+
+// ------------------------------------------------------------------ DFSPH
+phx_status phx_dfsph_create(const phx_domain_configuration* domain, const phx_dfsph_configuration* configuration,
+                            phx_execution_mode mode, phx_stream stream, phx_dfsph** solver) {
+    return guarded([&] {
+        phx::require(solver != nullptr, PHX_ERROR_INVALID_ARGUMENT, "null solver output");
+        *solver = nullptr;
+        *solver = new phx_dfsph(domain, configuration, mode, stream_of(stream));
+    });
+}
+
+phx_status phx_dfsph_destroy(phx_dfsph* solver) {
+    return guarded([&] { delete solver; });
+}
+
+phx_status phx_dfsph_allocate_step_cache(phx_dfsph* solver, phx_dfsph_step_cache** cache) {
+    return guarded([&] {
+        phx::require(solver != nullptr && cache != nullptr, PHX_ERROR_INVALID_ARGUMENT, "null argument");
+        *cache = nullptr;
+        auto* c = new phx_dfsph_step_cache();
+        c->owner = solver;
+        c->impl.stream = solver->impl.stream();
+        try {
+            c->impl.allocate(solver->impl.domain().config.particle_count, solver->impl.domain().cells);
+        } catch (...) {
+            delete c;
+            throw;
+        }
+        *cache = c;
+    });
+}
+
+phx_status phx_dfsph_free_step_cache(phx_dfsph_step_cache* cache) {
+    return guarded([&] { delete cache; });
+}
+
+phx_status phx_dfsph_forward_step(phx_dfsph* solver, const phx_sph_dfsph_state* state, const phx_control* control,
+                                  const phx_sph_dfsph_parameters* parameters, phx_sph_dfsph_state* next_state,
+                                  phx_dfsph_step_cache* cache) {
+    return guarded([&] {
+        phx::require(solver && state && control && parameters && next_state && cache, PHX_ERROR_INVALID_ARGUMENT,
+                     "null argument");
+        phx::require(cache->owner == solver, PHX_ERROR_INVALID_ARGUMENT, "step cache belongs to another solver");
+        cache->valid = false;
+        solver->impl.forward_step(*state, *control, *parameters, *next_state, cache->impl);
+        cache->valid = true;
+    });
+}
+
+phx_status phx_dfsph_reference_gradient_norm(const phx_dfsph* solver, float* value) {
+    return guarded([&] {
+        phx::require(solver != nullptr && value != nullptr, PHX_ERROR_INVALID_ARGUMENT, "null argument");
+        *value = solver->impl.gradient_norm();
+    });
+}
+
+phx_status phx_dfsph_last_iterations(const phx_dfsph* solver, uint32_t iterations[2]) {
+    return guarded([&] {
+        phx::require(solver != nullptr && iterations != nullptr, PHX_ERROR_INVALID_ARGUMENT, "null argument");
+        iterations[0] = solver->impl.last_iterations()[0];
+        iterations[1] = solver->impl.last_iterations()[1];
+    });
+}
+
+phx_status phx_dfsph_cache_neighborhood(const phx_dfsph_step_cache* cache, uint64_t* sorted_keys,
+                                        uint32_t* sorted_particle_indices, uint32_t* cell_offsets) {
+    return guarded([&] {
+        phx::require(cache != nullptr && cache->valid, PHX_ERROR_INVALID_ARGUMENT, "cache holds no step");
+        phx::export_neighborhood(cache->impl.stream, cache->impl.neighborhood, sorted_keys, sorted_particle_indices,
+                                 cell_offsets);
+    });
+}
+
+phx_status phx_dfsph_cache_densities(const phx_dfsph_step_cache* cache, float* densities) {
+    return guarded([&] {
+        phx::require(cache != nullptr && cache->valid, PHX_ERROR_INVALID_ARGUMENT, "cache holds no step");
+        phx::require(densities != nullptr, PHX_ERROR_INVALID_ARGUMENT, "null output");
+        phx::ChannelList ch;
+        ch.add(cache->impl.densities.data(), densities);
+        phx::scatter_to_original(cache->impl.stream, cache->impl.neighborhood.perm.data(),
+                                 cache->impl.neighborhood.particles, ch.get());
+    });
+}
+
+phx_status phx_dfsph_cache_non_pressure_accelerations(const phx_dfsph_step_cache* cache, phx_vector_view out) {
+    return guarded([&] {
+        phx::require(cache != nullptr && cache->valid, PHX_ERROR_INVALID_ARGUMENT, "cache holds no step");
+        phx::require_vec(out, "output");
+        phx::ChannelList ch;
+        ch.add(cache->impl.non_pressure.cview(), phx::vec(out));
+        phx::scatter_to_original(cache->impl.stream, cache->impl.neighborhood.perm.data(),
+                                 cache->impl.neighborhood.particles, ch.get());
+    });
+}
+
+phx_status phx_dfsph_set_profiling(phx_dfsph* solver, int enabled) {
+    return guarded([&] {
+        phx::require(solver != nullptr, PHX_ERROR_INVALID_ARGUMENT, "null solver");
+        solver->impl.profiler().reset();
+        solver->impl.profiler().set_enabled(enabled != 0);
+    });
+}
+
+phx_status phx_dfsph_read_profile(phx_dfsph* solver, phx_pass_profile* profile, uint32_t capacity, uint32_t* count) {
+    return guarded([&] {
+        phx::require(solver != nullptr && count != nullptr, PHX_ERROR_INVALID_ARGUMENT, "null argument");
+        phx::PassProfiler& p = solver->impl.profiler();
+        p.collect();
+        const auto& d = solver->impl.domain();
+        uint32_t k = 0;
+        for (int pass = 0; pass < phx::kDfsphPassCount; ++pass) {
+            if (p.launches(pass) == 0) continue;
+            if (profile != nullptr && k < capacity) {
+                profile[k].name = phx::dfsph_pass_name(pass);
+                profile[k].total_ms = p.total_ms(pass);
+                profile[k].launches = p.launches(pass);
+                profile[k].bytes_per_particle =
+                    phx::dfsph_pass_bytes(pass, d.config.particle_count, d.cells, solver->impl.warm_start());
+            }
+            ++k;
+        }
+        *count = k;
+        p.reset();
+    });
+}
+
+// -------------------------------------------------------------------- PBF
+phx_status phx_pbf_create(const phx_domain_configuration* domain, const phx_pbf_configuration* configuration,
+                          phx_execution_mode mode, phx_stream stream, phx_pbf** solver) {
+    return guarded([&] {
+        phx::require(solver != nullptr, PHX_ERROR_INVALID_ARGUMENT, "null solver output");
+        *solver = nullptr;
+        *solver = new phx_pbf(domain, configuration, mode, stream_of(stream));
+    });
+}
+
+phx_status phx_pbf_destroy(phx_pbf* solver) {
+    return guarded([&] { delete solver; });
+}
+
+phx_status phx_pbf_allocate_step_cache(phx_pbf* solver, phx_pbf_step_cache** cache) {
+    return guarded([&] {
+        phx::require(solver != nullptr && cache != nullptr, PHX_ERROR_INVALID_ARGUMENT, "null argument");
+        *cache = nullptr;
+        auto* c = new phx_pbf_step_cache();
+        c->owner = solver;
+        try {
+            c->impl.allocate(solver->impl.domain().config.particle_count, solver->impl.domain().cells);
+        } catch (...) {
+            delete c;
+            throw;
+        }
+        *cache = c;
+    });
+}
+
+phx_status phx_pbf_free_step_cache(phx_pbf_step_cache* cache) {
+    return guarded([&] { delete cache; });
+}
+
+phx_status phx_pbf_forward_step(phx_pbf* solver, const phx_pbf_state* state, const phx_control* control,
+                                const phx_pbf_parameters* parameters, phx_pbf_state* next_state,
+                                phx_pbf_step_cache* cache) {
+    return guarded([&] {
+        phx::require(solver && state && control && parameters && next_state && cache, PHX_ERROR_INVALID_ARGUMENT,
+                     "null argument");
+        phx::require(cache->owner == solver, PHX_ERROR_INVALID_ARGUMENT, "step cache belongs to another solver");
+        cache->valid = false;
+        solver->impl.forward_step(*state, *control, *parameters, *next_state, cache->impl);
+        cache->valid = true;
+    });
+}
+
+phx_status phx_pbf_set_profiling(phx_pbf* solver, int enabled) {
+    return guarded([&] {
+        phx::require(solver != nullptr, PHX_ERROR_INVALID_ARGUMENT, "null solver");
+        solver->impl.profiler().reset();
+        solver->impl.profiler().set_enabled(enabled != 0);
+    });
+}
+
+phx_status phx_pbf_read_profile(phx_pbf* solver, phx_pass_profile* profile, uint32_t capacity, uint32_t* count) {
+    return guarded([&] {
+        phx::require(solver != nullptr && count != nullptr, PHX_ERROR_INVALID_ARGUMENT, "null argument");
+        phx::PassProfiler& p = solver->impl.profiler();
+        p.collect();
+        const auto& d = solver->impl.domain();
+        uint32_t k = 0;
+        for (int pass = 0; pass < phx::kPbfPassCount; ++pass) {
+            if (p.launches(pass) == 0) continue;
+            if (profile != nullptr && k < capacity) {
+                profile[k].name = phx::pbf_pass_name(pass);
+                profile[k].total_ms = p.total_ms(pass);
+                profile[k].launches = p.launches(pass);
+                profile[k].bytes_per_particle = phx::pbf_pass_bytes(pass, d.config.particle_count, d.cells);
+            }
+            ++k;
+        }
+        *count = k;
+        p.reset();
+    });
+}
+
+phx_status phx_pbf_cache_neighborhood(const phx_pbf_step_cache* cache, uint64_t* sorted_keys,
+                                      uint32_t* sorted_particle_indices, uint32_t* cell_offsets) {
+    return guarded([&] {
+        phx::require(cache != nullptr && cache->valid, PHX_ERROR_INVALID_ARGUMENT, "cache holds no step");
+        phx::export_neighborhood(cache->owner->impl.stream(), cache->impl.neighborhood, sorted_keys,
+                                 sorted_particle_indices, cell_offsets);
+    });
+}
+
+phx_status phx_pbf_cache_predicted_positions(const phx_pbf_step_cache* cache, phx_vector_view out) {
+    return guarded([&] {
+        phx::require(cache != nullptr && cache->valid, PHX_ERROR_INVALID_ARGUMENT, "cache holds no step");
+        phx::require_vec(out, "output");
+        const size_t bytes = sizeof(float) * cache->impl.neighborhood.particles;
+        const hipStream_t s = cache->owner->impl.stream();
+        PHX_HIP(hipMemcpyAsync(out.x, cache->impl.predicted.x.data(), bytes, hipMemcpyDeviceToDevice, s));
+        PHX_HIP(hipMemcpyAsync(out.y, cache->impl.predicted.y.data(), bytes, hipMemcpyDeviceToDevice, s));
+        PHX_HIP(hipMemcpyAsync(out.z, cache->impl.predicted.z.data(), bytes, hipMemcpyDeviceToDevice, s));
+    });
+}
+
+phx_status phx_pbf_cache_last_iteration(const phx_pbf_step_cache* cache, float* densities, float* lambdas) {
+    return guarded([&] {
+        phx::require(cache != nullptr && cache->valid, PHX_ERROR_INVALID_ARGUMENT, "cache holds no step");
+        phx::ChannelList ch;
+        if (densities != nullptr) ch.add(cache->impl.densities.data(), densities);
+        if (lambdas != nullptr) ch.add(cache->impl.lambdas.data(), lambdas);
+        phx::scatter_to_original(cache->owner->impl.stream(), cache->impl.neighborhood.perm.data(),
+                                 cache->impl.neighborhood.particles, ch.get());
+    });
+}
+
+}  // extern "C"

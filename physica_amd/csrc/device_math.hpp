@@ -1,0 +1,189 @@
+// Device-side vector math and SPH kernel functions for gfx950.
+//
+// Every function keeps the operand order of the reference's device helpers so
+// that, compiled with -ffp-contract=off (no FMA formation) and HIP's default
+// correctly rounded FP32 division and square root, each result is the IEEE
+// value of the same expression. Reference: fluids/liquid/particle/
+// domain/device.cuh:8-82, density/device.cuh:10-88, pbf/pbf.cu:18-71.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace phx {
+
+constexpr float kPi = 3.14159265358979323846F;
+
+struct Float3 {
+    float x, y, z;
+};
+
+__device__ __forceinline__ Float3 make3(float x, float y, float z) { return Float3{x, y, z}; }
+__device__ __forceinline__ Float3 add(Float3 a, Float3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ Float3 sub(Float3 a, Float3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ Float3 scale(Float3 v, float f) { return {f * v.x, f * v.y, f * v.z}; }
+__device__ __forceinline__ float dot(Float3 a, Float3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ Float3 cross(Float3 a, Float3 b) {
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+__device__ __forceinline__ float length(Float3 a) { return sqrtf(dot(a, a)); }
+
+// SoA field of three float arrays (domain/device.h:20-36).
+struct Vec {
+    float* x;
+    float* y;
+    float* z;
+};
+struct CVec {
+    const float* x;
+    const float* y;
+    const float* z;
+};
+
+__device__ __forceinline__ Float3 load(CVec f, uint32_t i) { return {f.x[i], f.y[i], f.z[i]}; }
+__device__ __forceinline__ Float3 load(Vec f, uint32_t i) { return {f.x[i], f.y[i], f.z[i]}; }
+__device__ __forceinline__ void store(Vec f, uint32_t i, Float3 v) {
+    f.x[i] = v.x;
+    f.y[i] = v.y;
+    f.z[i] = v.z;
+}
+
+// ---------------------------------------------------------------- kernels
+// poly6, density/device.cuh:10-16
+__device__ __forceinline__ float poly6(Float3 d, float h) {
+    const float sq = dot(d, d);
+    const float hr = h * h;
+    if (sq >= hr) return 0.0F;
+    const float diff = hr - sq;
+    return 315.0F * diff * diff * diff / (64.0F * kPi * hr * hr * hr * h * h * h);
+}
+
+// cubic spline, density/device.cuh:26-34
+__device__ __forceinline__ float cubic(Float3 d, float h) {
+    const float dist = length(d);
+    if (dist >= h) return 0.0F;
+    const float q = dist / h;
+    const float coef = 8.0F / (kPi * h * h * h);
+    if (q <= 0.5F) return coef * (6.0F * q * q * q - 6.0F * q * q + 1.0F);
+    const float diff = 1.0F - q;
+    return coef * 2.0F * diff * diff * diff;
+}
+
+// cubic spline gradient, density/device.cuh:36-43
+__device__ __forceinline__ Float3 cubic_gradient(Float3 d, float h) {
+    const float dist = length(d);
+    if (dist == 0.0F || dist >= h) return {0.0F, 0.0F, 0.0F};
+    const float q = dist / h;
+    const float coef = 8.0F / (kPi * h * h * h);
+    const float deriv = q <= 0.5F ? 18.0F * q * q - 12.0F * q : -6.0F * (1.0F - q) * (1.0F - q);
+    return scale(d, coef * deriv / (h * dist));
+}
+
+// viscosity Laplacian, density/device.cuh:72-76
+__device__ __forceinline__ float viscosity_laplacian(Float3 d, float h) {
+    const float dist = length(d);
+    if (dist >= h) return 0.0F;
+    return 45.0F * (h - dist) / (kPi * h * h * h * h * h * h);
+}
+
+// spiky gradient, pbf.cu:26-32. The coefficient -45/(pi h^6) is computed once
+// on the host (the reference evaluates the same powf per pair).
+__device__ __forceinline__ Float3 spiky_gradient(Float3 d, float h, float coefficient) {
+    const float dist = length(d);
+    if (dist == 0.0F || dist >= h) return {0.0F, 0.0F, 0.0F};
+    const float diff = h - dist;
+    return scale(d, coefficient * diff * diff / dist);
+}
+
+// artificial pressure value, pbf.cu:56-71 (value only). `reference` is
+// poly6((radius,0,0), h) of the particle.
+__device__ __forceinline__ float artificial_value(Float3 d, float h, float strength, float exponent, float reference) {
+    const float kernel = poly6(d, h);
+    if (kernel == 0.0F) return 0.0F;
+    const float ratio = kernel / reference;
+    const float power = powf(ratio, exponent);
+    return -strength * power;
+}
+
+// min(C-1, max(0, floor(v))) with NaN and negatives mapped to 0, as the
+// reference's __float2int_rd + clamp (neighborhood.cu:34-36).
+__device__ __forceinline__ int cell_clamp(float v, uint32_t count) {
+    const float f = floorf(v);
+    if (!(f > 0.0F)) return 0;
+    if (f >= static_cast<float>(count - 1u)) return static_cast<int>(count) - 1;
+    return static_cast<int>(f);
+}
+
+// Collision box of the solvers (sph.cpp:21-35, pbf.cpp:17-31).
+struct Box {
+    float lo[3];
+    float hi[3];
+    float vel[3];
+    uint32_t no_slip;
+};
+
+// Semi-implicit Euler with box clamp: integrate/predict body of
+// sph/common.cu:302-320 and sph/dfsph.cu:26-44.
+__device__ __forceinline__ void predict_clamped(const Box& box, float dt, Float3 x, Float3 v, Float3 a, Float3& out_x,
+                                                Float3& out_v) {
+    Float3 vel = add(v, scale(a, dt));
+    Float3 pos = add(x, scale(vel, dt));
+    const bool cx = pos.x < box.lo[0] || pos.x > box.hi[0];
+    const bool cy = pos.y < box.lo[1] || pos.y > box.hi[1];
+    const bool cz = pos.z < box.lo[2] || pos.z > box.hi[2];
+    pos.x = fminf(box.hi[0], fmaxf(box.lo[0], pos.x));
+    pos.y = fminf(box.hi[1], fmaxf(box.lo[1], pos.y));
+    pos.z = fminf(box.hi[2], fmaxf(box.lo[2], pos.z));
+    if (box.no_slip != 0u && (cx || cy || cz)) {
+        vel = {box.vel[0], box.vel[1], box.vel[2]};
+    } else {
+        if (cx) vel.x = box.vel[0];
+        if (cy) vel.y = box.vel[1];
+        if (cz) vel.z = box.vel[2];
+    }
+    out_x = pos;
+    out_v = vel;
+}
+
+// Uniform grid in sorted (cell-major) particle order: the cell range of cell c
+// is [offsets[c], offsets[c+1]) of the sorted arrays.
+struct SortedGrid {
+    const uint32_t* offsets;
+    uint32_t cx, cy, cz;
+    float ox, oy, oz;
+    float cell;
+};
+
+// particle_cell, neighborhood/device.cuh:44-48
+__device__ __forceinline__ void particle_cell(const SortedGrid& g, Float3 p, int& x, int& y, int& z) {
+    x = cell_clamp((p.x - g.ox) / g.cell, g.cx);
+    y = cell_clamp((p.y - g.oy) / g.cell, g.cy);
+    z = cell_clamp((p.z - g.oz) / g.cell, g.cz);
+}
+
+// Visits the sorted candidates of the 27 cells around (cx, cy, cz) in the
+// reference order: offset_x outer, offset_y, offset_z inner; ascending sorted
+// position (= ascending original index) within a cell.
+template <class Visit>
+__device__ __forceinline__ void for_each_candidate(const SortedGrid& g, int cx, int cy, int cz, Visit&& visit) {
+    for (int ox = -1; ox <= 1; ++ox) {
+        const int X = cx + ox;
+        if (X < 0 || X >= static_cast<int>(g.cx)) continue;
+        for (int oy = -1; oy <= 1; ++oy) {
+            const int Y = cy + oy;
+            if (Y < 0 || Y >= static_cast<int>(g.cy)) continue;
+            const uint32_t row = (static_cast<uint32_t>(Y)) * g.cx + static_cast<uint32_t>(X);
+            for (int oz = -1; oz <= 1; ++oz) {
+                const int Z = cz + oz;
+                if (Z < 0 || Z >= static_cast<int>(g.cz)) continue;
+                const uint32_t c = static_cast<uint32_t>(Z) * g.cy * g.cx + row;
+                const uint32_t first = g.offsets[c];
+                const uint32_t last = g.offsets[c + 1u];
+                for (uint32_t s = first; s < last; ++s) visit(s);
+            }
+        }
+    }
+}
+
+}  // namespace phx

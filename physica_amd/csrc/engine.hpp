@@ -1,0 +1,236 @@
+// Host-side engine internals shared by the solvers: error handling, device
+// buffers, domain validation and the uniform-grid builder.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+
+#include "device_math.hpp"
+#include "physica_amd/physica.h"
+
+namespace phx {
+
+struct Error : std::runtime_error {
+    phx_status status;
+    Error(phx_status s, const std::string& message) : std::runtime_error(message), status(s) {}
+};
+
+#define PHX_HIP(call)                                                                                               \
+    do {                                                                                                            \
+        const hipError_t phx_err_ = (call);                                                                         \
+        if (phx_err_ != hipSuccess)                                                                                 \
+            throw ::phx::Error(phx_err_ == hipErrorOutOfMemory ? PHX_ERROR_OUT_OF_MEMORY : PHX_ERROR_DEVICE,        \
+                               std::string(#call) + ": " + hipGetErrorString(phx_err_));                            \
+    } while (0)
+
+#define PHX_LAUNCH_CHECK() PHX_HIP(hipGetLastError())
+
+inline void require(bool condition, phx_status status, const std::string& message) {
+    if (!condition) throw Error(status, message);
+}
+
+// Owning device allocation (uninitialised, like the reference's
+// ::cuda::no_init buffers, domain/domain.cpp:35-57).
+template <class T>
+class DeviceBuffer {
+  public:
+    DeviceBuffer() = default;
+    explicit DeviceBuffer(size_t count) { resize(count); }
+    DeviceBuffer(const DeviceBuffer&) = delete;
+    DeviceBuffer& operator=(const DeviceBuffer&) = delete;
+    DeviceBuffer(DeviceBuffer&& other) noexcept : ptr_(other.ptr_), count_(other.count_) {
+        other.ptr_ = nullptr;
+        other.count_ = 0;
+    }
+    DeviceBuffer& operator=(DeviceBuffer&& other) noexcept {
+        if (this != &other) {
+            release();
+            ptr_ = other.ptr_;
+            count_ = other.count_;
+            other.ptr_ = nullptr;
+            other.count_ = 0;
+        }
+        return *this;
+    }
+    ~DeviceBuffer() { release(); }
+
+    void resize(size_t count) {
+        if (count == count_) return;
+        release();
+        if (count > 0) PHX_HIP(hipMalloc(reinterpret_cast<void**>(&ptr_), count * sizeof(T)));
+        count_ = count;
+    }
+    T* data() { return ptr_; }
+    const T* data() const { return ptr_; }
+    size_t size() const { return count_; }
+
+  private:
+    void release() {
+        if (ptr_ != nullptr) (void)hipFree(ptr_);
+        ptr_ = nullptr;
+        count_ = 0;
+    }
+    T* ptr_ = nullptr;
+    size_t count_ = 0;
+};
+
+// Three device float arrays of one length (VectorField, domain/domain.ixx:58-62).
+struct VecBuffer {
+    DeviceBuffer<float> x, y, z;
+    void resize(size_t n) {
+        x.resize(n);
+        y.resize(n);
+        z.resize(n);
+    }
+    Vec view() { return {x.data(), y.data(), z.data()}; }
+    CVec cview() const { return {x.data(), y.data(), z.data()}; }
+};
+
+// Validated copy of a DomainConfiguration plus derived grid quantities.
+struct Domain {
+    phx_domain_configuration config{};
+    uint32_t resolution[3]{};
+    uint32_t cells = 0;
+};
+
+Domain make_domain(const phx_domain_configuration* config);
+void grid_resolution(const phx_domain_configuration& config, uint32_t resolution[3]);
+Box collision_box(const phx_domain_configuration& config, uint64_t step);
+float reference_gradient_norm(const phx_domain_configuration& config);
+float spiky_coefficient(float support_radius);
+
+inline CVec cvec(const phx_const_vector_view& v) { return {v.x, v.y, v.z}; }
+inline CVec cvec(const phx_vector_view& v) { return {v.x, v.y, v.z}; }
+inline Vec vec(const phx_vector_view& v) { return {v.x, v.y, v.z}; }
+void require_vec(const phx_vector_view& v, const char* what);
+void require_vec(const phx_const_vector_view& v, const char* what);
+
+inline uint32_t blocks_for(uint64_t count, uint32_t block) { return static_cast<uint32_t>((count + block - 1) / block); }
+
+// Per-particle float channels copied from original order into sorted order
+// while the grid is built.
+constexpr int kMaxChannels = 20;
+struct Channels {
+    const float* src[kMaxChannels];
+    float* dst[kMaxChannels];
+    uint32_t count;
+};
+
+class ChannelList {
+  public:
+    void add(const float* src, float* dst) {
+        require(list_.count < static_cast<uint32_t>(kMaxChannels), PHX_ERROR_INVALID_ARGUMENT, "too many channels");
+        require(src != nullptr, PHX_ERROR_INVALID_ARGUMENT, "null channel source");
+        list_.src[list_.count] = src;
+        list_.dst[list_.count] = dst;
+        ++list_.count;
+    }
+    void add(CVec src, Vec dst) {
+        add(src.x, dst.x);
+        add(src.y, dst.y);
+        add(src.z, dst.z);
+    }
+    const Channels& get() const { return list_; }
+
+  private:
+    Channels list_{};
+};
+
+// Optional predict fused into the key kernel (PBF predict_forward_kernel,
+// pbf.cu:73-78): x* = x + dt (v + dt (g + u)), written in original order.
+struct PredictInput {
+    CVec velocities{};
+    CVec controls{};
+    Float3 gravity{};
+    float dt = 0.0F;
+    Vec predicted{};
+};
+
+// Scratch of one grid build, carved from a single byte buffer.
+struct GridScratch {
+    uint32_t* keys;      // cell key per particle, original order
+    uint32_t* slots;     // arrival slot within the cell (atomic)
+    uint32_t* unstable;  // particles grouped by cell, arbitrary order inside
+    uint32_t* counts;    // C + 1 histogram
+    uint32_t* partials;  // scan tile sums
+};
+size_t grid_scratch_bytes(uint32_t particles, uint32_t cells);
+GridScratch carve_grid_scratch(void* base, size_t bytes, uint32_t particles, uint32_t cells);
+
+// Geometry of a built grid (Neighborhood::cell_resolution/cell_origin/
+// cell_size/boundary_time, neighborhood/neighborhood.ixx:12-23).
+struct GridGeometry {
+    uint32_t resolution[3]{};
+    float origin[3]{};
+    float cell_size = 0.0F;
+    float time = 0.0F;
+};
+
+// Counting-sort grid build (NeighborhoodSearch::build, neighborhood.cpp:44-89):
+// key + atomic cell histogram, exclusive scan into cell offsets, unstable
+// scatter, then a per-cell rank by original index that makes the order stable
+// and gathers the requested channels into sorted order. Writes perm[N]
+// (sorted position -> original index) and offsets[C+1].
+GridGeometry build_grid(hipStream_t stream, const Domain& domain, uint64_t step, CVec positions,
+                        const PredictInput* predict, const Channels& gather, const GridScratch& scratch, uint32_t* perm,
+                        uint32_t* offsets);
+
+// Owning grid state used by the solvers: scratch plus the permutation and
+// offsets of the last build (kept in the step cache for replay).
+struct Neighborhood {
+    DeviceBuffer<uint8_t> scratch_bytes;
+    GridScratch scratch{};
+    DeviceBuffer<uint32_t> perm;
+    DeviceBuffer<uint32_t> offsets;
+    GridGeometry geometry;
+    uint32_t particles = 0;
+    uint32_t cells = 0;
+
+    void allocate(uint32_t n, uint32_t c) {
+        if (perm.size() == n && offsets.size() == static_cast<size_t>(c) + 1u && scratch_bytes.size() != 0) return;
+        particles = n;
+        cells = c;
+        perm.resize(n);
+        offsets.resize(static_cast<size_t>(c) + 1u);
+        const size_t bytes = grid_scratch_bytes(n, c);
+        scratch_bytes.resize(bytes);
+        scratch = carve_grid_scratch(scratch_bytes.data(), bytes, n, c);
+    }
+    void build(hipStream_t stream, const Domain& domain, uint64_t step, CVec positions, const PredictInput* predict,
+               const Channels& gather) {
+        allocate(domain.config.particle_count, domain.cells);
+        geometry = build_grid(stream, domain, step, positions, predict, gather, scratch, perm.data(), offsets.data());
+    }
+    SortedGrid grid() const {
+        return SortedGrid{offsets.data(),      geometry.resolution[0], geometry.resolution[1],
+                          geometry.resolution[2], geometry.origin[0],     geometry.origin[1],
+                          geometry.origin[2],     geometry.cell_size};
+    }
+};
+
+// Device-wide exclusive scan of count u32 values (in and out may alias).
+size_t exclusive_scan_partials(size_t count);
+void exclusive_scan(hipStream_t stream, const uint32_t* in, uint32_t* out, size_t count, uint32_t* partials);
+
+// Writes the reference-layout neighborhood (u64 sorted keys, sorted indices,
+// offsets) from a built Neighborhood.
+void export_neighborhood(hipStream_t stream, const Neighborhood& nb, uint64_t* sorted_keys, uint32_t* sorted_indices,
+                         uint32_t* offsets);
+
+// sorted_keys[s] = keys[perm[s]] as u64 (the reference's sorted key array).
+void export_sorted_keys(hipStream_t stream, const uint32_t* perm, const uint32_t* keys, uint32_t particles,
+                        uint64_t* sorted_keys);
+
+// Scatter sorted-order channels back to original order: dst[perm[s]] = src[s].
+void scatter_to_original(hipStream_t stream, const uint32_t* perm, uint32_t particles, const Channels& channels);
+// Gather original-order channels into sorted order: dst[s] = src[perm[s]].
+void gather_to_sorted(hipStream_t stream, const uint32_t* perm, uint32_t particles, const Channels& channels);
+
+void fill_u32(hipStream_t stream, uint32_t* data, size_t count, uint32_t value);
+void fill_f32(hipStream_t stream, float* data, size_t count, float value);
+
+}  // namespace phx

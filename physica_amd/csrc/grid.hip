@@ -1,0 +1,412 @@
+// Uniform-grid neighbour search for gfx950: counting sort by cell key.
+//
+// Semantics follow NeighborhoodSearch::build (neighborhood/neighborhood.cpp:
+// 44-89, neighborhood/neighborhood.cu:20-70): cell key per particle, stable
+// ascending sort of (key, index), cell_offsets[c] = lower_bound(keys, c).
+// The implementation differs: keys are < C, so a histogram + exclusive scan
+// gives the offsets directly, and a stable order within a cell comes from
+// ranking each particle by original index among its cell's members.
+#include <cmath>
+
+#include "engine.hpp"
+
+namespace phx {
+namespace {
+
+constexpr uint32_t kBlock = 256;
+constexpr uint32_t kScanItems = 8;
+constexpr uint32_t kScanTile = kBlock * kScanItems;
+
+__global__ void __launch_bounds__(kBlock) fill_u32_kernel(uint32_t* data, size_t count, uint32_t value) {
+    const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < count) data[i] = value;
+}
+
+__global__ void __launch_bounds__(kBlock) fill_f32_kernel(float* data, size_t count, float value) {
+    const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < count) data[i] = value;
+}
+
+struct KeyBox {
+    float minimum[3];
+    float velocity[3];
+    float h;
+    float time;
+    uint32_t res[3];
+};
+
+// key_kernel, neighborhood.cu:20-39, fused with the cell histogram.
+template <bool kPredict>
+__global__ void __launch_bounds__(kBlock)
+    key_count_kernel(uint32_t n, KeyBox box, CVec positions, PredictInput predict, uint32_t* keys, uint32_t* slots,
+                     uint32_t* counts) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    Float3 p = load(positions, i);
+    if constexpr (kPredict) {
+        const Float3 pv = add(load(predict.velocities, i), scale(add(predict.gravity, load(predict.controls, i)), predict.dt));
+        p = add(p, scale(pv, predict.dt));
+        store(predict.predicted, i, p);
+    }
+    const int cx = cell_clamp((p.x - box.minimum[0] - box.time * box.velocity[0]) / box.h, box.res[0]);
+    const int cy = cell_clamp((p.y - box.minimum[1] - box.time * box.velocity[1]) / box.h, box.res[1]);
+    const int cz = cell_clamp((p.z - box.minimum[2] - box.time * box.velocity[2]) / box.h, box.res[2]);
+    const uint32_t key = (static_cast<uint32_t>(cz) * box.res[1] + static_cast<uint32_t>(cy)) * box.res[0] +
+                         static_cast<uint32_t>(cx);
+    keys[i] = key;
+    slots[i] = atomicAdd(&counts[key], 1u);
+}
+
+__global__ void __launch_bounds__(kBlock)
+    scatter_unstable_kernel(uint32_t n, const uint32_t* keys, const uint32_t* slots, const uint32_t* offsets,
+                            uint32_t* unstable) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    unstable[offsets[keys[i]] + slots[i]] = i;
+}
+
+// Final position of particle i = cell start + number of cell members with a
+// smaller original index; equal keys therefore keep input order, which is the
+// stable-sort order of cub::DeviceRadixSort::SortPairs (neighborhood.cu:57).
+__global__ void __launch_bounds__(kBlock)
+    rank_gather_kernel(uint32_t n, const uint32_t* keys, const uint32_t* offsets, const uint32_t* unstable,
+                       uint32_t* perm, Channels channels) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t key = keys[i];
+    const uint32_t first = offsets[key];
+    const uint32_t last = offsets[key + 1u];
+    uint32_t rank = 0;
+    for (uint32_t s = first; s < last; ++s) rank += unstable[s] < i ? 1u : 0u;
+    const uint32_t pos = first + rank;
+    perm[pos] = i;
+    for (uint32_t c = 0; c < channels.count; ++c) channels.dst[c][pos] = channels.src[c][i];
+}
+
+__global__ void __launch_bounds__(kBlock)
+    export_kernel(uint32_t n, const uint32_t* perm, const uint32_t* keys, uint64_t* sorted_keys,
+                  uint32_t* sorted_indices) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const uint32_t i = perm[s];
+    if (sorted_keys != nullptr) sorted_keys[s] = static_cast<uint64_t>(keys[i]);
+    if (sorted_indices != nullptr) sorted_indices[s] = i;
+}
+
+__global__ void __launch_bounds__(kBlock) copy_u32_kernel(const uint32_t* src, uint32_t* dst, size_t count) {
+    const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < count) dst[i] = src[i];
+}
+
+__global__ void __launch_bounds__(kBlock)
+    scatter_channels_kernel(uint32_t n, const uint32_t* perm, Channels channels) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const uint32_t i = perm[s];
+    for (uint32_t c = 0; c < channels.count; ++c) channels.dst[c][i] = channels.src[c][s];
+}
+
+__global__ void __launch_bounds__(kBlock)
+    gather_channels_kernel(uint32_t n, const uint32_t* perm, Channels channels) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const uint32_t i = perm[s];
+    for (uint32_t c = 0; c < channels.count; ++c) channels.dst[c][s] = channels.src[c][i];
+}
+
+// ------------------------------------------------------------------- scan
+// Inclusive scan of one value per lane across the block (wave64 shuffles,
+// then one LDS pass over the wave totals). Returns the exclusive prefix and
+// the block total.
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t value, uint32_t& total) {
+    __shared__ uint32_t wave_totals[kBlock / 64];
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t wave = threadIdx.x >> 6;
+    uint32_t inclusive = value;
+    for (uint32_t offset = 1; offset < 64; offset <<= 1) {
+        const uint32_t up = __shfl_up(inclusive, offset, 64);
+        if (lane >= offset) inclusive += up;
+    }
+    if (lane == 63u) wave_totals[wave] = inclusive;
+    __syncthreads();
+    uint32_t wave_prefix = 0;
+    total = 0;
+    for (uint32_t w = 0; w < kBlock / 64; ++w) {
+        const uint32_t t = wave_totals[w];
+        if (w < wave) wave_prefix += t;
+        total += t;
+    }
+    __syncthreads();
+    return wave_prefix + inclusive - value;
+}
+
+__global__ void __launch_bounds__(kBlock) scan_reduce_kernel(const uint32_t* in, size_t count, uint32_t* partials) {
+    const size_t base = static_cast<size_t>(blockIdx.x) * kScanTile;
+    uint32_t sum = 0;
+    for (uint32_t k = 0; k < kScanItems; ++k) {
+        const size_t idx = base + static_cast<size_t>(k) * kBlock + threadIdx.x;
+        if (idx < count) sum += in[idx];
+    }
+    uint32_t total;
+    (void)block_exclusive_scan(sum, total);
+    if (threadIdx.x == 0) partials[blockIdx.x] = total;
+}
+
+// Exclusive scan of the per-tile sums in one block: thread t owns a
+// contiguous chunk.
+__global__ void __launch_bounds__(kBlock) scan_partials_kernel(uint32_t* partials, uint32_t count) {
+    const uint32_t chunk = (count + kBlock - 1) / kBlock;
+    const uint32_t first = threadIdx.x * chunk;
+    uint32_t sum = 0;
+    for (uint32_t k = 0; k < chunk; ++k)
+        if (first + k < count) sum += partials[first + k];
+    uint32_t total;
+    uint32_t prefix = block_exclusive_scan(sum, total);
+    for (uint32_t k = 0; k < chunk; ++k)
+        if (first + k < count) {
+            const uint32_t v = partials[first + k];
+            partials[first + k] = prefix;
+            prefix += v;
+        }
+}
+
+__global__ void __launch_bounds__(kBlock)
+    scan_downsweep_kernel(const uint32_t* in, uint32_t* out, size_t count, const uint32_t* partials) {
+    const size_t base = static_cast<size_t>(blockIdx.x) * kScanTile + static_cast<size_t>(threadIdx.x) * kScanItems;
+    uint32_t items[kScanItems];
+    uint32_t sum = 0;
+    for (uint32_t k = 0; k < kScanItems; ++k) {
+        items[k] = base + k < count ? in[base + k] : 0u;
+        sum += items[k];
+    }
+    uint32_t total;
+    uint32_t prefix = block_exclusive_scan(sum, total) + partials[blockIdx.x];
+    for (uint32_t k = 0; k < kScanItems; ++k) {
+        if (base + k < count) out[base + k] = prefix;
+        prefix += items[k];
+    }
+}
+
+}  // namespace
+
+void fill_u32(hipStream_t stream, uint32_t* data, size_t count, uint32_t value) {
+    if (count == 0) return;
+    fill_u32_kernel<<<blocks_for(count, kBlock), kBlock, 0, stream>>>(data, count, value);
+    PHX_LAUNCH_CHECK();
+}
+
+void fill_f32(hipStream_t stream, float* data, size_t count, float value) {
+    if (count == 0) return;
+    fill_f32_kernel<<<blocks_for(count, kBlock), kBlock, 0, stream>>>(data, count, value);
+    PHX_LAUNCH_CHECK();
+}
+
+size_t exclusive_scan_partials(size_t count) { return (count + kScanTile - 1) / kScanTile; }
+
+void exclusive_scan(hipStream_t stream, const uint32_t* in, uint32_t* out, size_t count, uint32_t* partials) {
+    if (count == 0) return;
+    const size_t tiles = exclusive_scan_partials(count);
+    require(tiles <= static_cast<size_t>(kBlock) * 4096u, PHX_ERROR_CAPACITY, "grid has too many cells for the scan");
+    scan_reduce_kernel<<<static_cast<uint32_t>(tiles), kBlock, 0, stream>>>(in, count, partials);
+    PHX_LAUNCH_CHECK();
+    scan_partials_kernel<<<1, kBlock, 0, stream>>>(partials, static_cast<uint32_t>(tiles));
+    PHX_LAUNCH_CHECK();
+    scan_downsweep_kernel<<<static_cast<uint32_t>(tiles), kBlock, 0, stream>>>(in, out, count, partials);
+    PHX_LAUNCH_CHECK();
+}
+
+// ------------------------------------------------------------------- host
+void grid_resolution(const phx_domain_configuration& c, uint32_t res[3]) {
+    // neighborhood.cpp:25-30 (FP32 ceil of the extent over h)
+    res[0] = static_cast<uint32_t>(std::ceil((c.boundary.maximum.x - c.boundary.minimum.x) / c.support_radius));
+    res[1] = static_cast<uint32_t>(std::ceil((c.boundary.maximum.y - c.boundary.minimum.y) / c.support_radius));
+    res[2] = static_cast<uint32_t>(std::ceil((c.boundary.maximum.z - c.boundary.minimum.z) / c.support_radius));
+}
+
+void require_vec(const phx_vector_view& v, const char* what) {
+    require(v.x != nullptr && v.y != nullptr && v.z != nullptr, PHX_ERROR_INVALID_ARGUMENT,
+            std::string("null device pointer in ") + what);
+}
+void require_vec(const phx_const_vector_view& v, const char* what) {
+    require(v.x != nullptr && v.y != nullptr && v.z != nullptr, PHX_ERROR_INVALID_ARGUMENT,
+            std::string("null device pointer in ") + what);
+}
+
+Domain make_domain(const phx_domain_configuration* config) {
+    require(config != nullptr, PHX_ERROR_INVALID_ARGUMENT, "null domain configuration");
+    Domain d;
+    d.config = *config;
+    const auto& c = d.config;
+    require(std::isfinite(c.support_radius) && c.support_radius > 0.0F, PHX_ERROR_INVALID_ARGUMENT,
+            "support_radius must be positive and finite");
+    require(std::isfinite(c.time_step) && c.time_step > 0.0F, PHX_ERROR_INVALID_ARGUMENT,
+            "time_step must be positive and finite");
+    require(c.boundary.maximum.x > c.boundary.minimum.x && c.boundary.maximum.y > c.boundary.minimum.y &&
+                c.boundary.maximum.z > c.boundary.minimum.z,
+            PHX_ERROR_INVALID_ARGUMENT, "boundary box must have positive extent");
+    require(c.boundary_particle_count == 0u, PHX_ERROR_UNSUPPORTED,
+            "boundary particles are not supported yet (boundary_particle_count must be 0)");
+    grid_resolution(c, d.resolution);
+    const uint64_t cells = static_cast<uint64_t>(d.resolution[0]) * d.resolution[1] * d.resolution[2];
+    require(d.resolution[0] > 0 && d.resolution[1] > 0 && d.resolution[2] > 0, PHX_ERROR_INVALID_ARGUMENT,
+            "empty grid");
+    require(cells < (1ull << 31), PHX_ERROR_CAPACITY, "grid has 2^31 cells or more");
+    require(d.resolution[0] < (1u << 24) && d.resolution[1] < (1u << 24) && d.resolution[2] < (1u << 24),
+            PHX_ERROR_CAPACITY, "grid axis resolution must be below 2^24");
+    d.cells = static_cast<uint32_t>(cells);
+    return d;
+}
+
+Box collision_box(const phx_domain_configuration& c, uint64_t step) {
+    // sph.cpp:21-35, pbf.cpp:17-31
+    const float time = static_cast<float>(step) * c.time_step;
+    const float rp = c.particle_radius;
+    Box b{};
+    b.lo[0] = c.boundary.minimum.x + time * c.boundary.velocity.x + rp;
+    b.lo[1] = c.boundary.minimum.y + time * c.boundary.velocity.y + rp;
+    b.lo[2] = c.boundary.minimum.z + time * c.boundary.velocity.z + rp;
+    b.hi[0] = c.boundary.maximum.x + time * c.boundary.velocity.x - rp;
+    b.hi[1] = c.boundary.maximum.y + time * c.boundary.velocity.y - rp;
+    b.hi[2] = c.boundary.maximum.z + time * c.boundary.velocity.z - rp;
+    b.vel[0] = c.boundary.velocity.x;
+    b.vel[1] = c.boundary.velocity.y;
+    b.vel[2] = c.boundary.velocity.z;
+    b.no_slip = c.boundary.no_slip != 0u ? 1u : 0u;
+    return b;
+}
+
+float reference_gradient_norm(const phx_domain_configuration& c) {
+    // compute_reference_gradient_norm, sph.cpp:37-58: FP32 sum of |grad W|^2
+    // plus |sum grad W|^2 over a cubic lattice of spacing 2 r_p.
+    const float h = c.support_radius;
+    const float diameter = 2.0F * c.particle_radius;
+    const float coefficient = 8.0F / (kPi * h * h * h);
+    float gx = 0.0F, gy = 0.0F, gz = 0.0F, squared = 0.0F;
+    require(diameter > 0.0F, PHX_ERROR_INVALID_ARGUMENT, "particle_radius must be positive");
+    for (float x = -h; x <= h; x += diameter)
+        for (float y = -h; y <= h; y += diameter)
+            for (float z = -h; z <= h; z += diameter) {
+                const float distance = std::sqrt(x * x + y * y + z * z);
+                if (distance == 0.0F || distance >= h) continue;
+                const float q = distance / h;
+                const float derivative = q <= 0.5F ? 18.0F * q * q - 12.0F * q : -6.0F * (1.0F - q) * (1.0F - q);
+                const float s = coefficient * derivative / (h * distance);
+                gx += s * x;
+                gy += s * y;
+                gz += s * z;
+                squared += s * s * (x * x + y * y + z * z);
+            }
+    return gx * gx + gy * gy + gz * gz + squared;
+}
+
+float spiky_coefficient(float h) { return -45.0F / (kPi * std::pow(h, 6.0F)); }
+
+size_t grid_scratch_bytes(uint32_t particles, uint32_t cells) {
+    auto round = [](size_t b) { return (b + 255u) & ~static_cast<size_t>(255u); };
+    const size_t n = particles;
+    const size_t c = static_cast<size_t>(cells) + 1u;
+    return round(n * 4u) * 3u + round(c * 4u) + round(exclusive_scan_partials(c) * 4u) + 256u;
+}
+
+GridScratch carve_grid_scratch(void* base, size_t bytes, uint32_t particles, uint32_t cells) {
+    require(bytes >= grid_scratch_bytes(particles, cells), PHX_ERROR_WORKSPACE_TOO_SMALL, "grid workspace too small");
+    auto round = [](size_t b) { return (b + 255u) & ~static_cast<size_t>(255u); };
+    uintptr_t p = (reinterpret_cast<uintptr_t>(base) + 255u) & ~static_cast<uintptr_t>(255u);
+    const size_t n = particles;
+    const size_t c = static_cast<size_t>(cells) + 1u;
+    GridScratch s{};
+    s.keys = reinterpret_cast<uint32_t*>(p);
+    p += round(n * 4u);
+    s.slots = reinterpret_cast<uint32_t*>(p);
+    p += round(n * 4u);
+    s.unstable = reinterpret_cast<uint32_t*>(p);
+    p += round(n * 4u);
+    s.counts = reinterpret_cast<uint32_t*>(p);
+    p += round(c * 4u);
+    s.partials = reinterpret_cast<uint32_t*>(p);
+    return s;
+}
+
+GridGeometry build_grid(hipStream_t stream, const Domain& domain, uint64_t step, CVec positions,
+                        const PredictInput* predict, const Channels& gather, const GridScratch& scratch, uint32_t* perm,
+                        uint32_t* offsets) {
+    const auto& c = domain.config;
+    const uint32_t n = c.particle_count;
+    const uint32_t cells = domain.cells;
+    const float time = static_cast<float>(step) * c.time_step;
+    GridGeometry geo;
+    for (int a = 0; a < 3; ++a) geo.resolution[a] = domain.resolution[a];
+    // neighborhood.cpp:83-88: lookup origin min + t v_box, cell size h
+    geo.origin[0] = c.boundary.minimum.x + time * c.boundary.velocity.x;
+    geo.origin[1] = c.boundary.minimum.y + time * c.boundary.velocity.y;
+    geo.origin[2] = c.boundary.minimum.z + time * c.boundary.velocity.z;
+    geo.cell_size = c.support_radius;
+    geo.time = time;
+
+    fill_u32(stream, scratch.counts, static_cast<size_t>(cells) + 1u, 0u);
+    if (n == 0) {
+        fill_u32(stream, offsets, static_cast<size_t>(cells) + 1u, 0u);
+        return geo;
+    }
+    KeyBox box{};
+    box.minimum[0] = c.boundary.minimum.x;
+    box.minimum[1] = c.boundary.minimum.y;
+    box.minimum[2] = c.boundary.minimum.z;
+    box.velocity[0] = c.boundary.velocity.x;
+    box.velocity[1] = c.boundary.velocity.y;
+    box.velocity[2] = c.boundary.velocity.z;
+    box.h = c.support_radius;
+    box.time = time;
+    for (int a = 0; a < 3; ++a) box.res[a] = domain.resolution[a];
+    const uint32_t grid = blocks_for(n, kBlock);
+    if (predict != nullptr) {
+        key_count_kernel<true><<<grid, kBlock, 0, stream>>>(n, box, positions, *predict, scratch.keys, scratch.slots,
+                                                             scratch.counts);
+    } else {
+        key_count_kernel<false><<<grid, kBlock, 0, stream>>>(n, box, positions, PredictInput{}, scratch.keys,
+                                                              scratch.slots, scratch.counts);
+    }
+    PHX_LAUNCH_CHECK();
+    exclusive_scan(stream, scratch.counts, offsets, static_cast<size_t>(cells) + 1u, scratch.partials);
+    scatter_unstable_kernel<<<grid, kBlock, 0, stream>>>(n, scratch.keys, scratch.slots, offsets, scratch.unstable);
+    PHX_LAUNCH_CHECK();
+    rank_gather_kernel<<<grid, kBlock, 0, stream>>>(n, scratch.keys, offsets, scratch.unstable, perm, gather);
+    PHX_LAUNCH_CHECK();
+    return geo;
+}
+
+void export_neighborhood(hipStream_t stream, const Neighborhood& nb, uint64_t* sorted_keys, uint32_t* sorted_indices,
+                         uint32_t* offsets) {
+    const uint32_t particles = nb.particles;
+    if (particles > 0 && (sorted_keys != nullptr || sorted_indices != nullptr)) {
+        export_kernel<<<blocks_for(particles, kBlock), kBlock, 0, stream>>>(particles, nb.perm.data(),
+                                                                            nb.scratch.keys, sorted_keys, sorted_indices);
+        PHX_LAUNCH_CHECK();
+    }
+    if (offsets != nullptr) {
+        const size_t count = static_cast<size_t>(nb.cells) + 1u;
+        copy_u32_kernel<<<blocks_for(count, kBlock), kBlock, 0, stream>>>(nb.offsets.data(), offsets, count);
+        PHX_LAUNCH_CHECK();
+    }
+}
+
+void export_sorted_keys(hipStream_t stream, const uint32_t* perm, const uint32_t* keys, uint32_t particles,
+                        uint64_t* sorted_keys) {
+    if (particles == 0) return;
+    export_kernel<<<blocks_for(particles, kBlock), kBlock, 0, stream>>>(particles, perm, keys, sorted_keys, nullptr);
+    PHX_LAUNCH_CHECK();
+}
+
+void scatter_to_original(hipStream_t stream, const uint32_t* perm, uint32_t particles, const Channels& channels) {
+    if (particles == 0 || channels.count == 0) return;
+    scatter_channels_kernel<<<blocks_for(particles, kBlock), kBlock, 0, stream>>>(particles, perm, channels);
+    PHX_LAUNCH_CHECK();
+}
+
+void gather_to_sorted(hipStream_t stream, const uint32_t* perm, uint32_t particles, const Channels& channels) {
+    if (particles == 0 || channels.count == 0) return;
+    gather_channels_kernel<<<blocks_for(particles, kBlock), kBlock, 0, stream>>>(particles, perm, channels);
+    PHX_LAUNCH_CHECK();
+}
+
+}  // namespace phx

@@ -1,0 +1,365 @@
+// PBF forward step for gfx950 in sorted (cell-major) particle order.
+//
+// Reference: PBF::forward_step (pbf/pbf.cpp:167-191) and pbf/pbf.cu. Fusion:
+//   grid       predict x* fused into the cell-key pass (pbf.cu:73-78)
+//   per iter   A: density + lambda in one neighbour loop (density.cu:18-40,
+//                 pbf.cu:95-127; the two sums keep their own order)
+//              B: correction + project (pbf.cu:254-288, 417-431), written to
+//                 a second position buffer so neighbours read the old values;
+//                 the last iteration also reconstructs v = (p - x)/dt
+//   post       vorticity + magnitude (pbf.cu:476-497, 565-569)
+//              normal + confinement (pbf.cu:571-593, 674-678)
+//              XSPH + scatter of the next state (pbf.cu:700-721)
+// Per-particle sums run in the reference order, so each value is the IEEE
+// result of the reference expression.
+#include "pbf.hpp"
+
+namespace phx {
+namespace {
+
+constexpr uint32_t kBlock = 256;
+
+struct LambdaArgs {
+    uint32_t n;
+    float h;
+    float spiky;
+    SortedGrid grid;
+    CVec topo;
+    CVec pos;
+    const float* m;
+    const float* rho0;
+    const float* eps;
+    float* rho;
+    float* lambda;
+};
+
+__global__ void __launch_bounds__(kBlock) pbf_lambda_kernel(LambdaArgs a) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    const Float3 p = load(a.pos, i);
+    const float rest = a.rho0[i];
+    int cx, cy, cz;
+    particle_cell(a.grid, load(a.topo, i), cx, cy, cz);
+    float density = 0.0F;
+    Float3 gradient_sum{0.0F, 0.0F, 0.0F};
+    float denominator = a.eps[i];
+    for_each_candidate(a.grid, cx, cy, cz, [&](uint32_t s) {
+        const Float3 d = sub(p, load(a.pos, s));
+        const float ms = a.m[s];
+        density += ms * poly6(d, a.h);
+        if (s == i) return;
+        const Float3 g = spiky_gradient(d, a.h, a.spiky);
+        const float coefficient = ms / rest;
+        gradient_sum = add(gradient_sum, scale(g, coefficient));
+        denominator += coefficient * coefficient * dot(g, g);
+    });
+    denominator += dot(gradient_sum, gradient_sum);
+    a.rho[i] = density;
+    a.lambda[i] = -(density / rest - 1.0F) / denominator;
+}
+
+struct CorrectionArgs {
+    uint32_t n;
+    float h;
+    float spiky;
+    SortedGrid grid;
+    Box box;
+    CVec topo;
+    CVec pos;
+    const float* m;
+    const float* rho0;
+    const float* lambda;
+    const float* strength;
+    const float* exponent;
+    const float* radius;
+    Vec out;
+    uint32_t reconstruct;
+    CVec x0;
+    float inv_dt;
+    Vec vel;
+};
+
+__global__ void __launch_bounds__(kBlock) pbf_correction_kernel(CorrectionArgs a) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    const Float3 p = load(a.pos, i);
+    const float rest = a.rho0[i];
+    const float lambda_i = a.lambda[i];
+    const float strength = a.strength[i];
+    const float exponent = a.exponent[i];
+    const float reference = poly6(Float3{a.radius[i], 0.0F, 0.0F}, a.h);
+    int cx, cy, cz;
+    particle_cell(a.grid, load(a.topo, i), cx, cy, cz);
+    Float3 correction{0.0F, 0.0F, 0.0F};
+    for_each_candidate(a.grid, cx, cy, cz, [&](uint32_t s) {
+        if (s == i) return;
+        const Float3 d = sub(p, load(a.pos, s));
+        const float artificial = artificial_value(d, a.h, strength, exponent, reference);
+        const float coefficient = a.m[s] / rest;
+        correction = add(correction, scale(spiky_gradient(d, a.h, a.spiky), coefficient * (lambda_i + a.lambda[s] + artificial)));
+    });
+    const Float3 candidate = add(p, correction);
+    const Float3 projected{fminf(a.box.hi[0], fmaxf(a.box.lo[0], candidate.x)),
+                           fminf(a.box.hi[1], fmaxf(a.box.lo[1], candidate.y)),
+                           fminf(a.box.hi[2], fmaxf(a.box.lo[2], candidate.z))};
+    store(a.out, i, projected);
+    if (a.reconstruct != 0u) store(a.vel, i, scale(sub(projected, load(a.x0, i)), a.inv_dt));
+}
+
+__global__ void __launch_bounds__(kBlock) pbf_reconstruct_kernel(uint32_t n, CVec pos, CVec x0, float inv_dt, Vec vel) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    store(vel, i, scale(sub(load(pos, i), load(x0, i)), inv_dt));
+}
+
+struct PostArgs {
+    uint32_t n;
+    float h;
+    float spiky;
+    float dt;
+    SortedGrid grid;
+    CVec topo;
+    CVec pos;
+    CVec vel;
+    Vec vort;
+    float* magnitude;
+    const float* confinement;
+    Vec vc;
+    const float* xsph;
+    const uint32_t* perm;
+    Vec next_x, next_v;
+};
+
+__global__ void __launch_bounds__(kBlock) pbf_vorticity_kernel(PostArgs a) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    const Float3 p = load(a.pos, i);
+    const Float3 v = load(a.vel, i);
+    int cx, cy, cz;
+    particle_cell(a.grid, load(a.topo, i), cx, cy, cz);
+    Float3 w{0.0F, 0.0F, 0.0F};
+    for_each_candidate(a.grid, cx, cy, cz, [&](uint32_t s) {
+        if (s == i) return;
+        w = add(w, cross(sub(load(a.vel, s), v), spiky_gradient(sub(p, load(a.pos, s)), a.h, a.spiky)));
+    });
+    store(a.vort, i, w);
+    a.magnitude[i] = length(w);
+}
+
+__global__ void __launch_bounds__(kBlock) pbf_normal_kernel(PostArgs a) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    const Float3 p = load(a.pos, i);
+    const float mag_i = a.magnitude[i];
+    int cx, cy, cz;
+    particle_cell(a.grid, load(a.topo, i), cx, cy, cz);
+    Float3 g{0.0F, 0.0F, 0.0F};
+    for_each_candidate(a.grid, cx, cy, cz, [&](uint32_t s) {
+        if (s == i) return;
+        g = add(g, scale(spiky_gradient(sub(p, load(a.pos, s)), a.h, a.spiky), a.magnitude[s] - mag_i));
+    });
+    const float normalizer = length(g);
+    const Float3 normal = normalizer > 0.0F ? scale(g, 1.0F / normalizer) : Float3{0.0F, 0.0F, 0.0F};
+    const Float3 w = load(CVec{a.vort.x, a.vort.y, a.vort.z}, i);
+    store(a.vc, i, add(load(a.vel, i), scale(cross(normal, w), a.dt * a.confinement[i])));
+}
+
+__global__ void __launch_bounds__(kBlock) pbf_xsph_kernel(PostArgs a) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    const CVec vc{a.vc.x, a.vc.y, a.vc.z};
+    const Float3 p = load(a.pos, i);
+    const Float3 v = load(vc, i);
+    int cx, cy, cz;
+    particle_cell(a.grid, load(a.topo, i), cx, cy, cz);
+    Float3 smoothing{0.0F, 0.0F, 0.0F};
+    for_each_candidate(a.grid, cx, cy, cz, [&](uint32_t s) {
+        if (s == i) return;
+        smoothing = add(smoothing, scale(sub(load(vc, s), v), poly6(sub(p, load(a.pos, s)), a.h)));
+    });
+    const uint32_t j = a.perm[i];
+    store(a.next_v, j, add(v, scale(smoothing, a.xsph[i])));
+    store(a.next_x, j, p);
+}
+
+}  // namespace
+
+const char* pbf_pass_name(int pass) {
+    static const char* names[kPbfPassCount] = {"grid", "density_lambda", "correction_project", "vorticity",
+                                               "normal_confinement", "xsph"};
+    return pass >= 0 && pass < kPbfPassCount ? names[pass] : "unknown";
+}
+
+double pbf_pass_bytes(int pass, uint32_t particles, uint32_t cells) {
+    const double offsets = particles > 0 ? 4.0 * (static_cast<double>(cells) + 1.0) / particles : 0.0;
+    switch (pass) {
+        case kPbfGrid:  // x, v, u in, x* out (original order); x*, x + 8 parameters gathered; permutation
+            return 36.0 + 12.0 + 2.0 * (24.0 + 32.0) + 4.0 + 2.0 * offsets;
+        case kPbfLambda:  // x*, p, m, rho0, eps in; rho, lambda out
+            return 36.0 + 8.0 + offsets;
+        case kPbfCorrection:  // x*, p, m, rho0, lambda, s_corr k, n, dq in; p' out (+ x in, v out on the last)
+            return 48.0 + 12.0 + offsets;
+        case kPbfVorticity:  // x*, p, v in; omega, |omega| out
+            return 36.0 + 16.0 + offsets;
+        case kPbfNormal:  // x*, p, |omega|, omega, v, confinement in; v_c out
+            return 56.0 + 12.0 + offsets;
+        case kPbfXsph:  // x*, p, v_c, xsph in; next x, v out + permutation
+            return 40.0 + 24.0 + 4.0 + offsets;
+        default:
+            return 0.0;
+    }
+}
+
+void PbfCache::allocate(uint32_t n, uint32_t cells) {
+    neighborhood.allocate(n, cells);
+    predicted.resize(n);
+    densities.resize(n);
+    lambdas.resize(n);
+}
+
+PbfSolver::PbfSolver(const phx_domain_configuration* domain, const phx_pbf_configuration* configuration,
+                     phx_execution_mode mode, hipStream_t stream)
+    : domain_(make_domain(domain)), mode_(mode), stream_(stream) {
+    require(configuration != nullptr, PHX_ERROR_INVALID_ARGUMENT, "null PBF configuration");
+    config_ = *configuration;
+    require(config_.checkpoint_interval > 0u, PHX_ERROR_INVALID_ARGUMENT, "checkpoint_interval must be positive");
+    require(mode == PHX_MODE_FORWARD, PHX_ERROR_UNSUPPORTED, "differentiable mode is not supported yet");
+    spiky_ = spiky_coefficient(domain_.config.support_radius);
+    const uint32_t n = domain_.config.particle_count;
+    for (VecBuffer* b : {&xs_, &x0_, &pa_, &pb_, &vel_, &vort_, &vc_}) b->resize(n);
+    for (DeviceBuffer<float>* b : {&m_, &rho0_, &eps_, &strength_, &exponent_, &radius_, &xsph_, &confinement_, &magnitude_})
+        b->resize(n);
+}
+
+void PbfSolver::forward_step(const phx_pbf_state& state, const phx_control& control,
+                             const phx_pbf_parameters& parameters, phx_pbf_state& next, PbfCache& cache) {
+    const auto& c = domain_.config;
+    const uint32_t n = c.particle_count;
+    const uint64_t step = state.particles.step_index;
+    if (n == 0) {
+        next.particles.step_index = step + 1u;
+        return;
+    }
+    require_vec(state.particles.positions, "state positions");
+    require_vec(state.particles.velocities, "state velocities");
+    require_vec(control.external_accelerations, "control");
+    require_vec(next.particles.positions, "next positions");
+    require_vec(next.particles.velocities, "next velocities");
+    const auto& pp = parameters.particles;
+    require(pp.masses && pp.rest_densities, PHX_ERROR_INVALID_ARGUMENT, "null particle parameters");
+    require(parameters.relaxation && parameters.artificial_pressure_strength && parameters.artificial_pressure_exponent &&
+                parameters.artificial_pressure_radius && parameters.xsph_viscosity && parameters.vorticity_confinement,
+            PHX_ERROR_INVALID_ARGUMENT, "null PBF parameters");
+    cache.allocate(n, domain_.cells);
+
+    // predict (pbf.cu:73-78) fused into the key pass of build(step + 1, x*)
+    PredictInput predict{};
+    predict.velocities = cvec(state.particles.velocities);
+    predict.controls = cvec(control.external_accelerations);
+    predict.gravity = Float3{c.gravity.x, c.gravity.y, c.gravity.z};
+    predict.dt = c.time_step;
+    predict.predicted = cache.predicted.view();
+    ChannelList gather;
+    gather.add(cache.predicted.cview(), xs_.view());
+    gather.add(cvec(state.particles.positions), x0_.view());
+    gather.add(pp.masses, m_.data());
+    gather.add(pp.rest_densities, rho0_.data());
+    gather.add(parameters.relaxation, eps_.data());
+    gather.add(parameters.artificial_pressure_strength, strength_.data());
+    gather.add(parameters.artificial_pressure_exponent, exponent_.data());
+    gather.add(parameters.artificial_pressure_radius, radius_.data());
+    gather.add(parameters.xsph_viscosity, xsph_.data());
+    gather.add(parameters.vorticity_confinement, confinement_.data());
+    {
+        PassScope scope(profiler_, stream_, kPbfGrid);
+        cache.neighborhood.build(stream_, domain_, step + 1u, cvec(state.particles.positions), &predict,
+                                 gather.get());
+    }
+
+    const SortedGrid grid = cache.neighborhood.grid();
+    const uint32_t blocks = blocks_for(n, kBlock);
+    const float h = c.support_radius;
+    const float inv_dt = 1.0F / c.time_step;
+
+    LambdaArgs la{n, h, spiky_, grid, xs_.cview(), xs_.cview(), m_.data(), rho0_.data(), eps_.data(),
+                  cache.densities.data(), cache.lambdas.data()};
+    CorrectionArgs ca{};
+    ca.n = n;
+    ca.h = h;
+    ca.spiky = spiky_;
+    ca.grid = grid;
+    ca.box = collision_box(c, step + 1u);
+    ca.topo = xs_.cview();
+    ca.m = m_.data();
+    ca.rho0 = rho0_.data();
+    ca.lambda = cache.lambdas.data();
+    ca.strength = strength_.data();
+    ca.exponent = exponent_.data();
+    ca.radius = radius_.data();
+    ca.x0 = x0_.cview();
+    ca.inv_dt = inv_dt;
+    ca.vel = vel_.view();
+
+    // positions start at x* (pbf.cpp:171) and ping-pong between two buffers
+    CVec current = xs_.cview();
+    VecBuffer* targets[2] = {&pa_, &pb_};
+    const uint32_t iterations = config_.pressure_iterations;
+    for (uint32_t it = 0; it < iterations; ++it) {
+        la.pos = current;
+        {
+            PassScope scope(profiler_, stream_, kPbfLambda);
+            pbf_lambda_kernel<<<blocks, kBlock, 0, stream_>>>(la);
+            PHX_LAUNCH_CHECK();
+        }
+        VecBuffer* target = targets[it & 1u];
+        ca.pos = current;
+        ca.out = target->view();
+        ca.reconstruct = it + 1u == iterations ? 1u : 0u;
+        {
+            PassScope scope(profiler_, stream_, kPbfCorrection);
+            pbf_correction_kernel<<<blocks, kBlock, 0, stream_>>>(ca);
+            PHX_LAUNCH_CHECK();
+        }
+        current = target->cview();
+    }
+    if (iterations == 0u) {
+        pbf_reconstruct_kernel<<<blocks, kBlock, 0, stream_>>>(n, current, x0_.cview(), inv_dt, vel_.view());
+        PHX_LAUNCH_CHECK();
+    }
+
+    PostArgs post{};
+    post.n = n;
+    post.h = h;
+    post.spiky = spiky_;
+    post.dt = c.time_step;
+    post.grid = grid;
+    post.topo = xs_.cview();
+    post.pos = current;
+    post.vel = vel_.cview();
+    post.vort = vort_.view();
+    post.magnitude = magnitude_.data();
+    post.confinement = confinement_.data();
+    post.vc = vc_.view();
+    post.xsph = xsph_.data();
+    post.perm = cache.neighborhood.perm.data();
+    post.next_x = vec(next.particles.positions);
+    post.next_v = vec(next.particles.velocities);
+    {
+        PassScope scope(profiler_, stream_, kPbfVorticity);
+        pbf_vorticity_kernel<<<blocks, kBlock, 0, stream_>>>(post);
+        PHX_LAUNCH_CHECK();
+    }
+    {
+        PassScope scope(profiler_, stream_, kPbfNormal);
+        pbf_normal_kernel<<<blocks, kBlock, 0, stream_>>>(post);
+        PHX_LAUNCH_CHECK();
+    }
+    {
+        PassScope scope(profiler_, stream_, kPbfXsph);
+        pbf_xsph_kernel<<<blocks, kBlock, 0, stream_>>>(post);
+        PHX_LAUNCH_CHECK();
+    }
+    next.particles.step_index = step + 1u;
+}
+
+}  // namespace phx

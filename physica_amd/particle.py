@@ -1,0 +1,586 @@
+"""Python mirror of the reference's particle-liquid solver API.
+
+Names and argument meaning follow physica.fluids.liquid.particle
+(fluids/liquid/particle/{domain,sph,pbf}/*.ixx): a DomainConfiguration, SoA
+VectorFields, ParticleState/Control/ParticleParameters, and solver objects
+with allocate_* and forward_step(state, control, parameters, next_state,
+cache). Device memory is held in torch tensors; every step runs through the
+HIP library (libphysica_amd.so) and fails loudly if it is missing.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+from dataclasses import dataclass, field
+from enum import IntEnum
+from typing import Optional, Sequence, Tuple
+
+import numpy as np
+import torch
+
+from . import _capi
+
+
+# ------------------------------------------------------------ configuration
+@dataclass
+class BoxBoundary:
+    """BoxBoundary, domain/domain.ixx:18-23."""
+
+    minimum: Tuple[float, float, float] = (0.0, 0.0, 0.0)
+    maximum: Tuple[float, float, float] = (1.0, 1.0, 1.0)
+    velocity: Tuple[float, float, float] = (0.0, 0.0, 0.0)
+    no_slip: bool = True
+
+
+@dataclass
+class DomainConfiguration:
+    """DomainConfiguration, domain/domain.ixx:31-39."""
+
+    particle_count: int
+    time_step: float
+    support_radius: float
+    particle_radius: float
+    gravity: Tuple[float, float, float] = (0.0, -9.81, 0.0)
+    boundary: BoxBoundary = field(default_factory=BoxBoundary)
+    boundary_particles: Sequence = ()
+
+    def to_c(self) -> _capi.DomainConfiguration:
+        c = _capi.DomainConfiguration()
+        c.particle_count = int(self.particle_count)
+        c.time_step = float(self.time_step)
+        c.support_radius = float(self.support_radius)
+        c.particle_radius = float(self.particle_radius)
+        c.gravity = _capi.Vector3(*map(float, self.gravity))
+        c.boundary = _capi.BoxBoundary(
+            _capi.Vector3(*map(float, self.boundary.minimum)),
+            _capi.Vector3(*map(float, self.boundary.maximum)),
+            _capi.Vector3(*map(float, self.boundary.velocity)),
+            1 if self.boundary.no_slip else 0,
+        )
+        count = len(self.boundary_particles)
+        if count:
+            arr = (_capi.BoundaryParticle * count)()
+            for k, (pos, vel, vol) in enumerate(self.boundary_particles):
+                arr[k] = _capi.BoundaryParticle(_capi.Vector3(*pos), _capi.Vector3(*vel), float(vol))
+            c.boundary_particles = ctypes.cast(arr, ctypes.POINTER(_capi.BoundaryParticle))
+            c._keep = arr  # keep the host array alive with the struct
+        c.boundary_particle_count = count
+        return c
+
+    def grid_resolution(self) -> Tuple[int, int, int]:
+        res = (ctypes.c_uint32 * 3)()
+        _capi.call("phx_grid_resolution", ctypes.byref(self.to_c()), res)
+        return int(res[0]), int(res[1]), int(res[2])
+
+
+class ExecutionMode(IntEnum):
+    """ExecutionMode, domain/domain.ixx:41-44."""
+
+    forward = 0
+    differentiable = 1
+
+
+# ------------------------------------------------------------------ fields
+def _ptr(t: Optional[torch.Tensor]) -> Optional[int]:
+    if t is None:
+        return None
+    if not t.is_contiguous():
+        raise ValueError("device buffers must be contiguous")
+    return t.data_ptr()
+
+
+@dataclass
+class VectorField:
+    """VectorField (SoA x|y|z), domain/domain.ixx:58-62."""
+
+    x: torch.Tensor
+    y: torch.Tensor
+    z: torch.Tensor
+
+    def view(self) -> _capi.VectorView:
+        return _capi.VectorView(_ptr(self.x), _ptr(self.y), _ptr(self.z))
+
+    def numpy(self) -> np.ndarray:
+        return np.stack([self.x.cpu().numpy(), self.y.cpu().numpy(), self.z.cpu().numpy()])
+
+    def copy_(self, other: "VectorField") -> None:
+        self.x.copy_(other.x)
+        self.y.copy_(other.y)
+        self.z.copy_(other.z)
+
+    def upload(self, values: np.ndarray) -> None:
+        """Domain::upload for a (3, N) host array (domain.cpp:206-218)."""
+        values = np.asarray(values, dtype=np.float32)
+        self.x.copy_(torch.from_numpy(np.ascontiguousarray(values[0])))
+        self.y.copy_(torch.from_numpy(np.ascontiguousarray(values[1])))
+        self.z.copy_(torch.from_numpy(np.ascontiguousarray(values[2])))
+
+    @staticmethod
+    def zeros(count: int, device) -> "VectorField":
+        return VectorField(*(torch.zeros(count, dtype=torch.float32, device=device) for _ in range(3)))
+
+
+@dataclass
+class ParticleState:
+    """ParticleState, domain/domain.ixx:70-74."""
+
+    positions: VectorField
+    velocities: VectorField
+    step_index: int = 0
+
+    def to_c(self) -> _capi.ParticleState:
+        return _capi.ParticleState(self.positions.view(), self.velocities.view(), int(self.step_index))
+
+
+@dataclass
+class Control:
+    """Control, domain/domain.ixx:76-78."""
+
+    external_accelerations: VectorField
+
+    def to_c(self) -> _capi.Control:
+        return _capi.Control(self.external_accelerations.view())
+
+
+@dataclass
+class ParticleParameters:
+    """ParticleParameters, domain/domain.ixx:100-105."""
+
+    masses: torch.Tensor
+    rest_densities: torch.Tensor
+    viscosities: torch.Tensor
+    surface_tensions: torch.Tensor
+
+    def to_c(self) -> _capi.ParticleParameters:
+        return _capi.ParticleParameters(
+            _ptr(self.masses), _ptr(self.rest_densities), _ptr(self.viscosities), _ptr(self.surface_tensions)
+        )
+
+
+class Domain:
+    """Allocation helpers of Domain (domain/domain.ixx:121-165) on a torch device."""
+
+    def __init__(self, configuration: DomainConfiguration, device="cuda"):
+        self.configuration = configuration
+        self.device = torch.device(device)
+
+    def allocate_scalar_field(self, count: Optional[int] = None) -> torch.Tensor:
+        n = self.configuration.particle_count if count is None else count
+        return torch.zeros(n, dtype=torch.float32, device=self.device)
+
+    def allocate_vector_field(self, count: Optional[int] = None) -> VectorField:
+        n = self.configuration.particle_count if count is None else count
+        return VectorField.zeros(n, self.device)
+
+    def allocate_particle_state(self) -> ParticleState:
+        return ParticleState(self.allocate_vector_field(), self.allocate_vector_field(), 0)
+
+    def allocate_control(self) -> Control:
+        return Control(self.allocate_vector_field())
+
+    def allocate_particle_parameters(self) -> ParticleParameters:
+        return ParticleParameters(*(self.allocate_scalar_field() for _ in range(4)))
+
+    def copy(self, source: ParticleState, destination: ParticleState) -> None:
+        destination.positions.copy_(source.positions)
+        destination.velocities.copy_(source.velocities)
+        destination.step_index = source.step_index
+
+
+def _read_profile(function: str, handle) -> list:
+    entries = (_capi.PassProfile * 32)()
+    count = ctypes.c_uint32()
+    _capi.call(function, handle, entries, 32, ctypes.byref(count))
+    return [
+        {"name": e.name.decode(), "total_ms": e.total_ms, "launches": int(e.launches),
+         "bytes_per_particle": e.bytes_per_particle}
+        for e in entries[: count.value]
+    ]
+
+
+def _stream_handle(device: torch.device) -> int:
+    return torch.cuda.current_stream(device).cuda_stream
+
+
+def _require_gpu(device: torch.device) -> None:
+    if device.type != "cuda" or not torch.cuda.is_available():
+        raise RuntimeError("physica_amd solvers run on the GPU only (no CPU fallback); no HIP device is available")
+
+
+# -------------------------------------------------------------------- DFSPH
+class DFSPH:
+    """DFSPH method types, sph/sph.ixx:195-268."""
+
+    @dataclass
+    class Configuration:
+        divergence_iterations: int = 4
+        density_iterations: int = 6
+        checkpoint_interval: int = 2
+        pressure_warm_start: bool = True
+        density_error_tolerance: float = 0.0
+        max_iterations: int = 100
+
+        def to_c(self) -> _capi.DfsphConfiguration:
+            return _capi.DfsphConfiguration(
+                int(self.divergence_iterations),
+                int(self.density_iterations),
+                int(self.checkpoint_interval),
+                1 if self.pressure_warm_start else 0,
+                float(self.density_error_tolerance),
+                int(self.max_iterations),
+            )
+
+    @dataclass
+    class State:
+        warm_divergence_pressure: torch.Tensor
+        warm_density_pressure: torch.Tensor
+
+        def to_c(self) -> _capi.DfsphState:
+            return _capi.DfsphState(_ptr(self.warm_divergence_pressure), _ptr(self.warm_density_pressure))
+
+    @dataclass
+    class Parameters:
+        divergence_relaxation: torch.Tensor
+        density_relaxation: torch.Tensor
+
+        def to_c(self) -> _capi.DfsphParameters:
+            return _capi.DfsphParameters(_ptr(self.divergence_relaxation), _ptr(self.density_relaxation))
+
+
+class SPH:
+    """SPH<DFSPH> (sph/sph.ixx:273-372) over the HIP library."""
+
+    @dataclass
+    class State:
+        particles: ParticleState
+        method: DFSPH.State
+
+        def to_c(self) -> _capi.SphDfsphState:
+            return _capi.SphDfsphState(self.particles.to_c(), self.method.to_c())
+
+    @dataclass
+    class Parameters:
+        particles: ParticleParameters
+        method: DFSPH.Parameters
+
+        def to_c(self) -> _capi.SphDfsphParameters:
+            return _capi.SphDfsphParameters(self.particles.to_c(), self.method.to_c())
+
+    class StepCache:
+        """StepCache (sph/sph.ixx:299-303); fields are read back in original order."""
+
+        def __init__(self, solver: "SPH"):
+            self._solver = solver
+            handle = ctypes.c_void_p()
+            _capi.call("phx_dfsph_allocate_step_cache", solver._handle, ctypes.byref(handle))
+            self._handle = handle
+
+        def __del__(self):
+            if getattr(self, "_handle", None) and self._handle.value:
+                _capi.library().phx_dfsph_free_step_cache(self._handle)
+                self._handle = ctypes.c_void_p()
+
+        def neighborhood(self):
+            n = self._solver.domain.configuration.particle_count
+            cells = math.prod(self._solver.domain.configuration.grid_resolution())
+            dev = self._solver.domain.device
+            keys = torch.empty(n, dtype=torch.int64, device=dev)
+            idx = torch.empty(n, dtype=torch.int32, device=dev)
+            off = torch.empty(cells + 1, dtype=torch.int32, device=dev)
+            _capi.call("phx_dfsph_cache_neighborhood", self._handle, _ptr(keys), _ptr(idx), _ptr(off))
+            return keys, idx, off
+
+        def densities(self) -> torch.Tensor:
+            out = self._solver.domain.allocate_scalar_field()
+            _capi.call("phx_dfsph_cache_densities", self._handle, _ptr(out))
+            return out
+
+        def non_pressure_accelerations(self) -> VectorField:
+            out = self._solver.domain.allocate_vector_field()
+            _capi.call("phx_dfsph_cache_non_pressure_accelerations", self._handle, out.view())
+            return out
+
+    def __init__(
+        self,
+        domain_configuration: DomainConfiguration,
+        method_configuration: Optional[DFSPH.Configuration] = None,
+        mode: ExecutionMode = ExecutionMode.forward,
+        device="cuda",
+    ):
+        self.domain = Domain(domain_configuration, device)
+        self.method_configuration = method_configuration or DFSPH.Configuration()
+        _require_gpu(self.domain.device)
+        handle = ctypes.c_void_p()
+        self._domain_c = domain_configuration.to_c()
+        _capi.call(
+            "phx_dfsph_create",
+            ctypes.byref(self._domain_c),
+            ctypes.byref(self.method_configuration.to_c()),
+            int(mode),
+            _stream_handle(self.domain.device),
+            ctypes.byref(handle),
+        )
+        self._handle = handle
+
+    def __del__(self):
+        if getattr(self, "_handle", None) and self._handle.value:
+            _capi.library().phx_dfsph_destroy(self._handle)
+            self._handle = ctypes.c_void_p()
+
+    def allocate_state(self) -> "SPH.State":
+        d = self.domain
+        return SPH.State(d.allocate_particle_state(), DFSPH.State(d.allocate_scalar_field(), d.allocate_scalar_field()))
+
+    def allocate_control(self) -> Control:
+        return self.domain.allocate_control()
+
+    def allocate_parameters(self) -> "SPH.Parameters":
+        d = self.domain
+        return SPH.Parameters(
+            d.allocate_particle_parameters(), DFSPH.Parameters(d.allocate_scalar_field(), d.allocate_scalar_field())
+        )
+
+    def allocate_step_cache(self) -> "SPH.StepCache":
+        return SPH.StepCache(self)
+
+    def copy_state(self, source: "SPH.State", destination: "SPH.State") -> None:
+        self.domain.copy(source.particles, destination.particles)
+        destination.method.warm_divergence_pressure.copy_(source.method.warm_divergence_pressure)
+        destination.method.warm_density_pressure.copy_(source.method.warm_density_pressure)
+
+    def forward_step(self, state, control, parameters, next_state, cache) -> None:
+        s = state.to_c()
+        n = next_state.to_c()
+        _capi.call(
+            "phx_dfsph_forward_step",
+            self._handle,
+            ctypes.byref(s),
+            ctypes.byref(control.to_c()),
+            ctypes.byref(parameters.to_c()),
+            ctypes.byref(n),
+            cache._handle,
+        )
+        next_state.particles.step_index = int(n.particles.step_index)
+
+    @property
+    def reference_gradient_norm(self) -> float:
+        v = ctypes.c_float()
+        _capi.call("phx_dfsph_reference_gradient_norm", self._handle, ctypes.byref(v))
+        return float(v.value)
+
+    def set_profiling(self, enabled: bool) -> None:
+        """Bracket every pass with HIP events (per-pass timing)."""
+        _capi.call("phx_dfsph_set_profiling", self._handle, 1 if enabled else 0)
+
+    def read_profile(self) -> list:
+        """Per-pass totals since profiling was enabled or last read."""
+        return _read_profile("phx_dfsph_read_profile", self._handle)
+
+    def last_iterations(self) -> Tuple[int, int]:
+        v = (ctypes.c_uint32 * 2)()
+        _capi.call("phx_dfsph_last_iterations", self._handle, v)
+        return int(v[0]), int(v[1])
+
+
+# ---------------------------------------------------------------------- PBF
+class PBF:
+    """PBF (pbf/pbf.ixx:14-141) over the HIP library."""
+
+    @dataclass
+    class Configuration:
+        pressure_iterations: int = 5
+        checkpoint_interval: int = 2
+
+        def to_c(self) -> _capi.PbfConfiguration:
+            return _capi.PbfConfiguration(int(self.pressure_iterations), int(self.checkpoint_interval))
+
+    @dataclass
+    class State:
+        particles: ParticleState
+
+        def to_c(self) -> _capi.PbfState:
+            return _capi.PbfState(self.particles.to_c())
+
+    @dataclass
+    class Parameters:
+        particles: ParticleParameters
+        relaxation: torch.Tensor
+        artificial_pressure_strength: torch.Tensor
+        artificial_pressure_exponent: torch.Tensor
+        artificial_pressure_radius: torch.Tensor
+        xsph_viscosity: torch.Tensor
+        vorticity_confinement: torch.Tensor
+
+        def to_c(self) -> _capi.PbfParameters:
+            return _capi.PbfParameters(
+                self.particles.to_c(),
+                _ptr(self.relaxation),
+                _ptr(self.artificial_pressure_strength),
+                _ptr(self.artificial_pressure_exponent),
+                _ptr(self.artificial_pressure_radius),
+                _ptr(self.xsph_viscosity),
+                _ptr(self.vorticity_confinement),
+            )
+
+    class StepCache:
+        """StepCache (pbf/pbf.ixx:54-70); fields are read back in original order."""
+
+        def __init__(self, solver: "PBF"):
+            self._solver = solver
+            handle = ctypes.c_void_p()
+            _capi.call("phx_pbf_allocate_step_cache", solver._handle, ctypes.byref(handle))
+            self._handle = handle
+
+        def __del__(self):
+            if getattr(self, "_handle", None) and self._handle.value:
+                _capi.library().phx_pbf_free_step_cache(self._handle)
+                self._handle = ctypes.c_void_p()
+
+        def neighborhood(self):
+            n = self._solver.domain.configuration.particle_count
+            cells = math.prod(self._solver.domain.configuration.grid_resolution())
+            dev = self._solver.domain.device
+            keys = torch.empty(n, dtype=torch.int64, device=dev)
+            idx = torch.empty(n, dtype=torch.int32, device=dev)
+            off = torch.empty(cells + 1, dtype=torch.int32, device=dev)
+            _capi.call("phx_pbf_cache_neighborhood", self._handle, _ptr(keys), _ptr(idx), _ptr(off))
+            return keys, idx, off
+
+        def predicted_positions(self) -> VectorField:
+            out = self._solver.domain.allocate_vector_field()
+            _capi.call("phx_pbf_cache_predicted_positions", self._handle, out.view())
+            return out
+
+        def last_iteration(self):
+            rho = self._solver.domain.allocate_scalar_field()
+            lam = self._solver.domain.allocate_scalar_field()
+            _capi.call("phx_pbf_cache_last_iteration", self._handle, _ptr(rho), _ptr(lam))
+            return rho, lam
+
+    def __init__(
+        self,
+        domain_configuration: DomainConfiguration,
+        configuration: Optional["PBF.Configuration"] = None,
+        mode: ExecutionMode = ExecutionMode.forward,
+        device="cuda",
+    ):
+        self.domain = Domain(domain_configuration, device)
+        self.configuration = configuration or PBF.Configuration()
+        _require_gpu(self.domain.device)
+        handle = ctypes.c_void_p()
+        self._domain_c = domain_configuration.to_c()
+        _capi.call(
+            "phx_pbf_create",
+            ctypes.byref(self._domain_c),
+            ctypes.byref(self.configuration.to_c()),
+            int(mode),
+            _stream_handle(self.domain.device),
+            ctypes.byref(handle),
+        )
+        self._handle = handle
+
+    def __del__(self):
+        if getattr(self, "_handle", None) and self._handle.value:
+            _capi.library().phx_pbf_destroy(self._handle)
+            self._handle = ctypes.c_void_p()
+
+    def allocate_state(self) -> "PBF.State":
+        return PBF.State(self.domain.allocate_particle_state())
+
+    def allocate_control(self) -> Control:
+        return self.domain.allocate_control()
+
+    def allocate_parameters(self) -> "PBF.Parameters":
+        d = self.domain
+        return PBF.Parameters(d.allocate_particle_parameters(), *(d.allocate_scalar_field() for _ in range(6)))
+
+    def allocate_step_cache(self) -> "PBF.StepCache":
+        return PBF.StepCache(self)
+
+    def copy_state(self, source: "PBF.State", destination: "PBF.State") -> None:
+        self.domain.copy(source.particles, destination.particles)
+
+    def set_profiling(self, enabled: bool) -> None:
+        _capi.call("phx_pbf_set_profiling", self._handle, 1 if enabled else 0)
+
+    def read_profile(self) -> list:
+        return _read_profile("phx_pbf_read_profile", self._handle)
+
+    def forward_step(self, state, control, parameters, next_state, cache) -> None:
+        s = state.to_c()
+        n = next_state.to_c()
+        _capi.call(
+            "phx_pbf_forward_step",
+            self._handle,
+            ctypes.byref(s),
+            ctypes.byref(control.to_c()),
+            ctypes.byref(parameters.to_c()),
+            ctypes.byref(n),
+            cache._handle,
+        )
+        next_state.particles.step_index = int(n.particles.step_index)
+
+
+# ------------------------------------------------------------ neighborhood
+def neighborhood_workspace_size(configuration: DomainConfiguration) -> int:
+    size = ctypes.c_size_t()
+    _capi.call("phx_neighborhood_workspace_size", ctypes.byref(configuration.to_c()), ctypes.byref(size))
+    return int(size.value)
+
+
+def build_neighborhood(configuration: DomainConfiguration, step_index: int, positions: VectorField, device="cuda"):
+    """NeighborhoodSearch::build in the reference layout.
+
+    Returns (sorted_keys int64[N], sorted_particle_indices int32[N],
+    cell_offsets int32[C+1], NeighborhoodView)."""
+    dev = torch.device(device)
+    _require_gpu(dev)
+    n = configuration.particle_count
+    cells = math.prod(configuration.grid_resolution())
+    workspace = torch.empty(neighborhood_workspace_size(configuration), dtype=torch.uint8, device=dev)
+    keys = torch.empty(n, dtype=torch.int64, device=dev)
+    idx = torch.empty(n, dtype=torch.int32, device=dev)
+    off = torch.empty(cells + 1, dtype=torch.int32, device=dev)
+    boff = torch.empty(cells + 1, dtype=torch.int32, device=dev)
+    view = _capi.NeighborhoodView()
+    _capi.call(
+        "phx_build_neighborhood",
+        _stream_handle(dev),
+        ctypes.byref(configuration.to_c()),
+        int(step_index),
+        positions.view(),
+        _ptr(workspace),
+        workspace.numel(),
+        _ptr(keys),
+        _ptr(idx),
+        _ptr(off),
+        _ptr(boff),
+        ctypes.byref(view),
+    )
+    view._keep = (workspace, keys, idx, off, boff)
+    return keys, idx, off, view
+
+
+def density_forward(
+    support_radius: float,
+    pbf_kernel: bool,
+    topology: VectorField,
+    positions: VectorField,
+    parameters: ParticleParameters,
+    view: _capi.NeighborhoodView,
+    densities: torch.Tensor,
+) -> None:
+    """density::sph_forward / pbf_forward (density/density.cpp:12-34)."""
+    dev = densities.device
+    _capi.call(
+        "phx_density_forward",
+        _stream_handle(dev),
+        int(view.particle_count),
+        float(support_radius),
+        1 if pbf_kernel else 0,
+        topology.view(),
+        positions.view(),
+        parameters.to_c(),
+        view,
+        _ptr(densities),
+    )

@@ -1,0 +1,79 @@
+"""Shared set-up for GPU tests: upload a scene into solver buffers and read
+results back as numpy."""
+import numpy as np
+import torch
+
+from physica_amd.particle import PBF, SPH, DFSPH
+
+
+def bits_equal(a, b) -> bool:
+    """Bitwise equality of two float arrays (NaN patterns included)."""
+    a = np.ascontiguousarray(a)
+    b = np.ascontiguousarray(b)
+    if a.shape != b.shape or a.dtype != b.dtype:
+        return False
+    view = {4: np.uint32, 8: np.uint64}[a.dtype.itemsize]
+    return bool(np.array_equal(a.view(view), b.view(view)))
+
+
+def max_rel(a, b, floor=1e-30):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    scale = np.maximum(np.abs(b).max(), floor)
+    return float(np.abs(a - b).max() / scale) if a.size else 0.0
+
+
+def upload_scalar(t: torch.Tensor, values) -> None:
+    t.copy_(torch.from_numpy(np.ascontiguousarray(values, dtype=np.float32)))
+
+
+def dfsph_setup(scene, device, configuration=None, warm=None):
+    solver = SPH(scene.domain, configuration or DFSPH.Configuration(
+        divergence_iterations=scene.solver.get("divergence_iterations", 4),
+        density_iterations=scene.solver.get("density_iterations", 6)), device=device)
+    state = solver.allocate_state()
+    nxt = solver.allocate_state()
+    control = solver.allocate_control()
+    params = solver.allocate_parameters()
+    cache = solver.allocate_step_cache()
+    state.particles.positions.upload(scene.positions)
+    state.particles.velocities.upload(scene.velocities)
+    p = params.particles
+    for name in ("masses", "rest_densities", "viscosities", "surface_tensions"):
+        upload_scalar(getattr(p, name), scene.parameters[name])
+    upload_scalar(params.method.divergence_relaxation, scene.parameters["divergence_relaxation"])
+    upload_scalar(params.method.density_relaxation, scene.parameters["density_relaxation"])
+    if warm is not None:
+        upload_scalar(state.method.warm_divergence_pressure, warm[0])
+        upload_scalar(state.method.warm_density_pressure, warm[1])
+    return solver, state, nxt, control, params, cache
+
+
+def dfsph_state_numpy(state):
+    return {
+        "positions": state.particles.positions.numpy(),
+        "velocities": state.particles.velocities.numpy(),
+        "warm_divergence_pressure": state.method.warm_divergence_pressure.cpu().numpy(),
+        "warm_density_pressure": state.method.warm_density_pressure.cpu().numpy(),
+    }
+
+
+PBF_NAMES = ("relaxation", "artificial_pressure_strength", "artificial_pressure_exponent",
+             "artificial_pressure_radius", "xsph_viscosity", "vorticity_confinement")
+
+
+def pbf_setup(scene, device, configuration=None):
+    solver = PBF(scene.domain, configuration or PBF.Configuration(
+        pressure_iterations=scene.solver.get("pressure_iterations", 4)), device=device)
+    state = solver.allocate_state()
+    nxt = solver.allocate_state()
+    control = solver.allocate_control()
+    params = solver.allocate_parameters()
+    cache = solver.allocate_step_cache()
+    state.particles.positions.upload(scene.positions)
+    state.particles.velocities.upload(scene.velocities)
+    for name in ("masses", "rest_densities", "viscosities", "surface_tensions"):
+        upload_scalar(getattr(params.particles, name), scene.parameters[name])
+    for name in PBF_NAMES:
+        upload_scalar(getattr(params, name), scene.parameters[name])
+    return solver, state, nxt, control, params, cache
