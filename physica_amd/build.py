@@ -69,6 +69,26 @@ def build_library(verbose: bool = False) -> Path:
     return LIBRARY
 
 
+CPP_TESTS = ROOT / "tests" / "cpp"
+
+
+def build_cpp_tests(verbose: bool = False) -> list:
+    """Compile the C++ API programs under tests/cpp against the library."""
+    outdir = CPP_TESTS / "_bin"
+    outdir.mkdir(parents=True, exist_ok=True)
+    built = []
+    for src in sorted(CPP_TESTS.glob("*.cpp")):
+        exe = outdir / src.stem
+        cmd = [hipcc(), "-std=c++20", "-O2", "--offload-arch=gfx950", "-I", str(ROOT / "include"), str(src),
+               "-L", str(PACKAGE), "-lphysica_amd", "-Wl,-rpath," + str(PACKAGE), "-Wl,-rpath,$ORIGIN/../../../physica_amd",
+               "-o", str(exe)]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True)
+        built.append(exe)
+    return built
+
+
 def build_oracle(verbose: bool = False) -> Path:
     """Compile the C oracle (test infrastructure) with its Makefile."""
     cmd = ["make", "-C", str(ROOT / "oracle"), "-j4"]
@@ -80,6 +100,7 @@ def build_oracle(verbose: bool = False) -> Path:
 
 def main() -> int:
     build_library(verbose=True)
+    build_cpp_tests(verbose=True)
     build_oracle(verbose=True)
     return 0
 

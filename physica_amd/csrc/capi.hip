@@ -279,7 +279,10 @@ phx_status phx_dfsph_cache_non_pressure_accelerations(const phx_dfsph_step_cache
         phx::require(cache != nullptr && cache->valid, PHX_ERROR_INVALID_ARGUMENT, "cache holds no step");
         phx::require_vec(out, "output");
         phx::ChannelList ch;
-        ch.add(cache->impl.non_pressure.cview(), phx::vec(out));
+        const float* np = reinterpret_cast<const float*>(cache->impl.non_pressure.data());
+        ch.add(np + 0, out.x, 4u);
+        ch.add(np + 1, out.y, 4u);
+        ch.add(np + 2, out.z, 4u);
         phx::scatter_to_original(cache->impl.stream, cache->impl.neighborhood.perm.data(),
                                  cache->impl.neighborhood.particles, ch.get());
     });

@@ -4,49 +4,75 @@
 // DFSPH::forward (sph/sph.cpp:572-610) with kernels from sph/common.cu,
 // sph/dfsph.cu and density/density.cu. The reference runs one kernel per
 // operation in original particle order; here particles are reordered by cell
-// once per step and each per-particle chain is fused into the neighbour pass
-// that produces its input:
+// once per step, each per-particle chain is fused into the neighbour pass
+// that produces its input, and every neighbour pass walks a precomputed mask
+// instead of all 27 cells:
 //
-//   density        rho = sum m W(x_i - x_j);  t = p / rho^2 for both warm p
+//   masks+density  27-cell test once per step -> exact (r < h) and skin masks;
+//                  rho = sum m W over the exact mask; t = p / rho^2 (warm p)
 //   force (init)   non-pressure + pressure(t_div) + predict -> x*
-//   per iteration  A: rho* at x*, projection update -> p, t
-//                  B: pressure(t) + predict -> x*  (or the next state)
+//   per iteration  A: rho* at x* over the skin mask, projection update -> p, t
+//                  B: pressure(t) over the exact mask + predict -> x* (or the
+//                     next state)
 //
 // The end of the divergence phase and the density phase's initial pressure
-// are one pass with two accumulators. Every per-particle sum is evaluated in
-// the reference order (27 cells, offset_x outer, ascending original index), so
-// results equal the reference expression by expression.
+// are one pass with two accumulators. Every per-particle sum runs in the
+// reference order (27 cells, offset_x outer, ascending original index) and a
+// masked-out candidate contributes an exact zero in the reference, so results
+// equal the reference expression by expression. An A pass uses the skin mask
+// only if no particle moved farther than half the skin (read on the device
+// from the preceding B pass); otherwise it scans the full stencil.
+#include <cmath>
+#include <cstring>
+
 #include "dfsph.hpp"
+#include "neighbor_mask.hpp"
 
 namespace phx {
 namespace {
 
 constexpr uint32_t kBlock = 256;
 
-struct DensityArgs {
+__device__ __forceinline__ Float3 xyz(float4 q) { return Float3{q.x, q.y, q.z}; }
+
+struct MaskDensityArgs {
     uint32_t n;
     float h;
+    float t_exact;
+    float t_skin;
     SortedGrid grid;
-    CVec x;
-    const float* m;
+    const float4* xm;
+    uint16_t* exact;
+    uint16_t* skin;
     const float* p_div;
     const float* p_den;
     float* rho;
     float* t_div;
     float* t_den;
+    uint32_t* flags;
 };
 
-// density_forward_kernel (density.cu:18-40, cubic kernel, topology = positions)
-// followed by the first_term p/rho^2 of pressure_forward_kernel (common.cu:171)
-// for the warm-started pressures of both phases.
-__global__ void __launch_bounds__(kBlock) dfsph_density_kernel(DensityArgs a) {
+// Mask build, then density_forward_kernel (density.cu:18-40, cubic kernel,
+// topology = positions) over the exact mask and the first_term p/rho^2 of
+// pressure_forward_kernel (common.cu:171) for both warm-started pressures.
+__global__ void __launch_bounds__(kBlock) dfsph_mask_density_kernel(MaskDensityArgs a) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.n) return;
-    const Float3 p = load(a.x, i);
+    const Float3 p = xyz(a.xm[i]);
     int cx, cy, cz;
     particle_cell(a.grid, p, cx, cy, cz);
+    const bool complete = build_masks(a.grid, a.xm, a.n, i, p, cx, cy, cz, a.t_exact, a.t_skin, a.exact, a.skin);
+    if (!complete) atomicOr(&a.flags[kFlagOverflow], 1u);
     float result = 0.0F;
-    for_each_candidate(a.grid, cx, cy, cz, [&](uint32_t s) { result += a.m[s] * cubic(sub(p, load(a.x, s)), a.h); });
+    auto visit = [&](uint32_t s) {
+        const float4 q = a.xm[s];
+        result += q.w * cubic(sub(p, xyz(q)), a.h);
+    };
+    if (complete) {
+        for_each_masked(a.grid, a.exact, a.n, i, cx, cy, cz, visit);
+    } else {
+        for_each_candidate(a.grid, cx, cy, cz, visit);
+    }
     a.rho[i] = result;
     a.t_div[i] = a.p_div[i] / (result * result);
     a.t_den[i] = a.p_den[i] / (result * result);
@@ -57,16 +83,18 @@ struct ProjectionArgs {
     float h;
     float dt;
     float gradient_norm;
+    float limit2;  // largest squared displacement for which the skin mask is exact
+    uint32_t slot;
     SortedGrid grid;
-    CVec x;   // topology: state positions
-    CVec xs;  // predicted positions
-    const float* m;
+    const float4* xm;  // topology: state positions (+ masses)
+    const float4* xs;  // predicted positions (+ masses)
+    const uint16_t* skin;
+    const uint32_t* flags;
     const float* rho;
     const float* target;
     const float* relaxation;
     float* p;
     float* t;
-    float* rho_star;
 };
 
 // density at predicted positions (density.cu:18-40 with topology = x,
@@ -75,14 +103,24 @@ struct ProjectionArgs {
 __global__ void __launch_bounds__(kBlock) dfsph_projection_kernel(ProjectionArgs a) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.n) return;
-    const Float3 ps = load(a.xs, i);
+    const bool masked = a.flags[kFlagOverflow] == 0u &&
+                        __uint_as_float(a.flags[kFlagDisplacement + a.slot]) <= a.limit2;
+    const float4 self = a.xs[i];
+    const Float3 ps = xyz(self);
     int cx, cy, cz;
-    particle_cell(a.grid, load(a.x, i), cx, cy, cz);
+    particle_cell(a.grid, xyz(a.xm[i]), cx, cy, cz);
     float rs = 0.0F;
-    for_each_candidate(a.grid, cx, cy, cz, [&](uint32_t s) { rs += a.m[s] * cubic(sub(ps, load(a.xs, s)), a.h); });
-    if (a.rho_star != nullptr) a.rho_star[i] = rs;
+    auto visit = [&](uint32_t s) {
+        const float4 q = a.xs[s];
+        rs += q.w * cubic(sub(ps, xyz(q)), a.h);
+    };
+    if (masked) {
+        for_each_masked(a.grid, a.skin, a.n, i, cx, cy, cz, visit);
+    } else {
+        for_each_candidate(a.grid, cx, cy, cz, visit);
+    }
     const float rho = a.rho[i];
-    const float volume = a.m[i] / rho;
+    const float volume = self.w / rho;
     const float delta = 1.0F / (2.0F * a.dt * a.dt * volume * volume * a.gradient_norm);
     const float pressure = fmaxf(0.0F, a.p[i] + a.relaxation[i] * delta * (rs - a.target[i]));
     a.p[i] = pressure;
@@ -102,15 +140,18 @@ struct ForceArgs {
     SortedGrid grid;
     Box box;
     Float3 gravity;
-    CVec x, v, u;
-    const float* m;
-    const float* mu;
+    const float4* xm;
+    const float4* vmu;  // velocity (+ viscosity)
+    CVec u;
     const float* rho;
     const float* t1;
     const float* t2;
-    Vec anp;
-    Vec adiv;
-    Vec xs;
+    const uint16_t* exact;
+    uint32_t* flags;
+    uint32_t slot;
+    float4* anp;
+    float4* adiv;
+    float4* xs;
     uint32_t final_step;
     const uint32_t* perm;
     Vec next_x, next_v;
@@ -127,62 +168,74 @@ struct ForceArgs {
 template <bool kNonPressure, int kEpilogue>
 __global__ void __launch_bounds__(kBlock) dfsph_force_kernel(ForceArgs a) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= a.n) return;
-    const Float3 p = load(a.x, i);
-    const Float3 vel = load(a.v, i);
-    Float3 acc{0.0F, 0.0F, 0.0F};
-    float mu_i = 0.0F;
-    if constexpr (kNonPressure) {
-        acc = add(a.gravity, load(a.u, i));
-        mu_i = a.mu[i];
-    }
-    const float t1_i = a.t1[i];
-    float t2_i = 0.0F;
-    if constexpr (kEpilogue == kDivEnd) t2_i = a.t2[i];
-    Float3 ap1{0.0F, 0.0F, 0.0F};
-    Float3 ap2{0.0F, 0.0F, 0.0F};
-    int cx, cy, cz;
-    particle_cell(a.grid, p, cx, cy, cz);
-    for_each_candidate(a.grid, cx, cy, cz, [&](uint32_t s) {
-        if (s == i) return;
-        const Float3 d = sub(p, load(a.x, s));
-        const float ms = a.m[s];
-        if constexpr (kNonPressure) {
-            const float viscosity = 0.5F * (mu_i + a.mu[s]);
-            const float weight = viscosity * ms * viscosity_laplacian(d, a.h) / a.rho[s];
-            acc = add(acc, scale(sub(load(a.v, s), vel), weight));
+    const bool active = i < a.n;
+    float moved2 = 0.0F;
+    if (active) {
+        const bool masked = a.flags[kFlagOverflow] == 0u;
+        const float4 self = a.xm[i];
+        const Float3 p = xyz(self);
+        const float4 vm = a.vmu[i];
+        const Float3 vel = xyz(vm);
+        Float3 acc{0.0F, 0.0F, 0.0F};
+        if constexpr (kNonPressure) acc = add(a.gravity, load(a.u, i));
+        const float t1_i = a.t1[i];
+        float t2_i = 0.0F;
+        if constexpr (kEpilogue == kDivEnd) t2_i = a.t2[i];
+        Float3 ap1{0.0F, 0.0F, 0.0F};
+        Float3 ap2{0.0F, 0.0F, 0.0F};
+        int cx, cy, cz;
+        particle_cell(a.grid, p, cx, cy, cz);
+        auto visit = [&](uint32_t s) {
+            if (s == i) return;
+            const float4 q = a.xm[s];
+            const Float3 d = sub(p, xyz(q));
+            const float ms = q.w;
+            if constexpr (kNonPressure) {
+                const float4 vq = a.vmu[s];
+                const float viscosity = 0.5F * (vm.w + vq.w);
+                const float weight = viscosity * ms * viscosity_laplacian(d, a.h) / a.rho[s];
+                acc = add(acc, scale(sub(xyz(vq), vel), weight));
+            }
+            const Float3 gradient = cubic_gradient(d, a.h);
+            ap1 = add(ap1, scale(gradient, -ms * (t1_i + a.t1[s])));
+            if constexpr (kEpilogue == kDivEnd) ap2 = add(ap2, scale(gradient, -ms * (t2_i + a.t2[s])));
+        };
+        if (masked) {
+            for_each_masked(a.grid, a.exact, a.n, i, cx, cy, cz, visit);
+        } else {
+            for_each_candidate(a.grid, cx, cy, cz, visit);
         }
-        const Float3 gradient = cubic_gradient(d, a.h);
-        ap1 = add(ap1, scale(gradient, -ms * (t1_i + a.t1[s])));
-        if constexpr (kEpilogue == kDivEnd) ap2 = add(ap2, scale(gradient, -ms * (t2_i + a.t2[s])));
-    });
-    if constexpr (kNonPressure) {
-        store(a.anp, i, acc);
-    } else {
-        acc = load(a.anp, i);
+        if constexpr (kNonPressure) {
+            a.anp[i] = make_float4(acc.x, acc.y, acc.z, 0.0F);
+        } else {
+            acc = xyz(a.anp[i]);
+        }
+        Float3 total;
+        if constexpr (kEpilogue == kDivPredict) {
+            total = ap1;
+        } else if constexpr (kEpilogue == kDivEnd) {
+            a.adiv[i] = make_float4(ap1.x, ap1.y, ap1.z, 0.0F);
+            total = add(ap1, ap2);
+        } else {
+            total = add(xyz(a.adiv[i]), ap1);
+        }
+        Float3 nx, nv;
+        predict_clamped(a.box, a.dt, p, vel, add(acc, total), nx, nv);
+        if constexpr (kEpilogue == kDivEnd) {
+            if (a.warm_div_out != nullptr) a.warm_div_out[a.perm[i]] = a.p_div[i];
+        }
+        if (a.final_step != 0u) {
+            const uint32_t j = a.perm[i];
+            store(a.next_x, j, nx);
+            store(a.next_v, j, nv);
+            if (a.warm_den_out != nullptr) a.warm_den_out[j] = a.p_den[i];
+        } else {
+            a.xs[i] = make_float4(nx.x, nx.y, nx.z, self.w);
+            const Float3 moved = sub(nx, p);
+            moved2 = dot(moved, moved);
+        }
     }
-    Float3 total;
-    if constexpr (kEpilogue == kDivPredict) {
-        total = ap1;
-    } else if constexpr (kEpilogue == kDivEnd) {
-        store(a.adiv, i, ap1);
-        total = add(ap1, ap2);
-    } else {
-        total = add(load(a.adiv, i), ap1);
-    }
-    Float3 nx, nv;
-    predict_clamped(a.box, a.dt, p, vel, add(acc, total), nx, nv);
-    if constexpr (kEpilogue == kDivEnd) {
-        if (a.warm_div_out != nullptr) a.warm_div_out[a.perm[i]] = a.p_div[i];
-    }
-    if (a.final_step != 0u) {
-        const uint32_t j = a.perm[i];
-        store(a.next_x, j, nx);
-        store(a.next_v, j, nv);
-        if (a.warm_den_out != nullptr) a.warm_den_out[j] = a.p_den[i];
-    } else {
-        store(a.xs, i, nx);
-    }
+    if (a.final_step == 0u) block_max_to<kBlock>(moved2, &a.flags[kFlagDisplacement + a.slot]);
 }
 
 template <bool kNonPressure, int kEpilogue>
@@ -204,8 +257,19 @@ void launch_projection(PassProfiler& profiler, hipStream_t stream, const Project
 
 }  // namespace
 
+float sqrt_support_threshold(float h) {
+    float t = h * h;
+    while (std::sqrt(t) < h) t = std::nextafter(t, INFINITY);
+    for (;;) {
+        const float below = std::nextafter(t, 0.0F);
+        if (below <= 0.0F || std::sqrt(below) < h) break;
+        t = below;
+    }
+    return t;
+}
+
 const char* dfsph_pass_name(int pass) {
-    static const char* names[kDfsphPassCount] = {"grid",       "density",          "force_init",
+    static const char* names[kDfsphPassCount] = {"grid",       "mask_density",     "force_init",
                                                  "projection_divergence", "force_divergence",
                                                  "force_divergence_end",  "projection_density",
                                                  "force_density"};
@@ -251,22 +315,44 @@ DfsphSolver::DfsphSolver(const phx_domain_configuration* domain, const phx_dfsph
     require(config_.checkpoint_interval > 0u, PHX_ERROR_INVALID_ARGUMENT, "checkpoint_interval must be positive");
     require(mode == PHX_MODE_FORWARD, PHX_ERROR_UNSUPPORTED, "differentiable mode is not supported yet");
     require(config_.density_error_tolerance == 0.0F, PHX_ERROR_UNSUPPORTED, "tolerance mode is not supported yet");
+    require(config_.divergence_iterations + config_.density_iterations < 4096u, PHX_ERROR_INVALID_ARGUMENT,
+            "too many pressure iterations");
     gradient_norm_ = reference_gradient_norm(domain_.config);
+    exact_threshold_ = sqrt_support_threshold(domain_.config.support_radius);
     const uint32_t n = domain_.config.particle_count;
-    x_.resize(n);
-    v_.resize(n);
+    for (DeviceBuffer<float4>* b : {&xm_, &vmu_, &xs_, &adiv_}) b->resize(n);
     u_.resize(n);
-    xs_.resize(n);
-    adiv_.resize(n);
-    m_.resize(n);
-    rho0_.resize(n);
-    mu_.resize(n);
-    relax_div_.resize(n);
-    relax_den_.resize(n);
-    p_div_.resize(n);
-    p_den_.resize(n);
-    t_div_.resize(n);
-    t_den_.resize(n);
+    for (DeviceBuffer<float>* b : {&rho0_, &relax_div_, &relax_den_, &p_div_, &p_den_, &t_div_, &t_den_}) b->resize(n);
+    mask_exact_.resize(static_cast<size_t>(kStencil) * n);
+    mask_skin_.resize(static_cast<size_t>(kStencil) * n);
+    flag_count_ = kFlagDisplacement + config_.divergence_iterations + config_.density_iterations + 1u;
+    flags_.resize(flag_count_);
+    flags_host_.resize(sizeof(uint32_t) * flag_count_);
+    PHX_HIP(hipEventCreateWithFlags(&readback_, hipEventDisableTiming));
+}
+
+DfsphSolver::~DfsphSolver() {
+    if (readback_ != nullptr) {
+        (void)hipEventSynchronize(readback_);
+        (void)hipEventDestroy(readback_);
+    }
+}
+
+// Picks the skin of the next step from the largest predicted displacement of
+// a previous step (read back without blocking): twice the displacement (both
+// particles of a pair move) with a 30 % margin.
+void DfsphSolver::adapt_skin() {
+    if (!readback_pending_ || hipEventQuery(readback_) != hipSuccess) return;
+    readback_pending_ = false;
+    const auto* flags = static_cast<const uint32_t*>(flags_host_.data());
+    float moved2 = 0.0F;
+    for (uint32_t k = kFlagDisplacement; k < flag_count_; ++k) {
+        float v;
+        std::memcpy(&v, &flags[k], sizeof(float));
+        moved2 = std::fmax(moved2, v);
+    }
+    const float moved = std::sqrt(moved2) / domain_.config.support_radius;
+    skin_ = std::fmin(0.6F, std::fmax(0.02F, 2.6F * moved + 0.01F));
 }
 
 void DfsphSolver::forward_step(const phx_sph_dfsph_state& state, const phx_control& control,
@@ -298,15 +384,16 @@ void DfsphSolver::forward_step(const phx_sph_dfsph_state& state, const phx_contr
     require(next.method.warm_divergence_pressure && next.method.warm_density_pressure, PHX_ERROR_INVALID_ARGUMENT,
             "null next warm-start pressures");
     cache.allocate(n, domain_.cells);
+    adapt_skin();
 
     // neighborhood.build(step, x) with every input gathered into cell order
     ChannelList gather;
-    gather.add(cvec(state.particles.positions), x_.view());
-    gather.add(cvec(state.particles.velocities), v_.view());
+    gather.add(cvec(state.particles.positions), xm_.data());
+    gather.add(pp.masses, xm_.data(), 3);
+    gather.add(cvec(state.particles.velocities), vmu_.data());
+    gather.add(pp.viscosities, vmu_.data(), 3);
     gather.add(cvec(control.external_accelerations), u_.view());
-    gather.add(pp.masses, m_.data());
     gather.add(pp.rest_densities, rho0_.data());
-    gather.add(pp.viscosities, mu_.data());
     gather.add(parameters.method.divergence_relaxation, relax_div_.data());
     gather.add(parameters.method.density_relaxation, relax_den_.data());
     if (warm) {
@@ -320,16 +407,24 @@ void DfsphSolver::forward_step(const phx_sph_dfsph_state& state, const phx_contr
             fill_f32(stream_, p_div_.data(), n, 0.0F);
             fill_f32(stream_, p_den_.data(), n, 0.0F);
         }
+        fill_u32(stream_, flags_.data(), flag_count_, 0u);
     }
     const SortedGrid grid = cache.neighborhood.grid();
     const uint32_t blocks = blocks_for(n, kBlock);
     const float h = c.support_radius;
+    // skin radius R = h (1 + skin): the mask keeps d2 < R^2 (with margin), and
+    // an A pass may use it while every particle moved at most (R - h) / 2
+    const float radius = h * (1.0F + skin_);
+    const float t_skin = radius * radius * (1.0F + 1e-5F);
+    const float limit = 0.5F * (radius - h * (1.0F + 2e-5F));
+    const float limit2 = limit > 0.0F ? limit * limit : -1.0F;
 
-    DensityArgs da{n, h, grid, x_.cview(), m_.data(), p_div_.data(), p_den_.data(), cache.densities.data(),
-                   t_div_.data(), t_den_.data()};
+    MaskDensityArgs ma{n, h, exact_threshold_, t_skin, grid, xm_.data(), mask_exact_.data(), mask_skin_.data(),
+                       p_div_.data(), p_den_.data(), cache.densities.data(), t_div_.data(), t_den_.data(),
+                       flags_.data()};
     {
         PassScope scope(profiler_, stream_, kDfsphDensity);
-        dfsph_density_kernel<<<blocks, kBlock, 0, stream_>>>(da);
+        dfsph_mask_density_kernel<<<blocks, kBlock, 0, stream_>>>(ma);
         PHX_LAUNCH_CHECK();
     }
 
@@ -340,15 +435,15 @@ void DfsphSolver::forward_step(const phx_sph_dfsph_state& state, const phx_contr
     fa.grid = grid;
     fa.box = collision_box(c, step);
     fa.gravity = Float3{c.gravity.x, c.gravity.y, c.gravity.z};
-    fa.x = x_.cview();
-    fa.v = v_.cview();
+    fa.xm = xm_.data();
+    fa.vmu = vmu_.data();
     fa.u = u_.cview();
-    fa.m = m_.data();
-    fa.mu = mu_.data();
     fa.rho = cache.densities.data();
-    fa.anp = cache.non_pressure.view();
-    fa.adiv = adiv_.view();
-    fa.xs = xs_.view();
+    fa.exact = mask_exact_.data();
+    fa.flags = flags_.data();
+    fa.anp = cache.non_pressure.data();
+    fa.adiv = adiv_.data();
+    fa.xs = xs_.data();
     fa.perm = cache.neighborhood.perm.data();
     fa.next_x = vec(next.particles.positions);
     fa.next_v = vec(next.particles.velocities);
@@ -362,19 +457,22 @@ void DfsphSolver::forward_step(const phx_sph_dfsph_state& state, const phx_contr
     pa.h = h;
     pa.dt = c.time_step;
     pa.gradient_norm = gradient_norm_;
+    pa.limit2 = limit2;
     pa.grid = grid;
-    pa.x = x_.cview();
-    pa.xs = xs_.cview();
-    pa.m = m_.data();
+    pa.xm = xm_.data();
+    pa.xs = xs_.data();
+    pa.skin = mask_skin_.data();
+    pa.flags = flags_.data();
     pa.rho = cache.densities.data();
-    pa.rho_star = nullptr;
 
     const uint32_t nd = config_.divergence_iterations;
     const uint32_t ne = config_.density_iterations;
+    uint32_t slot = 0;  // displacement slot written by the latest predicting pass
 
     // divergence phase: initial pressure with the warm p_div (sph.cpp:574-578)
     fa.t1 = t_div_.data();
     fa.t2 = t_den_.data();
+    fa.slot = slot;
     if (nd == 0u) {
         fa.final_step = ne == 0u ? 1u : 0u;
         launch_force<true, kDivEnd>(profiler_, stream_, fa);
@@ -386,7 +484,9 @@ void DfsphSolver::forward_step(const phx_sph_dfsph_state& state, const phx_contr
         pa.p = p_div_.data();
         pa.t = t_div_.data();
         for (uint32_t it = 1; it <= nd; ++it) {  // sph.cpp:580-587
+            pa.slot = slot;
             launch_projection(profiler_, stream_, pa, kDfsphProjectionDivergence);
+            fa.slot = ++slot;
             if (it < nd) {
                 launch_force<false, kDivPredict>(profiler_, stream_, fa);
             } else {
@@ -404,10 +504,17 @@ void DfsphSolver::forward_step(const phx_sph_dfsph_state& state, const phx_contr
     fa.t2 = nullptr;
     fa.warm_div_out = nullptr;
     for (uint32_t it = 1; it <= ne; ++it) {
+        pa.slot = slot;
         launch_projection(profiler_, stream_, pa, kDfsphProjectionDensity);
+        fa.slot = ++slot;
         fa.final_step = it == ne ? 1u : 0u;
         launch_force<false, kDenPredict>(profiler_, stream_, fa);
     }
+    // read the step's displacements back for the next step's skin
+    PHX_HIP(hipMemcpyAsync(flags_host_.data(), flags_.data(), sizeof(uint32_t) * flag_count_, hipMemcpyDeviceToHost,
+                           stream_));
+    PHX_HIP(hipEventRecord(readback_, stream_));
+    readback_pending_ = true;
     next.particles.step_index = step + 1u;
     last_iterations_[0] = nd;
     last_iterations_[1] = ne;

@@ -12,7 +12,7 @@ struct DfsphCache {
     hipStream_t stream = nullptr;
     Neighborhood neighborhood;
     DeviceBuffer<float> densities;
-    VecBuffer non_pressure;
+    DeviceBuffer<float4> non_pressure;  // xyz used
     void allocate(uint32_t n, uint32_t cells);
 };
 
@@ -32,12 +32,40 @@ const char* dfsph_pass_name(int pass);
 // Algorithmic bytes per particle per launch: the pass's unique per-particle
 // inputs and outputs moved once, plus 4 (C + 1) / N for the cell offsets it
 // reads (the grid pass writes them and clears the histogram: 8 (C + 1) / N).
+// The neighbour masks are an acceleration structure and are not counted.
 double dfsph_pass_bytes(int pass, uint32_t particles, uint32_t cells, bool warm_start);
+
+// Pinned host buffer for asynchronous read-back.
+class PinnedBuffer {
+  public:
+    PinnedBuffer() = default;
+    PinnedBuffer(const PinnedBuffer&) = delete;
+    PinnedBuffer& operator=(const PinnedBuffer&) = delete;
+    ~PinnedBuffer() {
+        if (ptr_ != nullptr) (void)hipHostFree(ptr_);
+    }
+    void resize(size_t bytes) {
+        if (bytes <= bytes_) return;
+        if (ptr_ != nullptr) (void)hipHostFree(ptr_);
+        ptr_ = nullptr;
+        PHX_HIP(hipHostMalloc(&ptr_, bytes, hipHostMallocDefault));
+        bytes_ = bytes;
+    }
+    void* data() { return ptr_; }
+
+  private:
+    void* ptr_ = nullptr;
+    size_t bytes_ = 0;
+};
 
 class DfsphSolver {
   public:
     DfsphSolver(const phx_domain_configuration* domain, const phx_dfsph_configuration* configuration,
                 phx_execution_mode mode, hipStream_t stream);
+    ~DfsphSolver();
+    DfsphSolver(const DfsphSolver&) = delete;
+    DfsphSolver& operator=(const DfsphSolver&) = delete;
+
     void forward_step(const phx_sph_dfsph_state& state, const phx_control& control,
                       const phx_sph_dfsph_parameters& parameters, phx_sph_dfsph_state& next, DfsphCache& cache);
 
@@ -48,18 +76,34 @@ class DfsphSolver {
     PassProfiler& profiler() { return profiler_; }
     phx_execution_mode mode() const { return mode_; }
     bool warm_start() const { return config_.pressure_warm_start != 0u; }
+    float skin() const { return skin_; }
 
   private:
+    void adapt_skin();
+
     Domain domain_;
     phx_dfsph_configuration config_{};
     phx_execution_mode mode_;
     hipStream_t stream_;
     float gradient_norm_ = 0.0F;
+    float exact_threshold_ = 0.0F;
+    float skin_ = 0.3F;  // skin of the predicted-position masks, in units of h
     uint32_t last_iterations_[2]{};
+    uint32_t flag_count_ = 0;
     PassProfiler profiler_;
     // sorted-order workspace
-    VecBuffer x_, v_, u_, xs_, adiv_;
-    DeviceBuffer<float> m_, rho0_, mu_, relax_div_, relax_den_, p_div_, p_den_, t_div_, t_den_;
+    DeviceBuffer<float4> xm_, vmu_, xs_, adiv_;
+    VecBuffer u_;
+    DeviceBuffer<float> rho0_, relax_div_, relax_den_, p_div_, p_den_, t_div_, t_den_;
+    DeviceBuffer<uint16_t> mask_exact_, mask_skin_;
+    DeviceBuffer<uint32_t> flags_;
+    PinnedBuffer flags_host_;
+    hipEvent_t readback_ = nullptr;
+    bool readback_pending_ = false;
 };
+
+// Smallest float t with sqrtf(t) >= h: a candidate at squared distance d2
+// lies inside the support (sqrtf(d2) < h) exactly when d2 < t.
+float sqrt_support_threshold(float h);
 
 }  // namespace phx

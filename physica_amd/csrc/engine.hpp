@@ -111,28 +111,42 @@ void require_vec(const phx_const_vector_view& v, const char* what);
 
 inline uint32_t blocks_for(uint64_t count, uint32_t block) { return static_cast<uint32_t>((count + block - 1) / block); }
 
-// Per-particle float channels copied from original order into sorted order
-// while the grid is built.
+// Per-particle float channels copied between original order (contiguous
+// arrays) and sorted order (element s at sorted[s * stride], so four channels
+// can fill one float4 array).
 constexpr int kMaxChannels = 20;
 struct Channels {
     const float* src[kMaxChannels];
     float* dst[kMaxChannels];
+    uint32_t stride[kMaxChannels];
     uint32_t count;
 };
 
 class ChannelList {
   public:
-    void add(const float* src, float* dst) {
+    void add(const float* src, float* dst, uint32_t stride = 1u) {
         require(list_.count < static_cast<uint32_t>(kMaxChannels), PHX_ERROR_INVALID_ARGUMENT, "too many channels");
         require(src != nullptr, PHX_ERROR_INVALID_ARGUMENT, "null channel source");
         list_.src[list_.count] = src;
         list_.dst[list_.count] = dst;
+        list_.stride[list_.count] = stride;
         ++list_.count;
     }
     void add(CVec src, Vec dst) {
         add(src.x, dst.x);
         add(src.y, dst.y);
         add(src.z, dst.z);
+    }
+    // x, y, z into components 0..2 of a float4 array
+    void add(CVec src, float4* dst) {
+        float* base = reinterpret_cast<float*>(dst);
+        add(src.x, base + 0, 4u);
+        add(src.y, base + 1, 4u);
+        add(src.z, base + 2, 4u);
+    }
+    // one scalar into component `component` of a float4 array
+    void add(const float* src, float4* dst, int component) {
+        add(src, reinterpret_cast<float*>(dst) + component, 4u);
     }
     const Channels& get() const { return list_; }
 

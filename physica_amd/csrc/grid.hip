@@ -80,7 +80,7 @@ __global__ void __launch_bounds__(kBlock)
     for (uint32_t s = first; s < last; ++s) rank += unstable[s] < i ? 1u : 0u;
     const uint32_t pos = first + rank;
     perm[pos] = i;
-    for (uint32_t c = 0; c < channels.count; ++c) channels.dst[c][pos] = channels.src[c][i];
+    for (uint32_t c = 0; c < channels.count; ++c) channels.dst[c][pos * channels.stride[c]] = channels.src[c][i];
 }
 
 __global__ void __launch_bounds__(kBlock)
@@ -103,7 +103,7 @@ __global__ void __launch_bounds__(kBlock)
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
     const uint32_t i = perm[s];
-    for (uint32_t c = 0; c < channels.count; ++c) channels.dst[c][i] = channels.src[c][s];
+    for (uint32_t c = 0; c < channels.count; ++c) channels.dst[c][i] = channels.src[c][s * channels.stride[c]];
 }
 
 __global__ void __launch_bounds__(kBlock)
@@ -111,7 +111,7 @@ __global__ void __launch_bounds__(kBlock)
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
     const uint32_t i = perm[s];
-    for (uint32_t c = 0; c < channels.count; ++c) channels.dst[c][s] = channels.src[c][i];
+    for (uint32_t c = 0; c < channels.count; ++c) channels.dst[c][s * channels.stride[c]] = channels.src[c][i];
 }
 
 // ------------------------------------------------------------------- scan

@@ -1,0 +1,110 @@
+// Per-particle neighbour masks over the 27-cell stencil.
+//
+// For particle i and stencil slot k (offset_x outer, offset_z inner), bit b
+// of mask[k * N + i] marks the b-th particle (in sorted order) of that cell
+// as a neighbour. Walking slots in order and bits in ascending order visits
+// the marked candidates in exactly the reference's summation order (27 cells,
+// ascending original index), so a sum over the marked candidates equals the
+// full-stencil sum whenever every unmarked candidate contributes an exact
+// zero. Cells with more than kMaskBits particles cannot be encoded; the build
+// then raises a device flag and every pass takes the full-stencil path.
+#pragma once
+
+#include <cstdint>
+
+#include "device_math.hpp"
+
+namespace phx {
+
+constexpr int kStencil = 27;
+constexpr uint32_t kMaskBits = 16;
+
+// Device flags of a step: [0] mask overflow; [1 + k] largest squared
+// displacement |x* - x|^2 (float bits) written by the k-th predicting pass.
+constexpr uint32_t kFlagOverflow = 0;
+constexpr uint32_t kFlagDisplacement = 1;
+
+template <class Visit>
+__device__ __forceinline__ void for_each_masked(const SortedGrid& g, const uint16_t* __restrict__ masks, uint32_t n,
+                                                uint32_t i, int cx, int cy, int cz, Visit&& visit) {
+    int k = 0;
+    for (int ox = -1; ox <= 1; ++ox) {
+        for (int oy = -1; oy <= 1; ++oy) {
+            for (int oz = -1; oz <= 1; ++oz, ++k) {
+                uint32_t m = masks[static_cast<size_t>(k) * n + i];
+                if (m == 0u) continue;  // empty or outside the grid
+                const uint32_t c = static_cast<uint32_t>(cz + oz) * g.cy * g.cx +
+                                   static_cast<uint32_t>(cy + oy) * g.cx + static_cast<uint32_t>(cx + ox);
+                const uint32_t first = g.offsets[c];
+                do {
+                    const uint32_t b = static_cast<uint32_t>(__builtin_ctz(m));
+                    m &= m - 1u;
+                    visit(first + b);
+                } while (m != 0u);
+            }
+        }
+    }
+}
+
+// Builds both masks of particle i at position p: `exact` marks d2 < t_exact,
+// `skin` marks d2 < t_skin, with d2 = dot(p - x_s, p - x_s) evaluated as the
+// kernels do. Returns false if a stencil cell holds more than kMaskBits
+// particles (the masks of that cell are then truncated and must not be used).
+__device__ __forceinline__ bool build_masks(const SortedGrid& g, const float4* __restrict__ xm, uint32_t n, uint32_t i,
+                                            Float3 p, int cx, int cy, int cz, float t_exact, float t_skin,
+                                            uint16_t* __restrict__ exact, uint16_t* __restrict__ skin) {
+    bool complete = true;
+    int k = 0;
+    for (int ox = -1; ox <= 1; ++ox) {
+        const int X = cx + ox;
+        for (int oy = -1; oy <= 1; ++oy) {
+            const int Y = cy + oy;
+            for (int oz = -1; oz <= 1; ++oz, ++k) {
+                const int Z = cz + oz;
+                uint32_t me = 0u, ms = 0u;
+                if (X >= 0 && X < static_cast<int>(g.cx) && Y >= 0 && Y < static_cast<int>(g.cy) && Z >= 0 &&
+                    Z < static_cast<int>(g.cz)) {
+                    const uint32_t c = static_cast<uint32_t>(Z) * g.cy * g.cx + static_cast<uint32_t>(Y) * g.cx +
+                                       static_cast<uint32_t>(X);
+                    const uint32_t first = g.offsets[c];
+                    uint32_t last = g.offsets[c + 1u];
+                    if (last - first > kMaskBits) {
+                        complete = false;
+                        last = first + kMaskBits;
+                    }
+                    for (uint32_t s = first; s < last; ++s) {
+                        const float4 q = xm[s];
+                        const Float3 d = sub(p, Float3{q.x, q.y, q.z});
+                        const float d2 = dot(d, d);
+                        const uint32_t bit = 1u << (s - first);
+                        me |= d2 < t_exact ? bit : 0u;
+                        ms |= d2 < t_skin ? bit : 0u;
+                    }
+                }
+                exact[static_cast<size_t>(k) * n + i] = static_cast<uint16_t>(me);
+                skin[static_cast<size_t>(k) * n + i] = static_cast<uint16_t>(ms);
+            }
+        }
+    }
+    return complete;
+}
+
+// Block-wide maximum of a non-negative float, folded into *target with one
+// atomic per block (non-negative IEEE floats order like their bit patterns).
+template <uint32_t kBlockSize>
+__device__ __forceinline__ void block_max_to(float value, uint32_t* target) {
+    __shared__ float wave_max[kBlockSize / 64];
+    float v = value;
+    for (int offset = 32; offset > 0; offset >>= 1) v = fmaxf(v, __shfl_xor(v, offset, 64));
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t wave = threadIdx.x >> 6;
+    if (lane == 0u) wave_max[wave] = v;
+    __syncthreads();
+    if (threadIdx.x == 0u) {
+        float m = wave_max[0];
+        for (uint32_t w = 1; w < kBlockSize / 64; ++w) m = fmaxf(m, wave_max[w]);
+        atomicMax(target, __float_as_uint(m));
+    }
+}
+
+}  // namespace phx
