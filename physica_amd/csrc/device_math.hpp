@@ -80,6 +80,41 @@ __device__ __forceinline__ Float3 cubic_gradient(Float3 d, float h) {
     return scale(d, coef * deriv / (h * dist));
 }
 
+// Kernel constants evaluated once on the host with the reference's FP32
+// expressions (density/device.cuh:30,40,75): cubic 8/(pi h^3) and the
+// viscosity Laplacian's pi h^6.
+struct KernelConstants {
+    float h;
+    float cubic;        // 8 / (pi h h h)
+    float viscosity;    // pi h h h h h h
+    float sqrt_support; // smallest float t with sqrtf(t) >= h
+};
+
+// cubic(d, h) as a function of dist = length(d), with the constant supplied:
+// the same operations as cubic()
+__device__ __forceinline__ float cubic_value(float dist, const KernelConstants& k) {
+    if (dist >= k.h) return 0.0F;
+    const float q = dist / k.h;
+    if (q <= 0.5F) return k.cubic * (6.0F * q * q * q - 6.0F * q * q + 1.0F);
+    const float diff = 1.0F - q;
+    return k.cubic * 2.0F * diff * diff * diff;
+}
+
+// The radial factor f of cubic_gradient(d, h) = f * d (zero outside the
+// support and at d = 0), with the constant supplied.
+__device__ __forceinline__ float cubic_gradient_factor(float dist, const KernelConstants& k) {
+    if (dist == 0.0F || dist >= k.h) return 0.0F;
+    const float q = dist / k.h;
+    const float deriv = q <= 0.5F ? 18.0F * q * q - 12.0F * q : -6.0F * (1.0F - q) * (1.0F - q);
+    return k.cubic * deriv / (k.h * dist);
+}
+
+__device__ __forceinline__ float viscosity_laplacian_c(Float3 d, const KernelConstants& k) {
+    const float dist = length(d);
+    if (dist >= k.h) return 0.0F;
+    return 45.0F * (k.h - dist) / k.viscosity;
+}
+
 // viscosity Laplacian, density/device.cuh:72-76
 __device__ __forceinline__ float viscosity_laplacian(Float3 d, float h) {
     const float dist = length(d);

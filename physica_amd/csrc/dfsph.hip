@@ -31,14 +31,17 @@
 namespace phx {
 namespace {
 
+__device__ __forceinline__ Float3 xyz(float4 q) { return Float3{q.x, q.y, q.z}; }
+
 constexpr uint32_t kBlock = 256;
 
-__device__ __forceinline__ Float3 xyz(float4 q) { return Float3{q.x, q.y, q.z}; }
+__device__ __forceinline__ uint32_t cell_of(const SortedGrid& g, int x, int y, int z) {
+    return (static_cast<uint32_t>(z) * g.cy + static_cast<uint32_t>(y)) * g.cx + static_cast<uint32_t>(x);
+}
 
 struct MaskDensityArgs {
     uint32_t n;
-    float h;
-    float t_exact;
+    KernelConstants kc;
     float t_skin;
     SortedGrid grid;
     const float4* xm;
@@ -56,22 +59,32 @@ struct MaskDensityArgs {
 // topology = positions) over the exact mask and the first_term p/rho^2 of
 // pressure_forward_kernel (common.cu:171) for both warm-started pressures.
 __global__ void __launch_bounds__(kBlock) dfsph_mask_density_kernel(MaskDensityArgs a) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ uint16_t smask[kStencil * kBlock];
+    __shared__ int delta[kStencil];
+    fill_slot_deltas(a.grid, delta);
+    __syncthreads();
+    const uint32_t i = xcd_block() * kBlock + threadIdx.x;
     if (i >= a.n) return;
+    uint16_t* column = smask + threadIdx.x;
     const Float3 p = xyz(a.xm[i]);
     int cx, cy, cz;
     particle_cell(a.grid, p, cx, cy, cz);
-    const bool complete = build_masks(a.grid, a.xm, a.n, i, p, cx, cy, cz, a.t_exact, a.t_skin, a.exact, a.skin);
-    if (!complete) atomicOr(&a.flags[kFlagOverflow], 1u);
+    uint32_t nonzero = 0u;
+    const bool complete = build_masks(a.grid, a.xm, a.n, i, p, cx, cy, cz, a.kc.sqrt_support, a.t_skin, a.exact,
+                                      a.skin, column, kBlock, &nonzero);
     float result = 0.0F;
-    auto visit = [&](uint32_t s) {
-        const float4 q = a.xm[s];
-        result += q.w * cubic(sub(p, xyz(q)), a.h);
-    };
     if (complete) {
-        for_each_masked(a.grid, a.exact, a.n, i, cx, cy, cz, visit);
+        walk_masked(a.grid.offsets, cell_of(a.grid, cx, cy, cz), delta, column, kBlock, nonzero, [&](uint32_t s) {
+            const float4 q = a.xm[s];
+            const float dist = length(sub(p, xyz(q)));
+            result += q.w * cubic_value(dist, a.kc);
+        });
     } else {
-        for_each_candidate(a.grid, cx, cy, cz, visit);
+        atomicOr(&a.flags[kFlagOverflow], 1u);
+        for_each_candidate(a.grid, cx, cy, cz, [&](uint32_t s) {
+            const float4 q = a.xm[s];
+            result += q.w * cubic_value(length(sub(p, xyz(q))), a.kc);
+        });
     }
     a.rho[i] = result;
     a.t_div[i] = a.p_div[i] / (result * result);
@@ -80,7 +93,7 @@ __global__ void __launch_bounds__(kBlock) dfsph_mask_density_kernel(MaskDensityA
 
 struct ProjectionArgs {
     uint32_t n;
-    float h;
+    KernelConstants kc;
     float dt;
     float gradient_norm;
     float limit2;  // largest squared displacement for which the skin mask is exact
@@ -101,7 +114,11 @@ struct ProjectionArgs {
 // positions = x*) fused with projection_update_forward_kernel
 // (dfsph.cu:104-110) and the next pressure pass's first_term.
 __global__ void __launch_bounds__(kBlock) dfsph_projection_kernel(ProjectionArgs a) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ uint16_t smask[kStencil * kBlock];
+    __shared__ int delta[kStencil];
+    fill_slot_deltas(a.grid, delta);
+    __syncthreads();
+    const uint32_t i = xcd_block() * kBlock + threadIdx.x;
     if (i >= a.n) return;
     const bool masked = a.flags[kFlagOverflow] == 0u &&
                         __uint_as_float(a.flags[kFlagDisplacement + a.slot]) <= a.limit2;
@@ -112,17 +129,19 @@ __global__ void __launch_bounds__(kBlock) dfsph_projection_kernel(ProjectionArgs
     float rs = 0.0F;
     auto visit = [&](uint32_t s) {
         const float4 q = a.xs[s];
-        rs += q.w * cubic(sub(ps, xyz(q)), a.h);
+        rs += q.w * cubic_value(length(sub(ps, xyz(q))), a.kc);
     };
     if (masked) {
-        for_each_masked(a.grid, a.skin, a.n, i, cx, cy, cz, visit);
+        uint16_t* column = smask + threadIdx.x;
+        const uint32_t nonzero = load_lane_column(a.skin, a.n, i, column, kBlock);
+        walk_masked(a.grid.offsets, cell_of(a.grid, cx, cy, cz), delta, column, kBlock, nonzero, visit);
     } else {
         for_each_candidate(a.grid, cx, cy, cz, visit);
     }
     const float rho = a.rho[i];
     const float volume = self.w / rho;
-    const float delta = 1.0F / (2.0F * a.dt * a.dt * volume * volume * a.gradient_norm);
-    const float pressure = fmaxf(0.0F, a.p[i] + a.relaxation[i] * delta * (rs - a.target[i]));
+    const float delta_p = 1.0F / (2.0F * a.dt * a.dt * volume * volume * a.gradient_norm);
+    const float pressure = fmaxf(0.0F, a.p[i] + a.relaxation[i] * delta_p * (rs - a.target[i]));
     a.p[i] = pressure;
     a.t[i] = pressure / (rho * rho);
 }
@@ -135,7 +154,7 @@ enum Epilogue : int {
 
 struct ForceArgs {
     uint32_t n;
-    float h;
+    KernelConstants kc;
     float dt;
     SortedGrid grid;
     Box box;
@@ -165,9 +184,16 @@ struct ForceArgs {
 // optionally fused with the non-pressure acceleration
 // (non_pressure_forward_kernel, common.cu:14-47), then the predict of the
 // next iteration (dfsph.cu:26-44) or, on the last pass, the next state.
+// cubic_gradient(d, h) is evaluated as f * d with f its radial factor, the
+// same operations as the reference's expression.
 template <bool kNonPressure, int kEpilogue>
 __global__ void __launch_bounds__(kBlock) dfsph_force_kernel(ForceArgs a) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    constexpr bool kTwo = kEpilogue == kDivEnd;
+    __shared__ uint16_t smask[kStencil * kBlock];
+    __shared__ int delta[kStencil];
+    fill_slot_deltas(a.grid, delta);
+    __syncthreads();
+    const uint32_t i = xcd_block() * kBlock + threadIdx.x;
     const bool active = i < a.n;
     float moved2 = 0.0F;
     if (active) {
@@ -180,30 +206,40 @@ __global__ void __launch_bounds__(kBlock) dfsph_force_kernel(ForceArgs a) {
         if constexpr (kNonPressure) acc = add(a.gravity, load(a.u, i));
         const float t1_i = a.t1[i];
         float t2_i = 0.0F;
-        if constexpr (kEpilogue == kDivEnd) t2_i = a.t2[i];
+        if constexpr (kTwo) t2_i = a.t2[i];
         Float3 ap1{0.0F, 0.0F, 0.0F};
         Float3 ap2{0.0F, 0.0F, 0.0F};
         int cx, cy, cz;
         particle_cell(a.grid, p, cx, cy, cz);
-        auto visit = [&](uint32_t s) {
-            if (s == i) return;
-            const float4 q = a.xm[s];
-            const Float3 d = sub(p, xyz(q));
+        auto pair = [&](uint32_t s, float4 q, Float3 d, float f) {
             const float ms = q.w;
             if constexpr (kNonPressure) {
                 const float4 vq = a.vmu[s];
                 const float viscosity = 0.5F * (vm.w + vq.w);
-                const float weight = viscosity * ms * viscosity_laplacian(d, a.h) / a.rho[s];
+                const float weight = viscosity * ms * viscosity_laplacian_c(d, a.kc) / a.rho[s];
                 acc = add(acc, scale(sub(xyz(vq), vel), weight));
             }
-            const Float3 gradient = cubic_gradient(d, a.h);
+            const Float3 gradient = scale(d, f);
             ap1 = add(ap1, scale(gradient, -ms * (t1_i + a.t1[s])));
-            if constexpr (kEpilogue == kDivEnd) ap2 = add(ap2, scale(gradient, -ms * (t2_i + a.t2[s])));
+            if constexpr (kTwo) ap2 = add(ap2, scale(gradient, -ms * (t2_i + a.t2[s])));
         };
         if (masked) {
-            for_each_masked(a.grid, a.exact, a.n, i, cx, cy, cz, visit);
+            uint16_t* column = smask + threadIdx.x;
+            const uint32_t nonzero = load_lane_column(a.exact, a.n, i, column, kBlock);
+            walk_masked(a.grid.offsets, cell_of(a.grid, cx, cy, cz), delta, column, kBlock, nonzero,
+                        [&](uint32_t s) {
+                            if (s == i) return;
+                            const float4 q = a.xm[s];
+                            const Float3 d = sub(p, xyz(q));
+                            pair(s, q, d, cubic_gradient_factor(length(d), a.kc));
+                        });
         } else {
-            for_each_candidate(a.grid, cx, cy, cz, visit);
+            for_each_candidate(a.grid, cx, cy, cz, [&](uint32_t s) {
+                if (s == i) return;
+                const float4 q = a.xm[s];
+                const Float3 d = sub(p, xyz(q));
+                pair(s, q, d, cubic_gradient_factor(length(d), a.kc));
+            });
         }
         if constexpr (kNonPressure) {
             a.anp[i] = make_float4(acc.x, acc.y, acc.z, 0.0F);
@@ -213,7 +249,7 @@ __global__ void __launch_bounds__(kBlock) dfsph_force_kernel(ForceArgs a) {
         Float3 total;
         if constexpr (kEpilogue == kDivPredict) {
             total = ap1;
-        } else if constexpr (kEpilogue == kDivEnd) {
+        } else if constexpr (kTwo) {
             a.adiv[i] = make_float4(ap1.x, ap1.y, ap1.z, 0.0F);
             total = add(ap1, ap2);
         } else {
@@ -221,7 +257,7 @@ __global__ void __launch_bounds__(kBlock) dfsph_force_kernel(ForceArgs a) {
         }
         Float3 nx, nv;
         predict_clamped(a.box, a.dt, p, vel, add(acc, total), nx, nv);
-        if constexpr (kEpilogue == kDivEnd) {
+        if constexpr (kTwo) {
             if (a.warm_div_out != nullptr) a.warm_div_out[a.perm[i]] = a.p_div[i];
         }
         if (a.final_step != 0u) {
@@ -239,19 +275,20 @@ __global__ void __launch_bounds__(kBlock) dfsph_force_kernel(ForceArgs a) {
 }
 
 template <bool kNonPressure, int kEpilogue>
-void launch_force(PassProfiler& profiler, hipStream_t stream, const ForceArgs& args) {
+void launch_force(PassProfiler& profiler, hipStream_t stream, uint32_t blocks, const ForceArgs& args) {
     const int pass = kNonPressure ? kDfsphForceInit
                      : kEpilogue == kDivPredict ? kDfsphForceDivergence
                      : kEpilogue == kDivEnd     ? kDfsphForceDivergenceEnd
                                                 : kDfsphForceDensity;
     PassScope scope(profiler, stream, pass);
-    dfsph_force_kernel<kNonPressure, kEpilogue><<<blocks_for(args.n, kBlock), kBlock, 0, stream>>>(args);
+    dfsph_force_kernel<kNonPressure, kEpilogue><<<blocks, kBlock, 0, stream>>>(args);
     PHX_LAUNCH_CHECK();
 }
 
-void launch_projection(PassProfiler& profiler, hipStream_t stream, const ProjectionArgs& args, int pass) {
+void launch_projection(PassProfiler& profiler, hipStream_t stream, uint32_t blocks, const ProjectionArgs& args,
+                       int pass) {
     PassScope scope(profiler, stream, pass);
-    dfsph_projection_kernel<<<blocks_for(args.n, kBlock), kBlock, 0, stream>>>(args);
+    dfsph_projection_kernel<<<blocks, kBlock, 0, stream>>>(args);
     PHX_LAUNCH_CHECK();
 }
 
@@ -307,6 +344,15 @@ void DfsphCache::allocate(uint32_t n, uint32_t cells) {
     non_pressure.resize(n);
 }
 
+KernelConstants kernel_constants(float h) {
+    KernelConstants k{};
+    k.h = h;
+    k.cubic = 8.0F / (kPi * h * h * h);
+    k.viscosity = kPi * h * h * h * h * h * h;
+    k.sqrt_support = sqrt_support_threshold(h);
+    return k;
+}
+
 DfsphSolver::DfsphSolver(const phx_domain_configuration* domain, const phx_dfsph_configuration* configuration,
                          phx_execution_mode mode, hipStream_t stream)
     : domain_(make_domain(domain)), mode_(mode), stream_(stream) {
@@ -318,7 +364,7 @@ DfsphSolver::DfsphSolver(const phx_domain_configuration* domain, const phx_dfsph
     require(config_.divergence_iterations + config_.density_iterations < 4096u, PHX_ERROR_INVALID_ARGUMENT,
             "too many pressure iterations");
     gradient_norm_ = reference_gradient_norm(domain_.config);
-    exact_threshold_ = sqrt_support_threshold(domain_.config.support_radius);
+    constants_ = kernel_constants(domain_.config.support_radius);
     const uint32_t n = domain_.config.particle_count;
     for (DeviceBuffer<float4>* b : {&xm_, &vmu_, &xs_, &adiv_}) b->resize(n);
     u_.resize(n);
@@ -410,7 +456,8 @@ void DfsphSolver::forward_step(const phx_sph_dfsph_state& state, const phx_contr
         fill_u32(stream_, flags_.data(), flag_count_, 0u);
     }
     const SortedGrid grid = cache.neighborhood.grid();
-    const uint32_t blocks = blocks_for(n, kBlock);
+    // whole blocks of kBlock particles, padded to a multiple of the XCD count
+    const uint32_t blocks = (blocks_for(n, kBlock) + 7u) / 8u * 8u;
     const float h = c.support_radius;
     // skin radius R = h (1 + skin): the mask keeps d2 < R^2 (with margin), and
     // an A pass may use it while every particle moved at most (R - h) / 2
@@ -419,8 +466,18 @@ void DfsphSolver::forward_step(const phx_sph_dfsph_state& state, const phx_contr
     const float limit = 0.5F * (radius - h * (1.0F + 2e-5F));
     const float limit2 = limit > 0.0F ? limit * limit : -1.0F;
 
-    MaskDensityArgs ma{n, h, exact_threshold_, t_skin, grid, xm_.data(), mask_exact_.data(), mask_skin_.data(),
-                       p_div_.data(), p_den_.data(), cache.densities.data(), t_div_.data(), t_den_.data(),
+    MaskDensityArgs ma{n,
+                       constants_,
+                       t_skin,
+                       grid,
+                       xm_.data(),
+                       mask_exact_.data(),
+                       mask_skin_.data(),
+                       p_div_.data(),
+                       p_den_.data(),
+                       cache.densities.data(),
+                       t_div_.data(),
+                       t_den_.data(),
                        flags_.data()};
     {
         PassScope scope(profiler_, stream_, kDfsphDensity);
@@ -430,7 +487,7 @@ void DfsphSolver::forward_step(const phx_sph_dfsph_state& state, const phx_contr
 
     ForceArgs fa{};
     fa.n = n;
-    fa.h = h;
+    fa.kc = constants_;
     fa.dt = c.time_step;
     fa.grid = grid;
     fa.box = collision_box(c, step);
@@ -454,7 +511,7 @@ void DfsphSolver::forward_step(const phx_sph_dfsph_state& state, const phx_contr
 
     ProjectionArgs pa{};
     pa.n = n;
-    pa.h = h;
+    pa.kc = constants_;
     pa.dt = c.time_step;
     pa.gradient_norm = gradient_norm_;
     pa.limit2 = limit2;
@@ -475,23 +532,23 @@ void DfsphSolver::forward_step(const phx_sph_dfsph_state& state, const phx_contr
     fa.slot = slot;
     if (nd == 0u) {
         fa.final_step = ne == 0u ? 1u : 0u;
-        launch_force<true, kDivEnd>(profiler_, stream_, fa);
+        launch_force<true, kDivEnd>(profiler_, stream_, blocks, fa);
     } else {
         fa.final_step = 0u;
-        launch_force<true, kDivPredict>(profiler_, stream_, fa);
+        launch_force<true, kDivPredict>(profiler_, stream_, blocks, fa);
         pa.target = cache.densities.data();
         pa.relaxation = relax_div_.data();
         pa.p = p_div_.data();
         pa.t = t_div_.data();
         for (uint32_t it = 1; it <= nd; ++it) {  // sph.cpp:580-587
             pa.slot = slot;
-            launch_projection(profiler_, stream_, pa, kDfsphProjectionDivergence);
+            launch_projection(profiler_, stream_, blocks, pa, kDfsphProjectionDivergence);
             fa.slot = ++slot;
             if (it < nd) {
-                launch_force<false, kDivPredict>(profiler_, stream_, fa);
+                launch_force<false, kDivPredict>(profiler_, stream_, blocks, fa);
             } else {
                 fa.final_step = ne == 0u ? 1u : 0u;
-                launch_force<false, kDivEnd>(profiler_, stream_, fa);
+                launch_force<false, kDivEnd>(profiler_, stream_, blocks, fa);
             }
         }
     }
@@ -505,10 +562,10 @@ void DfsphSolver::forward_step(const phx_sph_dfsph_state& state, const phx_contr
     fa.warm_div_out = nullptr;
     for (uint32_t it = 1; it <= ne; ++it) {
         pa.slot = slot;
-        launch_projection(profiler_, stream_, pa, kDfsphProjectionDensity);
+        launch_projection(profiler_, stream_, blocks, pa, kDfsphProjectionDensity);
         fa.slot = ++slot;
         fa.final_step = it == ne ? 1u : 0u;
-        launch_force<false, kDenPredict>(profiler_, stream_, fa);
+        launch_force<false, kDenPredict>(profiler_, stream_, blocks, fa);
     }
     // read the step's displacements back for the next step's skin
     PHX_HIP(hipMemcpyAsync(flags_host_.data(), flags_.data(), sizeof(uint32_t) * flag_count_, hipMemcpyDeviceToHost,

@@ -86,7 +86,7 @@ class DfsphSolver {
     phx_execution_mode mode_;
     hipStream_t stream_;
     float gradient_norm_ = 0.0F;
-    float exact_threshold_ = 0.0F;
+    KernelConstants constants_{};
     float skin_ = 0.3F;  // skin of the predicted-position masks, in units of h
     uint32_t last_iterations_[2]{};
     uint32_t flag_count_ = 0;
@@ -101,6 +101,9 @@ class DfsphSolver {
     hipEvent_t readback_ = nullptr;
     bool readback_pending_ = false;
 };
+
+// Constants of the cubic and viscosity kernels for support radius h.
+KernelConstants kernel_constants(float h);
 
 // Smallest float t with sqrtf(t) >= h: a candidate at squared distance d2
 // lies inside the support (sqrtf(d2) < h) exactly when d2 < t.

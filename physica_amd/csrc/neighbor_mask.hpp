@@ -24,35 +24,72 @@ constexpr uint32_t kMaskBits = 16;
 constexpr uint32_t kFlagOverflow = 0;
 constexpr uint32_t kFlagDisplacement = 1;
 
-template <class Visit>
-__device__ __forceinline__ void for_each_masked(const SortedGrid& g, const uint16_t* __restrict__ masks, uint32_t n,
-                                                uint32_t i, int cx, int cy, int cz, Visit&& visit) {
-    int k = 0;
-    for (int ox = -1; ox <= 1; ++ox) {
-        for (int oy = -1; oy <= 1; ++oy) {
-            for (int oz = -1; oz <= 1; ++oz, ++k) {
-                uint32_t m = masks[static_cast<size_t>(k) * n + i];
-                if (m == 0u) continue;  // empty or outside the grid
-                const uint32_t c = static_cast<uint32_t>(cz + oz) * g.cy * g.cx +
-                                   static_cast<uint32_t>(cy + oy) * g.cx + static_cast<uint32_t>(cx + ox);
-                const uint32_t first = g.offsets[c];
-                do {
-                    const uint32_t b = static_cast<uint32_t>(__builtin_ctz(m));
-                    m &= m - 1u;
-                    visit(first + b);
-                } while (m != 0u);
-            }
-        }
+// Cell-index offset of each stencil slot k = (ox + 1) * 9 + (oy + 1) * 3 +
+// (oz + 1): ox + oy Cx + oz Cx Cy (two's complement). Filled by the first 27
+// threads of the block; callers synchronise before use.
+__device__ __forceinline__ void fill_slot_deltas(const SortedGrid& g, int* delta) {
+    const uint32_t k = threadIdx.x;
+    if (k < static_cast<uint32_t>(kStencil)) {
+        const int ox = static_cast<int>(k) / 9 - 1;
+        const int oy = (static_cast<int>(k) / 3) % 3 - 1;
+        const int oz = static_cast<int>(k) % 3 - 1;
+        delta[k] = ox + oy * static_cast<int>(g.cx) + oz * static_cast<int>(g.cx) * static_cast<int>(g.cy);
     }
+}
+
+// Copies the 27 masks of particle i into its LDS column and returns the
+// bitmap of slots with a non-empty mask.
+__device__ __forceinline__ uint32_t load_lane_column(const uint16_t* __restrict__ masks, uint32_t n, uint32_t i,
+                                                     uint16_t* column, uint32_t stride) {
+    uint32_t nonzero = 0u;
+#pragma unroll
+    for (int k = 0; k < kStencil; ++k) {
+        const uint16_t m = masks[static_cast<size_t>(k) * n + i];
+        column[static_cast<uint32_t>(k) * stride] = m;
+        nonzero |= m != 0u ? 1u << k : 0u;
+    }
+    return nonzero;
+}
+
+// Marked candidates of one lane in the reference order, one candidate per
+// loop trip; `nonzero` lists the slots to visit, `center` is the lane's cell.
+template <class Visit>
+__device__ __forceinline__ void walk_masked(const uint32_t* __restrict__ offsets, uint32_t center, const int* delta,
+                                            const uint16_t* column, uint32_t stride, uint32_t nonzero, Visit&& visit) {
+    uint32_t m = 0u;
+    uint32_t base = 0u;
+    for (;;) {
+        if (m == 0u) {
+            if (nonzero == 0u) break;
+            const uint32_t k = static_cast<uint32_t>(__builtin_ctz(nonzero));
+            nonzero &= nonzero - 1u;
+            m = column[k * stride];
+            base = offsets[center + static_cast<uint32_t>(delta[k])];
+        }
+        const uint32_t b = static_cast<uint32_t>(__builtin_ctz(m));
+        m &= m - 1u;
+        visit(base + b);
+    }
+}
+
+// Workgroup index with consecutive ranges of indices on one XCD (MI355X
+// dispatches workgroup b to XCD b % 8). gridDim.x must be a multiple of 8.
+__device__ __forceinline__ uint32_t xcd_block() {
+    const uint32_t per_xcd = gridDim.x / 8u;
+    return (blockIdx.x % 8u) * per_xcd + blockIdx.x / 8u;
 }
 
 // Builds both masks of particle i at position p: `exact` marks d2 < t_exact,
 // `skin` marks d2 < t_skin, with d2 = dot(p - x_s, p - x_s) evaluated as the
-// kernels do. Returns false if a stencil cell holds more than kMaskBits
-// particles (the masks of that cell are then truncated and must not be used).
+// kernels do. The exact mask is also copied to `column` and its non-empty
+// slots are added to the bitmap `nonzero`, if given. Returns false if a
+// stencil cell holds more than kMaskBits particles (the masks of that cell are
+// then truncated and must not be used).
 __device__ __forceinline__ bool build_masks(const SortedGrid& g, const float4* __restrict__ xm, uint32_t n, uint32_t i,
                                             Float3 p, int cx, int cy, int cz, float t_exact, float t_skin,
-                                            uint16_t* __restrict__ exact, uint16_t* __restrict__ skin) {
+                                            uint16_t* __restrict__ exact, uint16_t* __restrict__ skin,
+                                            uint16_t* column = nullptr, uint32_t stride = 0u,
+                                            uint32_t* nonzero = nullptr) {
     bool complete = true;
     int k = 0;
     for (int ox = -1; ox <= 1; ++ox) {
@@ -83,6 +120,8 @@ __device__ __forceinline__ bool build_masks(const SortedGrid& g, const float4* _
                 }
                 exact[static_cast<size_t>(k) * n + i] = static_cast<uint16_t>(me);
                 skin[static_cast<size_t>(k) * n + i] = static_cast<uint16_t>(ms);
+                if (column != nullptr) column[static_cast<uint32_t>(k) * stride] = static_cast<uint16_t>(me);
+                if (nonzero != nullptr && me != 0u) *nonzero |= 1u << k;
             }
         }
     }
