@@ -238,6 +238,69 @@ phx_status phx_dfsph_cache_neighborhood(const phx_dfsph_step_cache* cache, uint6
 phx_status phx_dfsph_cache_densities(const phx_dfsph_step_cache* cache, float* densities);
 phx_status phx_dfsph_cache_non_pressure_accelerations(const phx_dfsph_step_cache* cache, phx_vector_view out);
 
+/* ------------------------------------------------ WCSPH, PCISPH, IISPH */
+/* SPH<WCSPH>, SPH<PCISPH>, SPH<IISPH> (sph/sph.ixx:39-193, 273-372): one
+ * solver family selected by `method`. */
+typedef enum phx_sph_method { PHX_SPH_WCSPH = 0, PHX_SPH_PCISPH = 1, PHX_SPH_IISPH = 2 } phx_sph_method;
+
+/* WCSPH::Configuration is empty (sph/sph.ixx:40); PCISPH/IISPH::Configuration
+ * {pressure_iterations, checkpoint_interval} (sph/sph.ixx:104-107, 150-153). */
+typedef struct phx_sph_configuration {
+    phx_sph_method method;
+    uint32_t pressure_iterations; /* PCISPH, IISPH; reference default 6 */
+    uint32_t checkpoint_interval; /* PCISPH, IISPH; reference default 2 */
+} phx_sph_configuration;
+
+/* Method parameters, one value per particle (sph/sph.ixx:45-49, 112, 158).
+ * Only the arrays of the selected method are read; the others may be null. */
+typedef struct phx_sph_method_parameters {
+    const float* speed_of_sound;           /* WCSPH::Parameters */
+    const float* tait_exponent;            /* WCSPH::Parameters */
+    const float* boundary_surface_tension; /* WCSPH::Parameters */
+    const float* pressure_relaxation;      /* PCISPH::Parameters */
+    const float* jacobi_relaxation;        /* IISPH::Parameters */
+} phx_sph_method_parameters;
+
+/* SPH<Method>::State (the method states of these three are empty) and
+ * SPH<Method>::Parameters, sph/sph.ixx:275-298. */
+typedef struct phx_sph_state {
+    phx_particle_state particles;
+} phx_sph_state;
+
+typedef struct phx_sph_parameters {
+    phx_particle_parameters particles;
+    phx_sph_method_parameters method;
+} phx_sph_parameters;
+
+typedef struct phx_sph phx_sph;
+typedef struct phx_sph_step_cache phx_sph_step_cache;
+
+/* SPH<Method>(domain, configuration, mode, stream), sph/sph.ixx:309-312 */
+phx_status phx_sph_create(const phx_domain_configuration* domain, const phx_sph_configuration* configuration,
+                          phx_execution_mode mode, phx_stream stream, phx_sph** solver);
+phx_status phx_sph_destroy(phx_sph* solver);
+/* allocate_step_cache, sph/sph.ixx:322 */
+phx_status phx_sph_allocate_step_cache(phx_sph* solver, phx_sph_step_cache** cache);
+phx_status phx_sph_free_step_cache(phx_sph_step_cache* cache);
+/* forward_step(state, control, parameters, next_state, cache), sph/sph.ixx:350-354 */
+phx_status phx_sph_forward_step(phx_sph* solver, const phx_sph_state* state, const phx_control* control,
+                                const phx_sph_parameters* parameters, phx_sph_state* next_state,
+                                phx_sph_step_cache* cache);
+/* The reference_gradient_norm of PCISPH/IISPH (sph/sph.cpp:37-58, 252, 365). */
+phx_status phx_sph_reference_gradient_norm(const phx_sph* solver, float* value);
+phx_status phx_sph_set_profiling(phx_sph* solver, int enabled);
+phx_status phx_sph_read_profile(phx_sph* solver, phx_pass_profile* profile, uint32_t capacity, uint32_t* count);
+
+/* Cache inspection in original particle order, enqueued on the solver
+ * stream; any output may be null. Pressures: WCSPH::Cache::pressures, or the
+ * last iteration's pressures of PCISPH/IISPH (PressureIterationCache).
+ * Non-pressure accelerations: PCISPH/IISPH::Cache only. */
+phx_status phx_sph_cache_neighborhood(const phx_sph_step_cache* cache, uint64_t* sorted_keys,
+                                      uint32_t* sorted_particle_indices, uint32_t* cell_offsets);
+phx_status phx_sph_cache_densities(const phx_sph_step_cache* cache, float* densities);
+phx_status phx_sph_cache_pressures(const phx_sph_step_cache* cache, float* pressures);
+phx_status phx_sph_cache_non_pressure_accelerations(const phx_sph_step_cache* cache, phx_vector_view out);
+
 /* ------------------------------------------------------------------- PBF */
 /* PBF::Configuration, pbf/pbf.ixx:15-18 */
 typedef struct phx_pbf_configuration {

@@ -243,6 +243,51 @@ def dfsph_step(domain, step_index, positions, velocities, controls, params: Dict
             "densities": rho, "non_pressure_accelerations": anp}
 
 
+def pressure_solver_step(domain, step_index, positions, velocities, controls, params: Dict[str, np.ndarray],
+                         method="pcisph", pressure_iterations=6, precision="f32"):
+    """One SPH<PCISPH> or SPH<IISPH>::forward_step (params["relaxation"] is the
+    pressure or Jacobi relaxation). Returns next state and cache fields."""
+    if method not in ("pcisph", "iisph"):
+        raise ValueError(method)
+    n = domain.particle_count
+    pos, pp = _vec(positions, precision)
+    vel, vp = _vec(velocities, precision)
+    ctl, cp = _vec(controls, precision)
+    names = ("masses", "rest_densities", "viscosities", "relaxation")
+    arrs = {k: _arr(params[k], precision) for k in names}
+    nx, nxp = _out_vec(n, precision)
+    nv, nvp = _out_vec(n, precision)
+    rho = np.zeros(n, dtype=_dtype(precision))
+    prs = np.zeros(n, dtype=_dtype(precision))
+    anp, anpp = _out_vec(n, precision)
+    getattr(library(), "oracle_pressure_solver_step_" + precision)(
+        ctypes.byref(domain), c_int(1 if method == "iisph" else 0), c_uint32(pressure_iterations),
+        c_uint64(step_index), pp, vp, cp, *(c_void_p(arrs[k].ctypes.data) for k in names), nxp, nvp,
+        c_void_p(rho.ctypes.data), anpp, c_void_p(prs.ctypes.data))
+    return {"positions": nx, "velocities": nv, "densities": rho, "non_pressure_accelerations": anp,
+            "pressures": prs}
+
+
+def wcsph_step(domain, step_index, positions, velocities, controls, params: Dict[str, np.ndarray],
+               precision="f32"):
+    """One SPH<WCSPH>::forward_step. Returns next state, densities and pressures."""
+    n = domain.particle_count
+    pos, pp = _vec(positions, precision)
+    vel, vp = _vec(velocities, precision)
+    ctl, cp = _vec(controls, precision)
+    names = ("masses", "rest_densities", "viscosities", "surface_tensions", "speed_of_sound", "tait_exponent",
+             "boundary_surface_tension")
+    arrs = {k: _arr(params[k], precision) for k in names}
+    nx, nxp = _out_vec(n, precision)
+    nv, nvp = _out_vec(n, precision)
+    rho = np.zeros(n, dtype=_dtype(precision))
+    prs = np.zeros(n, dtype=_dtype(precision))
+    getattr(library(), "oracle_wcsph_step_" + precision)(
+        ctypes.byref(domain), c_uint64(step_index), pp, vp, cp, *(c_void_p(arrs[k].ctypes.data) for k in names),
+        nxp, nvp, c_void_p(rho.ctypes.data), c_void_p(prs.ctypes.data))
+    return {"positions": nx, "velocities": nv, "densities": rho, "pressures": prs}
+
+
 def pbf_step(domain, step_index, positions, velocities, controls, params: Dict[str, np.ndarray],
              pressure_iterations=4, precision="f32"):
     """One PBF::forward_step. Returns a dict of next state and cache fields."""

@@ -88,6 +88,20 @@ void oracle_set_threads(int threads);
                                 const real* warm_divergence, const real* warm_density, real* next_positions[3],         \
                                 real* next_velocities[3], real* next_warm_divergence, real* next_warm_density,          \
                                 real* densities_out, real* non_pressure_out[3]);                                        \
+    /* SPH<PCISPH|IISPH>::forward_step: sph.cpp:296-315 / 409-428 (iisph selects V = m / rho) */                        \
+    void oracle_pressure_solver_step##suf(const oracle_domain* domain, int iisph, uint32_t pressure_iterations,       \
+                                          uint64_t step_index, const real* positions[3], const real* velocities[3],    \
+                                          const real* controls[3], const real* masses, const real* rest_densities,     \
+                                          const real* viscosities, const real* relaxation, real* next_positions[3],    \
+                                          real* next_velocities[3], real* densities_out, real* non_pressure_out[3],    \
+                                          real* pressures_out);                                                         \
+    /* SPH<WCSPH>::forward_step: sph.cpp:214-226 */                                                                     \
+    void oracle_wcsph_step##suf(const oracle_domain* domain, uint64_t step_index, const real* positions[3],           \
+                                const real* velocities[3], const real* controls[3], const real* masses,                \
+                                const real* rest_densities, const real* viscosities, const real* surface_tensions,     \
+                                const real* speed_of_sound, const real* tait_exponent,                                  \
+                                const real* boundary_surface_tension, real* next_positions[3],                          \
+                                real* next_velocities[3], real* densities_out, real* pressures_out);                    \
     /* PBF::forward_step: pbf.cpp:167-191 */                                                                            \
     void oracle_pbf_step##suf(const oracle_domain* domain, uint32_t pressure_iterations, uint64_t step_index,         \
                               const real* positions[3], const real* velocities[3], const real* controls[3],            \

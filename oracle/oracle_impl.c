@@ -427,15 +427,17 @@ static void add_kernel(uint32_t n, const real* const a[3], const real* const b[3
         for (int c = 0; c < 3; ++c) out[c][i] = a[c][i] + b[c][i];
 }
 
-/* predict_forward_kernel, sph/dfsph.cu:26-44 (the same body as
- * integrate_forward_kernel, sph/common.cu:302-320, with a = a1 + a2) */
+/* predict_forward_kernel, sph/dfsph.cu:26-44 and pcisph.cu:26-44 / iisph.cu
+ * (a = a1 + a2); with a2 == NULL it is integrate_forward_kernel,
+ * sph/common.cu:302-320 (a = a1, no addition). */
 static void sph_predict_kernel(uint32_t n, real dt, cbox box, const real* const pos[3], const real* const vel[3],
                                const real* const a1[3], const real* const a2[3], real* const out_pos[3],
                                real* const out_vel[3]) {
 #pragma omp parallel for schedule(static)
     for (int64_t ii = 0; ii < (int64_t)n; ++ii) {
         const uint32_t i = (uint32_t)ii;
-        v3 v = v3_add(v3_load(vel, i), v3_scale(v3_add(v3_load(a1, i), v3_load(a2, i)), dt));
+        const v3 a = a2 ? v3_add(v3_load(a1, i), v3_load(a2, i)) : v3_load(a1, i);
+        v3 v = v3_add(v3_load(vel, i), v3_scale(a, dt));
         v3 x = v3_add(v3_load(pos, i), v3_scale(v, dt));
         const int cx = x.x < box.lo[0] || x.x > box.hi[0];
         const int cy = x.y < box.lo[1] || x.y > box.hi[1];
@@ -622,6 +624,194 @@ void SUF(oracle_dfsph_step)(const oracle_domain* d, uint32_t nd, uint32_t ne, in
     free_vfield(&total);
     free_vfield(&xs);
     free_vfield(&vs);
+    grid_free(&g);
+}
+
+/* -------------------------------------------------------- PCISPH, IISPH */
+/* SPH<PCISPH|IISPH>::forward_step (sph.ixx:350-354) = build(step, x) ->
+ * density(x, x) -> PCISPH::forward (sph.cpp:296-315) or IISPH::forward
+ * (sph.cpp:409-428). The pressure updates pressure_update_forward_kernel
+ * (pcisph.cu:104-110, V = m / rho0) and jacobi_update_forward_kernel
+ * (iisph.cu:104-110, V = m / rho) share projection_update_kernel with the
+ * volume density passed in; both target rho0. */
+void SUF(oracle_pressure_solver_step)(const oracle_domain* d, int iisph, uint32_t iterations, uint64_t step,
+                                      const real* pos[3], const real* vel[3], const real* ctl[3], const real* masses,
+                                      const real* rest_densities, const real* viscosities, const real* relaxation,
+                                      real* next_pos[3], real* next_vel[3], real* densities_out,
+                                      real* non_pressure_out[3], real* pressures_out) {
+    const uint32_t n = d->particle_count;
+    const real dt = (real)d->time_step;
+    const real G = SUF(oracle_reference_gradient_norm)(d);
+    const cbox box = collision_box(d, step);
+    grid_t g;
+    grid_build(&g, d, step, pos[0], pos[1], pos[2]);
+    real* rho = alloc_real(n);
+    real* rho_star = alloc_real(n);
+    real* p = alloc_real(n);
+    vfield anp = alloc_vfield(n), ap = alloc_vfield(n), xs = alloc_vfield(n), vs = alloc_vfield(n);
+    density_kernel(&g, 0, pos, pos, masses, rest_densities, rho);
+    non_pressure_kernel(&g, pos, vel, ctl, masses, viscosities, rho, anp.c);
+    /* clear_iteration (sph.cpp:298): p = 0, a_p = 0 */
+    for (uint32_t i = 0; i < n; ++i) {
+        p[i] = K(0.0);
+        for (int c = 0; c < 3; ++c) ap.c[c][i] = K(0.0);
+    }
+    for (uint32_t it = 1; it <= iterations; ++it) {
+        sph_predict_kernel(n, dt, box, pos, vel, CV(anp), CV(ap), xs.c, vs.c);
+        density_kernel(&g, 0, pos, CV(xs), masses, rest_densities, rho_star);
+        projection_update_kernel(n, dt, G, masses, iisph ? rho : rest_densities, rest_densities, rho_star, relaxation,
+                                 p);
+        pressure_kernel(&g, pos, masses, rest_densities, rho, p, ap.c);
+    }
+    sph_predict_kernel(n, dt, box, pos, vel, CV(anp), CV(ap), next_pos, next_vel);
+    if (densities_out) memcpy(densities_out, rho, sizeof(real) * n);
+    if (pressures_out) memcpy(pressures_out, p, sizeof(real) * n);
+    if (non_pressure_out)
+        for (int c = 0; c < 3; ++c)
+            if (non_pressure_out[c]) memcpy(non_pressure_out[c], anp.c[c], sizeof(real) * n);
+    free(rho);
+    free(rho_star);
+    free(p);
+    free_vfield(&anp);
+    free_vfield(&ap);
+    free_vfield(&xs);
+    free_vfield(&vs);
+    grid_free(&g);
+}
+
+/* ---------------------------------------------------------------- WCSPH */
+/* eos_forward_kernel, wcsph.cu:92-100 (Tait equation of state) */
+static void eos_kernel(uint32_t n, const real* densities, const real* rest_densities, const real* speed,
+                       const real* exponent, real* pressures) {
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < (int64_t)n; ++i) {
+        const real rho0 = rest_densities[i];
+        const real c = speed[i];
+        const real gamma = exponent[i];
+        const real ratio = densities[i] / rho0;
+        pressures[i] = rho0 * c * c * (R_POW(ratio, gamma) - K(1.0)) / gamma;
+    }
+}
+
+/* viscosity_forward_kernel, wcsph.cu:136-174 (artificial viscosity on
+ * approaching pairs) */
+static void artificial_viscosity_kernel(const grid_t* g, const real* const pos[3], const real* const vel[3],
+                                        const real* masses, const real* rest_densities, const real* viscosities,
+                                        const real* speed, const real* densities, real* const out[3]) {
+    const real h = (real)g->d->support_radius;
+#pragma omp parallel for schedule(static)
+    for (int64_t ii = 0; ii < (int64_t)g->n; ++ii) {
+        const uint32_t i = (uint32_t)ii;
+        const v3 p = v3_load(pos, i);
+        const v3 v = v3_load(vel, i);
+        v3 acc = v3_make(0, 0, 0);
+        cell_range r[27];
+        const int nr = stencil(g, p, r);
+        for (int k = 0; k < nr; ++k) {
+            for (uint32_t s = r[k].first; s < r[k].last; ++s) {
+                const uint32_t j = g->idx[s];
+                if (j == i) continue;
+                const v3 d = v3_sub(p, v3_load(pos, j));
+                const v3 dv = v3_sub(v, v3_load(vel, j));
+                const real compression = v3_dot(dv, d);
+                if (compression >= K(0.0)) continue;
+                const real alpha = K(0.5) * (viscosities[i] + viscosities[j]);
+                const real c = K(0.5) * (speed[i] + speed[j]);
+                const real nu = K(2.0) * alpha * h * c / (densities[i] + densities[j]);
+                const real pi = -nu * compression / (v3_dot(d, d) + K(0.01) * h * h);
+                acc = v3_add(acc, v3_scale(cubic_gradient(d, h), -masses[j] * pi));
+            }
+            for (uint32_t s = r[k].bfirst; s < r[k].blast; ++s) {
+                const uint32_t b = g->bidx[s];
+                const v3 d = v3_sub(p, boundary_position(g, b));
+                const v3 dv = v3_sub(v, boundary_velocity(g, b));
+                const real compression = v3_dot(dv, d);
+                if (compression >= K(0.0)) continue;
+                const real nu = viscosities[i] * h * speed[i] / densities[i];
+                const real pi = -nu * compression / (v3_dot(d, d) + K(0.01) * h * h);
+                acc = v3_add(acc, v3_scale(cubic_gradient(d, h), -rest_densities[i] * boundary_volume(g, b) * pi));
+            }
+        }
+        v3_store(out, i, acc);
+    }
+}
+
+/* surface_forward_kernel, wcsph.cu:306-339 (cohesion; diameter = 2 r_p,
+ * launch_surface_forward wcsph.cu:466-468) */
+static void surface_kernel(const grid_t* g, const real* const pos[3], const real* masses, const real* rest_densities,
+                           const real* tensions, const real* boundary_tension, real* const out[3]) {
+    const real h = (real)g->d->support_radius;
+    const real diameter = K(2.0) * (real)g->d->particle_radius;
+#pragma omp parallel for schedule(static)
+    for (int64_t ii = 0; ii < (int64_t)g->n; ++ii) {
+        const uint32_t i = (uint32_t)ii;
+        const v3 p = v3_load(pos, i);
+        v3 acc = v3_make(0, 0, 0);
+        cell_range r[27];
+        const int nr = stencil(g, p, r);
+        for (int k = 0; k < nr; ++k) {
+            for (uint32_t s = r[k].first; s < r[k].last; ++s) {
+                const uint32_t j = g->idx[s];
+                if (j == i) continue;
+                const v3 d = v3_sub(p, v3_load(pos, j));
+                const real dist = v3_length(d);
+                if (dist >= h) continue;
+                const real w = dist > diameter ? cubic(d, h) : cubic(v3_make(diameter, K(0.0), K(0.0)), h);
+                const real factor = -tensions[i] * masses[j] * w / masses[i];
+                acc = v3_add(acc, v3_scale(d, factor));
+            }
+            for (uint32_t s = r[k].bfirst; s < r[k].blast; ++s) {
+                const uint32_t b = g->bidx[s];
+                const v3 d = v3_sub(p, boundary_position(g, b));
+                const real dist = v3_length(d);
+                if (dist >= h) continue;
+                const real w = dist > diameter ? cubic(d, h) : cubic(v3_make(diameter, K(0.0), K(0.0)), h);
+                const real factor = -boundary_tension[i] * rest_densities[i] * boundary_volume(g, b) * w / masses[i];
+                acc = v3_add(acc, v3_scale(d, factor));
+            }
+        }
+        v3_store(out, i, acc);
+    }
+}
+
+/* SPH<WCSPH>::forward_step (sph.ixx:350-354) = build(step, x) -> density(x, x)
+ * -> WCSPH::forward (sph.cpp:214-226): EOS, pressure, artificial viscosity,
+ * surface, a = ((a_p + a_visc) + a_surf) + (g + u), integrate. */
+void SUF(oracle_wcsph_step)(const oracle_domain* d, uint64_t step, const real* pos[3], const real* vel[3],
+                            const real* ctl[3], const real* masses, const real* rest_densities,
+                            const real* viscosities, const real* tensions, const real* speed, const real* exponent,
+                            const real* boundary_tension, real* next_pos[3], real* next_vel[3], real* densities_out,
+                            real* pressures_out) {
+    const uint32_t n = d->particle_count;
+    const real dt = (real)d->time_step;
+    const cbox box = collision_box(d, step);
+    grid_t g;
+    grid_build(&g, d, step, pos[0], pos[1], pos[2]);
+    real* rho = alloc_real(n);
+    real* p = alloc_real(n);
+    vfield ap = alloc_vfield(n), av = alloc_vfield(n), as = alloc_vfield(n), ext = alloc_vfield(n),
+           total = alloc_vfield(n);
+    density_kernel(&g, 0, pos, pos, masses, rest_densities, rho);
+    eos_kernel(n, rho, rest_densities, speed, exponent, p);
+    pressure_kernel(&g, pos, masses, rest_densities, rho, p, ap.c);
+    artificial_viscosity_kernel(&g, pos, vel, masses, rest_densities, viscosities, speed, rho, av.c);
+    surface_kernel(&g, pos, masses, rest_densities, tensions, boundary_tension, as.c);
+    add_kernel(n, CV(ap), CV(av), total.c);
+    add_kernel(n, CV(total), CV(as), total.c);
+    /* external_forward_kernel, wcsph.cu:86-90 */
+    const v3 gravity = v3_make((real)d->gravity[0], (real)d->gravity[1], (real)d->gravity[2]);
+    for (uint32_t i = 0; i < n; ++i) v3_store(ext.c, i, v3_add(gravity, v3_load(ctl, i)));
+    add_kernel(n, CV(total), CV(ext), total.c);
+    sph_predict_kernel(n, dt, box, pos, vel, CV(total), NULL, next_pos, next_vel);
+    if (densities_out) memcpy(densities_out, rho, sizeof(real) * n);
+    if (pressures_out) memcpy(pressures_out, p, sizeof(real) * n);
+    free(rho);
+    free(p);
+    free_vfield(&ap);
+    free_vfield(&av);
+    free_vfield(&as);
+    free_vfield(&ext);
+    free_vfield(&total);
     grid_free(&g);
 }
 

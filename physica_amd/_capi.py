@@ -139,6 +139,31 @@ class SphDfsphParameters(ctypes.Structure):
     _fields_ = [("particles", ParticleParameters), ("method", DfsphParameters)]
 
 
+PHX_SPH_WCSPH, PHX_SPH_PCISPH, PHX_SPH_IISPH = 0, 1, 2
+
+
+class SphConfiguration(ctypes.Structure):
+    _fields_ = [("method", c_int), ("pressure_iterations", c_uint32), ("checkpoint_interval", c_uint32)]
+
+
+class SphMethodParameters(ctypes.Structure):
+    _fields_ = [
+        ("speed_of_sound", c_void_p),
+        ("tait_exponent", c_void_p),
+        ("boundary_surface_tension", c_void_p),
+        ("pressure_relaxation", c_void_p),
+        ("jacobi_relaxation", c_void_p),
+    ]
+
+
+class SphState(ctypes.Structure):
+    _fields_ = [("particles", ParticleState)]
+
+
+class SphParameters(ctypes.Structure):
+    _fields_ = [("particles", ParticleParameters), ("method", SphMethodParameters)]
+
+
 class PbfConfiguration(ctypes.Structure):
     _fields_ = [("pressure_iterations", c_uint32), ("checkpoint_interval", c_uint32)]
 
@@ -193,6 +218,24 @@ PROTOTYPES = {
     "phx_dfsph_cache_neighborhood": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
     "phx_dfsph_cache_densities": (c_int, [c_void_p, c_void_p]),
     "phx_dfsph_cache_non_pressure_accelerations": (c_int, [c_void_p, VectorView]),
+    "phx_sph_create": (
+        c_int,
+        [POINTER(DomainConfiguration), POINTER(SphConfiguration), c_int, c_void_p, POINTER(c_void_p)],
+    ),
+    "phx_sph_destroy": (c_int, [c_void_p]),
+    "phx_sph_allocate_step_cache": (c_int, [c_void_p, POINTER(c_void_p)]),
+    "phx_sph_free_step_cache": (c_int, [c_void_p]),
+    "phx_sph_forward_step": (
+        c_int,
+        [c_void_p, POINTER(SphState), POINTER(Control), POINTER(SphParameters), POINTER(SphState), c_void_p],
+    ),
+    "phx_sph_reference_gradient_norm": (c_int, [c_void_p, POINTER(c_float)]),
+    "phx_sph_set_profiling": (c_int, [c_void_p, c_int]),
+    "phx_sph_read_profile": (c_int, [c_void_p, POINTER(PassProfile), c_uint32, POINTER(c_uint32)]),
+    "phx_sph_cache_neighborhood": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
+    "phx_sph_cache_densities": (c_int, [c_void_p, c_void_p]),
+    "phx_sph_cache_pressures": (c_int, [c_void_p, c_void_p]),
+    "phx_sph_cache_non_pressure_accelerations": (c_int, [c_void_p, VectorView]),
     "phx_pbf_create": (
         c_int,
         [POINTER(DomainConfiguration), POINTER(PbfConfiguration), c_int, c_void_p, POINTER(c_void_p)],

@@ -7,6 +7,7 @@
 
 #include "dfsph.hpp"
 #include "pbf.hpp"
+#include "sph.hpp"
 
 struct phx_dfsph {
     phx::DfsphSolver impl;
@@ -16,6 +17,16 @@ struct phx_dfsph {
 struct phx_dfsph_step_cache {
     const phx_dfsph* owner = nullptr;
     phx::DfsphCache impl;
+    bool valid = false;
+};
+struct phx_sph {
+    phx::SphSolver impl;
+    phx_sph(const phx_domain_configuration* d, const phx_sph_configuration* c, phx_execution_mode m, hipStream_t s)
+        : impl(d, c, m, s) {}
+};
+struct phx_sph_step_cache {
+    const phx_sph* owner = nullptr;
+    phx::SphCache impl;
     bool valid = false;
 };
 struct phx_pbf {
@@ -316,6 +327,140 @@ phx_status phx_dfsph_read_profile(phx_dfsph* solver, phx_pass_profile* profile, 
         }
         *count = k;
         p.reset();
+    });
+}
+
+// ---------------------------------------------------- WCSPH, PCISPH, IISPH
+phx_status phx_sph_create(const phx_domain_configuration* domain, const phx_sph_configuration* configuration,
+                          phx_execution_mode mode, phx_stream stream, phx_sph** solver) {
+    return guarded([&] {
+        phx::require(solver != nullptr, PHX_ERROR_INVALID_ARGUMENT, "null solver output");
+        *solver = nullptr;
+        *solver = new phx_sph(domain, configuration, mode, stream_of(stream));
+    });
+}
+
+phx_status phx_sph_destroy(phx_sph* solver) {
+    return guarded([&] { delete solver; });
+}
+
+phx_status phx_sph_allocate_step_cache(phx_sph* solver, phx_sph_step_cache** cache) {
+    return guarded([&] {
+        phx::require(solver != nullptr && cache != nullptr, PHX_ERROR_INVALID_ARGUMENT, "null argument");
+        *cache = nullptr;
+        auto* c = new phx_sph_step_cache();
+        c->owner = solver;
+        c->impl.stream = solver->impl.stream();
+        c->impl.method = solver->impl.method();
+        try {
+            c->impl.allocate(solver->impl.domain().config.particle_count, solver->impl.domain().cells);
+        } catch (...) {
+            delete c;
+            throw;
+        }
+        *cache = c;
+    });
+}
+
+phx_status phx_sph_free_step_cache(phx_sph_step_cache* cache) {
+    return guarded([&] { delete cache; });
+}
+
+phx_status phx_sph_forward_step(phx_sph* solver, const phx_sph_state* state, const phx_control* control,
+                                const phx_sph_parameters* parameters, phx_sph_state* next_state,
+                                phx_sph_step_cache* cache) {
+    return guarded([&] {
+        phx::require(solver && state && control && parameters && next_state && cache, PHX_ERROR_INVALID_ARGUMENT,
+                     "null argument");
+        phx::require(cache->owner == solver, PHX_ERROR_INVALID_ARGUMENT, "step cache belongs to another solver");
+        cache->valid = false;
+        solver->impl.forward_step(*state, *control, *parameters, *next_state, cache->impl);
+        cache->valid = true;
+    });
+}
+
+phx_status phx_sph_reference_gradient_norm(const phx_sph* solver, float* value) {
+    return guarded([&] {
+        phx::require(solver != nullptr && value != nullptr, PHX_ERROR_INVALID_ARGUMENT, "null argument");
+        *value = solver->impl.gradient_norm();
+    });
+}
+
+phx_status phx_sph_set_profiling(phx_sph* solver, int enabled) {
+    return guarded([&] {
+        phx::require(solver != nullptr, PHX_ERROR_INVALID_ARGUMENT, "null solver");
+        solver->impl.profiler().reset();
+        solver->impl.profiler().set_enabled(enabled != 0);
+    });
+}
+
+phx_status phx_sph_read_profile(phx_sph* solver, phx_pass_profile* profile, uint32_t capacity, uint32_t* count) {
+    return guarded([&] {
+        phx::require(solver != nullptr && count != nullptr, PHX_ERROR_INVALID_ARGUMENT, "null argument");
+        phx::PassProfiler& p = solver->impl.profiler();
+        p.collect();
+        const auto& d = solver->impl.domain();
+        uint32_t k = 0;
+        for (int pass = 0; pass < phx::kSphPassCount; ++pass) {
+            if (p.launches(pass) == 0) continue;
+            if (profile != nullptr && k < capacity) {
+                profile[k].name = phx::sph_pass_name(pass);
+                profile[k].total_ms = p.total_ms(pass);
+                profile[k].launches = p.launches(pass);
+                profile[k].bytes_per_particle =
+                    phx::sph_pass_bytes(solver->impl.method(), pass, d.config.particle_count, d.cells);
+            }
+            ++k;
+        }
+        *count = k;
+        p.reset();
+    });
+}
+
+phx_status phx_sph_cache_neighborhood(const phx_sph_step_cache* cache, uint64_t* sorted_keys,
+                                      uint32_t* sorted_particle_indices, uint32_t* cell_offsets) {
+    return guarded([&] {
+        phx::require(cache != nullptr && cache->valid, PHX_ERROR_INVALID_ARGUMENT, "cache holds no step");
+        phx::export_neighborhood(cache->impl.stream, cache->impl.neighborhood, sorted_keys, sorted_particle_indices,
+                                 cell_offsets);
+    });
+}
+
+phx_status phx_sph_cache_densities(const phx_sph_step_cache* cache, float* densities) {
+    return guarded([&] {
+        phx::require(cache != nullptr && cache->valid, PHX_ERROR_INVALID_ARGUMENT, "cache holds no step");
+        phx::require(densities != nullptr, PHX_ERROR_INVALID_ARGUMENT, "null output");
+        phx::ChannelList ch;
+        ch.add(cache->impl.densities.data(), densities);
+        phx::scatter_to_original(cache->impl.stream, cache->impl.neighborhood.perm.data(),
+                                 cache->impl.neighborhood.particles, ch.get());
+    });
+}
+
+phx_status phx_sph_cache_pressures(const phx_sph_step_cache* cache, float* pressures) {
+    return guarded([&] {
+        phx::require(cache != nullptr && cache->valid, PHX_ERROR_INVALID_ARGUMENT, "cache holds no step");
+        phx::require(pressures != nullptr, PHX_ERROR_INVALID_ARGUMENT, "null output");
+        phx::ChannelList ch;
+        ch.add(cache->impl.pressures.data(), pressures);
+        phx::scatter_to_original(cache->impl.stream, cache->impl.neighborhood.perm.data(),
+                                 cache->impl.neighborhood.particles, ch.get());
+    });
+}
+
+phx_status phx_sph_cache_non_pressure_accelerations(const phx_sph_step_cache* cache, phx_vector_view out) {
+    return guarded([&] {
+        phx::require(cache != nullptr && cache->valid, PHX_ERROR_INVALID_ARGUMENT, "cache holds no step");
+        phx::require(cache->impl.method != PHX_SPH_WCSPH, PHX_ERROR_INVALID_ARGUMENT,
+                     "WCSPH keeps no non-pressure accelerations");
+        phx::require_vec(out, "output");
+        phx::ChannelList ch;
+        const float* np = reinterpret_cast<const float*>(cache->impl.non_pressure.data());
+        ch.add(np + 0, out.x, 4u);
+        ch.add(np + 1, out.y, 4u);
+        ch.add(np + 2, out.z, 4u);
+        phx::scatter_to_original(cache->impl.stream, cache->impl.neighborhood.perm.data(),
+                                 cache->impl.neighborhood.particles, ch.get());
     });
 }
 

@@ -26,273 +26,9 @@
 #include <cstring>
 
 #include "dfsph.hpp"
-#include "neighbor_mask.hpp"
+#include "sph_kernels.hpp"
 
 namespace phx {
-namespace {
-
-__device__ __forceinline__ Float3 xyz(float4 q) { return Float3{q.x, q.y, q.z}; }
-
-constexpr uint32_t kBlock = 256;
-
-__device__ __forceinline__ uint32_t cell_of(const SortedGrid& g, int x, int y, int z) {
-    return (static_cast<uint32_t>(z) * g.cy + static_cast<uint32_t>(y)) * g.cx + static_cast<uint32_t>(x);
-}
-
-struct MaskDensityArgs {
-    uint32_t n;
-    KernelConstants kc;
-    float t_skin;
-    SortedGrid grid;
-    const float4* xm;
-    uint16_t* exact;
-    uint16_t* skin;
-    const float* p_div;
-    const float* p_den;
-    float* rho;
-    float* t_div;
-    float* t_den;
-    uint32_t* flags;
-};
-
-// Mask build, then density_forward_kernel (density.cu:18-40, cubic kernel,
-// topology = positions) over the exact mask and the first_term p/rho^2 of
-// pressure_forward_kernel (common.cu:171) for both warm-started pressures.
-__global__ void __launch_bounds__(kBlock) dfsph_mask_density_kernel(MaskDensityArgs a) {
-    __shared__ uint16_t smask[kStencil * kBlock];
-    __shared__ int delta[kStencil];
-    fill_slot_deltas(a.grid, delta);
-    __syncthreads();
-    const uint32_t i = xcd_block() * kBlock + threadIdx.x;
-    if (i >= a.n) return;
-    uint16_t* column = smask + threadIdx.x;
-    const Float3 p = xyz(a.xm[i]);
-    int cx, cy, cz;
-    particle_cell(a.grid, p, cx, cy, cz);
-    uint32_t nonzero = 0u;
-    const bool complete = build_masks(a.grid, a.xm, a.n, i, p, cx, cy, cz, a.kc.sqrt_support, a.t_skin, a.exact,
-                                      a.skin, column, kBlock, &nonzero);
-    float result = 0.0F;
-    if (complete) {
-        walk_masked(a.grid.offsets, cell_of(a.grid, cx, cy, cz), delta, column, kBlock, nonzero, [&](uint32_t s) {
-            const float4 q = a.xm[s];
-            const float dist = length(sub(p, xyz(q)));
-            result += q.w * cubic_value(dist, a.kc);
-        });
-    } else {
-        atomicOr(&a.flags[kFlagOverflow], 1u);
-        for_each_candidate(a.grid, cx, cy, cz, [&](uint32_t s) {
-            const float4 q = a.xm[s];
-            result += q.w * cubic_value(length(sub(p, xyz(q))), a.kc);
-        });
-    }
-    a.rho[i] = result;
-    a.t_div[i] = a.p_div[i] / (result * result);
-    a.t_den[i] = a.p_den[i] / (result * result);
-}
-
-struct ProjectionArgs {
-    uint32_t n;
-    KernelConstants kc;
-    float dt;
-    float gradient_norm;
-    float limit2;  // largest squared displacement for which the skin mask is exact
-    uint32_t slot;
-    SortedGrid grid;
-    const float4* xm;  // topology: state positions (+ masses)
-    const float4* xs;  // predicted positions (+ masses)
-    const uint16_t* skin;
-    const uint32_t* flags;
-    const float* rho;
-    const float* target;
-    const float* relaxation;
-    float* p;
-    float* t;
-};
-
-// density at predicted positions (density.cu:18-40 with topology = x,
-// positions = x*) fused with projection_update_forward_kernel
-// (dfsph.cu:104-110) and the next pressure pass's first_term.
-__global__ void __launch_bounds__(kBlock) dfsph_projection_kernel(ProjectionArgs a) {
-    __shared__ uint16_t smask[kStencil * kBlock];
-    __shared__ int delta[kStencil];
-    fill_slot_deltas(a.grid, delta);
-    __syncthreads();
-    const uint32_t i = xcd_block() * kBlock + threadIdx.x;
-    if (i >= a.n) return;
-    const bool masked = a.flags[kFlagOverflow] == 0u &&
-                        __uint_as_float(a.flags[kFlagDisplacement + a.slot]) <= a.limit2;
-    const float4 self = a.xs[i];
-    const Float3 ps = xyz(self);
-    int cx, cy, cz;
-    particle_cell(a.grid, xyz(a.xm[i]), cx, cy, cz);
-    float rs = 0.0F;
-    auto visit = [&](uint32_t s) {
-        const float4 q = a.xs[s];
-        rs += q.w * cubic_value(length(sub(ps, xyz(q))), a.kc);
-    };
-    if (masked) {
-        uint16_t* column = smask + threadIdx.x;
-        const uint32_t nonzero = load_lane_column(a.skin, a.n, i, column, kBlock);
-        walk_masked(a.grid.offsets, cell_of(a.grid, cx, cy, cz), delta, column, kBlock, nonzero, visit);
-    } else {
-        for_each_candidate(a.grid, cx, cy, cz, visit);
-    }
-    const float rho = a.rho[i];
-    const float volume = self.w / rho;
-    const float delta_p = 1.0F / (2.0F * a.dt * a.dt * volume * volume * a.gradient_norm);
-    const float pressure = fmaxf(0.0F, a.p[i] + a.relaxation[i] * delta_p * (rs - a.target[i]));
-    a.p[i] = pressure;
-    a.t[i] = pressure / (rho * rho);
-}
-
-enum Epilogue : int {
-    kDivPredict = 0,  // total = a_p(t1)
-    kDivEnd = 1,      // a_div = a_p(t1); total = a_div + a_p(t2)
-    kDenPredict = 2,  // total = a_div + a_p(t1)
-};
-
-struct ForceArgs {
-    uint32_t n;
-    KernelConstants kc;
-    float dt;
-    SortedGrid grid;
-    Box box;
-    Float3 gravity;
-    const float4* xm;
-    const float4* vmu;  // velocity (+ viscosity)
-    CVec u;
-    const float* rho;
-    const float* t1;
-    const float* t2;
-    const uint16_t* exact;
-    uint32_t* flags;
-    uint32_t slot;
-    float4* anp;
-    float4* adiv;
-    float4* xs;
-    uint32_t final_step;
-    const uint32_t* perm;
-    Vec next_x, next_v;
-    const float* p_div;
-    float* warm_div_out;
-    const float* p_den;
-    float* warm_den_out;
-};
-
-// Pressure acceleration (pressure_forward_kernel, common.cu:165-194),
-// optionally fused with the non-pressure acceleration
-// (non_pressure_forward_kernel, common.cu:14-47), then the predict of the
-// next iteration (dfsph.cu:26-44) or, on the last pass, the next state.
-// cubic_gradient(d, h) is evaluated as f * d with f its radial factor, the
-// same operations as the reference's expression.
-template <bool kNonPressure, int kEpilogue>
-__global__ void __launch_bounds__(kBlock) dfsph_force_kernel(ForceArgs a) {
-    constexpr bool kTwo = kEpilogue == kDivEnd;
-    __shared__ uint16_t smask[kStencil * kBlock];
-    __shared__ int delta[kStencil];
-    fill_slot_deltas(a.grid, delta);
-    __syncthreads();
-    const uint32_t i = xcd_block() * kBlock + threadIdx.x;
-    const bool active = i < a.n;
-    float moved2 = 0.0F;
-    if (active) {
-        const bool masked = a.flags[kFlagOverflow] == 0u;
-        const float4 self = a.xm[i];
-        const Float3 p = xyz(self);
-        const float4 vm = a.vmu[i];
-        const Float3 vel = xyz(vm);
-        Float3 acc{0.0F, 0.0F, 0.0F};
-        if constexpr (kNonPressure) acc = add(a.gravity, load(a.u, i));
-        const float t1_i = a.t1[i];
-        float t2_i = 0.0F;
-        if constexpr (kTwo) t2_i = a.t2[i];
-        Float3 ap1{0.0F, 0.0F, 0.0F};
-        Float3 ap2{0.0F, 0.0F, 0.0F};
-        int cx, cy, cz;
-        particle_cell(a.grid, p, cx, cy, cz);
-        auto pair = [&](uint32_t s, float4 q, Float3 d, float f) {
-            const float ms = q.w;
-            if constexpr (kNonPressure) {
-                const float4 vq = a.vmu[s];
-                const float viscosity = 0.5F * (vm.w + vq.w);
-                const float weight = viscosity * ms * viscosity_laplacian_c(d, a.kc) / a.rho[s];
-                acc = add(acc, scale(sub(xyz(vq), vel), weight));
-            }
-            const Float3 gradient = scale(d, f);
-            ap1 = add(ap1, scale(gradient, -ms * (t1_i + a.t1[s])));
-            if constexpr (kTwo) ap2 = add(ap2, scale(gradient, -ms * (t2_i + a.t2[s])));
-        };
-        if (masked) {
-            uint16_t* column = smask + threadIdx.x;
-            const uint32_t nonzero = load_lane_column(a.exact, a.n, i, column, kBlock);
-            walk_masked(a.grid.offsets, cell_of(a.grid, cx, cy, cz), delta, column, kBlock, nonzero,
-                        [&](uint32_t s) {
-                            if (s == i) return;
-                            const float4 q = a.xm[s];
-                            const Float3 d = sub(p, xyz(q));
-                            pair(s, q, d, cubic_gradient_factor(length(d), a.kc));
-                        });
-        } else {
-            for_each_candidate(a.grid, cx, cy, cz, [&](uint32_t s) {
-                if (s == i) return;
-                const float4 q = a.xm[s];
-                const Float3 d = sub(p, xyz(q));
-                pair(s, q, d, cubic_gradient_factor(length(d), a.kc));
-            });
-        }
-        if constexpr (kNonPressure) {
-            a.anp[i] = make_float4(acc.x, acc.y, acc.z, 0.0F);
-        } else {
-            acc = xyz(a.anp[i]);
-        }
-        Float3 total;
-        if constexpr (kEpilogue == kDivPredict) {
-            total = ap1;
-        } else if constexpr (kTwo) {
-            a.adiv[i] = make_float4(ap1.x, ap1.y, ap1.z, 0.0F);
-            total = add(ap1, ap2);
-        } else {
-            total = add(xyz(a.adiv[i]), ap1);
-        }
-        Float3 nx, nv;
-        predict_clamped(a.box, a.dt, p, vel, add(acc, total), nx, nv);
-        if constexpr (kTwo) {
-            if (a.warm_div_out != nullptr) a.warm_div_out[a.perm[i]] = a.p_div[i];
-        }
-        if (a.final_step != 0u) {
-            const uint32_t j = a.perm[i];
-            store(a.next_x, j, nx);
-            store(a.next_v, j, nv);
-            if (a.warm_den_out != nullptr) a.warm_den_out[j] = a.p_den[i];
-        } else {
-            a.xs[i] = make_float4(nx.x, nx.y, nx.z, self.w);
-            const Float3 moved = sub(nx, p);
-            moved2 = dot(moved, moved);
-        }
-    }
-    if (a.final_step == 0u) block_max_to<kBlock>(moved2, &a.flags[kFlagDisplacement + a.slot]);
-}
-
-template <bool kNonPressure, int kEpilogue>
-void launch_force(PassProfiler& profiler, hipStream_t stream, uint32_t blocks, const ForceArgs& args) {
-    const int pass = kNonPressure ? kDfsphForceInit
-                     : kEpilogue == kDivPredict ? kDfsphForceDivergence
-                     : kEpilogue == kDivEnd     ? kDfsphForceDivergenceEnd
-                                                : kDfsphForceDensity;
-    PassScope scope(profiler, stream, pass);
-    dfsph_force_kernel<kNonPressure, kEpilogue><<<blocks, kBlock, 0, stream>>>(args);
-    PHX_LAUNCH_CHECK();
-}
-
-void launch_projection(PassProfiler& profiler, hipStream_t stream, uint32_t blocks, const ProjectionArgs& args,
-                       int pass) {
-    PassScope scope(profiler, stream, pass);
-    dfsph_projection_kernel<<<blocks, kBlock, 0, stream>>>(args);
-    PHX_LAUNCH_CHECK();
-}
-
-}  // namespace
 
 float sqrt_support_threshold(float h) {
     float t = h * h;
@@ -369,37 +105,10 @@ DfsphSolver::DfsphSolver(const phx_domain_configuration* domain, const phx_dfsph
     for (DeviceBuffer<float4>* b : {&xm_, &vmu_, &xs_, &adiv_}) b->resize(n);
     u_.resize(n);
     for (DeviceBuffer<float>* b : {&rho0_, &relax_div_, &relax_den_, &p_div_, &p_den_, &t_div_, &t_den_}) b->resize(n);
-    mask_exact_.resize(static_cast<size_t>(kStencil) * n);
-    mask_skin_.resize(static_cast<size_t>(kStencil) * n);
-    flag_count_ = kFlagDisplacement + config_.divergence_iterations + config_.density_iterations + 1u;
-    flags_.resize(flag_count_);
-    flags_host_.resize(sizeof(uint32_t) * flag_count_);
-    PHX_HIP(hipEventCreateWithFlags(&readback_, hipEventDisableTiming));
+    masks_.allocate(n, config_.divergence_iterations + config_.density_iterations + 1u, domain_.config.support_radius);
 }
 
-DfsphSolver::~DfsphSolver() {
-    if (readback_ != nullptr) {
-        (void)hipEventSynchronize(readback_);
-        (void)hipEventDestroy(readback_);
-    }
-}
-
-// Picks the skin of the next step from the largest predicted displacement of
-// a previous step (read back without blocking): twice the displacement (both
-// particles of a pair move) with a 30 % margin.
-void DfsphSolver::adapt_skin() {
-    if (!readback_pending_ || hipEventQuery(readback_) != hipSuccess) return;
-    readback_pending_ = false;
-    const auto* flags = static_cast<const uint32_t*>(flags_host_.data());
-    float moved2 = 0.0F;
-    for (uint32_t k = kFlagDisplacement; k < flag_count_; ++k) {
-        float v;
-        std::memcpy(&v, &flags[k], sizeof(float));
-        moved2 = std::fmax(moved2, v);
-    }
-    const float moved = std::sqrt(moved2) / domain_.config.support_radius;
-    skin_ = std::fmin(0.6F, std::fmax(0.02F, 2.6F * moved + 0.01F));
-}
+DfsphSolver::~DfsphSolver() = default;
 
 void DfsphSolver::forward_step(const phx_sph_dfsph_state& state, const phx_control& control,
                                const phx_sph_dfsph_parameters& parameters, phx_sph_dfsph_state& next,
@@ -430,7 +139,6 @@ void DfsphSolver::forward_step(const phx_sph_dfsph_state& state, const phx_contr
     require(next.method.warm_divergence_pressure && next.method.warm_density_pressure, PHX_ERROR_INVALID_ARGUMENT,
             "null next warm-start pressures");
     cache.allocate(n, domain_.cells);
-    adapt_skin();
 
     // neighborhood.build(step, x) with every input gathered into cell order
     ChannelList gather;
@@ -453,37 +161,25 @@ void DfsphSolver::forward_step(const phx_sph_dfsph_state& state, const phx_contr
             fill_f32(stream_, p_div_.data(), n, 0.0F);
             fill_f32(stream_, p_den_.data(), n, 0.0F);
         }
-        fill_u32(stream_, flags_.data(), flag_count_, 0u);
+        masks_.begin_step(stream_);
     }
     const SortedGrid grid = cache.neighborhood.grid();
-    // whole blocks of kBlock particles, padded to a multiple of the XCD count
-    const uint32_t blocks = (blocks_for(n, kBlock) + 7u) / 8u * 8u;
-    const float h = c.support_radius;
-    // skin radius R = h (1 + skin): the mask keeps d2 < R^2 (with margin), and
-    // an A pass may use it while every particle moved at most (R - h) / 2
-    const float radius = h * (1.0F + skin_);
-    const float t_skin = radius * radius * (1.0F + 1e-5F);
-    const float limit = 0.5F * (radius - h * (1.0F + 2e-5F));
-    const float limit2 = limit > 0.0F ? limit * limit : -1.0F;
+    const uint32_t blocks = pass_blocks(n);
 
     MaskDensityArgs ma{n,
                        constants_,
-                       t_skin,
+                       masks_.t_skin(),
                        grid,
                        xm_.data(),
-                       mask_exact_.data(),
-                       mask_skin_.data(),
+                       masks_.exact(),
+                       masks_.skin_mask(),
                        p_div_.data(),
                        p_den_.data(),
                        cache.densities.data(),
                        t_div_.data(),
                        t_den_.data(),
-                       flags_.data()};
-    {
-        PassScope scope(profiler_, stream_, kDfsphDensity);
-        dfsph_mask_density_kernel<<<blocks, kBlock, 0, stream_>>>(ma);
-        PHX_LAUNCH_CHECK();
-    }
+                       masks_.flags()};
+    launch_mask_density<false>(profiler_, stream_, blocks, ma, kDfsphDensity);
 
     ForceArgs fa{};
     fa.n = n;
@@ -496,8 +192,8 @@ void DfsphSolver::forward_step(const phx_sph_dfsph_state& state, const phx_contr
     fa.vmu = vmu_.data();
     fa.u = u_.cview();
     fa.rho = cache.densities.data();
-    fa.exact = mask_exact_.data();
-    fa.flags = flags_.data();
+    fa.exact = masks_.exact();
+    fa.flags = masks_.flags();
     fa.anp = cache.non_pressure.data();
     fa.adiv = adiv_.data();
     fa.xs = xs_.data();
@@ -514,13 +210,14 @@ void DfsphSolver::forward_step(const phx_sph_dfsph_state& state, const phx_contr
     pa.kc = constants_;
     pa.dt = c.time_step;
     pa.gradient_norm = gradient_norm_;
-    pa.limit2 = limit2;
+    pa.limit2 = masks_.limit2();
     pa.grid = grid;
     pa.xm = xm_.data();
     pa.xs = xs_.data();
-    pa.skin = mask_skin_.data();
-    pa.flags = flags_.data();
+    pa.skin = masks_.skin_mask();
+    pa.flags = masks_.flags();
     pa.rho = cache.densities.data();
+    pa.volume_density = cache.densities.data();
 
     const uint32_t nd = config_.divergence_iterations;
     const uint32_t ne = config_.density_iterations;
@@ -532,10 +229,10 @@ void DfsphSolver::forward_step(const phx_sph_dfsph_state& state, const phx_contr
     fa.slot = slot;
     if (nd == 0u) {
         fa.final_step = ne == 0u ? 1u : 0u;
-        launch_force<true, kDivEnd>(profiler_, stream_, blocks, fa);
+        launch_force<true, kDivEnd>(profiler_, stream_, blocks, fa, kDfsphForceInit);
     } else {
         fa.final_step = 0u;
-        launch_force<true, kDivPredict>(profiler_, stream_, blocks, fa);
+        launch_force<true, kDivPredict>(profiler_, stream_, blocks, fa, kDfsphForceInit);
         pa.target = cache.densities.data();
         pa.relaxation = relax_div_.data();
         pa.p = p_div_.data();
@@ -545,10 +242,10 @@ void DfsphSolver::forward_step(const phx_sph_dfsph_state& state, const phx_contr
             launch_projection(profiler_, stream_, blocks, pa, kDfsphProjectionDivergence);
             fa.slot = ++slot;
             if (it < nd) {
-                launch_force<false, kDivPredict>(profiler_, stream_, blocks, fa);
+                launch_force<false, kDivPredict>(profiler_, stream_, blocks, fa, kDfsphForceDivergence);
             } else {
                 fa.final_step = ne == 0u ? 1u : 0u;
-                launch_force<false, kDivEnd>(profiler_, stream_, blocks, fa);
+                launch_force<false, kDivEnd>(profiler_, stream_, blocks, fa, kDfsphForceDivergenceEnd);
             }
         }
     }
@@ -565,13 +262,10 @@ void DfsphSolver::forward_step(const phx_sph_dfsph_state& state, const phx_contr
         launch_projection(profiler_, stream_, blocks, pa, kDfsphProjectionDensity);
         fa.slot = ++slot;
         fa.final_step = it == ne ? 1u : 0u;
-        launch_force<false, kDenPredict>(profiler_, stream_, blocks, fa);
+        launch_force<false, kDenPredict>(profiler_, stream_, blocks, fa, kDfsphForceDensity);
     }
     // read the step's displacements back for the next step's skin
-    PHX_HIP(hipMemcpyAsync(flags_host_.data(), flags_.data(), sizeof(uint32_t) * flag_count_, hipMemcpyDeviceToHost,
-                           stream_));
-    PHX_HIP(hipEventRecord(readback_, stream_));
-    readback_pending_ = true;
+    masks_.end_step(stream_);
     next.particles.step_index = step + 1u;
     last_iterations_[0] = nd;
     last_iterations_[1] = ne;

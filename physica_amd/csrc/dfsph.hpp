@@ -2,6 +2,7 @@
 #pragma once
 
 #include "engine.hpp"
+#include "masks.hpp"
 #include "profiler.hpp"
 
 namespace phx {
@@ -35,29 +36,6 @@ const char* dfsph_pass_name(int pass);
 // The neighbour masks are an acceleration structure and are not counted.
 double dfsph_pass_bytes(int pass, uint32_t particles, uint32_t cells, bool warm_start);
 
-// Pinned host buffer for asynchronous read-back.
-class PinnedBuffer {
-  public:
-    PinnedBuffer() = default;
-    PinnedBuffer(const PinnedBuffer&) = delete;
-    PinnedBuffer& operator=(const PinnedBuffer&) = delete;
-    ~PinnedBuffer() {
-        if (ptr_ != nullptr) (void)hipHostFree(ptr_);
-    }
-    void resize(size_t bytes) {
-        if (bytes <= bytes_) return;
-        if (ptr_ != nullptr) (void)hipHostFree(ptr_);
-        ptr_ = nullptr;
-        PHX_HIP(hipHostMalloc(&ptr_, bytes, hipHostMallocDefault));
-        bytes_ = bytes;
-    }
-    void* data() { return ptr_; }
-
-  private:
-    void* ptr_ = nullptr;
-    size_t bytes_ = 0;
-};
-
 class DfsphSolver {
   public:
     DfsphSolver(const phx_domain_configuration* domain, const phx_dfsph_configuration* configuration,
@@ -76,10 +54,9 @@ class DfsphSolver {
     PassProfiler& profiler() { return profiler_; }
     phx_execution_mode mode() const { return mode_; }
     bool warm_start() const { return config_.pressure_warm_start != 0u; }
-    float skin() const { return skin_; }
+    float skin() const { return masks_.skin(); }
 
   private:
-    void adapt_skin();
 
     Domain domain_;
     phx_dfsph_configuration config_{};
@@ -87,19 +64,13 @@ class DfsphSolver {
     hipStream_t stream_;
     float gradient_norm_ = 0.0F;
     KernelConstants constants_{};
-    float skin_ = 0.3F;  // skin of the predicted-position masks, in units of h
     uint32_t last_iterations_[2]{};
-    uint32_t flag_count_ = 0;
     PassProfiler profiler_;
     // sorted-order workspace
     DeviceBuffer<float4> xm_, vmu_, xs_, adiv_;
     VecBuffer u_;
     DeviceBuffer<float> rho0_, relax_div_, relax_den_, p_div_, p_den_, t_div_, t_den_;
-    DeviceBuffer<uint16_t> mask_exact_, mask_skin_;
-    DeviceBuffer<uint32_t> flags_;
-    PinnedBuffer flags_host_;
-    hipEvent_t readback_ = nullptr;
-    bool readback_pending_ = false;
+    MaskSet masks_;
 };
 
 // Constants of the cubic and viscosity kernels for support radius h.

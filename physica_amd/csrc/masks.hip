@@ -1,0 +1,67 @@
+// Skin adaptation and flag handling of the neighbour masks.
+#include "masks.hpp"
+
+#include <cmath>
+#include <cstring>
+
+#include "neighbor_mask.hpp"
+
+namespace phx {
+
+MaskSet::~MaskSet() {
+    if (readback_ != nullptr) {
+        (void)hipEventSynchronize(readback_);
+        (void)hipEventDestroy(readback_);
+    }
+}
+
+void MaskSet::allocate(uint32_t n, uint32_t predicting_passes, float support_radius) {
+    h_ = support_radius;
+    exact_.resize(static_cast<size_t>(kStencil) * n);
+    skin_mask_.resize(static_cast<size_t>(kStencil) * n);
+    flag_count_ = kFlagDisplacement + predicting_passes;
+    flags_.resize(flag_count_);
+    flags_host_.resize(sizeof(uint32_t) * flag_count_);
+    if (readback_ == nullptr) PHX_HIP(hipEventCreateWithFlags(&readback_, hipEventDisableTiming));
+}
+
+// Picks the skin from the largest predicted displacement of the step read
+// back last: twice the displacement (both particles of a pair move) with a
+// 30 % margin, in [0.02, 0.6] h.
+void MaskSet::begin_step(hipStream_t stream) {
+    if (readback_pending_ && hipEventQuery(readback_) == hipSuccess) {
+        readback_pending_ = false;
+        const auto* flags = static_cast<const uint32_t*>(flags_host_.data());
+        float moved2 = 0.0F;
+        for (uint32_t k = kFlagDisplacement; k < flag_count_; ++k) {
+            float v;
+            std::memcpy(&v, &flags[k], sizeof(float));
+            moved2 = std::fmax(moved2, v);
+        }
+        const float moved = std::sqrt(moved2) / h_;
+        skin_ = std::fmin(0.6F, std::fmax(0.02F, 2.6F * moved + 0.01F));
+    }
+    fill_u32(stream, flags_.data(), flag_count_, 0u);
+}
+
+void MaskSet::end_step(hipStream_t stream) {
+    PHX_HIP(hipMemcpyAsync(flags_host_.data(), flags_.data(), sizeof(uint32_t) * flag_count_, hipMemcpyDeviceToHost,
+                           stream));
+    PHX_HIP(hipEventRecord(readback_, stream));
+    readback_pending_ = true;
+}
+
+// R = h (1 + skin); the mask keeps d2 < R^2 (with a relative margin), and a
+// pass may use it while every particle moved at most (R - h) / 2.
+float MaskSet::t_skin() const {
+    const float radius = h_ * (1.0F + skin_);
+    return radius * radius * (1.0F + 1e-5F);
+}
+
+float MaskSet::limit2() const {
+    const float radius = h_ * (1.0F + skin_);
+    const float limit = 0.5F * (radius - h_ * (1.0F + 2e-5F));
+    return limit > 0.0F ? limit * limit : -1.0F;
+}
+
+}  // namespace phx

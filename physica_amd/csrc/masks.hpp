@@ -1,0 +1,69 @@
+// Host side of a solver's neighbour masks: the exact and skin mask buffers,
+// the device flags of a step, and the skin radius, adapted from the predicted
+// displacements of a previous step read back without blocking.
+#pragma once
+
+#include "engine.hpp"
+
+namespace phx {
+
+// Pinned host buffer for asynchronous read-back.
+class PinnedBuffer {
+  public:
+    PinnedBuffer() = default;
+    PinnedBuffer(const PinnedBuffer&) = delete;
+    PinnedBuffer& operator=(const PinnedBuffer&) = delete;
+    ~PinnedBuffer() {
+        if (ptr_ != nullptr) (void)hipHostFree(ptr_);
+    }
+    void resize(size_t bytes) {
+        if (bytes <= bytes_) return;
+        if (ptr_ != nullptr) (void)hipHostFree(ptr_);
+        ptr_ = nullptr;
+        PHX_HIP(hipHostMalloc(&ptr_, bytes, hipHostMallocDefault));
+        bytes_ = bytes;
+    }
+    void* data() { return ptr_; }
+
+  private:
+    void* ptr_ = nullptr;
+    size_t bytes_ = 0;
+};
+
+class MaskSet {
+  public:
+    MaskSet() = default;
+    MaskSet(const MaskSet&) = delete;
+    MaskSet& operator=(const MaskSet&) = delete;
+    ~MaskSet();
+
+    // Masks of n particles and flags for `predicting_passes` displacement
+    // slots (one per pass that writes predicted positions).
+    void allocate(uint32_t n, uint32_t predicting_passes, float support_radius);
+    // Adopts the skin of a completed read-back, if any, and clears the flags.
+    void begin_step(hipStream_t stream);
+    // Enqueues the read-back of this step's displacement flags.
+    void end_step(hipStream_t stream);
+
+    // The skin mask keeps d2 < t_skin; a pass may use it while every particle
+    // moved at most sqrt(limit2) (negative: never).
+    float t_skin() const;
+    float limit2() const;
+    float skin() const { return skin_; }
+
+    uint16_t* exact() { return exact_.data(); }
+    uint16_t* skin_mask() { return skin_mask_.data(); }
+    uint32_t* flags() { return flags_.data(); }
+
+  private:
+    float h_ = 0.0F;
+    float skin_ = 0.3F;  // in units of h
+    uint32_t flag_count_ = 0;
+    DeviceBuffer<uint16_t> exact_, skin_mask_;
+    DeviceBuffer<uint32_t> flags_;
+    PinnedBuffer flags_host_;
+    hipEvent_t readback_ = nullptr;
+    bool readback_pending_ = false;
+};
+
+}  // namespace phx

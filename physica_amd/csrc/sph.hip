@@ -1,0 +1,362 @@
+// SPH<WCSPH>, SPH<PCISPH> and SPH<IISPH> forward steps for gfx950 in sorted
+// (cell-major) particle order.
+//
+// Reference: SPH<Method>::forward_step (sph/sph.ixx:350-354), WCSPH::forward
+// (sph/sph.cpp:214-226, kernels sph/wcsph.cu), PCISPH::forward
+// (sph/sph.cpp:296-315, sph/pcisph.cu) and IISPH::forward (sph/sph.cpp:409-428,
+// sph/iisph.cu). Each step reorders the particles by cell once and fuses every
+// per-particle chain into the neighbour pass that produces its input:
+//
+//   WCSPH          masks + density + Tait pressure -> rho, p, p / rho^2
+//                  pressure + artificial viscosity + surface + external +
+//                  integrate -> next state
+//   PCISPH/IISPH   masks + density; non-pressure + first predict -> x*
+//                  per iteration  A: rho* at x* (skin mask) + pressure update
+//                                 B: pressure acceleration + predict -> x*
+//                                    (the last one writes the next state)
+//
+// PCISPH and IISPH differ only in the volume of the pressure update
+// (V = m / rho0 and V = m / rho). Sums run in the reference order over the
+// neighbour masks, so results equal the reference expression by expression
+// (the Tait pressure goes through the platform's powf).
+#include <cmath>
+
+#include "sph.hpp"
+#include "sph_kernels.hpp"
+
+namespace phx {
+namespace {
+
+struct WcsphForceArgs {
+    uint32_t n;
+    KernelConstants kc;
+    float dt;
+    float diameter;
+    SortedGrid grid;
+    Box box;
+    Float3 gravity;
+    const float4* xm;   // position + mass
+    const float4* vmu;  // velocity + viscosity
+    const float4* aux;  // density, p / rho^2, speed of sound, surface tension
+    CVec u;
+    const uint16_t* exact;
+    const uint32_t* flags;
+    const uint32_t* perm;
+    Vec next_x, next_v;
+};
+
+// pressure_forward_kernel (common.cu:165-194), viscosity_forward_kernel
+// (wcsph.cu:136-174), surface_forward_kernel (wcsph.cu:306-339) as three
+// accumulators over one neighbour walk, then the sums of WCSPH::forward
+// (sph.cpp:219-222), external_forward_kernel (wcsph.cu:86-90) and
+// integrate_forward_kernel (common.cu:302-320).
+__global__ void __launch_bounds__(kBlock) wcsph_force_kernel(WcsphForceArgs a) {
+    __shared__ uint16_t smask[kStencil * kBlock];
+    __shared__ int delta[kStencil];
+    fill_slot_deltas(a.grid, delta);
+    __syncthreads();
+    const uint32_t i = xcd_block() * kBlock + threadIdx.x;
+    if (i >= a.n) return;
+    const float h = a.kc.h;
+    const float4 self = a.xm[i];
+    const Float3 p = xyz(self);
+    const float4 vm = a.vmu[i];
+    const Float3 vel = xyz(vm);
+    const float4 own = a.aux[i];
+    const float rho_i = own.x;
+    const float t_i = own.y;
+    const float c_i = own.z;
+    const float sigma_i = own.w;
+    const float softening = 0.01F * h * h;
+    const float contact_weight = cubic_value(length(Float3{a.diameter, 0.0F, 0.0F}), a.kc);
+    Float3 ap{0.0F, 0.0F, 0.0F};
+    Float3 av{0.0F, 0.0F, 0.0F};
+    Float3 as{0.0F, 0.0F, 0.0F};
+    auto visit = [&](uint32_t s) {
+        if (s == i) return;
+        const float4 q = a.xm[s];
+        const float ms = q.w;
+        const Float3 d = sub(p, xyz(q));
+        const float dist = length(d);
+        const Float3 gradient = scale(d, cubic_gradient_factor(dist, a.kc));
+        const float4 other = a.aux[s];
+        ap = add(ap, scale(gradient, -ms * (t_i + other.y)));
+        const float4 vq = a.vmu[s];
+        const float compression = dot(sub(vel, xyz(vq)), d);
+        if (!(compression >= 0.0F)) {
+            const float alpha = 0.5F * (vm.w + vq.w);
+            const float speed = 0.5F * (c_i + other.z);
+            const float nu = 2.0F * alpha * h * speed / (rho_i + other.x);
+            const float pi_value = -nu * compression / (dot(d, d) + softening);
+            av = add(av, scale(gradient, -ms * pi_value));
+        }
+        if (!(dist >= h)) {
+            const float weight = dist > a.diameter ? cubic_value(dist, a.kc) : contact_weight;
+            const float factor = -sigma_i * ms * weight / self.w;
+            as = add(as, scale(d, factor));
+        }
+    };
+    int cx, cy, cz;
+    particle_cell(a.grid, p, cx, cy, cz);
+    if (a.flags[kFlagOverflow] == 0u) {
+        uint16_t* column = smask + threadIdx.x;
+        const uint32_t nonzero = load_lane_column(a.exact, a.n, i, column, kBlock);
+        walk_masked(a.grid.offsets, cell_of(a.grid, cx, cy, cz), delta, column, kBlock, nonzero, visit);
+    } else {
+        for_each_candidate(a.grid, cx, cy, cz, visit);
+    }
+    Float3 total = add(ap, av);
+    total = add(total, as);
+    total = add(total, add(a.gravity, load(a.u, i)));
+    Float3 nx, nv;
+    predict_clamped(a.box, a.dt, p, vel, total, nx, nv);
+    const uint32_t j = a.perm[i];
+    store(a.next_x, j, nx);
+    store(a.next_v, j, nv);
+}
+
+}  // namespace
+
+const char* sph_pass_name(int pass) {
+    static const char* names[kSphPassCount] = {"grid", "mask_density", "force_init", "projection", "force",
+                                               "forces"};
+    return pass >= 0 && pass < kSphPassCount ? names[pass] : "unknown";
+}
+
+double sph_pass_bytes(phx_sph_method method, int pass, uint32_t particles, uint32_t cells) {
+    const double offsets = particles > 0 ? 4.0 * (static_cast<double>(cells) + 1.0) / particles : 0.0;
+    const bool wcsph = method == PHX_SPH_WCSPH;
+    switch (pass) {
+        case kSphGrid:  // x, v, u (36) + m, rho0, mu and the method's per-particle parameters, read and written
+                        // in cell order, + the permutation (4)
+            return 2.0 * (36.0 + (wcsph ? 24.0 : 16.0)) + 4.0 + 2.0 * offsets;
+        case kSphDensity:  // x, m in; rho out (WCSPH: + rho0, c, gamma, sigma in; p, p / rho^2 out)
+            return wcsph ? 32.0 + 12.0 + offsets : 16.0 + 4.0 + offsets;
+        case kSphForceInit:  // x, v, u, m, mu, rho in; a_np, x* out
+            return 48.0 + 24.0 + offsets;
+        case kSphProjection:  // x, x*, m, rho, rho0, relaxation, p in; p, p / rho^2 out
+            return 44.0 + 8.0 + offsets;
+        case kSphForce:  // x, v, m, p / rho^2, a_np in; x* out
+            return 44.0 + 12.0 + offsets;
+        case kSphForces:  // x, v, u, m, mu, rho, p / rho^2, c, sigma in; next x, v out
+            return 60.0 + 24.0 + offsets;
+        default:
+            return 0.0;
+    }
+}
+
+void SphCache::allocate(uint32_t n, uint32_t cells) {
+    neighborhood.allocate(n, cells);
+    densities.resize(n);
+    pressures.resize(n);
+    if (method != PHX_SPH_WCSPH) non_pressure.resize(n);
+}
+
+SphSolver::SphSolver(const phx_domain_configuration* domain, const phx_sph_configuration* configuration,
+                     phx_execution_mode mode, hipStream_t stream)
+    : domain_(make_domain(domain)), mode_(mode), stream_(stream) {
+    require(configuration != nullptr, PHX_ERROR_INVALID_ARGUMENT, "null SPH configuration");
+    config_ = *configuration;
+    require(config_.method == PHX_SPH_WCSPH || config_.method == PHX_SPH_PCISPH || config_.method == PHX_SPH_IISPH,
+            PHX_ERROR_INVALID_ARGUMENT, "unknown SPH method");
+    require(mode == PHX_MODE_FORWARD, PHX_ERROR_UNSUPPORTED, "differentiable mode is not supported yet");
+    if (config_.method != PHX_SPH_WCSPH) {
+        require(config_.checkpoint_interval > 0u, PHX_ERROR_INVALID_ARGUMENT, "checkpoint_interval must be positive");
+        require(config_.pressure_iterations < 4096u, PHX_ERROR_INVALID_ARGUMENT, "too many pressure iterations");
+    } else {
+        config_.pressure_iterations = 0u;
+    }
+    gradient_norm_ = reference_gradient_norm(domain_.config);
+    constants_ = kernel_constants(domain_.config.support_radius);
+    const uint32_t n = domain_.config.particle_count;
+    for (DeviceBuffer<float4>* b : {&xm_, &vmu_, &xs_, &aux_}) b->resize(n);
+    u_.resize(n);
+    for (DeviceBuffer<float>* b : {&rho0_, &relax_, &tait_, &t_}) b->resize(n);
+    masks_.allocate(n, config_.pressure_iterations + 1u, domain_.config.support_radius);
+}
+
+void SphSolver::forward_step(const phx_sph_state& state, const phx_control& control,
+                             const phx_sph_parameters& parameters, phx_sph_state& next, SphCache& cache) {
+    const uint32_t n = domain_.config.particle_count;
+    if (n == 0) {
+        next.particles.step_index = state.particles.step_index + 1u;
+        return;
+    }
+    require_vec(state.particles.positions, "state positions");
+    require_vec(state.particles.velocities, "state velocities");
+    require_vec(control.external_accelerations, "control");
+    require_vec(next.particles.positions, "next positions");
+    require_vec(next.particles.velocities, "next velocities");
+    const auto& pp = parameters.particles;
+    require(pp.masses && pp.rest_densities && pp.viscosities, PHX_ERROR_INVALID_ARGUMENT, "null particle parameters");
+    cache.method = config_.method;
+    cache.allocate(n, domain_.cells);
+    if (config_.method == PHX_SPH_WCSPH) {
+        forward_wcsph(state, control, parameters, next, cache);
+    } else {
+        forward_pressure(state, control, parameters, next, cache);
+    }
+    next.particles.step_index = state.particles.step_index + 1u;
+}
+
+void SphSolver::forward_wcsph(const phx_sph_state& state, const phx_control& control,
+                              const phx_sph_parameters& parameters, phx_sph_state& next, SphCache& cache) {
+    const auto& c = domain_.config;
+    const uint32_t n = c.particle_count;
+    const uint64_t step = state.particles.step_index;
+    const auto& pp = parameters.particles;
+    const auto& mp = parameters.method;
+    require(pp.surface_tensions && mp.speed_of_sound && mp.tait_exponent && mp.boundary_surface_tension,
+            PHX_ERROR_INVALID_ARGUMENT, "null WCSPH parameters");
+    ChannelList gather;
+    gather.add(cvec(state.particles.positions), xm_.data());
+    gather.add(pp.masses, xm_.data(), 3);
+    gather.add(cvec(state.particles.velocities), vmu_.data());
+    gather.add(pp.viscosities, vmu_.data(), 3);
+    gather.add(cvec(control.external_accelerations), u_.view());
+    gather.add(pp.rest_densities, rho0_.data());
+    gather.add(mp.speed_of_sound, aux_.data(), 2);
+    gather.add(pp.surface_tensions, aux_.data(), 3);
+    gather.add(mp.tait_exponent, tait_.data());
+    {
+        PassScope scope(profiler_, stream_, kSphGrid);
+        cache.neighborhood.build(stream_, domain_, step, cvec(state.particles.positions), nullptr, gather.get());
+        masks_.begin_step(stream_);
+    }
+    const SortedGrid grid = cache.neighborhood.grid();
+    const uint32_t blocks = pass_blocks(n);
+
+    MaskDensityArgs ma{};
+    ma.n = n;
+    ma.kc = constants_;
+    ma.t_skin = masks_.t_skin();
+    ma.grid = grid;
+    ma.xm = xm_.data();
+    ma.exact = masks_.exact();
+    ma.skin = masks_.skin_mask();
+    ma.rho = cache.densities.data();
+    ma.flags = masks_.flags();
+    ma.rest_density = rho0_.data();
+    ma.tait_exponent = tait_.data();
+    ma.pressures = cache.pressures.data();
+    ma.aux = aux_.data();
+    launch_mask_density<true>(profiler_, stream_, blocks, ma, kSphDensity);
+
+    WcsphForceArgs wa{};
+    wa.n = n;
+    wa.kc = constants_;
+    wa.dt = c.time_step;
+    wa.diameter = 2.0F * c.particle_radius;
+    wa.grid = grid;
+    wa.box = collision_box(c, step);
+    wa.gravity = Float3{c.gravity.x, c.gravity.y, c.gravity.z};
+    wa.xm = xm_.data();
+    wa.vmu = vmu_.data();
+    wa.aux = aux_.data();
+    wa.u = u_.cview();
+    wa.exact = masks_.exact();
+    wa.flags = masks_.flags();
+    wa.perm = cache.neighborhood.perm.data();
+    wa.next_x = vec(next.particles.positions);
+    wa.next_v = vec(next.particles.velocities);
+    {
+        PassScope scope(profiler_, stream_, kSphForces);
+        wcsph_force_kernel<<<blocks, kBlock, 0, stream_>>>(wa);
+        PHX_LAUNCH_CHECK();
+    }
+    masks_.end_step(stream_);
+}
+
+void SphSolver::forward_pressure(const phx_sph_state& state, const phx_control& control,
+                                 const phx_sph_parameters& parameters, phx_sph_state& next, SphCache& cache) {
+    const auto& c = domain_.config;
+    const uint32_t n = c.particle_count;
+    const uint64_t step = state.particles.step_index;
+    const auto& pp = parameters.particles;
+    const bool iisph = config_.method == PHX_SPH_IISPH;
+    const float* relaxation = iisph ? parameters.method.jacobi_relaxation : parameters.method.pressure_relaxation;
+    require(relaxation != nullptr, PHX_ERROR_INVALID_ARGUMENT,
+            iisph ? "null IISPH jacobi_relaxation" : "null PCISPH pressure_relaxation");
+    ChannelList gather;
+    gather.add(cvec(state.particles.positions), xm_.data());
+    gather.add(pp.masses, xm_.data(), 3);
+    gather.add(cvec(state.particles.velocities), vmu_.data());
+    gather.add(pp.viscosities, vmu_.data(), 3);
+    gather.add(cvec(control.external_accelerations), u_.view());
+    gather.add(pp.rest_densities, rho0_.data());
+    gather.add(relaxation, relax_.data());
+    {
+        PassScope scope(profiler_, stream_, kSphGrid);
+        cache.neighborhood.build(stream_, domain_, step, cvec(state.particles.positions), nullptr, gather.get());
+        masks_.begin_step(stream_);
+        fill_f32(stream_, cache.pressures.data(), n, 0.0F);  // clear_iteration (sph.cpp:298)
+    }
+    const SortedGrid grid = cache.neighborhood.grid();
+    const uint32_t blocks = pass_blocks(n);
+
+    MaskDensityArgs ma{};
+    ma.n = n;
+    ma.kc = constants_;
+    ma.t_skin = masks_.t_skin();
+    ma.grid = grid;
+    ma.xm = xm_.data();
+    ma.exact = masks_.exact();
+    ma.skin = masks_.skin_mask();
+    ma.rho = cache.densities.data();
+    ma.flags = masks_.flags();
+    launch_mask_density<false>(profiler_, stream_, blocks, ma, kSphDensity);
+
+    ForceArgs fa{};
+    fa.n = n;
+    fa.kc = constants_;
+    fa.dt = c.time_step;
+    fa.grid = grid;
+    fa.box = collision_box(c, step);
+    fa.gravity = Float3{c.gravity.x, c.gravity.y, c.gravity.z};
+    fa.xm = xm_.data();
+    fa.vmu = vmu_.data();
+    fa.u = u_.cview();
+    fa.rho = cache.densities.data();
+    fa.t1 = t_.data();
+    fa.exact = masks_.exact();
+    fa.flags = masks_.flags();
+    fa.anp = cache.non_pressure.data();
+    fa.xs = xs_.data();
+    fa.perm = cache.neighborhood.perm.data();
+    fa.next_x = vec(next.particles.positions);
+    fa.next_v = vec(next.particles.velocities);
+
+    ProjectionArgs pa{};
+    pa.n = n;
+    pa.kc = constants_;
+    pa.dt = c.time_step;
+    pa.gradient_norm = gradient_norm_;
+    pa.limit2 = masks_.limit2();
+    pa.grid = grid;
+    pa.xm = xm_.data();
+    pa.xs = xs_.data();
+    pa.skin = masks_.skin_mask();
+    pa.flags = masks_.flags();
+    pa.rho = cache.densities.data();
+    pa.volume_density = iisph ? cache.densities.data() : rho0_.data();
+    pa.target = rho0_.data();
+    pa.relaxation = relax_.data();
+    pa.p = cache.pressures.data();
+    pa.t = t_.data();
+
+    const uint32_t iterations = config_.pressure_iterations;
+    uint32_t slot = 0;  // displacement slot written by the latest predicting pass
+    fa.slot = slot;
+    fa.final_step = iterations == 0u ? 1u : 0u;
+    launch_force<true, kInitPredict>(profiler_, stream_, blocks, fa, kSphForceInit);
+    for (uint32_t it = 1; it <= iterations; ++it) {  // sph.cpp:300-307 / 413-420
+        pa.slot = slot;
+        launch_projection(profiler_, stream_, blocks, pa, kSphProjection);
+        fa.slot = ++slot;
+        fa.final_step = it == iterations ? 1u : 0u;
+        launch_force<false, kDivPredict>(profiler_, stream_, blocks, fa, kSphForce);
+    }
+    masks_.end_step(stream_);
+}
+
+}  // namespace phx

@@ -1,0 +1,76 @@
+// SPH<WCSPH>, SPH<PCISPH>, SPH<IISPH> (sph/sph.ixx:39-193, 273-372) on the
+// sorted-order engine.
+#pragma once
+
+#include "dfsph.hpp"
+#include "engine.hpp"
+#include "masks.hpp"
+#include "profiler.hpp"
+
+namespace phx {
+
+// StepCache of SPH<Method> (sph/sph.ixx:299-303) with the method caches used
+// here: densities, pressures (WCSPH::Cache::pressures, or the last iteration
+// of PCISPH/IISPH) and the non-pressure accelerations (PCISPH/IISPH), all in
+// sorted order.
+struct SphCache {
+    hipStream_t stream = nullptr;
+    phx_sph_method method = PHX_SPH_WCSPH;
+    Neighborhood neighborhood;
+    DeviceBuffer<float> densities;
+    DeviceBuffer<float> pressures;
+    DeviceBuffer<float4> non_pressure;  // xyz used
+    void allocate(uint32_t n, uint32_t cells);
+};
+
+enum SphPass : int {
+    kSphGrid = 0,
+    kSphDensity,     // masks + density (+ Tait pressure for WCSPH)
+    kSphForceInit,   // PCISPH/IISPH: non-pressure + first predict
+    kSphProjection,  // PCISPH/IISPH: density at x* + pressure update
+    kSphForce,       // PCISPH/IISPH: pressure acceleration + predict
+    kSphForces,      // WCSPH: pressure, artificial viscosity, surface, external + integrate
+    kSphPassCount
+};
+const char* sph_pass_name(int pass);
+// Algorithmic bytes per particle per launch (unique per-particle inputs and
+// outputs once, plus the cell offsets).
+double sph_pass_bytes(phx_sph_method method, int pass, uint32_t particles, uint32_t cells);
+
+class SphSolver {
+  public:
+    SphSolver(const phx_domain_configuration* domain, const phx_sph_configuration* configuration,
+              phx_execution_mode mode, hipStream_t stream);
+    SphSolver(const SphSolver&) = delete;
+    SphSolver& operator=(const SphSolver&) = delete;
+
+    void forward_step(const phx_sph_state& state, const phx_control& control, const phx_sph_parameters& parameters,
+                      phx_sph_state& next, SphCache& cache);
+
+    const Domain& domain() const { return domain_; }
+    hipStream_t stream() const { return stream_; }
+    phx_sph_method method() const { return config_.method; }
+    float gradient_norm() const { return gradient_norm_; }
+    PassProfiler& profiler() { return profiler_; }
+
+  private:
+    void forward_wcsph(const phx_sph_state& state, const phx_control& control, const phx_sph_parameters& parameters,
+                       phx_sph_state& next, SphCache& cache);
+    void forward_pressure(const phx_sph_state& state, const phx_control& control,
+                          const phx_sph_parameters& parameters, phx_sph_state& next, SphCache& cache);
+
+    Domain domain_;
+    phx_sph_configuration config_{};
+    phx_execution_mode mode_;
+    hipStream_t stream_;
+    float gradient_norm_ = 0.0F;
+    KernelConstants constants_{};
+    PassProfiler profiler_;
+    MaskSet masks_;
+    // sorted-order workspace
+    DeviceBuffer<float4> xm_, vmu_, xs_, aux_;
+    VecBuffer u_;
+    DeviceBuffer<float> rho0_, relax_, tait_, t_;
+};
+
+}  // namespace phx

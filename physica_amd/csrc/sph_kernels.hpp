@@ -1,0 +1,317 @@
+// Neighbour-pass kernels shared by the SPH solvers (DFSPH, PCISPH, IISPH,
+// WCSPH) in sorted (cell-major) particle order.
+//
+// Every per-particle sum runs in the reference order (27 cells, offset_x
+// outer, ascending original index) over the precomputed neighbour masks, and
+// every masked-out candidate contributes an exact zero in the reference, so
+// results equal the reference expressions operation by operation. The kernels
+// live in an unnamed namespace: each solver translation unit instantiates the
+// ones it launches.
+#pragma once
+
+#include "engine.hpp"
+#include "neighbor_mask.hpp"
+#include "profiler.hpp"
+
+namespace phx {
+namespace {
+
+__device__ __forceinline__ Float3 xyz(float4 q) { return Float3{q.x, q.y, q.z}; }
+
+constexpr uint32_t kBlock = 256;
+
+__device__ __forceinline__ uint32_t cell_of(const SortedGrid& g, int x, int y, int z) {
+    return (static_cast<uint32_t>(z) * g.cy + static_cast<uint32_t>(y)) * g.cx + static_cast<uint32_t>(x);
+}
+
+struct MaskDensityArgs {
+    uint32_t n;
+    KernelConstants kc;
+    float t_skin;
+    SortedGrid grid;
+    const float4* xm;
+    uint16_t* exact;
+    uint16_t* skin;
+    const float* p_div;  // optional: t_div = p_div / rho^2
+    const float* p_den;  // optional: t_den = p_den / rho^2
+    float* rho;
+    float* t_div;
+    float* t_den;
+    uint32_t* flags;
+    // equation of state (WCSPH): p = EOS(rho), aux = (rho, p / rho^2, c, sigma)
+    const float* rest_density;
+    const float* tait_exponent;
+    float* pressures;
+    float4* aux;  // z (speed of sound) and w (surface tension) are inputs
+};
+
+// Mask build, then density_forward_kernel (density.cu:18-40, cubic kernel,
+// topology = positions) over the exact mask and the first_term p/rho^2 of
+// pressure_forward_kernel (common.cu:171) for the given pressures, or with
+// kEos the Tait pressure of eos_forward_kernel (wcsph.cu:92-100) and its
+// first_term.
+template <bool kEos>
+__global__ void __launch_bounds__(kBlock) mask_density_kernel(MaskDensityArgs a) {
+    __shared__ uint16_t smask[kStencil * kBlock];
+    __shared__ int delta[kStencil];
+    fill_slot_deltas(a.grid, delta);
+    __syncthreads();
+    const uint32_t i = xcd_block() * kBlock + threadIdx.x;
+    if (i >= a.n) return;
+    uint16_t* column = smask + threadIdx.x;
+    const Float3 p = xyz(a.xm[i]);
+    int cx, cy, cz;
+    particle_cell(a.grid, p, cx, cy, cz);
+    uint32_t nonzero = 0u;
+    const bool complete = build_masks(a.grid, a.xm, a.n, i, p, cx, cy, cz, a.kc.sqrt_support, a.t_skin, a.exact,
+                                      a.skin, column, kBlock, &nonzero);
+    float result = 0.0F;
+    if (complete) {
+        walk_masked(a.grid.offsets, cell_of(a.grid, cx, cy, cz), delta, column, kBlock, nonzero, [&](uint32_t s) {
+            const float4 q = a.xm[s];
+            const float dist = length(sub(p, xyz(q)));
+            result += q.w * cubic_value(dist, a.kc);
+        });
+    } else {
+        atomicOr(&a.flags[kFlagOverflow], 1u);
+        for_each_candidate(a.grid, cx, cy, cz, [&](uint32_t s) {
+            const float4 q = a.xm[s];
+            result += q.w * cubic_value(length(sub(p, xyz(q))), a.kc);
+        });
+    }
+    a.rho[i] = result;
+    if constexpr (kEos) {
+        const float4 aux = a.aux[i];
+        const float rest_density = a.rest_density[i];
+        const float speed = aux.z;
+        const float exponent = a.tait_exponent[i];
+        const float ratio = result / rest_density;
+        const float pressure = rest_density * speed * speed * (powf(ratio, exponent) - 1.0F) / exponent;
+        a.pressures[i] = pressure;
+        a.aux[i] = make_float4(result, pressure / (result * result), speed, aux.w);
+    } else {
+        if (a.p_div != nullptr) a.t_div[i] = a.p_div[i] / (result * result);
+        if (a.p_den != nullptr) a.t_den[i] = a.p_den[i] / (result * result);
+    }
+}
+
+struct ProjectionArgs {
+    uint32_t n;
+    KernelConstants kc;
+    float dt;
+    float gradient_norm;
+    float limit2;  // largest squared displacement for which the skin mask is exact
+    uint32_t slot;
+    SortedGrid grid;
+    const float4* xm;  // topology: state positions (+ masses)
+    const float4* xs;  // predicted positions (+ masses)
+    const uint16_t* skin;
+    const uint32_t* flags;
+    const float* rho;
+    const float* volume_density;  // rho (DFSPH, IISPH) or rho0 (PCISPH) in V = m / density
+    const float* target;
+    const float* relaxation;
+    float* p;
+    float* t;
+};
+
+// density at predicted positions (density.cu:18-40 with topology = x,
+// positions = x*) fused with the pressure update and the next pressure
+// pass's first_term: projection_update_forward_kernel (dfsph.cu:104-110),
+// pressure_update_forward_kernel (pcisph.cu:104-110) or
+// jacobi_update_forward_kernel (iisph.cu:104-110), which differ only in the
+// density of the volume V = m / density.
+__global__ void __launch_bounds__(kBlock) projection_kernel(ProjectionArgs a) {
+    __shared__ uint16_t smask[kStencil * kBlock];
+    __shared__ int delta[kStencil];
+    fill_slot_deltas(a.grid, delta);
+    __syncthreads();
+    const uint32_t i = xcd_block() * kBlock + threadIdx.x;
+    if (i >= a.n) return;
+    const bool masked = a.flags[kFlagOverflow] == 0u &&
+                        __uint_as_float(a.flags[kFlagDisplacement + a.slot]) <= a.limit2;
+    const float4 self = a.xs[i];
+    const Float3 ps = xyz(self);
+    int cx, cy, cz;
+    particle_cell(a.grid, xyz(a.xm[i]), cx, cy, cz);
+    float rs = 0.0F;
+    auto visit = [&](uint32_t s) {
+        const float4 q = a.xs[s];
+        rs += q.w * cubic_value(length(sub(ps, xyz(q))), a.kc);
+    };
+    if (masked) {
+        uint16_t* column = smask + threadIdx.x;
+        const uint32_t nonzero = load_lane_column(a.skin, a.n, i, column, kBlock);
+        walk_masked(a.grid.offsets, cell_of(a.grid, cx, cy, cz), delta, column, kBlock, nonzero, visit);
+    } else {
+        for_each_candidate(a.grid, cx, cy, cz, visit);
+    }
+    const float rho = a.rho[i];
+    const float volume = self.w / a.volume_density[i];
+    const float delta_p = 1.0F / (2.0F * a.dt * a.dt * volume * volume * a.gradient_norm);
+    const float pressure = fmaxf(0.0F, a.p[i] + a.relaxation[i] * delta_p * (rs - a.target[i]));
+    a.p[i] = pressure;
+    a.t[i] = pressure / (rho * rho);
+}
+
+enum Epilogue : int {
+    kDivPredict = 0,   // total = a_p(t1)
+    kDivEnd = 1,       // a_div = a_p(t1); total = a_div + a_p(t2)
+    kDenPredict = 2,   // total = a_div + a_p(t1)
+    kInitPredict = 3,  // total = 0 (cleared pressure acceleration), no pressure pass
+};
+
+struct ForceArgs {
+    uint32_t n;
+    KernelConstants kc;
+    float dt;
+    SortedGrid grid;
+    Box box;
+    Float3 gravity;
+    const float4* xm;
+    const float4* vmu;  // velocity (+ viscosity)
+    CVec u;
+    const float* rho;
+    const float* t1;
+    const float* t2;
+    const uint16_t* exact;
+    uint32_t* flags;
+    uint32_t slot;
+    float4* anp;
+    float4* adiv;
+    float4* xs;
+    uint32_t final_step;
+    const uint32_t* perm;
+    Vec next_x, next_v;
+    const float* p_div;
+    float* warm_div_out;
+    const float* p_den;
+    float* warm_den_out;
+};
+
+// Pressure acceleration (pressure_forward_kernel, common.cu:165-194),
+// optionally fused with the non-pressure acceleration
+// (non_pressure_forward_kernel, common.cu:14-47), then the predict of the
+// next iteration (dfsph.cu:26-44) or, on the last pass, the next state.
+// cubic_gradient(d, h) is evaluated as f * d with f its radial factor, the
+// same operations as the reference's expression.
+template <bool kNonPressure, int kEpilogue>
+__global__ void __launch_bounds__(kBlock) force_kernel(ForceArgs a) {
+    constexpr bool kTwo = kEpilogue == kDivEnd;
+    constexpr bool kPressure = kEpilogue != kInitPredict;
+    __shared__ uint16_t smask[kStencil * kBlock];
+    __shared__ int delta[kStencil];
+    fill_slot_deltas(a.grid, delta);
+    __syncthreads();
+    const uint32_t i = xcd_block() * kBlock + threadIdx.x;
+    const bool active = i < a.n;
+    float moved2 = 0.0F;
+    if (active) {
+        const bool masked = a.flags[kFlagOverflow] == 0u;
+        const float4 self = a.xm[i];
+        const Float3 p = xyz(self);
+        const float4 vm = a.vmu[i];
+        const Float3 vel = xyz(vm);
+        Float3 acc{0.0F, 0.0F, 0.0F};
+        if constexpr (kNonPressure) acc = add(a.gravity, load(a.u, i));
+        const float t1_i = kPressure ? a.t1[i] : 0.0F;
+        float t2_i = 0.0F;
+        if constexpr (kTwo) t2_i = a.t2[i];
+        Float3 ap1{0.0F, 0.0F, 0.0F};
+        Float3 ap2{0.0F, 0.0F, 0.0F};
+        int cx, cy, cz;
+        particle_cell(a.grid, p, cx, cy, cz);
+        auto pair = [&](uint32_t s, float4 q, Float3 d, float f) {
+            const float ms = q.w;
+            if constexpr (kNonPressure) {
+                const float4 vq = a.vmu[s];
+                const float viscosity = 0.5F * (vm.w + vq.w);
+                const float weight = viscosity * ms * viscosity_laplacian_c(d, a.kc) / a.rho[s];
+                acc = add(acc, scale(sub(xyz(vq), vel), weight));
+            }
+            if constexpr (kPressure) {
+                const Float3 gradient = scale(d, f);
+                ap1 = add(ap1, scale(gradient, -ms * (t1_i + a.t1[s])));
+                if constexpr (kTwo) ap2 = add(ap2, scale(gradient, -ms * (t2_i + a.t2[s])));
+            }
+        };
+        if (masked) {
+            uint16_t* column = smask + threadIdx.x;
+            const uint32_t nonzero = load_lane_column(a.exact, a.n, i, column, kBlock);
+            walk_masked(a.grid.offsets, cell_of(a.grid, cx, cy, cz), delta, column, kBlock, nonzero,
+                        [&](uint32_t s) {
+                            if (s == i) return;
+                            const float4 q = a.xm[s];
+                            const Float3 d = sub(p, xyz(q));
+                            pair(s, q, d, kPressure ? cubic_gradient_factor(length(d), a.kc) : 0.0F);
+                        });
+        } else {
+            for_each_candidate(a.grid, cx, cy, cz, [&](uint32_t s) {
+                if (s == i) return;
+                const float4 q = a.xm[s];
+                const Float3 d = sub(p, xyz(q));
+                pair(s, q, d, kPressure ? cubic_gradient_factor(length(d), a.kc) : 0.0F);
+            });
+        }
+        if constexpr (kNonPressure) {
+            a.anp[i] = make_float4(acc.x, acc.y, acc.z, 0.0F);
+        } else {
+            acc = xyz(a.anp[i]);
+        }
+        Float3 total;
+        if constexpr (kEpilogue == kInitPredict) {
+            total = Float3{0.0F, 0.0F, 0.0F};
+        } else if constexpr (kEpilogue == kDivPredict) {
+            total = ap1;
+        } else if constexpr (kTwo) {
+            a.adiv[i] = make_float4(ap1.x, ap1.y, ap1.z, 0.0F);
+            total = add(ap1, ap2);
+        } else {
+            total = add(xyz(a.adiv[i]), ap1);
+        }
+        Float3 nx, nv;
+        predict_clamped(a.box, a.dt, p, vel, add(acc, total), nx, nv);
+        if constexpr (kTwo) {
+            if (a.warm_div_out != nullptr) a.warm_div_out[a.perm[i]] = a.p_div[i];
+        }
+        if (a.final_step != 0u) {
+            const uint32_t j = a.perm[i];
+            store(a.next_x, j, nx);
+            store(a.next_v, j, nv);
+            if (a.warm_den_out != nullptr) a.warm_den_out[j] = a.p_den[i];
+        } else {
+            a.xs[i] = make_float4(nx.x, nx.y, nx.z, self.w);
+            const Float3 moved = sub(nx, p);
+            moved2 = dot(moved, moved);
+        }
+    }
+    if (a.final_step == 0u) block_max_to<kBlock>(moved2, &a.flags[kFlagDisplacement + a.slot]);
+}
+
+template <bool kNonPressure, int kEpilogue>
+void launch_force(PassProfiler& profiler, hipStream_t stream, uint32_t blocks, const ForceArgs& args, int pass) {
+    PassScope scope(profiler, stream, pass);
+    force_kernel<kNonPressure, kEpilogue><<<blocks, kBlock, 0, stream>>>(args);
+    PHX_LAUNCH_CHECK();
+}
+
+void launch_projection(PassProfiler& profiler, hipStream_t stream, uint32_t blocks, const ProjectionArgs& args,
+                       int pass) {
+    PassScope scope(profiler, stream, pass);
+    projection_kernel<<<blocks, kBlock, 0, stream>>>(args);
+    PHX_LAUNCH_CHECK();
+}
+
+template <bool kEos>
+void launch_mask_density(PassProfiler& profiler, hipStream_t stream, uint32_t blocks, const MaskDensityArgs& args,
+                         int pass) {
+    PassScope scope(profiler, stream, pass);
+    mask_density_kernel<kEos><<<blocks, kBlock, 0, stream>>>(args);
+    PHX_LAUNCH_CHECK();
+}
+
+// Whole blocks of kBlock particles, padded to a multiple of the XCD count.
+inline uint32_t pass_blocks(uint32_t n) { return (blocks_for(n, kBlock) + 7u) / 8u * 8u; }
+
+}  // namespace
+}  // namespace phx

@@ -247,37 +247,120 @@ class DFSPH:
             return _capi.DfsphParameters(_ptr(self.divergence_relaxation), _ptr(self.density_relaxation))
 
 
+# ------------------------------------------------------ WCSPH, PCISPH, IISPH
+class WCSPH:
+    """WCSPH method types, sph/sph.ixx:39-101."""
+
+    @dataclass
+    class Configuration:
+        def to_c(self) -> _capi.SphConfiguration:
+            return _capi.SphConfiguration(_capi.PHX_SPH_WCSPH, 0, 2)
+
+    @dataclass
+    class State:
+        pass
+
+    @dataclass
+    class Parameters:
+        speed_of_sound: torch.Tensor
+        tait_exponent: torch.Tensor
+        boundary_surface_tension: torch.Tensor
+
+        def to_c(self) -> _capi.SphMethodParameters:
+            return _capi.SphMethodParameters(
+                _ptr(self.speed_of_sound), _ptr(self.tait_exponent), _ptr(self.boundary_surface_tension), None, None
+            )
+
+
+class PCISPH:
+    """PCISPH method types, sph/sph.ixx:103-147."""
+
+    @dataclass
+    class Configuration:
+        pressure_iterations: int = 6
+        checkpoint_interval: int = 2
+
+        def to_c(self) -> _capi.SphConfiguration:
+            return _capi.SphConfiguration(
+                _capi.PHX_SPH_PCISPH, int(self.pressure_iterations), int(self.checkpoint_interval)
+            )
+
+    @dataclass
+    class State:
+        pass
+
+    @dataclass
+    class Parameters:
+        pressure_relaxation: torch.Tensor
+
+        def to_c(self) -> _capi.SphMethodParameters:
+            return _capi.SphMethodParameters(None, None, None, _ptr(self.pressure_relaxation), None)
+
+
+class IISPH:
+    """IISPH method types, sph/sph.ixx:149-193."""
+
+    @dataclass
+    class Configuration:
+        pressure_iterations: int = 6
+        checkpoint_interval: int = 2
+
+        def to_c(self) -> _capi.SphConfiguration:
+            return _capi.SphConfiguration(_capi.PHX_SPH_IISPH, int(self.pressure_iterations), int(self.checkpoint_interval))
+
+    @dataclass
+    class State:
+        pass
+
+    @dataclass
+    class Parameters:
+        jacobi_relaxation: torch.Tensor
+
+        def to_c(self) -> _capi.SphMethodParameters:
+            return _capi.SphMethodParameters(None, None, None, None, _ptr(self.jacobi_relaxation))
+
+
+_METHODS = {DFSPH.Configuration: DFSPH, WCSPH.Configuration: WCSPH, PCISPH.Configuration: PCISPH,
+            IISPH.Configuration: IISPH}
+
+
 class SPH:
-    """SPH<DFSPH> (sph/sph.ixx:273-372) over the HIP library."""
+    """SPH<Method> (sph/sph.ixx:273-372) over the HIP library; the method is
+    chosen by the type of the method configuration (DFSPH by default)."""
 
     @dataclass
     class State:
         particles: ParticleState
-        method: DFSPH.State
+        method: object
 
-        def to_c(self) -> _capi.SphDfsphState:
-            return _capi.SphDfsphState(self.particles.to_c(), self.method.to_c())
+        def to_c(self):
+            if isinstance(self.method, DFSPH.State):
+                return _capi.SphDfsphState(self.particles.to_c(), self.method.to_c())
+            return _capi.SphState(self.particles.to_c())
 
     @dataclass
     class Parameters:
         particles: ParticleParameters
-        method: DFSPH.Parameters
+        method: object
 
-        def to_c(self) -> _capi.SphDfsphParameters:
-            return _capi.SphDfsphParameters(self.particles.to_c(), self.method.to_c())
+        def to_c(self):
+            if isinstance(self.method, DFSPH.Parameters):
+                return _capi.SphDfsphParameters(self.particles.to_c(), self.method.to_c())
+            return _capi.SphParameters(self.particles.to_c(), self.method.to_c())
 
     class StepCache:
         """StepCache (sph/sph.ixx:299-303); fields are read back in original order."""
 
         def __init__(self, solver: "SPH"):
             self._solver = solver
+            self._prefix = solver._prefix
             handle = ctypes.c_void_p()
-            _capi.call("phx_dfsph_allocate_step_cache", solver._handle, ctypes.byref(handle))
+            _capi.call(self._prefix + "_allocate_step_cache", solver._handle, ctypes.byref(handle))
             self._handle = handle
 
         def __del__(self):
             if getattr(self, "_handle", None) and self._handle.value:
-                _capi.library().phx_dfsph_free_step_cache(self._handle)
+                getattr(_capi.library(), self._prefix + "_free_step_cache")(self._handle)
                 self._handle = ctypes.c_void_p()
 
         def neighborhood(self):
@@ -287,33 +370,45 @@ class SPH:
             keys = torch.empty(n, dtype=torch.int64, device=dev)
             idx = torch.empty(n, dtype=torch.int32, device=dev)
             off = torch.empty(cells + 1, dtype=torch.int32, device=dev)
-            _capi.call("phx_dfsph_cache_neighborhood", self._handle, _ptr(keys), _ptr(idx), _ptr(off))
+            _capi.call(self._prefix + "_cache_neighborhood", self._handle, _ptr(keys), _ptr(idx), _ptr(off))
             return keys, idx, off
 
         def densities(self) -> torch.Tensor:
             out = self._solver.domain.allocate_scalar_field()
-            _capi.call("phx_dfsph_cache_densities", self._handle, _ptr(out))
+            _capi.call(self._prefix + "_cache_densities", self._handle, _ptr(out))
+            return out
+
+        def pressures(self) -> torch.Tensor:
+            """WCSPH::Cache::pressures, or the last PCISPH/IISPH iteration's pressures."""
+            if self._prefix != "phx_sph":
+                raise AttributeError("the DFSPH step cache keeps its pressures in the method state")
+            out = self._solver.domain.allocate_scalar_field()
+            _capi.call("phx_sph_cache_pressures", self._handle, _ptr(out))
             return out
 
         def non_pressure_accelerations(self) -> VectorField:
             out = self._solver.domain.allocate_vector_field()
-            _capi.call("phx_dfsph_cache_non_pressure_accelerations", self._handle, out.view())
+            _capi.call(self._prefix + "_cache_non_pressure_accelerations", self._handle, out.view())
             return out
 
     def __init__(
         self,
         domain_configuration: DomainConfiguration,
-        method_configuration: Optional[DFSPH.Configuration] = None,
+        method_configuration=None,
         mode: ExecutionMode = ExecutionMode.forward,
         device="cuda",
     ):
         self.domain = Domain(domain_configuration, device)
-        self.method_configuration = method_configuration or DFSPH.Configuration()
+        self.method_configuration = method_configuration if method_configuration is not None else DFSPH.Configuration()
+        self.method = _METHODS.get(type(self.method_configuration))
+        if self.method is None:
+            raise TypeError(f"unknown SPH method configuration {type(self.method_configuration).__name__}")
+        self._prefix = "phx_dfsph" if self.method is DFSPH else "phx_sph"
         _require_gpu(self.domain.device)
         handle = ctypes.c_void_p()
         self._domain_c = domain_configuration.to_c()
         _capi.call(
-            "phx_dfsph_create",
+            self._prefix + "_create",
             ctypes.byref(self._domain_c),
             ctypes.byref(self.method_configuration.to_c()),
             int(mode),
@@ -324,35 +419,46 @@ class SPH:
 
     def __del__(self):
         if getattr(self, "_handle", None) and self._handle.value:
-            _capi.library().phx_dfsph_destroy(self._handle)
+            getattr(_capi.library(), self._prefix + "_destroy")(self._handle)
             self._handle = ctypes.c_void_p()
 
     def allocate_state(self) -> "SPH.State":
         d = self.domain
-        return SPH.State(d.allocate_particle_state(), DFSPH.State(d.allocate_scalar_field(), d.allocate_scalar_field()))
+        if self.method is DFSPH:
+            method = DFSPH.State(d.allocate_scalar_field(), d.allocate_scalar_field())
+        else:
+            method = self.method.State()
+        return SPH.State(d.allocate_particle_state(), method)
 
     def allocate_control(self) -> Control:
         return self.domain.allocate_control()
 
     def allocate_parameters(self) -> "SPH.Parameters":
         d = self.domain
-        return SPH.Parameters(
-            d.allocate_particle_parameters(), DFSPH.Parameters(d.allocate_scalar_field(), d.allocate_scalar_field())
-        )
+        if self.method is DFSPH:
+            method = DFSPH.Parameters(d.allocate_scalar_field(), d.allocate_scalar_field())
+        elif self.method is WCSPH:
+            method = WCSPH.Parameters(d.allocate_scalar_field(), d.allocate_scalar_field(), d.allocate_scalar_field())
+        elif self.method is PCISPH:
+            method = PCISPH.Parameters(d.allocate_scalar_field())
+        else:
+            method = IISPH.Parameters(d.allocate_scalar_field())
+        return SPH.Parameters(d.allocate_particle_parameters(), method)
 
     def allocate_step_cache(self) -> "SPH.StepCache":
         return SPH.StepCache(self)
 
     def copy_state(self, source: "SPH.State", destination: "SPH.State") -> None:
         self.domain.copy(source.particles, destination.particles)
-        destination.method.warm_divergence_pressure.copy_(source.method.warm_divergence_pressure)
-        destination.method.warm_density_pressure.copy_(source.method.warm_density_pressure)
+        if self.method is DFSPH:
+            destination.method.warm_divergence_pressure.copy_(source.method.warm_divergence_pressure)
+            destination.method.warm_density_pressure.copy_(source.method.warm_density_pressure)
 
     def forward_step(self, state, control, parameters, next_state, cache) -> None:
         s = state.to_c()
         n = next_state.to_c()
         _capi.call(
-            "phx_dfsph_forward_step",
+            self._prefix + "_forward_step",
             self._handle,
             ctypes.byref(s),
             ctypes.byref(control.to_c()),
@@ -365,18 +471,21 @@ class SPH:
     @property
     def reference_gradient_norm(self) -> float:
         v = ctypes.c_float()
-        _capi.call("phx_dfsph_reference_gradient_norm", self._handle, ctypes.byref(v))
+        _capi.call(self._prefix + "_reference_gradient_norm", self._handle, ctypes.byref(v))
         return float(v.value)
 
     def set_profiling(self, enabled: bool) -> None:
         """Bracket every pass with HIP events (per-pass timing)."""
-        _capi.call("phx_dfsph_set_profiling", self._handle, 1 if enabled else 0)
+        _capi.call(self._prefix + "_set_profiling", self._handle, 1 if enabled else 0)
 
     def read_profile(self) -> list:
         """Per-pass totals since profiling was enabled or last read."""
-        return _read_profile("phx_dfsph_read_profile", self._handle)
+        return _read_profile(self._prefix + "_read_profile", self._handle)
 
     def last_iterations(self) -> Tuple[int, int]:
+        """Iterations of the last DFSPH step (divergence, density)."""
+        if self.method is not DFSPH:
+            return (int(getattr(self.method_configuration, "pressure_iterations", 0)), 0)
         v = (ctypes.c_uint32 * 2)()
         _capi.call("phx_dfsph_last_iterations", self._handle, v)
         return int(v[0]), int(v[1])

@@ -156,3 +156,58 @@ def pbf_dam_break(shape=(32, 32, 32), spacing=0.05, seed=20260001, box=(3.2, 2.0
         "vorticity_confinement": np.zeros(n, dtype=np.float32),
     }
     return Scene("pbf_dam_break", domain, pos, np.zeros_like(pos), params, {"pressure_iterations": 4})
+
+
+def two_cubes(method="iisph", shape=(128, 128, 128), spacing=0.01, seed=20260003, box=(4.0, 4.0, 2.0),
+              jitter=None) -> Scene:
+    """Two falling cubes of shape[0] x shape[1] x shape[2] particles (spacing
+    0.01, jitter U(+-0.001)) into an empty basin, for IISPH, PCISPH and WCSPH.
+
+    The cubes' lower corners are at x in U[0.2, 0.6], z in U[0.2, 0.5]
+    (seeded, seed + 100) and y = 1.2 and 2.6. The basin is 4.0 x 4.0 x 2.0 m:
+    the upper cube reaches y = 3.88, so a 3 m tall basin would clip it.
+    h = 2 spacing, r_p = spacing / 2, rest density 1000, viscosity 0.
+    IISPH / PCISPH: dt = 1e-3, relaxation 0.5. WCSPH: dt = 2e-4, Tait exponent
+    7, speed of sound sqrt(7) m/s (rho0 c^2 / gamma = 1000), no surface
+    tension."""
+    s = float(spacing)
+    h = 2.0 * s
+    if jitter is None:
+        jitter = 0.1 * s
+    offsets = uniform(seed + 100, 4)
+    corners = [
+        (0.2 + 0.4 * float(offsets[0]), 1.2, 0.2 + 0.3 * float(offsets[1])),
+        (0.2 + 0.4 * float(offsets[2]), 2.6, 0.2 + 0.3 * float(offsets[3])),
+    ]
+    blocks = []
+    n_block = shape[0] * shape[1] * shape[2]
+    for k, corner in enumerate(corners):
+        blocks.append(lattice_block(shape, s, tuple(c + 0.5 * s for c in corner), jitter, seed + k * 7919))
+    pos = np.concatenate(blocks, axis=1)
+    n = pos.shape[1]
+    assert n == 2 * n_block
+    wcsph = method == "wcsph"
+    domain = DomainConfiguration(
+        particle_count=n,
+        time_step=2.0e-4 if wcsph else 1.0e-3,
+        support_radius=h,
+        particle_radius=0.5 * s,
+        gravity=(0.0, -9.81, 0.0),
+        boundary=BoxBoundary((0.0, 0.0, 0.0), tuple(map(float, box)), (0.0, 0.0, 0.0), True),
+    )
+    rho0 = np.float32(1000.0)
+    params = {
+        "masses": np.full(n, rho0 * np.float32(s) ** 3, dtype=np.float32),
+        "rest_densities": np.full(n, rho0, dtype=np.float32),
+        "viscosities": np.zeros(n, dtype=np.float32),
+        "surface_tensions": np.zeros(n, dtype=np.float32),
+    }
+    if wcsph:
+        params["speed_of_sound"] = np.full(n, np.sqrt(7.0), dtype=np.float32)
+        params["tait_exponent"] = np.full(n, 7.0, dtype=np.float32)
+        params["boundary_surface_tension"] = np.zeros(n, dtype=np.float32)
+        solver = {}
+    else:
+        params["relaxation"] = np.full(n, 0.5, dtype=np.float32)
+        solver = {"pressure_iterations": 6}
+    return Scene(f"two_cubes_{method}", domain, pos, np.zeros_like(pos), params, solver)
