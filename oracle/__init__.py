@@ -308,3 +308,54 @@ def pbf_step(domain, step_index, positions, velocities, controls, params: Dict[s
         *(c_void_p(arrs[k].ctypes.data) for k in names), nxp, nvp, c_void_p(rho.ctypes.data),
         c_void_p(lam.ctypes.data), predp)
     return {"positions": nx, "velocities": nv, "densities": rho, "lambdas": lam, "predicted_positions": pred}
+
+
+PBF_PARAMETERS = ("masses", "rest_densities", "relaxation", "artificial_pressure_strength",
+                  "artificial_pressure_exponent", "artificial_pressure_radius", "xsph_viscosity",
+                  "vorticity_confinement")
+
+
+def pbf_jvp(domain, step_index, positions, velocities, controls, params: Dict[str, np.ndarray],
+            tangents: Dict[str, np.ndarray], pressure_iterations=4, precision="f32"):
+    """PBF::jvp_step: tangents of the next positions and velocities. `tangents`
+    holds "positions", "velocities", "controls" (3, N) and any of
+    PBF_PARAMETERS (missing ones are zero)."""
+    n = domain.particle_count
+    dt = _dtype(precision)
+    pos, pp = _vec(positions, precision)
+    vel, vp = _vec(velocities, precision)
+    ctl, cp = _vec(controls, precision)
+    arrs = {k: _arr(params[k], precision) for k in PBF_PARAMETERS}
+    zero3 = np.zeros((3, n))
+    tv = [_vec(tangents.get(k, zero3), precision) for k in ("positions", "velocities", "controls")]
+    ts = {k: _arr(tangents.get(k, np.zeros(n)), precision) for k in PBF_PARAMETERS}
+    nx, nxp = _out_vec(n, precision)
+    nv, nvp = _out_vec(n, precision)
+    getattr(library(), "oracle_pbf_jvp_" + precision)(
+        ctypes.byref(domain), c_uint32(pressure_iterations), c_uint64(step_index), pp, vp, cp,
+        *(c_void_p(arrs[k].ctypes.data) for k in PBF_PARAMETERS), *(t[1] for t in tv),
+        *(c_void_p(ts[k].ctypes.data) for k in PBF_PARAMETERS), nxp, nvp)
+    return {"positions": nx, "velocities": nv}
+
+
+def pbf_vjp(domain, step_index, positions, velocities, controls, params: Dict[str, np.ndarray],
+            next_position_adjoint, next_velocity_adjoint, pressure_iterations=4, precision="f32"):
+    """PBF::vjp_step with FP64 adjoints. Returns the adjoints of the state
+    ("positions", "velocities"), the control and every PBF_PARAMETERS entry."""
+    n = domain.particle_count
+    pos, pp = _vec(positions, precision)
+    vel, vp = _vec(velocities, precision)
+    ctl, cp = _vec(controls, precision)
+    arrs = {k: _arr(params[k], precision) for k in PBF_PARAMETERS}
+    nxa, nxap = _vec(next_position_adjoint, "f64")
+    nva, nvap = _vec(next_velocity_adjoint, "f64")
+    outs = {k: _out_vec(n, "f64") for k in ("positions", "velocities", "controls")}
+    scal = {k: np.zeros(n, dtype=np.float64) for k in PBF_PARAMETERS}
+    getattr(library(), "oracle_pbf_vjp_" + precision)(
+        ctypes.byref(domain), c_uint32(pressure_iterations), c_uint64(step_index), pp, vp, cp,
+        *(c_void_p(arrs[k].ctypes.data) for k in PBF_PARAMETERS), nxap, nvap,
+        *(outs[k][1] for k in ("positions", "velocities", "controls")),
+        *(c_void_p(scal[k].ctypes.data) for k in PBF_PARAMETERS))
+    result = {k: outs[k][0] for k in outs}
+    result.update(scal)
+    return result

@@ -109,6 +109,29 @@ void oracle_set_threads(int threads);
                               const real* strength, const real* exponent, const real* radius, const real* xsph,         \
                               const real* confinement, real* next_positions[3], real* next_velocities[3],               \
                               real* densities_out, real* lambdas_out, real* predicted_out[3]);                           \
+    /* PBF::jvp_step (pbf.cpp:193-215): tangents in the primal precision */                                           \
+    void oracle_pbf_jvp##suf(const oracle_domain* domain, uint32_t pressure_iterations, uint64_t step_index,          \
+                             const real* positions[3], const real* velocities[3], const real* controls[3],             \
+                             const real* masses, const real* rest_densities, const real* relaxation,                   \
+                             const real* strength, const real* exponent, const real* radius, const real* xsph,         \
+                             const real* confinement, const real* position_tangent[3],                                 \
+                             const real* velocity_tangent[3], const real* control_tangent[3],                          \
+                             const real* masses_tangent, const real* rest_densities_tangent,                           \
+                             const real* relaxation_tangent, const real* strength_tangent,                             \
+                             const real* exponent_tangent, const real* radius_tangent, const real* xsph_tangent,        \
+                             const real* confinement_tangent, real* next_position_tangent[3],                          \
+                             real* next_velocity_tangent[3]);                                                           \
+    /* PBF::vjp_step (pbf.cpp:217-258): FP64 adjoints, accumulated into the outputs */                                \
+    void oracle_pbf_vjp##suf(const oracle_domain* domain, uint32_t pressure_iterations, uint64_t step_index,          \
+                             const real* positions[3], const real* velocities[3], const real* controls[3],             \
+                             const real* masses, const real* rest_densities, const real* relaxation,                   \
+                             const real* strength, const real* exponent, const real* radius, const real* xsph,         \
+                             const real* confinement, const double* next_position_adjoint[3],                          \
+                             const double* next_velocity_adjoint[3], double* position_adjoint[3],                      \
+                             double* velocity_adjoint[3], double* control_adjoint[3], double* masses_adjoint,          \
+                             double* rest_densities_adjoint, double* relaxation_adjoint, double* strength_adjoint,      \
+                             double* exponent_adjoint, double* radius_adjoint, double* xsph_adjoint,                    \
+                             double* confinement_adjoint);                                                              \
     /* single kernels on an existing grid, for unit tests (sph/common.cu:14-47, 165-194) */                            \
     void oracle_non_pressure##suf(const oracle_domain* domain, uint64_t grid_step, const real* positions[3],          \
                                   const real* velocities[3], const real* controls[3], const real* masses,              \

@@ -1046,6 +1046,713 @@ void SUF(oracle_pbf_step)(const oracle_domain* d, uint32_t iterations, uint64_t 
     grid_free(&g);
 }
 
+/* ======================================================= PBF derivatives */
+/* Forward-mode (JVP) and reverse-mode (VJP) derivatives of PBF::forward_step
+ * with the neighbour graph, the box-collision masks and every branch frozen
+ * at the primal step: PBF::jvp_step (pbf.cpp:193-215) and PBF::vjp_step
+ * (pbf.cpp:217-258) with the kernels of pbf.cu and density.cu. The JVP runs
+ * in the primal precision; the VJP promotes the primal values and works in
+ * FP64, and every adjoint output is accumulated (+=). Pair terms are
+ * differentiated in gather form: particle a collects the terms in which it
+ * is the centre and those in which it is a neighbour, so no output is
+ * written by two particles. Boundary particles are not supported here. */
+
+typedef struct {
+    double x, y, z;
+} d3;
+static inline d3 d3_make(double x, double y, double z) {
+    d3 r = {x, y, z};
+    return r;
+}
+static inline d3 d3_of(v3 a) { return d3_make((double)a.x, (double)a.y, (double)a.z); }
+static inline d3 d3_add(d3 a, d3 b) { return d3_make(a.x + b.x, a.y + b.y, a.z + b.z); }
+static inline d3 d3_sub(d3 a, d3 b) { return d3_make(a.x - b.x, a.y - b.y, a.z - b.z); }
+static inline d3 d3_scale(d3 v, double f) { return d3_make(f * v.x, f * v.y, f * v.z); }
+static inline double d3_dot(d3 a, d3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static inline d3 d3_cross(d3 a, d3 b) {
+    return d3_make(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+static inline d3 d3_load(double* const f[3], uint32_t i) { return d3_make(f[0][i], f[1][i], f[2][i]); }
+static inline d3 d3_cload(const double* const f[3], uint32_t i) { return d3_make(f[0][i], f[1][i], f[2][i]); }
+static inline void d3_acc(double* const f[3], uint32_t i, d3 v) {
+    f[0][i] += v.x;
+    f[1][i] += v.y;
+    f[2][i] += v.z;
+}
+
+/* Derivative of the spiky gradient G(d) = f(r) d, f = k (h - r)^2 / r, along
+ * w: H(d) w = f w + (f'(r) / r) (d . w) d with f' = -k (h - r)(h + r) / r^2
+ * (zero where G is zero: r = 0 or r >= h). H is symmetric and even in d. */
+static d3 spiky_hessian_d(v3 d, d3 w, double h, double k) {
+    const d3 dd = d3_of(d);
+    const double r = sqrt(d3_dot(dd, dd));
+    if (r == 0.0 || r >= h) return d3_make(0, 0, 0);
+    const double f = k * (h - r) * (h - r) / r;
+    const double fp = -k * (h - r) * (h + r) / (r * r);
+    return d3_add(d3_scale(w, f), d3_scale(dd, fp / r * d3_dot(dd, w)));
+}
+
+/* The same product in the primal precision (the JVP's G-dot). */
+static v3 spiky_hessian_r(v3 d, v3 w, real h, real k) {
+    const real r = v3_length(d);
+    if (r == K(0.0) || r >= h) return v3_make(0, 0, 0);
+    const real f = k * (h - r) * (h - r) / r;
+    const real fp = -k * (h - r) * (h + r) / (r * r);
+    return v3_add(v3_scale(w, f), v3_scale(d, fp / r * v3_dot(d, w)));
+}
+
+/* Artificial pressure s(d) = -k (W(d) / W(dq))^n (pbf.cu:56-71) with its
+ * gradient in d and its partial derivatives in k, n and dq. */
+typedef struct {
+    real value;
+    v3 gradient;
+    real d_strength, d_exponent, d_radius;
+} artificial_t;
+
+static artificial_t artificial_full(v3 d, real h, real strength, real exponent, real radius) {
+    artificial_t a;
+    memset(&a, 0, sizeof(a));
+    const real reference = poly6(v3_make(radius, K(0.0), K(0.0)), h);
+    const real kernel = poly6(d, h);
+    if (kernel == K(0.0)) return a;
+    const real ratio = kernel / reference;
+    const real power = R_POW(ratio, exponent);
+    a.value = -strength * power;
+    a.d_strength = -power;
+    a.d_exponent = a.value * R_LOG(ratio);
+    /* dW(dq)/d dq = -6 alpha (h^2 - dq^2)^2 dq, alpha = 315 / (64 pi h^9) */
+    const real hr = h * h;
+    const real diff = hr - radius * radius;
+    const real radial = radius >= h ? K(0.0)
+                                    : K(-6.0) * K(315.0) / (K(64.0) * R_PI * hr * hr * hr * h * h * h) * radius * diff * diff;
+    a.d_radius = -a.value * exponent * radial / reference;
+    a.gradient = v3_scale(poly6_gradient(d, h), -strength * exponent * R_POW(ratio, exponent - K(1.0)) / reference);
+    return a;
+}
+
+/* Every primal quantity of one PBF step that the derivatives read. */
+typedef struct {
+    uint32_t n, iterations;
+    grid_t g;
+    real h, spiky, dt, inv_dt;
+    vfield xs;     /* predicted positions: topology and p[0] */
+    vfield* p;     /* positions entering iteration k; p[iterations] is the result */
+    real** rho;    /* per iteration */
+    real** lam;
+    real** den;
+    vfield* gsum;
+    uint8_t** mask; /* collision mask of the projection of iteration k */
+    vfield vrec, w, nrm, vc;
+    real *mag, *nz;
+} pbf_record;
+
+static void pbf_record_forward(pbf_record* r, const oracle_domain* d, uint32_t iterations, uint64_t step,
+                               const real* pos[3], const real* vel[3], const real* ctl[3], const real* masses,
+                               const real* rest_densities, const real* relaxation, const real* strength,
+                               const real* exponent, const real* radius, const real* confinement) {
+    const uint32_t n = d->particle_count;
+    memset(r, 0, sizeof(*r));
+    r->n = n;
+    r->iterations = iterations;
+    r->h = (real)d->support_radius;
+    r->spiky = spiky_coefficient(r->h);
+    r->dt = (real)d->time_step;
+    r->inv_dt = K(1.0) / r->dt;
+    const v3 gravity = v3_make((real)d->gravity[0], (real)d->gravity[1], (real)d->gravity[2]);
+    r->xs = alloc_vfield(n);
+    for (uint32_t i = 0; i < n; ++i) {
+        const v3 pv = v3_add(v3_load(vel, i), v3_scale(v3_add(gravity, v3_load(ctl, i)), r->dt));
+        v3_store(r->xs.c, i, v3_add(v3_load(pos, i), v3_scale(pv, r->dt)));
+    }
+    grid_build(&r->g, d, step + 1u, r->xs.c[0], r->xs.c[1], r->xs.c[2]);
+    r->p = (vfield*)calloc(iterations + 1u, sizeof(vfield));
+    r->rho = (real**)calloc(iterations + 1u, sizeof(real*));
+    r->lam = (real**)calloc(iterations + 1u, sizeof(real*));
+    r->den = (real**)calloc(iterations + 1u, sizeof(real*));
+    r->gsum = (vfield*)calloc(iterations + 1u, sizeof(vfield));
+    r->mask = (uint8_t**)calloc(iterations + 1u, sizeof(uint8_t*));
+    r->p[0] = alloc_vfield(n);
+    for (int c = 0; c < 3; ++c) memcpy(r->p[0].c[c], r->xs.c[c], sizeof(real) * n);
+    const cbox box = collision_box(d, step + 1u);
+    vfield corr = alloc_vfield(n);
+    for (uint32_t it = 0; it < iterations; ++it) {
+        r->rho[it] = alloc_real(n);
+        r->lam[it] = alloc_real(n);
+        r->den[it] = alloc_real(n);
+        r->gsum[it] = alloc_vfield(n);
+        r->mask[it] = (uint8_t*)malloc(n + 16u);
+        r->p[it + 1] = alloc_vfield(n);
+        density_kernel(&r->g, 1, CV(r->xs), CV(r->p[it]), masses, rest_densities, r->rho[it]);
+        lambda_kernel(&r->g, CV(r->xs), CV(r->p[it]), masses, rest_densities, r->rho[it], relaxation, r->gsum[it].c,
+                      r->den[it], r->lam[it]);
+        correction_kernel(&r->g, CV(r->xs), CV(r->p[it]), masses, rest_densities, r->lam[it], strength, exponent, radius,
+                          corr.c);
+        for (uint32_t i = 0; i < n; ++i) {
+            const v3 cand = v3_add(v3_load(CV(r->p[it]), i), v3_load(CV(corr), i));
+            uint8_t m = 0;
+            if (cand.x < box.lo[0] || cand.x > box.hi[0]) m |= 1u;
+            if (cand.y < box.lo[1] || cand.y > box.hi[1]) m |= 2u;
+            if (cand.z < box.lo[2] || cand.z > box.hi[2]) m |= 4u;
+            r->mask[it][i] = m;
+            v3_store(r->p[it + 1].c, i,
+                     v3_make(R_FMIN(box.hi[0], R_FMAX(box.lo[0], cand.x)), R_FMIN(box.hi[1], R_FMAX(box.lo[1], cand.y)),
+                             R_FMIN(box.hi[2], R_FMAX(box.lo[2], cand.z))));
+        }
+    }
+    free_vfield(&corr);
+    const vfield pc = r->p[iterations];
+    r->vrec = alloc_vfield(n);
+    r->w = alloc_vfield(n);
+    r->nrm = alloc_vfield(n);
+    r->vc = alloc_vfield(n);
+    r->mag = alloc_real(n);
+    r->nz = alloc_real(n);
+    for (uint32_t i = 0; i < n; ++i) v3_store(r->vrec.c, i, v3_scale(v3_sub(v3_load(CV(pc), i), v3_load(pos, i)), r->inv_dt));
+    vorticity_kernel(&r->g, CV(r->xs), CV(pc), CV(r->vrec), r->w.c);
+    for (uint32_t i = 0; i < n; ++i) r->mag[i] = v3_length(v3_load(CV(r->w), i));
+    normal_kernel(&r->g, CV(r->xs), CV(pc), r->mag, r->nrm.c, r->nz);
+    for (uint32_t i = 0; i < n; ++i)
+        v3_store(r->vc.c, i,
+                 v3_add(v3_load(CV(r->vrec), i),
+                        v3_scale(v3_cross(v3_load(CV(r->nrm), i), v3_load(CV(r->w), i)), r->dt * confinement[i])));
+}
+
+static void pbf_record_free(pbf_record* r) {
+    for (uint32_t it = 0; it <= r->iterations; ++it) {
+        free_vfield(&r->p[it]);
+        if (it < r->iterations) {
+            free(r->rho[it]);
+            free(r->lam[it]);
+            free(r->den[it]);
+            free_vfield(&r->gsum[it]);
+            free(r->mask[it]);
+        }
+    }
+    free(r->p);
+    free(r->rho);
+    free(r->lam);
+    free(r->den);
+    free(r->gsum);
+    free(r->mask);
+    free_vfield(&r->xs);
+    free_vfield(&r->vrec);
+    free_vfield(&r->w);
+    free_vfield(&r->nrm);
+    free_vfield(&r->vc);
+    free(r->mag);
+    free(r->nz);
+    grid_free(&r->g);
+}
+
+/* ---------------------------------------------------------------- PBF JVP */
+void SUF(oracle_pbf_jvp)(const oracle_domain* d, uint32_t iterations, uint64_t step, const real* pos[3],
+                         const real* vel[3], const real* ctl[3], const real* masses, const real* rest_densities,
+                         const real* relaxation, const real* strength, const real* exponent, const real* radius,
+                         const real* xsph, const real* confinement, const real* pos_t[3], const real* vel_t[3],
+                         const real* ctl_t[3], const real* masses_t, const real* rest_densities_t,
+                         const real* relaxation_t, const real* strength_t, const real* exponent_t,
+                         const real* radius_t, const real* xsph_t, const real* confinement_t, real* next_pos_t[3],
+                         real* next_vel_t[3]) {
+    pbf_record r;
+    pbf_record_forward(&r, d, iterations, step, pos, vel, ctl, masses, rest_densities, relaxation, strength, exponent,
+                       radius, confinement);
+    const uint32_t n = r.n;
+    const real h = r.h, k = r.spiky, dt = r.dt;
+    const grid_t* g = &r.g;
+    vfield pt = alloc_vfield(n), pn = alloc_vfield(n);
+    real* rho_t = alloc_real(n);
+    real* lam_t = alloc_real(n);
+    /* predict_jvp_kernel, pbf.cu:80-84 */
+    for (uint32_t i = 0; i < n; ++i)
+        v3_store(pt.c, i,
+                 v3_add(v3_load(pos_t, i), v3_scale(v3_add(v3_load(vel_t, i), v3_scale(v3_load(ctl_t, i), dt)), dt)));
+    for (uint32_t it = 0; it < iterations; ++it) {
+        const real* const* p = CV(r.p[it]);
+        /* density_jvp_kernel (density.cu:42-67, poly6) */
+#pragma omp parallel for schedule(static)
+        for (int64_t ii = 0; ii < (int64_t)n; ++ii) {
+            const uint32_t i = (uint32_t)ii;
+            const v3 pi = v3_load(p, i), ti = v3_load(CV(pt), i);
+            cell_range cr[27];
+            const int nr = stencil(g, v3_load(CV(r.xs), i), cr);
+            real acc = K(0.0);
+            for (int c = 0; c < nr; ++c)
+                for (uint32_t s = cr[c].first; s < cr[c].last; ++s) {
+                    const uint32_t j = g->idx[s];
+                    const v3 dd = v3_sub(pi, v3_load(p, j));
+                    acc += masses_t[j] * poly6(dd, h) + masses[j] * v3_dot(poly6_gradient(dd, h), v3_sub(ti, v3_load(CV(pt), j)));
+                }
+            rho_t[i] = acc;
+        }
+        /* lambda_jvp_kernel (pbf.cu:129-176) */
+#pragma omp parallel for schedule(static)
+        for (int64_t ii = 0; ii < (int64_t)n; ++ii) {
+            const uint32_t i = (uint32_t)ii;
+            const v3 pi = v3_load(p, i), ti = v3_load(CV(pt), i);
+            const real rho0 = rest_densities[i], rho0_t = rest_densities_t[i];
+            v3 sg = v3_make(0, 0, 0), sg_t = v3_make(0, 0, 0);
+            real den_t = relaxation_t[i];
+            cell_range cr[27];
+            const int nr = stencil(g, v3_load(CV(r.xs), i), cr);
+            for (int c = 0; c < nr; ++c)
+                for (uint32_t s = cr[c].first; s < cr[c].last; ++s) {
+                    const uint32_t j = g->idx[s];
+                    if (j == i) continue;
+                    const v3 dd = v3_sub(pi, v3_load(p, j));
+                    const v3 gr = spiky_gradient(dd, h, k);
+                    const v3 gr_t = spiky_hessian_r(dd, v3_sub(ti, v3_load(CV(pt), j)), h, k);
+                    const real cf = masses[j] / rho0;
+                    const real cf_t = masses_t[j] / rho0 - masses[j] * rho0_t / (rho0 * rho0);
+                    const v3 ng = v3_scale(gr, cf);
+                    const v3 ng_t = v3_add(v3_scale(gr, cf_t), v3_scale(gr_t, cf));
+                    sg = v3_add(sg, ng);
+                    sg_t = v3_add(sg_t, ng_t);
+                    den_t += K(2.0) * v3_dot(ng, ng_t);
+                }
+            den_t += K(2.0) * v3_dot(sg, sg_t);
+            const real den = r.den[it][i];
+            const real con = r.rho[it][i] / rho0 - K(1.0);
+            const real con_t = rho_t[i] / rho0 - r.rho[it][i] * rho0_t / (rho0 * rho0);
+            lam_t[i] = -con_t / den + con * den_t / (den * den);
+        }
+        /* correction_jvp_kernel (pbf.cu:290-333) + project_jvp_kernel (pbf.cu:433-443) */
+#pragma omp parallel for schedule(static)
+        for (int64_t ii = 0; ii < (int64_t)n; ++ii) {
+            const uint32_t i = (uint32_t)ii;
+            const v3 pi = v3_load(p, i), ti = v3_load(CV(pt), i);
+            const real rho0 = rest_densities[i], rho0_t = rest_densities_t[i];
+            v3 res = v3_make(0, 0, 0);
+            cell_range cr[27];
+            const int nr = stencil(g, v3_load(CV(r.xs), i), cr);
+            for (int c = 0; c < nr; ++c)
+                for (uint32_t s = cr[c].first; s < cr[c].last; ++s) {
+                    const uint32_t j = g->idx[s];
+                    if (j == i) continue;
+                    const v3 dd = v3_sub(pi, v3_load(p, j));
+                    const v3 dd_t = v3_sub(ti, v3_load(CV(pt), j));
+                    const artificial_t a = artificial_full(dd, h, strength[i], exponent[i], radius[i]);
+                    const real a_t = v3_dot(a.gradient, dd_t) + a.d_strength * strength_t[i] +
+                                     a.d_exponent * exponent_t[i] + a.d_radius * radius_t[i];
+                    const real cf = masses[j] / rho0;
+                    const real cf_t = masses_t[j] / rho0 - masses[j] * rho0_t / (rho0 * rho0);
+                    const real mu = r.lam[it][i] + r.lam[it][j] + a.value;
+                    const real mu_t = lam_t[i] + lam_t[j] + a_t;
+                    res = v3_add(res, v3_add(v3_scale(spiky_gradient(dd, h, k), cf_t * mu + cf * mu_t),
+                                             v3_scale(spiky_hessian_r(dd, dd_t, h, k), cf * mu)));
+                }
+            v3 out = v3_add(ti, res);
+            const uint8_t m = r.mask[it][i];
+            if (m & 1u) out.x = K(0.0);
+            if (m & 2u) out.y = K(0.0);
+            if (m & 4u) out.z = K(0.0);
+            v3_store(pn.c, i, out);
+        }
+        vfield tmp = pt;
+        pt = pn;
+        pn = tmp;
+    }
+    const real* const* pc = CV(r.p[iterations]);
+    vfield vt = alloc_vfield(n), wt = alloc_vfield(n), nt = alloc_vfield(n), vct = alloc_vfield(n);
+    real* mag_t = alloc_real(n);
+    /* reconstruct_jvp_kernel (pbf.cu:462-466) */
+    for (uint32_t i = 0; i < n; ++i)
+        v3_store(vt.c, i, v3_scale(v3_sub(v3_load(CV(pt), i), v3_load(pos_t, i)), r.inv_dt));
+    /* vorticity_jvp_kernel (pbf.cu:499-527) + magnitude_jvp_kernel (pbf.cu:595-599) */
+#pragma omp parallel for schedule(static)
+    for (int64_t ii = 0; ii < (int64_t)n; ++ii) {
+        const uint32_t i = (uint32_t)ii;
+        const v3 pi = v3_load(pc, i), ti = v3_load(CV(pt), i);
+        const v3 vi = v3_load(CV(r.vrec), i), vti = v3_load(CV(vt), i);
+        v3 res = v3_make(0, 0, 0);
+        cell_range cr[27];
+        const int nr = stencil(g, v3_load(CV(r.xs), i), cr);
+        for (int c = 0; c < nr; ++c)
+            for (uint32_t s = cr[c].first; s < cr[c].last; ++s) {
+                const uint32_t j = g->idx[s];
+                if (j == i) continue;
+                const v3 dd = v3_sub(pi, v3_load(pc, j));
+                const v3 dd_t = v3_sub(ti, v3_load(CV(pt), j));
+                const v3 diff = v3_sub(v3_load(CV(r.vrec), j), vi);
+                const v3 diff_t = v3_sub(v3_load(CV(vt), j), vti);
+                res = v3_add(res, v3_add(v3_cross(diff_t, spiky_gradient(dd, h, k)),
+                                         v3_cross(diff, spiky_hessian_r(dd, dd_t, h, k))));
+            }
+        v3_store(wt.c, i, res);
+        mag_t[i] = r.mag[i] > K(0.0) ? v3_dot(v3_load(CV(r.w), i), res) / r.mag[i] : K(0.0);
+    }
+    /* normal_jvp_kernel (pbf.cu:600-628) + confinement_jvp_kernel (pbf.cu:680-686) */
+#pragma omp parallel for schedule(static)
+    for (int64_t ii = 0; ii < (int64_t)n; ++ii) {
+        const uint32_t i = (uint32_t)ii;
+        const v3 pi = v3_load(pc, i), ti = v3_load(CV(pt), i);
+        v3 gt = v3_make(0, 0, 0);
+        cell_range cr[27];
+        const int nr = stencil(g, v3_load(CV(r.xs), i), cr);
+        for (int c = 0; c < nr; ++c)
+            for (uint32_t s = cr[c].first; s < cr[c].last; ++s) {
+                const uint32_t j = g->idx[s];
+                if (j == i) continue;
+                const v3 dd = v3_sub(pi, v3_load(pc, j));
+                const v3 dd_t = v3_sub(ti, v3_load(CV(pt), j));
+                gt = v3_add(gt, v3_add(v3_scale(spiky_gradient(dd, h, k), mag_t[j] - mag_t[i]),
+                                       v3_scale(spiky_hessian_r(dd, dd_t, h, k), r.mag[j] - r.mag[i])));
+            }
+        const v3 nv = v3_load(CV(r.nrm), i);
+        const v3 ntan = r.nz[i] == K(0.0) ? v3_make(0, 0, 0)
+                                          : v3_scale(v3_add(gt, v3_scale(nv, -v3_dot(nv, gt))), K(1.0) / r.nz[i]);
+        v3_store(nt.c, i, ntan);
+        const v3 wv = v3_load(CV(r.w), i);
+        const v3 force = v3_cross(nv, wv);
+        const v3 force_t = v3_add(v3_cross(ntan, wv), v3_cross(nv, v3_load(CV(wt), i)));
+        v3_store(vct.c, i,
+                 v3_add(v3_load(CV(vt), i),
+                        v3_scale(v3_add(v3_scale(force, confinement_t[i]), v3_scale(force_t, confinement[i])), dt)));
+    }
+    /* xsph_jvp_kernel (pbf.cu:723-754) */
+#pragma omp parallel for schedule(static)
+    for (int64_t ii = 0; ii < (int64_t)n; ++ii) {
+        const uint32_t i = (uint32_t)ii;
+        const v3 pi = v3_load(pc, i), ti = v3_load(CV(pt), i);
+        const v3 vi = v3_load(CV(r.vc), i), vti = v3_load(CV(vct), i);
+        v3 sm = v3_make(0, 0, 0), sm_t = v3_make(0, 0, 0);
+        cell_range cr[27];
+        const int nr = stencil(g, v3_load(CV(r.xs), i), cr);
+        for (int c = 0; c < nr; ++c)
+            for (uint32_t s = cr[c].first; s < cr[c].last; ++s) {
+                const uint32_t j = g->idx[s];
+                if (j == i) continue;
+                const v3 dd = v3_sub(pi, v3_load(pc, j));
+                const v3 dd_t = v3_sub(ti, v3_load(CV(pt), j));
+                const v3 diff = v3_sub(v3_load(CV(r.vc), j), vi);
+                const v3 diff_t = v3_sub(v3_load(CV(vct), j), vti);
+                const real wgt = poly6(dd, h);
+                const real wgt_t = v3_dot(poly6_gradient(dd, h), dd_t);
+                sm = v3_add(sm, v3_scale(diff, wgt));
+                sm_t = v3_add(sm_t, v3_add(v3_scale(diff_t, wgt), v3_scale(diff, wgt_t)));
+            }
+        v3_store(next_vel_t, i, v3_add(vti, v3_add(v3_scale(sm, xsph_t[i]), v3_scale(sm_t, xsph[i]))));
+        v3_store(next_pos_t, i, ti);
+    }
+    free_vfield(&pt);
+    free_vfield(&pn);
+    free_vfield(&vt);
+    free_vfield(&wt);
+    free_vfield(&nt);
+    free_vfield(&vct);
+    free(rho_t);
+    free(lam_t);
+    free(mag_t);
+    pbf_record_free(&r);
+}
+
+/* ---------------------------------------------------------------- PBF VJP */
+static double* dzeros(uint32_t n) { return (double*)calloc((size_t)n + 2u, sizeof(double)); }
+typedef struct {
+    double* c[3];
+} dfield;
+static dfield dfield_zeros(uint32_t n) {
+    dfield f;
+    for (int a = 0; a < 3; ++a) f.c[a] = dzeros(n);
+    return f;
+}
+static void dfield_free(dfield* f) {
+    for (int a = 0; a < 3; ++a) free(f->c[a]);
+}
+#define DV(f) ((const double* const*)(f).c)
+
+void SUF(oracle_pbf_vjp)(const oracle_domain* d, uint32_t iterations, uint64_t step, const real* pos[3],
+                         const real* vel[3], const real* ctl[3], const real* masses, const real* rest_densities,
+                         const real* relaxation, const real* strength, const real* exponent, const real* radius,
+                         const real* xsph, const real* confinement, const double* next_pos_a[3],
+                         const double* next_vel_a[3], double* pos_a[3], double* vel_a[3], double* ctl_a[3],
+                         double* masses_a, double* rest_densities_a, double* relaxation_a, double* strength_a,
+                         double* exponent_a, double* radius_a, double* xsph_a, double* confinement_a) {
+    pbf_record r;
+    pbf_record_forward(&r, d, iterations, step, pos, vel, ctl, masses, rest_densities, relaxation, strength, exponent,
+                       radius, confinement);
+    const uint32_t n = r.n;
+    const double h = (double)r.h, k = (double)r.spiky;
+    const real hr = r.h, kr = r.spiky;
+    const grid_t* g = &r.g;
+    const real* const* pc = CV(r.p[iterations]);
+    dfield pa = dfield_zeros(n);  /* adjoint of the positions p[current] */
+    dfield vca = dfield_zeros(n); /* adjoint of v_c */
+    dfield va = dfield_zeros(n);  /* adjoint of v_rec */
+    dfield wa = dfield_zeros(n);  /* adjoint of omega */
+    dfield na = dfield_zeros(n);  /* adjoint of the normals */
+    double* maga = dzeros(n);
+    for (uint32_t i = 0; i < n; ++i) d3_acc(pa.c, i, d3_cload(next_pos_a, i));
+
+    /* xsph_vjp_kernel (pbf.cu:756-794) */
+#pragma omp parallel for schedule(static)
+    for (int64_t ii = 0; ii < (int64_t)n; ++ii) {
+        const uint32_t a = (uint32_t)ii;
+        const v3 pi = v3_load(pc, a);
+        const d3 va_out = d3_cload(next_vel_a, a);
+        const d3 vca_ = d3_of(v3_load(CV(r.vc), a));
+        d3 pcon = d3_make(0, 0, 0), vcon = va_out;
+        double ccon = 0.0;
+        cell_range cr[27];
+        const int nr = stencil(g, v3_load(CV(r.xs), a), cr);
+        for (int c = 0; c < nr; ++c)
+            for (uint32_t s = cr[c].first; s < cr[c].last; ++s) {
+                const uint32_t j = g->idx[s];
+                if (j == a) continue;
+                const v3 dd = v3_sub(pi, v3_load(pc, j));
+                const double wgt = (double)poly6(dd, hr);
+                const d3 gw = d3_of(poly6_gradient(dd, hr));
+                const d3 diff = d3_sub(d3_of(v3_load(CV(r.vc), j)), vca_);
+                const d3 vb_j = d3_cload(next_vel_a, j);
+                vcon = d3_add(vcon, d3_scale(va_out, -(double)xsph[a] * wgt));
+                vcon = d3_add(vcon, d3_scale(vb_j, (double)xsph[j] * wgt));
+                pcon = d3_add(pcon, d3_scale(gw, (double)xsph[a] * d3_dot(va_out, diff) -
+                                                    (double)xsph[j] * d3_dot(vb_j, diff)));
+                ccon += wgt * d3_dot(va_out, diff);
+            }
+        d3_acc(pa.c, a, pcon);
+        d3_acc(vca.c, a, vcon);
+        xsph_a[a] += ccon;
+    }
+    /* confinement_vjp_kernel (pbf.cu:688-698) */
+    const double dt = (double)r.dt;
+    for (uint32_t i = 0; i < n; ++i) {
+        const d3 out = d3_load(vca.c, i);
+        const d3 wv = d3_of(v3_load(CV(r.w), i));
+        const d3 nv = d3_of(v3_load(CV(r.nrm), i));
+        const double f = dt * (double)confinement[i];
+        d3_acc(va.c, i, out);
+        d3_acc(wa.c, i, d3_scale(d3_cross(out, nv), f));
+        d3_acc(na.c, i, d3_scale(d3_cross(wv, out), f));
+        confinement_a[i] += dt * d3_dot(out, d3_cross(nv, wv));
+    }
+    /* normal_vjp_kernel (pbf.cu:635-672): gradient adjoint (n_bar - n (n . n_bar)) / N */
+    dfield ga = dfield_zeros(n);
+    for (uint32_t i = 0; i < n; ++i) {
+        if (r.nz[i] == K(0.0)) continue;
+        const d3 nb = d3_load(na.c, i);
+        const d3 nv = d3_of(v3_load(CV(r.nrm), i));
+        d3_acc(ga.c, i, d3_scale(d3_sub(nb, d3_scale(nv, d3_dot(nv, nb))), 1.0 / (double)r.nz[i]));
+    }
+#pragma omp parallel for schedule(static)
+    for (int64_t ii = 0; ii < (int64_t)n; ++ii) {
+        const uint32_t a = (uint32_t)ii;
+        const v3 pi = v3_load(pc, a);
+        const d3 gaa = d3_load(ga.c, a);
+        d3 pcon = d3_make(0, 0, 0);
+        double mcon = 0.0;
+        cell_range cr[27];
+        const int nr = stencil(g, v3_load(CV(r.xs), a), cr);
+        for (int c = 0; c < nr; ++c)
+            for (uint32_t s = cr[c].first; s < cr[c].last; ++s) {
+                const uint32_t j = g->idx[s];
+                if (j == a) continue;
+                const v3 dd = v3_sub(pi, v3_load(pc, j));
+                const d3 gr = d3_of(spiky_gradient(dd, hr, kr));
+                const d3 both = d3_add(gaa, d3_load(ga.c, j));
+                pcon = d3_add(pcon, spiky_hessian_d(dd, d3_scale(both, (double)r.mag[j] - (double)r.mag[a]), h, k));
+                mcon -= d3_dot(gr, both);
+            }
+        d3_acc(pa.c, a, pcon);
+        maga[a] += mcon;
+    }
+    for (uint32_t i = 0; i < n; ++i)
+        if (r.mag[i] > K(0.0)) d3_acc(wa.c, i, d3_scale(d3_of(v3_load(CV(r.w), i)), maga[i] / (double)r.mag[i]));
+    /* vorticity_vjp_kernel (pbf.cu:529-563) */
+#pragma omp parallel for schedule(static)
+    for (int64_t ii = 0; ii < (int64_t)n; ++ii) {
+        const uint32_t a = (uint32_t)ii;
+        const v3 pi = v3_load(pc, a);
+        const d3 vi = d3_of(v3_load(CV(r.vrec), a));
+        const d3 wb = d3_load(wa.c, a);
+        d3 pcon = d3_make(0, 0, 0), vcon = d3_make(0, 0, 0);
+        cell_range cr[27];
+        const int nr = stencil(g, v3_load(CV(r.xs), a), cr);
+        for (int c = 0; c < nr; ++c)
+            for (uint32_t s = cr[c].first; s < cr[c].last; ++s) {
+                const uint32_t j = g->idx[s];
+                if (j == a) continue;
+                const v3 dd = v3_sub(pi, v3_load(pc, j));
+                const d3 gr = d3_of(spiky_gradient(dd, hr, kr));
+                const d3 both = d3_add(wb, d3_load(wa.c, j));
+                const d3 diff = d3_sub(d3_of(v3_load(CV(r.vrec), j)), vi);
+                vcon = d3_sub(vcon, d3_cross(gr, both));
+                pcon = d3_add(pcon, spiky_hessian_d(dd, d3_cross(both, diff), h, k));
+            }
+        d3_acc(pa.c, a, pcon);
+        d3_acc(va.c, a, vcon);
+    }
+    /* reconstruct_vjp_kernel (pbf.cu:468-474) */
+    const double inv_dt = (double)r.inv_dt;
+    for (uint32_t i = 0; i < n; ++i) {
+        const d3 cb = d3_scale(d3_load(va.c, i), inv_dt);
+        d3_acc(pos_a, i, d3_scale(cb, -1.0));
+        d3_acc(pa.c, i, cb);
+    }
+    /* constraint iterations in reverse (pbf.cpp:226-256) */
+    dfield pn = dfield_zeros(n), ca = dfield_zeros(n);
+    double* lama = dzeros(n);
+    double* rhoa = dzeros(n);
+    for (uint32_t it = iterations; it-- > 0u;) {
+        const real* const* p = CV(r.p[it]);
+        const real* lam = r.lam[it];
+        for (uint32_t i = 0; i < n; ++i) {
+            for (int c = 0; c < 3; ++c) pn.c[c][i] = 0.0;
+            lama[i] = 0.0;
+            rhoa[i] = 0.0;
+            /* project_vjp_kernel (pbf.cu:445-454) */
+            d3 b = d3_load(pa.c, i);
+            const uint8_t m = r.mask[it][i];
+            if (m & 1u) b.x = 0.0;
+            if (m & 2u) b.y = 0.0;
+            if (m & 4u) b.z = 0.0;
+            ca.c[0][i] = b.x;
+            ca.c[1][i] = b.y;
+            ca.c[2][i] = b.z;
+            d3_acc(pn.c, i, b);
+        }
+        /* correction_vjp_kernel (pbf.cu:335-415) */
+#pragma omp parallel for schedule(static)
+        for (int64_t ii = 0; ii < (int64_t)n; ++ii) {
+            const uint32_t a = (uint32_t)ii;
+            const v3 pi = v3_load(p, a);
+            const double rho0 = (double)rest_densities[a];
+            const d3 oa = d3_load(ca.c, a);
+            d3 pcon = d3_make(0, 0, 0);
+            double lcon = 0.0, mcon = 0.0, rcon = 0.0, scon = 0.0, econ = 0.0, qcon = 0.0;
+            cell_range cr[27];
+            const int nr = stencil(g, v3_load(CV(r.xs), a), cr);
+            for (int c = 0; c < nr; ++c)
+                for (uint32_t s = cr[c].first; s < cr[c].last; ++s) {
+                    const uint32_t j = g->idx[s];
+                    if (j == a) continue;
+                    const v3 dd = v3_sub(pi, v3_load(p, j));
+                    const d3 gr = d3_of(spiky_gradient(dd, hr, kr));
+                    /* a as the centre */
+                    const artificial_t art = artificial_full(dd, hr, strength[a], exponent[a], radius[a]);
+                    const double cf = (double)masses[j] / rho0;
+                    const double mu = (double)lam[a] + (double)lam[j] + (double)art.value;
+                    const double sa = cf * d3_dot(oa, gr);
+                    pcon = d3_add(pcon, spiky_hessian_d(dd, d3_scale(oa, cf * mu), h, k));
+                    pcon = d3_add(pcon, d3_scale(d3_of(art.gradient), sa));
+                    lcon += sa;
+                    rcon -= cf * mu * d3_dot(oa, gr) / rho0;
+                    scon += sa * (double)art.d_strength;
+                    econ += sa * (double)art.d_exponent;
+                    qcon += sa * (double)art.d_radius;
+                    /* a as a neighbour of j: d_ja = -d_aj, G(d_ja) = -G(d_aj), H even, grad s odd */
+                    const v3 rd = v3_scale(dd, K(-1.0));
+                    const artificial_t rart = artificial_full(rd, hr, strength[j], exponent[j], radius[j]);
+                    const double rrho0 = (double)rest_densities[j];
+                    const double rcf = (double)masses[a] / rrho0;
+                    const double rmu = (double)lam[j] + (double)lam[a] + (double)rart.value;
+                    const d3 ob = d3_load(ca.c, j);
+                    const d3 rgr = d3_of(spiky_gradient(rd, hr, kr));
+                    const double rsa = rcf * d3_dot(ob, rgr);
+                    pcon = d3_sub(pcon, spiky_hessian_d(rd, d3_scale(ob, rcf * rmu), h, k));
+                    pcon = d3_sub(pcon, d3_scale(d3_of(rart.gradient), rsa));
+                    lcon += rsa;
+                    mcon += rmu / rrho0 * d3_dot(ob, rgr);
+                }
+            d3_acc(pn.c, a, pcon);
+            lama[a] += lcon;
+            masses_a[a] += mcon;
+            rest_densities_a[a] += rcon;
+            strength_a[a] += scon;
+            exponent_a[a] += econ;
+            radius_a[a] += qcon;
+        }
+        /* lambda_vjp_kernel (pbf.cu:178-252): D_bar = lam_bar C / D^2 */
+#pragma omp parallel for schedule(static)
+        for (int64_t ii = 0; ii < (int64_t)n; ++ii) {
+            const uint32_t a = (uint32_t)ii;
+            const v3 pi = v3_load(p, a);
+            const double rho0 = (double)rest_densities[a];
+            const double den = (double)r.den[it][a];
+            const double con = (double)r.rho[it][a] / rho0 - 1.0;
+            const double dbar = lama[a] * con / (den * den);
+            const d3 sgi = d3_of(v3_load(CV(r.gsum[it]), a));
+            d3 pcon = d3_make(0, 0, 0);
+            double mcon = 0.0, rcon = lama[a] * (double)r.rho[it][a] / (den * rho0 * rho0);
+            cell_range cr[27];
+            const int nr = stencil(g, v3_load(CV(r.xs), a), cr);
+            for (int c = 0; c < nr; ++c)
+                for (uint32_t s = cr[c].first; s < cr[c].last; ++s) {
+                    const uint32_t j = g->idx[s];
+                    if (j == a) continue;
+                    const v3 dd = v3_sub(pi, v3_load(p, j));
+                    const d3 gr = d3_of(spiky_gradient(dd, hr, kr));
+                    /* a as the centre: g_bar = 2 D_bar c (c g + S); c_bar = 2 D_bar (c |g|^2 + g . S) */
+                    const double cf = (double)masses[j] / rho0;
+                    const d3 gbar = d3_scale(d3_add(d3_scale(gr, cf), sgi), 2.0 * dbar * cf);
+                    pcon = d3_add(pcon, spiky_hessian_d(dd, gbar, h, k));
+                    const double cbar = 2.0 * dbar * (cf * d3_dot(gr, gr) + d3_dot(gr, sgi));
+                    rcon -= cbar * cf / rho0;
+                    /* a as a neighbour of j */
+                    const double jrho0 = (double)rest_densities[j];
+                    const double jden = (double)r.den[it][j];
+                    const double jcon = (double)r.rho[it][j] / jrho0 - 1.0;
+                    const double jdbar = lama[j] * jcon / (jden * jden);
+                    const d3 sgj = d3_of(v3_load(CV(r.gsum[it]), j));
+                    const v3 rd = v3_scale(dd, K(-1.0));
+                    const d3 rgr = d3_of(spiky_gradient(rd, hr, kr));
+                    const double rcf = (double)masses[a] / jrho0;
+                    const d3 rgbar = d3_scale(d3_add(d3_scale(rgr, rcf), sgj), 2.0 * jdbar * rcf);
+                    pcon = d3_sub(pcon, spiky_hessian_d(rd, rgbar, h, k));
+                    const double rcbar = 2.0 * jdbar * (rcf * d3_dot(rgr, rgr) + d3_dot(rgr, sgj));
+                    mcon += rcbar / jrho0;
+                }
+            d3_acc(pn.c, a, pcon);
+            rhoa[a] -= lama[a] / (den * rho0);
+            masses_a[a] += mcon;
+            rest_densities_a[a] += rcon;
+            relaxation_a[a] += dbar;
+        }
+        /* density_vjp_kernel (density.cu:69-98, poly6; the self term has zero gradient) */
+#pragma omp parallel for schedule(static)
+        for (int64_t ii = 0; ii < (int64_t)n; ++ii) {
+            const uint32_t a = (uint32_t)ii;
+            const v3 pi = v3_load(p, a);
+            d3 pcon = d3_make(0, 0, 0);
+            double mcon = 0.0;
+            cell_range cr[27];
+            const int nr = stencil(g, v3_load(CV(r.xs), a), cr);
+            for (int c = 0; c < nr; ++c)
+                for (uint32_t s = cr[c].first; s < cr[c].last; ++s) {
+                    const uint32_t j = g->idx[s];
+                    const v3 dd = v3_sub(pi, v3_load(p, j));
+                    const d3 gw = d3_of(poly6_gradient(dd, hr));
+                    pcon = d3_add(pcon, d3_scale(gw, rhoa[a] * (double)masses[j] + rhoa[j] * (double)masses[a]));
+                    mcon += rhoa[j] * (double)poly6(dd, hr);
+                }
+            d3_acc(pn.c, a, pcon);
+            masses_a[a] += mcon;
+        }
+        dfield tmp = pa;
+        pa = pn;
+        pn = tmp;
+    }
+    /* predict_vjp_kernel (pbf.cu:86-93) */
+    const double dt2 = (double)(r.dt * r.dt);
+    for (uint32_t i = 0; i < n; ++i) {
+        const d3 b = d3_load(pa.c, i);
+        d3_acc(pos_a, i, b);
+        d3_acc(vel_a, i, d3_scale(b, dt));
+        d3_acc(ctl_a, i, d3_scale(b, dt2));
+    }
+    dfield_free(&pa);
+    dfield_free(&pn);
+    dfield_free(&ca);
+    dfield_free(&vca);
+    dfield_free(&va);
+    dfield_free(&wa);
+    dfield_free(&na);
+    dfield_free(&ga);
+    free(maga);
+    free(lama);
+    free(rhoa);
+    pbf_record_free(&r);
+}
+
 #ifndef ORACLE_F64
 int oracle_threads(void) {
 #ifdef _OPENMP

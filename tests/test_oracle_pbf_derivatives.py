@@ -1,0 +1,164 @@
+"""Pins of the oracle's PBF JVP and VJP: the dot-product identity
+<J t, w> = <t, J^T w> between the two (FP64), central finite differences of
+the FP64 forward step along random directions, and FP32 tracking FP64.
+
+Every term of the step is active: artificial pressure, vorticity
+confinement and XSPH use non-zero parameters, and the particles stay clear
+of the box so no collision mask changes under the perturbations."""
+import numpy as np
+import pytest
+
+import oracle
+from physica_amd import scenes
+
+PARAMS = oracle.PBF_PARAMETERS
+
+
+def scene(shape=(6, 6, 5), seed=3):
+    sc = scenes.pbf_dam_break(shape=shape, seed=seed)
+    n = sc.count
+    rng = np.random.default_rng(seed)
+    # move the block to the middle of the box and give it a swirl
+    sc.positions = (sc.positions + np.array([[1.2], [0.7], [0.5]], dtype=np.float32)).astype(np.float32)
+    c = sc.positions.mean(axis=1, keepdims=True)
+    r = sc.positions - c
+    swirl = np.stack([-r[2], 0.2 * r[0], r[0]]) * 2.0
+    sc.velocities = (swirl + rng.normal(0, 0.05, size=(3, n))).astype(np.float32)
+    p = sc.parameters
+    p["artificial_pressure_strength"] = np.full(n, 0.05, dtype=np.float32)
+    p["xsph_viscosity"] = rng.uniform(0.005, 0.02, n).astype(np.float32)
+    p["vorticity_confinement"] = rng.uniform(0.5, 2.0, n).astype(np.float32)
+    p["masses"] = (p["masses"] * rng.uniform(0.95, 1.05, n)).astype(np.float32)
+    p["rest_densities"] = (p["rest_densities"] * rng.uniform(0.97, 1.03, n)).astype(np.float32)
+    return sc
+
+
+def directions(sc, rng):
+    n = sc.count
+    t = {k: rng.normal(0, 1, size=(3, n)) for k in ("positions", "velocities", "controls")}
+    t["positions"] *= 1e-3
+    t["velocities"] *= 1e-2
+    for k in PARAMS:
+        t[k] = rng.normal(0, 1, n) * 0.1 * np.abs(sc.parameters[k].astype(np.float64)).mean()
+    w = {"positions": rng.normal(0, 1, size=(3, n)), "velocities": rng.normal(0, 1, size=(3, n))}
+    return t, w
+
+
+def forward(d, sc, x, v, u, params, iterations, precision="f64"):
+    r = oracle.pbf_step(d, 0, x, v, u, params, iterations, precision)
+    return r["positions"], r["velocities"]
+
+
+@pytest.mark.parametrize("iterations", [0, 1, 4])
+def test_dot_product_identity(iterations):
+    sc = scene()
+    d = oracle.make_domain(sc.domain)
+    rng = np.random.default_rng(10 + iterations)
+    t, w = directions(sc, rng)
+    u = np.zeros_like(sc.positions)
+    jt = oracle.pbf_jvp(d, 0, sc.positions, sc.velocities, u, sc.parameters, t, iterations, "f64")
+    jw = oracle.pbf_vjp(d, 0, sc.positions, sc.velocities, u, sc.parameters, w["positions"], w["velocities"],
+                        iterations, "f64")
+    left = np.sum(jt["positions"] * w["positions"]) + np.sum(jt["velocities"] * w["velocities"])
+    right = sum(np.sum(t[k] * jw[k]) for k in ("positions", "velocities", "controls")) + \
+        sum(np.sum(t[k] * jw[k]) for k in PARAMS)
+    scale = abs(left) + abs(right)
+    assert abs(left - right) <= 1e-11 * scale, (left, right)
+
+
+@pytest.mark.parametrize("iterations", [1, 3, 4])
+def test_jvp_matches_central_differences(iterations):
+    """Richardson-extrapolated central differences (error O(eps^4); the
+    constraint iterations make the step strongly nonlinear)."""
+    sc = scene()
+    d = oracle.make_domain(sc.domain)
+    rng = np.random.default_rng(20 + iterations)
+    u = np.zeros_like(sc.positions, dtype=np.float64)
+    x = sc.positions.astype(np.float64)
+    v = sc.velocities.astype(np.float64)
+    base = {k: sc.parameters[k].astype(np.float64) for k in sc.parameters}
+    for trial in range(3):
+        t, _ = directions(sc, rng)
+        jt = oracle.pbf_jvp(d, 0, x, v, u, base, t, iterations, "f64")
+
+        def central(eps):
+            def shifted(sign):
+                p = {k: base[k] + sign * eps * t[k] if k in PARAMS else base[k] for k in base}
+                return forward(d, sc, x + sign * eps * t["positions"], v + sign * eps * t["velocities"],
+                               u + sign * eps * t["controls"], p, iterations)
+
+            (xp, vp), (xm, vm) = shifted(1.0), shifted(-1.0)
+            return (xp - xm) / (2 * eps), (vp - vm) / (2 * eps)
+
+        (fx1, fv1), (fx2, fv2) = central(2e-6), central(1e-6)
+        fx = (4 * fx2 - fx1) / 3
+        fv = (4 * fv2 - fv1) / 3
+        for got, fd in ((jt["positions"], fx), (jt["velocities"], fv)):
+            err = np.abs(got - fd).max() / np.abs(fd).max()
+            assert err < 2e-6, (trial, err)
+
+
+def test_vjp_of_each_input_matches_finite_differences():
+    """Directional derivative along one input group at a time, read off the VJP."""
+    sc = scene(shape=(5, 5, 4), seed=5)
+    d = oracle.make_domain(sc.domain)
+    rng = np.random.default_rng(30)
+    u = np.zeros((3, sc.count))
+    x = sc.positions.astype(np.float64)
+    v = sc.velocities.astype(np.float64)
+    base = {k: sc.parameters[k].astype(np.float64) for k in sc.parameters}
+    t_all, w = directions(sc, rng)
+    jw = oracle.pbf_vjp(d, 0, x, v, u, base, w["positions"], w["velocities"], 2, "f64")
+    eps = 1e-5
+    for group in ("positions", "velocities", "controls") + PARAMS:
+        t = t_all[group]
+
+        def value(sign):
+            xs, vs, us, p = x, v, u, dict(base)
+            if group == "positions":
+                xs = x + sign * eps * t
+            elif group == "velocities":
+                vs = v + sign * eps * t
+            elif group == "controls":
+                us = u + sign * eps * t
+            else:
+                p[group] = base[group] + sign * eps * t
+            nx, nv = forward(d, sc, xs, vs, us, p, 2)
+            return np.sum(nx * w["positions"]) + np.sum(nv * w["velocities"])
+
+        fd = (value(1.0) - value(-1.0)) / (2 * eps)
+        got = np.sum(t * jw[group])
+        assert abs(got - fd) <= 1e-5 * max(abs(fd), 1e-12 * np.abs(w["positions"]).sum()), (group, got, fd)
+
+
+def test_f32_derivatives_track_f64():
+    """One iteration: the swirl scene is ill-conditioned (the vorticity normal
+    divides by |grad |omega||), so with more iterations even the FP32 and FP64
+    forward steps drift apart by percents; the FP64 identities above are the
+    pins."""
+    sc = scene()
+    d = oracle.make_domain(sc.domain)
+    rng = np.random.default_rng(40)
+    t, w = directions(sc, rng)
+    u = np.zeros_like(sc.positions)
+    j32 = oracle.pbf_jvp(d, 0, sc.positions, sc.velocities, u, sc.parameters, t, 1, "f32")
+    j64 = oracle.pbf_jvp(d, 0, sc.positions, sc.velocities, u, sc.parameters, t, 1, "f64")
+    for k in ("positions", "velocities"):
+        assert np.abs(j32[k] - j64[k]).max() <= 2e-3 * np.abs(j64[k]).max(), k
+    a32 = oracle.pbf_vjp(d, 0, sc.positions, sc.velocities, u, sc.parameters, w["positions"], w["velocities"], 1, "f32")
+    a64 = oracle.pbf_vjp(d, 0, sc.positions, sc.velocities, u, sc.parameters, w["positions"], w["velocities"], 1, "f64")
+    for k in ("positions", "velocities", "controls") + PARAMS:
+        assert np.abs(a32[k] - a64[k]).max() <= 2e-3 * np.abs(a64[k]).max(), k
+
+
+def test_vjp_accumulates_into_outputs():
+    sc = scene(shape=(4, 4, 4))
+    d = oracle.make_domain(sc.domain)
+    rng = np.random.default_rng(50)
+    _, w = directions(sc, rng)
+    u = np.zeros_like(sc.positions)
+    a = oracle.pbf_vjp(d, 0, sc.positions, sc.velocities, u, sc.parameters, w["positions"], w["velocities"], 2, "f64")
+    b = oracle.pbf_vjp(d, 0, sc.positions, sc.velocities, u, sc.parameters, 2 * w["positions"], 2 * w["velocities"],
+                       2, "f64")
+    for k in ("positions", "velocities", "controls") + PARAMS:
+        np.testing.assert_allclose(b[k], 2 * a[k], rtol=1e-12, atol=1e-300)
