@@ -112,6 +112,42 @@ typedef struct phx_particle_parameters {
     const float* surface_tensions;
 } phx_particle_parameters;
 
+/* Tangent and adjoint counterparts (domain/domain.ixx:76-113; adjoint
+ * fields are FP64, domain/device.h). */
+typedef struct phx_adjoint_vector_view {
+    double* x;
+    double* y;
+    double* z;
+} phx_adjoint_vector_view;
+
+typedef struct phx_const_adjoint_vector_view {
+    const double* x;
+    const double* y;
+    const double* z;
+} phx_const_adjoint_vector_view;
+
+typedef struct phx_control_tangent {
+    phx_const_vector_view external_accelerations;
+} phx_control_tangent;
+
+typedef struct phx_particle_parameter_tangent {
+    const float* masses;
+    const float* rest_densities;
+    const float* viscosities;
+    const float* surface_tensions;
+} phx_particle_parameter_tangent;
+
+typedef struct phx_control_adjoint {
+    phx_adjoint_vector_view external_accelerations;
+} phx_control_adjoint;
+
+typedef struct phx_particle_parameter_adjoint {
+    double* masses;
+    double* rest_densities;
+    double* viscosities;
+    double* surface_tensions;
+} phx_particle_parameter_adjoint;
+
 /* Grid resolution ceil((max - min) / h) per axis (neighborhood.cpp:25-30). */
 phx_status phx_grid_resolution(const phx_domain_configuration* domain, uint32_t resolution[3]);
 
@@ -323,6 +359,38 @@ typedef struct phx_pbf_parameters {
     const float* vorticity_confinement;
 } phx_pbf_parameters;
 
+/* PBF::StateTangent / ParameterTangent / StateAdjoint / ParameterAdjoint
+ * (pbf/pbf.ixx:20-51). */
+typedef struct phx_pbf_state_tangent {
+    phx_vector_view positions;
+    phx_vector_view velocities;
+} phx_pbf_state_tangent;
+
+typedef struct phx_pbf_parameter_tangent {
+    phx_particle_parameter_tangent particles;
+    const float* relaxation;
+    const float* artificial_pressure_strength;
+    const float* artificial_pressure_exponent;
+    const float* artificial_pressure_radius;
+    const float* xsph_viscosity;
+    const float* vorticity_confinement;
+} phx_pbf_parameter_tangent;
+
+typedef struct phx_pbf_state_adjoint {
+    phx_adjoint_vector_view positions;
+    phx_adjoint_vector_view velocities;
+} phx_pbf_state_adjoint;
+
+typedef struct phx_pbf_parameter_adjoint {
+    phx_particle_parameter_adjoint particles;
+    double* relaxation;
+    double* artificial_pressure_strength;
+    double* artificial_pressure_exponent;
+    double* artificial_pressure_radius;
+    double* xsph_viscosity;
+    double* vorticity_confinement;
+} phx_pbf_parameter_adjoint;
+
 typedef struct phx_pbf phx_pbf;
 typedef struct phx_pbf_step_cache phx_pbf_step_cache;
 
@@ -336,6 +404,26 @@ phx_status phx_pbf_free_step_cache(phx_pbf_step_cache* cache);
 phx_status phx_pbf_forward_step(phx_pbf* solver, const phx_pbf_state* state, const phx_control* control,
                                 const phx_pbf_parameters* parameters, phx_pbf_state* next_state,
                                 phx_pbf_step_cache* cache);
+/* jvp_step(state, control, parameters, next_state, cache, state_tangent,
+ * control_tangent, parameter_tangent, next_state_tangent), pbf/pbf.ixx:93 and
+ * pbf/pbf.cpp:193-215. Needs a solver created in PHX_MODE_DIFFERENTIABLE and
+ * the cache of that step's forward_step (state, control and next state are
+ * read from the cache). Tangents are FP32 and overwrite next_state_tangent;
+ * null parameter-tangent arrays count as zero. */
+phx_status phx_pbf_jvp_step(phx_pbf* solver, const phx_pbf_parameters* parameters, const phx_pbf_step_cache* cache,
+                            const phx_pbf_state_tangent* state_tangent, const phx_control_tangent* control_tangent,
+                            const phx_pbf_parameter_tangent* parameter_tangent,
+                            phx_pbf_state_tangent* next_state_tangent);
+/* vjp_step(..., next_state_adjoint, previous_state_adjoint, control_adjoint,
+ * parameter_adjoint), pbf/pbf.ixx:94 and pbf/pbf.cpp:217-258: the exact VJP
+ * of the recorded step with its neighbour graph and collision masks frozen,
+ * in FP64. Every output is accumulated (+=); null adjoint arrays are
+ * skipped. The adjoint of next_state positions and velocities is read only. */
+phx_status phx_pbf_vjp_step(phx_pbf* solver, const phx_pbf_parameters* parameters, const phx_pbf_step_cache* cache,
+                            const phx_pbf_state_adjoint* next_state_adjoint,
+                            phx_pbf_state_adjoint* previous_state_adjoint, phx_control_adjoint* control_adjoint,
+                            phx_pbf_parameter_adjoint* parameter_adjoint);
+
 phx_status phx_pbf_set_profiling(phx_pbf* solver, int enabled);
 phx_status phx_pbf_read_profile(phx_pbf* solver, phx_pass_profile* profile, uint32_t capacity, uint32_t* count);
 /* StepCache fields (pbf/pbf.ixx:54-70), original particle order. */

@@ -139,6 +139,28 @@ class SphDfsphParameters(ctypes.Structure):
     _fields_ = [("particles", ParticleParameters), ("method", DfsphParameters)]
 
 
+class AdjointVectorView(ctypes.Structure):
+    _fields_ = [("x", c_void_p), ("y", c_void_p), ("z", c_void_p)]
+
+
+class ControlTangent(ctypes.Structure):
+    _fields_ = [("external_accelerations", VectorView)]
+
+
+class ParticleParameterTangent(ctypes.Structure):
+    _fields_ = [("masses", c_void_p), ("rest_densities", c_void_p), ("viscosities", c_void_p),
+                ("surface_tensions", c_void_p)]
+
+
+class ControlAdjoint(ctypes.Structure):
+    _fields_ = [("external_accelerations", AdjointVectorView)]
+
+
+class ParticleParameterAdjoint(ctypes.Structure):
+    _fields_ = [("masses", c_void_p), ("rest_densities", c_void_p), ("viscosities", c_void_p),
+                ("surface_tensions", c_void_p)]
+
+
 PHX_SPH_WCSPH, PHX_SPH_PCISPH, PHX_SPH_IISPH = 0, 1, 2
 
 
@@ -182,6 +204,26 @@ class PbfParameters(ctypes.Structure):
         ("xsph_viscosity", c_void_p),
         ("vorticity_confinement", c_void_p),
     ]
+
+
+_PBF_FIELDS = ("relaxation", "artificial_pressure_strength", "artificial_pressure_exponent",
+               "artificial_pressure_radius", "xsph_viscosity", "vorticity_confinement")
+
+
+class PbfStateTangent(ctypes.Structure):
+    _fields_ = [("positions", VectorView), ("velocities", VectorView)]
+
+
+class PbfParameterTangent(ctypes.Structure):
+    _fields_ = [("particles", ParticleParameterTangent)] + [(k, c_void_p) for k in _PBF_FIELDS]
+
+
+class PbfStateAdjoint(ctypes.Structure):
+    _fields_ = [("positions", AdjointVectorView), ("velocities", AdjointVectorView)]
+
+
+class PbfParameterAdjoint(ctypes.Structure):
+    _fields_ = [("particles", ParticleParameterAdjoint)] + [(k, c_void_p) for k in _PBF_FIELDS]
 
 
 # name -> (restype, argtypes); every function declared in physica.h
@@ -246,6 +288,16 @@ PROTOTYPES = {
     "phx_pbf_forward_step": (
         c_int,
         [c_void_p, POINTER(PbfState), POINTER(Control), POINTER(PbfParameters), POINTER(PbfState), c_void_p],
+    ),
+    "phx_pbf_jvp_step": (
+        c_int,
+        [c_void_p, POINTER(PbfParameters), c_void_p, POINTER(PbfStateTangent), POINTER(ControlTangent),
+         POINTER(PbfParameterTangent), POINTER(PbfStateTangent)],
+    ),
+    "phx_pbf_vjp_step": (
+        c_int,
+        [c_void_p, POINTER(PbfParameters), c_void_p, POINTER(PbfStateAdjoint), POINTER(PbfStateAdjoint),
+         POINTER(ControlAdjoint), POINTER(PbfParameterAdjoint)],
     ),
     "phx_pbf_set_profiling": (c_int, [c_void_p, c_int]),
     "phx_pbf_read_profile": (c_int, [c_void_p, POINTER(PassProfile), c_uint32, POINTER(c_uint32)]),

@@ -484,8 +484,10 @@ phx_status phx_pbf_allocate_step_cache(phx_pbf* solver, phx_pbf_step_cache** cac
         *cache = nullptr;
         auto* c = new phx_pbf_step_cache();
         c->owner = solver;
+        c->impl.record = solver->impl.mode() == PHX_MODE_DIFFERENTIABLE;
         try {
-            c->impl.allocate(solver->impl.domain().config.particle_count, solver->impl.domain().cells);
+            c->impl.allocate(solver->impl.domain().config.particle_count, solver->impl.domain().cells,
+                             solver->impl.iterations());
         } catch (...) {
             delete c;
             throw;
@@ -508,6 +510,36 @@ phx_status phx_pbf_forward_step(phx_pbf* solver, const phx_pbf_state* state, con
         cache->valid = false;
         solver->impl.forward_step(*state, *control, *parameters, *next_state, cache->impl);
         cache->valid = true;
+    });
+}
+
+phx_status phx_pbf_jvp_step(phx_pbf* solver, const phx_pbf_parameters* parameters, const phx_pbf_step_cache* cache,
+                            const phx_pbf_state_tangent* state_tangent, const phx_control_tangent* control_tangent,
+                            const phx_pbf_parameter_tangent* parameter_tangent,
+                            phx_pbf_state_tangent* next_state_tangent) {
+    return guarded([&] {
+        phx::require(solver && parameters && cache && state_tangent && control_tangent && parameter_tangent &&
+                         next_state_tangent,
+                     PHX_ERROR_INVALID_ARGUMENT, "null argument");
+        phx::require(cache->owner == solver && cache->valid, PHX_ERROR_INVALID_ARGUMENT,
+                     "step cache holds no step of this solver");
+        solver->impl.jvp_step(*parameters, cache->impl, *state_tangent, *control_tangent, *parameter_tangent,
+                              *next_state_tangent);
+    });
+}
+
+phx_status phx_pbf_vjp_step(phx_pbf* solver, const phx_pbf_parameters* parameters, const phx_pbf_step_cache* cache,
+                            const phx_pbf_state_adjoint* next_state_adjoint,
+                            phx_pbf_state_adjoint* previous_state_adjoint, phx_control_adjoint* control_adjoint,
+                            phx_pbf_parameter_adjoint* parameter_adjoint) {
+    return guarded([&] {
+        phx::require(solver && parameters && cache && next_state_adjoint && previous_state_adjoint && control_adjoint &&
+                         parameter_adjoint,
+                     PHX_ERROR_INVALID_ARGUMENT, "null argument");
+        phx::require(cache->owner == solver && cache->valid, PHX_ERROR_INVALID_ARGUMENT,
+                     "step cache holds no step of this solver");
+        solver->impl.vjp_step(*parameters, cache->impl, *next_state_adjoint, *previous_state_adjoint,
+                              *control_adjoint, *parameter_adjoint);
     });
 }
 

@@ -59,6 +59,15 @@ __device__ __forceinline__ float poly6(Float3 d, float h) {
     return 315.0F * diff * diff * diff / (64.0F * kPi * hr * hr * hr * h * h * h);
 }
 
+// poly6 gradient, density/device.cuh:18-24
+__device__ __forceinline__ Float3 poly6_gradient(Float3 d, float h) {
+    const float sq = dot(d, d);
+    const float hr = h * h;
+    if (sq >= hr) return {0.0F, 0.0F, 0.0F};
+    const float diff = hr - sq;
+    return scale(d, -945.0F * diff * diff / (32.0F * kPi * hr * hr * hr * h * h * h));
+}
+
 // cubic spline, density/device.cuh:26-34
 __device__ __forceinline__ float cubic(Float3 d, float h) {
     const float dist = length(d);

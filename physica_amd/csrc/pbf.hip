@@ -31,6 +31,8 @@ struct LambdaArgs {
     const float* eps;
     float* rho;
     float* lambda;
+    float* den;  // optional (recorded step)
+    Vec gsum;    // optional (recorded step)
 };
 
 __global__ void __launch_bounds__(kBlock) pbf_lambda_kernel(LambdaArgs a) {
@@ -56,6 +58,10 @@ __global__ void __launch_bounds__(kBlock) pbf_lambda_kernel(LambdaArgs a) {
     denominator += dot(gradient_sum, gradient_sum);
     a.rho[i] = density;
     a.lambda[i] = -(density / rest - 1.0F) / denominator;
+    if (a.den != nullptr) {
+        a.den[i] = denominator;
+        store(a.gsum, i, gradient_sum);
+    }
 }
 
 struct CorrectionArgs {
@@ -73,6 +79,7 @@ struct CorrectionArgs {
     const float* exponent;
     const float* radius;
     Vec out;
+    uint8_t* mask;  // optional (recorded step): box-collision bits of the projection
     uint32_t reconstruct;
     CVec x0;
     float inv_dt;
@@ -99,6 +106,13 @@ __global__ void __launch_bounds__(kBlock) pbf_correction_kernel(CorrectionArgs a
         correction = add(correction, scale(spiky_gradient(d, a.h, a.spiky), coefficient * (lambda_i + a.lambda[s] + artificial)));
     });
     const Float3 candidate = add(p, correction);
+    if (a.mask != nullptr) {
+        uint32_t bits = 0u;
+        if (candidate.x < a.box.lo[0] || candidate.x > a.box.hi[0]) bits |= 1u;
+        if (candidate.y < a.box.lo[1] || candidate.y > a.box.hi[1]) bits |= 2u;
+        if (candidate.z < a.box.lo[2] || candidate.z > a.box.hi[2]) bits |= 4u;
+        a.mask[i] = static_cast<uint8_t>(bits);
+    }
     const Float3 projected{fminf(a.box.hi[0], fmaxf(a.box.lo[0], candidate.x)),
                            fminf(a.box.hi[1], fmaxf(a.box.lo[1], candidate.y)),
                            fminf(a.box.hi[2], fmaxf(a.box.lo[2], candidate.z))};
@@ -125,6 +139,8 @@ struct PostArgs {
     float* magnitude;
     const float* confinement;
     Vec vc;
+    Vec normals;         // optional (recorded step)
+    float* normalizers;  // optional (recorded step)
     const float* xsph;
     const uint32_t* perm;
     Vec next_x, next_v;
@@ -160,6 +176,10 @@ __global__ void __launch_bounds__(kBlock) pbf_normal_kernel(PostArgs a) {
     });
     const float normalizer = length(g);
     const Float3 normal = normalizer > 0.0F ? scale(g, 1.0F / normalizer) : Float3{0.0F, 0.0F, 0.0F};
+    if (a.normalizers != nullptr) {
+        a.normalizers[i] = normalizer;
+        store(a.normals, i, normal);
+    }
     const Float3 w = load(CVec{a.vort.x, a.vort.y, a.vort.z}, i);
     store(a.vc, i, add(load(a.vel, i), scale(cross(normal, w), a.dt * a.confinement[i])));
 }
@@ -185,8 +205,9 @@ __global__ void __launch_bounds__(kBlock) pbf_xsph_kernel(PostArgs a) {
 }  // namespace
 
 const char* pbf_pass_name(int pass) {
-    static const char* names[kPbfPassCount] = {"grid", "density_lambda", "correction_project", "vorticity",
-                                               "normal_confinement", "xsph"};
+    static const char* names[kPbfPassCount] = {"grid",      "density_lambda",     "correction_project",
+                                               "vorticity", "normal_confinement", "xsph",
+                                               "jvp",       "vjp"};
     return pass >= 0 && pass < kPbfPassCount ? names[pass] : "unknown";
 }
 
@@ -205,16 +226,30 @@ double pbf_pass_bytes(int pass, uint32_t particles, uint32_t cells) {
             return 56.0 + 12.0 + offsets;
         case kPbfXsph:  // x*, p, v_c, xsph in; next x, v out + permutation
             return 40.0 + 24.0 + 4.0 + offsets;
+        case kPbfJvp:  // whole-step passes, not a single roofline figure
+        case kPbfVjp:
+            return 0.0;
         default:
             return 0.0;
     }
 }
 
-void PbfCache::allocate(uint32_t n, uint32_t cells) {
+void PbfCache::allocate(uint32_t n, uint32_t cells, uint32_t pressure_iterations) {
     neighborhood.allocate(n, cells);
     predicted.resize(n);
     densities.resize(n);
     lambdas.resize(n);
+    if (!record) return;
+    const size_t per = static_cast<size_t>(pressure_iterations) * n;
+    for (VecBuffer* b : {&xs, &x0, &velocities, &vorticities, &normals, &confined}) b->resize(n);
+    positions.resize(n, pressure_iterations);
+    gradient_sums.resize(n, pressure_iterations);
+    rho.resize(per);
+    lambda.resize(per);
+    den.resize(per);
+    masks.resize(per);
+    magnitudes.resize(n);
+    normalizers.resize(n);
 }
 
 PbfSolver::PbfSolver(const phx_domain_configuration* domain, const phx_pbf_configuration* configuration,
@@ -223,7 +258,8 @@ PbfSolver::PbfSolver(const phx_domain_configuration* domain, const phx_pbf_confi
     require(configuration != nullptr, PHX_ERROR_INVALID_ARGUMENT, "null PBF configuration");
     config_ = *configuration;
     require(config_.checkpoint_interval > 0u, PHX_ERROR_INVALID_ARGUMENT, "checkpoint_interval must be positive");
-    require(mode == PHX_MODE_FORWARD, PHX_ERROR_UNSUPPORTED, "differentiable mode is not supported yet");
+    require(mode == PHX_MODE_FORWARD || mode == PHX_MODE_DIFFERENTIABLE, PHX_ERROR_INVALID_ARGUMENT,
+            "unknown execution mode");
     spiky_ = spiky_coefficient(domain_.config.support_radius);
     const uint32_t n = domain_.config.particle_count;
     for (VecBuffer* b : {&xs_, &x0_, &pa_, &pb_, &vel_, &vort_, &vc_}) b->resize(n);
@@ -250,7 +286,17 @@ void PbfSolver::forward_step(const phx_pbf_state& state, const phx_control& cont
     require(parameters.relaxation && parameters.artificial_pressure_strength && parameters.artificial_pressure_exponent &&
                 parameters.artificial_pressure_radius && parameters.xsph_viscosity && parameters.vorticity_confinement,
             PHX_ERROR_INVALID_ARGUMENT, "null PBF parameters");
-    cache.allocate(n, domain_.cells);
+    cache.record = mode_ == PHX_MODE_DIFFERENTIABLE;
+    cache.recorded = false;
+    cache.allocate(n, domain_.cells, config_.pressure_iterations);
+    const bool rec = cache.record;
+    // recorded quantities live in the cache, the others in the workspace
+    VecBuffer& xs = rec ? cache.xs : xs_;
+    VecBuffer& x0 = rec ? cache.x0 : x0_;
+    VecBuffer& vel = rec ? cache.velocities : vel_;
+    VecBuffer& vort = rec ? cache.vorticities : vort_;
+    VecBuffer& vc = rec ? cache.confined : vc_;
+    float* magnitude = rec ? cache.magnitudes.data() : magnitude_.data();
 
     // predict (pbf.cu:73-78) fused into the key pass of build(step + 1, x*)
     PredictInput predict{};
@@ -260,8 +306,8 @@ void PbfSolver::forward_step(const phx_pbf_state& state, const phx_control& cont
     predict.dt = c.time_step;
     predict.predicted = cache.predicted.view();
     ChannelList gather;
-    gather.add(cache.predicted.cview(), xs_.view());
-    gather.add(cvec(state.particles.positions), x0_.view());
+    gather.add(cache.predicted.cview(), xs.view());
+    gather.add(cvec(state.particles.positions), x0.view());
     gather.add(pp.masses, m_.data());
     gather.add(pp.rest_densities, rho0_.data());
     gather.add(parameters.relaxation, eps_.data());
@@ -281,49 +327,67 @@ void PbfSolver::forward_step(const phx_pbf_state& state, const phx_control& cont
     const float h = c.support_radius;
     const float inv_dt = 1.0F / c.time_step;
 
-    LambdaArgs la{n, h, spiky_, grid, xs_.cview(), xs_.cview(), m_.data(), rho0_.data(), eps_.data(),
-                  cache.densities.data(), cache.lambdas.data()};
+    LambdaArgs la{n, h, spiky_, grid, xs.cview(), xs.cview(), m_.data(), rho0_.data(), eps_.data(),
+                  cache.densities.data(), cache.lambdas.data(), nullptr, Vec{}};
     CorrectionArgs ca{};
     ca.n = n;
     ca.h = h;
     ca.spiky = spiky_;
     ca.grid = grid;
     ca.box = collision_box(c, step + 1u);
-    ca.topo = xs_.cview();
+    ca.topo = xs.cview();
     ca.m = m_.data();
     ca.rho0 = rho0_.data();
     ca.lambda = cache.lambdas.data();
     ca.strength = strength_.data();
     ca.exponent = exponent_.data();
     ca.radius = radius_.data();
-    ca.x0 = x0_.cview();
+    ca.x0 = x0.cview();
     ca.inv_dt = inv_dt;
-    ca.vel = vel_.view();
+    ca.vel = vel.view();
 
     // positions start at x* (pbf.cpp:171) and ping-pong between two buffers
-    CVec current = xs_.cview();
+    // (a recorded step keeps every iterate)
+    CVec current = xs.cview();
     VecBuffer* targets[2] = {&pa_, &pb_};
     const uint32_t iterations = config_.pressure_iterations;
     for (uint32_t it = 0; it < iterations; ++it) {
         la.pos = current;
+        if (rec) {
+            const size_t off = static_cast<size_t>(it) * n;
+            la.rho = cache.rho.data() + off;
+            la.lambda = cache.lambda.data() + off;
+            la.den = cache.den.data() + off;
+            la.gsum = cache.gradient_sums.view(it);
+            ca.lambda = la.lambda;
+            ca.mask = cache.masks.data() + off;
+        }
         {
             PassScope scope(profiler_, stream_, kPbfLambda);
             pbf_lambda_kernel<<<blocks, kBlock, 0, stream_>>>(la);
             PHX_LAUNCH_CHECK();
         }
-        VecBuffer* target = targets[it & 1u];
+        const Vec target = rec ? cache.positions.view(it) : targets[it & 1u]->view();
         ca.pos = current;
-        ca.out = target->view();
+        ca.out = target;
         ca.reconstruct = it + 1u == iterations ? 1u : 0u;
         {
             PassScope scope(profiler_, stream_, kPbfCorrection);
             pbf_correction_kernel<<<blocks, kBlock, 0, stream_>>>(ca);
             PHX_LAUNCH_CHECK();
         }
-        current = target->cview();
+        current = CVec{target.x, target.y, target.z};
+    }
+    if (rec && iterations > 0u) {
+        // the cache's densities and lambdas are those of the last iteration
+        const size_t off = static_cast<size_t>(iterations - 1u) * n;
+        PHX_HIP(hipMemcpyAsync(cache.densities.data(), cache.rho.data() + off, sizeof(float) * n,
+                               hipMemcpyDeviceToDevice, stream_));
+        PHX_HIP(hipMemcpyAsync(cache.lambdas.data(), cache.lambda.data() + off, sizeof(float) * n,
+                               hipMemcpyDeviceToDevice, stream_));
     }
     if (iterations == 0u) {
-        pbf_reconstruct_kernel<<<blocks, kBlock, 0, stream_>>>(n, current, x0_.cview(), inv_dt, vel_.view());
+        pbf_reconstruct_kernel<<<blocks, kBlock, 0, stream_>>>(n, current, x0.cview(), inv_dt, vel.view());
         PHX_LAUNCH_CHECK();
     }
 
@@ -333,13 +397,17 @@ void PbfSolver::forward_step(const phx_pbf_state& state, const phx_control& cont
     post.spiky = spiky_;
     post.dt = c.time_step;
     post.grid = grid;
-    post.topo = xs_.cview();
+    post.topo = xs.cview();
     post.pos = current;
-    post.vel = vel_.cview();
-    post.vort = vort_.view();
-    post.magnitude = magnitude_.data();
+    post.vel = vel.cview();
+    post.vort = vort.view();
+    post.magnitude = magnitude;
     post.confinement = confinement_.data();
-    post.vc = vc_.view();
+    post.vc = vc.view();
+    if (rec) {
+        post.normals = cache.normals.view();
+        post.normalizers = cache.normalizers.data();
+    }
     post.xsph = xsph_.data();
     post.perm = cache.neighborhood.perm.data();
     post.next_x = vec(next.particles.positions);
@@ -358,6 +426,11 @@ void PbfSolver::forward_step(const phx_pbf_state& state, const phx_control& cont
         PassScope scope(profiler_, stream_, kPbfXsph);
         pbf_xsph_kernel<<<blocks, kBlock, 0, stream_>>>(post);
         PHX_LAUNCH_CHECK();
+    }
+    if (rec) {
+        cache.recorded = true;
+        cache.iterations = iterations;
+        cache.step_index = step;
     }
     next.particles.step_index = step + 1u;
 }

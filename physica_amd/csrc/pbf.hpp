@@ -1,4 +1,5 @@
-// PBF solver (pbf/pbf.ixx:14-141) on the sorted-order engine.
+// PBF solver (pbf/pbf.ixx:14-141) on the sorted-order engine, with the
+// forward-mode and reverse-mode derivatives of one step (pbf/pbf.cpp:193-258).
 #pragma once
 
 #include "engine.hpp"
@@ -6,15 +7,67 @@
 
 namespace phx {
 
+// FP64 SoA vector field (the adjoint views of domain/device.h).
+struct DVec {
+    double* x;
+    double* y;
+    double* z;
+};
+
+struct DVecBuffer {
+    DeviceBuffer<double> x, y, z;
+    void resize(size_t n) {
+        x.resize(n);
+        y.resize(n);
+        z.resize(n);
+    }
+    DVec view() { return {x.data(), y.data(), z.data()}; }
+};
+
+// `count` vector fields of n particles in one allocation; field k is
+// [3k n, 3k n + 3 n).
+struct VecStack {
+    DeviceBuffer<float> data;
+    size_t n = 0;
+    void resize(size_t particles, size_t count) {
+        n = particles;
+        data.resize(3u * particles * count);
+    }
+    Vec view(size_t k) {
+        float* b = data.data() + 3u * n * k;
+        return {b, b + n, b + 2u * n};
+    }
+    CVec cview(size_t k) const {
+        const float* b = data.data() + 3u * n * k;
+        return {b, b + n, b + 2u * n};
+    }
+};
+
 // StepCache of PBF (pbf/pbf.ixx:54-70): neighbourhood built on the predicted
 // positions, the predicted positions (original order) and the densities and
-// lambdas of the last constraint iteration (sorted order).
+// lambdas of the last constraint iteration (sorted order). A differentiable
+// solver also records every intermediate of the step in sorted order, so its
+// jvp_step and vjp_step read them instead of recomputing constraint
+// iterations from checkpoints.
 struct PbfCache {
     Neighborhood neighborhood;
     VecBuffer predicted;  // original order
     DeviceBuffer<float> densities;
     DeviceBuffer<float> lambdas;
-    void allocate(uint32_t n, uint32_t cells);
+
+    bool record = false;  // set by the solver in differentiable mode
+    bool recorded = false;
+    uint32_t iterations = 0;
+    uint64_t step_index = 0;
+    VecBuffer xs, x0;                      // x* (topology and first iterate), state positions
+    VecStack positions;                    // position after constraint iteration k
+    DeviceBuffer<float> rho, lambda, den;  // [iterations][n]
+    VecStack gradient_sums;                // sum_j (m_j / rho0_i) grad W_ij per iteration
+    DeviceBuffer<uint8_t> masks;           // [iterations][n] box-collision bits of the projection
+    VecBuffer velocities, vorticities, normals, confined;
+    DeviceBuffer<float> magnitudes, normalizers;
+
+    void allocate(uint32_t n, uint32_t cells, uint32_t pressure_iterations);
 };
 
 enum PbfPass : int {
@@ -24,10 +77,26 @@ enum PbfPass : int {
     kPbfVorticity,
     kPbfNormal,
     kPbfXsph,
+    kPbfJvp,
+    kPbfVjp,
     kPbfPassCount
 };
 const char* pbf_pass_name(int pass);
 double pbf_pass_bytes(int pass, uint32_t particles, uint32_t cells);
+
+// Sorted-order parameter and tangent workspace of the derivative passes.
+struct PbfDerivativeWorkspace {
+    DeviceBuffer<float> m, rho0, eps, strength, exponent, radius, xsph, confinement;
+    // tangents (FP32) of parameters, state and control, sorted
+    DeviceBuffer<float> m_t, rho0_t, eps_t, strength_t, exponent_t, radius_t, xsph_t, confinement_t;
+    VecBuffer x_t, v_t, u_t, p_t[2], vrec_t, w_t, vc_t;
+    DeviceBuffer<float> rho_t, lambda_t, magnitude_t;
+    // adjoints (FP64), sorted
+    DVecBuffer p_a[2], vrec_a, w_a, g_a, next_x_a, next_v_a, x_a, vel_a;
+    DeviceBuffer<double> lambda_a, magnitude_a;
+    DeviceBuffer<double> m_a, rho0_a, eps_a, strength_a, exponent_a, radius_a, xsph_a, confinement_a;
+    void allocate(uint32_t n);
+};
 
 class PbfSolver {
   public:
@@ -35,13 +104,22 @@ class PbfSolver {
               phx_execution_mode mode, hipStream_t stream);
     void forward_step(const phx_pbf_state& state, const phx_control& control, const phx_pbf_parameters& parameters,
                       phx_pbf_state& next, PbfCache& cache);
+    void jvp_step(const phx_pbf_parameters& parameters, const PbfCache& cache, const phx_pbf_state_tangent& state_t,
+                  const phx_control_tangent& control_t, const phx_pbf_parameter_tangent& parameter_t,
+                  phx_pbf_state_tangent& next_t);
+    void vjp_step(const phx_pbf_parameters& parameters, const PbfCache& cache, const phx_pbf_state_adjoint& next_a,
+                  phx_pbf_state_adjoint& previous_a, phx_control_adjoint& control_a,
+                  phx_pbf_parameter_adjoint& parameter_a);
 
     const Domain& domain() const { return domain_; }
     hipStream_t stream() const { return stream_; }
     PassProfiler& profiler() { return profiler_; }
     phx_execution_mode mode() const { return mode_; }
+    uint32_t iterations() const { return config_.pressure_iterations; }
 
   private:
+    void gather_parameters(const phx_pbf_parameters& parameters, const PbfCache& cache);
+
     Domain domain_;
     phx_pbf_configuration config_{};
     phx_execution_mode mode_;
@@ -51,6 +129,7 @@ class PbfSolver {
     // sorted-order workspace
     VecBuffer xs_, x0_, pa_, pb_, vel_, vort_, vc_;
     DeviceBuffer<float> m_, rho0_, eps_, strength_, exponent_, radius_, xsph_, confinement_, magnitude_;
+    PbfDerivativeWorkspace dw_;
 };
 
 }  // namespace phx

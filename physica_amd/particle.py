@@ -121,6 +121,107 @@ class VectorField:
 
 
 @dataclass
+class VectorAdjointField:
+    """VectorAdjointField (FP64 SoA x|y|z), domain/domain.ixx:64-68."""
+
+    x: torch.Tensor
+    y: torch.Tensor
+    z: torch.Tensor
+
+    def view(self) -> _capi.AdjointVectorView:
+        return _capi.AdjointVectorView(_ptr(self.x), _ptr(self.y), _ptr(self.z))
+
+    def numpy(self) -> np.ndarray:
+        return np.stack([self.x.cpu().numpy(), self.y.cpu().numpy(), self.z.cpu().numpy()])
+
+    def copy_(self, other: "VectorAdjointField") -> None:
+        self.x.copy_(other.x)
+        self.y.copy_(other.y)
+        self.z.copy_(other.z)
+
+    def add_(self, other: "VectorAdjointField") -> None:
+        """Domain::accumulate (domain/domain.cu:8-29)."""
+        self.x.add_(other.x)
+        self.y.add_(other.y)
+        self.z.add_(other.z)
+
+    def zero_(self) -> None:
+        self.x.zero_()
+        self.y.zero_()
+        self.z.zero_()
+
+    def upload(self, values: np.ndarray) -> None:
+        values = np.asarray(values, dtype=np.float64)
+        self.x.copy_(torch.from_numpy(np.ascontiguousarray(values[0])))
+        self.y.copy_(torch.from_numpy(np.ascontiguousarray(values[1])))
+        self.z.copy_(torch.from_numpy(np.ascontiguousarray(values[2])))
+
+    @staticmethod
+    def zeros(count: int, device) -> "VectorAdjointField":
+        return VectorAdjointField(*(torch.zeros(count, dtype=torch.float64, device=device) for _ in range(3)))
+
+
+@dataclass
+class ParticleStateTangent:
+    """ParticleStateTangent, domain/domain.ixx:76-79."""
+
+    positions: VectorField
+    velocities: VectorField
+
+
+@dataclass
+class ParticleStateAdjoint:
+    """ParticleStateAdjoint, domain/domain.ixx:85-88."""
+
+    positions: VectorAdjointField
+    velocities: VectorAdjointField
+
+
+@dataclass
+class ControlTangent:
+    external_accelerations: VectorField
+
+    def to_c(self) -> _capi.ControlTangent:
+        return _capi.ControlTangent(self.external_accelerations.view())
+
+
+@dataclass
+class ControlAdjoint:
+    external_accelerations: VectorAdjointField
+
+    def to_c(self) -> _capi.ControlAdjoint:
+        return _capi.ControlAdjoint(self.external_accelerations.view())
+
+
+@dataclass
+class ParticleParameterTangent:
+    """ParticleParameterTangent (FP32), domain/domain.ixx:101-106."""
+
+    masses: torch.Tensor
+    rest_densities: torch.Tensor
+    viscosities: torch.Tensor
+    surface_tensions: torch.Tensor
+
+    def to_c(self) -> _capi.ParticleParameterTangent:
+        return _capi.ParticleParameterTangent(_ptr(self.masses), _ptr(self.rest_densities), _ptr(self.viscosities),
+                                              _ptr(self.surface_tensions))
+
+
+@dataclass
+class ParticleParameterAdjoint:
+    """ParticleParameterAdjoint (FP64), domain/domain.ixx:108-113."""
+
+    masses: torch.Tensor
+    rest_densities: torch.Tensor
+    viscosities: torch.Tensor
+    surface_tensions: torch.Tensor
+
+    def to_c(self) -> _capi.ParticleParameterAdjoint:
+        return _capi.ParticleParameterAdjoint(_ptr(self.masses), _ptr(self.rest_densities), _ptr(self.viscosities),
+                                              _ptr(self.surface_tensions))
+
+
+@dataclass
 class ParticleState:
     """ParticleState, domain/domain.ixx:70-74."""
 
@@ -167,6 +268,14 @@ class Domain:
     def allocate_scalar_field(self, count: Optional[int] = None) -> torch.Tensor:
         n = self.configuration.particle_count if count is None else count
         return torch.zeros(n, dtype=torch.float32, device=self.device)
+
+    def allocate_scalar_adjoint_field(self, count: Optional[int] = None) -> torch.Tensor:
+        n = self.configuration.particle_count if count is None else count
+        return torch.zeros(n, dtype=torch.float64, device=self.device)
+
+    def allocate_vector_adjoint_field(self, count: Optional[int] = None) -> VectorAdjointField:
+        n = self.configuration.particle_count if count is None else count
+        return VectorAdjointField.zeros(n, self.device)
 
     def allocate_vector_field(self, count: Optional[int] = None) -> VectorField:
         n = self.configuration.particle_count if count is None else count
@@ -531,6 +640,46 @@ class PBF:
                 _ptr(self.vorticity_confinement),
             )
 
+    @dataclass
+    class StateTangent:
+        particles: ParticleStateTangent
+
+        def to_c(self) -> _capi.PbfStateTangent:
+            return _capi.PbfStateTangent(self.particles.positions.view(), self.particles.velocities.view())
+
+    @dataclass
+    class StateAdjoint:
+        particles: ParticleStateAdjoint
+
+        def to_c(self) -> _capi.PbfStateAdjoint:
+            return _capi.PbfStateAdjoint(self.particles.positions.view(), self.particles.velocities.view())
+
+    @dataclass
+    class ParameterTangent:
+        particles: ParticleParameterTangent
+        relaxation: torch.Tensor
+        artificial_pressure_strength: torch.Tensor
+        artificial_pressure_exponent: torch.Tensor
+        artificial_pressure_radius: torch.Tensor
+        xsph_viscosity: torch.Tensor
+        vorticity_confinement: torch.Tensor
+
+        def to_c(self) -> _capi.PbfParameterTangent:
+            return _capi.PbfParameterTangent(self.particles.to_c(), *(_ptr(getattr(self, k)) for k in _capi._PBF_FIELDS))
+
+    @dataclass
+    class ParameterAdjoint:
+        particles: ParticleParameterAdjoint
+        relaxation: torch.Tensor
+        artificial_pressure_strength: torch.Tensor
+        artificial_pressure_exponent: torch.Tensor
+        artificial_pressure_radius: torch.Tensor
+        xsph_viscosity: torch.Tensor
+        vorticity_confinement: torch.Tensor
+
+        def to_c(self) -> _capi.PbfParameterAdjoint:
+            return _capi.PbfParameterAdjoint(self.particles.to_c(), *(_ptr(getattr(self, k)) for k in _capi._PBF_FIELDS))
+
     class StepCache:
         """StepCache (pbf/pbf.ixx:54-70); fields are read back in original order."""
 
@@ -608,6 +757,73 @@ class PBF:
 
     def copy_state(self, source: "PBF.State", destination: "PBF.State") -> None:
         self.domain.copy(source.particles, destination.particles)
+
+    # tangents and adjoints (pbf/pbf.ixx:84-94); allocated zero
+    def allocate_state_tangent(self) -> "PBF.StateTangent":
+        d = self.domain
+        return PBF.StateTangent(ParticleStateTangent(VectorField.zeros(d.configuration.particle_count, d.device),
+                                                     VectorField.zeros(d.configuration.particle_count, d.device)))
+
+    def allocate_control_tangent(self) -> ControlTangent:
+        return ControlTangent(VectorField.zeros(self.domain.configuration.particle_count, self.domain.device))
+
+    def allocate_parameter_tangent(self) -> "PBF.ParameterTangent":
+        d = self.domain
+        z = lambda: torch.zeros(d.configuration.particle_count, dtype=torch.float32, device=d.device)  # noqa: E731
+        return PBF.ParameterTangent(ParticleParameterTangent(z(), z(), z(), z()), *(z() for _ in range(6)))
+
+    def allocate_state_adjoint(self) -> "PBF.StateAdjoint":
+        d = self.domain
+        return PBF.StateAdjoint(ParticleStateAdjoint(d.allocate_vector_adjoint_field(), d.allocate_vector_adjoint_field()))
+
+    def allocate_control_adjoint(self) -> ControlAdjoint:
+        return ControlAdjoint(self.domain.allocate_vector_adjoint_field())
+
+    def allocate_parameter_adjoint(self) -> "PBF.ParameterAdjoint":
+        d = self.domain
+        z = d.allocate_scalar_adjoint_field
+        return PBF.ParameterAdjoint(ParticleParameterAdjoint(z(), z(), z(), z()), *(z() for _ in range(6)))
+
+    def copy_state_tangent(self, source: "PBF.StateTangent", destination: "PBF.StateTangent") -> None:
+        destination.particles.positions.copy_(source.particles.positions)
+        destination.particles.velocities.copy_(source.particles.velocities)
+
+    def copy_state_adjoint(self, source: "PBF.StateAdjoint", destination: "PBF.StateAdjoint") -> None:
+        destination.particles.positions.copy_(source.particles.positions)
+        destination.particles.velocities.copy_(source.particles.velocities)
+
+    def accumulate_state_adjoint(self, source: "PBF.StateAdjoint", destination: "PBF.StateAdjoint") -> None:
+        destination.particles.positions.add_(source.particles.positions)
+        destination.particles.velocities.add_(source.particles.velocities)
+
+    def jvp_step(self, state, control, parameters, next_state, cache, state_tangent, control_tangent,
+                 parameter_tangent, next_state_tangent) -> None:
+        """PBF::jvp_step (pbf/pbf.cpp:193-215) of the step recorded in `cache`
+        (state, control and next_state are those of that forward step)."""
+        _capi.call(
+            "phx_pbf_jvp_step",
+            self._handle,
+            ctypes.byref(parameters.to_c()),
+            cache._handle,
+            ctypes.byref(state_tangent.to_c()),
+            ctypes.byref(control_tangent.to_c()),
+            ctypes.byref(parameter_tangent.to_c()),
+            ctypes.byref(next_state_tangent.to_c()),
+        )
+
+    def vjp_step(self, state, control, parameters, next_state, cache, next_state_adjoint, previous_state_adjoint,
+                 control_adjoint, parameter_adjoint) -> None:
+        """PBF::vjp_step (pbf/pbf.cpp:217-258); every output is accumulated."""
+        _capi.call(
+            "phx_pbf_vjp_step",
+            self._handle,
+            ctypes.byref(parameters.to_c()),
+            cache._handle,
+            ctypes.byref(next_state_adjoint.to_c()),
+            ctypes.byref(previous_state_adjoint.to_c()),
+            ctypes.byref(control_adjoint.to_c()),
+            ctypes.byref(parameter_adjoint.to_c()),
+        )
 
     def set_profiling(self, enabled: bool) -> None:
         _capi.call("phx_pbf_set_profiling", self._handle, 1 if enabled else 0)

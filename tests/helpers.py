@@ -3,7 +3,7 @@ results back as numpy."""
 import numpy as np
 import torch
 
-from physica_amd.particle import DFSPH, IISPH, PBF, PCISPH, SPH, WCSPH
+from physica_amd.particle import DFSPH, IISPH, PBF, PCISPH, SPH, WCSPH, ExecutionMode
 
 
 def bits_equal(a, b) -> bool:
@@ -62,9 +62,9 @@ PBF_NAMES = ("relaxation", "artificial_pressure_strength", "artificial_pressure_
              "artificial_pressure_radius", "xsph_viscosity", "vorticity_confinement")
 
 
-def pbf_setup(scene, device, configuration=None):
+def pbf_setup(scene, device, configuration=None, mode=ExecutionMode.forward):
     solver = PBF(scene.domain, configuration or PBF.Configuration(
-        pressure_iterations=scene.solver.get("pressure_iterations", 4)), device=device)
+        pressure_iterations=scene.solver.get("pressure_iterations", 4)), mode=mode, device=device)
     state = solver.allocate_state()
     nxt = solver.allocate_state()
     control = solver.allocate_control()

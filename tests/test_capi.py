@@ -87,15 +87,18 @@ def test_invalid_method_configuration_rejected():
     handle = ctypes.c_void_p()
     got = _capi.library().phx_pbf_create(ctypes.byref(cfg().to_c()), None, 0, None, ctypes.byref(handle))
     assert got == _capi.PHX_ERROR_INVALID_ARGUMENT
-    got = _capi.library().phx_pbf_create(ctypes.byref(cfg().to_c()), ctypes.byref(PBF.Configuration().to_c()), 1,
+    got = _capi.library().phx_pbf_create(ctypes.byref(cfg().to_c()), ctypes.byref(PBF.Configuration().to_c()), 7,
                                          None, ctypes.byref(handle))
-    assert got == _capi.PHX_ERROR_UNSUPPORTED
+    assert got == _capi.PHX_ERROR_INVALID_ARGUMENT
+    assert "mode" in _capi.library().phx_last_error().decode()
 
 
 def test_null_arguments_rejected():
     lib = _capi.library()
     assert lib.phx_dfsph_forward_step(None, None, None, None, None, None) == _capi.PHX_ERROR_INVALID_ARGUMENT
     assert lib.phx_pbf_forward_step(None, None, None, None, None, None) == _capi.PHX_ERROR_INVALID_ARGUMENT
+    assert lib.phx_pbf_jvp_step(None, None, None, None, None, None, None) == _capi.PHX_ERROR_INVALID_ARGUMENT
+    assert lib.phx_pbf_vjp_step(None, None, None, None, None, None, None) == _capi.PHX_ERROR_INVALID_ARGUMENT
     assert lib.phx_grid_resolution(None, None) == _capi.PHX_ERROR_INVALID_ARGUMENT
     assert lib.phx_dfsph_cache_densities(None, None) == _capi.PHX_ERROR_INVALID_ARGUMENT
     size = ctypes.c_size_t()
