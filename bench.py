@@ -7,6 +7,14 @@ every input resident in HBM. `value` is particle-steps/s over the timed steps;
 average launch time (HIP events on the solver stream during the timed steps);
 `cpu_baseline` is the FP32 oracle (OpenMP over particles, the same
 reference-order arithmetic) timed on this host over one step of the same scene.
+
+`fwd_adjoint_ratio` is the differentiable PBF rollout: 262,144 particles (64^3
+lattice, spacing 0.02, h 0.04), 120 steps at dt 1/120, 4 iterations, loss the
+mean squared distance of the final positions to a seeded target cloud,
+gradient with respect to the initial velocities. Ratio = (recording forward
+rollout + chained VJPs) / plain forward rollout, with every step cache kept in
+HBM; the same with a state checkpoint every 10 steps and recompute is reported
+beside it.
 """
 from __future__ import annotations
 
@@ -33,6 +41,7 @@ def parse_args(argv=None):
     p.add_argument("--workload", choices=["dfsph", "pbf"], default="dfsph")
     p.add_argument("--no-cpu-baseline", action="store_true", help="skip the CPU oracle timing")
     p.add_argument("--no-pass-timing", action="store_true", help="time without per-pass events")
+    p.add_argument("--no-adjoint", action="store_true", help="skip the differentiable PBF rollout ratio")
     return p.parse_args(argv)
 
 
@@ -125,6 +134,77 @@ def cpu_baseline(workload, sc):
             "sample": sample, "seconds": elapsed}
 
 
+def adjoint_ratio(device, steps=120):
+    """Config of the differentiable PBF rollout; returns the timing record."""
+    import torch
+
+    from physica_amd import scenes
+    from physica_amd.particle import PBF, ExecutionMode
+    from physica_amd.rollout import PbfRollout
+
+    sc = scenes.pbf_dam_break(shape=(64, 64, 64), spacing=0.02, seed=20260004)
+    n = sc.count
+    rng = np.random.default_rng(20260104)
+    target = (sc.positions + np.array([[0.1], [-0.05], [0.08]]) + rng.normal(0, 0.01, size=(3, n))).astype(np.float64)
+
+    def make(mode):
+        solver = PBF(sc.domain, PBF.Configuration(pressure_iterations=4), mode=mode, device=device)
+        state = solver.allocate_state()
+        state.particles.positions.upload(sc.positions)
+        state.particles.velocities.upload(sc.velocities)
+        params = solver.allocate_parameters()
+        for name in ("masses", "rest_densities", "viscosities", "surface_tensions"):
+            getattr(params.particles, name).copy_(torch.from_numpy(sc.parameters[name]))
+        for name in ("relaxation", "artificial_pressure_strength", "artificial_pressure_exponent",
+                     "artificial_pressure_radius", "xsph_viscosity", "vorticity_confinement"):
+            getattr(params, name).copy_(torch.from_numpy(sc.parameters[name]))
+        return solver, state, solver.allocate_control(), params
+
+    def timed(fn):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        out = fn()
+        torch.cuda.synchronize()
+        return out, (time.perf_counter() - t0) * 1e3
+
+    # plain forward rollout (forward mode, one cache reused)
+    solver, state, control, params = make(ExecutionMode.forward)
+    cache = solver.allocate_step_cache()
+    states = [solver.allocate_state(), solver.allocate_state()]
+
+    def plain():
+        solver.copy_state(state, states[0])
+        states[0].particles.step_index = 0
+        for k in range(steps):
+            solver.forward_step(states[k % 2], control, params, states[(k + 1) % 2], cache)
+
+    plain()  # warm-up (allocations, code loading)
+    _, plain_ms = timed(plain)
+    del solver, states, cache
+
+    record = {"workload": "differentiable PBF rollout: 262,144 particles (64^3 lattice, spacing 0.02, h 0.04), "
+                          f"{steps} steps, dt 1/120, 4 iterations, gradient of the final-position loss w.r.t. v0",
+              "plain_forward_ms": round(plain_ms, 2)}
+    dsolver, dstate, dcontrol, dparams = make(ExecutionMode.differentiable)
+    for every in (1, 10):
+        rollout = PbfRollout(dsolver, steps, checkpoint_every=every)
+        final, fwd_ms = timed(lambda: rollout.forward(dstate, dcontrol, dparams))
+        xT = final.particles.positions.numpy().astype(np.float64)
+        seed = dsolver.allocate_state_adjoint()
+        seed.particles.positions.upload(2.0 * (xT - target) / n)
+        (initial, _, _), bwd_ms = timed(lambda: rollout.backward(seed))
+        grad = initial.particles.velocities.numpy()
+        if not np.isfinite(grad).all():
+            raise RuntimeError("non-finite rollout gradient")
+        key = "store_all" if every == 1 else f"checkpoint_{every}"
+        record[key] = {"forward_ms": round(fwd_ms, 2), "backward_ms": round(bwd_ms, 2),
+                       "ratio": round((fwd_ms + bwd_ms) / plain_ms, 3)}
+        rollout.release()
+        del rollout, final, initial
+    record["ratio"] = record["store_all"]["ratio"]
+    return record
+
+
 def main(argv=None) -> int:
     args = parse_args(argv)
     if args.gpus != 1:
@@ -201,6 +281,10 @@ def main(argv=None) -> int:
     baseline = None
     if not args.no_cpu_baseline:
         baseline = cpu_baseline(args.workload, sc)
+    adjoint = None
+    if not args.no_adjoint:
+        del states, cache
+        adjoint = adjoint_ratio(device)
 
     line = {
         "metric": METRIC,
@@ -220,7 +304,8 @@ def main(argv=None) -> int:
         "cpu_baseline": baseline,
         "hbm_fraction_step": round(step_achieved / HBM_PEAK_GBS, 5),
         "algorithmic_bytes_per_particle_step": round(bytes_step, 2),
-        "fwd_adjoint_ratio": None,
+        "fwd_adjoint_ratio": None if adjoint is None else adjoint["ratio"],
+        "fwd_adjoint": adjoint,
         "passes": [{k: (round(v, 4) if isinstance(v, float) else v) for k, v in p.items()} for p in passes],
     }
     print(json.dumps(line), flush=True)
