@@ -359,3 +359,78 @@ def pbf_vjp(domain, step_index, positions, velocities, controls, params: Dict[st
     result = {k: outs[k][0] for k in outs}
     result.update(scal)
     return result
+
+
+DFSPH_PARAMETERS = ("masses", "rest_densities", "viscosities", "divergence_relaxation", "density_relaxation")
+DFSPH_WARM = ("warm_divergence_pressure", "warm_density_pressure")
+
+
+def _pointers(arrays):
+    return (c_void_p * len(arrays))(*(a.ctypes.data for a in arrays))
+
+
+def dfsph_jvp(domain, step_index, positions, velocities, controls, params: Dict[str, np.ndarray],
+              warm_divergence, warm_density, tangents: Dict[str, np.ndarray], divergence_iterations=4,
+              density_iterations=6, warm_start=True, precision="f32"):
+    """SPH<DFSPH>::jvp_step: tangents of the next positions, velocities and
+    warm-start pressures. `tangents` holds "positions", "velocities",
+    "controls" (3, N), any of DFSPH_PARAMETERS and DFSPH_WARM (missing ones
+    are zero)."""
+    n = domain.particle_count
+    pos, pp = _vec(positions, precision)
+    vel, vp = _vec(velocities, precision)
+    ctl, cp = _vec(controls, precision)
+    prm = [_arr(params[k], precision) for k in DFSPH_PARAMETERS]
+    wd, we = _arr(warm_divergence, precision), _arr(warm_density, precision)
+    zero3 = np.zeros((3, n))
+    tv = [_vec(tangents.get(k, zero3), precision) for k in ("positions", "velocities", "controls")]
+    prm_t = [_arr(tangents.get(k, np.zeros(n)), precision) for k in DFSPH_PARAMETERS]
+    wt = [_arr(tangents.get(k, np.zeros(n)), precision) for k in DFSPH_WARM]
+    nx, nxp = _out_vec(n, precision)
+    nv, nvp = _out_vec(n, precision)
+    nw = [np.zeros(n, dtype=_dtype(precision)) for _ in DFSPH_WARM]
+    getattr(library(), "oracle_dfsph_jvp_" + precision)(
+        ctypes.byref(domain), c_uint32(divergence_iterations), c_uint32(density_iterations),
+        c_int(1 if warm_start else 0), c_uint64(step_index), pp, vp, cp, _pointers(prm),
+        c_void_p(wd.ctypes.data), c_void_p(we.ctypes.data), *(t[1] for t in tv), _pointers(prm_t),
+        c_void_p(wt[0].ctypes.data), c_void_p(wt[1].ctypes.data), nxp, nvp, c_void_p(nw[0].ctypes.data),
+        c_void_p(nw[1].ctypes.data))
+    return {"positions": nx, "velocities": nv, DFSPH_WARM[0]: nw[0], DFSPH_WARM[1]: nw[1]}
+
+
+def dfsph_vjp(domain, step_index, positions, velocities, controls, params: Dict[str, np.ndarray],
+              warm_divergence, warm_density, next_adjoints: Dict[str, np.ndarray], divergence_iterations=4,
+              density_iterations=6, warm_start=True, precision="f32"):
+    """SPH<DFSPH>::vjp_step with FP64 adjoints. `next_adjoints` holds
+    "positions", "velocities" (3, N) and optionally DFSPH_WARM entries.
+    Returns the adjoints of the state, the control, every DFSPH_PARAMETERS
+    entry and, with warm_start, the warm-start pressures (zero otherwise)."""
+    n = domain.particle_count
+    pos, pp = _vec(positions, precision)
+    vel, vp = _vec(velocities, precision)
+    ctl, cp = _vec(controls, precision)
+    prm = [_arr(params[k], precision) for k in DFSPH_PARAMETERS]
+    wd, we = _arr(warm_divergence, precision), _arr(warm_density, precision)
+    nxa, nxap = _vec(next_adjoints["positions"], "f64")
+    nva, nvap = _vec(next_adjoints["velocities"], "f64")
+    nwa = [_arr(next_adjoints.get(k, np.zeros(n)), "f64") for k in DFSPH_WARM]
+    outs = {k: _out_vec(n, "f64") for k in ("positions", "velocities", "controls")}
+    scal = {k: np.zeros(n, dtype=np.float64) for k in DFSPH_PARAMETERS + DFSPH_WARM}
+    getattr(library(), "oracle_dfsph_vjp_" + precision)(
+        ctypes.byref(domain), c_uint32(divergence_iterations), c_uint32(density_iterations),
+        c_int(1 if warm_start else 0), c_uint64(step_index), pp, vp, cp, _pointers(prm),
+        c_void_p(wd.ctypes.data), c_void_p(we.ctypes.data), nxap, nvap, c_void_p(nwa[0].ctypes.data),
+        c_void_p(nwa[1].ctypes.data), *(outs[k][1] for k in ("positions", "velocities", "controls")),
+        _pointers([scal[k] for k in DFSPH_PARAMETERS]), c_void_p(scal[DFSPH_WARM[0]].ctypes.data),
+        c_void_p(scal[DFSPH_WARM[1]].ctypes.data))
+    result = {k: outs[k][0] for k in outs}
+    result.update(scal)
+    return result
+
+
+def set_reference_no_slip_adjoint(enabled: bool) -> None:
+    """Switches the SPH prediction's VJP between the exact transpose (default)
+    and the reference's variant, which drops dt * xbar of the components that
+    did not collide when a no-slip collision resets the velocity."""
+    for precision in ("f32", "f64"):
+        getattr(library(), "oracle_set_reference_no_slip_adjoint_" + precision)(c_int(1 if enabled else 0))

@@ -132,6 +132,33 @@ void oracle_set_threads(int threads);
                              double* rest_densities_adjoint, double* relaxation_adjoint, double* strength_adjoint,      \
                              double* exponent_adjoint, double* radius_adjoint, double* xsph_adjoint,                    \
                              double* confinement_adjoint);                                                              \
+    /* SPH<DFSPH>::jvp_step (sph.ixx:356-359, sph.cpp:612-659). parameters: masses, rest densities, viscosities,      \
+     * divergence relaxation, density relaxation */                                                                     \
+    void oracle_dfsph_jvp##suf(const oracle_domain* domain, uint32_t divergence_iterations,                           \
+                               uint32_t density_iterations, int warm_start, uint64_t step_index,                        \
+                               const real* positions[3], const real* velocities[3], const real* controls[3],           \
+                               const real* const parameters[5], const real* warm_divergence,                            \
+                               const real* warm_density, const real* position_tangent[3],                               \
+                               const real* velocity_tangent[3], const real* control_tangent[3],                         \
+                               const real* const parameter_tangent[5], const real* warm_divergence_tangent,             \
+                               const real* warm_density_tangent, real* next_position_tangent[3],                        \
+                               real* next_velocity_tangent[3], real* next_warm_divergence_tangent,                      \
+                               real* next_warm_density_tangent);                                                        \
+    /* SPH<DFSPH>::vjp_step (sph.ixx:361-365, sph.cpp:661-723): FP64 adjoints accumulated into the outputs, warm     \
+     * pressure adjoints copied when warm_start */                                                                      \
+    void oracle_dfsph_vjp##suf(const oracle_domain* domain, uint32_t divergence_iterations,                           \
+                               uint32_t density_iterations, int warm_start, uint64_t step_index,                        \
+                               const real* positions[3], const real* velocities[3], const real* controls[3],           \
+                               const real* const parameters[5], const real* warm_divergence,                            \
+                               const real* warm_density, const double* next_position_adjoint[3],                        \
+                               const double* next_velocity_adjoint[3], const double* next_warm_divergence_adjoint,      \
+                               const double* next_warm_density_adjoint, double* position_adjoint[3],                    \
+                               double* velocity_adjoint[3], double* control_adjoint[3],                                 \
+                               double* const parameter_adjoint[5], double* warm_divergence_adjoint,                     \
+                               double* warm_density_adjoint);                                                           \
+    /* 1: restate the reference's VJP of a no-slip collision in the SPH prediction (sph/dfsph.cu:81); 0 (default): \
+     * the exact transpose of the prediction's JVP */                                                                   \
+    void oracle_set_reference_no_slip_adjoint##suf(int enabled);                                                        \
     /* single kernels on an existing grid, for unit tests (sph/common.cu:14-47, 165-194) */                            \
     void oracle_non_pressure##suf(const oracle_domain* domain, uint64_t grid_step, const real* positions[3],          \
                                   const real* velocities[3], const real* controls[3], const real* masses,              \

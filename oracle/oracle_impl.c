@@ -1753,6 +1753,722 @@ void SUF(oracle_pbf_vjp)(const oracle_domain* d, uint32_t iterations, uint64_t s
     pbf_record_free(&r);
 }
 
+/* ===================================================== DFSPH derivatives */
+/* SPH<DFSPH>::jvp_step and vjp_step (sph.ixx:356-365) around DFSPH::jvp
+ * (sph.cpp:612-659) and DFSPH::vjp (sph.cpp:661-723), with the derivative
+ * kernels of sph/common.cu, sph/dfsph.cu and density/density.cu. The
+ * neighbour graph, the box-collision masks and the active set of every
+ * max(0, .) pressure update are frozen at the primal step. The JVP runs in
+ * the primal precision (the reference's FP32 tangents). The VJP promotes the
+ * primal values and accumulates FP64 adjoints (+=) like the reference's
+ * kernels, keeping its mixed-precision operand order (products of two FP32
+ * primal values are rounded to FP32 before promotion), except the warm-start
+ * pressure adjoints, which are copied (sph.cpp:713). Pair terms are gathered
+ * from both ends of a pair: particle i collects the terms in which it is the
+ * centre and those in which it is the neighbour. */
+
+enum { DF_MASS = 0, DF_REST, DF_VISC, DF_DIV, DF_DEN, DF_PARAMS };
+
+/* Every primal quantity of one DFSPH step that the derivatives read. Phase
+ * q = 0 is the divergence solve, q = 1 the density solve; iterate k holds
+ * the pressures p[q][k] and accelerations a[q][k] after k updates, and
+ * the predicted positions xs[q][k] and densities rs[q][k] of update k >= 1. */
+typedef struct {
+    uint32_t n, iters[2];
+    int warm;
+    grid_t g;
+    real h, dt, G;
+    cbox box;
+    const real* pos[3];
+    const real* vel[3];
+    const real* prm[DF_PARAMS];
+    real* rho;
+    vfield anp, adiv, total;
+    real** p[2];
+    vfield* a[2];
+    vfield* xs[2];
+    real** rs[2];
+} dfsph_record;
+
+/* DFSPH::forward (sph.cpp:572-610) keeping every iterate. */
+static void dfsph_record_forward(dfsph_record* r, const oracle_domain* d, uint32_t nd, uint32_t ne, int warm,
+                                 uint64_t step, const real* pos[3], const real* vel[3], const real* ctl[3],
+                                 const real* const prm[DF_PARAMS], const real* const warm_p[2]) {
+    const uint32_t n = d->particle_count;
+    memset(r, 0, sizeof(*r));
+    r->n = n;
+    r->iters[0] = nd;
+    r->iters[1] = ne;
+    r->warm = warm;
+    r->h = (real)d->support_radius;
+    r->dt = (real)d->time_step;
+    r->G = SUF(oracle_reference_gradient_norm)(d);
+    r->box = collision_box(d, step);
+    for (int c = 0; c < 3; ++c) {
+        r->pos[c] = pos[c];
+        r->vel[c] = vel[c];
+    }
+    for (int k = 0; k < DF_PARAMS; ++k) r->prm[k] = prm[k];
+    grid_build(&r->g, d, step, pos[0], pos[1], pos[2]);
+    r->rho = alloc_real(n);
+    r->anp = alloc_vfield(n);
+    r->adiv = alloc_vfield(n);
+    r->total = alloc_vfield(n);
+    density_kernel(&r->g, 0, pos, pos, prm[DF_MASS], prm[DF_REST], r->rho);
+    non_pressure_kernel(&r->g, pos, vel, ctl, prm[DF_MASS], prm[DF_VISC], r->rho, r->anp.c);
+    vfield acc = alloc_vfield(n);
+    for (int q = 0; q < 2; ++q) {
+        const uint32_t it = r->iters[q];
+        r->p[q] = (real**)calloc(it + 1u, sizeof(real*));
+        r->rs[q] = (real**)calloc(it + 1u, sizeof(real*));
+        r->a[q] = (vfield*)calloc(it + 1u, sizeof(vfield));
+        r->xs[q] = (vfield*)calloc(it + 1u, sizeof(vfield));
+        r->p[q][0] = alloc_real(n);
+        for (uint32_t i = 0; i < n; ++i) r->p[q][0][i] = warm ? warm_p[q][i] : K(0.0);
+        r->a[q][0] = alloc_vfield(n);
+        pressure_kernel(&r->g, pos, prm[DF_MASS], prm[DF_REST], r->rho, r->p[q][0], r->a[q][0].c);
+        for (uint32_t k = 1; k <= it; ++k) {
+            const real* const* ap = CV(r->a[q][k - 1u]);
+            if (q == 1) {
+                add_kernel(n, CV(r->adiv), ap, acc.c);
+                ap = CV(acc);
+            }
+            r->xs[q][k] = alloc_vfield(n);
+            sph_predict_kernel(n, r->dt, r->box, pos, vel, CV(r->anp), ap, r->xs[q][k].c, NULL);
+            r->rs[q][k] = alloc_real(n);
+            density_kernel(&r->g, 0, pos, CV(r->xs[q][k]), prm[DF_MASS], prm[DF_REST], r->rs[q][k]);
+            r->p[q][k] = alloc_real(n);
+            memcpy(r->p[q][k], r->p[q][k - 1u], sizeof(real) * n);
+            projection_update_kernel(n, r->dt, r->G, prm[DF_MASS], r->rho, q ? prm[DF_REST] : r->rho, r->rs[q][k],
+                                     prm[q ? DF_DEN : DF_DIV], r->p[q][k]);
+            r->a[q][k] = alloc_vfield(n);
+            pressure_kernel(&r->g, pos, prm[DF_MASS], prm[DF_REST], r->rho, r->p[q][k], r->a[q][k].c);
+        }
+        if (q == 0)
+            for (int c = 0; c < 3; ++c) memcpy(r->adiv.c[c], r->a[0][it].c[c], sizeof(real) * n);
+    }
+    add_kernel(n, CV(r->adiv), CV(r->a[1][ne]), r->total.c);
+    free_vfield(&acc);
+}
+
+static void dfsph_record_free(dfsph_record* r) {
+    for (int q = 0; q < 2; ++q) {
+        for (uint32_t k = 0; k <= r->iters[q]; ++k) {
+            free(r->p[q][k]);
+            free(r->rs[q][k]);
+            free_vfield(&r->a[q][k]);
+            free_vfield(&r->xs[q][k]);
+        }
+        free(r->p[q]);
+        free(r->rs[q]);
+        free(r->a[q]);
+        free(r->xs[q]);
+    }
+    free(r->rho);
+    free_vfield(&r->anp);
+    free_vfield(&r->adiv);
+    free_vfield(&r->total);
+    grid_free(&r->g);
+}
+
+/* ------------------------------------------------------------ SPH JVP */
+/* cubic_gradient_tangent, density/device.cuh:45-57 */
+static v3 cubic_gradient_tangent(v3 d, v3 dd, real h) {
+    const real dist = v3_length(d);
+    const real coef = K(8.0) / (R_PI * h * h * h);
+    if (dist == K(0.0)) return v3_scale(dd, K(-12.0) * coef / (h * h));
+    if (dist >= h) return v3_make(0, 0, 0);
+    const real q = dist / h;
+    const real deriv = q <= K(0.5) ? K(18.0) * q * q - K(12.0) * q : K(-6.0) * (K(1.0) - q) * (K(1.0) - q);
+    const real second = q <= K(0.5) ? K(36.0) * q - K(12.0) : K(12.0) * (K(1.0) - q);
+    const real radial = v3_dot(d, dd) / dist;
+    const real rs = coef * deriv / (h * dist);
+    const real rsd = coef * (second / (h * h * dist) - deriv / (h * dist * dist));
+    return v3_add(v3_scale(dd, rs), v3_scale(d, rsd * radial));
+}
+
+/* viscosity_laplacian_tangent, density/device.cuh:78-82 */
+static real viscosity_laplacian_tangent(v3 d, v3 dd, real h) {
+    const real dist = v3_length(d);
+    if (dist == K(0.0) || dist >= h) return K(0.0);
+    return K(-45.0) * v3_dot(d, dd) / (R_PI * h * h * h * h * h * h * dist);
+}
+
+/* density_jvp_kernel, density/density.cu:42-67 (cubic kernel) */
+static void density_jvp_kernel(const grid_t* g, const real* const topo[3], const real* const pos[3],
+                               const real* const pos_t[3], const real* masses, const real* masses_t,
+                               const real* rest, const real* rest_t, real* out) {
+    const real h = (real)g->d->support_radius;
+#pragma omp parallel for schedule(static)
+    for (int64_t ii = 0; ii < (int64_t)g->n; ++ii) {
+        const uint32_t i = (uint32_t)ii;
+        const v3 p = v3_load(pos, i);
+        const v3 t = v3_load(pos_t, i);
+        cell_range cr[27];
+        const int nr = stencil(g, v3_load(topo, i), cr);
+        real result = K(0.0);
+        for (int k = 0; k < nr; ++k) {
+            for (uint32_t s = cr[k].first; s < cr[k].last; ++s) {
+                const uint32_t j = g->idx[s];
+                const v3 d = v3_sub(p, v3_load(pos, j));
+                result += masses_t[j] * cubic(d, h) +
+                          masses[j] * v3_dot(cubic_gradient(d, h), v3_sub(t, v3_load(pos_t, j)));
+            }
+            for (uint32_t s = cr[k].bfirst; s < cr[k].blast; ++s) {
+                const uint32_t b = g->bidx[s];
+                const v3 d = v3_sub(p, boundary_position(g, b));
+                result += boundary_volume(g, b) *
+                          (rest_t[i] * cubic(d, h) + rest[i] * v3_dot(cubic_gradient(d, h), t));
+            }
+        }
+        out[i] = result;
+    }
+}
+
+/* non_pressure_jvp_kernel, sph/common.cu:49-96 */
+static void non_pressure_jvp_kernel(const grid_t* g, const real* const pos[3], const real* const vel[3],
+                                    const real* const ctl_t[3], const real* const pos_t[3],
+                                    const real* const vel_t[3], const real* masses, const real* masses_t,
+                                    const real* visc, const real* visc_t, const real* rho, const real* rho_t,
+                                    real* const out[3]) {
+    const real h = (real)g->d->support_radius;
+#pragma omp parallel for schedule(static)
+    for (int64_t ii = 0; ii < (int64_t)g->n; ++ii) {
+        const uint32_t i = (uint32_t)ii;
+        const v3 p = v3_load(pos, i);
+        const v3 v = v3_load(vel, i);
+        const v3 pt = v3_load(pos_t, i);
+        const v3 vt = v3_load(vel_t, i);
+        v3 result = v3_load(ctl_t, i);
+        cell_range cr[27];
+        const int nr = stencil(g, p, cr);
+        for (int k = 0; k < nr; ++k) {
+            for (uint32_t s = cr[k].first; s < cr[k].last; ++s) {
+                const uint32_t j = g->idx[s];
+                if (j == i) continue;
+                const v3 d = v3_sub(p, v3_load(pos, j));
+                const v3 dd = v3_sub(pt, v3_load(pos_t, j));
+                const real lap = viscosity_laplacian(d, h);
+                const real lap_t = viscosity_laplacian_tangent(d, dd, h);
+                const real nu = K(0.5) * (visc[i] + visc[j]);
+                const real nu_t = K(0.5) * (visc_t[i] + visc_t[j]);
+                const real mass = masses[j];
+                const real dens = rho[j];
+                const real w = nu * mass * lap / dens;
+                const real w_t = nu_t * mass * lap / dens + nu * masses_t[j] * lap / dens + nu * mass * lap_t / dens -
+                                 nu * mass * lap * rho_t[j] / (dens * dens);
+                const v3 diff = v3_sub(v3_load(vel, j), v);
+                const v3 diff_t = v3_sub(v3_load(vel_t, j), vt);
+                result = v3_add(result, v3_add(v3_scale(diff, w_t), v3_scale(diff_t, w)));
+            }
+            for (uint32_t s = cr[k].bfirst; s < cr[k].blast; ++s) {
+                const uint32_t b = g->bidx[s];
+                const v3 d = v3_sub(p, boundary_position(g, b));
+                const real lap = viscosity_laplacian(d, h);
+                const real w = visc[i] * boundary_volume(g, b) * lap;
+                const real w_t = boundary_volume(g, b) * (visc_t[i] * lap + visc[i] * viscosity_laplacian_tangent(d, pt, h));
+                const v3 diff = v3_sub(boundary_velocity(g, b), v);
+                result = v3_add(result, v3_add(v3_scale(diff, w_t), v3_scale(vt, -w)));
+            }
+        }
+        v3_store(out, i, result);
+    }
+}
+
+/* pressure_jvp_kernel, sph/common.cu:196-237 */
+static void pressure_jvp_kernel(const grid_t* g, const real* const pos[3], const real* const pos_t[3],
+                                const real* masses, const real* masses_t, const real* rest, const real* rest_t,
+                                const real* rho, const real* rho_t, const real* p, const real* p_t,
+                                real* const out[3]) {
+    const real h = (real)g->d->support_radius;
+#pragma omp parallel for schedule(static)
+    for (int64_t ii = 0; ii < (int64_t)g->n; ++ii) {
+        const uint32_t i = (uint32_t)ii;
+        const v3 x = v3_load(pos, i);
+        const v3 xt = v3_load(pos_t, i);
+        const real dens = rho[i];
+        const real first = p[i] / (dens * dens);
+        const real first_t = p_t[i] / (dens * dens) - K(2.0) * p[i] * rho_t[i] / (dens * dens * dens);
+        v3 result = v3_make(0, 0, 0);
+        cell_range cr[27];
+        const int nr = stencil(g, x, cr);
+        for (int k = 0; k < nr; ++k) {
+            for (uint32_t s = cr[k].first; s < cr[k].last; ++s) {
+                const uint32_t j = g->idx[s];
+                if (j == i) continue;
+                const real nd = rho[j];
+                const real second = p[j] / (nd * nd);
+                const real second_t = p_t[j] / (nd * nd) - K(2.0) * p[j] * rho_t[j] / (nd * nd * nd);
+                const v3 d = v3_sub(x, v3_load(pos, j));
+                const v3 dd = v3_sub(xt, v3_load(pos_t, j));
+                const real factor = -masses[j] * (first + second);
+                const real factor_t = -masses_t[j] * (first + second) - masses[j] * (first_t + second_t);
+                result = v3_add(result, v3_add(v3_scale(cubic_gradient(d, h), factor_t),
+                                               v3_scale(cubic_gradient_tangent(d, dd, h), factor)));
+            }
+            for (uint32_t s = cr[k].bfirst; s < cr[k].blast; ++s) {
+                const uint32_t b = g->bidx[s];
+                const v3 d = v3_sub(x, boundary_position(g, b));
+                const real vol = boundary_volume(g, b);
+                const real factor = -rest[i] * vol * first;
+                const real factor_t = -vol * (rest_t[i] * first + rest[i] * first_t);
+                result = v3_add(result, v3_add(v3_scale(cubic_gradient(d, h), factor_t),
+                                               v3_scale(cubic_gradient_tangent(d, xt, h), factor)));
+            }
+        }
+        v3_store(out, i, result);
+    }
+}
+
+/* Box-collision bits of the clamp in predict_forward_kernel
+ * (sph/dfsph.cu:26-44) for acceleration a1 + a2. */
+static int predict_collisions(cbox box, real dt, v3 x, v3 v, v3 a1, v3 a2, v3* xs) {
+    const v3 vs = v3_add(v, v3_scale(v3_add(a1, a2), dt));
+    const v3 p = v3_add(x, v3_scale(vs, dt));
+    if (xs) *xs = p;
+    return (p.x < box.lo[0] || p.x > box.hi[0] ? 1 : 0) | (p.y < box.lo[1] || p.y > box.hi[1] ? 2 : 0) |
+           (p.z < box.lo[2] || p.z > box.hi[2] ? 4 : 0);
+}
+
+/* predict_jvp_kernel, sph/dfsph.cu:46-66 */
+static void predict_jvp_kernel(uint32_t n, real dt, cbox box, const real* const pos[3], const real* const vel[3],
+                               const real* const a1[3], const real* const a2[3], const real* const pos_t[3],
+                               const real* const vel_t[3], const real* const a1_t[3], const real* const a2_t[3],
+                               real* const out_pos_t[3], real* const out_vel_t[3]) {
+#pragma omp parallel for schedule(static)
+    for (int64_t ii = 0; ii < (int64_t)n; ++ii) {
+        const uint32_t i = (uint32_t)ii;
+        const int col = predict_collisions(box, dt, v3_load(pos, i), v3_load(vel, i), v3_load(a1, i), v3_load(a2, i), NULL);
+        v3 vd = v3_add(v3_load(vel_t, i), v3_scale(v3_add(v3_load(a1_t, i), v3_load(a2_t, i)), dt));
+        v3 xd = v3_add(v3_load(pos_t, i), v3_scale(vd, dt));
+        if (col & 1) xd.x = K(0.0);
+        if (col & 2) xd.y = K(0.0);
+        if (col & 4) xd.z = K(0.0);
+        if (box.no_slip && col) {
+            vd = v3_make(0, 0, 0);
+        } else {
+            if (col & 1) vd.x = K(0.0);
+            if (col & 2) vd.y = K(0.0);
+            if (col & 4) vd.z = K(0.0);
+        }
+        if (out_pos_t) v3_store(out_pos_t, i, xd);
+        if (out_vel_t) v3_store(out_vel_t, i, vd);
+    }
+}
+
+/* projection_update_jvp_kernel, sph/dfsph.cu:112-121; out may alias prev_t */
+static void projection_update_jvp_kernel(uint32_t n, real dt, real G, const real* masses, const real* masses_t,
+                                         const real* rho, const real* rho_t, const real* targets,
+                                         const real* targets_t, const real* prev, const real* predicted,
+                                         const real* relax, const real* prev_t, const real* predicted_t,
+                                         const real* relax_t, real* out) {
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < (int64_t)n; ++i) {
+        const real error = predicted[i] - targets[i];
+        const real volume = masses[i] / rho[i];
+        const real delta = K(1.0) / (K(2.0) * dt * dt * volume * volume * G);
+        const real delta_t = K(-2.0) * delta * (masses_t[i] / masses[i] - rho_t[i] / rho[i]);
+        const real candidate = prev[i] + relax[i] * delta * error;
+        out[i] = candidate > K(0.0) ? prev_t[i] + relax_t[i] * delta * error + relax[i] * delta_t * error +
+                                          relax[i] * delta * (predicted_t[i] - targets_t[i])
+                                    : K(0.0);
+    }
+}
+
+void SUF(oracle_dfsph_jvp)(const oracle_domain* d, uint32_t nd, uint32_t ne, int warm_start, uint64_t step,
+                           const real* pos[3], const real* vel[3], const real* ctl[3],
+                           const real* const prm[DF_PARAMS], const real* warm_div, const real* warm_den,
+                           const real* pos_t[3], const real* vel_t[3], const real* ctl_t[3],
+                           const real* const prm_t[DF_PARAMS], const real* warm_div_t, const real* warm_den_t,
+                           real* next_pos_t[3], real* next_vel_t[3], real* next_warm_div_t, real* next_warm_den_t) {
+    dfsph_record r;
+    const real* const warm_p[2] = {warm_div, warm_den};
+    dfsph_record_forward(&r, d, nd, ne, warm_start, step, pos, vel, ctl, prm, warm_p);
+    const uint32_t n = r.n;
+    const grid_t* g = &r.g;
+    const real *m = prm[DF_MASS], *rest = prm[DF_REST], *m_t = prm_t[DF_MASS], *rest_t = prm_t[DF_REST];
+    /* sph.ixx:357: tangent of the state densities */
+    real* rho_t = alloc_real(n);
+    density_jvp_kernel(g, pos, pos, pos_t, m, m_t, rest, rest_t, rho_t);
+    vfield anp_t = alloc_vfield(n), ap_t = alloc_vfield(n), adiv_t = alloc_vfield(n), tot_t = alloc_vfield(n);
+    vfield xs_t = alloc_vfield(n), acc = alloc_vfield(n);
+    non_pressure_jvp_kernel(g, pos, vel, ctl_t, pos_t, vel_t, m, m_t, prm[DF_VISC], prm_t[DF_VISC], r.rho, rho_t,
+                            anp_t.c);
+    real* p_t = alloc_real(n);
+    real* rs_t = alloc_real(n);
+    const real* const warm_t[2] = {warm_div_t, warm_den_t};
+    real* const out_warm_t[2] = {next_warm_div_t, next_warm_den_t};
+    for (int q = 0; q < 2; ++q) {
+        for (uint32_t i = 0; i < n; ++i) p_t[i] = warm_start ? warm_t[q][i] : K(0.0);
+        pressure_jvp_kernel(g, pos, pos_t, m, m_t, rest, rest_t, r.rho, rho_t, r.p[q][0], p_t, ap_t.c);
+        for (uint32_t k = 1; k <= r.iters[q]; ++k) {
+            const real* const* a = CV(r.a[q][k - 1u]);
+            const real* const* at = CV(ap_t);
+            if (q == 1) {
+                add_kernel(n, CV(r.adiv), a, acc.c);
+                add_kernel(n, CV(adiv_t), at, tot_t.c);
+                a = CV(acc);
+                at = CV(tot_t);
+            }
+            predict_jvp_kernel(n, r.dt, r.box, pos, vel, CV(r.anp), a, pos_t, vel_t, CV(anp_t), at, xs_t.c, NULL);
+            density_jvp_kernel(g, pos, CV(r.xs[q][k]), CV(xs_t), m, m_t, rest, rest_t, rs_t);
+            projection_update_jvp_kernel(n, r.dt, r.G, m, m_t, r.rho, rho_t, q ? rest : r.rho, q ? rest_t : rho_t,
+                                         r.p[q][k - 1u], r.rs[q][k], prm[q ? DF_DEN : DF_DIV], p_t, rs_t,
+                                         prm_t[q ? DF_DEN : DF_DIV], p_t);
+            pressure_jvp_kernel(g, pos, pos_t, m, m_t, rest, rest_t, r.rho, rho_t, r.p[q][k], p_t, ap_t.c);
+        }
+        if (q == 0)
+            for (int c = 0; c < 3; ++c) memcpy(adiv_t.c[c], ap_t.c[c], sizeof(real) * n);
+        if (out_warm_t[q]) memcpy(out_warm_t[q], p_t, sizeof(real) * n);
+    }
+    add_kernel(n, CV(adiv_t), CV(ap_t), tot_t.c);
+    predict_jvp_kernel(n, r.dt, r.box, pos, vel, CV(r.anp), CV(r.total), pos_t, vel_t, CV(anp_t), CV(tot_t), next_pos_t,
+                       next_vel_t);
+    free(rho_t);
+    free(p_t);
+    free(rs_t);
+    free_vfield(&anp_t);
+    free_vfield(&ap_t);
+    free_vfield(&adiv_t);
+    free_vfield(&tot_t);
+    free_vfield(&xs_t);
+    free_vfield(&acc);
+    dfsph_record_free(&r);
+}
+
+/* ------------------------------------------------------------ SPH VJP */
+/* cubic_hessian_product, density/device.cuh:59-70 */
+static d3 cubic_hessian_product(v3 d, d3 w, real h) {
+    const double dist = sqrt((double)v3_dot(d, d));
+    const double coef = 8.0 / ((double)R_PI * h * h * h);
+    if (dist == 0.0) return d3_scale(w, -12.0 * coef / (h * h));
+    if (dist >= h) return d3_make(0, 0, 0);
+    const double q = dist / h;
+    const double deriv = q <= 0.5 ? 18.0 * q * q - 12.0 * q : -6.0 * (1.0 - q) * (1.0 - q);
+    const double second = q <= 0.5 ? 36.0 * q - 12.0 : 12.0 * (1.0 - q);
+    const double rs = coef * deriv / (h * dist);
+    const double rsd = coef * (second / (h * h * dist) - deriv / (h * dist * dist));
+    const d3 dd = d3_of(d);
+    return d3_add(d3_scale(w, rs), d3_scale(dd, rsd * d3_dot(w, dd) / dist));
+}
+
+/* viscosity_laplacian_gradient, density/device.cuh:84-88 */
+static d3 viscosity_laplacian_gradient(v3 d, real h, double s) {
+    const double dist = sqrt((double)v3_dot(d, d));
+    if (dist == 0.0 || dist >= h) return d3_make(0, 0, 0);
+    return d3_scale(d3_of(d), -45.0 * s / ((double)R_PI * pow((double)h, 6.0) * dist));
+}
+
+/* pressure_vjp_kernel, sph/common.cu:239-294 */
+static void pressure_vjp_kernel(const grid_t* g, const real* const pos[3], const real* masses, const real* rest,
+                                const real* rho, const real* p, const double* const acc_a[3], double* const pos_a[3],
+                                double* rho_a, double* p_a, double* masses_a, double* rest_a) {
+    const real h = (real)g->d->support_radius;
+#pragma omp parallel for schedule(static)
+    for (int64_t ii = 0; ii < (int64_t)g->n; ++ii) {
+        const uint32_t i = (uint32_t)ii;
+        const v3 x = v3_load(pos, i);
+        const d3 local = d3_cload(acc_a, i);
+        const double first = (double)p[i] / (rho[i] * rho[i]);
+        d3 position = d3_make(0, 0, 0);
+        double first_a = 0.0, mass_c = 0.0, rest_c = 0.0;
+        cell_range cr[27];
+        const int nr = stencil(g, x, cr);
+        for (int k = 0; k < nr; ++k) {
+            for (uint32_t s = cr[k].first; s < cr[k].last; ++s) {
+                const uint32_t j = g->idx[s];
+                if (j == i) continue;
+                const v3 d = v3_sub(x, v3_load(pos, j));
+                const d3 grad = d3_of(cubic_gradient(d, h));
+                const double second = (double)p[j] / (rho[j] * rho[j]);
+                const double sum = first + second;
+                const double local_factor = -masses[j] * sum;
+                first_a -= masses[j] * d3_dot(local, grad);
+                position = d3_add(position, cubic_hessian_product(d, d3_scale(local, local_factor), h));
+                const d3 cross = d3_cload(acc_a, j);
+                const double cross_projection = d3_dot(cross, grad);
+                mass_c += sum * cross_projection;
+                first_a += masses[i] * cross_projection;
+                position = d3_add(position, cubic_hessian_product(d, d3_scale(cross, masses[i] * sum), h));
+            }
+            for (uint32_t s = cr[k].bfirst; s < cr[k].blast; ++s) {
+                const uint32_t b = g->bidx[s];
+                const v3 d = v3_sub(x, boundary_position(g, b));
+                const d3 grad = d3_of(cubic_gradient(d, h));
+                const real vol = boundary_volume(g, b);
+                const double factor = -rest[i] * vol * first;
+                const double factor_a = d3_dot(local, grad);
+                rest_c -= factor_a * vol * first;
+                first_a -= factor_a * rest[i] * vol;
+                position = d3_add(position, cubic_hessian_product(d, d3_scale(local, factor), h));
+            }
+        }
+        d3_acc(pos_a, i, position);
+        masses_a[i] += mass_c;
+        rest_a[i] += rest_c;
+        p_a[i] += first_a / (rho[i] * rho[i]);
+        rho_a[i] -= 2.0 * first_a * p[i] / (rho[i] * rho[i] * rho[i]);
+    }
+}
+
+/* non_pressure_vjp_kernel, sph/common.cu:98-163 */
+static void non_pressure_vjp_kernel(const grid_t* g, const real* const pos[3], const real* const vel[3],
+                                    const real* masses, const real* visc, const real* rho, const double* const acc_a[3],
+                                    double* const pos_a[3], double* const vel_a[3], double* const ctl_a[3],
+                                    double* rho_a, double* masses_a, double* visc_a) {
+    const real h = (real)g->d->support_radius;
+#pragma omp parallel for schedule(static)
+    for (int64_t ii = 0; ii < (int64_t)g->n; ++ii) {
+        const uint32_t i = (uint32_t)ii;
+        const v3 x = v3_load(pos, i);
+        const v3 v = v3_load(vel, i);
+        const d3 local = d3_cload(acc_a, i);
+        d3 position = d3_make(0, 0, 0), velocity = d3_make(0, 0, 0);
+        double dens_c = 0.0, mass_c = 0.0, visc_c = 0.0;
+        cell_range cr[27];
+        const int nr = stencil(g, x, cr);
+        for (int k = 0; k < nr; ++k) {
+            for (uint32_t s = cr[k].first; s < cr[k].last; ++s) {
+                const uint32_t j = g->idx[s];
+                if (j == i) continue;
+                const v3 d = v3_sub(x, v3_load(pos, j));
+                const v3 diff = v3_sub(v3_load(vel, j), v);
+                const real lap = viscosity_laplacian(d, h);
+                const real nu = K(0.5) * (visc[i] + visc[j]);
+                const real mass = masses[j];
+                const real dens = rho[j];
+                const real w = nu * mass * lap / dens;
+                const double w_a = d3_dot(local, d3_of(diff));
+                velocity = d3_add(velocity, d3_scale(local, -w));
+                const double nu_a = w_a * mass * lap / dens;
+                visc_c += 0.5 * nu_a;
+                position = d3_add(position, viscosity_laplacian_gradient(d, h, w_a * nu * mass / dens));
+                const d3 cross = d3_cload(acc_a, j);
+                const real cross_w = nu * masses[i] * lap / rho[i];
+                const double cross_w_a = d3_dot(cross, d3_of(v3_scale(diff, K(-1.0))));
+                velocity = d3_add(velocity, d3_scale(cross, cross_w));
+                mass_c += cross_w_a * nu * lap / rho[i];
+                dens_c -= cross_w_a * nu * masses[i] * lap / (rho[i] * rho[i]);
+                visc_c += 0.5 * cross_w_a * masses[i] * lap / rho[i];
+                position = d3_add(position, viscosity_laplacian_gradient(d, h, cross_w_a * nu * masses[i] / rho[i]));
+            }
+            for (uint32_t s = cr[k].bfirst; s < cr[k].blast; ++s) {
+                const uint32_t b = g->bidx[s];
+                const v3 d = v3_sub(x, boundary_position(g, b));
+                const v3 diff = v3_sub(boundary_velocity(g, b), v);
+                const real lap = viscosity_laplacian(d, h);
+                const real vol = boundary_volume(g, b);
+                const real w = visc[i] * vol * lap;
+                const double w_a = d3_dot(local, d3_of(diff));
+                velocity = d3_add(velocity, d3_scale(local, -w));
+                visc_c += w_a * vol * lap;
+                position = d3_add(position, viscosity_laplacian_gradient(d, h, w_a * visc[i] * vol));
+            }
+        }
+        d3_acc(pos_a, i, position);
+        d3_acc(vel_a, i, velocity);
+        d3_acc(ctl_a, i, local);
+        rho_a[i] += dens_c;
+        masses_a[i] += mass_c;
+        visc_a[i] += visc_c;
+    }
+}
+
+/* density_vjp_kernel, density/density.cu:69-100 (cubic kernel) */
+static void density_vjp_kernel(const grid_t* g, const real* const topo[3], const real* const pos[3],
+                               const real* masses, const real* rest, const double* rho_a, double* const pos_a[3],
+                               double* masses_a, double* rest_a) {
+    const real h = (real)g->d->support_radius;
+#pragma omp parallel for schedule(static)
+    for (int64_t ii = 0; ii < (int64_t)g->n; ++ii) {
+        const uint32_t i = (uint32_t)ii;
+        const v3 x = v3_load(pos, i);
+        const double out_a = rho_a[i];
+        d3 position = d3_make(0, 0, 0);
+        double mass_c = 0.0, rest_c = 0.0;
+        cell_range cr[27];
+        const int nr = stencil(g, v3_load(topo, i), cr);
+        for (int k = 0; k < nr; ++k) {
+            for (uint32_t s = cr[k].first; s < cr[k].last; ++s) {
+                const uint32_t j = g->idx[s];
+                const v3 d = v3_sub(x, v3_load(pos, j));
+                position = d3_add(position, d3_scale(d3_of(cubic_gradient(d, h)), out_a * masses[j] + rho_a[j] * masses[i]));
+                mass_c += rho_a[j] * cubic(d, h);
+            }
+            for (uint32_t s = cr[k].bfirst; s < cr[k].blast; ++s) {
+                const uint32_t b = g->bidx[s];
+                const v3 d = v3_sub(x, boundary_position(g, b));
+                const real vol = boundary_volume(g, b);
+                position = d3_add(position, d3_scale(d3_of(cubic_gradient(d, h)), out_a * rest[i] * vol));
+                rest_c += out_a * vol * cubic(d, h);
+            }
+        }
+        d3_acc(pos_a, i, position);
+        masses_a[i] += mass_c;
+        rest_a[i] += rest_c;
+    }
+}
+
+/* projection_update_vjp_kernel, sph/dfsph.cu:123-139 */
+static void projection_update_vjp_kernel(uint32_t n, real dt, real G, const real* masses, const real* rho,
+                                         const real* targets, const real* prev, const real* predicted,
+                                         const real* relax, const double* p_a, double* masses_a, double* rho_a,
+                                         double* targets_a, double* prev_a, double* predicted_a, double* relax_a) {
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < (int64_t)n; ++i) {
+        const double error = (double)predicted[i] - targets[i];
+        const double volume = (double)masses[i] / rho[i];
+        const double delta = 1.0 / (2.0 * dt * dt * volume * volume * G);
+        const double candidate = prev[i] + relax[i] * delta * error;
+        if (candidate > 0.0) {
+            const double adj = p_a[i];
+            prev_a[i] += adj;
+            predicted_a[i] += relax[i] * delta * adj;
+            targets_a[i] -= relax[i] * delta * adj;
+            masses_a[i] -= 2.0 * relax[i] * delta * error * adj / masses[i];
+            rho_a[i] += 2.0 * relax[i] * delta * error * adj / rho[i];
+            relax_a[i] += delta * error * adj;
+        }
+    }
+}
+
+/* When set, predict_vjp_kernel restates the reference's own VJP of the
+ * prediction (sph/dfsph.cu:68-102, also integrate_vjp_kernel,
+ * sph/common.cu:344-366): a no-slip collision there zeroes the whole velocity
+ * adjoint, including dt * xbar_c of the components that did not collide,
+ * although those positions still depend on v_c and a_c. That is neither the
+ * transpose of predict_jvp_kernel nor the derivative of the prediction; by
+ * default the exact transpose is used (the two agree everywhere else). */
+static int reference_no_slip_adjoint = 0;
+void SUF(oracle_set_reference_no_slip_adjoint)(int enabled) { reference_no_slip_adjoint = enabled != 0; }
+
+/* predict_vjp_kernel, sph/dfsph.cu:68-102 (vs_a may be NULL: zero) */
+static void predict_vjp_kernel(uint32_t n, real dt, cbox box, const real* const pos[3], const real* const vel[3],
+                               const real* const a1[3], const real* const a2[3], const double* const xs_a[3],
+                               const double* const vs_a[3], double* const pos_a[3], double* const vel_a[3],
+                               double* const a1_a[3], double* const a2_a[3]) {
+#pragma omp parallel for schedule(static)
+    for (int64_t ii = 0; ii < (int64_t)n; ++ii) {
+        const uint32_t i = (uint32_t)ii;
+        const int col = predict_collisions(box, dt, v3_load(pos, i), v3_load(vel, i), v3_load(a1, i), v3_load(a2, i), NULL);
+        const d3 xa = d3_cload(xs_a, i);
+        const d3 va = vs_a ? d3_cload(vs_a, i) : d3_make(0, 0, 0);
+        const d3 position = d3_make(col & 1 ? 0.0 : xa.x, col & 2 ? 0.0 : xa.y, col & 4 ? 0.0 : xa.z);
+        d3 velocity;
+        if (box.no_slip && col) {
+            /* every output velocity component is reset; the positions of the
+             * components that did not collide keep x + dt (v + dt a) */
+            velocity = reference_no_slip_adjoint ? d3_make(0, 0, 0) : d3_scale(position, dt);
+        } else {
+            velocity = d3_add(va, d3_scale(position, dt));
+            if (col & 1) velocity.x = 0.0;
+            if (col & 2) velocity.y = 0.0;
+            if (col & 4) velocity.z = 0.0;
+        }
+        d3_acc(pos_a, i, position);
+        d3_acc(vel_a, i, velocity);
+        const d3 acc = d3_scale(velocity, dt);
+        d3_acc(a1_a, i, acc);
+        d3_acc(a2_a, i, acc);
+    }
+}
+
+static void dfield_copy(dfield* dst, const double* const src[3], uint32_t n) {
+    for (int c = 0; c < 3; ++c) memcpy(dst->c[c], src[c], sizeof(double) * n);
+}
+static void dfield_clear(dfield* f, uint32_t n) {
+    for (int c = 0; c < 3; ++c) memset(f->c[c], 0, sizeof(double) * n);
+}
+
+void SUF(oracle_dfsph_vjp)(const oracle_domain* d, uint32_t nd, uint32_t ne, int warm_start, uint64_t step,
+                           const real* pos[3], const real* vel[3], const real* ctl[3],
+                           const real* const prm[DF_PARAMS], const real* warm_div, const real* warm_den,
+                           const double* next_pos_a[3], const double* next_vel_a[3], const double* next_warm_div_a,
+                           const double* next_warm_den_a, double* pos_a[3], double* vel_a[3], double* ctl_a[3],
+                           double* const prm_a[DF_PARAMS], double* warm_div_a, double* warm_den_a) {
+    dfsph_record r;
+    const real* const warm_p[2] = {warm_div, warm_den};
+    dfsph_record_forward(&r, d, nd, ne, warm_start, step, pos, vel, ctl, prm, warm_p);
+    const uint32_t n = r.n;
+    const grid_t* g = &r.g;
+    const real *m = prm[DF_MASS], *rest = prm[DF_REST];
+    double *m_a = prm_a[DF_MASS], *rest_a = prm_a[DF_REST];
+    double* rho_a = dzeros(n);
+    double* target_a = dzeros(n);
+    double* cur_p = dzeros(n);
+    double* prev_p = dzeros(n);
+    double* rs_a = dzeros(n);
+    dfield anp_a = dfield_zeros(n), tot_a = dfield_zeros(n), adiv_a = dfield_zeros(n);
+    dfield cur_ap = dfield_zeros(n), prev_ap = dfield_zeros(n), xs_a = dfield_zeros(n);
+    vfield acc = alloc_vfield(n);
+    /* sph.cpp:671: the last prediction gives the next state */
+    predict_vjp_kernel(n, r.dt, r.box, pos, vel, CV(r.anp), CV(r.total), next_pos_a, next_vel_a, pos_a, vel_a, anp_a.c,
+                       tot_a.c);
+    dfield_copy(&adiv_a, DV(tot_a), n);
+    dfield_copy(&cur_ap, DV(tot_a), n);
+    for (uint32_t i = 0; i < n; ++i) cur_p[i] = next_warm_den_a ? next_warm_den_a[i] : 0.0;
+    double* const warm_a[2] = {warm_div_a, warm_den_a};
+    /* reverse_phase (sph.cpp:675-714): density phase, then divergence phase */
+    for (int q = 1; q >= 0; --q) {
+        if (q == 0) {
+            dfield_copy(&cur_ap, DV(adiv_a), n);
+            for (uint32_t i = 0; i < n; ++i) cur_p[i] = next_warm_div_a ? next_warm_div_a[i] : 0.0;
+        }
+        for (uint32_t k = r.iters[q]; k >= 1u; --k) {
+            memset(prev_p, 0, sizeof(double) * n);
+            memset(rs_a, 0, sizeof(double) * n);
+            dfield_clear(&prev_ap, n);
+            dfield_clear(&xs_a, n);
+            pressure_vjp_kernel(g, pos, m, rest, r.rho, r.p[q][k], DV(cur_ap), pos_a, rho_a, cur_p, m_a, rest_a);
+            projection_update_vjp_kernel(n, r.dt, r.G, m, r.rho, q ? rest : r.rho, r.p[q][k - 1u], r.rs[q][k],
+                                         prm[q ? DF_DEN : DF_DIV], cur_p, m_a, rho_a, q ? rest_a : target_a, prev_p,
+                                         rs_a, prm_a[q ? DF_DEN : DF_DIV]);
+            density_vjp_kernel(g, pos, CV(r.xs[q][k]), m, rest, rs_a, xs_a.c, m_a, rest_a);
+            if (q == 0) {
+                predict_vjp_kernel(n, r.dt, r.box, pos, vel, CV(r.anp), CV(r.a[0][k - 1u]), DV(xs_a), NULL, pos_a,
+                                   vel_a, anp_a.c, prev_ap.c);
+            } else {
+                add_kernel(n, CV(r.adiv), CV(r.a[1][k - 1u]), acc.c);
+                dfield_clear(&tot_a, n);
+                predict_vjp_kernel(n, r.dt, r.box, pos, vel, CV(r.anp), CV(acc), DV(xs_a), NULL, pos_a, vel_a,
+                                   anp_a.c, tot_a.c);
+                /* add_adjoint_kernel, sph/common.cu:376-388 */
+                for (int c = 0; c < 3; ++c)
+                    for (uint32_t i = 0; i < n; ++i) {
+                        adiv_a.c[c][i] += tot_a.c[c][i];
+                        prev_ap.c[c][i] += tot_a.c[c][i];
+                    }
+            }
+            double* t = cur_p;
+            cur_p = prev_p;
+            prev_p = t;
+            dfield tf = cur_ap;
+            cur_ap = prev_ap;
+            prev_ap = tf;
+        }
+        pressure_vjp_kernel(g, pos, m, rest, r.rho, r.p[q][0], DV(cur_ap), pos_a, rho_a, cur_p, m_a, rest_a);
+        if (warm_start && warm_a[q]) memcpy(warm_a[q], cur_p, sizeof(double) * n);
+    }
+    for (uint32_t i = 0; i < n; ++i) rho_a[i] += target_a[i];
+    non_pressure_vjp_kernel(g, pos, vel, m, prm[DF_VISC], r.rho, DV(anp_a), pos_a, vel_a, ctl_a, rho_a, m_a,
+                            prm_a[DF_VISC]);
+    /* sph.ixx:364: the state densities */
+    density_vjp_kernel(g, pos, pos, m, rest, rho_a, pos_a, m_a, rest_a);
+    free(rho_a);
+    free(target_a);
+    free(cur_p);
+    free(prev_p);
+    free(rs_a);
+    dfield_free(&anp_a);
+    dfield_free(&tot_a);
+    dfield_free(&adiv_a);
+    dfield_free(&cur_ap);
+    dfield_free(&prev_ap);
+    dfield_free(&xs_a);
+    free_vfield(&acc);
+    dfsph_record_free(&r);
+}
+
 #ifndef ORACLE_F64
 int oracle_threads(void) {
 #ifdef _OPENMP

@@ -1,0 +1,262 @@
+"""Pins of the oracle's DFSPH JVP and VJP (SPH<DFSPH>::jvp_step/vjp_step):
+the dot-product identity <J t, w> = <t, J^T w> between the two (FP64),
+central finite differences of the FP64 forward step along random directions
+and along one input group at a time, and FP32 tracking FP64.
+
+The scene is a compressed, swirling block in the middle of the box, so every
+pressure update has interior particles in its active set and no collision
+mask changes under the perturbations; a second scene pushes the block into
+a wall so the collision masks of every prediction are exercised (the
+dot-product identity holds there too), and a third adds a plane of boundary
+particles so the boundary terms of every kernel are differentiated."""
+import numpy as np
+import pytest
+
+import oracle
+from physica_amd import scenes
+
+PARAMS = oracle.DFSPH_PARAMETERS
+WARM = oracle.DFSPH_WARM
+STATE = ("positions", "velocities", "controls")
+
+
+class Case:
+    def __init__(self, sc, domain, warm):
+        self.sc = sc
+        self.d = domain
+        self.warm = warm
+
+
+def case(shape=(6, 6, 5), seed=3, wall=False, boundary=False):
+    sc = scenes.dfsph_dam_break(shape=shape, seed=seed, box=(0.2, 0.2, 0.2))
+    n = sc.count
+    rng = np.random.default_rng(seed)
+    c = sc.positions.mean(axis=1, keepdims=True)
+    pos = 0.95 * (sc.positions - c) + np.array([[0.1], [0.1], [0.1]])
+    if wall:
+        # lowest layer 0.1 mm above the shrunk box: the predictions clamp
+        pos[1] += sc.domain.particle_radius + 1e-4 - pos[1].min()
+    sc.positions = pos.astype(np.float32)
+    c = sc.positions.mean(axis=1, keepdims=True)
+    r = sc.positions - c
+    swirl = np.stack([-r[2], 0.2 * r[0], r[0]]) * 2.0
+    sc.velocities = (swirl + rng.normal(0, 0.02, size=(3, n))).astype(np.float32)
+    if wall:
+        sc.velocities[1] -= np.float32(0.3)
+    p = sc.parameters
+    p["masses"] = (p["masses"] * rng.uniform(0.95, 1.05, n)).astype(np.float32)
+    p["rest_densities"] = (p["rest_densities"] * rng.uniform(0.97, 1.03, n)).astype(np.float32)
+    p["viscosities"] = rng.uniform(0.005, 0.02, n).astype(np.float32)
+    p["divergence_relaxation"] = rng.uniform(0.3, 0.7, n).astype(np.float32)
+    p["density_relaxation"] = rng.uniform(0.3, 0.7, n).astype(np.float32)
+    bp = None
+    if boundary:
+        s = 0.01
+        xs = np.arange(-4, 5) * s
+        gx, gz = np.meshgrid(xs, xs, indexing="ij")
+        m = gx.size
+        ymin = sc.positions[1].min() - 0.012
+        bpos = np.stack([gx.ravel() + c[0, 0], np.full(m, ymin), gz.ravel() + c[2, 0]]).astype(np.float32)
+        bvel = np.zeros((3, m), dtype=np.float32)
+        bvel[0] = 0.05
+        bp = (bpos, bvel, np.full(m, s ** 3, dtype=np.float32))
+    d = oracle.make_domain(sc.domain, bp)
+    # warm-start pressures from one step of the same scene
+    z = np.zeros(n, dtype=np.float32)
+    first = oracle.dfsph_step(d, 0, sc.positions, sc.velocities, np.zeros_like(sc.positions), p, z, z, 2, 3)
+    warm = (first[WARM[0]], first[WARM[1]])
+    return Case(sc, d, warm)
+
+
+def directions(cs, rng):
+    sc = cs.sc
+    n = sc.count
+    t = {k: rng.normal(0, 1, size=(3, n)) for k in STATE}
+    t["positions"] *= 1e-4
+    t["velocities"] *= 1e-2
+    for k in PARAMS:
+        t[k] = rng.normal(0, 1, n) * 0.1 * np.abs(sc.parameters[k].astype(np.float64)).mean()
+    for k, w0 in zip(WARM, cs.warm):
+        t[k] = rng.normal(0, 1, n) * 0.1 * max(np.abs(w0).mean(), 1.0)
+    w = {"positions": rng.normal(0, 1, size=(3, n)), "velocities": rng.normal(0, 1, size=(3, n))}
+    for k in WARM:
+        w[k] = rng.normal(0, 1, n) * 1e-6
+    return t, w
+
+
+def inputs(cs, precision="f64"):
+    dt = np.float64 if precision == "f64" else np.float32
+    sc = cs.sc
+    x = sc.positions.astype(dt)
+    v = sc.velocities.astype(dt)
+    u = np.zeros_like(x)
+    p = {k: sc.parameters[k].astype(dt) for k in sc.parameters}
+    return x, v, u, p, cs.warm[0].astype(dt), cs.warm[1].astype(dt)
+
+
+def forward(cs, x, v, u, p, wd, we, nd, ne, warm):
+    r = oracle.dfsph_step(cs.d, 0, x, v, u, p, wd, we, nd, ne, warm, "f64")
+    return r
+
+
+def jvp(cs, t, nd, ne, warm, precision="f64"):
+    x, v, u, p, wd, we = inputs(cs, precision)
+    return oracle.dfsph_jvp(cs.d, 0, x, v, u, p, wd, we, t, nd, ne, warm, precision)
+
+
+def vjp(cs, w, nd, ne, warm, precision="f64"):
+    x, v, u, p, wd, we = inputs(cs, precision)
+    return oracle.dfsph_vjp(cs.d, 0, x, v, u, p, wd, we, w, nd, ne, warm, precision)
+
+
+def pairing(jt, w):
+    return sum(np.sum(jt[k] * w[k]) for k in ("positions", "velocities") + WARM)
+
+
+@pytest.mark.parametrize("nd,ne,warm,variant", [(0, 0, True, "free"), (1, 1, False, "free"), (2, 3, True, "free"),
+                                                (4, 6, True, "free"), (2, 3, True, "wall"),
+                                                (2, 2, True, "boundary")])
+def test_dot_product_identity(nd, ne, warm, variant):
+    cs = case(wall=variant == "wall", boundary=variant == "boundary")
+    if variant == "wall":
+        # the predictions clamp: some next positions sit on the shrunk box
+        nxt = forward(cs, *inputs(cs), nd, ne, warm)
+        assert np.sum(nxt["positions"][1] <= cs.sc.domain.particle_radius + 1e-12) > 5
+    rng = np.random.default_rng(10 + nd + 7 * ne)
+    t, w = directions(cs, rng)
+    jt = jvp(cs, t, nd, ne, warm)
+    jw = vjp(cs, w, nd, ne, warm)
+    left = pairing(jt, w)
+    right = sum(np.sum(t[k] * jw[k]) for k in STATE + PARAMS) + \
+        (sum(np.sum(t[k] * jw[k]) for k in WARM) if warm else 0.0)
+    scale = abs(left) + abs(right)
+    assert abs(left - right) <= 1e-11 * scale, (left, right)
+    if not warm:
+        for k in WARM:
+            assert not np.any(jw[k])
+
+
+@pytest.mark.parametrize("nd,ne,warm", [(1, 2, False), (2, 3, True)])
+def test_jvp_matches_central_differences(nd, ne, warm):
+    """Richardson-extrapolated central differences (error O(eps^4))."""
+    cs = case()
+    rng = np.random.default_rng(20 + nd)
+    x, v, u, p, wd, we = inputs(cs)
+    for trial in range(2):
+        t, _ = directions(cs, rng)
+        jt = jvp(cs, t, nd, ne, warm)
+
+        def central(eps):
+            def shifted(sign):
+                q = {k: p[k] + sign * eps * t[k] if k in PARAMS else p[k] for k in p}
+                return forward(cs, x + sign * eps * t["positions"], v + sign * eps * t["velocities"],
+                               u + sign * eps * t["controls"], q, wd + sign * eps * t[WARM[0]],
+                               we + sign * eps * t[WARM[1]], nd, ne, warm)
+
+            a, b = shifted(1.0), shifted(-1.0)
+            return {k: (a[k] - b[k]) / (2 * eps) for k in ("positions", "velocities") + WARM}
+
+        c1, c2 = central(2e-6), central(1e-6)
+        for k in ("positions", "velocities") + WARM:
+            fd = (4 * c2[k] - c1[k]) / 3
+            err = np.abs(jt[k] - fd).max() / np.abs(fd).max()
+            assert err < 2e-6, (trial, k, err)
+
+
+@pytest.mark.parametrize("boundary", [False, True])
+def test_vjp_of_each_input_matches_finite_differences(boundary):
+    """Directional derivative along one input group at a time, read off the VJP."""
+    cs = case(shape=(5, 5, 4), seed=5, boundary=boundary)
+    nd, ne, warm = 2, 2, True
+    rng = np.random.default_rng(30)
+    x, v, u, p, wd, we = inputs(cs)
+    t_all, w = directions(cs, rng)
+    jw = vjp(cs, w, nd, ne, warm)
+    for group in STATE + PARAMS + WARM:
+        t = t_all[group]
+
+        def value(sign, eps):
+            xs, vs, us, q, a, b = x, v, u, dict(p), wd, we
+            if group == "positions":
+                xs = x + sign * eps * t
+            elif group == "velocities":
+                vs = v + sign * eps * t
+            elif group == "controls":
+                us = u + sign * eps * t
+            elif group == WARM[0]:
+                a = wd + sign * eps * t
+            elif group == WARM[1]:
+                b = we + sign * eps * t
+            else:
+                q[group] = p[group] + sign * eps * t
+            r = forward(cs, xs, vs, us, q, a, b, nd, ne, warm)
+            return pairing(r, w)
+
+        def central(eps):
+            return (value(1.0, eps) - value(-1.0, eps)) / (2 * eps)
+
+        # Richardson extrapolation: truncation O(eps^4); smaller steps drown
+        # in the rounding of the paired sums
+        fd = (4 * central(1.5e-3) - central(3e-3)) / 3
+        got = np.sum(t * jw[group])
+        assert abs(got - fd) <= 1e-6 * abs(fd), (group, got, fd)
+
+
+def test_f32_derivatives_track_f64():
+    cs = case()
+    rng = np.random.default_rng(40)
+    t, w = directions(cs, rng)
+    j32 = jvp(cs, t, 2, 3, True, "f32")
+    j64 = jvp(cs, t, 2, 3, True, "f64")
+    for k in ("positions", "velocities") + WARM:
+        assert np.abs(j32[k] - j64[k]).max() <= 2e-3 * np.abs(j64[k]).max(), k
+    a32 = vjp(cs, w, 2, 3, True, "f32")
+    a64 = vjp(cs, w, 2, 3, True, "f64")
+    for k in STATE + PARAMS + WARM:
+        assert np.abs(a32[k] - a64[k]).max() <= 2e-3 * np.abs(a64[k]).max(), k
+
+
+def test_vjp_is_linear_in_the_output_adjoint():
+    cs = case(shape=(4, 4, 4))
+    rng = np.random.default_rng(50)
+    _, w = directions(cs, rng)
+    a = vjp(cs, w, 2, 2, True)
+    b = vjp(cs, {k: 2 * w[k] for k in w}, 2, 2, True)
+    for k in STATE + PARAMS + WARM:
+        np.testing.assert_allclose(b[k], 2 * a[k], rtol=1e-12, atol=1e-300)
+
+
+def test_reference_no_slip_adjoint_differs_only_where_it_breaks_the_identity():
+    """The reference's prediction VJP (sph/dfsph.cu:81, sph/common.cu:357)
+    zeroes the whole velocity adjoint under a no-slip collision, dropping
+    dt * xbar_c of the components that did not collide. Restated here behind
+    a switch: it breaks the dot-product identity in the wall scene, while the
+    exact transpose (the default) satisfies it; in a slip box the two agree
+    bit for bit."""
+    cs = case(wall=True)
+    rng = np.random.default_rng(60)
+    t, w = directions(cs, rng)
+    jt = jvp(cs, t, 2, 3, True)
+    exact = vjp(cs, w, 2, 3, True)
+    oracle.set_reference_no_slip_adjoint(True)
+    try:
+        ref = vjp(cs, w, 2, 3, True)
+    finally:
+        oracle.set_reference_no_slip_adjoint(False)
+    left = pairing(jt, w)
+
+    def right(a):
+        return sum(np.sum(t[k] * a[k]) for k in STATE + PARAMS + WARM)
+
+    scale = abs(left) + abs(right(exact))
+    assert abs(left - right(exact)) <= 1e-11 * scale
+    assert abs(left - right(ref)) > 1e-4 * scale
+    cs.d.no_slip = 0
+    exact = vjp(cs, w, 2, 3, True)
+    oracle.set_reference_no_slip_adjoint(True)
+    try:
+        ref = vjp(cs, w, 2, 3, True)
+    finally:
+        oracle.set_reference_no_slip_adjoint(False)
+    for k in STATE + PARAMS + WARM:
+        assert np.array_equal(exact[k], ref[k]), k
