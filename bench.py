@@ -11,7 +11,7 @@ reference-order arithmetic) timed on this host over one step of the same scene.
 `fwd_adjoint_ratio` is the differentiable PBF rollout: 262,144 particles (64^3
 lattice, spacing 0.02, h 0.04), 120 steps at dt 1/120, 4 iterations, loss the
 mean squared distance of the final positions to a seeded target cloud,
-gradient with respect to the initial velocities. Ratio = (recording forward
+gradient with respect to the initial velocities (no parameter gradients). Ratio = (recording forward
 rollout + chained VJPs) / plain forward rollout, with every step cache kept in
 HBM; the same with a state checkpoint every 10 steps and recompute is reported
 beside it.
@@ -187,7 +187,7 @@ def adjoint_ratio(device, steps=120):
               "plain_forward_ms": round(plain_ms, 2)}
     dsolver, dstate, dcontrol, dparams = make(ExecutionMode.differentiable)
     for every in (1, 10):
-        rollout = PbfRollout(dsolver, steps, checkpoint_every=every)
+        rollout = PbfRollout(dsolver, steps, checkpoint_every=every, parameter_gradients=False)
         final, fwd_ms = timed(lambda: rollout.forward(dstate, dcontrol, dparams))
         xT = final.particles.positions.numpy().astype(np.float64)
         seed = dsolver.allocate_state_adjoint()

@@ -150,6 +150,16 @@ __device__ __forceinline__ float artificial_value(Float3 d, float h, float stren
     return -strength * power;
 }
 
+// True when the artificial pressure -k (W(d) / W(dq))^n of a particle is the
+// signed zero -k for every pair with W(d) > 0: k is +-0 and the power stays
+// finite for every ratio up to W(0) / W(dq) (powf is non-decreasing in the
+// base for n >= 0). `reference` is W(dq).
+__device__ __forceinline__ bool artificial_vanishes(float h, float strength, float exponent, float reference) {
+    if (strength != 0.0F || !(reference > 0.0F) || !(exponent >= 0.0F)) return false;
+    const float largest = poly6(Float3{0.0F, 0.0F, 0.0F}, h) / reference;
+    return powf(largest, exponent) < 1.0e37F;
+}
+
 // min(C-1, max(0, floor(v))) with NaN and negatives mapped to 0, as the
 // reference's __float2int_rd + clamp (neighborhood.cu:34-36).
 __device__ __forceinline__ int cell_clamp(float v, uint32_t count) {
@@ -188,6 +198,14 @@ __device__ __forceinline__ void predict_clamped(const Box& box, float dt, Float3
     }
     out_x = pos;
     out_v = vel;
+}
+
+// Box-collision bits (x: 1, y: 2, z: 4) of the unclamped prediction
+// x + dt (v + dt a) of predict_clamped.
+__device__ __forceinline__ uint32_t predict_collisions(const Box& box, float dt, Float3 x, Float3 v, Float3 a) {
+    const Float3 pos = add(x, scale(add(v, scale(a, dt)), dt));
+    return (pos.x < box.lo[0] || pos.x > box.hi[0] ? 1u : 0u) | (pos.y < box.lo[1] || pos.y > box.hi[1] ? 2u : 0u) |
+           (pos.z < box.lo[2] || pos.z > box.hi[2] ? 4u : 0u);
 }
 
 // Uniform grid in sorted (cell-major) particle order: the cell range of cell c

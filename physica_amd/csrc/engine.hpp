@@ -90,6 +90,23 @@ struct VecBuffer {
     CVec cview() const { return {x.data(), y.data(), z.data()}; }
 };
 
+// FP64 SoA vector field (the adjoint views of domain/device.h).
+struct DVec {
+    double* x;
+    double* y;
+    double* z;
+};
+
+struct DVecBuffer {
+    DeviceBuffer<double> x, y, z;
+    void resize(size_t n) {
+        x.resize(n);
+        y.resize(n);
+        z.resize(n);
+    }
+    DVec view() { return {x.data(), y.data(), z.data()}; }
+};
+
 // Validated copy of a DomainConfiguration plus derived grid quantities.
 struct Domain {
     phx_domain_configuration config{};

@@ -50,6 +50,7 @@ class MaskSet {
     float t_skin() const;
     float limit2() const;
     float skin() const { return skin_; }
+    uint32_t flag_count() const { return flag_count_; }
 
     uint16_t* exact() { return exact_.data(); }
     uint16_t* skin_mask() { return skin_mask_.data(); }

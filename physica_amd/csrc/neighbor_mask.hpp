@@ -24,6 +24,11 @@ constexpr uint32_t kMaskBits = 16;
 constexpr uint32_t kFlagOverflow = 0;
 constexpr uint32_t kFlagDisplacement = 1;
 
+// Linear index of cell (x, y, z), x fastest.
+__device__ __forceinline__ uint32_t cell_of(const SortedGrid& g, int x, int y, int z) {
+    return (static_cast<uint32_t>(z) * g.cy + static_cast<uint32_t>(y)) * g.cx + static_cast<uint32_t>(x);
+}
+
 // Cell-index offset of each stencil slot k = (ox + 1) * 9 + (oy + 1) * 3 +
 // (oz + 1): ox + oy Cx + oz Cx Cy (two's complement). Filled by the first 27
 // threads of the block; callers synchronise before use.
@@ -122,6 +127,72 @@ __device__ __forceinline__ bool build_masks(const SortedGrid& g, const float4* _
                 skin[static_cast<size_t>(k) * n + i] = static_cast<uint16_t>(ms);
                 if (column != nullptr) column[static_cast<uint32_t>(k) * stride] = static_cast<uint16_t>(me);
                 if (nonzero != nullptr && me != 0u) *nonzero |= 1u << k;
+            }
+        }
+    }
+    return complete;
+}
+
+// A step's skin masks and the flags that decide, pass by pass, whether they
+// apply: a pass at positions that moved at most sqrt(flags[kFlagDisplacement
+// + slot]) from the positions the masks were built at may use them while
+// that is within limit2 and no cell overflowed.
+struct MaskView {
+    const uint16_t* masks;  // [kStencil][n]; null: always scan the full stencil
+    const uint32_t* flags;
+    uint32_t n;
+    float limit2;
+};
+
+__device__ __forceinline__ bool masks_apply(const MaskView& m, uint32_t slot) {
+    return m.masks != nullptr && m.flags[kFlagOverflow] == 0u &&
+           __uint_as_float(m.flags[kFlagDisplacement + slot]) <= m.limit2;
+}
+
+// Visits the candidates of particle i in the reference order: the marked
+// ones of its masks (copied to its LDS column) or every particle of the 27
+// cells around (cx, cy, cz).
+template <class Visit>
+__device__ __forceinline__ void visit_candidates(const SortedGrid& g, const MaskView& m, bool masked, uint32_t i, int cx,
+                                                 int cy, int cz, const int* delta, uint16_t* column, uint32_t stride,
+                                                 Visit&& visit) {
+    if (masked) {
+        const uint32_t nonzero = load_lane_column(m.masks, m.n, i, column, stride);
+        walk_masked(g.offsets, cell_of(g, cx, cy, cz), delta, column, stride, nonzero, visit);
+    } else {
+        for_each_candidate(g, cx, cy, cz, visit);
+    }
+}
+
+// Skin mask of particle i at p from SoA positions: bit b of slot k marks the
+// b-th particle of that stencil cell with squared distance below t. Returns
+// false if a stencil cell holds more than kMaskBits particles.
+__device__ __forceinline__ bool build_skin_mask(const SortedGrid& g, CVec pos, uint32_t n, uint32_t i, Float3 p,
+                                                int cx, int cy, int cz, float t, uint16_t* __restrict__ masks) {
+    bool complete = true;
+    int k = 0;
+    for (int ox = -1; ox <= 1; ++ox) {
+        const int X = cx + ox;
+        for (int oy = -1; oy <= 1; ++oy) {
+            const int Y = cy + oy;
+            for (int oz = -1; oz <= 1; ++oz, ++k) {
+                const int Z = cz + oz;
+                uint32_t bits = 0u;
+                if (X >= 0 && X < static_cast<int>(g.cx) && Y >= 0 && Y < static_cast<int>(g.cy) && Z >= 0 &&
+                    Z < static_cast<int>(g.cz)) {
+                    const uint32_t c = cell_of(g, X, Y, Z);
+                    const uint32_t first = g.offsets[c];
+                    uint32_t last = g.offsets[c + 1u];
+                    if (last - first > kMaskBits) {
+                        complete = false;
+                        last = first + kMaskBits;
+                    }
+                    for (uint32_t s = first; s < last; ++s) {
+                        const Float3 d = sub(p, load(pos, s));
+                        bits |= dot(d, d) < t ? 1u << (s - first) : 0u;
+                    }
+                }
+                masks[static_cast<size_t>(k) * n + i] = static_cast<uint16_t>(bits);
             }
         }
     }

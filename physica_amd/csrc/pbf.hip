@@ -2,6 +2,10 @@
 //
 // Reference: PBF::forward_step (pbf/pbf.cpp:167-191) and pbf/pbf.cu. Fusion:
 //   grid       predict x* fused into the cell-key pass (pbf.cu:73-78)
+//   mask       skin masks at x* (radius h (1 + skin)); every later pass walks
+//              them while no particle moved more than half the skin from x*
+//              (checked on the device from the correction passes' largest
+//              displacement), and the full 27-cell stencil otherwise
 //   per iter   A: density + lambda in one neighbour loop (density.cu:18-40,
 //                 pbf.cu:95-127; the two sums keep their own order)
 //              B: correction + project (pbf.cu:254-288, 417-431), written to
@@ -14,10 +18,37 @@
 // result of the reference expression.
 #include "pbf.hpp"
 
+#include "neighbor_mask.hpp"
+
 namespace phx {
 namespace {
 
 constexpr uint32_t kBlock = 256;
+
+// LDS of a masked neighbour pass: the lanes' mask columns and the slot table.
+#define PHX_MASK_LDS(grid)                          \
+    __shared__ uint16_t smask[kStencil * kBlock];   \
+    __shared__ int delta[kStencil];                 \
+    fill_slot_deltas(grid, delta);                  \
+    __syncthreads()
+
+struct MaskArgs {
+    uint32_t n;
+    SortedGrid grid;
+    CVec xs;
+    float t_skin;
+    uint16_t* masks;
+    uint32_t* flags;
+};
+
+__global__ void __launch_bounds__(kBlock) pbf_mask_kernel(MaskArgs a) {
+    const uint32_t i = xcd_block() * kBlock + threadIdx.x;
+    if (i >= a.n) return;
+    const Float3 p = load(a.xs, i);
+    int cx, cy, cz;
+    particle_cell(a.grid, p, cx, cy, cz);
+    if (!build_skin_mask(a.grid, a.xs, a.n, i, p, cx, cy, cz, a.t_skin, a.masks)) atomicOr(&a.flags[kFlagOverflow], 1u);
+}
 
 struct LambdaArgs {
     uint32_t n;
@@ -33,11 +64,15 @@ struct LambdaArgs {
     float* lambda;
     float* den;  // optional (recorded step)
     Vec gsum;    // optional (recorded step)
+    MaskView mv;
+    uint32_t slot;  // displacement slot of the positions read
 };
 
 __global__ void __launch_bounds__(kBlock) pbf_lambda_kernel(LambdaArgs a) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    PHX_MASK_LDS(a.grid);
+    const uint32_t i = xcd_block() * kBlock + threadIdx.x;
     if (i >= a.n) return;
+    const bool masked = masks_apply(a.mv, a.slot);
     const Float3 p = load(a.pos, i);
     const float rest = a.rho0[i];
     int cx, cy, cz;
@@ -45,7 +80,7 @@ __global__ void __launch_bounds__(kBlock) pbf_lambda_kernel(LambdaArgs a) {
     float density = 0.0F;
     Float3 gradient_sum{0.0F, 0.0F, 0.0F};
     float denominator = a.eps[i];
-    for_each_candidate(a.grid, cx, cy, cz, [&](uint32_t s) {
+    visit_candidates(a.grid, a.mv, masked, i, cx, cy, cz, delta, smask + threadIdx.x, kBlock, [&](uint32_t s) {
         const Float3 d = sub(p, load(a.pos, s));
         const float ms = a.m[s];
         density += ms * poly6(d, a.h);
@@ -84,44 +119,61 @@ struct CorrectionArgs {
     CVec x0;
     float inv_dt;
     Vec vel;
+    MaskView mv;
+    uint32_t slot;           // displacement slot of the positions read
+    uint32_t* moved_to;      // largest squared displacement of the output from x*
 };
 
+// correction_forward_kernel (pbf.cu:254-288) and project_forward_kernel
+// (pbf.cu:417-431), plus the largest displacement of the projected positions
+// from x* for the mask checks of later passes.
 __global__ void __launch_bounds__(kBlock) pbf_correction_kernel(CorrectionArgs a) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= a.n) return;
-    const Float3 p = load(a.pos, i);
-    const float rest = a.rho0[i];
-    const float lambda_i = a.lambda[i];
-    const float strength = a.strength[i];
-    const float exponent = a.exponent[i];
-    const float reference = poly6(Float3{a.radius[i], 0.0F, 0.0F}, a.h);
-    int cx, cy, cz;
-    particle_cell(a.grid, load(a.topo, i), cx, cy, cz);
-    Float3 correction{0.0F, 0.0F, 0.0F};
-    for_each_candidate(a.grid, cx, cy, cz, [&](uint32_t s) {
-        if (s == i) return;
-        const Float3 d = sub(p, load(a.pos, s));
-        const float artificial = artificial_value(d, a.h, strength, exponent, reference);
-        const float coefficient = a.m[s] / rest;
-        correction = add(correction, scale(spiky_gradient(d, a.h, a.spiky), coefficient * (lambda_i + a.lambda[s] + artificial)));
-    });
-    const Float3 candidate = add(p, correction);
-    if (a.mask != nullptr) {
-        uint32_t bits = 0u;
-        if (candidate.x < a.box.lo[0] || candidate.x > a.box.hi[0]) bits |= 1u;
-        if (candidate.y < a.box.lo[1] || candidate.y > a.box.hi[1]) bits |= 2u;
-        if (candidate.z < a.box.lo[2] || candidate.z > a.box.hi[2]) bits |= 4u;
-        a.mask[i] = static_cast<uint8_t>(bits);
+    PHX_MASK_LDS(a.grid);
+    const uint32_t i = xcd_block() * kBlock + threadIdx.x;
+    float moved2 = 0.0F;
+    if (i < a.n) {
+        const bool masked = masks_apply(a.mv, a.slot);
+        const Float3 p = load(a.pos, i);
+        const float rest = a.rho0[i];
+        const float lambda_i = a.lambda[i];
+        const float strength = a.strength[i];
+        const float exponent = a.exponent[i];
+        const float reference = poly6(Float3{a.radius[i], 0.0F, 0.0F}, a.h);
+        // with k = +-0 the term is -k wherever W(d) > 0 (no powf per pair)
+        const bool vanishes = artificial_vanishes(a.h, strength, exponent, reference);
+        int cx, cy, cz;
+        particle_cell(a.grid, load(a.topo, i), cx, cy, cz);
+        Float3 correction{0.0F, 0.0F, 0.0F};
+        visit_candidates(a.grid, a.mv, masked, i, cx, cy, cz, delta, smask + threadIdx.x, kBlock, [&](uint32_t s) {
+            if (s == i) return;
+            const Float3 d = sub(p, load(a.pos, s));
+            const float artificial = vanishes ? (poly6(d, a.h) == 0.0F ? 0.0F : -strength)
+                                              : artificial_value(d, a.h, strength, exponent, reference);
+            const float coefficient = a.m[s] / rest;
+            correction = add(correction, scale(spiky_gradient(d, a.h, a.spiky),
+                                               coefficient * (lambda_i + a.lambda[s] + artificial)));
+        });
+        const Float3 candidate = add(p, correction);
+        if (a.mask != nullptr) {
+            uint32_t bits = 0u;
+            if (candidate.x < a.box.lo[0] || candidate.x > a.box.hi[0]) bits |= 1u;
+            if (candidate.y < a.box.lo[1] || candidate.y > a.box.hi[1]) bits |= 2u;
+            if (candidate.z < a.box.lo[2] || candidate.z > a.box.hi[2]) bits |= 4u;
+            a.mask[i] = static_cast<uint8_t>(bits);
+        }
+        const Float3 projected{fminf(a.box.hi[0], fmaxf(a.box.lo[0], candidate.x)),
+                               fminf(a.box.hi[1], fmaxf(a.box.lo[1], candidate.y)),
+                               fminf(a.box.hi[2], fmaxf(a.box.lo[2], candidate.z))};
+        store(a.out, i, projected);
+        if (a.reconstruct != 0u) store(a.vel, i, scale(sub(projected, load(a.x0, i)), a.inv_dt));
+        const Float3 moved = sub(projected, load(a.topo, i));
+        moved2 = dot(moved, moved);
     }
-    const Float3 projected{fminf(a.box.hi[0], fmaxf(a.box.lo[0], candidate.x)),
-                           fminf(a.box.hi[1], fmaxf(a.box.lo[1], candidate.y)),
-                           fminf(a.box.hi[2], fmaxf(a.box.lo[2], candidate.z))};
-    store(a.out, i, projected);
-    if (a.reconstruct != 0u) store(a.vel, i, scale(sub(projected, load(a.x0, i)), a.inv_dt));
+    block_max_to<kBlock>(moved2, a.moved_to);
 }
 
 __global__ void __launch_bounds__(kBlock) pbf_reconstruct_kernel(uint32_t n, CVec pos, CVec x0, float inv_dt, Vec vel) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t i = xcd_block() * kBlock + threadIdx.x;
     if (i >= n) return;
     store(vel, i, scale(sub(load(pos, i), load(x0, i)), inv_dt));
 }
@@ -144,17 +196,21 @@ struct PostArgs {
     const float* xsph;
     const uint32_t* perm;
     Vec next_x, next_v;
+    MaskView mv;
+    uint32_t slot;  // displacement slot of the final positions
 };
 
 __global__ void __launch_bounds__(kBlock) pbf_vorticity_kernel(PostArgs a) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    PHX_MASK_LDS(a.grid);
+    const uint32_t i = xcd_block() * kBlock + threadIdx.x;
     if (i >= a.n) return;
+    const bool masked = masks_apply(a.mv, a.slot);
     const Float3 p = load(a.pos, i);
     const Float3 v = load(a.vel, i);
     int cx, cy, cz;
     particle_cell(a.grid, load(a.topo, i), cx, cy, cz);
     Float3 w{0.0F, 0.0F, 0.0F};
-    for_each_candidate(a.grid, cx, cy, cz, [&](uint32_t s) {
+    visit_candidates(a.grid, a.mv, masked, i, cx, cy, cz, delta, smask + threadIdx.x, kBlock, [&](uint32_t s) {
         if (s == i) return;
         w = add(w, cross(sub(load(a.vel, s), v), spiky_gradient(sub(p, load(a.pos, s)), a.h, a.spiky)));
     });
@@ -163,14 +219,16 @@ __global__ void __launch_bounds__(kBlock) pbf_vorticity_kernel(PostArgs a) {
 }
 
 __global__ void __launch_bounds__(kBlock) pbf_normal_kernel(PostArgs a) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    PHX_MASK_LDS(a.grid);
+    const uint32_t i = xcd_block() * kBlock + threadIdx.x;
     if (i >= a.n) return;
+    const bool masked = masks_apply(a.mv, a.slot);
     const Float3 p = load(a.pos, i);
     const float mag_i = a.magnitude[i];
     int cx, cy, cz;
     particle_cell(a.grid, load(a.topo, i), cx, cy, cz);
     Float3 g{0.0F, 0.0F, 0.0F};
-    for_each_candidate(a.grid, cx, cy, cz, [&](uint32_t s) {
+    visit_candidates(a.grid, a.mv, masked, i, cx, cy, cz, delta, smask + threadIdx.x, kBlock, [&](uint32_t s) {
         if (s == i) return;
         g = add(g, scale(spiky_gradient(sub(p, load(a.pos, s)), a.h, a.spiky), a.magnitude[s] - mag_i));
     });
@@ -185,15 +243,17 @@ __global__ void __launch_bounds__(kBlock) pbf_normal_kernel(PostArgs a) {
 }
 
 __global__ void __launch_bounds__(kBlock) pbf_xsph_kernel(PostArgs a) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    PHX_MASK_LDS(a.grid);
+    const uint32_t i = xcd_block() * kBlock + threadIdx.x;
     if (i >= a.n) return;
+    const bool masked = masks_apply(a.mv, a.slot);
     const CVec vc{a.vc.x, a.vc.y, a.vc.z};
     const Float3 p = load(a.pos, i);
     const Float3 v = load(vc, i);
     int cx, cy, cz;
     particle_cell(a.grid, load(a.topo, i), cx, cy, cz);
     Float3 smoothing{0.0F, 0.0F, 0.0F};
-    for_each_candidate(a.grid, cx, cy, cz, [&](uint32_t s) {
+    visit_candidates(a.grid, a.mv, masked, i, cx, cy, cz, delta, smask + threadIdx.x, kBlock, [&](uint32_t s) {
         if (s == i) return;
         smoothing = add(smoothing, scale(sub(load(vc, s), v), poly6(sub(p, load(a.pos, s)), a.h)));
     });
@@ -205,9 +265,9 @@ __global__ void __launch_bounds__(kBlock) pbf_xsph_kernel(PostArgs a) {
 }  // namespace
 
 const char* pbf_pass_name(int pass) {
-    static const char* names[kPbfPassCount] = {"grid",      "density_lambda",     "correction_project",
-                                               "vorticity", "normal_confinement", "xsph",
-                                               "jvp",       "vjp"};
+    static const char* names[kPbfPassCount] = {"grid",      "mask",               "density_lambda",
+                                               "correction_project", "vorticity", "normal_confinement",
+                                               "xsph",      "jvp",                "vjp"};
     return pass >= 0 && pass < kPbfPassCount ? names[pass] : "unknown";
 }
 
@@ -216,6 +276,8 @@ double pbf_pass_bytes(int pass, uint32_t particles, uint32_t cells) {
     switch (pass) {
         case kPbfGrid:  // x, v, u in, x* out (original order); x*, x + 8 parameters gathered; permutation
             return 36.0 + 12.0 + 2.0 * (24.0 + 32.0) + 4.0 + 2.0 * offsets;
+        case kPbfMask:  // x* in (the masks are an acceleration structure)
+            return 12.0 + offsets;
         case kPbfLambda:  // x*, p, m, rho0, eps in; rho, lambda out
             return 36.0 + 8.0 + offsets;
         case kPbfCorrection:  // x*, p, m, rho0, lambda, s_corr k, n, dq in; p' out (+ x in, v out on the last)
@@ -250,6 +312,8 @@ void PbfCache::allocate(uint32_t n, uint32_t cells, uint32_t pressure_iterations
     masks.resize(per);
     magnitudes.resize(n);
     normalizers.resize(n);
+    masks_skin.resize(static_cast<size_t>(kStencil) * n);
+    mask_flags.resize(kFlagDisplacement + pressure_iterations + 1u);
 }
 
 PbfSolver::PbfSolver(const phx_domain_configuration* domain, const phx_pbf_configuration* configuration,
@@ -265,6 +329,8 @@ PbfSolver::PbfSolver(const phx_domain_configuration* domain, const phx_pbf_confi
     for (VecBuffer* b : {&xs_, &x0_, &pa_, &pb_, &vel_, &vort_, &vc_}) b->resize(n);
     for (DeviceBuffer<float>* b : {&m_, &rho0_, &eps_, &strength_, &exponent_, &radius_, &xsph_, &confinement_, &magnitude_})
         b->resize(n);
+    // displacement slots: one per iterate (the first, x* itself, stays zero)
+    masks_.allocate(n, config_.pressure_iterations + 1u, domain_.config.support_radius);
 }
 
 void PbfSolver::forward_step(const phx_pbf_state& state, const phx_control& control,
@@ -320,15 +386,26 @@ void PbfSolver::forward_step(const phx_pbf_state& state, const phx_control& cont
         PassScope scope(profiler_, stream_, kPbfGrid);
         cache.neighborhood.build(stream_, domain_, step + 1u, cvec(state.particles.positions), &predict,
                                  gather.get());
+        masks_.begin_step(stream_);
     }
 
     const SortedGrid grid = cache.neighborhood.grid();
-    const uint32_t blocks = blocks_for(n, kBlock);
+    // whole blocks of kBlock particles, padded to a multiple of the XCD count
+    const uint32_t blocks = (blocks_for(n, kBlock) + 7u) / 8u * 8u;
     const float h = c.support_radius;
     const float inv_dt = 1.0F / c.time_step;
+    // a recorded step keeps its masks and flags for its derivative passes
+    uint16_t* skin = rec ? cache.masks_skin.data() : masks_.skin_mask();
+    {
+        PassScope scope(profiler_, stream_, kPbfMask);
+        MaskArgs ma{n, grid, xs.cview(), masks_.t_skin(), skin, masks_.flags()};
+        pbf_mask_kernel<<<blocks, kBlock, 0, stream_>>>(ma);
+        PHX_LAUNCH_CHECK();
+    }
+    const MaskView mv{skin, masks_.flags(), n, masks_.limit2()};
 
-    LambdaArgs la{n, h, spiky_, grid, xs.cview(), xs.cview(), m_.data(), rho0_.data(), eps_.data(),
-                  cache.densities.data(), cache.lambdas.data(), nullptr, Vec{}};
+    LambdaArgs la{n,           h,        spiky_,     grid, xs.cview(), xs.cview(), m_.data(), rho0_.data(),
+                  eps_.data(), cache.densities.data(), cache.lambdas.data(), nullptr, Vec{}, mv, 0u};
     CorrectionArgs ca{};
     ca.n = n;
     ca.h = h;
@@ -345,6 +422,7 @@ void PbfSolver::forward_step(const phx_pbf_state& state, const phx_control& cont
     ca.x0 = x0.cview();
     ca.inv_dt = inv_dt;
     ca.vel = vel.view();
+    ca.mv = mv;
 
     // positions start at x* (pbf.cpp:171) and ping-pong between two buffers
     // (a recorded step keeps every iterate)
@@ -353,6 +431,9 @@ void PbfSolver::forward_step(const phx_pbf_state& state, const phx_control& cont
     const uint32_t iterations = config_.pressure_iterations;
     for (uint32_t it = 0; it < iterations; ++it) {
         la.pos = current;
+        la.slot = it;
+        ca.slot = it;
+        ca.moved_to = masks_.flags() + kFlagDisplacement + it + 1u;
         if (rec) {
             const size_t off = static_cast<size_t>(it) * n;
             la.rho = cache.rho.data() + off;
@@ -410,6 +491,8 @@ void PbfSolver::forward_step(const phx_pbf_state& state, const phx_control& cont
     }
     post.xsph = xsph_.data();
     post.perm = cache.neighborhood.perm.data();
+    post.mv = mv;
+    post.slot = iterations;
     post.next_x = vec(next.particles.positions);
     post.next_v = vec(next.particles.velocities);
     {
@@ -428,10 +511,14 @@ void PbfSolver::forward_step(const phx_pbf_state& state, const phx_control& cont
         PHX_LAUNCH_CHECK();
     }
     if (rec) {
+        PHX_HIP(hipMemcpyAsync(cache.mask_flags.data(), masks_.flags(), sizeof(uint32_t) * masks_.flag_count(),
+                               hipMemcpyDeviceToDevice, stream_));
+        cache.mask_limit2 = masks_.limit2();
         cache.recorded = true;
         cache.iterations = iterations;
         cache.step_index = step;
     }
+    masks_.end_step(stream_);
     next.particles.step_index = step + 1u;
 }
 

@@ -3,26 +3,10 @@
 #pragma once
 
 #include "engine.hpp"
+#include "masks.hpp"
 #include "profiler.hpp"
 
 namespace phx {
-
-// FP64 SoA vector field (the adjoint views of domain/device.h).
-struct DVec {
-    double* x;
-    double* y;
-    double* z;
-};
-
-struct DVecBuffer {
-    DeviceBuffer<double> x, y, z;
-    void resize(size_t n) {
-        x.resize(n);
-        y.resize(n);
-        z.resize(n);
-    }
-    DVec view() { return {x.data(), y.data(), z.data()}; }
-};
 
 // `count` vector fields of n particles in one allocation; field k is
 // [3k n, 3k n + 3 n).
@@ -66,12 +50,18 @@ struct PbfCache {
     DeviceBuffer<uint8_t> masks;           // [iterations][n] box-collision bits of the projection
     VecBuffer velocities, vorticities, normals, confined;
     DeviceBuffer<float> magnitudes, normalizers;
+    // the step's neighbour masks and mask flags, so the derivative passes
+    // walk the same candidates as the forward passes
+    DeviceBuffer<uint16_t> masks_skin;
+    DeviceBuffer<uint32_t> mask_flags;
+    float mask_limit2 = -1.0F;
 
     void allocate(uint32_t n, uint32_t cells, uint32_t pressure_iterations);
 };
 
 enum PbfPass : int {
     kPbfGrid = 0,
+    kPbfMask,
     kPbfLambda,
     kPbfCorrection,
     kPbfVorticity,
@@ -95,6 +85,7 @@ struct PbfDerivativeWorkspace {
     DVecBuffer p_a[2], vrec_a, w_a, g_a, next_x_a, next_v_a, x_a, vel_a;
     DeviceBuffer<double> lambda_a, magnitude_a;
     DeviceBuffer<double> m_a, rho0_a, eps_a, strength_a, exponent_a, radius_a, xsph_a, confinement_a;
+    DeviceBuffer<uint8_t> vanishes;  // per particle: the artificial pressure is the signed zero -k
     void allocate(uint32_t n);
 };
 
@@ -126,6 +117,7 @@ class PbfSolver {
     hipStream_t stream_;
     float spiky_ = 0.0F;
     PassProfiler profiler_;
+    MaskSet masks_;
     // sorted-order workspace
     VecBuffer xs_, x0_, pa_, pb_, vel_, vort_, vc_;
     DeviceBuffer<float> m_, rho0_, eps_, strength_, exponent_, radius_, xsph_, confinement_, magnitude_;

@@ -24,12 +24,21 @@
 // projection, vorticity (+ reconstruct), normal + confinement, XSPH.
 #include <cmath>
 
+#include "neighbor_mask.hpp"
 #include "pbf.hpp"
 
 namespace phx {
 namespace {
 
 constexpr uint32_t kBlock = 256;
+
+// LDS of a masked neighbour pass: the lanes' mask columns and the slot table
+// (every thread reaches the barrier before any returns).
+#define PHX_MASK_LDS(grid)                          \
+    __shared__ uint16_t smask[kStencil * kBlock];   \
+    __shared__ int delta[kStencil];                 \
+    fill_slot_deltas(grid, delta);                  \
+    __syncthreads()
 
 struct D3 {
     double x, y, z;
@@ -80,12 +89,22 @@ struct Artificial {
     float d_strength, d_exponent, d_radius;
 };
 
-__device__ __forceinline__ Artificial artificial_full(Float3 d, float h, float strength, float exponent, float radius) {
+// With `vanishes` (artificial_vanishes of the particle) the value is -k and
+// its gradient, exponent and radius derivatives are signed zeros that no sum
+// can notice; only the strength derivative -(W / W(dq))^n needs a powf, and
+// only when `strength_wanted`.
+__device__ __forceinline__ Artificial artificial_full(Float3 d, float h, float strength, float exponent, float radius,
+                                                      bool vanishes = false, bool strength_wanted = true) {
     Artificial a{0.0F, {0.0F, 0.0F, 0.0F}, 0.0F, 0.0F, 0.0F};
     const float reference = poly6(Float3{radius, 0.0F, 0.0F}, h);
     const float kernel = poly6(d, h);
     if (kernel == 0.0F) return a;
     const float ratio = kernel / reference;
+    if (vanishes) {
+        a.value = -strength;
+        if (strength_wanted) a.d_strength = -powf(ratio, exponent);
+        return a;
+    }
     const float power = powf(ratio, exponent);
     a.value = -strength * power;
     a.d_strength = -power;
@@ -106,13 +125,14 @@ struct Step {
     SortedGrid grid;
     CVec xs, x0;
     const float *m, *rho0, *eps, *strength, *exponent, *radius, *xsph, *confinement;
+    MaskView mv;  // the recorded step's masks and flags
 };
 
 __device__ __forceinline__ void cell_of_topology(const Step& s, uint32_t i, int& cx, int& cy, int& cz) {
     particle_cell(s.grid, load(s.xs, i), cx, cy, cz);
 }
 
-__device__ __forceinline__ D3 masked(D3 b, uint8_t m) {
+__device__ __forceinline__ D3 unclamped(D3 b, uint8_t m) {
     if (m & 1u) b.x = 0.0;
     if (m & 2u) b.y = 0.0;
     if (m & 4u) b.z = 0.0;
@@ -122,6 +142,7 @@ __device__ __forceinline__ D3 masked(D3 b, uint8_t m) {
 // ----------------------------------------------------------------------- VJP
 struct PostVjpArgs {
     Step s;
+    uint32_t slot;  // displacement slot of the positions read
     CVec pos, vc, w, nrm;
     const float* nz;
     DVec next_x_a, next_v_a;  // adjoints of the next state, sorted
@@ -133,8 +154,10 @@ struct PostVjpArgs {
 // xsph_vjp_kernel (pbf.cu:756-794), confinement_vjp_kernel (pbf.cu:688-698)
 // and the adjoint of the normalised gradient (pbf.cu:630-633).
 __global__ void __launch_bounds__(kBlock) pbf_vjp_post_kernel(PostVjpArgs a) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    PHX_MASK_LDS(a.s.grid);
+    const uint32_t i = xcd_block() * kBlock + threadIdx.x;
     if (i >= a.s.n) return;
+    const bool masked = masks_apply(a.s.mv, a.slot);
     const Step& s = a.s;
     const Float3 pi = load(a.pos, i);
     const D3 out = load(a.next_v_a, i);
@@ -145,7 +168,7 @@ __global__ void __launch_bounds__(kBlock) pbf_vjp_post_kernel(PostVjpArgs a) {
     double ccon = 0.0;
     int cx, cy, cz;
     cell_of_topology(s, i, cx, cy, cz);
-    for_each_candidate(s.grid, cx, cy, cz, [&](uint32_t j) {
+    visit_candidates(s.grid, s.mv, masked, i, cx, cy, cz, delta, smask + threadIdx.x, kBlock, [&](uint32_t j) {
         if (j == i) return;
         const Float3 dd = sub(pi, load(a.pos, j));
         const double wgt = static_cast<double>(poly6(dd, s.h));
@@ -176,6 +199,7 @@ __global__ void __launch_bounds__(kBlock) pbf_vjp_post_kernel(PostVjpArgs a) {
 
 struct NormalVjpArgs {
     Step s;
+    uint32_t slot;  // displacement slot of the positions read
     CVec pos, w;
     const float* mag;
     DVec p_a, w_a, g_a;
@@ -183,8 +207,10 @@ struct NormalVjpArgs {
 
 // normal_vjp_kernel (pbf.cu:635-672) and the magnitude adjoint into omega.
 __global__ void __launch_bounds__(kBlock) pbf_vjp_normal_kernel(NormalVjpArgs a) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    PHX_MASK_LDS(a.s.grid);
+    const uint32_t i = xcd_block() * kBlock + threadIdx.x;
     if (i >= a.s.n) return;
+    const bool masked = masks_apply(a.s.mv, a.slot);
     const Step& s = a.s;
     const Float3 pi = load(a.pos, i);
     const D3 gi = load(a.g_a, i);
@@ -193,7 +219,7 @@ __global__ void __launch_bounds__(kBlock) pbf_vjp_normal_kernel(NormalVjpArgs a)
     double mcon = 0.0;
     int cx, cy, cz;
     cell_of_topology(s, i, cx, cy, cz);
-    for_each_candidate(s.grid, cx, cy, cz, [&](uint32_t j) {
+    visit_candidates(s.grid, s.mv, masked, i, cx, cy, cz, delta, smask + threadIdx.x, kBlock, [&](uint32_t j) {
         if (j == i) return;
         const Float3 dd = sub(pi, load(a.pos, j));
         const D3 gr = d3(spiky_gradient(dd, s.h, s.spiky));
@@ -208,6 +234,7 @@ __global__ void __launch_bounds__(kBlock) pbf_vjp_normal_kernel(NormalVjpArgs a)
 
 struct VorticityVjpArgs {
     Step s;
+    uint32_t slot;  // displacement slot of the positions read
     CVec pos, vrec;
     DVec p_a, vrec_a, w_a, x_a;
 };
@@ -215,8 +242,10 @@ struct VorticityVjpArgs {
 // vorticity_vjp_kernel (pbf.cu:529-563) and reconstruct_vjp_kernel
 // (pbf.cu:468-474).
 __global__ void __launch_bounds__(kBlock) pbf_vjp_vorticity_kernel(VorticityVjpArgs a) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    PHX_MASK_LDS(a.s.grid);
+    const uint32_t i = xcd_block() * kBlock + threadIdx.x;
     if (i >= a.s.n) return;
+    const bool masked = masks_apply(a.s.mv, a.slot);
     const Step& s = a.s;
     const Float3 pi = load(a.pos, i);
     const D3 vi = d3(load(a.vrec, i));
@@ -225,7 +254,7 @@ __global__ void __launch_bounds__(kBlock) pbf_vjp_vorticity_kernel(VorticityVjpA
     D3 vcon{0.0, 0.0, 0.0};
     int cx, cy, cz;
     cell_of_topology(s, i, cx, cy, cz);
-    for_each_candidate(s.grid, cx, cy, cz, [&](uint32_t j) {
+    visit_candidates(s.grid, s.mv, masked, i, cx, cy, cz, delta, smask + threadIdx.x, kBlock, [&](uint32_t j) {
         if (j == i) return;
         const Float3 dd = sub(pi, load(a.pos, j));
         const D3 gr = d3(spiky_gradient(dd, s.h, s.spiky));
@@ -242,6 +271,9 @@ __global__ void __launch_bounds__(kBlock) pbf_vjp_vorticity_kernel(VorticityVjpA
 
 struct CorrectionVjpArgs {
     Step s;
+    uint32_t slot;  // displacement slot of the positions read
+    const uint8_t* vanishes;  // per particle: artificial_vanishes
+    uint32_t strength_wanted;
     CVec pos;  // positions entering the iteration
     const float* lambda;
     const uint8_t* mask;
@@ -254,25 +286,28 @@ struct CorrectionVjpArgs {
 // project_vjp_kernel (pbf.cu:445-454) and correction_vjp_kernel
 // (pbf.cu:335-415).
 __global__ void __launch_bounds__(kBlock) pbf_vjp_correction_kernel(CorrectionVjpArgs a) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    PHX_MASK_LDS(a.s.grid);
+    const uint32_t i = xcd_block() * kBlock + threadIdx.x;
     if (i >= a.s.n) return;
+    const bool masked = masks_apply(a.s.mv, a.slot);
     const Step& s = a.s;
     const double h = static_cast<double>(s.h), k = static_cast<double>(s.spiky);
     const Float3 pi = load(a.pos, i);
     const double rho0 = static_cast<double>(s.rho0[i]);
-    const D3 oa = masked(load(a.p_in, i), a.mask[i]);
+    const D3 oa = unclamped(load(a.p_in, i), a.mask[i]);
     const float lam_i = a.lambda[i];
     D3 pcon{0.0, 0.0, 0.0};
     double lcon = 0.0, mcon = 0.0, rcon = 0.0, scon = 0.0, econ = 0.0, qcon = 0.0;
     int cx, cy, cz;
     cell_of_topology(s, i, cx, cy, cz);
-    for_each_candidate(s.grid, cx, cy, cz, [&](uint32_t j) {
+    visit_candidates(s.grid, s.mv, masked, i, cx, cy, cz, delta, smask + threadIdx.x, kBlock, [&](uint32_t j) {
         if (j == i) return;
         const Float3 dd = sub(pi, load(a.pos, j));
         const D3 gr = d3(spiky_gradient(dd, s.h, s.spiky));
         const float lam_j = a.lambda[j];
         // i as the centre
-        const Artificial art = artificial_full(dd, s.h, s.strength[i], s.exponent[i], s.radius[i]);
+        const Artificial art = artificial_full(dd, s.h, s.strength[i], s.exponent[i], s.radius[i], a.vanishes[i] != 0u,
+                                               a.strength_wanted != 0u);
         const double cf = static_cast<double>(s.m[j]) / rho0;
         const double mu = static_cast<double>(lam_i) + static_cast<double>(lam_j) + static_cast<double>(art.value);
         const double sa = cf * dot(oa, gr);
@@ -285,11 +320,12 @@ __global__ void __launch_bounds__(kBlock) pbf_vjp_correction_kernel(CorrectionVj
         qcon += sa * static_cast<double>(art.d_radius);
         // i as a neighbour of j: d_ji = -d_ij
         const Float3 rd = scale(dd, -1.0F);
-        const Artificial rart = artificial_full(rd, s.h, s.strength[j], s.exponent[j], s.radius[j]);
+        const Artificial rart = artificial_full(rd, s.h, s.strength[j], s.exponent[j], s.radius[j], a.vanishes[j] != 0u,
+                                                a.strength_wanted != 0u);
         const double rrho0 = static_cast<double>(s.rho0[j]);
         const double rcf = static_cast<double>(s.m[i]) / rrho0;
         const double rmu = static_cast<double>(lam_j) + static_cast<double>(lam_i) + static_cast<double>(rart.value);
-        const D3 ob = masked(load(a.p_in, j), a.mask[j]);
+        const D3 ob = unclamped(load(a.p_in, j), a.mask[j]);
         const D3 rgr = d3(spiky_gradient(rd, s.h, s.spiky));
         const double rsa = rcf * dot(ob, rgr);
         pcon = sub(pcon, spiky_hessian(rd, scale(ob, rcf * rmu), h, k));
@@ -308,6 +344,7 @@ __global__ void __launch_bounds__(kBlock) pbf_vjp_correction_kernel(CorrectionVj
 
 struct LambdaVjpArgs {
     Step s;
+    uint32_t slot;  // displacement slot of the positions read
     CVec pos;
     const float *rho, *den;
     CVec gsum;
@@ -320,8 +357,10 @@ struct LambdaVjpArgs {
 // poly6), whose density adjoint -lambda_bar / (D rho0) is rebuilt per
 // neighbour.
 __global__ void __launch_bounds__(kBlock) pbf_vjp_lambda_kernel(LambdaVjpArgs a) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    PHX_MASK_LDS(a.s.grid);
+    const uint32_t i = xcd_block() * kBlock + threadIdx.x;
     if (i >= a.s.n) return;
+    const bool masked = masks_apply(a.s.mv, a.slot);
     const Step& s = a.s;
     const double h = static_cast<double>(s.h), k = static_cast<double>(s.spiky);
     const Float3 pi = load(a.pos, i);
@@ -338,7 +377,7 @@ __global__ void __launch_bounds__(kBlock) pbf_vjp_lambda_kernel(LambdaVjpArgs a)
     double rcon = lam_a * static_cast<double>(a.rho[i]) / (den * rho0 * rho0);
     int cx, cy, cz;
     cell_of_topology(s, i, cx, cy, cz);
-    for_each_candidate(s.grid, cx, cy, cz, [&](uint32_t j) {
+    visit_candidates(s.grid, s.mv, masked, i, cx, cy, cz, delta, smask + threadIdx.x, kBlock, [&](uint32_t j) {
         const Float3 dd = sub(pi, load(a.pos, j));
         const double jrho0 = static_cast<double>(s.rho0[j]);
         const double jden = static_cast<double>(a.den[j]);
@@ -388,7 +427,7 @@ struct ScatterVjpArgs {
 // predict_vjp_kernel (pbf.cu:86-93) and the accumulation of every adjoint
 // into the caller's original-order arrays.
 __global__ void __launch_bounds__(kBlock) pbf_vjp_scatter_kernel(ScatterVjpArgs a) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t i = xcd_block() * kBlock + threadIdx.x;
     if (i >= a.n) return;
     const uint32_t o = a.perm[i];
     const D3 b = load(a.p_a, i);
@@ -413,6 +452,15 @@ __global__ void __launch_bounds__(kBlock) pbf_vjp_scatter_kernel(ScatterVjpArgs 
         if (a.original[c] != nullptr) a.original[c][o] += a.sorted[c][i];
 }
 
+__global__ void __launch_bounds__(kBlock) pbf_vanishes_kernel(uint32_t n, float h, const float* strength,
+                                                              const float* exponent, const float* radius,
+                                                              uint8_t* vanishes) {
+    const uint32_t i = xcd_block() * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const float reference = poly6(Float3{radius[i], 0.0F, 0.0F}, h);
+    vanishes[i] = artificial_vanishes(h, strength[i], exponent[i], reference) ? 1u : 0u;
+}
+
 struct GatherAdjointArgs {
     uint32_t n;
     const uint32_t* perm;
@@ -421,7 +469,7 @@ struct GatherAdjointArgs {
 };
 
 __global__ void __launch_bounds__(kBlock) pbf_gather_adjoint_kernel(GatherAdjointArgs a) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t i = xcd_block() * kBlock + threadIdx.x;
     if (i >= a.n) return;
     const uint32_t o = a.perm[i];
     store(a.x_a, i, D3{a.xx[o], a.xy[o], a.xz[o]});
@@ -439,7 +487,7 @@ struct PredictJvpArgs {
 
 // predict_jvp_kernel (pbf.cu:80-84), gathered into sorted order.
 __global__ void __launch_bounds__(kBlock) pbf_jvp_predict_kernel(PredictJvpArgs a) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t i = xcd_block() * kBlock + threadIdx.x;
     if (i >= a.n) return;
     const uint32_t o = a.perm[i];
     const Float3 xt = load(a.x_t, o);
@@ -454,6 +502,7 @@ struct TangentParams {
 struct LambdaJvpArgs {
     Step s;
     TangentParams t;
+    uint32_t slot;  // displacement slot of the positions read
     CVec pos, p_t;
     const float *rho, *den;
     float* lambda_t;
@@ -462,8 +511,10 @@ struct LambdaJvpArgs {
 // density_jvp_kernel (density.cu:42-67, poly6) and lambda_jvp_kernel
 // (pbf.cu:129-176); the density tangent is the particle's own.
 __global__ void __launch_bounds__(kBlock) pbf_jvp_lambda_kernel(LambdaJvpArgs a) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    PHX_MASK_LDS(a.s.grid);
+    const uint32_t i = xcd_block() * kBlock + threadIdx.x;
     if (i >= a.s.n) return;
+    const bool masked = masks_apply(a.s.mv, a.slot);
     const Step& s = a.s;
     const Float3 pi = load(a.pos, i), ti = load(a.p_t, i);
     const float rho0 = s.rho0[i], rho0_t = a.t.rho0[i];
@@ -472,7 +523,7 @@ __global__ void __launch_bounds__(kBlock) pbf_jvp_lambda_kernel(LambdaJvpArgs a)
     float den_t = a.t.eps[i];
     int cx, cy, cz;
     cell_of_topology(s, i, cx, cy, cz);
-    for_each_candidate(s.grid, cx, cy, cz, [&](uint32_t j) {
+    visit_candidates(s.grid, s.mv, masked, i, cx, cy, cz, delta, smask + threadIdx.x, kBlock, [&](uint32_t j) {
         const Float3 dd = sub(pi, load(a.pos, j));
         const Float3 dd_t = sub(ti, load(a.p_t, j));
         rho_t += a.t.m[j] * poly6(dd, s.h) + s.m[j] * dot(poly6_gradient(dd, s.h), dd_t);
@@ -497,6 +548,7 @@ __global__ void __launch_bounds__(kBlock) pbf_jvp_lambda_kernel(LambdaJvpArgs a)
 struct CorrectionJvpArgs {
     Step s;
     TangentParams t;
+    uint32_t slot;  // displacement slot of the positions read
     CVec pos, p_t;
     const float *lambda, *lambda_t;
     const uint8_t* mask;
@@ -506,15 +558,17 @@ struct CorrectionJvpArgs {
 // correction_jvp_kernel (pbf.cu:290-333) and project_jvp_kernel
 // (pbf.cu:433-443).
 __global__ void __launch_bounds__(kBlock) pbf_jvp_correction_kernel(CorrectionJvpArgs a) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    PHX_MASK_LDS(a.s.grid);
+    const uint32_t i = xcd_block() * kBlock + threadIdx.x;
     if (i >= a.s.n) return;
+    const bool masked = masks_apply(a.s.mv, a.slot);
     const Step& s = a.s;
     const Float3 pi = load(a.pos, i), ti = load(a.p_t, i);
     const float rho0 = s.rho0[i], rho0_t = a.t.rho0[i];
     Float3 res{0.0F, 0.0F, 0.0F};
     int cx, cy, cz;
     cell_of_topology(s, i, cx, cy, cz);
-    for_each_candidate(s.grid, cx, cy, cz, [&](uint32_t j) {
+    visit_candidates(s.grid, s.mv, masked, i, cx, cy, cz, delta, smask + threadIdx.x, kBlock, [&](uint32_t j) {
         if (j == i) return;
         const Float3 dd = sub(pi, load(a.pos, j));
         const Float3 dd_t = sub(ti, load(a.p_t, j));
@@ -539,6 +593,7 @@ __global__ void __launch_bounds__(kBlock) pbf_jvp_correction_kernel(CorrectionJv
 struct PostJvpArgs {
     Step s;
     TangentParams t;
+    uint32_t slot;  // displacement slot of the positions read
     CVec pos, p_t, x_t, vrec, w, nrm, vc;
     const float *mag, *nz;
     Vec w_t, vc_t;
@@ -554,15 +609,17 @@ __device__ __forceinline__ Float3 vrec_tangent(const PostJvpArgs& a, uint32_t j)
 // reconstruct_jvp_kernel (pbf.cu:462-466), vorticity_jvp_kernel
 // (pbf.cu:499-527) and magnitude_jvp_kernel (pbf.cu:595-599).
 __global__ void __launch_bounds__(kBlock) pbf_jvp_vorticity_kernel(PostJvpArgs a) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    PHX_MASK_LDS(a.s.grid);
+    const uint32_t i = xcd_block() * kBlock + threadIdx.x;
     if (i >= a.s.n) return;
+    const bool masked = masks_apply(a.s.mv, a.slot);
     const Step& s = a.s;
     const Float3 pi = load(a.pos, i), ti = load(a.p_t, i);
     const Float3 vi = load(a.vrec, i), vti = vrec_tangent(a, i);
     Float3 res{0.0F, 0.0F, 0.0F};
     int cx, cy, cz;
     cell_of_topology(s, i, cx, cy, cz);
-    for_each_candidate(s.grid, cx, cy, cz, [&](uint32_t j) {
+    visit_candidates(s.grid, s.mv, masked, i, cx, cy, cz, delta, smask + threadIdx.x, kBlock, [&](uint32_t j) {
         if (j == i) return;
         const Float3 dd = sub(pi, load(a.pos, j));
         const Float3 dd_t = sub(ti, load(a.p_t, j));
@@ -578,14 +635,16 @@ __global__ void __launch_bounds__(kBlock) pbf_jvp_vorticity_kernel(PostJvpArgs a
 // normal_jvp_kernel (pbf.cu:600-628) and confinement_jvp_kernel
 // (pbf.cu:680-686).
 __global__ void __launch_bounds__(kBlock) pbf_jvp_normal_kernel(PostJvpArgs a) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    PHX_MASK_LDS(a.s.grid);
+    const uint32_t i = xcd_block() * kBlock + threadIdx.x;
     if (i >= a.s.n) return;
+    const bool masked = masks_apply(a.s.mv, a.slot);
     const Step& s = a.s;
     const Float3 pi = load(a.pos, i), ti = load(a.p_t, i);
     Float3 gt{0.0F, 0.0F, 0.0F};
     int cx, cy, cz;
     cell_of_topology(s, i, cx, cy, cz);
-    for_each_candidate(s.grid, cx, cy, cz, [&](uint32_t j) {
+    visit_candidates(s.grid, s.mv, masked, i, cx, cy, cz, delta, smask + threadIdx.x, kBlock, [&](uint32_t j) {
         if (j == i) return;
         const Float3 dd = sub(pi, load(a.pos, j));
         const Float3 dd_t = sub(ti, load(a.p_t, j));
@@ -606,8 +665,10 @@ __global__ void __launch_bounds__(kBlock) pbf_jvp_normal_kernel(PostJvpArgs a) {
 // xsph_jvp_kernel (pbf.cu:723-754), written to original order together with
 // the position tangent.
 __global__ void __launch_bounds__(kBlock) pbf_jvp_xsph_kernel(PostJvpArgs a) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    PHX_MASK_LDS(a.s.grid);
+    const uint32_t i = xcd_block() * kBlock + threadIdx.x;
     if (i >= a.s.n) return;
+    const bool masked = masks_apply(a.s.mv, a.slot);
     const Step& s = a.s;
     const CVec vct{a.vc_t.x, a.vc_t.y, a.vc_t.z};
     const Float3 pi = load(a.pos, i), ti = load(a.p_t, i);
@@ -615,7 +676,7 @@ __global__ void __launch_bounds__(kBlock) pbf_jvp_xsph_kernel(PostJvpArgs a) {
     Float3 sm{0.0F, 0.0F, 0.0F}, sm_t{0.0F, 0.0F, 0.0F};
     int cx, cy, cz;
     cell_of_topology(s, i, cx, cy, cz);
-    for_each_candidate(s.grid, cx, cy, cz, [&](uint32_t j) {
+    visit_candidates(s.grid, s.mv, masked, i, cx, cy, cz, delta, smask + threadIdx.x, kBlock, [&](uint32_t j) {
         if (j == i) return;
         const Float3 dd = sub(pi, load(a.pos, j));
         const Float3 dd_t = sub(ti, load(a.p_t, j));
@@ -643,6 +704,7 @@ void PbfDerivativeWorkspace::allocate(uint32_t n) {
     for (DeviceBuffer<double>* b : {&lambda_a, &magnitude_a, &m_a, &rho0_a, &eps_a, &strength_a, &exponent_a, &radius_a,
                                     &xsph_a, &confinement_a})
         b->resize(n);
+    vanishes.resize(n);
 }
 
 void PbfSolver::gather_parameters(const phx_pbf_parameters& parameters, const PbfCache& cache) {
@@ -706,7 +768,7 @@ void PbfSolver::jvp_step(const phx_pbf_parameters& parameters, const PbfCache& c
     }
     if (ch.get().count > 0) gather_to_sorted(stream_, cache.neighborhood.perm.data(), n, ch.get());
 
-    const uint32_t blocks = blocks_for(n, kBlock);
+    const uint32_t blocks = (blocks_for(n, kBlock) + 7u) / 8u * 8u;
     const uint32_t iterations = cache.iterations;
     Step s{n,
            c.support_radius,
@@ -723,7 +785,8 @@ void PbfSolver::jvp_step(const phx_pbf_parameters& parameters, const PbfCache& c
            dw_.exponent.data(),
            dw_.radius.data(),
            dw_.xsph.data(),
-           dw_.confinement.data()};
+           dw_.confinement.data(),
+           MaskView{cache.masks_skin.data(), cache.mask_flags.data(), n, cache.mask_limit2}};
     const TangentParams tp{dw_.m_t.data(),        dw_.rho0_t.data(),   dw_.eps_t.data(),  dw_.strength_t.data(),
                            dw_.exponent_t.data(), dw_.radius_t.data(), dw_.xsph_t.data(), dw_.confinement_t.data()};
     PredictJvpArgs pa{n,
@@ -740,12 +803,13 @@ void PbfSolver::jvp_step(const phx_pbf_parameters& parameters, const PbfCache& c
     for (uint32_t it = 0; it < iterations; ++it) {
         const CVec pos = it == 0u ? cache.xs.cview() : cache.positions.cview(it - 1u);
         const size_t off = static_cast<size_t>(it) * n;
-        LambdaJvpArgs la{s, tp, pos, dw_.p_t[cur].cview(), cache.rho.data() + off, cache.den.data() + off,
+        LambdaJvpArgs la{s, tp, it, pos, dw_.p_t[cur].cview(), cache.rho.data() + off, cache.den.data() + off,
                          dw_.lambda_t.data()};
         pbf_jvp_lambda_kernel<<<blocks, kBlock, 0, stream_>>>(la);
         PHX_LAUNCH_CHECK();
         CorrectionJvpArgs ca{s,
                              tp,
+                             it,
                              pos,
                              dw_.p_t[cur].cview(),
                              cache.lambda.data() + off,
@@ -759,6 +823,7 @@ void PbfSolver::jvp_step(const phx_pbf_parameters& parameters, const PbfCache& c
     PostJvpArgs post{};
     post.s = s;
     post.t = tp;
+    post.slot = iterations;
     post.pos = iterations == 0u ? cache.xs.cview() : cache.positions.cview(iterations - 1u);
     post.p_t = dw_.p_t[cur].cview();
     post.x_t = dw_.x_t.cview();
@@ -795,7 +860,7 @@ void PbfSolver::vjp_step(const phx_pbf_parameters& parameters, const PbfCache& c
     PassScope scope(profiler_, stream_, kPbfVjp);
     dw_.allocate(n);
     gather_parameters(parameters, cache);
-    const uint32_t blocks = blocks_for(n, kBlock);
+    const uint32_t blocks = (blocks_for(n, kBlock) + 7u) / 8u * 8u;
     const uint32_t iterations = cache.iterations;
     const uint32_t* perm = cache.neighborhood.perm.data();
     Step s{n,
@@ -813,11 +878,15 @@ void PbfSolver::vjp_step(const phx_pbf_parameters& parameters, const PbfCache& c
            dw_.exponent.data(),
            dw_.radius.data(),
            dw_.xsph.data(),
-           dw_.confinement.data()};
+           dw_.confinement.data(),
+           MaskView{cache.masks_skin.data(), cache.mask_flags.data(), n, cache.mask_limit2}};
     // sorted parameter adjoints start at zero (+0.0 is the all-zero bit pattern)
     for (DeviceBuffer<double>* b : {&dw_.m_a, &dw_.rho0_a, &dw_.eps_a, &dw_.strength_a, &dw_.exponent_a,
                                     &dw_.radius_a, &dw_.xsph_a, &dw_.confinement_a})
         fill_u32(stream_, reinterpret_cast<uint32_t*>(b->data()), 2u * static_cast<size_t>(n), 0u);
+    pbf_vanishes_kernel<<<blocks, kBlock, 0, stream_>>>(n, c.support_radius, dw_.strength.data(), dw_.exponent.data(),
+                                                         dw_.radius.data(), dw_.vanishes.data());
+    PHX_LAUNCH_CHECK();
     GatherAdjointArgs ga{n, perm, nx.x, nx.y, nx.z, nv.x, nv.y, nv.z, dw_.next_x_a.view(), dw_.next_v_a.view()};
     pbf_gather_adjoint_kernel<<<blocks, kBlock, 0, stream_>>>(ga);
     PHX_LAUNCH_CHECK();
@@ -825,6 +894,7 @@ void PbfSolver::vjp_step(const phx_pbf_parameters& parameters, const PbfCache& c
     const CVec final_pos = iterations == 0u ? cache.xs.cview() : cache.positions.cview(iterations - 1u);
     int cur = 0;
     PostVjpArgs pv{s,
+                   iterations,
                    final_pos,
                    cache.confined.cview(),
                    cache.vorticities.cview(),
@@ -840,11 +910,11 @@ void PbfSolver::vjp_step(const phx_pbf_parameters& parameters, const PbfCache& c
                    dw_.confinement_a.data()};
     pbf_vjp_post_kernel<<<blocks, kBlock, 0, stream_>>>(pv);
     PHX_LAUNCH_CHECK();
-    NormalVjpArgs nva{s, final_pos, cache.vorticities.cview(), cache.magnitudes.data(), dw_.p_a[cur].view(),
+    NormalVjpArgs nva{s, iterations, final_pos, cache.vorticities.cview(), cache.magnitudes.data(), dw_.p_a[cur].view(),
                       dw_.w_a.view(), dw_.g_a.view()};
     pbf_vjp_normal_kernel<<<blocks, kBlock, 0, stream_>>>(nva);
     PHX_LAUNCH_CHECK();
-    VorticityVjpArgs vva{s, final_pos, cache.velocities.cview(), dw_.p_a[cur].view(), dw_.vrec_a.view(),
+    VorticityVjpArgs vva{s, iterations, final_pos, cache.velocities.cview(), dw_.p_a[cur].view(), dw_.vrec_a.view(),
                          dw_.w_a.view(), dw_.x_a.view()};
     pbf_vjp_vorticity_kernel<<<blocks, kBlock, 0, stream_>>>(vva);
     PHX_LAUNCH_CHECK();
@@ -852,6 +922,9 @@ void PbfSolver::vjp_step(const phx_pbf_parameters& parameters, const PbfCache& c
         const CVec pos = it == 0u ? cache.xs.cview() : cache.positions.cview(it - 1u);
         const size_t off = static_cast<size_t>(it) * n;
         CorrectionVjpArgs cva{s,
+                              it,
+                              dw_.vanishes.data(),
+                              parameter_a.artificial_pressure_strength != nullptr ? 1u : 0u,
                               pos,
                               cache.lambda.data() + off,
                               cache.masks.data() + off,
@@ -866,6 +939,7 @@ void PbfSolver::vjp_step(const phx_pbf_parameters& parameters, const PbfCache& c
         pbf_vjp_correction_kernel<<<blocks, kBlock, 0, stream_>>>(cva);
         PHX_LAUNCH_CHECK();
         LambdaVjpArgs lva{s,
+                          it,
                           pos,
                           cache.rho.data() + off,
                           cache.den.data() + off,

@@ -13,18 +13,21 @@ from typing import List, Optional
 
 import torch
 
-from .particle import PBF, Control
+from .particle import PBF, Control, ParticleParameterAdjoint
 
 
 class PbfRollout:
     """A fixed-length rollout of one PBF solver created in differentiable mode."""
 
-    def __init__(self, solver: PBF, steps: int, checkpoint_every: int = 1):
+    def __init__(self, solver: PBF, steps: int, checkpoint_every: int = 1, parameter_gradients: bool = True):
         if steps < 0 or checkpoint_every < 1:
             raise ValueError("steps must be >= 0 and checkpoint_every >= 1")
         self.solver = solver
         self.steps = steps
         self.checkpoint_every = checkpoint_every
+        # without parameter gradients the VJPs receive null parameter adjoints
+        # and skip the terms that only feed them
+        self.parameter_gradients = parameter_gradients
         self._states: List[Optional[PBF.State]] = []
         self._caches: List[Optional[PBF.StepCache]] = []
         self._control: Optional[Control] = None
@@ -57,10 +60,15 @@ class PbfRollout:
     def backward(self, final_adjoint: PBF.StateAdjoint):
         """Chains the VJPs from the adjoint of the final state. Returns
         (adjoint of the initial state, control adjoint, parameter adjoint),
-        all freshly allocated and accumulated over every step."""
+        all freshly allocated and accumulated over every step (the parameter
+        adjoint is None without parameter gradients)."""
         solver = self.solver
         control_adjoint = solver.allocate_control_adjoint()
-        parameter_adjoint = solver.allocate_parameter_adjoint()
+        if self.parameter_gradients:
+            parameter_adjoint = solver.allocate_parameter_adjoint()
+        else:
+            parameter_adjoint = PBF.ParameterAdjoint(ParticleParameterAdjoint(None, None, None, None),
+                                                     None, None, None, None, None, None)
         adjoint = solver.allocate_state_adjoint()
         solver.copy_state_adjoint(final_adjoint, adjoint)
         k = self.steps
@@ -81,7 +89,7 @@ class PbfRollout:
                                 adjoint, previous, control_adjoint, parameter_adjoint)
                 adjoint = previous
             k = first if self.checkpoint_every > 1 else k - 1
-        return adjoint, control_adjoint, parameter_adjoint
+        return adjoint, control_adjoint, parameter_adjoint if self.parameter_gradients else None
 
     def release(self) -> None:
         """Frees the recorded states and caches."""
