@@ -148,6 +148,17 @@ typedef struct phx_particle_parameter_adjoint {
     double* surface_tensions;
 } phx_particle_parameter_adjoint;
 
+/* ParticleStateTangent / ParticleStateAdjoint (domain/domain.ixx:76-88) */
+typedef struct phx_particle_state_tangent {
+    phx_vector_view positions;
+    phx_vector_view velocities;
+} phx_particle_state_tangent;
+
+typedef struct phx_particle_state_adjoint {
+    phx_adjoint_vector_view positions;
+    phx_adjoint_vector_view velocities;
+} phx_particle_state_adjoint;
+
 /* Grid resolution ceil((max - min) / h) per axis (neighborhood.cpp:25-30). */
 phx_status phx_grid_resolution(const phx_domain_configuration* domain, uint32_t resolution[3]);
 
@@ -242,6 +253,48 @@ typedef struct phx_sph_dfsph_parameters {
     phx_dfsph_parameters method;
 } phx_sph_dfsph_parameters;
 
+/* DFSPH::StateTangent / StateAdjoint / ParameterTangent / ParameterAdjoint
+ * (sph/sph.ixx:206-225) and their SPH<DFSPH> wrappers (sph/sph.ixx:279-298). */
+typedef struct phx_dfsph_state_tangent {
+    float* warm_divergence_pressure;
+    float* warm_density_pressure;
+} phx_dfsph_state_tangent;
+
+typedef struct phx_dfsph_state_adjoint {
+    double* warm_divergence_pressure;
+    double* warm_density_pressure;
+} phx_dfsph_state_adjoint;
+
+typedef struct phx_dfsph_parameter_tangent {
+    const float* divergence_relaxation;
+    const float* density_relaxation;
+} phx_dfsph_parameter_tangent;
+
+typedef struct phx_dfsph_parameter_adjoint {
+    double* divergence_relaxation;
+    double* density_relaxation;
+} phx_dfsph_parameter_adjoint;
+
+typedef struct phx_sph_dfsph_state_tangent {
+    phx_particle_state_tangent particles;
+    phx_dfsph_state_tangent method;
+} phx_sph_dfsph_state_tangent;
+
+typedef struct phx_sph_dfsph_state_adjoint {
+    phx_particle_state_adjoint particles;
+    phx_dfsph_state_adjoint method;
+} phx_sph_dfsph_state_adjoint;
+
+typedef struct phx_sph_dfsph_parameter_tangent {
+    phx_particle_parameter_tangent particles;
+    phx_dfsph_parameter_tangent method;
+} phx_sph_dfsph_parameter_tangent;
+
+typedef struct phx_sph_dfsph_parameter_adjoint {
+    phx_particle_parameter_adjoint particles;
+    phx_dfsph_parameter_adjoint method;
+} phx_sph_dfsph_parameter_adjoint;
+
 typedef struct phx_dfsph phx_dfsph;
 typedef struct phx_dfsph_step_cache phx_dfsph_step_cache;
 
@@ -273,6 +326,35 @@ phx_status phx_dfsph_cache_neighborhood(const phx_dfsph_step_cache* cache, uint6
                                         uint32_t* sorted_particle_indices, uint32_t* cell_offsets);
 phx_status phx_dfsph_cache_densities(const phx_dfsph_step_cache* cache, float* densities);
 phx_status phx_dfsph_cache_non_pressure_accelerations(const phx_dfsph_step_cache* cache, phx_vector_view out);
+
+/* jvp_step(state, control, parameters, next_state, cache, state_tangent,
+ * control_tangent, parameter_tangent, next_state_tangent), sph/sph.ixx:356-359
+ * and DFSPH::jvp, sph/sph.cpp:612-659. Needs a solver created in
+ * PHX_MODE_DIFFERENTIABLE and the cache of that step's forward_step, which
+ * records the state, control, parameters and every iterate, so they are not
+ * passed again. Tangents are FP32 and overwrite next_state_tangent (its warm
+ * pressure tangents may be null: not written); null parameter-tangent and
+ * warm-pressure tangent arrays count as zero. */
+phx_status phx_dfsph_jvp_step(phx_dfsph* solver, const phx_dfsph_step_cache* cache,
+                              const phx_sph_dfsph_state_tangent* state_tangent,
+                              const phx_control_tangent* control_tangent,
+                              const phx_sph_dfsph_parameter_tangent* parameter_tangent,
+                              phx_sph_dfsph_state_tangent* next_state_tangent);
+/* vjp_step(..., next_state_adjoint, previous_state_adjoint, control_adjoint,
+ * parameter_adjoint), sph/sph.ixx:361-365 and DFSPH::vjp, sph/sph.cpp:661-723:
+ * the VJP of the recorded step with its neighbour graph, collision masks and
+ * pressure active sets frozen, in FP64. Particle, control and parameter
+ * adjoints are accumulated (+=); the warm-start pressure adjoints are written
+ * (copied, sph.cpp:713) with warm start and left alone without. Null output
+ * arrays are skipped; null next warm-pressure adjoints count as zero. Under a
+ * no-slip collision the prediction's VJP is the exact transpose of its JVP,
+ * which keeps dt * xbar of the components that did not collide (the
+ * reference's sph/dfsph.cu:81 zeroes it). */
+phx_status phx_dfsph_vjp_step(phx_dfsph* solver, const phx_dfsph_step_cache* cache,
+                              const phx_sph_dfsph_state_adjoint* next_state_adjoint,
+                              phx_sph_dfsph_state_adjoint* previous_state_adjoint,
+                              phx_control_adjoint* control_adjoint,
+                              phx_sph_dfsph_parameter_adjoint* parameter_adjoint);
 
 /* ------------------------------------------------ WCSPH, PCISPH, IISPH */
 /* SPH<WCSPH>, SPH<PCISPH>, SPH<IISPH> (sph/sph.ixx:39-193, 273-372): one

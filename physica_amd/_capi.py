@@ -161,6 +161,46 @@ class ParticleParameterAdjoint(ctypes.Structure):
                 ("surface_tensions", c_void_p)]
 
 
+class ParticleStateTangent(ctypes.Structure):
+    _fields_ = [("positions", VectorView), ("velocities", VectorView)]
+
+
+class ParticleStateAdjoint(ctypes.Structure):
+    _fields_ = [("positions", AdjointVectorView), ("velocities", AdjointVectorView)]
+
+
+class DfsphStateTangent(ctypes.Structure):
+    _fields_ = [("warm_divergence_pressure", c_void_p), ("warm_density_pressure", c_void_p)]
+
+
+class DfsphStateAdjoint(ctypes.Structure):
+    _fields_ = [("warm_divergence_pressure", c_void_p), ("warm_density_pressure", c_void_p)]
+
+
+class DfsphParameterTangent(ctypes.Structure):
+    _fields_ = [("divergence_relaxation", c_void_p), ("density_relaxation", c_void_p)]
+
+
+class DfsphParameterAdjoint(ctypes.Structure):
+    _fields_ = [("divergence_relaxation", c_void_p), ("density_relaxation", c_void_p)]
+
+
+class SphDfsphStateTangent(ctypes.Structure):
+    _fields_ = [("particles", ParticleStateTangent), ("method", DfsphStateTangent)]
+
+
+class SphDfsphStateAdjoint(ctypes.Structure):
+    _fields_ = [("particles", ParticleStateAdjoint), ("method", DfsphStateAdjoint)]
+
+
+class SphDfsphParameterTangent(ctypes.Structure):
+    _fields_ = [("particles", ParticleParameterTangent), ("method", DfsphParameterTangent)]
+
+
+class SphDfsphParameterAdjoint(ctypes.Structure):
+    _fields_ = [("particles", ParticleParameterAdjoint), ("method", DfsphParameterAdjoint)]
+
+
 PHX_SPH_WCSPH, PHX_SPH_PCISPH, PHX_SPH_IISPH = 0, 1, 2
 
 
@@ -260,6 +300,16 @@ PROTOTYPES = {
     "phx_dfsph_cache_neighborhood": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
     "phx_dfsph_cache_densities": (c_int, [c_void_p, c_void_p]),
     "phx_dfsph_cache_non_pressure_accelerations": (c_int, [c_void_p, VectorView]),
+    "phx_dfsph_jvp_step": (
+        c_int,
+        [c_void_p, c_void_p, POINTER(SphDfsphStateTangent), POINTER(ControlTangent),
+         POINTER(SphDfsphParameterTangent), POINTER(SphDfsphStateTangent)],
+    ),
+    "phx_dfsph_vjp_step": (
+        c_int,
+        [c_void_p, c_void_p, POINTER(SphDfsphStateAdjoint), POINTER(SphDfsphStateAdjoint), POINTER(ControlAdjoint),
+         POINTER(SphDfsphParameterAdjoint)],
+    ),
     "phx_sph_create": (
         c_int,
         [POINTER(DomainConfiguration), POINTER(SphConfiguration), c_int, c_void_p, POINTER(c_void_p)],

@@ -299,6 +299,37 @@ phx_status phx_dfsph_cache_non_pressure_accelerations(const phx_dfsph_step_cache
     });
 }
 
+phx_status phx_dfsph_jvp_step(phx_dfsph* solver, const phx_dfsph_step_cache* cache,
+                              const phx_sph_dfsph_state_tangent* state_tangent,
+                              const phx_control_tangent* control_tangent,
+                              const phx_sph_dfsph_parameter_tangent* parameter_tangent,
+                              phx_sph_dfsph_state_tangent* next_state_tangent) {
+    return guarded([&] {
+        phx::require(solver && cache && state_tangent && control_tangent && parameter_tangent && next_state_tangent,
+                     PHX_ERROR_INVALID_ARGUMENT, "null argument");
+        phx::require(cache->owner == solver && cache->valid, PHX_ERROR_INVALID_ARGUMENT,
+                     "step cache holds no step of this solver");
+        solver->impl.jvp_step(cache->impl, *state_tangent, *control_tangent, *parameter_tangent,
+                              *next_state_tangent);
+    });
+}
+
+phx_status phx_dfsph_vjp_step(phx_dfsph* solver, const phx_dfsph_step_cache* cache,
+                              const phx_sph_dfsph_state_adjoint* next_state_adjoint,
+                              phx_sph_dfsph_state_adjoint* previous_state_adjoint,
+                              phx_control_adjoint* control_adjoint,
+                              phx_sph_dfsph_parameter_adjoint* parameter_adjoint) {
+    return guarded([&] {
+        phx::require(solver && cache && next_state_adjoint && previous_state_adjoint && control_adjoint &&
+                         parameter_adjoint,
+                     PHX_ERROR_INVALID_ARGUMENT, "null argument");
+        phx::require(cache->owner == solver && cache->valid, PHX_ERROR_INVALID_ARGUMENT,
+                     "step cache holds no step of this solver");
+        solver->impl.vjp_step(cache->impl, *next_state_adjoint, *previous_state_adjoint, *control_adjoint,
+                              *parameter_adjoint);
+    });
+}
+
 phx_status phx_dfsph_set_profiling(phx_dfsph* solver, int enabled) {
     return guarded([&] {
         phx::require(solver != nullptr, PHX_ERROR_INVALID_ARGUMENT, "null solver");

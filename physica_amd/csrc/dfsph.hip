@@ -45,7 +45,8 @@ const char* dfsph_pass_name(int pass) {
     static const char* names[kDfsphPassCount] = {"grid",       "mask_density",     "force_init",
                                                  "projection_divergence", "force_divergence",
                                                  "force_divergence_end",  "projection_density",
-                                                 "force_density"};
+                                                 "force_density",         "jvp",
+                                                 "vjp"};
     return pass >= 0 && pass < kDfsphPassCount ? names[pass] : "unknown";
 }
 
@@ -69,15 +70,30 @@ double dfsph_pass_bytes(int pass, uint32_t particles, uint32_t cells, bool warm_
             return 44.0 + 8.0 + offsets;
         case kDfsphForceDensity:  // x, v, m, t, a_np, a_div in; x* out
             return 56.0 + 12.0 + offsets;
+        case kDfsphJvp:  // whole-step derivative passes, not a single roofline figure
+        case kDfsphVjp:
+            return 0.0;
         default:
             return 0.0;
     }
 }
 
-void DfsphCache::allocate(uint32_t n, uint32_t cells) {
+void DfsphCache::allocate(uint32_t n, uint32_t cells, uint32_t divergence_iterations, uint32_t density_iterations) {
     neighborhood.allocate(n, cells);
     densities.resize(n);
     non_pressure.resize(n);
+    if (!record) return;
+    const size_t nn = n;
+    const size_t updates = static_cast<size_t>(divergence_iterations) + density_iterations;
+    for (DeviceBuffer<float4>* b : {&xm, &vmu}) b->resize(nn);
+    for (DeviceBuffer<float>* b : {&rho0, &relax_div, &relax_den}) b->resize(nn);
+    xs.resize(updates * nn);
+    rs.resize(updates * nn);
+    p.resize((updates + 2u) * nn);
+    collisions.resize((updates + 1u) * nn);
+    exact.resize(static_cast<size_t>(kStencil) * nn);
+    skin.resize(static_cast<size_t>(kStencil) * nn);
+    mask_flags.resize(kFlagDisplacement + updates + 1u);
 }
 
 KernelConstants kernel_constants(float h) {
@@ -95,7 +111,8 @@ DfsphSolver::DfsphSolver(const phx_domain_configuration* domain, const phx_dfsph
     require(configuration != nullptr, PHX_ERROR_INVALID_ARGUMENT, "null DFSPH configuration");
     config_ = *configuration;
     require(config_.checkpoint_interval > 0u, PHX_ERROR_INVALID_ARGUMENT, "checkpoint_interval must be positive");
-    require(mode == PHX_MODE_FORWARD, PHX_ERROR_UNSUPPORTED, "differentiable mode is not supported yet");
+    require(mode == PHX_MODE_FORWARD || mode == PHX_MODE_DIFFERENTIABLE, PHX_ERROR_INVALID_ARGUMENT,
+            "unknown execution mode");
     require(config_.density_error_tolerance == 0.0F, PHX_ERROR_UNSUPPORTED, "tolerance mode is not supported yet");
     require(config_.divergence_iterations + config_.density_iterations < 4096u, PHX_ERROR_INVALID_ARGUMENT,
             "too many pressure iterations");
@@ -138,28 +155,46 @@ void DfsphSolver::forward_step(const phx_sph_dfsph_state& state, const phx_contr
     }
     require(next.method.warm_divergence_pressure && next.method.warm_density_pressure, PHX_ERROR_INVALID_ARGUMENT,
             "null next warm-start pressures");
-    cache.allocate(n, domain_.cells);
+    const uint32_t nd = config_.divergence_iterations;
+    const uint32_t ne = config_.density_iterations;
+    cache.record = mode_ == PHX_MODE_DIFFERENTIABLE;
+    cache.recorded = false;
+    cache.iterations[0] = nd;
+    cache.iterations[1] = ne;
+    cache.allocate(n, domain_.cells, nd, ne);
+    const bool rec = cache.record;
+    const size_t nn = n;
+    // a recorded step keeps its inputs, masks and iterates in the cache
+    float4* xm = rec ? cache.xm.data() : xm_.data();
+    float4* vmu = rec ? cache.vmu.data() : vmu_.data();
+    float* rho0 = rec ? cache.rho0.data() : rho0_.data();
+    float* relax_div = rec ? cache.relax_div.data() : relax_div_.data();
+    float* relax_den = rec ? cache.relax_den.data() : relax_den_.data();
+    float* p_div0 = rec ? cache.pressures(0u, 0u) : p_div_.data();
+    float* p_den0 = rec ? cache.pressures(1u, 0u) : p_den_.data();
+    uint16_t* exact = rec ? cache.exact.data() : masks_.exact();
+    uint16_t* skin = rec ? cache.skin.data() : masks_.skin_mask();
 
     // neighborhood.build(step, x) with every input gathered into cell order
     ChannelList gather;
-    gather.add(cvec(state.particles.positions), xm_.data());
-    gather.add(pp.masses, xm_.data(), 3);
-    gather.add(cvec(state.particles.velocities), vmu_.data());
-    gather.add(pp.viscosities, vmu_.data(), 3);
+    gather.add(cvec(state.particles.positions), xm);
+    gather.add(pp.masses, xm, 3);
+    gather.add(cvec(state.particles.velocities), vmu);
+    gather.add(pp.viscosities, vmu, 3);
     gather.add(cvec(control.external_accelerations), u_.view());
-    gather.add(pp.rest_densities, rho0_.data());
-    gather.add(parameters.method.divergence_relaxation, relax_div_.data());
-    gather.add(parameters.method.density_relaxation, relax_den_.data());
+    gather.add(pp.rest_densities, rho0);
+    gather.add(parameters.method.divergence_relaxation, relax_div);
+    gather.add(parameters.method.density_relaxation, relax_den);
     if (warm) {
-        gather.add(state.method.warm_divergence_pressure, p_div_.data());
-        gather.add(state.method.warm_density_pressure, p_den_.data());
+        gather.add(state.method.warm_divergence_pressure, p_div0);
+        gather.add(state.method.warm_density_pressure, p_den0);
     }
     {
         PassScope scope(profiler_, stream_, kDfsphGrid);
         cache.neighborhood.build(stream_, domain_, step, cvec(state.particles.positions), nullptr, gather.get());
         if (!warm) {
-            fill_f32(stream_, p_div_.data(), n, 0.0F);
-            fill_f32(stream_, p_den_.data(), n, 0.0F);
+            fill_f32(stream_, p_div0, n, 0.0F);
+            fill_f32(stream_, p_den0, n, 0.0F);
         }
         masks_.begin_step(stream_);
     }
@@ -170,11 +205,11 @@ void DfsphSolver::forward_step(const phx_sph_dfsph_state& state, const phx_contr
                        constants_,
                        masks_.t_skin(),
                        grid,
-                       xm_.data(),
-                       masks_.exact(),
-                       masks_.skin_mask(),
-                       p_div_.data(),
-                       p_den_.data(),
+                       xm,
+                       exact,
+                       skin,
+                       p_div0,
+                       p_den0,
                        cache.densities.data(),
                        t_div_.data(),
                        t_den_.data(),
@@ -188,22 +223,27 @@ void DfsphSolver::forward_step(const phx_sph_dfsph_state& state, const phx_contr
     fa.grid = grid;
     fa.box = collision_box(c, step);
     fa.gravity = Float3{c.gravity.x, c.gravity.y, c.gravity.z};
-    fa.xm = xm_.data();
-    fa.vmu = vmu_.data();
+    fa.xm = xm;
+    fa.vmu = vmu;
     fa.u = u_.cview();
     fa.rho = cache.densities.data();
-    fa.exact = masks_.exact();
+    fa.exact = exact;
     fa.flags = masks_.flags();
     fa.anp = cache.non_pressure.data();
     fa.adiv = adiv_.data();
-    fa.xs = xs_.data();
     fa.perm = cache.neighborhood.perm.data();
     fa.next_x = vec(next.particles.positions);
     fa.next_v = vec(next.particles.velocities);
-    fa.p_div = p_div_.data();
+    fa.p_div = rec ? cache.pressures(0u, nd) : p_div_.data();
     fa.warm_div_out = next.method.warm_divergence_pressure;
-    fa.p_den = p_den_.data();
+    fa.p_den = rec ? cache.pressures(1u, ne) : p_den_.data();
     fa.warm_den_out = next.method.warm_density_pressure;
+    // prediction `slot` writes x* (recorded per slot) and, recorded, its collision bits
+    auto set_slot = [&](uint32_t slot) {
+        fa.slot = slot;
+        fa.xs = rec ? cache.xs.data() + slot * nn : xs_.data();
+        fa.collisions = rec ? cache.collisions.data() + slot * nn : nullptr;
+    };
 
     ProjectionArgs pa{};
     pa.n = n;
@@ -212,21 +252,27 @@ void DfsphSolver::forward_step(const phx_sph_dfsph_state& state, const phx_contr
     pa.gradient_norm = gradient_norm_;
     pa.limit2 = masks_.limit2();
     pa.grid = grid;
-    pa.xm = xm_.data();
-    pa.xs = xs_.data();
-    pa.skin = masks_.skin_mask();
+    pa.xm = xm;
+    pa.skin = skin;
     pa.flags = masks_.flags();
     pa.rho = cache.densities.data();
     pa.volume_density = cache.densities.data();
+    // update `it` of phase `phase` reads the x* of prediction `slot`
+    auto set_update = [&](uint32_t phase, uint32_t it, uint32_t slot) {
+        pa.slot = slot;
+        pa.xs = rec ? cache.xs.data() + slot * nn : xs_.data();
+        pa.rs_out = rec ? cache.rs.data() + slot * nn : nullptr;
+        float* base = phase == 0u ? p_div_.data() : p_den_.data();
+        pa.p_prev = rec ? cache.pressures(phase, it - 1u) : base;
+        pa.p = rec ? cache.pressures(phase, it) : base;
+    };
 
-    const uint32_t nd = config_.divergence_iterations;
-    const uint32_t ne = config_.density_iterations;
     uint32_t slot = 0;  // displacement slot written by the latest predicting pass
 
     // divergence phase: initial pressure with the warm p_div (sph.cpp:574-578)
     fa.t1 = t_div_.data();
     fa.t2 = t_den_.data();
-    fa.slot = slot;
+    set_slot(slot);
     if (nd == 0u) {
         fa.final_step = ne == 0u ? 1u : 0u;
         launch_force<true, kDivEnd>(profiler_, stream_, blocks, fa, kDfsphForceInit);
@@ -234,13 +280,12 @@ void DfsphSolver::forward_step(const phx_sph_dfsph_state& state, const phx_contr
         fa.final_step = 0u;
         launch_force<true, kDivPredict>(profiler_, stream_, blocks, fa, kDfsphForceInit);
         pa.target = cache.densities.data();
-        pa.relaxation = relax_div_.data();
-        pa.p = p_div_.data();
+        pa.relaxation = relax_div;
         pa.t = t_div_.data();
         for (uint32_t it = 1; it <= nd; ++it) {  // sph.cpp:580-587
-            pa.slot = slot;
+            set_update(0u, it, slot);
             launch_projection(profiler_, stream_, blocks, pa, kDfsphProjectionDivergence);
-            fa.slot = ++slot;
+            set_slot(++slot);
             if (it < nd) {
                 launch_force<false, kDivPredict>(profiler_, stream_, blocks, fa, kDfsphForceDivergence);
             } else {
@@ -250,19 +295,26 @@ void DfsphSolver::forward_step(const phx_sph_dfsph_state& state, const phx_contr
         }
     }
     // density phase (sph.cpp:591-609)
-    pa.target = rho0_.data();
-    pa.relaxation = relax_den_.data();
-    pa.p = p_den_.data();
+    pa.target = rho0;
+    pa.relaxation = relax_den;
     pa.t = t_den_.data();
     fa.t1 = t_den_.data();
     fa.t2 = nullptr;
     fa.warm_div_out = nullptr;
     for (uint32_t it = 1; it <= ne; ++it) {
-        pa.slot = slot;
+        set_update(1u, it, slot);
         launch_projection(profiler_, stream_, blocks, pa, kDfsphProjectionDensity);
-        fa.slot = ++slot;
+        set_slot(++slot);
         fa.final_step = it == ne ? 1u : 0u;
         launch_force<false, kDenPredict>(profiler_, stream_, blocks, fa, kDfsphForceDensity);
+    }
+    if (rec) {
+        PHX_HIP(hipMemcpyAsync(cache.mask_flags.data(), masks_.flags(), sizeof(uint32_t) * masks_.flag_count(),
+                               hipMemcpyDeviceToDevice, stream_));
+        cache.mask_limit2 = masks_.limit2();
+        cache.step_index = step;
+        cache.warm_start = warm;
+        cache.recorded = true;
     }
     // read the step's displacements back for the next step's skin
     masks_.end_step(stream_);

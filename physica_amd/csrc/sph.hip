@@ -341,6 +341,7 @@ void SphSolver::forward_pressure(const phx_sph_state& state, const phx_control& 
     pa.volume_density = iisph ? cache.densities.data() : rho0_.data();
     pa.target = rho0_.data();
     pa.relaxation = relax_.data();
+    pa.p_prev = cache.pressures.data();
     pa.p = cache.pressures.data();
     pa.t = t_.data();
 

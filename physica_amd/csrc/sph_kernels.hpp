@@ -20,10 +20,6 @@ __device__ __forceinline__ Float3 xyz(float4 q) { return Float3{q.x, q.y, q.z}; 
 
 constexpr uint32_t kBlock = 256;
 
-__device__ __forceinline__ uint32_t cell_of(const SortedGrid& g, int x, int y, int z) {
-    return (static_cast<uint32_t>(z) * g.cy + static_cast<uint32_t>(y)) * g.cx + static_cast<uint32_t>(x);
-}
-
 struct MaskDensityArgs {
     uint32_t n;
     KernelConstants kc;
@@ -111,8 +107,10 @@ struct ProjectionArgs {
     const float* volume_density;  // rho (DFSPH, IISPH) or rho0 (PCISPH) in V = m / density
     const float* target;
     const float* relaxation;
+    const float* p_prev;  // pressures before the update (may equal p)
     float* p;
     float* t;
+    float* rs_out;  // optional (recorded step): the predicted densities
 };
 
 // density at predicted positions (density.cu:18-40 with topology = x,
@@ -149,9 +147,10 @@ __global__ void __launch_bounds__(kBlock) projection_kernel(ProjectionArgs a) {
     const float rho = a.rho[i];
     const float volume = self.w / a.volume_density[i];
     const float delta_p = 1.0F / (2.0F * a.dt * a.dt * volume * volume * a.gradient_norm);
-    const float pressure = fmaxf(0.0F, a.p[i] + a.relaxation[i] * delta_p * (rs - a.target[i]));
+    const float pressure = fmaxf(0.0F, a.p_prev[i] + a.relaxation[i] * delta_p * (rs - a.target[i]));
     a.p[i] = pressure;
     a.t[i] = pressure / (rho * rho);
+    if (a.rs_out != nullptr) a.rs_out[i] = rs;
 }
 
 enum Epilogue : int {
@@ -180,6 +179,7 @@ struct ForceArgs {
     float4* anp;
     float4* adiv;
     float4* xs;
+    uint8_t* collisions;  // optional (recorded step): box-collision bits of the prediction
     uint32_t final_step;
     const uint32_t* perm;
     Vec next_x, next_v;
@@ -270,7 +270,10 @@ __global__ void __launch_bounds__(kBlock) force_kernel(ForceArgs a) {
             total = add(xyz(a.adiv[i]), ap1);
         }
         Float3 nx, nv;
-        predict_clamped(a.box, a.dt, p, vel, add(acc, total), nx, nv);
+        const Float3 a_total = add(acc, total);
+        predict_clamped(a.box, a.dt, p, vel, a_total, nx, nv);
+        if (a.collisions != nullptr)
+            a.collisions[i] = static_cast<uint8_t>(predict_collisions(a.box, a.dt, p, vel, a_total));
         if constexpr (kTwo) {
             if (a.warm_div_out != nullptr) a.warm_div_out[a.perm[i]] = a.p_div[i];
         }

@@ -355,6 +355,42 @@ class DFSPH:
         def to_c(self) -> _capi.DfsphParameters:
             return _capi.DfsphParameters(_ptr(self.divergence_relaxation), _ptr(self.density_relaxation))
 
+    @dataclass
+    class StateTangent:
+        """FP32 (sph/sph.ixx:206-209); None counts as zero on input."""
+
+        warm_divergence_pressure: Optional[torch.Tensor]
+        warm_density_pressure: Optional[torch.Tensor]
+
+        def to_c(self) -> _capi.DfsphStateTangent:
+            return _capi.DfsphStateTangent(_ptr(self.warm_divergence_pressure), _ptr(self.warm_density_pressure))
+
+    @dataclass
+    class StateAdjoint:
+        """FP64 (sph/sph.ixx:210-213)."""
+
+        warm_divergence_pressure: Optional[torch.Tensor]
+        warm_density_pressure: Optional[torch.Tensor]
+
+        def to_c(self) -> _capi.DfsphStateAdjoint:
+            return _capi.DfsphStateAdjoint(_ptr(self.warm_divergence_pressure), _ptr(self.warm_density_pressure))
+
+    @dataclass
+    class ParameterTangent:
+        divergence_relaxation: Optional[torch.Tensor]
+        density_relaxation: Optional[torch.Tensor]
+
+        def to_c(self) -> _capi.DfsphParameterTangent:
+            return _capi.DfsphParameterTangent(_ptr(self.divergence_relaxation), _ptr(self.density_relaxation))
+
+    @dataclass
+    class ParameterAdjoint:
+        divergence_relaxation: Optional[torch.Tensor]
+        density_relaxation: Optional[torch.Tensor]
+
+        def to_c(self) -> _capi.DfsphParameterAdjoint:
+            return _capi.DfsphParameterAdjoint(_ptr(self.divergence_relaxation), _ptr(self.density_relaxation))
+
 
 # ------------------------------------------------------ WCSPH, PCISPH, IISPH
 class WCSPH:
@@ -456,6 +492,42 @@ class SPH:
             if isinstance(self.method, DFSPH.Parameters):
                 return _capi.SphDfsphParameters(self.particles.to_c(), self.method.to_c())
             return _capi.SphParameters(self.particles.to_c(), self.method.to_c())
+
+    @dataclass
+    class StateTangent:
+        particles: ParticleStateTangent
+        method: object
+
+        def to_c(self):
+            return _capi.SphDfsphStateTangent(
+                _capi.ParticleStateTangent(self.particles.positions.view(), self.particles.velocities.view()),
+                self.method.to_c())
+
+    @dataclass
+    class StateAdjoint:
+        particles: ParticleStateAdjoint
+        method: object
+
+        def to_c(self):
+            return _capi.SphDfsphStateAdjoint(
+                _capi.ParticleStateAdjoint(self.particles.positions.view(), self.particles.velocities.view()),
+                self.method.to_c())
+
+    @dataclass
+    class ParameterTangent:
+        particles: ParticleParameterTangent
+        method: object
+
+        def to_c(self):
+            return _capi.SphDfsphParameterTangent(self.particles.to_c(), self.method.to_c())
+
+    @dataclass
+    class ParameterAdjoint:
+        particles: ParticleParameterAdjoint
+        method: object
+
+        def to_c(self):
+            return _capi.SphDfsphParameterAdjoint(self.particles.to_c(), self.method.to_c())
 
     class StepCache:
         """StepCache (sph/sph.ixx:299-303); fields are read back in original order."""
@@ -590,6 +662,94 @@ class SPH:
     def read_profile(self) -> list:
         """Per-pass totals since profiling was enabled or last read."""
         return _read_profile(self._prefix + "_read_profile", self._handle)
+
+    # tangents and adjoints (sph/sph.ixx:313-348), DFSPH; allocated zero
+    def _require_dfsph_derivatives(self) -> None:
+        if self.method is not DFSPH:
+            raise NotImplementedError(f"derivatives of SPH<{self.method.__name__}> are not provided")
+
+    def allocate_state_tangent(self) -> "SPH.StateTangent":
+        self._require_dfsph_derivatives()
+        d = self.domain
+        n = d.configuration.particle_count
+        z = lambda: torch.zeros(n, dtype=torch.float32, device=d.device)  # noqa: E731
+        return SPH.StateTangent(ParticleStateTangent(VectorField.zeros(n, d.device), VectorField.zeros(n, d.device)),
+                                DFSPH.StateTangent(z(), z()))
+
+    def allocate_control_tangent(self) -> ControlTangent:
+        return ControlTangent(VectorField.zeros(self.domain.configuration.particle_count, self.domain.device))
+
+    def allocate_parameter_tangent(self) -> "SPH.ParameterTangent":
+        self._require_dfsph_derivatives()
+        d = self.domain
+        z = lambda: torch.zeros(d.configuration.particle_count, dtype=torch.float32, device=d.device)  # noqa: E731
+        return SPH.ParameterTangent(ParticleParameterTangent(z(), z(), z(), z()), DFSPH.ParameterTangent(z(), z()))
+
+    def allocate_state_adjoint(self) -> "SPH.StateAdjoint":
+        self._require_dfsph_derivatives()
+        d = self.domain
+        z = d.allocate_scalar_adjoint_field
+        return SPH.StateAdjoint(ParticleStateAdjoint(d.allocate_vector_adjoint_field(),
+                                                     d.allocate_vector_adjoint_field()),
+                                DFSPH.StateAdjoint(z(), z()))
+
+    def allocate_control_adjoint(self) -> ControlAdjoint:
+        return ControlAdjoint(self.domain.allocate_vector_adjoint_field())
+
+    def allocate_parameter_adjoint(self) -> "SPH.ParameterAdjoint":
+        self._require_dfsph_derivatives()
+        z = self.domain.allocate_scalar_adjoint_field
+        return SPH.ParameterAdjoint(ParticleParameterAdjoint(z(), z(), z(), z()), DFSPH.ParameterAdjoint(z(), z()))
+
+    def copy_state_tangent(self, source: "SPH.StateTangent", destination: "SPH.StateTangent") -> None:
+        destination.particles.positions.copy_(source.particles.positions)
+        destination.particles.velocities.copy_(source.particles.velocities)
+        destination.method.warm_divergence_pressure.copy_(source.method.warm_divergence_pressure)
+        destination.method.warm_density_pressure.copy_(source.method.warm_density_pressure)
+
+    def copy_state_adjoint(self, source: "SPH.StateAdjoint", destination: "SPH.StateAdjoint") -> None:
+        destination.particles.positions.copy_(source.particles.positions)
+        destination.particles.velocities.copy_(source.particles.velocities)
+        destination.method.warm_divergence_pressure.copy_(source.method.warm_divergence_pressure)
+        destination.method.warm_density_pressure.copy_(source.method.warm_density_pressure)
+
+    def accumulate_state_adjoint(self, source: "SPH.StateAdjoint", destination: "SPH.StateAdjoint") -> None:
+        destination.particles.positions.add_(source.particles.positions)
+        destination.particles.velocities.add_(source.particles.velocities)
+        destination.method.warm_divergence_pressure.add_(source.method.warm_divergence_pressure)
+        destination.method.warm_density_pressure.add_(source.method.warm_density_pressure)
+
+    def jvp_step(self, state, control, parameters, next_state, cache, state_tangent, control_tangent,
+                 parameter_tangent, next_state_tangent) -> None:
+        """SPH<DFSPH>::jvp_step (sph/sph.ixx:356-359) of the step recorded in
+        `cache`: state, control, parameters and next_state are those of that
+        forward step, which the cache recorded."""
+        self._require_dfsph_derivatives()
+        _capi.call(
+            "phx_dfsph_jvp_step",
+            self._handle,
+            cache._handle,
+            ctypes.byref(state_tangent.to_c()),
+            ctypes.byref(control_tangent.to_c()),
+            ctypes.byref(parameter_tangent.to_c()),
+            ctypes.byref(next_state_tangent.to_c()),
+        )
+
+    def vjp_step(self, state, control, parameters, next_state, cache, next_state_adjoint, previous_state_adjoint,
+                 control_adjoint, parameter_adjoint) -> None:
+        """SPH<DFSPH>::vjp_step (sph/sph.ixx:361-365): particle, control and
+        parameter adjoints accumulate; warm-start pressure adjoints are
+        written (with warm start)."""
+        self._require_dfsph_derivatives()
+        _capi.call(
+            "phx_dfsph_vjp_step",
+            self._handle,
+            cache._handle,
+            ctypes.byref(next_state_adjoint.to_c()),
+            ctypes.byref(previous_state_adjoint.to_c()),
+            ctypes.byref(control_adjoint.to_c()),
+            ctypes.byref(parameter_adjoint.to_c()),
+        )
 
     def last_iterations(self) -> Tuple[int, int]:
         """Iterations of the last DFSPH step (divergence, density)."""

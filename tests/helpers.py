@@ -27,10 +27,10 @@ def upload_scalar(t: torch.Tensor, values) -> None:
     t.copy_(torch.from_numpy(np.ascontiguousarray(values, dtype=np.float32)))
 
 
-def dfsph_setup(scene, device, configuration=None, warm=None):
+def dfsph_setup(scene, device, configuration=None, warm=None, mode=ExecutionMode.forward):
     solver = SPH(scene.domain, configuration or DFSPH.Configuration(
         divergence_iterations=scene.solver.get("divergence_iterations", 4),
-        density_iterations=scene.solver.get("density_iterations", 6)), device=device)
+        density_iterations=scene.solver.get("density_iterations", 6)), mode=mode, device=device)
     state = solver.allocate_state()
     nxt = solver.allocate_state()
     control = solver.allocate_control()
