@@ -30,17 +30,6 @@
 
 namespace phx {
 
-float sqrt_support_threshold(float h) {
-    float t = h * h;
-    while (std::sqrt(t) < h) t = std::nextafter(t, INFINITY);
-    for (;;) {
-        const float below = std::nextafter(t, 0.0F);
-        if (below <= 0.0F || std::sqrt(below) < h) break;
-        t = below;
-    }
-    return t;
-}
-
 const char* dfsph_pass_name(int pass) {
     static const char* names[kDfsphPassCount] = {"grid",       "mask_density",     "force_init",
                                                  "projection_divergence", "force_divergence",

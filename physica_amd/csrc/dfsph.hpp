@@ -134,8 +134,4 @@ class DfsphSolver {
 // Constants of the cubic and viscosity kernels for support radius h.
 KernelConstants kernel_constants(float h);
 
-// Smallest float t with sqrtf(t) >= h: a candidate at squared distance d2
-// lies inside the support (sqrtf(d2) < h) exactly when d2 < t.
-float sqrt_support_threshold(float h);
-
 }  // namespace phx

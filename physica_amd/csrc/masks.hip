@@ -8,6 +8,19 @@
 
 namespace phx {
 
+float sqrt_support_threshold(float h) {
+    float t = h * h;
+    while (std::sqrt(t) < h) t = std::nextafter(t, INFINITY);
+    for (;;) {
+        const float below = std::nextafter(t, 0.0F);
+        if (below <= 0.0F || std::sqrt(below) < h) break;
+        t = below;
+    }
+    return t;
+}
+
+float exact_mask_threshold(float h) { return std::fmax(sqrt_support_threshold(h), h * h) * (1.0F + 4e-6F); }
+
 MaskSet::~MaskSet() {
     if (readback_ != nullptr) {
         (void)hipEventSynchronize(readback_);

@@ -7,6 +7,16 @@
 
 namespace phx {
 
+// Smallest float t with sqrtf(t) >= h: a candidate at squared distance d2
+// lies inside the support (sqrtf(d2) < h) exactly when d2 < t.
+float sqrt_support_threshold(float h);
+
+// Squared-distance bound of an exact mask that keeps every pair on which any
+// kernel of the support h can be non-zero: inside sqrtf(d2) < h (spiky,
+// cubic), d2 < h * h (poly6), and the FP64 radius of the Hessian products,
+// with a relative margin for the FP32 rounding of d2.
+float exact_mask_threshold(float h);
+
 // Pinned host buffer for asynchronous read-back.
 class PinnedBuffer {
   public:

@@ -86,7 +86,11 @@ struct PbfDerivativeWorkspace {
     DeviceBuffer<double> lambda_a, magnitude_a;
     DeviceBuffer<double> m_a, rho0_a, eps_a, strength_a, exponent_a, radius_a, xsph_a, confinement_a;
     DeviceBuffer<uint8_t> vanishes;  // per particle: the artificial pressure is the signed zero -k
-    void allocate(uint32_t n);
+    DeviceBuffer<double> inv_rho0;   // 1 / rho0
+    DeviceBuffer<double2> lambda_terms;  // (density adjoint, D_bar) of one iteration
+    DeviceBuffer<uint16_t> exact;     // exact masks of one set of positions
+    DeviceBuffer<uint32_t> exact_flags;
+    void allocate(uint32_t n, uint32_t iterations);
 };
 
 class PbfSolver {
@@ -110,6 +114,9 @@ class PbfSolver {
 
   private:
     void gather_parameters(const phx_pbf_parameters& parameters, const PbfCache& cache);
+    // Exact masks of `pos` (displacement slot `slot` of the step) into the
+    // derivative workspace.
+    void build_exact_masks(const PbfCache& cache, CVec pos, uint32_t slot, uint32_t blocks);
 
     Domain domain_;
     phx_pbf_configuration config_{};

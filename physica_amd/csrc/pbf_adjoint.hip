@@ -126,6 +126,7 @@ struct Step {
     CVec xs, x0;
     const float *m, *rho0, *eps, *strength, *exponent, *radius, *xsph, *confinement;
     MaskView mv;  // the recorded step's masks and flags
+    const double* inv_rho0;  // 1 / rho0 (reverse passes)
 };
 
 __device__ __forceinline__ void cell_of_topology(const Step& s, uint32_t i, int& cx, int& cy, int& cz) {
@@ -269,6 +270,69 @@ __global__ void __launch_bounds__(kBlock) pbf_vjp_vorticity_kernel(VorticityVjpA
     store(a.p_a, i, add(add(load(a.p_a, i), pcon), cb));
 }
 
+struct ExactMaskArgs {
+    uint32_t n;
+    SortedGrid grid;
+    CVec topo;  // x*: the cells of the step
+    CVec pos;   // the positions the masks are for
+    float t;    // exact_mask_threshold
+    MaskView skin;
+    uint32_t slot;  // displacement slot of pos
+    uint16_t* masks;
+    uint32_t* flags;
+};
+
+// Exact masks of the positions `pos`: bit b of stencil slot k marks the b-th
+// particle of that cell within the support. Every kernel of a derivative pass
+// vanishes outside the support, so the passes at `pos` visit these candidates
+// only. Where the step's skin mask is valid at `pos`, only its candidates are
+// tested; a cell holding more than kMaskBits particles sets the overflow flag
+// (the passes then scan the full stencil).
+__global__ void __launch_bounds__(kBlock) pbf_exact_mask_kernel(ExactMaskArgs a) {
+    const uint32_t i = xcd_block() * kBlock + threadIdx.x;
+    if (i >= a.n) return;
+    const Float3 p = load(a.pos, i);
+    int cx, cy, cz;
+    particle_cell(a.grid, load(a.topo, i), cx, cy, cz);
+    const bool skin = masks_apply(a.skin, a.slot);
+    bool complete = true;
+    int k = 0;
+    for (int ox = -1; ox <= 1; ++ox) {
+        const int X = cx + ox;
+        for (int oy = -1; oy <= 1; ++oy) {
+            const int Y = cy + oy;
+            for (int oz = -1; oz <= 1; ++oz, ++k) {
+                const int Z = cz + oz;
+                uint32_t bits = 0u;
+                if (X >= 0 && X < static_cast<int>(a.grid.cx) && Y >= 0 && Y < static_cast<int>(a.grid.cy) &&
+                    Z >= 0 && Z < static_cast<int>(a.grid.cz)) {
+                    const uint32_t c = cell_of(a.grid, X, Y, Z);
+                    const uint32_t first = a.grid.offsets[c];
+                    uint32_t count = a.grid.offsets[c + 1u] - first;
+                    uint32_t candidates;
+                    if (skin) {
+                        candidates = a.skin.masks[static_cast<size_t>(k) * a.n + i];
+                    } else {
+                        if (count > kMaskBits) {
+                            complete = false;
+                            count = kMaskBits;
+                        }
+                        candidates = (1u << count) - 1u;
+                    }
+                    while (candidates != 0u) {
+                        const uint32_t b = static_cast<uint32_t>(__builtin_ctz(candidates));
+                        candidates &= candidates - 1u;
+                        const Float3 d = sub(p, load(a.pos, first + b));
+                        if (dot(d, d) < a.t) bits |= 1u << b;
+                    }
+                }
+                a.masks[static_cast<size_t>(k) * a.n + i] = static_cast<uint16_t>(bits);
+            }
+        }
+    }
+    if (!complete) atomicOr(&a.flags[kFlagOverflow], 1u);
+}
+
 struct CorrectionVjpArgs {
     Step s;
     uint32_t slot;  // displacement slot of the positions read
@@ -281,10 +345,28 @@ struct CorrectionVjpArgs {
     DVec p_out;  // adjoint of the positions entering it (written)
     double* lambda_a;
     double *m_a, *rho0_a, *strength_a, *exponent_a, *radius_a;
+    // the iteration's densities and denominators, for the per-particle terms
+    // of the lambda VJP
+    const float *rho, *den;
+    double2* lambda_terms;  // (density adjoint, D_bar) per particle (written)
 };
 
+// Derivative of G(d) = f(r) d (spiky gradient) along w, with one division.
+__device__ __forceinline__ D3 spiky_hessian_one_division(Float3 d, D3 w, double h, double k) {
+    const D3 dd = d3(d);
+    const double r = sqrt(dot(dd, dd));
+    if (r == 0.0 || r >= h) return {0.0, 0.0, 0.0};
+    const double inv_r = 1.0 / r;
+    const double hr = h - r;
+    const double f = k * hr * hr * inv_r;
+    const double fpr = -k * hr * (h + r) * inv_r * inv_r * inv_r;  // f'(r) / r
+    return add(scale(w, f), scale(dd, fpr * dot(dd, w)));
+}
+
 // project_vjp_kernel (pbf.cu:445-454) and correction_vjp_kernel
-// (pbf.cu:335-415).
+// (pbf.cu:335-415). The Hessian is even in d, so the terms where i is the
+// centre and where it is the neighbour share one product; the neighbour's
+// spiky gradient is the exact negation of the centre's.
 __global__ void __launch_bounds__(kBlock) pbf_vjp_correction_kernel(CorrectionVjpArgs a) {
     PHX_MASK_LDS(a.s.grid);
     const uint32_t i = xcd_block() * kBlock + threadIdx.x;
@@ -293,9 +375,10 @@ __global__ void __launch_bounds__(kBlock) pbf_vjp_correction_kernel(CorrectionVj
     const Step& s = a.s;
     const double h = static_cast<double>(s.h), k = static_cast<double>(s.spiky);
     const Float3 pi = load(a.pos, i);
-    const double rho0 = static_cast<double>(s.rho0[i]);
+    const double inv_rho0 = s.inv_rho0[i];
     const D3 oa = unclamped(load(a.p_in, i), a.mask[i]);
     const float lam_i = a.lambda[i];
+    const double m_i = static_cast<double>(s.m[i]);
     D3 pcon{0.0, 0.0, 0.0};
     double lcon = 0.0, mcon = 0.0, rcon = 0.0, scon = 0.0, econ = 0.0, qcon = 0.0;
     int cx, cy, cz;
@@ -308,30 +391,28 @@ __global__ void __launch_bounds__(kBlock) pbf_vjp_correction_kernel(CorrectionVj
         // i as the centre
         const Artificial art = artificial_full(dd, s.h, s.strength[i], s.exponent[i], s.radius[i], a.vanishes[i] != 0u,
                                                a.strength_wanted != 0u);
-        const double cf = static_cast<double>(s.m[j]) / rho0;
+        const double cf = static_cast<double>(s.m[j]) * inv_rho0;
         const double mu = static_cast<double>(lam_i) + static_cast<double>(lam_j) + static_cast<double>(art.value);
-        const double sa = cf * dot(oa, gr);
-        pcon = add(pcon, spiky_hessian(dd, scale(oa, cf * mu), h, k));
-        pcon = add(pcon, scale(d3(art.gradient), sa));
+        const double oa_gr = dot(oa, gr);
+        const double sa = cf * oa_gr;
         lcon += sa;
-        rcon -= cf * mu * dot(oa, gr) / rho0;
+        rcon -= cf * mu * oa_gr * inv_rho0;
         scon += sa * static_cast<double>(art.d_strength);
         econ += sa * static_cast<double>(art.d_exponent);
         qcon += sa * static_cast<double>(art.d_radius);
-        // i as a neighbour of j: d_ji = -d_ij
-        const Float3 rd = scale(dd, -1.0F);
-        const Artificial rart = artificial_full(rd, s.h, s.strength[j], s.exponent[j], s.radius[j], a.vanishes[j] != 0u,
-                                                a.strength_wanted != 0u);
-        const double rrho0 = static_cast<double>(s.rho0[j]);
-        const double rcf = static_cast<double>(s.m[i]) / rrho0;
+        // i as a neighbour of j: d_ji = -d_ij, gradient -gr
+        const Artificial rart = artificial_full(scale(dd, -1.0F), s.h, s.strength[j], s.exponent[j], s.radius[j],
+                                                a.vanishes[j] != 0u, a.strength_wanted != 0u);
+        const double inv_rrho0 = s.inv_rho0[j];
+        const double rcf = m_i * inv_rrho0;
         const double rmu = static_cast<double>(lam_j) + static_cast<double>(lam_i) + static_cast<double>(rart.value);
         const D3 ob = unclamped(load(a.p_in, j), a.mask[j]);
-        const D3 rgr = d3(spiky_gradient(rd, s.h, s.spiky));
-        const double rsa = rcf * dot(ob, rgr);
-        pcon = sub(pcon, spiky_hessian(rd, scale(ob, rcf * rmu), h, k));
-        pcon = sub(pcon, scale(d3(rart.gradient), rsa));
+        const double ob_rgr = -dot(ob, gr);
+        const double rsa = rcf * ob_rgr;
         lcon += rsa;
-        mcon += rmu / rrho0 * dot(ob, rgr);
+        mcon += rmu * inv_rrho0 * ob_rgr;
+        pcon = add(pcon, spiky_hessian_one_division(dd, sub(scale(oa, cf * mu), scale(ob, rcf * rmu)), h, k));
+        pcon = add(pcon, sub(scale(d3(art.gradient), sa), scale(d3(rart.gradient), rsa)));
     });
     store(a.p_out, i, add(oa, pcon));
     a.lambda_a[i] = lcon;
@@ -340,6 +421,12 @@ __global__ void __launch_bounds__(kBlock) pbf_vjp_correction_kernel(CorrectionVj
     a.strength_a[i] += scon;
     a.exponent_a[i] += econ;
     a.radius_a[i] += qcon;
+    // lambda_vjp_kernel's per-particle terms: the density adjoint
+    // -lambda_bar / (D rho0) and D_bar = lambda_bar C / D^2 (pbf.cu:207-214)
+    const double rho0 = static_cast<double>(s.rho0[i]);
+    const double den = static_cast<double>(a.den[i]);
+    const double con = static_cast<double>(a.rho[i]) / rho0 - 1.0;
+    a.lambda_terms[i] = make_double2(-(lcon / (den * rho0)), lcon * con / (den * den));
 }
 
 struct LambdaVjpArgs {
@@ -349,13 +436,15 @@ struct LambdaVjpArgs {
     const float *rho, *den;
     CVec gsum;
     const double* lambda_a;
+    const double2* lambda_terms;
     DVec p_out;
     double *m_a, *rho0_a, *eps_a;
 };
 
 // lambda_vjp_kernel (pbf.cu:178-252) and density_vjp_kernel (density.cu:69-98,
-// poly6), whose density adjoint -lambda_bar / (D rho0) is rebuilt per
-// neighbour.
+// poly6) with the neighbours' density adjoints and D_bar read from the
+// per-particle terms; the centre and neighbour Hessian terms share one
+// product.
 __global__ void __launch_bounds__(kBlock) pbf_vjp_lambda_kernel(LambdaVjpArgs a) {
     PHX_MASK_LDS(a.s.grid);
     const uint32_t i = xcd_block() * kBlock + threadIdx.x;
@@ -365,11 +454,12 @@ __global__ void __launch_bounds__(kBlock) pbf_vjp_lambda_kernel(LambdaVjpArgs a)
     const double h = static_cast<double>(s.h), k = static_cast<double>(s.spiky);
     const Float3 pi = load(a.pos, i);
     const double rho0 = static_cast<double>(s.rho0[i]);
+    const double inv_rho0 = s.inv_rho0[i];
     const double den = static_cast<double>(a.den[i]);
     const double lam_a = a.lambda_a[i];
-    const double con = static_cast<double>(a.rho[i]) / rho0 - 1.0;
-    const double dbar = lam_a * con / (den * den);
-    const double rho_a = -(lam_a / (den * rho0));
+    const double2 terms = a.lambda_terms[i];
+    const double rho_a = terms.x;
+    const double dbar = terms.y;
     const D3 sgi = d3(load(a.gsum, i));
     const double mi = static_cast<double>(s.m[i]);
     D3 pl{0.0, 0.0, 0.0}, pd{0.0, 0.0, 0.0};
@@ -379,33 +469,27 @@ __global__ void __launch_bounds__(kBlock) pbf_vjp_lambda_kernel(LambdaVjpArgs a)
     cell_of_topology(s, i, cx, cy, cz);
     visit_candidates(s.grid, s.mv, masked, i, cx, cy, cz, delta, smask + threadIdx.x, kBlock, [&](uint32_t j) {
         const Float3 dd = sub(pi, load(a.pos, j));
-        const double jrho0 = static_cast<double>(s.rho0[j]);
-        const double jden = static_cast<double>(a.den[j]);
-        const double jlam_a = a.lambda_a[j];
+        const double2 jterms = a.lambda_terms[j];
         // density term (the self term has a zero gradient)
-        const double jrho_a = -(jlam_a / (jden * jrho0));
         const D3 gw = d3(poly6_gradient(dd, s.h));
-        pd = add(pd, scale(gw, rho_a * static_cast<double>(s.m[j]) + jrho_a * mi));
-        md += jrho_a * static_cast<double>(poly6(dd, s.h));
+        pd = add(pd, scale(gw, rho_a * static_cast<double>(s.m[j]) + jterms.x * mi));
+        md += jterms.x * static_cast<double>(poly6(dd, s.h));
         if (j == i) return;
         const D3 gr = d3(spiky_gradient(dd, s.h, s.spiky));
+        const double gr2 = dot(gr, gr);
         // i as the centre: g_bar = 2 D_bar c (c g + S), c_bar = 2 D_bar (c |g|^2 + g . S)
-        const double cf = static_cast<double>(s.m[j]) / rho0;
+        const double cf = static_cast<double>(s.m[j]) * inv_rho0;
         const D3 gbar = scale(add(scale(gr, cf), sgi), 2.0 * dbar * cf);
-        pl = add(pl, spiky_hessian(dd, gbar, h, k));
-        const double cbar = 2.0 * dbar * (cf * dot(gr, gr) + dot(gr, sgi));
-        rcon -= cbar * cf / rho0;
-        // i as a neighbour of j
-        const double jcon = static_cast<double>(a.rho[j]) / jrho0 - 1.0;
-        const double jdbar = jlam_a * jcon / (jden * jden);
+        const double cbar = 2.0 * dbar * (cf * gr2 + dot(gr, sgi));
+        rcon -= cbar * cf * inv_rho0;
+        // i as a neighbour of j: gradient -gr
+        const double inv_jrho0 = s.inv_rho0[j];
         const D3 sgj = d3(load(a.gsum, j));
-        const Float3 rd = scale(dd, -1.0F);
-        const D3 rgr = d3(spiky_gradient(rd, s.h, s.spiky));
-        const double rcf = mi / jrho0;
-        const D3 rgbar = scale(add(scale(rgr, rcf), sgj), 2.0 * jdbar * rcf);
-        pl = sub(pl, spiky_hessian(rd, rgbar, h, k));
-        const double rcbar = 2.0 * jdbar * (rcf * dot(rgr, rgr) + dot(rgr, sgj));
-        ml += rcbar / jrho0;
+        const double rcf = mi * inv_jrho0;
+        const D3 rgbar = scale(sub(sgj, scale(gr, rcf)), 2.0 * jterms.y * rcf);
+        const double rcbar = 2.0 * jterms.y * (rcf * gr2 - dot(gr, sgj));
+        ml += rcbar * inv_jrho0;
+        pl = add(pl, spiky_hessian_one_division(dd, sub(gbar, rgbar), h, k));
     });
     store(a.p_out, i, add(add(load(a.p_out, i), pl), pd));
     a.m_a[i] += ml;
@@ -452,13 +536,17 @@ __global__ void __launch_bounds__(kBlock) pbf_vjp_scatter_kernel(ScatterVjpArgs 
         if (a.original[c] != nullptr) a.original[c][o] += a.sorted[c][i];
 }
 
-__global__ void __launch_bounds__(kBlock) pbf_vanishes_kernel(uint32_t n, float h, const float* strength,
-                                                              const float* exponent, const float* radius,
-                                                              uint8_t* vanishes) {
+// Per-particle constants of the reverse passes: whether the artificial
+// pressure vanishes, and 1 / rho0.
+__global__ void __launch_bounds__(kBlock) pbf_vjp_setup_kernel(uint32_t n, float h, const float* strength,
+                                                               const float* exponent, const float* radius,
+                                                               const float* rho0, uint8_t* vanishes,
+                                                               double* inv_rho0) {
     const uint32_t i = xcd_block() * kBlock + threadIdx.x;
     if (i >= n) return;
     const float reference = poly6(Float3{radius[i], 0.0F, 0.0F}, h);
     vanishes[i] = artificial_vanishes(h, strength[i], exponent[i], reference) ? 1u : 0u;
+    inv_rho0[i] = 1.0 / static_cast<double>(rho0[i]);
 }
 
 struct GatherAdjointArgs {
@@ -694,7 +782,9 @@ __global__ void __launch_bounds__(kBlock) pbf_jvp_xsph_kernel(PostJvpArgs a) {
 
 }  // namespace
 
-void PbfDerivativeWorkspace::allocate(uint32_t n) {
+void PbfDerivativeWorkspace::allocate(uint32_t n, uint32_t iterations) {
+    exact.resize(static_cast<size_t>(kStencil) * n);
+    exact_flags.resize(kFlagDisplacement + iterations + 1u);
     for (DeviceBuffer<float>* b : {&m, &rho0, &eps, &strength, &exponent, &radius, &xsph, &confinement, &m_t, &rho0_t,
                                    &eps_t, &strength_t, &exponent_t, &radius_t, &xsph_t, &confinement_t, &rho_t,
                                    &lambda_t, &magnitude_t})
@@ -702,8 +792,9 @@ void PbfDerivativeWorkspace::allocate(uint32_t n) {
     for (VecBuffer* b : {&x_t, &v_t, &u_t, &p_t[0], &p_t[1], &vrec_t, &w_t, &vc_t}) b->resize(n);
     for (DVecBuffer* b : {&p_a[0], &p_a[1], &vrec_a, &w_a, &g_a, &next_x_a, &next_v_a, &x_a, &vel_a}) b->resize(n);
     for (DeviceBuffer<double>* b : {&lambda_a, &magnitude_a, &m_a, &rho0_a, &eps_a, &strength_a, &exponent_a, &radius_a,
-                                    &xsph_a, &confinement_a})
+                                    &xsph_a, &confinement_a, &inv_rho0})
         b->resize(n);
+    lambda_terms.resize(n);
     vanishes.resize(n);
 }
 
@@ -736,6 +827,30 @@ void require_recorded(const PbfCache& cache, phx_execution_mode mode, uint32_t n
 
 }  // namespace
 
+void PbfSolver::build_exact_masks(const PbfCache& cache, CVec pos, uint32_t slot, uint32_t blocks) {
+    const uint32_t n = domain_.config.particle_count;
+    fill_u32(stream_, dw_.exact_flags.data(), dw_.exact_flags.size(), 0u);
+    ExactMaskArgs ea{n,
+                     cache.neighborhood.grid(),
+                     cache.xs.cview(),
+                     pos,
+                     exact_mask_threshold(domain_.config.support_radius),
+                     MaskView{cache.masks_skin.data(), cache.mask_flags.data(), n, cache.mask_limit2},
+                     slot,
+                     dw_.exact.data(),
+                     dw_.exact_flags.data()};
+    pbf_exact_mask_kernel<<<blocks, kBlock, 0, stream_>>>(ea);
+    PHX_LAUNCH_CHECK();
+}
+
+namespace {
+// The view of the workspace's exact masks: every displacement word stays
+// zero, so it applies to any slot unless the build overflowed.
+MaskView exact_view(PbfDerivativeWorkspace& dw, uint32_t n) {
+    return MaskView{dw.exact.data(), dw.exact_flags.data(), n, 0.0F};
+}
+}  // namespace
+
 void PbfSolver::jvp_step(const phx_pbf_parameters& parameters, const PbfCache& cache,
                          const phx_pbf_state_tangent& state_t, const phx_control_tangent& control_t,
                          const phx_pbf_parameter_tangent& parameter_t, phx_pbf_state_tangent& next_t) {
@@ -749,7 +864,7 @@ void PbfSolver::jvp_step(const phx_pbf_parameters& parameters, const PbfCache& c
     require_vec(next_t.positions, "next position tangent");
     require_vec(next_t.velocities, "next velocity tangent");
     PassScope scope(profiler_, stream_, kPbfJvp);
-    dw_.allocate(n);
+    dw_.allocate(n, cache.iterations);
     gather_parameters(parameters, cache);
     // parameter tangents (null = zero)
     const float* tangents[8] = {parameter_t.particles.masses,         parameter_t.particles.rest_densities,
@@ -800,14 +915,17 @@ void PbfSolver::jvp_step(const phx_pbf_parameters& parameters, const PbfCache& c
     pbf_jvp_predict_kernel<<<blocks, kBlock, 0, stream_>>>(pa);
     PHX_LAUNCH_CHECK();
     int cur = 0;
+    Step se = s;  // the passes at one set of positions walk its exact masks
     for (uint32_t it = 0; it < iterations; ++it) {
         const CVec pos = it == 0u ? cache.xs.cview() : cache.positions.cview(it - 1u);
         const size_t off = static_cast<size_t>(it) * n;
-        LambdaJvpArgs la{s, tp, it, pos, dw_.p_t[cur].cview(), cache.rho.data() + off, cache.den.data() + off,
+        build_exact_masks(cache, pos, it, blocks);
+        se.mv = exact_view(dw_, n);
+        LambdaJvpArgs la{se, tp, it, pos, dw_.p_t[cur].cview(), cache.rho.data() + off, cache.den.data() + off,
                          dw_.lambda_t.data()};
         pbf_jvp_lambda_kernel<<<blocks, kBlock, 0, stream_>>>(la);
         PHX_LAUNCH_CHECK();
-        CorrectionJvpArgs ca{s,
+        CorrectionJvpArgs ca{se,
                              tp,
                              it,
                              pos,
@@ -821,10 +939,12 @@ void PbfSolver::jvp_step(const phx_pbf_parameters& parameters, const PbfCache& c
         cur ^= 1;
     }
     PostJvpArgs post{};
-    post.s = s;
     post.t = tp;
     post.slot = iterations;
     post.pos = iterations == 0u ? cache.xs.cview() : cache.positions.cview(iterations - 1u);
+    build_exact_masks(cache, post.pos, iterations, blocks);
+    se.mv = exact_view(dw_, n);
+    post.s = se;
     post.p_t = dw_.p_t[cur].cview();
     post.x_t = dw_.x_t.cview();
     post.vrec = cache.velocities.cview();
@@ -858,7 +978,7 @@ void PbfSolver::vjp_step(const phx_pbf_parameters& parameters, const PbfCache& c
     const auto& nv = next_a.velocities;
     require(nx.x && nx.y && nx.z && nv.x && nv.y && nv.z, PHX_ERROR_INVALID_ARGUMENT, "null next-state adjoint");
     PassScope scope(profiler_, stream_, kPbfVjp);
-    dw_.allocate(n);
+    dw_.allocate(n, cache.iterations);
     gather_parameters(parameters, cache);
     const uint32_t blocks = (blocks_for(n, kBlock) + 7u) / 8u * 8u;
     const uint32_t iterations = cache.iterations;
@@ -879,13 +999,15 @@ void PbfSolver::vjp_step(const phx_pbf_parameters& parameters, const PbfCache& c
            dw_.radius.data(),
            dw_.xsph.data(),
            dw_.confinement.data(),
-           MaskView{cache.masks_skin.data(), cache.mask_flags.data(), n, cache.mask_limit2}};
+           MaskView{cache.masks_skin.data(), cache.mask_flags.data(), n, cache.mask_limit2},
+           dw_.inv_rho0.data()};
     // sorted parameter adjoints start at zero (+0.0 is the all-zero bit pattern)
     for (DeviceBuffer<double>* b : {&dw_.m_a, &dw_.rho0_a, &dw_.eps_a, &dw_.strength_a, &dw_.exponent_a,
                                     &dw_.radius_a, &dw_.xsph_a, &dw_.confinement_a})
         fill_u32(stream_, reinterpret_cast<uint32_t*>(b->data()), 2u * static_cast<size_t>(n), 0u);
-    pbf_vanishes_kernel<<<blocks, kBlock, 0, stream_>>>(n, c.support_radius, dw_.strength.data(), dw_.exponent.data(),
-                                                         dw_.radius.data(), dw_.vanishes.data());
+    pbf_vjp_setup_kernel<<<blocks, kBlock, 0, stream_>>>(n, c.support_radius, dw_.strength.data(),
+                                                          dw_.exponent.data(), dw_.radius.data(), dw_.rho0.data(),
+                                                          dw_.vanishes.data(), dw_.inv_rho0.data());
     PHX_LAUNCH_CHECK();
     GatherAdjointArgs ga{n, perm, nx.x, nx.y, nx.z, nv.x, nv.y, nv.z, dw_.next_x_a.view(), dw_.next_v_a.view()};
     pbf_gather_adjoint_kernel<<<blocks, kBlock, 0, stream_>>>(ga);
@@ -893,7 +1015,10 @@ void PbfSolver::vjp_step(const phx_pbf_parameters& parameters, const PbfCache& c
 
     const CVec final_pos = iterations == 0u ? cache.xs.cview() : cache.positions.cview(iterations - 1u);
     int cur = 0;
-    PostVjpArgs pv{s,
+    Step se = s;  // the passes at one set of positions walk its exact masks
+    build_exact_masks(cache, final_pos, iterations, blocks);
+    se.mv = exact_view(dw_, n);
+    PostVjpArgs pv{se,
                    iterations,
                    final_pos,
                    cache.confined.cview(),
@@ -910,18 +1035,20 @@ void PbfSolver::vjp_step(const phx_pbf_parameters& parameters, const PbfCache& c
                    dw_.confinement_a.data()};
     pbf_vjp_post_kernel<<<blocks, kBlock, 0, stream_>>>(pv);
     PHX_LAUNCH_CHECK();
-    NormalVjpArgs nva{s, iterations, final_pos, cache.vorticities.cview(), cache.magnitudes.data(), dw_.p_a[cur].view(),
+    NormalVjpArgs nva{se, iterations, final_pos, cache.vorticities.cview(), cache.magnitudes.data(), dw_.p_a[cur].view(),
                       dw_.w_a.view(), dw_.g_a.view()};
     pbf_vjp_normal_kernel<<<blocks, kBlock, 0, stream_>>>(nva);
     PHX_LAUNCH_CHECK();
-    VorticityVjpArgs vva{s, iterations, final_pos, cache.velocities.cview(), dw_.p_a[cur].view(), dw_.vrec_a.view(),
+    VorticityVjpArgs vva{se, iterations, final_pos, cache.velocities.cview(), dw_.p_a[cur].view(), dw_.vrec_a.view(),
                          dw_.w_a.view(), dw_.x_a.view()};
     pbf_vjp_vorticity_kernel<<<blocks, kBlock, 0, stream_>>>(vva);
     PHX_LAUNCH_CHECK();
     for (uint32_t it = iterations; it-- > 0u;) {
         const CVec pos = it == 0u ? cache.xs.cview() : cache.positions.cview(it - 1u);
         const size_t off = static_cast<size_t>(it) * n;
-        CorrectionVjpArgs cva{s,
+        build_exact_masks(cache, pos, it, blocks);
+        se.mv = exact_view(dw_, n);
+        CorrectionVjpArgs cva{se,
                               it,
                               dw_.vanishes.data(),
                               parameter_a.artificial_pressure_strength != nullptr ? 1u : 0u,
@@ -935,16 +1062,20 @@ void PbfSolver::vjp_step(const phx_pbf_parameters& parameters, const PbfCache& c
                               dw_.rho0_a.data(),
                               dw_.strength_a.data(),
                               dw_.exponent_a.data(),
-                              dw_.radius_a.data()};
+                              dw_.radius_a.data(),
+                              cache.rho.data() + off,
+                              cache.den.data() + off,
+                              dw_.lambda_terms.data()};
         pbf_vjp_correction_kernel<<<blocks, kBlock, 0, stream_>>>(cva);
         PHX_LAUNCH_CHECK();
-        LambdaVjpArgs lva{s,
+        LambdaVjpArgs lva{se,
                           it,
                           pos,
                           cache.rho.data() + off,
                           cache.den.data() + off,
                           cache.gradient_sums.cview(it),
                           dw_.lambda_a.data(),
+                          dw_.lambda_terms.data(),
                           dw_.p_a[cur ^ 1].view(),
                           dw_.m_a.data(),
                           dw_.rho0_a.data(),
