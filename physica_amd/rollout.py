@@ -5,7 +5,10 @@ PBF::vjp_step chained backwards (pbf/pbf.cpp:217-258).
 particle per step at 4 constraint iterations, so a 262,144-particle,
 120-step rollout holds roughly 8 GB of HBM and the backward pass recomputes
 nothing). `checkpoint_every=k` keeps only every k-th state and recomputes each
-k-step segment's caches before differentiating it.
+k-step segment's caches before differentiating it; the segment's states and
+caches are one workspace reused by every segment. The steps whose caches are
+not kept can run on `forward_solver`, a solver of the same scene created in
+forward mode: it skips the recording and gives the same states bit for bit.
 """
 from __future__ import annotations
 
@@ -19,10 +22,12 @@ from .particle import PBF, Control, ParticleParameterAdjoint
 class PbfRollout:
     """A fixed-length rollout of one PBF solver created in differentiable mode."""
 
-    def __init__(self, solver: PBF, steps: int, checkpoint_every: int = 1, parameter_gradients: bool = True):
+    def __init__(self, solver: PBF, steps: int, checkpoint_every: int = 1, parameter_gradients: bool = True,
+                 forward_solver: Optional[PBF] = None):
         if steps < 0 or checkpoint_every < 1:
             raise ValueError("steps must be >= 0 and checkpoint_every >= 1")
         self.solver = solver
+        self.forward_solver = forward_solver
         self.steps = steps
         self.checkpoint_every = checkpoint_every
         # without parameter gradients the VJPs receive null parameter adjoints
@@ -32,30 +37,55 @@ class PbfRollout:
         self._caches: List[Optional[PBF.StepCache]] = []
         self._control: Optional[Control] = None
         self._parameters: Optional[PBF.Parameters] = None
-
-    def _step(self, state: PBF.State, keep_cache: bool):
-        nxt = self.solver.allocate_state()
-        cache = self.solver.allocate_step_cache()
-        self.solver.forward_step(state, self._control, self._parameters, nxt, cache)
-        return nxt, cache if keep_cache else None
+        # segment workspace (checkpointed rollouts)
+        self._segment_states: List[PBF.State] = []
+        self._segment_caches: List[PBF.StepCache] = []
+        self._scratch_states: List[PBF.State] = []
+        self._scratch_cache: Optional[PBF.StepCache] = None
 
     def forward(self, state: PBF.State, control: Control, parameters: PBF.Parameters) -> PBF.State:
         """Runs the rollout from `state` (not modified) and returns the final state."""
         self._control, self._parameters = control, parameters
-        start = self.solver.allocate_state()
-        self.solver.copy_state(state, start)
+        solver = self.solver
+        start = solver.allocate_state()
+        solver.copy_state(state, start)
         start.particles.step_index = state.particles.step_index
         self._states = [start]
         self._caches = []
+        keep_all = self.checkpoint_every == 1
+        if not keep_all and not self._scratch_states:
+            self._scratch_states = [solver.allocate_state(), solver.allocate_state()]
         current = start
         for k in range(self.steps):
-            keep = self.checkpoint_every == 1
-            nxt, cache = self._step(current, keep)
-            self._caches.append(cache)
-            checkpoint = self.checkpoint_every == 1 or (k + 1) % self.checkpoint_every == 0 or k + 1 == self.steps
+            checkpoint = keep_all or (k + 1) % self.checkpoint_every == 0 or k + 1 == self.steps
+            nxt = solver.allocate_state() if checkpoint else self._scratch_states[k % 2]
+            if keep_all:
+                cache = solver.allocate_step_cache()
+                solver.forward_step(current, control, parameters, nxt, cache)
+                self._caches.append(cache)
+            else:
+                stepper = self.forward_solver or solver
+                if self._scratch_cache is None:
+                    self._scratch_cache = stepper.allocate_step_cache()
+                stepper.forward_step(current, control, parameters, nxt, self._scratch_cache)
+                self._caches.append(None)
             self._states.append(nxt if checkpoint else None)
             current = nxt
         return current
+
+    def _segment(self, first: int, last: int):
+        """Recomputes the states and caches of steps [first, last) from the
+        checkpoint at `first` into the segment workspace."""
+        solver = self.solver
+        count = last - first
+        while len(self._segment_caches) < count:
+            self._segment_caches.append(solver.allocate_step_cache())
+            self._segment_states.append(solver.allocate_state())
+        states = [self._states[first]] + self._segment_states[:count]
+        for local in range(count):
+            solver.forward_step(states[local], self._control, self._parameters, states[local + 1],
+                                self._segment_caches[local])
+        return states, self._segment_caches[:count]
 
     def backward(self, final_adjoint: PBF.StateAdjoint):
         """Chains the VJPs from the adjoint of the final state. Returns
@@ -73,25 +103,23 @@ class PbfRollout:
         solver.copy_state_adjoint(final_adjoint, adjoint)
         k = self.steps
         while k > 0:
-            first = ((k - 1) // self.checkpoint_every) * self.checkpoint_every
             if self.checkpoint_every == 1:
+                first = k - 1
                 states, caches = [self._states[k - 1], self._states[k]], [self._caches[k - 1]]
             else:
-                # recompute the segment [first, k) from its checkpoint
-                states, caches = [self._states[first]], []
-                for _ in range(first, k):
-                    nxt, cache = self._step(states[-1], True)
-                    states.append(nxt)
-                    caches.append(cache)
+                first = ((k - 1) // self.checkpoint_every) * self.checkpoint_every
+                states, caches = self._segment(first, k)
             for local in reversed(range(len(caches))):
                 previous = solver.allocate_state_adjoint()
                 solver.vjp_step(states[local], self._control, self._parameters, states[local + 1], caches[local],
                                 adjoint, previous, control_adjoint, parameter_adjoint)
                 adjoint = previous
-            k = first if self.checkpoint_every > 1 else k - 1
+            k = first
         return adjoint, control_adjoint, parameter_adjoint if self.parameter_gradients else None
 
     def release(self) -> None:
-        """Frees the recorded states and caches."""
+        """Frees the recorded states, caches and segment workspace."""
         self._states, self._caches = [], []
+        self._segment_states, self._segment_caches = [], []
+        self._scratch_states, self._scratch_cache = [], None
         torch.cuda.synchronize()
