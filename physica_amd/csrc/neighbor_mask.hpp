@@ -142,11 +142,14 @@ struct MaskView {
     const uint32_t* flags;
     uint32_t n;
     float limit2;
+    // exact masks of one set of positions: they apply unless their build
+    // overflowed (flags[kFlagOverflow]); no displacement slot is read
+    uint32_t exact = 0u;
 };
 
 __device__ __forceinline__ bool masks_apply(const MaskView& m, uint32_t slot) {
-    return m.masks != nullptr && m.flags[kFlagOverflow] == 0u &&
-           __uint_as_float(m.flags[kFlagDisplacement + slot]) <= m.limit2;
+    if (m.masks == nullptr || m.flags[kFlagOverflow] != 0u) return false;
+    return m.exact != 0u || __uint_as_float(m.flags[kFlagDisplacement + slot]) <= m.limit2;
 }
 
 // Visits the candidates of particle i in the reference order: the marked
@@ -162,6 +165,72 @@ __device__ __forceinline__ void visit_candidates(const SortedGrid& g, const Mask
     } else {
         for_each_candidate(g, cx, cy, cz, visit);
     }
+}
+
+// Visits the candidates of particle i like visit_candidates, where visit(s)
+// returns whether candidate s is kept, and writes the kept candidates of each
+// stencil slot k as the mask out[k * n + i] (every slot is written). Returns
+// false if a kept candidate lies beyond the first kMaskBits particles of its
+// cell: the masks are then incomplete and must not be used.
+template <class Visit>
+__device__ __forceinline__ bool visit_and_mask(const SortedGrid& g, const MaskView& m, bool masked, uint32_t i,
+                                               int cx, int cy, int cz, const int* delta, uint16_t* column,
+                                               uint32_t stride, uint16_t* __restrict__ out, uint32_t n,
+                                               Visit&& visit) {
+    uint32_t written = 0u;
+    auto emit = [&](uint32_t k, uint32_t bits) {
+        for (; written < k; ++written) out[static_cast<size_t>(written) * n + i] = 0u;
+        out[static_cast<size_t>(k) * n + i] = static_cast<uint16_t>(bits);
+        written = k + 1u;
+    };
+    bool complete = true;
+    if (masked) {
+        uint32_t nonzero = load_lane_column(m.masks, m.n, i, column, stride);
+        const uint32_t center = cell_of(g, cx, cy, cz);
+        while (nonzero != 0u) {
+            const uint32_t k = static_cast<uint32_t>(__builtin_ctz(nonzero));
+            nonzero &= nonzero - 1u;
+            uint32_t candidates = column[k * stride];
+            const uint32_t base = g.offsets[center + static_cast<uint32_t>(delta[k])];
+            uint32_t bits = 0u;
+            while (candidates != 0u) {
+                const uint32_t b = static_cast<uint32_t>(__builtin_ctz(candidates));
+                candidates &= candidates - 1u;
+                if (visit(base + b)) bits |= 1u << b;
+            }
+            emit(k, bits);
+        }
+    } else {
+        uint32_t k = 0u;
+        for (int ox = -1; ox <= 1; ++ox) {
+            const int X = cx + ox;
+            for (int oy = -1; oy <= 1; ++oy) {
+                const int Y = cy + oy;
+                for (int oz = -1; oz <= 1; ++oz, ++k) {
+                    const int Z = cz + oz;
+                    if (X < 0 || X >= static_cast<int>(g.cx) || Y < 0 || Y >= static_cast<int>(g.cy) || Z < 0 ||
+                        Z >= static_cast<int>(g.cz))
+                        continue;
+                    const uint32_t c = cell_of(g, X, Y, Z);
+                    const uint32_t first = g.offsets[c];
+                    const uint32_t last = g.offsets[c + 1u];
+                    uint32_t bits = 0u;
+                    for (uint32_t s = first; s < last; ++s) {
+                        if (visit(s)) {
+                            if (s - first < kMaskBits) {
+                                bits |= 1u << (s - first);
+                            } else {
+                                complete = false;
+                            }
+                        }
+                    }
+                    emit(k, bits);
+                }
+            }
+        }
+    }
+    for (; written < static_cast<uint32_t>(kStencil); ++written) out[static_cast<size_t>(written) * n + i] = 0u;
+    return complete;
 }
 
 // Skin mask of particle i at p from SoA positions: bit b of slot k marks the

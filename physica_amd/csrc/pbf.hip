@@ -66,8 +66,16 @@ struct LambdaArgs {
     Vec gsum;    // optional (recorded step)
     MaskView mv;
     uint32_t slot;  // displacement slot of the positions read
+    // exact masks of the positions read (pairs inside exact_mask_threshold),
+    // for the correction pass and the derivatives; flags[kFlagOverflow] is
+    // raised if a cell cannot be encoded
+    float t_exact;
+    uint16_t* exact_out;
+    uint32_t* exact_flags;
 };
 
+// density_forward_kernel (density.cu:18-40, poly6) and lambda_forward_kernel
+// (pbf.cu:95-127); the walk also writes the exact masks of the positions.
 __global__ void __launch_bounds__(kBlock) pbf_lambda_kernel(LambdaArgs a) {
     PHX_MASK_LDS(a.grid);
     const uint32_t i = xcd_block() * kBlock + threadIdx.x;
@@ -80,16 +88,20 @@ __global__ void __launch_bounds__(kBlock) pbf_lambda_kernel(LambdaArgs a) {
     float density = 0.0F;
     Float3 gradient_sum{0.0F, 0.0F, 0.0F};
     float denominator = a.eps[i];
-    visit_candidates(a.grid, a.mv, masked, i, cx, cy, cz, delta, smask + threadIdx.x, kBlock, [&](uint32_t s) {
-        const Float3 d = sub(p, load(a.pos, s));
-        const float ms = a.m[s];
-        density += ms * poly6(d, a.h);
-        if (s == i) return;
-        const Float3 g = spiky_gradient(d, a.h, a.spiky);
-        const float coefficient = ms / rest;
-        gradient_sum = add(gradient_sum, scale(g, coefficient));
-        denominator += coefficient * coefficient * dot(g, g);
-    });
+    const bool complete = visit_and_mask(
+        a.grid, a.mv, masked, i, cx, cy, cz, delta, smask + threadIdx.x, kBlock, a.exact_out, a.n, [&](uint32_t s) {
+            const Float3 d = sub(p, load(a.pos, s));
+            const float ms = a.m[s];
+            density += ms * poly6(d, a.h);
+            if (s != i) {
+                const Float3 g = spiky_gradient(d, a.h, a.spiky);
+                const float coefficient = ms / rest;
+                gradient_sum = add(gradient_sum, scale(g, coefficient));
+                denominator += coefficient * coefficient * dot(g, g);
+            }
+            return dot(d, d) < a.t_exact;
+        });
+    if (!complete) atomicOr(&a.exact_flags[kFlagOverflow], 1u);
     denominator += dot(gradient_sum, gradient_sum);
     a.rho[i] = density;
     a.lambda[i] = -(density / rest - 1.0F) / denominator;
@@ -198,6 +210,11 @@ struct PostArgs {
     Vec next_x, next_v;
     MaskView mv;
     uint32_t slot;  // displacement slot of the final positions
+    // exact masks of the final positions: written by the vorticity pass and
+    // walked by the normal and XSPH passes (through mv, slot 0)
+    float t_exact;
+    uint16_t* exact_out;
+    uint32_t* exact_flags;
 };
 
 __global__ void __launch_bounds__(kBlock) pbf_vorticity_kernel(PostArgs a) {
@@ -210,10 +227,13 @@ __global__ void __launch_bounds__(kBlock) pbf_vorticity_kernel(PostArgs a) {
     int cx, cy, cz;
     particle_cell(a.grid, load(a.topo, i), cx, cy, cz);
     Float3 w{0.0F, 0.0F, 0.0F};
-    visit_candidates(a.grid, a.mv, masked, i, cx, cy, cz, delta, smask + threadIdx.x, kBlock, [&](uint32_t s) {
-        if (s == i) return;
-        w = add(w, cross(sub(load(a.vel, s), v), spiky_gradient(sub(p, load(a.pos, s)), a.h, a.spiky)));
-    });
+    const bool complete = visit_and_mask(
+        a.grid, a.mv, masked, i, cx, cy, cz, delta, smask + threadIdx.x, kBlock, a.exact_out, a.n, [&](uint32_t s) {
+            const Float3 d = sub(p, load(a.pos, s));
+            if (s != i) w = add(w, cross(sub(load(a.vel, s), v), spiky_gradient(d, a.h, a.spiky)));
+            return dot(d, d) < a.t_exact;
+        });
+    if (!complete) atomicOr(&a.exact_flags[kFlagOverflow], 1u);
     store(a.vort, i, w);
     a.magnitude[i] = length(w);
 }
@@ -314,6 +334,8 @@ void PbfCache::allocate(uint32_t n, uint32_t cells, uint32_t pressure_iterations
     normalizers.resize(n);
     masks_skin.resize(static_cast<size_t>(kStencil) * n);
     mask_flags.resize(kFlagDisplacement + pressure_iterations + 1u);
+    exact_masks.resize(static_cast<size_t>(pressure_iterations + 1u) * kStencil * n);
+    exact_flags.resize(pressure_iterations + 1u);
 }
 
 PbfSolver::PbfSolver(const phx_domain_configuration* domain, const phx_pbf_configuration* configuration,
@@ -329,6 +351,8 @@ PbfSolver::PbfSolver(const phx_domain_configuration* domain, const phx_pbf_confi
     for (VecBuffer* b : {&xs_, &x0_, &pa_, &pb_, &vel_, &vort_, &vc_}) b->resize(n);
     for (DeviceBuffer<float>* b : {&m_, &rho0_, &eps_, &strength_, &exponent_, &radius_, &xsph_, &confinement_, &magnitude_})
         b->resize(n);
+    exact_.resize(static_cast<size_t>(kStencil) * n);
+    exact_flags_.resize(config_.pressure_iterations + 1u);
     // displacement slots: one per iterate (the first, x* itself, stays zero)
     masks_.allocate(n, config_.pressure_iterations + 1u, domain_.config.support_radius);
 }
@@ -403,9 +427,23 @@ void PbfSolver::forward_step(const phx_pbf_state& state, const phx_control& cont
         PHX_LAUNCH_CHECK();
     }
     const MaskView mv{skin, masks_.flags(), n, masks_.limit2()};
+    // exact masks of position set k (iterate k, the final positions last):
+    // written by the pass that first walks those positions, walked by the
+    // later passes at them; a recorded step keeps every set. Each set has one
+    // overflow word.
+    const float t_exact = exact_mask_threshold(h);
+    const uint32_t sets = config_.pressure_iterations + 1u;
+    uint32_t* exact_flags = rec ? cache.exact_flags.data() : exact_flags_.data();
+    fill_u32(stream_, exact_flags, sets, 0u);
+    auto exact_set = [&](uint32_t k) {
+        require(k < sets, PHX_ERROR_INVALID_ARGUMENT, "exact mask set out of range");
+        return rec ? cache.exact_masks.data() + static_cast<size_t>(k) * kStencil * n : exact_.data();
+    };
+    auto exact_view = [&](uint32_t k) { return MaskView{exact_set(k), exact_flags + k, n, 0.0F, 1u}; };
 
     LambdaArgs la{n,           h,        spiky_,     grid, xs.cview(), xs.cview(), m_.data(), rho0_.data(),
                   eps_.data(), cache.densities.data(), cache.lambdas.data(), nullptr, Vec{}, mv, 0u};
+    la.t_exact = t_exact;
     CorrectionArgs ca{};
     ca.n = n;
     ca.h = h;
@@ -432,7 +470,10 @@ void PbfSolver::forward_step(const phx_pbf_state& state, const phx_control& cont
     for (uint32_t it = 0; it < iterations; ++it) {
         la.pos = current;
         la.slot = it;
-        ca.slot = it;
+        la.exact_out = exact_set(it);
+        la.exact_flags = exact_flags + it;
+        ca.mv = exact_view(it);
+        ca.slot = 0u;
         ca.moved_to = masks_.flags() + kFlagDisplacement + it + 1u;
         if (rec) {
             const size_t off = static_cast<size_t>(it) * n;
@@ -493,8 +534,15 @@ void PbfSolver::forward_step(const phx_pbf_state& state, const phx_control& cont
     post.perm = cache.neighborhood.perm.data();
     post.mv = mv;
     post.slot = iterations;
+    post.t_exact = t_exact;
+    post.exact_out = exact_set(iterations);
+    post.exact_flags = exact_flags + iterations;
     post.next_x = vec(next.particles.positions);
     post.next_v = vec(next.particles.velocities);
+    // the normal and XSPH passes: every field of `post`, with the exact masks
+    PostArgs post_exact = post;
+    post_exact.mv = exact_view(iterations);
+    post_exact.slot = 0u;
     {
         PassScope scope(profiler_, stream_, kPbfVorticity);
         pbf_vorticity_kernel<<<blocks, kBlock, 0, stream_>>>(post);
@@ -502,12 +550,12 @@ void PbfSolver::forward_step(const phx_pbf_state& state, const phx_control& cont
     }
     {
         PassScope scope(profiler_, stream_, kPbfNormal);
-        pbf_normal_kernel<<<blocks, kBlock, 0, stream_>>>(post);
+        pbf_normal_kernel<<<blocks, kBlock, 0, stream_>>>(post_exact);
         PHX_LAUNCH_CHECK();
     }
     {
         PassScope scope(profiler_, stream_, kPbfXsph);
-        pbf_xsph_kernel<<<blocks, kBlock, 0, stream_>>>(post);
+        pbf_xsph_kernel<<<blocks, kBlock, 0, stream_>>>(post_exact);
         PHX_LAUNCH_CHECK();
     }
     if (rec) {

@@ -55,6 +55,10 @@ struct PbfCache {
     DeviceBuffer<uint16_t> masks_skin;
     DeviceBuffer<uint32_t> mask_flags;
     float mask_limit2 = -1.0F;
+    // exact masks of every set of positions (iterates 0 .. I - 1, then the
+    // final positions), [set][kStencil][n], and one overflow word per set
+    DeviceBuffer<uint16_t> exact_masks;
+    DeviceBuffer<uint32_t> exact_flags;
 
     void allocate(uint32_t n, uint32_t cells, uint32_t pressure_iterations);
 };
@@ -88,9 +92,7 @@ struct PbfDerivativeWorkspace {
     DeviceBuffer<uint8_t> vanishes;  // per particle: the artificial pressure is the signed zero -k
     DeviceBuffer<double> inv_rho0;   // 1 / rho0
     DeviceBuffer<double2> lambda_terms;  // (density adjoint, D_bar) of one iteration
-    DeviceBuffer<uint16_t> exact;     // exact masks of one set of positions
-    DeviceBuffer<uint32_t> exact_flags;
-    void allocate(uint32_t n, uint32_t iterations);
+    void allocate(uint32_t n);
 };
 
 class PbfSolver {
@@ -114,9 +116,7 @@ class PbfSolver {
 
   private:
     void gather_parameters(const phx_pbf_parameters& parameters, const PbfCache& cache);
-    // Exact masks of `pos` (displacement slot `slot` of the step) into the
-    // derivative workspace.
-    void build_exact_masks(const PbfCache& cache, CVec pos, uint32_t slot, uint32_t blocks);
+
 
     Domain domain_;
     phx_pbf_configuration config_{};
@@ -128,6 +128,8 @@ class PbfSolver {
     // sorted-order workspace
     VecBuffer xs_, x0_, pa_, pb_, vel_, vort_, vc_;
     DeviceBuffer<float> m_, rho0_, eps_, strength_, exponent_, radius_, xsph_, confinement_, magnitude_;
+    DeviceBuffer<uint16_t> exact_;  // exact masks of the current set of positions (forward mode)
+    DeviceBuffer<uint32_t> exact_flags_;
     PbfDerivativeWorkspace dw_;
 };
 

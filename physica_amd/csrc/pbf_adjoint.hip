@@ -270,69 +270,6 @@ __global__ void __launch_bounds__(kBlock) pbf_vjp_vorticity_kernel(VorticityVjpA
     store(a.p_a, i, add(add(load(a.p_a, i), pcon), cb));
 }
 
-struct ExactMaskArgs {
-    uint32_t n;
-    SortedGrid grid;
-    CVec topo;  // x*: the cells of the step
-    CVec pos;   // the positions the masks are for
-    float t;    // exact_mask_threshold
-    MaskView skin;
-    uint32_t slot;  // displacement slot of pos
-    uint16_t* masks;
-    uint32_t* flags;
-};
-
-// Exact masks of the positions `pos`: bit b of stencil slot k marks the b-th
-// particle of that cell within the support. Every kernel of a derivative pass
-// vanishes outside the support, so the passes at `pos` visit these candidates
-// only. Where the step's skin mask is valid at `pos`, only its candidates are
-// tested; a cell holding more than kMaskBits particles sets the overflow flag
-// (the passes then scan the full stencil).
-__global__ void __launch_bounds__(kBlock) pbf_exact_mask_kernel(ExactMaskArgs a) {
-    const uint32_t i = xcd_block() * kBlock + threadIdx.x;
-    if (i >= a.n) return;
-    const Float3 p = load(a.pos, i);
-    int cx, cy, cz;
-    particle_cell(a.grid, load(a.topo, i), cx, cy, cz);
-    const bool skin = masks_apply(a.skin, a.slot);
-    bool complete = true;
-    int k = 0;
-    for (int ox = -1; ox <= 1; ++ox) {
-        const int X = cx + ox;
-        for (int oy = -1; oy <= 1; ++oy) {
-            const int Y = cy + oy;
-            for (int oz = -1; oz <= 1; ++oz, ++k) {
-                const int Z = cz + oz;
-                uint32_t bits = 0u;
-                if (X >= 0 && X < static_cast<int>(a.grid.cx) && Y >= 0 && Y < static_cast<int>(a.grid.cy) &&
-                    Z >= 0 && Z < static_cast<int>(a.grid.cz)) {
-                    const uint32_t c = cell_of(a.grid, X, Y, Z);
-                    const uint32_t first = a.grid.offsets[c];
-                    uint32_t count = a.grid.offsets[c + 1u] - first;
-                    uint32_t candidates;
-                    if (skin) {
-                        candidates = a.skin.masks[static_cast<size_t>(k) * a.n + i];
-                    } else {
-                        if (count > kMaskBits) {
-                            complete = false;
-                            count = kMaskBits;
-                        }
-                        candidates = (1u << count) - 1u;
-                    }
-                    while (candidates != 0u) {
-                        const uint32_t b = static_cast<uint32_t>(__builtin_ctz(candidates));
-                        candidates &= candidates - 1u;
-                        const Float3 d = sub(p, load(a.pos, first + b));
-                        if (dot(d, d) < a.t) bits |= 1u << b;
-                    }
-                }
-                a.masks[static_cast<size_t>(k) * a.n + i] = static_cast<uint16_t>(bits);
-            }
-        }
-    }
-    if (!complete) atomicOr(&a.flags[kFlagOverflow], 1u);
-}
-
 struct CorrectionVjpArgs {
     Step s;
     uint32_t slot;  // displacement slot of the positions read
@@ -782,9 +719,7 @@ __global__ void __launch_bounds__(kBlock) pbf_jvp_xsph_kernel(PostJvpArgs a) {
 
 }  // namespace
 
-void PbfDerivativeWorkspace::allocate(uint32_t n, uint32_t iterations) {
-    exact.resize(static_cast<size_t>(kStencil) * n);
-    exact_flags.resize(kFlagDisplacement + iterations + 1u);
+void PbfDerivativeWorkspace::allocate(uint32_t n) {
     for (DeviceBuffer<float>* b : {&m, &rho0, &eps, &strength, &exponent, &radius, &xsph, &confinement, &m_t, &rho0_t,
                                    &eps_t, &strength_t, &exponent_t, &radius_t, &xsph_t, &confinement_t, &rho_t,
                                    &lambda_t, &magnitude_t})
@@ -827,27 +762,13 @@ void require_recorded(const PbfCache& cache, phx_execution_mode mode, uint32_t n
 
 }  // namespace
 
-void PbfSolver::build_exact_masks(const PbfCache& cache, CVec pos, uint32_t slot, uint32_t blocks) {
-    const uint32_t n = domain_.config.particle_count;
-    fill_u32(stream_, dw_.exact_flags.data(), dw_.exact_flags.size(), 0u);
-    ExactMaskArgs ea{n,
-                     cache.neighborhood.grid(),
-                     cache.xs.cview(),
-                     pos,
-                     exact_mask_threshold(domain_.config.support_radius),
-                     MaskView{cache.masks_skin.data(), cache.mask_flags.data(), n, cache.mask_limit2},
-                     slot,
-                     dw_.exact.data(),
-                     dw_.exact_flags.data()};
-    pbf_exact_mask_kernel<<<blocks, kBlock, 0, stream_>>>(ea);
-    PHX_LAUNCH_CHECK();
-}
-
 namespace {
-// The view of the workspace's exact masks: every displacement word stays
-// zero, so it applies to any slot unless the build overflowed.
-MaskView exact_view(PbfDerivativeWorkspace& dw, uint32_t n) {
-    return MaskView{dw.exact.data(), dw.exact_flags.data(), n, 0.0F};
+// The recorded exact masks of position set k (iterate k; the final positions
+// are set I), which apply unless their build overflowed.
+MaskView exact_view(const PbfCache& cache, uint32_t k, uint32_t n) {
+    require(k <= cache.iterations, PHX_ERROR_INVALID_ARGUMENT, "exact mask set out of range");
+    return MaskView{cache.exact_masks.data() + static_cast<size_t>(k) * kStencil * n, cache.exact_flags.data() + k, n,
+                    0.0F, 1u};
 }
 }  // namespace
 
@@ -864,7 +785,7 @@ void PbfSolver::jvp_step(const phx_pbf_parameters& parameters, const PbfCache& c
     require_vec(next_t.positions, "next position tangent");
     require_vec(next_t.velocities, "next velocity tangent");
     PassScope scope(profiler_, stream_, kPbfJvp);
-    dw_.allocate(n, cache.iterations);
+    dw_.allocate(n);
     gather_parameters(parameters, cache);
     // parameter tangents (null = zero)
     const float* tangents[8] = {parameter_t.particles.masses,         parameter_t.particles.rest_densities,
@@ -919,8 +840,7 @@ void PbfSolver::jvp_step(const phx_pbf_parameters& parameters, const PbfCache& c
     for (uint32_t it = 0; it < iterations; ++it) {
         const CVec pos = it == 0u ? cache.xs.cview() : cache.positions.cview(it - 1u);
         const size_t off = static_cast<size_t>(it) * n;
-        build_exact_masks(cache, pos, it, blocks);
-        se.mv = exact_view(dw_, n);
+        se.mv = exact_view(cache, it, n);
         LambdaJvpArgs la{se, tp, it, pos, dw_.p_t[cur].cview(), cache.rho.data() + off, cache.den.data() + off,
                          dw_.lambda_t.data()};
         pbf_jvp_lambda_kernel<<<blocks, kBlock, 0, stream_>>>(la);
@@ -942,8 +862,7 @@ void PbfSolver::jvp_step(const phx_pbf_parameters& parameters, const PbfCache& c
     post.t = tp;
     post.slot = iterations;
     post.pos = iterations == 0u ? cache.xs.cview() : cache.positions.cview(iterations - 1u);
-    build_exact_masks(cache, post.pos, iterations, blocks);
-    se.mv = exact_view(dw_, n);
+    se.mv = exact_view(cache, iterations, n);
     post.s = se;
     post.p_t = dw_.p_t[cur].cview();
     post.x_t = dw_.x_t.cview();
@@ -978,7 +897,7 @@ void PbfSolver::vjp_step(const phx_pbf_parameters& parameters, const PbfCache& c
     const auto& nv = next_a.velocities;
     require(nx.x && nx.y && nx.z && nv.x && nv.y && nv.z, PHX_ERROR_INVALID_ARGUMENT, "null next-state adjoint");
     PassScope scope(profiler_, stream_, kPbfVjp);
-    dw_.allocate(n, cache.iterations);
+    dw_.allocate(n);
     gather_parameters(parameters, cache);
     const uint32_t blocks = (blocks_for(n, kBlock) + 7u) / 8u * 8u;
     const uint32_t iterations = cache.iterations;
@@ -1016,8 +935,7 @@ void PbfSolver::vjp_step(const phx_pbf_parameters& parameters, const PbfCache& c
     const CVec final_pos = iterations == 0u ? cache.xs.cview() : cache.positions.cview(iterations - 1u);
     int cur = 0;
     Step se = s;  // the passes at one set of positions walk its exact masks
-    build_exact_masks(cache, final_pos, iterations, blocks);
-    se.mv = exact_view(dw_, n);
+    se.mv = exact_view(cache, iterations, n);
     PostVjpArgs pv{se,
                    iterations,
                    final_pos,
@@ -1046,8 +964,7 @@ void PbfSolver::vjp_step(const phx_pbf_parameters& parameters, const PbfCache& c
     for (uint32_t it = iterations; it-- > 0u;) {
         const CVec pos = it == 0u ? cache.xs.cview() : cache.positions.cview(it - 1u);
         const size_t off = static_cast<size_t>(it) * n;
-        build_exact_masks(cache, pos, it, blocks);
-        se.mv = exact_view(dw_, n);
+        se.mv = exact_view(cache, it, n);
         CorrectionVjpArgs cva{se,
                               it,
                               dw_.vanishes.data(),
