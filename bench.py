@@ -42,6 +42,8 @@ def parse_args(argv=None):
     p.add_argument("--no-cpu-baseline", action="store_true", help="skip the CPU oracle timing")
     p.add_argument("--no-pass-timing", action="store_true", help="time without per-pass events")
     p.add_argument("--no-adjoint", action="store_true", help="skip the differentiable PBF rollout ratio")
+    p.add_argument("--adjoint-steps", type=int, default=120,
+                   help="steps of the differentiable PBF rollout (120: the configured rollout)")
     return p.parse_args(argv)
 
 
@@ -286,7 +288,7 @@ def main(argv=None) -> int:
     adjoint = None
     if not args.no_adjoint:
         del states, cache
-        adjoint = adjoint_ratio(device)
+        adjoint = adjoint_ratio(device, steps=args.adjoint_steps)
 
     line = {
         "metric": METRIC,
