@@ -89,6 +89,21 @@ def build_cpp_tests(verbose: bool = False) -> list:
     return built
 
 
+MATH_CHECK = ROOT / "tests" / "native" / "libphx_math_check.so"
+
+
+def build_math_check(verbose: bool = False) -> Path:
+    """Compile the rounding-helper checks (tests/native/math_check.hip)."""
+    src = ROOT / "tests" / "native" / "math_check.hip"
+    tmp = MATH_CHECK.with_suffix(".so.tmp")
+    cmd = [hipcc(), *HIPCC_FLAGS, "-shared", "-I", str(CSRC), str(src), "-o", str(tmp)]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, MATH_CHECK)
+    return MATH_CHECK
+
+
 def build_oracle(verbose: bool = False) -> Path:
     """Compile the C oracle (test infrastructure) with its Makefile."""
     cmd = ["make", "-C", str(ROOT / "oracle"), "-j4"]
@@ -101,6 +116,7 @@ def build_oracle(verbose: bool = False) -> Path:
 def main() -> int:
     build_library(verbose=True)
     build_cpp_tests(verbose=True)
+    build_math_check(verbose=True)
     build_oracle(verbose=True)
     return 0
 

@@ -97,31 +97,81 @@ struct KernelConstants {
     float cubic;        // 8 / (pi h h h)
     float viscosity;    // pi h h h h h h
     float sqrt_support; // smallest float t with sqrtf(t) >= h
+    float inv_h;        // RN(1 / h)
+    float inv_viscosity; // RN(1 / viscosity)
 };
 
-// cubic(d, h) as a function of dist = length(d), with the constant supplied:
-// the same operations as cubic()
+// Correctly rounded x / b from inv_b = RN(1 / b), for b > 0: the estimate
+// q0 = x inv_b is refined once, q1 = q0 + (x - b q0) inv_b, which leaves it
+// within an ulp of x / b; the remainder x - b q1 is then exact and
+// q1 + (x - b q1) inv_b rounds to RN(x / b) (P. Markstein, IBM J. Res. Dev.
+// 34(1), 1990). Each step is one IEEE multiply or fused multiply-add, so the
+// host and the device agree. Exact for x = 0 and for x >= 2^-100 while the
+// quotient stays normal (below 2^-105 the remainders underflow and it can
+// miss); the callers' arguments are distances, at least 2^-74.5 when
+// nonzero, and multiples of them. Checked against x / b for every float x
+// of two binades for many b and for every x in [2^-100, h) on the GPU.
+__host__ __device__ inline float div_rn(float x, float b, float inv_b) {
+    const float q0 = x * inv_b;
+    const float r0 = fmaf(-q0, b, x);
+    const float q1 = fmaf(r0, inv_b, q0);
+    const float r1 = fmaf(-q1, b, x);
+    return fmaf(r1, inv_b, q1);
+}
+
+// Correctly rounded square root of 0 <= x < 2^64: the hardware estimate s of
+// sqrt(y), y = x 2^64 (a normal number or zero for every such x), moved to
+// the float neighbour that the signs of y - s' s select (the correction of
+// the AMDGPU backend's sqrtf), then scaled back by 2^-32, which is exact
+// since every nonzero result is at least 2^-74.5. It equals sqrtf(x) on that
+// range (checked on the GPU for every float below 2^64) without the tests
+// sqrtf makes for denormal and special arguments; from 2^64 on it returns
+// +inf, which the pair kernels only compare with the support radius.
+__device__ __forceinline__ float sqrt_rn(float x) {
+    const float y = x * 0x1p64F;
+    const float s = __builtin_amdgcn_sqrtf(y);
+    const float down = __uint_as_float(__float_as_uint(s) - 1u);
+    const float up = __uint_as_float(__float_as_uint(s) + 1u);
+    const float below = fmaf(-down, s, y);
+    const float above = fmaf(-up, s, y);
+    float r = below <= 0.0F ? down : s;
+    r = above > 0.0F ? up : r;
+    return r * 0x1p-32F;
+}
+
+// length(d) of a neighbour pair (the same value; any |d| >= 2^32 gives +inf,
+// which lies outside every support).
+__device__ __forceinline__ float pair_distance(Float3 d) { return sqrt_rn(dot(d, d)); }
+
+// cubic(d, h) as a function of dist = length(d), with the constants
+// supplied: the same operations as cubic(), both branches evaluated and
+// selected, so that a wave walking neighbours does not diverge.
 __device__ __forceinline__ float cubic_value(float dist, const KernelConstants& k) {
-    if (dist >= k.h) return 0.0F;
-    const float q = dist / k.h;
-    if (q <= 0.5F) return k.cubic * (6.0F * q * q * q - 6.0F * q * q + 1.0F);
+    const float q = div_rn(dist, k.h, k.inv_h);
+    const float near = k.cubic * (6.0F * q * q * q - 6.0F * q * q + 1.0F);
     const float diff = 1.0F - q;
-    return k.cubic * 2.0F * diff * diff * diff;
+    const float far = k.cubic * 2.0F * diff * diff * diff;
+    const float w = q <= 0.5F ? near : far;
+    return dist >= k.h ? 0.0F : w;
 }
 
 // The radial factor f of cubic_gradient(d, h) = f * d (zero outside the
-// support and at d = 0), with the constant supplied.
+// support and at d = 0), with the constants supplied; branch-free like
+// cubic_value().
 __device__ __forceinline__ float cubic_gradient_factor(float dist, const KernelConstants& k) {
-    if (dist == 0.0F || dist >= k.h) return 0.0F;
-    const float q = dist / k.h;
-    const float deriv = q <= 0.5F ? 18.0F * q * q - 12.0F * q : -6.0F * (1.0F - q) * (1.0F - q);
-    return k.cubic * deriv / (k.h * dist);
+    const float q = div_rn(dist, k.h, k.inv_h);
+    const float near = 18.0F * q * q - 12.0F * q;
+    const float diff = 1.0F - q;
+    const float far = -6.0F * diff * diff;
+    const float deriv = q <= 0.5F ? near : far;
+    const float f = k.cubic * deriv / (k.h * dist);
+    return dist == 0.0F || dist >= k.h ? 0.0F : f;
 }
 
 __device__ __forceinline__ float viscosity_laplacian_c(Float3 d, const KernelConstants& k) {
-    const float dist = length(d);
-    if (dist >= k.h) return 0.0F;
-    return 45.0F * (k.h - dist) / k.viscosity;
+    const float dist = pair_distance(d);
+    const float l = div_rn(45.0F * (k.h - dist), k.viscosity, k.inv_viscosity);
+    return dist >= k.h ? 0.0F : l;
 }
 
 // viscosity Laplacian, density/device.cuh:72-76

@@ -91,6 +91,8 @@ KernelConstants kernel_constants(float h) {
     k.cubic = 8.0F / (kPi * h * h * h);
     k.viscosity = kPi * h * h * h * h * h * h;
     k.sqrt_support = sqrt_support_threshold(h);
+    k.inv_h = 1.0F / h;
+    k.inv_viscosity = 1.0F / k.viscosity;
     return k;
 }
 

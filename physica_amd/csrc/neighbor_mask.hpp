@@ -58,14 +58,15 @@ __device__ __forceinline__ uint32_t load_lane_column(const uint16_t* __restrict_
 
 // Marked candidates of one lane in the reference order, one candidate per
 // loop trip; `nonzero` lists the slots to visit, `center` is the lane's cell.
+// visit() should not branch on its candidate: a wave then runs one trip per
+// candidate of its busiest lane with no other divergence.
 template <class Visit>
 __device__ __forceinline__ void walk_masked(const uint32_t* __restrict__ offsets, uint32_t center, const int* delta,
                                             const uint16_t* column, uint32_t stride, uint32_t nonzero, Visit&& visit) {
     uint32_t m = 0u;
     uint32_t base = 0u;
-    for (;;) {
+    while ((m | nonzero) != 0u) {
         if (m == 0u) {
-            if (nonzero == 0u) break;
             const uint32_t k = static_cast<uint32_t>(__builtin_ctz(nonzero));
             nonzero &= nonzero - 1u;
             m = column[k * stride];
@@ -75,6 +76,66 @@ __device__ __forceinline__ void walk_masked(const uint32_t* __restrict__ offsets
         m &= m - 1u;
         visit(base + b);
     }
+}
+
+// walk_masked with the memory latency of each step taken off the critical
+// path: the base index of the next non-empty slot is fetched when the
+// current slot starts, and the data of the next candidate (load(s), which
+// returns a value) is fetched before the current one is processed
+// (process(value)), in the same order as walk_masked visits them. load()
+// must be safe to call with any s < n (lanes without a next candidate load
+// `fallback`, a valid index, and discard it).
+template <class Load, class Process>
+__device__ __forceinline__ void walk_masked_prefetch(const uint32_t* __restrict__ offsets, uint32_t center,
+                                                     const int* delta, const uint16_t* column, uint32_t stride,
+                                                     uint32_t nonzero, uint32_t fallback, Load&& load,
+                                                     Process&& process) {
+    auto fetch = [&](uint32_t& mask, uint32_t& base) {
+        mask = 0u;
+        if (nonzero != 0u) {
+            const uint32_t k = static_cast<uint32_t>(__builtin_ctz(nonzero));
+            nonzero &= nonzero - 1u;
+            mask = column[k * stride];
+            base = offsets[center + static_cast<uint32_t>(delta[k])];
+        }
+    };
+    uint32_t m = 0u, base = 0u, m_next = 0u, base_next = 0u;
+    fetch(m, base);
+    fetch(m_next, base_next);
+    // the next candidate in the reference order, or fallback when none is left
+    auto next = [&](bool& have) -> uint32_t {
+        if (m == 0u) {
+            m = m_next;
+            base = base_next;
+            fetch(m_next, base_next);
+        }
+        have = m != 0u;
+        const uint32_t s = base + static_cast<uint32_t>(__builtin_ctz(m));
+        m &= m - 1u;
+        return have ? s : fallback;
+    };
+    bool have;
+    auto data = load(next(have));
+    while (have) {
+        const auto current = data;
+        data = load(next(have));
+        // keeps the next candidate's loads issued ahead of this one's
+        // arithmetic (the compiler would otherwise sink them to their use)
+        asm volatile("" ::: "memory");
+        process(current);
+    }
+}
+
+// Clears particle i's own bit from its LDS column (slot 13, its own cell,
+// whose first sorted particle is cell_first) for the sums that skip j == i;
+// returns the bitmap of non-empty slots without slot 13 if that empties it.
+__device__ __forceinline__ uint32_t drop_self(uint32_t i, uint32_t cell_first, uint16_t* column, uint32_t stride,
+                                              uint32_t nonzero) {
+    constexpr uint32_t kCenter = 13u;
+    const uint32_t bit = i - cell_first;
+    const uint32_t m = column[kCenter * stride] & ~(bit < kMaskBits ? 1u << bit : 0u);
+    column[kCenter * stride] = static_cast<uint16_t>(m);
+    return m == 0u ? nonzero & ~(1u << kCenter) : nonzero;
 }
 
 // Workgroup index with consecutive ranges of indices on one XCD (MI355X

@@ -68,7 +68,7 @@ __global__ void __launch_bounds__(kBlock) wcsph_force_kernel(WcsphForceArgs a) {
     const float c_i = own.z;
     const float sigma_i = own.w;
     const float softening = 0.01F * h * h;
-    const float contact_weight = cubic_value(length(Float3{a.diameter, 0.0F, 0.0F}), a.kc);
+    const float contact_weight = cubic_value(pair_distance(Float3{a.diameter, 0.0F, 0.0F}), a.kc);
     Float3 ap{0.0F, 0.0F, 0.0F};
     Float3 av{0.0F, 0.0F, 0.0F};
     Float3 as{0.0F, 0.0F, 0.0F};
@@ -77,7 +77,7 @@ __global__ void __launch_bounds__(kBlock) wcsph_force_kernel(WcsphForceArgs a) {
         const float4 q = a.xm[s];
         const float ms = q.w;
         const Float3 d = sub(p, xyz(q));
-        const float dist = length(d);
+        const float dist = pair_distance(d);
         const Float3 gradient = scale(d, cubic_gradient_factor(dist, a.kc));
         const float4 other = a.aux[s];
         ap = add(ap, scale(gradient, -ms * (t_i + other.y)));

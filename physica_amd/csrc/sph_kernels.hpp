@@ -65,14 +65,14 @@ __global__ void __launch_bounds__(kBlock) mask_density_kernel(MaskDensityArgs a)
     if (complete) {
         walk_masked(a.grid.offsets, cell_of(a.grid, cx, cy, cz), delta, column, kBlock, nonzero, [&](uint32_t s) {
             const float4 q = a.xm[s];
-            const float dist = length(sub(p, xyz(q)));
+            const float dist = pair_distance(sub(p, xyz(q)));
             result += q.w * cubic_value(dist, a.kc);
         });
     } else {
         atomicOr(&a.flags[kFlagOverflow], 1u);
         for_each_candidate(a.grid, cx, cy, cz, [&](uint32_t s) {
             const float4 q = a.xm[s];
-            result += q.w * cubic_value(length(sub(p, xyz(q))), a.kc);
+            result += q.w * cubic_value(pair_distance(sub(p, xyz(q))), a.kc);
         });
     }
     a.rho[i] = result;
@@ -135,12 +135,15 @@ __global__ void __launch_bounds__(kBlock) projection_kernel(ProjectionArgs a) {
     float rs = 0.0F;
     auto visit = [&](uint32_t s) {
         const float4 q = a.xs[s];
-        rs += q.w * cubic_value(length(sub(ps, xyz(q))), a.kc);
+        rs += q.w * cubic_value(pair_distance(sub(ps, xyz(q))), a.kc);
     };
     if (masked) {
         uint16_t* column = smask + threadIdx.x;
         const uint32_t nonzero = load_lane_column(a.skin, a.n, i, column, kBlock);
-        walk_masked(a.grid.offsets, cell_of(a.grid, cx, cy, cz), delta, column, kBlock, nonzero, visit);
+        walk_masked_prefetch(
+            a.grid.offsets, cell_of(a.grid, cx, cy, cz), delta, column, kBlock, nonzero, i,
+            [&](uint32_t s) { return a.xs[s]; },
+            [&](float4 q) { rs += q.w * cubic_value(pair_distance(sub(ps, xyz(q))), a.kc); });
     } else {
         for_each_candidate(a.grid, cx, cy, cz, visit);
     }
@@ -221,7 +224,7 @@ __global__ void __launch_bounds__(kBlock) force_kernel(ForceArgs a) {
         Float3 ap2{0.0F, 0.0F, 0.0F};
         int cx, cy, cz;
         particle_cell(a.grid, p, cx, cy, cz);
-        auto pair = [&](uint32_t s, float4 q, Float3 d, float f) {
+        auto pair = [&](uint32_t s, float4 q, float t1_s, float t2_s, Float3 d, float f) {
             const float ms = q.w;
             if constexpr (kNonPressure) {
                 const float4 vq = a.vmu[s];
@@ -231,26 +234,44 @@ __global__ void __launch_bounds__(kBlock) force_kernel(ForceArgs a) {
             }
             if constexpr (kPressure) {
                 const Float3 gradient = scale(d, f);
-                ap1 = add(ap1, scale(gradient, -ms * (t1_i + a.t1[s])));
-                if constexpr (kTwo) ap2 = add(ap2, scale(gradient, -ms * (t2_i + a.t2[s])));
+                ap1 = add(ap1, scale(gradient, -ms * (t1_i + t1_s)));
+                if constexpr (kTwo) ap2 = add(ap2, scale(gradient, -ms * (t2_i + t2_s)));
             }
         };
         if (masked) {
             uint16_t* column = smask + threadIdx.x;
-            const uint32_t nonzero = load_lane_column(a.exact, a.n, i, column, kBlock);
-            walk_masked(a.grid.offsets, cell_of(a.grid, cx, cy, cz), delta, column, kBlock, nonzero,
-                        [&](uint32_t s) {
-                            if (s == i) return;
-                            const float4 q = a.xm[s];
-                            const Float3 d = sub(p, xyz(q));
-                            pair(s, q, d, kPressure ? cubic_gradient_factor(length(d), a.kc) : 0.0F);
-                        });
+            const uint32_t center = cell_of(a.grid, cx, cy, cz);
+            uint32_t nonzero = load_lane_column(a.exact, a.n, i, column, kBlock);
+            nonzero = drop_self(i, a.grid.offsets[center], column, kBlock, nonzero);
+            if constexpr (kNonPressure) {
+                walk_masked(a.grid.offsets, center, delta, column, kBlock, nonzero, [&](uint32_t s) {
+                    const float4 q = a.xm[s];
+                    const float t1_s = kPressure ? a.t1[s] : 0.0F;
+                    const float t2_s = kTwo ? a.t2[s] : 0.0F;
+                    const Float3 d = sub(p, xyz(q));
+                    pair(s, q, t1_s, t2_s, d, kPressure ? cubic_gradient_factor(pair_distance(d), a.kc) : 0.0F);
+                });
+            } else {
+                struct Candidate {
+                    float4 q;
+                    float t1, t2;
+                };
+                walk_masked_prefetch(
+                    a.grid.offsets, center, delta, column, kBlock, nonzero, i,
+                    [&](uint32_t s) { return Candidate{a.xm[s], a.t1[s], kTwo ? a.t2[s] : 0.0F}; },
+                    [&](const Candidate& c) {
+                        const Float3 d = sub(p, xyz(c.q));
+                        pair(0u, c.q, c.t1, c.t2, d, cubic_gradient_factor(pair_distance(d), a.kc));
+                    });
+            }
         } else {
             for_each_candidate(a.grid, cx, cy, cz, [&](uint32_t s) {
                 if (s == i) return;
                 const float4 q = a.xm[s];
+                const float t1_s = kPressure ? a.t1[s] : 0.0F;
+                const float t2_s = kTwo ? a.t2[s] : 0.0F;
                 const Float3 d = sub(p, xyz(q));
-                pair(s, q, d, kPressure ? cubic_gradient_factor(length(d), a.kc) : 0.0F);
+                pair(s, q, t1_s, t2_s, d, kPressure ? cubic_gradient_factor(pair_distance(d), a.kc) : 0.0F);
             });
         }
         if constexpr (kNonPressure) {
