@@ -369,13 +369,34 @@ def _pointers(arrays):
     return (c_void_p * len(arrays))(*(a.ctypes.data for a in arrays))
 
 
+def dfsph_record_step(domain, step_index, positions, velocities, controls, params: Dict[str, np.ndarray],
+                      warm_divergence, warm_density, divergence_iterations=4, density_iterations=6, warm_start=True,
+                      volume_rest=False, precision="f32"):
+    """The primal step that dfsph_jvp / dfsph_vjp differentiate (next
+    positions and velocities), with the same options."""
+    n = domain.particle_count
+    pos, pp = _vec(positions, precision)
+    vel, vp = _vec(velocities, precision)
+    ctl, cp = _vec(controls, precision)
+    prm = [_arr(params[k], precision) for k in DFSPH_PARAMETERS]
+    wd, we = _arr(warm_divergence, precision), _arr(warm_density, precision)
+    nx, nxp = _out_vec(n, precision)
+    nv, nvp = _out_vec(n, precision)
+    getattr(library(), "oracle_dfsph_record_step_" + precision)(
+        ctypes.byref(domain), c_uint32(divergence_iterations), c_uint32(density_iterations),
+        c_int(1 if warm_start else 0), c_int(1 if volume_rest else 0), c_uint64(step_index), pp, vp, cp,
+        _pointers(prm), c_void_p(wd.ctypes.data), c_void_p(we.ctypes.data), nxp, nvp)
+    return {"positions": nx, "velocities": nv}
+
+
 def dfsph_jvp(domain, step_index, positions, velocities, controls, params: Dict[str, np.ndarray],
               warm_divergence, warm_density, tangents: Dict[str, np.ndarray], divergence_iterations=4,
-              density_iterations=6, warm_start=True, precision="f32"):
+              density_iterations=6, warm_start=True, precision="f32", volume_rest=False):
     """SPH<DFSPH>::jvp_step: tangents of the next positions, velocities and
     warm-start pressures. `tangents` holds "positions", "velocities",
     "controls" (3, N), any of DFSPH_PARAMETERS and DFSPH_WARM (missing ones
-    are zero)."""
+    are zero). volume_rest takes V = m / rho0 in the pressure update (the
+    PCISPH step, see pressure_solver_jvp)."""
     n = domain.particle_count
     pos, pp = _vec(positions, precision)
     vel, vp = _vec(velocities, precision)
@@ -391,7 +412,8 @@ def dfsph_jvp(domain, step_index, positions, velocities, controls, params: Dict[
     nw = [np.zeros(n, dtype=_dtype(precision)) for _ in DFSPH_WARM]
     getattr(library(), "oracle_dfsph_jvp_" + precision)(
         ctypes.byref(domain), c_uint32(divergence_iterations), c_uint32(density_iterations),
-        c_int(1 if warm_start else 0), c_uint64(step_index), pp, vp, cp, _pointers(prm),
+        c_int(1 if warm_start else 0), c_int(1 if volume_rest else 0), c_uint64(step_index), pp, vp, cp,
+        _pointers(prm),
         c_void_p(wd.ctypes.data), c_void_p(we.ctypes.data), *(t[1] for t in tv), _pointers(prm_t),
         c_void_p(wt[0].ctypes.data), c_void_p(wt[1].ctypes.data), nxp, nvp, c_void_p(nw[0].ctypes.data),
         c_void_p(nw[1].ctypes.data))
@@ -400,7 +422,7 @@ def dfsph_jvp(domain, step_index, positions, velocities, controls, params: Dict[
 
 def dfsph_vjp(domain, step_index, positions, velocities, controls, params: Dict[str, np.ndarray],
               warm_divergence, warm_density, next_adjoints: Dict[str, np.ndarray], divergence_iterations=4,
-              density_iterations=6, warm_start=True, precision="f32"):
+              density_iterations=6, warm_start=True, precision="f32", volume_rest=False):
     """SPH<DFSPH>::vjp_step with FP64 adjoints. `next_adjoints` holds
     "positions", "velocities" (3, N) and optionally DFSPH_WARM entries.
     Returns the adjoints of the state, the control, every DFSPH_PARAMETERS
@@ -418,7 +440,8 @@ def dfsph_vjp(domain, step_index, positions, velocities, controls, params: Dict[
     scal = {k: np.zeros(n, dtype=np.float64) for k in DFSPH_PARAMETERS + DFSPH_WARM}
     getattr(library(), "oracle_dfsph_vjp_" + precision)(
         ctypes.byref(domain), c_uint32(divergence_iterations), c_uint32(density_iterations),
-        c_int(1 if warm_start else 0), c_uint64(step_index), pp, vp, cp, _pointers(prm),
+        c_int(1 if warm_start else 0), c_int(1 if volume_rest else 0), c_uint64(step_index), pp, vp, cp,
+        _pointers(prm),
         c_void_p(wd.ctypes.data), c_void_p(we.ctypes.data), nxap, nvap, c_void_p(nwa[0].ctypes.data),
         c_void_p(nwa[1].ctypes.data), *(outs[k][1] for k in ("positions", "velocities", "controls")),
         _pointers([scal[k] for k in DFSPH_PARAMETERS]), c_void_p(scal[DFSPH_WARM[0]].ctypes.data),
@@ -426,6 +449,55 @@ def dfsph_vjp(domain, step_index, positions, velocities, controls, params: Dict[
     result = {k: outs[k][0] for k in outs}
     result.update(scal)
     return result
+
+
+# SPH<PCISPH> / SPH<IISPH> parameters (sph/sph.ixx:112, 158) in the order of
+# their tangents and adjoints; "relaxation" is the pressure (PCISPH) or Jacobi
+# (IISPH) relaxation.
+PRESSURE_SOLVER_PARAMETERS = ("masses", "rest_densities", "viscosities", "relaxation")
+
+
+def _as_dfsph(params, n):
+    """PCISPH/IISPH parameters as the DFSPH parameters of the same step: no
+    divergence phase, the relaxation in the density phase."""
+    zero = np.zeros(n)
+    return {"masses": params.get("masses", zero), "rest_densities": params.get("rest_densities", zero),
+            "viscosities": params.get("viscosities", zero), "divergence_relaxation": zero,
+            "density_relaxation": params.get("relaxation", zero)}
+
+
+def pressure_solver_jvp(domain, step_index, positions, velocities, controls, params, tangents, method="pcisph",
+                        pressure_iterations=6, precision="f32"):
+    """SPH<PCISPH> / SPH<IISPH>::jvp_step (sph/sph.cpp:312-331, 423-442): the
+    DFSPH derivative with no divergence phase and no warm start, and V = m /
+    rho0 for PCISPH. `tangents` holds "positions", "velocities", "controls" and
+    any of PRESSURE_SOLVER_PARAMETERS."""
+    if method not in ("pcisph", "iisph"):
+        raise ValueError(method)
+    n = domain.particle_count
+    zero = np.zeros(n)
+    t = dict(_as_dfsph({k: tangents[k] for k in PRESSURE_SOLVER_PARAMETERS if k in tangents}, n))
+    t.update({k: tangents[k] for k in ("positions", "velocities", "controls") if k in tangents})
+    r = dfsph_jvp(domain, step_index, positions, velocities, controls, _as_dfsph(params, n), zero, zero, t,
+                  divergence_iterations=0, density_iterations=pressure_iterations, warm_start=False,
+                  precision=precision, volume_rest=method == "pcisph")
+    return {"positions": r["positions"], "velocities": r["velocities"]}
+
+
+def pressure_solver_vjp(domain, step_index, positions, velocities, controls, params, next_adjoints,
+                        method="pcisph", pressure_iterations=6, precision="f32"):
+    """SPH<PCISPH> / SPH<IISPH>::vjp_step (sph/sph.cpp:333-363, 444-476) with
+    FP64 adjoints: the state, control and PRESSURE_SOLVER_PARAMETERS adjoints."""
+    if method not in ("pcisph", "iisph"):
+        raise ValueError(method)
+    n = domain.particle_count
+    zero = np.zeros(n)
+    r = dfsph_vjp(domain, step_index, positions, velocities, controls, _as_dfsph(params, n), zero, zero,
+                  next_adjoints, divergence_iterations=0, density_iterations=pressure_iterations, warm_start=False,
+                  precision=precision, volume_rest=method == "pcisph")
+    out = {k: r[k] for k in ("positions", "velocities", "controls", "masses", "rest_densities", "viscosities")}
+    out["relaxation"] = r["density_relaxation"]
+    return out
 
 
 def set_reference_no_slip_adjoint(enabled: bool) -> None:

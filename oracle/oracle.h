@@ -133,9 +133,11 @@ void oracle_set_threads(int threads);
                              double* exponent_adjoint, double* radius_adjoint, double* xsph_adjoint,                    \
                              double* confinement_adjoint);                                                              \
     /* SPH<DFSPH>::jvp_step (sph.ixx:356-359, sph.cpp:612-659). parameters: masses, rest densities, viscosities,      \
-     * divergence relaxation, density relaxation */                                                                     \
+     * divergence relaxation, density relaxation. volume_rest = 1 takes V = m / rho0 in the pressure update: with     \
+     * divergence_iterations = 0 and no warm start the step is PCISPH's (volume_rest = 1, sph.cpp:312-331) or        \
+     * IISPH's (volume_rest = 0), the density relaxation being their pressure / Jacobi relaxation */                   \
     void oracle_dfsph_jvp##suf(const oracle_domain* domain, uint32_t divergence_iterations,                           \
-                               uint32_t density_iterations, int warm_start, uint64_t step_index,                        \
+                               uint32_t density_iterations, int warm_start, int volume_rest, uint64_t step_index,       \
                                const real* positions[3], const real* velocities[3], const real* controls[3],           \
                                const real* const parameters[5], const real* warm_divergence,                            \
                                const real* warm_density, const real* position_tangent[3],                               \
@@ -147,7 +149,7 @@ void oracle_set_threads(int threads);
     /* SPH<DFSPH>::vjp_step (sph.ixx:361-365, sph.cpp:661-723): FP64 adjoints accumulated into the outputs, warm     \
      * pressure adjoints copied when warm_start */                                                                      \
     void oracle_dfsph_vjp##suf(const oracle_domain* domain, uint32_t divergence_iterations,                           \
-                               uint32_t density_iterations, int warm_start, uint64_t step_index,                        \
+                               uint32_t density_iterations, int warm_start, int volume_rest, uint64_t step_index,       \
                                const real* positions[3], const real* velocities[3], const real* controls[3],           \
                                const real* const parameters[5], const real* warm_divergence,                            \
                                const real* warm_density, const double* next_position_adjoint[3],                        \
@@ -156,6 +158,13 @@ void oracle_set_threads(int threads);
                                double* velocity_adjoint[3], double* control_adjoint[3],                                 \
                                double* const parameter_adjoint[5], double* warm_divergence_adjoint,                     \
                                double* warm_density_adjoint);                                                           \
+    /* the primal step the two functions above differentiate: next positions and velocities */                        \
+    void oracle_dfsph_record_step##suf(const oracle_domain* domain, uint32_t divergence_iterations,                   \
+                                       uint32_t density_iterations, int warm_start, int volume_rest,                    \
+                                       uint64_t step_index, const real* positions[3], const real* velocities[3],      \
+                                       const real* controls[3], const real* const parameters[5],                       \
+                                       const real* warm_divergence, const real* warm_density,                          \
+                                       real* next_positions[3], real* next_velocities[3]);                             \
     /* 1: restate the reference's VJP of a no-slip collision in the SPH prediction (sph/dfsph.cu:81); 0 (default): \
      * the exact transpose of the prediction's JVP */                                                                   \
     void oracle_set_reference_no_slip_adjoint##suf(int enabled);                                                        \

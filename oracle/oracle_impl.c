@@ -1776,6 +1776,9 @@ enum { DF_MASS = 0, DF_REST, DF_VISC, DF_DIV, DF_DEN, DF_PARAMS };
 typedef struct {
     uint32_t n, iters[2];
     int warm;
+    /* density of the volume V = m / density in the pressure update: the state
+     * density (DFSPH, IISPH) or, when set, the rest density (PCISPH) */
+    int volume_rest;
     grid_t g;
     real h, dt, G;
     cbox box;
@@ -1790,16 +1793,22 @@ typedef struct {
     real** rs[2];
 } dfsph_record;
 
-/* DFSPH::forward (sph.cpp:572-610) keeping every iterate. */
+/* DFSPH::forward (sph.cpp:572-610) keeping every iterate. With nd = 0, no
+ * warm start and the density-phase relaxation this is also PCISPH::forward
+ * (sph.cpp:295-309, volume_rest set: V = m / rho0, pcisph.cu:104-110) and
+ * IISPH::forward (sph.cpp:408-421, V = m / rho, iisph.cu:104-110): the
+ * divergence phase then only adds a zero acceleration (+0 + a = a) and the
+ * pressures start from zero (clear_iteration, sph.cpp:298, 411). */
 static void dfsph_record_forward(dfsph_record* r, const oracle_domain* d, uint32_t nd, uint32_t ne, int warm,
-                                 uint64_t step, const real* pos[3], const real* vel[3], const real* ctl[3],
-                                 const real* const prm[DF_PARAMS], const real* const warm_p[2]) {
+                                 int volume_rest, uint64_t step, const real* pos[3], const real* vel[3],
+                                 const real* ctl[3], const real* const prm[DF_PARAMS], const real* const warm_p[2]) {
     const uint32_t n = d->particle_count;
     memset(r, 0, sizeof(*r));
     r->n = n;
     r->iters[0] = nd;
     r->iters[1] = ne;
     r->warm = warm;
+    r->volume_rest = volume_rest;
     r->h = (real)d->support_radius;
     r->dt = (real)d->time_step;
     r->G = SUF(oracle_reference_gradient_norm)(d);
@@ -1839,8 +1848,8 @@ static void dfsph_record_forward(dfsph_record* r, const oracle_domain* d, uint32
             density_kernel(&r->g, 0, pos, CV(r->xs[q][k]), prm[DF_MASS], prm[DF_REST], r->rs[q][k]);
             r->p[q][k] = alloc_real(n);
             memcpy(r->p[q][k], r->p[q][k - 1u], sizeof(real) * n);
-            projection_update_kernel(n, r->dt, r->G, prm[DF_MASS], r->rho, q ? prm[DF_REST] : r->rho, r->rs[q][k],
-                                     prm[q ? DF_DEN : DF_DIV], r->p[q][k]);
+            projection_update_kernel(n, r->dt, r->G, prm[DF_MASS], volume_rest ? prm[DF_REST] : r->rho,
+                                     q ? prm[DF_REST] : r->rho, r->rs[q][k], prm[q ? DF_DEN : DF_DIV], r->p[q][k]);
             r->a[q][k] = alloc_vfield(n);
             pressure_kernel(&r->g, pos, prm[DF_MASS], prm[DF_REST], r->rho, r->p[q][k], r->a[q][k].c);
         }
@@ -2075,15 +2084,15 @@ static void projection_update_jvp_kernel(uint32_t n, real dt, real G, const real
     }
 }
 
-void SUF(oracle_dfsph_jvp)(const oracle_domain* d, uint32_t nd, uint32_t ne, int warm_start, uint64_t step,
-                           const real* pos[3], const real* vel[3], const real* ctl[3],
+void SUF(oracle_dfsph_jvp)(const oracle_domain* d, uint32_t nd, uint32_t ne, int warm_start, int volume_rest,
+                           uint64_t step, const real* pos[3], const real* vel[3], const real* ctl[3],
                            const real* const prm[DF_PARAMS], const real* warm_div, const real* warm_den,
                            const real* pos_t[3], const real* vel_t[3], const real* ctl_t[3],
                            const real* const prm_t[DF_PARAMS], const real* warm_div_t, const real* warm_den_t,
                            real* next_pos_t[3], real* next_vel_t[3], real* next_warm_div_t, real* next_warm_den_t) {
     dfsph_record r;
     const real* const warm_p[2] = {warm_div, warm_den};
-    dfsph_record_forward(&r, d, nd, ne, warm_start, step, pos, vel, ctl, prm, warm_p);
+    dfsph_record_forward(&r, d, nd, ne, warm_start, volume_rest, step, pos, vel, ctl, prm, warm_p);
     const uint32_t n = r.n;
     const grid_t* g = &r.g;
     const real *m = prm[DF_MASS], *rest = prm[DF_REST], *m_t = prm_t[DF_MASS], *rest_t = prm_t[DF_REST];
@@ -2112,7 +2121,8 @@ void SUF(oracle_dfsph_jvp)(const oracle_domain* d, uint32_t nd, uint32_t ne, int
             }
             predict_jvp_kernel(n, r.dt, r.box, pos, vel, CV(r.anp), a, pos_t, vel_t, CV(anp_t), at, xs_t.c, NULL);
             density_jvp_kernel(g, pos, CV(r.xs[q][k]), CV(xs_t), m, m_t, rest, rest_t, rs_t);
-            projection_update_jvp_kernel(n, r.dt, r.G, m, m_t, r.rho, rho_t, q ? rest : r.rho, q ? rest_t : rho_t,
+            projection_update_jvp_kernel(n, r.dt, r.G, m, m_t, volume_rest ? rest : r.rho,
+                                         volume_rest ? rest_t : rho_t, q ? rest : r.rho, q ? rest_t : rho_t,
                                          r.p[q][k - 1u], r.rs[q][k], prm[q ? DF_DEN : DF_DIV], p_t, rs_t,
                                          prm_t[q ? DF_DEN : DF_DIV], p_t);
             pressure_jvp_kernel(g, pos, pos_t, m, m_t, rest, rest_t, r.rho, rho_t, r.p[q][k], p_t, ap_t.c);
@@ -2133,6 +2143,20 @@ void SUF(oracle_dfsph_jvp)(const oracle_domain* d, uint32_t nd, uint32_t ne, int
     free_vfield(&tot_t);
     free_vfield(&xs_t);
     free_vfield(&acc);
+    dfsph_record_free(&r);
+}
+
+/* The primal of the recorded step, next positions and velocities (the last
+ * prediction, sph.cpp:609): pins dfsph_record_forward, with the options the
+ * derivatives use, to the forward steps of DFSPH, PCISPH and IISPH. */
+void SUF(oracle_dfsph_record_step)(const oracle_domain* d, uint32_t nd, uint32_t ne, int warm_start, int volume_rest,
+                                   uint64_t step, const real* pos[3], const real* vel[3], const real* ctl[3],
+                                   const real* const prm[DF_PARAMS], const real* warm_div, const real* warm_den,
+                                   real* next_pos[3], real* next_vel[3]) {
+    dfsph_record r;
+    const real* const warm_p[2] = {warm_div, warm_den};
+    dfsph_record_forward(&r, d, nd, ne, warm_start, volume_rest, step, pos, vel, ctl, prm, warm_p);
+    sph_predict_kernel(r.n, r.dt, r.box, pos, vel, CV(r.anp), CV(r.total), next_pos, next_vel);
     dfsph_record_free(&r);
 }
 
@@ -2380,15 +2404,15 @@ static void dfield_clear(dfield* f, uint32_t n) {
     for (int c = 0; c < 3; ++c) memset(f->c[c], 0, sizeof(double) * n);
 }
 
-void SUF(oracle_dfsph_vjp)(const oracle_domain* d, uint32_t nd, uint32_t ne, int warm_start, uint64_t step,
-                           const real* pos[3], const real* vel[3], const real* ctl[3],
+void SUF(oracle_dfsph_vjp)(const oracle_domain* d, uint32_t nd, uint32_t ne, int warm_start, int volume_rest,
+                           uint64_t step, const real* pos[3], const real* vel[3], const real* ctl[3],
                            const real* const prm[DF_PARAMS], const real* warm_div, const real* warm_den,
                            const double* next_pos_a[3], const double* next_vel_a[3], const double* next_warm_div_a,
                            const double* next_warm_den_a, double* pos_a[3], double* vel_a[3], double* ctl_a[3],
                            double* const prm_a[DF_PARAMS], double* warm_div_a, double* warm_den_a) {
     dfsph_record r;
     const real* const warm_p[2] = {warm_div, warm_den};
-    dfsph_record_forward(&r, d, nd, ne, warm_start, step, pos, vel, ctl, prm, warm_p);
+    dfsph_record_forward(&r, d, nd, ne, warm_start, volume_rest, step, pos, vel, ctl, prm, warm_p);
     const uint32_t n = r.n;
     const grid_t* g = &r.g;
     const real *m = prm[DF_MASS], *rest = prm[DF_REST];
@@ -2420,9 +2444,10 @@ void SUF(oracle_dfsph_vjp)(const oracle_domain* d, uint32_t nd, uint32_t ne, int
             dfield_clear(&prev_ap, n);
             dfield_clear(&xs_a, n);
             pressure_vjp_kernel(g, pos, m, rest, r.rho, r.p[q][k], DV(cur_ap), pos_a, rho_a, cur_p, m_a, rest_a);
-            projection_update_vjp_kernel(n, r.dt, r.G, m, r.rho, q ? rest : r.rho, r.p[q][k - 1u], r.rs[q][k],
-                                         prm[q ? DF_DEN : DF_DIV], cur_p, m_a, rho_a, q ? rest_a : target_a, prev_p,
-                                         rs_a, prm_a[q ? DF_DEN : DF_DIV]);
+            projection_update_vjp_kernel(n, r.dt, r.G, m, volume_rest ? rest : r.rho, q ? rest : r.rho,
+                                         r.p[q][k - 1u], r.rs[q][k], prm[q ? DF_DEN : DF_DIV], cur_p, m_a,
+                                         volume_rest ? rest_a : rho_a, q ? rest_a : target_a, prev_p, rs_a,
+                                         prm_a[q ? DF_DEN : DF_DIV]);
             density_vjp_kernel(g, pos, CV(r.xs[q][k]), m, rest, rs_a, xs_a.c, m_a, rest_a);
             if (q == 0) {
                 predict_vjp_kernel(n, r.dt, r.box, pos, vel, CV(r.anp), CV(r.a[0][k - 1u]), DV(xs_a), NULL, pos_a,
