@@ -419,6 +419,65 @@ phx_status phx_sph_cache_densities(const phx_sph_step_cache* cache, float* densi
 phx_status phx_sph_cache_pressures(const phx_sph_step_cache* cache, float* pressures);
 phx_status phx_sph_cache_non_pressure_accelerations(const phx_sph_step_cache* cache, phx_vector_view out);
 
+/* StateTangent / StateAdjoint of the three methods are empty and their
+ * ParameterTangent / ParameterAdjoint mirror Parameters (sph/sph.ixx:42-60,
+ * 108-113, 154-159); SPH<Method> wrappers, sph/sph.ixx:279-298. Only the
+ * arrays of the solver's method are read or written; the others may be null. */
+typedef struct phx_sph_state_tangent {
+    phx_particle_state_tangent particles;
+} phx_sph_state_tangent;
+
+typedef struct phx_sph_state_adjoint {
+    phx_particle_state_adjoint particles;
+} phx_sph_state_adjoint;
+
+typedef struct phx_sph_method_parameter_tangent {
+    const float* speed_of_sound;
+    const float* tait_exponent;
+    const float* boundary_surface_tension;
+    const float* pressure_relaxation;
+    const float* jacobi_relaxation;
+} phx_sph_method_parameter_tangent;
+
+typedef struct phx_sph_method_parameter_adjoint {
+    double* speed_of_sound;
+    double* tait_exponent;
+    double* boundary_surface_tension;
+    double* pressure_relaxation;
+    double* jacobi_relaxation;
+} phx_sph_method_parameter_adjoint;
+
+typedef struct phx_sph_parameter_tangent {
+    phx_particle_parameter_tangent particles;
+    phx_sph_method_parameter_tangent method;
+} phx_sph_parameter_tangent;
+
+typedef struct phx_sph_parameter_adjoint {
+    phx_particle_parameter_adjoint particles;
+    phx_sph_method_parameter_adjoint method;
+} phx_sph_parameter_adjoint;
+
+/* SPH<PCISPH|IISPH>::jvp_step (sph/sph.ixx:356-359; PCISPH::jvp
+ * sph/sph.cpp:312-331, IISPH::jvp sph/sph.cpp:423-442). Needs a solver
+ * created in PHX_MODE_DIFFERENTIABLE (PCISPH or IISPH; WCSPH derivatives
+ * return PHX_ERROR_UNSUPPORTED) and the cache of that step's forward_step,
+ * which records the step. FP32 tangents overwrite next_state_tangent; null
+ * parameter-tangent arrays count as zero. */
+phx_status phx_sph_jvp_step(phx_sph* solver, const phx_sph_step_cache* cache,
+                            const phx_sph_state_tangent* state_tangent, const phx_control_tangent* control_tangent,
+                            const phx_sph_parameter_tangent* parameter_tangent,
+                            phx_sph_state_tangent* next_state_tangent);
+/* SPH<PCISPH|IISPH>::vjp_step (sph/sph.ixx:361-365; PCISPH::vjp
+ * sph/sph.cpp:333-363, IISPH::vjp sph/sph.cpp:444-476): FP64 adjoints of the
+ * recorded step (neighbour graph, collision masks and the active sets of the
+ * max(0, .) updates frozen), accumulated (+=) into the outputs; null outputs
+ * are skipped. The prediction's VJP under a no-slip collision is the exact
+ * transpose of its JVP (see phx_dfsph_vjp_step). */
+phx_status phx_sph_vjp_step(phx_sph* solver, const phx_sph_step_cache* cache,
+                            const phx_sph_state_adjoint* next_state_adjoint,
+                            phx_sph_state_adjoint* previous_state_adjoint, phx_control_adjoint* control_adjoint,
+                            phx_sph_parameter_adjoint* parameter_adjoint);
+
 /* ------------------------------------------------------------------- PBF */
 /* PBF::Configuration, pbf/pbf.ixx:15-18 */
 typedef struct phx_pbf_configuration {

@@ -226,6 +226,30 @@ class SphParameters(ctypes.Structure):
     _fields_ = [("particles", ParticleParameters), ("method", SphMethodParameters)]
 
 
+class SphStateTangent(ctypes.Structure):
+    _fields_ = [("particles", ParticleStateTangent)]
+
+
+class SphStateAdjoint(ctypes.Structure):
+    _fields_ = [("particles", ParticleStateAdjoint)]
+
+
+class SphMethodParameterTangent(ctypes.Structure):
+    _fields_ = SphMethodParameters._fields_
+
+
+class SphMethodParameterAdjoint(ctypes.Structure):
+    _fields_ = SphMethodParameters._fields_
+
+
+class SphParameterTangent(ctypes.Structure):
+    _fields_ = [("particles", ParticleParameterTangent), ("method", SphMethodParameterTangent)]
+
+
+class SphParameterAdjoint(ctypes.Structure):
+    _fields_ = [("particles", ParticleParameterAdjoint), ("method", SphMethodParameterAdjoint)]
+
+
 class PbfConfiguration(ctypes.Structure):
     _fields_ = [("pressure_iterations", c_uint32), ("checkpoint_interval", c_uint32)]
 
@@ -328,6 +352,16 @@ PROTOTYPES = {
     "phx_sph_cache_densities": (c_int, [c_void_p, c_void_p]),
     "phx_sph_cache_pressures": (c_int, [c_void_p, c_void_p]),
     "phx_sph_cache_non_pressure_accelerations": (c_int, [c_void_p, VectorView]),
+    "phx_sph_jvp_step": (
+        c_int,
+        [c_void_p, c_void_p, POINTER(SphStateTangent), POINTER(ControlTangent), POINTER(SphParameterTangent),
+         POINTER(SphStateTangent)],
+    ),
+    "phx_sph_vjp_step": (
+        c_int,
+        [c_void_p, c_void_p, POINTER(SphStateAdjoint), POINTER(SphStateAdjoint), POINTER(ControlAdjoint),
+         POINTER(SphParameterAdjoint)],
+    ),
     "phx_pbf_create": (
         c_int,
         [POINTER(DomainConfiguration), POINTER(PbfConfiguration), c_int, c_void_p, POINTER(c_void_p)],

@@ -452,7 +452,7 @@ phx_status phx_sph_cache_neighborhood(const phx_sph_step_cache* cache, uint64_t*
                                       uint32_t* sorted_particle_indices, uint32_t* cell_offsets) {
     return guarded([&] {
         phx::require(cache != nullptr && cache->valid, PHX_ERROR_INVALID_ARGUMENT, "cache holds no step");
-        phx::export_neighborhood(cache->impl.stream, cache->impl.neighborhood, sorted_keys, sorted_particle_indices,
+        phx::export_neighborhood(cache->impl.stream, cache->impl.grid(), sorted_keys, sorted_particle_indices,
                                  cell_offsets);
     });
 }
@@ -462,9 +462,9 @@ phx_status phx_sph_cache_densities(const phx_sph_step_cache* cache, float* densi
         phx::require(cache != nullptr && cache->valid, PHX_ERROR_INVALID_ARGUMENT, "cache holds no step");
         phx::require(densities != nullptr, PHX_ERROR_INVALID_ARGUMENT, "null output");
         phx::ChannelList ch;
-        ch.add(cache->impl.densities.data(), densities);
-        phx::scatter_to_original(cache->impl.stream, cache->impl.neighborhood.perm.data(),
-                                 cache->impl.neighborhood.particles, ch.get());
+        ch.add(cache->impl.density_data(), densities);
+        phx::scatter_to_original(cache->impl.stream, cache->impl.grid().perm.data(), cache->impl.grid().particles,
+                                 ch.get());
     });
 }
 
@@ -473,9 +473,9 @@ phx_status phx_sph_cache_pressures(const phx_sph_step_cache* cache, float* press
         phx::require(cache != nullptr && cache->valid, PHX_ERROR_INVALID_ARGUMENT, "cache holds no step");
         phx::require(pressures != nullptr, PHX_ERROR_INVALID_ARGUMENT, "null output");
         phx::ChannelList ch;
-        ch.add(cache->impl.pressures.data(), pressures);
-        phx::scatter_to_original(cache->impl.stream, cache->impl.neighborhood.perm.data(),
-                                 cache->impl.neighborhood.particles, ch.get());
+        ch.add(cache->impl.pressure_data(), pressures);
+        phx::scatter_to_original(cache->impl.stream, cache->impl.grid().perm.data(), cache->impl.grid().particles,
+                                 ch.get());
     });
 }
 
@@ -486,12 +486,41 @@ phx_status phx_sph_cache_non_pressure_accelerations(const phx_sph_step_cache* ca
                      "WCSPH keeps no non-pressure accelerations");
         phx::require_vec(out, "output");
         phx::ChannelList ch;
-        const float* np = reinterpret_cast<const float*>(cache->impl.non_pressure.data());
+        const float* np = reinterpret_cast<const float*>(cache->impl.non_pressure_data());
         ch.add(np + 0, out.x, 4u);
         ch.add(np + 1, out.y, 4u);
         ch.add(np + 2, out.z, 4u);
-        phx::scatter_to_original(cache->impl.stream, cache->impl.neighborhood.perm.data(),
-                                 cache->impl.neighborhood.particles, ch.get());
+        phx::scatter_to_original(cache->impl.stream, cache->impl.grid().perm.data(), cache->impl.grid().particles,
+                                 ch.get());
+    });
+}
+
+phx_status phx_sph_jvp_step(phx_sph* solver, const phx_sph_step_cache* cache,
+                            const phx_sph_state_tangent* state_tangent, const phx_control_tangent* control_tangent,
+                            const phx_sph_parameter_tangent* parameter_tangent,
+                            phx_sph_state_tangent* next_state_tangent) {
+    return guarded([&] {
+        phx::require(solver && cache && state_tangent && control_tangent && parameter_tangent && next_state_tangent,
+                     PHX_ERROR_INVALID_ARGUMENT, "null argument");
+        phx::require(cache->owner == solver && cache->valid, PHX_ERROR_INVALID_ARGUMENT,
+                     "step cache holds no step of this solver");
+        solver->impl.jvp_step(cache->impl, *state_tangent, *control_tangent, *parameter_tangent,
+                              *next_state_tangent);
+    });
+}
+
+phx_status phx_sph_vjp_step(phx_sph* solver, const phx_sph_step_cache* cache,
+                            const phx_sph_state_adjoint* next_state_adjoint,
+                            phx_sph_state_adjoint* previous_state_adjoint, phx_control_adjoint* control_adjoint,
+                            phx_sph_parameter_adjoint* parameter_adjoint) {
+    return guarded([&] {
+        phx::require(solver && cache && next_state_adjoint && previous_state_adjoint && control_adjoint &&
+                         parameter_adjoint,
+                     PHX_ERROR_INVALID_ARGUMENT, "null argument");
+        phx::require(cache->owner == solver && cache->valid, PHX_ERROR_INVALID_ARGUMENT,
+                     "step cache holds no step of this solver");
+        solver->impl.vjp_step(cache->impl, *next_state_adjoint, *previous_state_adjoint, *control_adjoint,
+                              *parameter_adjoint);
     });
 }
 

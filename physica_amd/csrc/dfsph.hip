@@ -247,7 +247,7 @@ void DfsphSolver::forward_step(const phx_sph_dfsph_state& state, const phx_contr
     pa.skin = skin;
     pa.flags = masks_.flags();
     pa.rho = cache.densities.data();
-    pa.volume_density = cache.densities.data();
+    pa.volume_density = volume_rest_ ? rho0 : cache.densities.data();
     // update `it` of phase `phase` reads the x* of prediction `slot`
     auto set_update = [&](uint32_t phase, uint32_t it, uint32_t slot) {
         pa.slot = slot;
@@ -305,6 +305,7 @@ void DfsphSolver::forward_step(const phx_sph_dfsph_state& state, const phx_contr
         cache.mask_limit2 = masks_.limit2();
         cache.step_index = step;
         cache.warm_start = warm;
+        cache.volume_rest = volume_rest_;
         cache.recorded = true;
     }
     // read the step's displacements back for the next step's skin

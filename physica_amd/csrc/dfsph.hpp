@@ -23,6 +23,7 @@ struct DfsphCache {
     uint32_t iterations[2] = {0u, 0u};  // divergence, density
     uint64_t step_index = 0;
     bool warm_start = false;
+    bool volume_rest = false;  // V = m / rho0 in the pressure update (PCISPH)
     DeviceBuffer<float4> xm, vmu;  // positions + masses, velocities + viscosities
     DeviceBuffer<float> rho0, relax_div, relax_den;
     // projection pass r = 1 .. nd + ne (divergence first): predicted positions
@@ -111,6 +112,10 @@ class DfsphSolver {
     phx_execution_mode mode() const { return mode_; }
     bool warm_start() const { return config_.pressure_warm_start != 0u; }
     float skin() const { return masks_.skin(); }
+    // Takes V = m / rho0 instead of m / rho in the density phase's pressure
+    // update: with no divergence iterations and no warm start the step is
+    // then PCISPH's (pcisph.cu:104-110), as it is IISPH's without.
+    void set_volume_rest_density(bool enabled) { volume_rest_ = enabled; }
 
   private:
     void require_recorded(const DfsphCache& cache) const;
@@ -118,6 +123,7 @@ class DfsphSolver {
     Domain domain_;
     phx_dfsph_configuration config_{};
     phx_execution_mode mode_;
+    bool volume_rest_ = false;
     hipStream_t stream_;
     float gradient_norm_ = 0.0F;
     KernelConstants constants_{};

@@ -388,6 +388,8 @@ struct ProjectionJvpArgs {
     const float4* xt;  // .w: mass tangents
     const float4* xs_t;
     const float* rho_t;
+    const float* volume;    // density of V = m / density: rho, or rho0 (PCISPH)
+    const float* volume_t;  // its tangent
     float* p_t;  // in: tangent of p_prev; out: tangent of p
     float2* t_out;
 };
@@ -419,9 +421,10 @@ __global__ void __launch_bounds__(kBlock) dfsph_jvp_projection_kernel(Projection
     const float rho = s.rho[i];
     const float rho_t = a.rho_t[i];
     const float error = a.rs[i] - a.target[i];
-    const float volume = m / rho;
+    const float density = a.volume[i];
+    const float volume = m / density;
     const float dp = 1.0F / (2.0F * s.dt * s.dt * volume * volume * s.gradient_norm);
-    const float dp_t = -2.0F * dp * (m_t / m - rho_t / rho);
+    const float dp_t = -2.0F * dp * (m_t / m - a.volume_t[i] / density);
     const float relax = a.relax[i];
     const float candidate = a.p_prev[i] + relax * dp * error;
     const float pt = candidate > 0.0F ? a.p_t[i] + a.relax_t[i] * dp * error + relax * dp_t * error +
@@ -486,6 +489,8 @@ struct PressureVjpArgs {
     double* rs_a;      // written
     double* target_a;  // accumulated: rest densities (density phase); null: the
                        // target is the state density (divergence phase)
+    const float* volume;  // density of V = m / density; null: the state density
+    double* volume_a;     // accumulated when volume is set (rest densities, PCISPH)
     double* relax_a;
     // warm-start adjoint of an initial pressure field (!kUpdate), copied into
     // original order; null: dropped
@@ -579,7 +584,8 @@ __global__ void __launch_bounds__(kBlock) dfsph_vjp_pressure_kernel(PressureVjpA
     }
     if constexpr (kUpdate) {
         const double error = static_cast<double>(a.rs[i]) - a.target[i];
-        const double volume = static_cast<double>(m_i) / rho_i;
+        const float density = a.volume != nullptr ? a.volume[i] : rho_i;
+        const double volume = static_cast<double>(m_i) / density;
         const double dp = 1.0 / (2.0 * s.dt * s.dt * volume * volume * s.gradient_norm);
         const float relax = a.relax[i];
         const double candidate = a.p_prev[i] + relax * dp * error;
@@ -592,7 +598,11 @@ __global__ void __launch_bounds__(kBlock) dfsph_vjp_pressure_kernel(PressureVjpA
                 rho_a -= relax * dp * p_a;
             }
             m_a -= 2.0 * relax * dp * error * p_a / m_i;
-            rho_a += 2.0 * relax * dp * error * p_a / rho_i;
+            if (a.volume_a != nullptr) {
+                a.volume_a[i] += 2.0 * relax * dp * error * p_a / density;
+            } else {
+                rho_a += 2.0 * relax * dp * error * p_a / rho_i;
+            }
             a.relax_a[i] += dp * error * p_a;
         } else {
             a.p_prev_a[i] = 0.0;
@@ -845,6 +855,8 @@ void DfsphSolver::jvp_step(const DfsphCache& cache, const phx_sph_dfsph_state_ta
     pa.xt = dw_.xt.data();
     pa.xs_t = dw_.xs_t.data();
     pa.rho_t = dw_.rho_t.data();
+    pa.volume = cache.volume_rest ? cache.rho0.data() : cache.densities.data();
+    pa.volume_t = cache.volume_rest ? dw_.rest_t.data() : dw_.rho_t.data();
     const size_t nn = n;
     uint32_t slot = 0;  // prediction index
     auto collisions = [&](uint32_t k) { return cache.collisions.data() + k * nn; };
@@ -969,6 +981,8 @@ void DfsphSolver::vjp_step(const DfsphCache& cache, const phx_sph_dfsph_state_ad
     pv.rho_a = dw_.rho_a.data();
     pv.m_a = dw_.m_a.data();
     pv.rs_a = dw_.rs_a.data();
+    pv.volume = cache.volume_rest ? cache.rho0.data() : nullptr;
+    pv.volume_a = cache.volume_rest ? dw_.rest_a.data() : nullptr;
     DensityVjpArgs dv{};
     dv.s = s;
     dv.pos_a = dw_.pos_a.view();

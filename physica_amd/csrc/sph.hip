@@ -159,7 +159,10 @@ SphSolver::SphSolver(const phx_domain_configuration* domain, const phx_sph_confi
     config_ = *configuration;
     require(config_.method == PHX_SPH_WCSPH || config_.method == PHX_SPH_PCISPH || config_.method == PHX_SPH_IISPH,
             PHX_ERROR_INVALID_ARGUMENT, "unknown SPH method");
-    require(mode == PHX_MODE_FORWARD, PHX_ERROR_UNSUPPORTED, "differentiable mode is not supported yet");
+    require(mode == PHX_MODE_FORWARD || mode == PHX_MODE_DIFFERENTIABLE, PHX_ERROR_INVALID_ARGUMENT,
+            "unknown execution mode");
+    require(mode == PHX_MODE_FORWARD || config_.method != PHX_SPH_WCSPH, PHX_ERROR_UNSUPPORTED,
+            "WCSPH derivatives are not supported yet");
     if (config_.method != PHX_SPH_WCSPH) {
         require(config_.checkpoint_interval > 0u, PHX_ERROR_INVALID_ARGUMENT, "checkpoint_interval must be positive");
         require(config_.pressure_iterations < 4096u, PHX_ERROR_INVALID_ARGUMENT, "too many pressure iterations");
@@ -173,6 +176,53 @@ SphSolver::SphSolver(const phx_domain_configuration* domain, const phx_sph_confi
     u_.resize(n);
     for (DeviceBuffer<float>* b : {&rho0_, &relax_, &tait_, &t_}) b->resize(n);
     masks_.allocate(n, config_.pressure_iterations + 1u, domain_.config.support_radius);
+    if (mode == PHX_MODE_DIFFERENTIABLE) {
+        // PCISPH::forward (sph.cpp:295-309) and IISPH::forward (sph.cpp:408-421)
+        // are the DFSPH step with nd = 0, no warm start and their relaxation in
+        // the density phase: the divergence phase then adds a zero
+        // acceleration, and the pressure update's volume is m / rho0 (PCISPH)
+        // or m / rho (IISPH)
+        phx_dfsph_configuration dc{};
+        dc.divergence_iterations = 0u;
+        dc.density_iterations = config_.pressure_iterations;
+        dc.checkpoint_interval = config_.checkpoint_interval;
+        dc.pressure_warm_start = 0u;
+        engine_ = std::make_unique<DfsphSolver>(domain, &dc, PHX_MODE_DIFFERENTIABLE, stream);
+        engine_->set_volume_rest_density(config_.method == PHX_SPH_PCISPH);
+        for (auto& b : engine_warm_) b.resize(n);
+    }
+}
+
+void SphSolver::require_derivatives(const SphCache& cache) const {
+    require(engine_ != nullptr, PHX_ERROR_UNSUPPORTED, "derivatives need a solver created in differentiable mode");
+    require(cache.record != nullptr && cache.method == config_.method, PHX_ERROR_INVALID_ARGUMENT,
+            "the step cache holds no step recorded by this solver");
+}
+
+void SphSolver::jvp_step(const SphCache& cache, const phx_sph_state_tangent& state_t,
+                         const phx_control_tangent& control_t, const phx_sph_parameter_tangent& parameter_t,
+                         phx_sph_state_tangent& next_t) {
+    require_derivatives(cache);
+    const bool iisph = config_.method == PHX_SPH_IISPH;
+    phx_sph_dfsph_state_tangent st{state_t.particles, {nullptr, nullptr}};
+    phx_sph_dfsph_state_tangent nt{next_t.particles, {nullptr, nullptr}};
+    phx_sph_dfsph_parameter_tangent pt{
+        parameter_t.particles,
+        {nullptr, iisph ? parameter_t.method.jacobi_relaxation : parameter_t.method.pressure_relaxation}};
+    engine_->jvp_step(*cache.record, st, control_t, pt, nt);
+}
+
+void SphSolver::vjp_step(const SphCache& cache, const phx_sph_state_adjoint& next_a,
+                         phx_sph_state_adjoint& previous_a, phx_control_adjoint& control_a,
+                         phx_sph_parameter_adjoint& parameter_a) {
+    require_derivatives(cache);
+    const bool iisph = config_.method == PHX_SPH_IISPH;
+    phx_sph_dfsph_state_adjoint na{next_a.particles, {nullptr, nullptr}};
+    phx_sph_dfsph_state_adjoint pa{previous_a.particles, {nullptr, nullptr}};
+    phx_sph_dfsph_parameter_adjoint pp{
+        parameter_a.particles,
+        {nullptr, iisph ? parameter_a.method.jacobi_relaxation : parameter_a.method.pressure_relaxation}};
+    engine_->vjp_step(*cache.record, na, pa, control_a, pp);
 }
 
 void SphSolver::forward_step(const phx_sph_state& state, const phx_control& control,
@@ -190,6 +240,21 @@ void SphSolver::forward_step(const phx_sph_state& state, const phx_control& cont
     const auto& pp = parameters.particles;
     require(pp.masses && pp.rest_densities && pp.viscosities, PHX_ERROR_INVALID_ARGUMENT, "null particle parameters");
     cache.method = config_.method;
+    if (engine_) {
+        const float* relaxation = config_.method == PHX_SPH_IISPH ? parameters.method.jacobi_relaxation
+                                                                  : parameters.method.pressure_relaxation;
+        require(relaxation != nullptr, PHX_ERROR_INVALID_ARGUMENT,
+                config_.method == PHX_SPH_IISPH ? "null IISPH jacobi_relaxation" : "null PCISPH pressure_relaxation");
+        if (!cache.record) cache.record = std::make_unique<DfsphCache>();
+        cache.stream = stream_;
+        phx_sph_dfsph_state s{state.particles, {nullptr, nullptr}};
+        phx_sph_dfsph_state nx{next.particles, {engine_warm_[0].data(), engine_warm_[1].data()}};
+        phx_sph_dfsph_parameters p{parameters.particles, {relaxation, relaxation}};
+        engine_->forward_step(s, control, p, nx, *cache.record);
+        next.particles.step_index = nx.particles.step_index;
+        return;
+    }
+    cache.record.reset();
     cache.allocate(n, domain_.cells);
     if (config_.method == PHX_SPH_WCSPH) {
         forward_wcsph(state, control, parameters, next, cache);

@@ -2,6 +2,8 @@
 // sorted-order engine.
 #pragma once
 
+#include <memory>
+
 #include "dfsph.hpp"
 #include "engine.hpp"
 #include "masks.hpp"
@@ -20,7 +22,18 @@ struct SphCache {
     DeviceBuffer<float> densities;
     DeviceBuffer<float> pressures;
     DeviceBuffer<float4> non_pressure;  // xyz used
+    // differentiable mode (PCISPH, IISPH): the step recorded by the
+    // projection engine, which then holds every field above
+    std::unique_ptr<DfsphCache> record;
     void allocate(uint32_t n, uint32_t cells);
+
+    const Neighborhood& grid() const { return record ? record->neighborhood : neighborhood; }
+    const float* density_data() const { return record ? record->densities.data() : densities.data(); }
+    // the last iteration's pressures
+    const float* pressure_data() const {
+        return record ? record->pressures(1u, record->iterations[1]) : pressures.data();
+    }
+    const float4* non_pressure_data() const { return record ? record->non_pressure.data() : non_pressure.data(); }
 };
 
 enum SphPass : int {
@@ -46,6 +59,12 @@ class SphSolver {
 
     void forward_step(const phx_sph_state& state, const phx_control& control, const phx_sph_parameters& parameters,
                       phx_sph_state& next, SphCache& cache);
+    // SPH<PCISPH|IISPH>::jvp_step / vjp_step (sph/sph.ixx:356-365) of a step
+    // recorded in differentiable mode
+    void jvp_step(const SphCache& cache, const phx_sph_state_tangent& state_t, const phx_control_tangent& control_t,
+                  const phx_sph_parameter_tangent& parameter_t, phx_sph_state_tangent& next_t);
+    void vjp_step(const SphCache& cache, const phx_sph_state_adjoint& next_a, phx_sph_state_adjoint& previous_a,
+                  phx_control_adjoint& control_a, phx_sph_parameter_adjoint& parameter_a);
 
     const Domain& domain() const { return domain_; }
     hipStream_t stream() const { return stream_; }
@@ -59,10 +78,17 @@ class SphSolver {
     void forward_pressure(const phx_sph_state& state, const phx_control& control,
                           const phx_sph_parameters& parameters, phx_sph_state& next, SphCache& cache);
 
+    void require_derivatives(const SphCache& cache) const;
+
     Domain domain_;
     phx_sph_configuration config_{};
     phx_execution_mode mode_;
     hipStream_t stream_;
+    // differentiable PCISPH/IISPH: the DFSPH projection engine without a
+    // divergence phase or warm start (V = m / rho0 for PCISPH), which steps,
+    // records and differentiates exactly these steps
+    std::unique_ptr<DfsphSolver> engine_;
+    DeviceBuffer<float> engine_warm_[2];  // its (unused) warm-start outputs
     float gradient_norm_ = 0.0F;
     KernelConstants constants_{};
     PassProfiler profiler_;

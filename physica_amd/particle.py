@@ -441,6 +441,32 @@ class PCISPH:
         def to_c(self) -> _capi.SphMethodParameters:
             return _capi.SphMethodParameters(None, None, None, _ptr(self.pressure_relaxation), None)
 
+    @dataclass
+    class StateTangent:
+        """Empty (sph/sph.ixx:109)."""
+
+    @dataclass
+    class StateAdjoint:
+        """Empty (sph/sph.ixx:110)."""
+
+    @dataclass
+    class ParameterTangent:
+        """FP32 (sph/sph.ixx:112); None counts as zero."""
+
+        pressure_relaxation: Optional[torch.Tensor]
+
+        def to_c(self) -> _capi.SphMethodParameterTangent:
+            return _capi.SphMethodParameterTangent(None, None, None, _ptr(self.pressure_relaxation), None)
+
+    @dataclass
+    class ParameterAdjoint:
+        """FP64 (sph/sph.ixx:113); None is skipped."""
+
+        pressure_relaxation: Optional[torch.Tensor]
+
+        def to_c(self) -> _capi.SphMethodParameterAdjoint:
+            return _capi.SphMethodParameterAdjoint(None, None, None, _ptr(self.pressure_relaxation), None)
+
 
 class IISPH:
     """IISPH method types, sph/sph.ixx:149-193."""
@@ -463,6 +489,32 @@ class IISPH:
 
         def to_c(self) -> _capi.SphMethodParameters:
             return _capi.SphMethodParameters(None, None, None, None, _ptr(self.jacobi_relaxation))
+
+    @dataclass
+    class StateTangent:
+        """Empty (sph/sph.ixx:155)."""
+
+    @dataclass
+    class StateAdjoint:
+        """Empty (sph/sph.ixx:156)."""
+
+    @dataclass
+    class ParameterTangent:
+        """FP32 (sph/sph.ixx:158); None counts as zero."""
+
+        jacobi_relaxation: Optional[torch.Tensor]
+
+        def to_c(self) -> _capi.SphMethodParameterTangent:
+            return _capi.SphMethodParameterTangent(None, None, None, None, _ptr(self.jacobi_relaxation))
+
+    @dataclass
+    class ParameterAdjoint:
+        """FP64 (sph/sph.ixx:159); None is skipped."""
+
+        jacobi_relaxation: Optional[torch.Tensor]
+
+        def to_c(self) -> _capi.SphMethodParameterAdjoint:
+            return _capi.SphMethodParameterAdjoint(None, None, None, None, _ptr(self.jacobi_relaxation))
 
 
 _METHODS = {DFSPH.Configuration: DFSPH, WCSPH.Configuration: WCSPH, PCISPH.Configuration: PCISPH,
@@ -499,9 +551,10 @@ class SPH:
         method: object
 
         def to_c(self):
-            return _capi.SphDfsphStateTangent(
-                _capi.ParticleStateTangent(self.particles.positions.view(), self.particles.velocities.view()),
-                self.method.to_c())
+            particles = _capi.ParticleStateTangent(self.particles.positions.view(), self.particles.velocities.view())
+            if isinstance(self.method, DFSPH.StateTangent):
+                return _capi.SphDfsphStateTangent(particles, self.method.to_c())
+            return _capi.SphStateTangent(particles)
 
     @dataclass
     class StateAdjoint:
@@ -509,9 +562,10 @@ class SPH:
         method: object
 
         def to_c(self):
-            return _capi.SphDfsphStateAdjoint(
-                _capi.ParticleStateAdjoint(self.particles.positions.view(), self.particles.velocities.view()),
-                self.method.to_c())
+            particles = _capi.ParticleStateAdjoint(self.particles.positions.view(), self.particles.velocities.view())
+            if isinstance(self.method, DFSPH.StateAdjoint):
+                return _capi.SphDfsphStateAdjoint(particles, self.method.to_c())
+            return _capi.SphStateAdjoint(particles)
 
     @dataclass
     class ParameterTangent:
@@ -519,7 +573,9 @@ class SPH:
         method: object
 
         def to_c(self):
-            return _capi.SphDfsphParameterTangent(self.particles.to_c(), self.method.to_c())
+            if isinstance(self.method, DFSPH.ParameterTangent):
+                return _capi.SphDfsphParameterTangent(self.particles.to_c(), self.method.to_c())
+            return _capi.SphParameterTangent(self.particles.to_c(), self.method.to_c())
 
     @dataclass
     class ParameterAdjoint:
@@ -527,7 +583,9 @@ class SPH:
         method: object
 
         def to_c(self):
-            return _capi.SphDfsphParameterAdjoint(self.particles.to_c(), self.method.to_c())
+            if isinstance(self.method, DFSPH.ParameterAdjoint):
+                return _capi.SphDfsphParameterAdjoint(self.particles.to_c(), self.method.to_c())
+            return _capi.SphParameterAdjoint(self.particles.to_c(), self.method.to_c())
 
     class StepCache:
         """StepCache (sph/sph.ixx:299-303); fields are read back in original order."""
@@ -663,70 +721,77 @@ class SPH:
         """Per-pass totals since profiling was enabled or last read."""
         return _read_profile(self._prefix + "_read_profile", self._handle)
 
-    # tangents and adjoints (sph/sph.ixx:313-348), DFSPH; allocated zero
-    def _require_dfsph_derivatives(self) -> None:
-        if self.method is not DFSPH:
-            raise NotImplementedError(f"derivatives of SPH<{self.method.__name__}> are not provided")
+    # tangents and adjoints (sph/sph.ixx:313-348): DFSPH, PCISPH, IISPH;
+    # allocated zero
+    def _require_derivatives(self) -> None:
+        if self.method is WCSPH:
+            raise NotImplementedError("derivatives of SPH<WCSPH> are not provided")
 
     def allocate_state_tangent(self) -> "SPH.StateTangent":
-        self._require_dfsph_derivatives()
+        self._require_derivatives()
         d = self.domain
         n = d.configuration.particle_count
         z = lambda: torch.zeros(n, dtype=torch.float32, device=d.device)  # noqa: E731
+        method = DFSPH.StateTangent(z(), z()) if self.method is DFSPH else self.method.StateTangent()
         return SPH.StateTangent(ParticleStateTangent(VectorField.zeros(n, d.device), VectorField.zeros(n, d.device)),
-                                DFSPH.StateTangent(z(), z()))
+                                method)
 
     def allocate_control_tangent(self) -> ControlTangent:
         return ControlTangent(VectorField.zeros(self.domain.configuration.particle_count, self.domain.device))
 
     def allocate_parameter_tangent(self) -> "SPH.ParameterTangent":
-        self._require_dfsph_derivatives()
+        self._require_derivatives()
         d = self.domain
         z = lambda: torch.zeros(d.configuration.particle_count, dtype=torch.float32, device=d.device)  # noqa: E731
-        return SPH.ParameterTangent(ParticleParameterTangent(z(), z(), z(), z()), DFSPH.ParameterTangent(z(), z()))
+        method = DFSPH.ParameterTangent(z(), z()) if self.method is DFSPH else self.method.ParameterTangent(z())
+        return SPH.ParameterTangent(ParticleParameterTangent(z(), z(), z(), z()), method)
 
     def allocate_state_adjoint(self) -> "SPH.StateAdjoint":
-        self._require_dfsph_derivatives()
+        self._require_derivatives()
         d = self.domain
         z = d.allocate_scalar_adjoint_field
+        method = DFSPH.StateAdjoint(z(), z()) if self.method is DFSPH else self.method.StateAdjoint()
         return SPH.StateAdjoint(ParticleStateAdjoint(d.allocate_vector_adjoint_field(),
-                                                     d.allocate_vector_adjoint_field()),
-                                DFSPH.StateAdjoint(z(), z()))
+                                                     d.allocate_vector_adjoint_field()), method)
 
     def allocate_control_adjoint(self) -> ControlAdjoint:
         return ControlAdjoint(self.domain.allocate_vector_adjoint_field())
 
     def allocate_parameter_adjoint(self) -> "SPH.ParameterAdjoint":
-        self._require_dfsph_derivatives()
+        self._require_derivatives()
         z = self.domain.allocate_scalar_adjoint_field
-        return SPH.ParameterAdjoint(ParticleParameterAdjoint(z(), z(), z(), z()), DFSPH.ParameterAdjoint(z(), z()))
+        method = DFSPH.ParameterAdjoint(z(), z()) if self.method is DFSPH else self.method.ParameterAdjoint(z())
+        return SPH.ParameterAdjoint(ParticleParameterAdjoint(z(), z(), z(), z()), method)
 
     def copy_state_tangent(self, source: "SPH.StateTangent", destination: "SPH.StateTangent") -> None:
         destination.particles.positions.copy_(source.particles.positions)
         destination.particles.velocities.copy_(source.particles.velocities)
-        destination.method.warm_divergence_pressure.copy_(source.method.warm_divergence_pressure)
-        destination.method.warm_density_pressure.copy_(source.method.warm_density_pressure)
+        if self.method is DFSPH:
+            destination.method.warm_divergence_pressure.copy_(source.method.warm_divergence_pressure)
+            destination.method.warm_density_pressure.copy_(source.method.warm_density_pressure)
 
     def copy_state_adjoint(self, source: "SPH.StateAdjoint", destination: "SPH.StateAdjoint") -> None:
         destination.particles.positions.copy_(source.particles.positions)
         destination.particles.velocities.copy_(source.particles.velocities)
-        destination.method.warm_divergence_pressure.copy_(source.method.warm_divergence_pressure)
-        destination.method.warm_density_pressure.copy_(source.method.warm_density_pressure)
+        if self.method is DFSPH:
+            destination.method.warm_divergence_pressure.copy_(source.method.warm_divergence_pressure)
+            destination.method.warm_density_pressure.copy_(source.method.warm_density_pressure)
 
     def accumulate_state_adjoint(self, source: "SPH.StateAdjoint", destination: "SPH.StateAdjoint") -> None:
         destination.particles.positions.add_(source.particles.positions)
         destination.particles.velocities.add_(source.particles.velocities)
-        destination.method.warm_divergence_pressure.add_(source.method.warm_divergence_pressure)
-        destination.method.warm_density_pressure.add_(source.method.warm_density_pressure)
+        if self.method is DFSPH:
+            destination.method.warm_divergence_pressure.add_(source.method.warm_divergence_pressure)
+            destination.method.warm_density_pressure.add_(source.method.warm_density_pressure)
 
     def jvp_step(self, state, control, parameters, next_state, cache, state_tangent, control_tangent,
                  parameter_tangent, next_state_tangent) -> None:
-        """SPH<DFSPH>::jvp_step (sph/sph.ixx:356-359) of the step recorded in
-        `cache`: state, control, parameters and next_state are those of that
-        forward step, which the cache recorded."""
-        self._require_dfsph_derivatives()
+        """SPH<Method>::jvp_step (sph/sph.ixx:356-359) of the step recorded in
+        `cache` (DFSPH, PCISPH, IISPH): state, control, parameters and
+        next_state are those of that forward step, which the cache recorded."""
+        self._require_derivatives()
         _capi.call(
-            "phx_dfsph_jvp_step",
+            self._prefix + "_jvp_step",
             self._handle,
             cache._handle,
             ctypes.byref(state_tangent.to_c()),
@@ -737,12 +802,12 @@ class SPH:
 
     def vjp_step(self, state, control, parameters, next_state, cache, next_state_adjoint, previous_state_adjoint,
                  control_adjoint, parameter_adjoint) -> None:
-        """SPH<DFSPH>::vjp_step (sph/sph.ixx:361-365): particle, control and
-        parameter adjoints accumulate; warm-start pressure adjoints are
-        written (with warm start)."""
-        self._require_dfsph_derivatives()
+        """SPH<Method>::vjp_step (sph/sph.ixx:361-365): particle, control and
+        parameter adjoints accumulate; DFSPH's warm-start pressure adjoints
+        are written (with warm start)."""
+        self._require_derivatives()
         _capi.call(
-            "phx_dfsph_vjp_step",
+            self._prefix + "_vjp_step",
             self._handle,
             cache._handle,
             ctypes.byref(next_state_adjoint.to_c()),

@@ -79,7 +79,7 @@ def pbf_setup(scene, device, configuration=None, mode=ExecutionMode.forward):
     return solver, state, nxt, control, params, cache
 
 
-def sph_setup(scene, device, method, pressure_iterations=None):
+def sph_setup(scene, device, method, pressure_iterations=None, mode=ExecutionMode.forward):
     """SPH<WCSPH|PCISPH|IISPH> with the scene uploaded; `method` is "wcsph",
     "pcisph" or "iisph" (the scene's "relaxation" feeds PCISPH's pressure or
     IISPH's Jacobi relaxation)."""
@@ -87,7 +87,7 @@ def sph_setup(scene, device, method, pressure_iterations=None):
     configuration = {"wcsph": lambda: WCSPH.Configuration(),
                      "pcisph": lambda: PCISPH.Configuration(pressure_iterations=iterations),
                      "iisph": lambda: IISPH.Configuration(pressure_iterations=iterations)}[method]()
-    solver = SPH(scene.domain, configuration, device=device)
+    solver = SPH(scene.domain, configuration, mode=mode, device=device)
     state = solver.allocate_state()
     nxt = solver.allocate_state()
     control = solver.allocate_control()

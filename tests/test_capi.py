@@ -101,6 +101,8 @@ def test_null_arguments_rejected():
     assert lib.phx_pbf_vjp_step(None, None, None, None, None, None, None) == _capi.PHX_ERROR_INVALID_ARGUMENT
     assert lib.phx_dfsph_jvp_step(None, None, None, None, None, None) == _capi.PHX_ERROR_INVALID_ARGUMENT
     assert lib.phx_dfsph_vjp_step(None, None, None, None, None, None) == _capi.PHX_ERROR_INVALID_ARGUMENT
+    assert lib.phx_sph_jvp_step(None, None, None, None, None, None) == _capi.PHX_ERROR_INVALID_ARGUMENT
+    assert lib.phx_sph_vjp_step(None, None, None, None, None, None) == _capi.PHX_ERROR_INVALID_ARGUMENT
     assert lib.phx_grid_resolution(None, None) == _capi.PHX_ERROR_INVALID_ARGUMENT
     assert lib.phx_dfsph_cache_densities(None, None) == _capi.PHX_ERROR_INVALID_ARGUMENT
     size = ctypes.c_size_t()
