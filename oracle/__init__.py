@@ -500,6 +500,52 @@ def pressure_solver_vjp(domain, step_index, positions, velocities, controls, par
     return out
 
 
+# SPH<WCSPH> particle and method parameters in the order of its tangents and
+# adjoints (sph/sph.ixx:45-60 and domain/domain.ixx:58-63).
+WCSPH_PARAMETERS = ("masses", "rest_densities", "viscosities", "surface_tensions", "speed_of_sound",
+                    "tait_exponent", "boundary_surface_tension")
+
+
+def wcsph_jvp(domain, step_index, positions, velocities, controls, params, tangents, precision="f32"):
+    """SPH<WCSPH>::jvp_step (sph/sph.cpp:228-237): tangents of the next
+    positions and velocities. `tangents` holds "positions", "velocities",
+    "controls" and any of WCSPH_PARAMETERS (missing ones are zero)."""
+    n = domain.particle_count
+    pos, pp = _vec(positions, precision)
+    vel, vp = _vec(velocities, precision)
+    ctl, cp = _vec(controls, precision)
+    prm = [_arr(params[k], precision) for k in WCSPH_PARAMETERS]
+    zero3 = np.zeros((3, n))
+    tv = [_vec(tangents.get(k, zero3), precision) for k in ("positions", "velocities", "controls")]
+    prm_t = [_arr(tangents.get(k, np.zeros(n)), precision) for k in WCSPH_PARAMETERS]
+    nx, nxp = _out_vec(n, precision)
+    nv, nvp = _out_vec(n, precision)
+    getattr(library(), "oracle_wcsph_jvp_" + precision)(
+        ctypes.byref(domain), c_uint64(step_index), pp, vp, cp, _pointers(prm), *(t[1] for t in tv),
+        _pointers(prm_t), nxp, nvp)
+    return {"positions": nx, "velocities": nv}
+
+
+def wcsph_vjp(domain, step_index, positions, velocities, controls, params, next_adjoints, precision="f32"):
+    """SPH<WCSPH>::vjp_step (sph/sph.cpp:239-249) with FP64 adjoints of the
+    state, the control and every WCSPH_PARAMETERS entry."""
+    n = domain.particle_count
+    pos, pp = _vec(positions, precision)
+    vel, vp = _vec(velocities, precision)
+    ctl, cp = _vec(controls, precision)
+    prm = [_arr(params[k], precision) for k in WCSPH_PARAMETERS]
+    nxa, nxap = _vec(next_adjoints["positions"], "f64")
+    nva, nvap = _vec(next_adjoints["velocities"], "f64")
+    outs = {k: _out_vec(n, "f64") for k in ("positions", "velocities", "controls")}
+    scal = {k: np.zeros(n, dtype=np.float64) for k in WCSPH_PARAMETERS}
+    getattr(library(), "oracle_wcsph_vjp_" + precision)(
+        ctypes.byref(domain), c_uint64(step_index), pp, vp, cp, _pointers(prm), nxap, nvap,
+        *(outs[k][1] for k in ("positions", "velocities", "controls")), _pointers([scal[k] for k in WCSPH_PARAMETERS]))
+    result = {k: outs[k][0] for k in outs}
+    result.update(scal)
+    return result
+
+
 def set_reference_no_slip_adjoint(enabled: bool) -> None:
     """Switches the SPH prediction's VJP between the exact transpose (default)
     and the reference's variant, which drops dt * xbar of the components that

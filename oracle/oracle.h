@@ -158,6 +158,19 @@ void oracle_set_threads(int threads);
                                double* velocity_adjoint[3], double* control_adjoint[3],                                 \
                                double* const parameter_adjoint[5], double* warm_divergence_adjoint,                     \
                                double* warm_density_adjoint);                                                           \
+    /* SPH<WCSPH>::jvp_step / vjp_step (sph.ixx:356-365, sph.cpp:228-249). parameters: masses, rest densities,     \
+     * viscosities, surface tensions, speed of sound, Tait exponent, boundary surface tension. The VJP accumulates   \
+     * FP64 adjoints into its outputs */                                                                               \
+    void oracle_wcsph_jvp##suf(const oracle_domain* domain, uint64_t step_index, const real* positions[3],            \
+                               const real* velocities[3], const real* controls[3], const real* const parameters[7],    \
+                               const real* position_tangent[3], const real* velocity_tangent[3],                        \
+                               const real* control_tangent[3], const real* const parameter_tangent[7],                  \
+                               real* next_position_tangent[3], real* next_velocity_tangent[3]);                         \
+    void oracle_wcsph_vjp##suf(const oracle_domain* domain, uint64_t step_index, const real* positions[3],            \
+                               const real* velocities[3], const real* controls[3], const real* const parameters[7],    \
+                               const double* next_position_adjoint[3], const double* next_velocity_adjoint[3],          \
+                               double* position_adjoint[3], double* velocity_adjoint[3], double* control_adjoint[3],   \
+                               double* const parameter_adjoint[7]);                                                     \
     /* the primal step the two functions above differentiate: next positions and velocities */                        \
     void oracle_dfsph_record_step##suf(const oracle_domain* domain, uint32_t divergence_iterations,                   \
                                        uint32_t density_iterations, int warm_start, int volume_rest,                    \

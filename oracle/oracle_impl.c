@@ -2494,6 +2494,480 @@ void SUF(oracle_dfsph_vjp)(const oracle_domain* d, uint32_t nd, uint32_t ne, int
     dfsph_record_free(&r);
 }
 
+/* ------------------------------------------------------ WCSPH derivatives */
+/* SPH<WCSPH>::jvp_step / vjp_step (sph.ixx:356-365) -> WCSPH::jvp / vjp
+ * (sph.cpp:228-249): EOS, pressure, artificial viscosity, surface cohesion,
+ * external acceleration and integration, differentiated with the neighbour
+ * graph, the compression test of every pair and the box-collision masks
+ * frozen at the primal step. Parameters: masses, rest densities, viscosities,
+ * surface tensions, speed of sound, Tait exponent, boundary surface tension. */
+enum { WS_MASS = 0, WS_REST, WS_VISC, WS_TENSION, WS_SPEED, WS_EXP, WS_BTENSION, WS_PARAMS };
+
+/* eos_jvp_kernel, wcsph.cu:102-117 */
+static void eos_jvp_kernel(uint32_t n, const real* rho, const real* rho_t, const real* rest, const real* rest_t,
+                           const real* speed, const real* speed_t, const real* exponent, const real* exponent_t,
+                           real* out) {
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < (int64_t)n; ++i) {
+        const real r0 = rest[i];
+        const real c = speed[i];
+        const real g = exponent[i];
+        const real ratio = rho[i] / r0;
+        const real powered = R_POW(ratio, g);
+        const real difference = powered - K(1.0);
+        const real d_rho = c * c * R_POW(ratio, g - K(1.0));
+        const real d_rest = c * c * difference / g - c * c * powered;
+        const real d_speed = K(2.0) * r0 * c * difference / g;
+        const real d_exponent = r0 * c * c * (g * powered * R_LOG(ratio) - difference) / (g * g);
+        out[i] = d_rho * rho_t[i] + d_rest * rest_t[i] + d_speed * speed_t[i] + d_exponent * exponent_t[i];
+    }
+}
+
+/* eos_vjp_kernel, wcsph.cu:119-134 (FP64 from the FP32 primal) */
+static void eos_vjp_kernel(uint32_t n, const real* rho, const real* rest, const real* speed, const real* exponent,
+                           const double* p_a, double* rho_a, double* rest_a, double* speed_a, double* exponent_a) {
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < (int64_t)n; ++i) {
+        const double r0 = rest[i];
+        const double c = speed[i];
+        const double g = exponent[i];
+        const double ratio = (double)rho[i] / r0;
+        const double powered = pow(ratio, g);
+        const double difference = powered - 1.0;
+        const double a = p_a[i];
+        rho_a[i] += a * c * c * pow(ratio, g - 1.0);
+        rest_a[i] += a * (c * c * difference / g - c * c * powered);
+        speed_a[i] += a * 2.0 * r0 * c * difference / g;
+        exponent_a[i] += a * r0 * c * c * (g * powered * log(ratio) - difference) / (g * g);
+    }
+}
+
+/* The tangent of one artificial-viscosity pair term -m pi W'(d) given the
+ * tangents of its inputs (viscosity_jvp_kernel, wcsph.cu:197-217 and, for a
+ * boundary sample, :226-234). nu = k (alpha c / rho_sum) with k = 2 h (fluid)
+ * or h (boundary, where alpha c / rho are particle i's own). */
+static v3 viscosity_pair_tangent(v3 d, v3 dd, v3 dv, v3 dvd, real nu, real nu_t, real mass, real mass_t, real h) {
+    const real compression = v3_dot(dv, d);
+    const real compression_t = v3_dot(dvd, d) + v3_dot(dv, dd);
+    const real denominator = v3_dot(d, d) + K(0.01) * h * h;
+    const real denominator_t = K(2.0) * v3_dot(d, dd);
+    const real pi = -nu * compression / denominator;
+    const real pi_t = -(nu_t * compression + nu * compression_t) / denominator +
+                      nu * compression * denominator_t / (denominator * denominator);
+    const v3 gradient = cubic_gradient(d, h);
+    const v3 gradient_t = cubic_gradient_tangent(d, dd, h);
+    return v3_add(v3_scale(gradient, -(mass_t * pi + mass * pi_t)), v3_scale(gradient_t, -mass * pi));
+}
+
+/* viscosity_jvp_kernel, wcsph.cu:176-237 */
+static void artificial_viscosity_jvp_kernel(const grid_t* g, const real* const pos[3], const real* const vel[3],
+                                            const real* const pos_t[3], const real* const vel_t[3],
+                                            const real* const prm[WS_PARAMS], const real* const prm_t[WS_PARAMS],
+                                            const real* rho, const real* rho_t, real* const out[3]) {
+    const real h = (real)g->d->support_radius;
+    const real *m = prm[WS_MASS], *m_t = prm_t[WS_MASS], *rest = prm[WS_REST], *rest_t = prm_t[WS_REST];
+    const real *mu = prm[WS_VISC], *mu_t = prm_t[WS_VISC], *c = prm[WS_SPEED], *c_t = prm_t[WS_SPEED];
+#pragma omp parallel for schedule(static)
+    for (int64_t ii = 0; ii < (int64_t)g->n; ++ii) {
+        const uint32_t i = (uint32_t)ii;
+        const v3 p = v3_load(pos, i);
+        const v3 v = v3_load(vel, i);
+        const v3 pt = v3_load(pos_t, i);
+        const v3 vt = v3_load(vel_t, i);
+        v3 result = v3_make(0, 0, 0);
+        cell_range cr[27];
+        const int nr = stencil(g, p, cr);
+        for (int k = 0; k < nr; ++k) {
+            for (uint32_t s = cr[k].first; s < cr[k].last; ++s) {
+                const uint32_t j = g->idx[s];
+                if (j == i) continue;
+                const v3 d = v3_sub(p, v3_load(pos, j));
+                const v3 dv = v3_sub(v, v3_load(vel, j));
+                if (v3_dot(dv, d) >= K(0.0)) continue;
+                const real alpha = K(0.5) * (mu[i] + mu[j]);
+                const real alpha_t = K(0.5) * (mu_t[i] + mu_t[j]);
+                const real speed = K(0.5) * (c[i] + c[j]);
+                const real speed_t = K(0.5) * (c_t[i] + c_t[j]);
+                const real rho_sum = rho[i] + rho[j];
+                const real rho_sum_t = rho_t[i] + rho_t[j];
+                const real nu = K(2.0) * alpha * h * speed / rho_sum;
+                const real nu_t = K(2.0) * h * (alpha_t * speed + alpha * speed_t) / rho_sum - nu * rho_sum_t / rho_sum;
+                result = v3_add(result, viscosity_pair_tangent(d, v3_sub(pt, v3_load(pos_t, j)), dv,
+                                                               v3_sub(vt, v3_load(vel_t, j)), nu, nu_t, m[j], m_t[j],
+                                                               h));
+            }
+            for (uint32_t s = cr[k].bfirst; s < cr[k].blast; ++s) {
+                const uint32_t b = g->bidx[s];
+                const v3 d = v3_sub(p, boundary_position(g, b));
+                const v3 dv = v3_sub(v, boundary_velocity(g, b));
+                if (v3_dot(dv, d) >= K(0.0)) continue;
+                const real nu = mu[i] * h * c[i] / rho[i];
+                const real nu_t = h * (mu_t[i] * c[i] + mu[i] * c_t[i]) / rho[i] - nu * rho_t[i] / rho[i];
+                const real vol = boundary_volume(g, b);
+                result = v3_add(result, viscosity_pair_tangent(d, pt, dv, vt, nu, nu_t, rest[i] * vol,
+                                                               rest_t[i] * vol, h));
+            }
+        }
+        v3_store(out, i, result);
+    }
+}
+
+/* The weight of a cohesion pair (surface_forward_kernel, wcsph.cu:324): the
+ * kernel value at the distance, clamped below at the particle diameter. */
+static real cohesion_weight(v3 d, real h, real diameter) {
+    return v3_length(d) > diameter ? cubic(d, h) : cubic(v3_make(diameter, K(0.0), K(0.0)), h);
+}
+
+/* surface_jvp_kernel, wcsph.cu:341-386; a pair term is -s m_b w / m_i d with
+ * (s, m_b) = (sigma_i, m_j) for a fluid neighbour and (sigma_b,i,
+ * rho0_i V_b) for a boundary sample */
+static void surface_jvp_kernel(const grid_t* g, const real* const pos[3], const real* const pos_t[3],
+                               const real* const prm[WS_PARAMS], const real* const prm_t[WS_PARAMS],
+                               real* const out[3]) {
+    const real h = (real)g->d->support_radius;
+    const real diameter = K(2.0) * (real)g->d->particle_radius;
+    const real *m = prm[WS_MASS], *m_t = prm_t[WS_MASS], *rest = prm[WS_REST], *rest_t = prm_t[WS_REST];
+    const real *st = prm[WS_TENSION], *st_t = prm_t[WS_TENSION], *bt = prm[WS_BTENSION], *bt_t = prm_t[WS_BTENSION];
+#pragma omp parallel for schedule(static)
+    for (int64_t ii = 0; ii < (int64_t)g->n; ++ii) {
+        const uint32_t i = (uint32_t)ii;
+        const v3 p = v3_load(pos, i);
+        const v3 pt = v3_load(pos_t, i);
+        const real mi = m[i];
+        v3 result = v3_make(0, 0, 0);
+        cell_range cr[27];
+        const int nr = stencil(g, p, cr);
+        for (int k = 0; k < nr; ++k) {
+            for (uint32_t s = cr[k].first; s < cr[k].last; ++s) {
+                const uint32_t j = g->idx[s];
+                if (j == i) continue;
+                const v3 d = v3_sub(p, v3_load(pos, j));
+                const v3 dd = v3_sub(pt, v3_load(pos_t, j));
+                const real dist = v3_length(d);
+                if (dist >= h) continue;
+                const real w = cohesion_weight(d, h, diameter);
+                const real w_t = dist > diameter ? v3_dot(cubic_gradient(d, h), dd) : K(0.0);
+                const real mj = m[j];
+                const real factor = -st[i] * mj * w / mi;
+                const real factor_t = -st_t[i] * mj * w / mi - st[i] * m_t[j] * w / mi - st[i] * mj * w_t / mi +
+                                      st[i] * mj * w * m_t[i] / (mi * mi);
+                result = v3_add(result, v3_add(v3_scale(d, factor_t), v3_scale(dd, factor)));
+            }
+            for (uint32_t s = cr[k].bfirst; s < cr[k].blast; ++s) {
+                const uint32_t b = g->bidx[s];
+                const v3 d = v3_sub(p, boundary_position(g, b));
+                const real dist = v3_length(d);
+                if (dist >= h) continue;
+                const real w = cohesion_weight(d, h, diameter);
+                const real w_t = dist > diameter ? v3_dot(cubic_gradient(d, h), pt) : K(0.0);
+                const real vol = boundary_volume(g, b);
+                const real mb = rest[i] * vol;
+                const real mb_t = rest_t[i] * vol;
+                const real factor = -bt[i] * mb * w / mi;
+                const real factor_t = -bt_t[i] * mb * w / mi - bt[i] * mb_t * w / mi - bt[i] * mb * w_t / mi +
+                                      bt[i] * mb * w * m_t[i] / (mi * mi);
+                result = v3_add(result, v3_add(v3_scale(d, factor_t), v3_scale(pt, factor)));
+            }
+        }
+        v3_store(out, i, result);
+    }
+}
+
+void SUF(oracle_wcsph_jvp)(const oracle_domain* d, uint64_t step, const real* pos[3], const real* vel[3],
+                           const real* ctl[3], const real* const prm[WS_PARAMS], const real* pos_t[3],
+                           const real* vel_t[3], const real* ctl_t[3], const real* const prm_t[WS_PARAMS],
+                           real* next_pos_t[3], real* next_vel_t[3]) {
+    const uint32_t n = d->particle_count;
+    const real dt = (real)d->time_step;
+    const cbox box = collision_box(d, step);
+    grid_t g;
+    grid_build(&g, d, step, pos[0], pos[1], pos[2]);
+    const real *m = prm[WS_MASS], *rest = prm[WS_REST];
+    real* rho = alloc_real(n);
+    real* p = alloc_real(n);
+    vfield ap = alloc_vfield(n), av = alloc_vfield(n), as = alloc_vfield(n), total = alloc_vfield(n);
+    density_kernel(&g, 0, pos, pos, m, rest, rho);
+    eos_kernel(n, rho, rest, prm[WS_SPEED], prm[WS_EXP], p);
+    pressure_kernel(&g, pos, m, rest, rho, p, ap.c);
+    artificial_viscosity_kernel(&g, pos, vel, m, rest, prm[WS_VISC], prm[WS_SPEED], rho, av.c);
+    surface_kernel(&g, pos, m, rest, prm[WS_TENSION], prm[WS_BTENSION], as.c);
+    add_kernel(n, CV(ap), CV(av), total.c);
+    add_kernel(n, CV(total), CV(as), total.c);
+    const v3 gravity = v3_make((real)d->gravity[0], (real)d->gravity[1], (real)d->gravity[2]);
+    for (uint32_t i = 0; i < n; ++i) v3_store(ap.c, i, v3_add(gravity, v3_load(ctl, i)));
+    add_kernel(n, CV(total), CV(ap), total.c);
+    /* tangents (sph.cpp:228-237; the state densities' tangent, sph.ixx:357) */
+    real* rho_t = alloc_real(n);
+    real* p_t = alloc_real(n);
+    vfield apt = alloc_vfield(n), avt = alloc_vfield(n), ast = alloc_vfield(n), tt = alloc_vfield(n),
+           zero = alloc_vfield(n);
+    for (int c = 0; c < 3; ++c) memset(zero.c[c], 0, sizeof(real) * n);
+    density_jvp_kernel(&g, pos, pos, pos_t, m, prm_t[WS_MASS], rest, prm_t[WS_REST], rho_t);
+    eos_jvp_kernel(n, rho, rho_t, rest, prm_t[WS_REST], prm[WS_SPEED], prm_t[WS_SPEED], prm[WS_EXP], prm_t[WS_EXP],
+                   p_t);
+    pressure_jvp_kernel(&g, pos, pos_t, m, prm_t[WS_MASS], rest, prm_t[WS_REST], rho, rho_t, p, p_t, apt.c);
+    artificial_viscosity_jvp_kernel(&g, pos, vel, pos_t, vel_t, prm, prm_t, rho, rho_t, avt.c);
+    surface_jvp_kernel(&g, pos, pos_t, prm, prm_t, ast.c);
+    add_kernel(n, CV(apt), CV(avt), tt.c);
+    add_kernel(n, CV(tt), CV(ast), tt.c);
+    add_kernel(n, CV(tt), ctl_t, tt.c);
+    predict_jvp_kernel(n, dt, box, pos, vel, CV(total), CV(zero), pos_t, vel_t, CV(tt), CV(zero), next_pos_t,
+                       next_vel_t);
+    free(rho);
+    free(p);
+    free(rho_t);
+    free(p_t);
+    free_vfield(&ap);
+    free_vfield(&av);
+    free_vfield(&as);
+    free_vfield(&total);
+    free_vfield(&apt);
+    free_vfield(&avt);
+    free_vfield(&ast);
+    free_vfield(&tt);
+    free_vfield(&zero);
+    grid_free(&g);
+}
+
+/* viscosity_reverse, wcsph.cu:40-72: the adjoint of one pair term
+ * -m_b pi(d, dv) W'(d) with pi = -nu (dv . d) / (|d|^2 + 0.01 h^2),
+ * nu = 2 h alpha c / (rho_a + rho_b), for the acceleration adjoint w of its
+ * centre; zero for a separating pair. */
+typedef struct {
+    d3 displacement, velocity;
+    double mass_b, rho_a, rho_b, mu_a, mu_b, c_a, c_b;
+} viscosity_adjoint_t;
+
+static viscosity_adjoint_t viscosity_reverse(v3 d, v3 dv, real rho_a, real rho_b, real mu_a, real mu_b, real c_a,
+                                             real c_b, real mass_b, real h, d3 w) {
+    viscosity_adjoint_t r;
+    memset(&r, 0, sizeof(r));
+    const double compression = v3_dot(dv, d);
+    if (compression >= 0.0) return r;
+    const double denominator = (double)v3_dot(d, d) + 0.01 * h * h;
+    /* the sums of two FP32 primal values are FP32 sums, as in the
+     * reference's float arguments */
+    const double alpha = 0.5 * (double)(mu_a + mu_b);
+    const double speed = 0.5 * (double)(c_a + c_b);
+    const double rho_sum = (double)(rho_a + rho_b);
+    const double nu = 2.0 * alpha * h * speed / rho_sum;
+    const double pi = -nu * compression / denominator;
+    const d3 gradient = d3_of(cubic_gradient(d, h));
+    r.mass_b = -pi * d3_dot(w, gradient);
+    const double pi_a = -(double)mass_b * d3_dot(w, gradient);
+    r.displacement = cubic_hessian_product(d, d3_scale(w, -(double)mass_b * pi), h);
+    const double nu_a = -compression * pi_a / denominator;
+    const double compression_a = -nu * pi_a / denominator;
+    const double denominator_a = nu * compression * pi_a / (denominator * denominator);
+    r.velocity = d3_scale(d3_of(d), compression_a);
+    r.displacement = d3_add(r.displacement, d3_add(d3_scale(d3_of(dv), compression_a),
+                                                   d3_scale(d3_of(d), 2.0 * denominator_a)));
+    const double alpha_a = 2.0 * h * speed * nu_a / rho_sum;
+    const double speed_a = 2.0 * alpha * h * nu_a / rho_sum;
+    const double rho_sum_a = -nu * nu_a / rho_sum;
+    r.mu_a = r.mu_b = 0.5 * alpha_a;
+    r.c_a = r.c_b = 0.5 * speed_a;
+    r.rho_a = r.rho_b = rho_sum_a;
+    return r;
+}
+
+/* viscosity_vjp_kernel, wcsph.cu:239-304, gather form: particle i collects
+ * the terms where it is the centre (w_i) and those where it is the
+ * neighbour (w_j, with d and dv reversed) */
+static void artificial_viscosity_vjp_kernel(const grid_t* g, const real* const pos[3], const real* const vel[3],
+                                            const real* const prm[WS_PARAMS], const real* rho,
+                                            const double* const acc_a[3], double* const pos_a[3],
+                                            double* const vel_a[3], double* rho_a, double* const prm_a[WS_PARAMS]) {
+    const real h = (real)g->d->support_radius;
+    const real *m = prm[WS_MASS], *rest = prm[WS_REST], *mu = prm[WS_VISC], *c = prm[WS_SPEED];
+#pragma omp parallel for schedule(static)
+    for (int64_t ii = 0; ii < (int64_t)g->n; ++ii) {
+        const uint32_t i = (uint32_t)ii;
+        const v3 p = v3_load(pos, i);
+        const v3 v = v3_load(vel, i);
+        const d3 wi = d3_cload(acc_a, i);
+        d3 position = d3_make(0, 0, 0), velocity = d3_make(0, 0, 0);
+        double mass_c = 0.0, rest_c = 0.0, rho_c = 0.0, mu_c = 0.0, c_c = 0.0;
+        cell_range cr[27];
+        const int nr = stencil(g, p, cr);
+        for (int k = 0; k < nr; ++k) {
+            for (uint32_t s = cr[k].first; s < cr[k].last; ++s) {
+                const uint32_t j = g->idx[s];
+                if (j == i) continue;
+                const v3 d = v3_sub(p, v3_load(pos, j));
+                const v3 dv = v3_sub(v, v3_load(vel, j));
+                const viscosity_adjoint_t own =
+                    viscosity_reverse(d, dv, rho[i], rho[j], mu[i], mu[j], c[i], c[j], m[j], h, wi);
+                position = d3_add(position, own.displacement);
+                velocity = d3_add(velocity, own.velocity);
+                rho_c += own.rho_a;
+                mu_c += own.mu_a;
+                c_c += own.c_a;
+                const viscosity_adjoint_t other =
+                    viscosity_reverse(v3_scale(d, K(-1.0)), v3_scale(dv, K(-1.0)), rho[j], rho[i], mu[j], mu[i], c[j],
+                                      c[i], m[i], h, d3_cload(acc_a, j));
+                position = d3_sub(position, other.displacement);
+                velocity = d3_sub(velocity, other.velocity);
+                mass_c += other.mass_b;
+                rho_c += other.rho_b;
+                mu_c += other.mu_b;
+                c_c += other.c_b;
+            }
+            for (uint32_t s = cr[k].bfirst; s < cr[k].blast; ++s) {
+                const uint32_t b = g->bidx[s];
+                const v3 d = v3_sub(p, boundary_position(g, b));
+                const v3 dv = v3_sub(v, boundary_velocity(g, b));
+                const real vol = boundary_volume(g, b);
+                /* the boundary term's nu = mu_i h c_i / rho_i is the fluid
+                 * form with both ends equal to particle i (wcsph.cu:290) */
+                const viscosity_adjoint_t own =
+                    viscosity_reverse(d, dv, rho[i], rho[i], mu[i], mu[i], c[i], c[i], rest[i] * vol, h, wi);
+                position = d3_add(position, own.displacement);
+                velocity = d3_add(velocity, own.velocity);
+                rest_c += own.mass_b * vol;
+                rho_c += own.rho_a + own.rho_b;
+                mu_c += own.mu_a + own.mu_b;
+                c_c += own.c_a + own.c_b;
+            }
+        }
+        d3_acc(pos_a, i, position);
+        d3_acc(vel_a, i, velocity);
+        rho_a[i] += rho_c;
+        prm_a[WS_MASS][i] += mass_c;
+        prm_a[WS_REST][i] += rest_c;
+        prm_a[WS_VISC][i] += mu_c;
+        prm_a[WS_SPEED][i] += c_c;
+    }
+}
+
+/* surface_reverse, wcsph.cu:74-88: the adjoint of one cohesion term
+ * -s m_b w(d) / m_a d for the acceleration adjoint w of its centre */
+typedef struct {
+    d3 displacement;
+    double mass_a, mass_b, tension;
+} surface_adjoint_t;
+
+static surface_adjoint_t surface_reverse(v3 d, real mass_a, real mass_b, real tension, real h, real diameter, d3 w) {
+    surface_adjoint_t r;
+    memset(&r, 0, sizeof(r));
+    const real dist = v3_length(d);
+    if (dist >= h) return r;
+    const real weight = cohesion_weight(d, h, diameter);
+    const double factor = -(double)tension * mass_b * weight / mass_a;
+    const double factor_a = d3_dot(w, d3_of(d));
+    r.displacement = d3_scale(w, factor);
+    if (dist > diameter)
+        r.displacement = d3_add(r.displacement, d3_scale(d3_of(cubic_gradient(d, h)),
+                                                         -(double)tension * mass_b * factor_a / mass_a));
+    r.tension = -(double)mass_b * weight * factor_a / mass_a;
+    r.mass_b = -(double)tension * weight * factor_a / mass_a;
+    r.mass_a = (double)tension * mass_b * weight * factor_a / ((double)mass_a * mass_a);
+    return r;
+}
+
+/* surface_vjp_kernel, wcsph.cu:388-435, gather form */
+static void surface_vjp_kernel(const grid_t* g, const real* const pos[3], const real* const prm[WS_PARAMS],
+                               const double* const acc_a[3], double* const pos_a[3],
+                               double* const prm_a[WS_PARAMS]) {
+    const real h = (real)g->d->support_radius;
+    const real diameter = K(2.0) * (real)g->d->particle_radius;
+    const real *m = prm[WS_MASS], *rest = prm[WS_REST], *st = prm[WS_TENSION], *bt = prm[WS_BTENSION];
+#pragma omp parallel for schedule(static)
+    for (int64_t ii = 0; ii < (int64_t)g->n; ++ii) {
+        const uint32_t i = (uint32_t)ii;
+        const v3 p = v3_load(pos, i);
+        const d3 wi = d3_cload(acc_a, i);
+        d3 position = d3_make(0, 0, 0);
+        double mass_c = 0.0, rest_c = 0.0, tension_c = 0.0, boundary_c = 0.0;
+        cell_range cr[27];
+        const int nr = stencil(g, p, cr);
+        for (int k = 0; k < nr; ++k) {
+            for (uint32_t s = cr[k].first; s < cr[k].last; ++s) {
+                const uint32_t j = g->idx[s];
+                if (j == i) continue;
+                const v3 d = v3_sub(p, v3_load(pos, j));
+                const surface_adjoint_t own = surface_reverse(d, m[i], m[j], st[i], h, diameter, wi);
+                position = d3_add(position, own.displacement);
+                mass_c += own.mass_a;
+                tension_c += own.tension;
+                const surface_adjoint_t other =
+                    surface_reverse(v3_scale(d, K(-1.0)), m[j], m[i], st[j], h, diameter, d3_cload(acc_a, j));
+                position = d3_sub(position, other.displacement);
+                mass_c += other.mass_b;
+            }
+            for (uint32_t s = cr[k].bfirst; s < cr[k].blast; ++s) {
+                const uint32_t b = g->bidx[s];
+                const v3 d = v3_sub(p, boundary_position(g, b));
+                const real vol = boundary_volume(g, b);
+                const surface_adjoint_t own = surface_reverse(d, m[i], rest[i] * vol, bt[i], h, diameter, wi);
+                position = d3_add(position, own.displacement);
+                mass_c += own.mass_a;
+                rest_c += own.mass_b * vol;
+                boundary_c += own.tension;
+            }
+        }
+        d3_acc(pos_a, i, position);
+        prm_a[WS_MASS][i] += mass_c;
+        prm_a[WS_REST][i] += rest_c;
+        prm_a[WS_TENSION][i] += tension_c;
+        prm_a[WS_BTENSION][i] += boundary_c;
+    }
+}
+
+void SUF(oracle_wcsph_vjp)(const oracle_domain* d, uint64_t step, const real* pos[3], const real* vel[3],
+                           const real* ctl[3], const real* const prm[WS_PARAMS], const double* next_pos_a[3],
+                           const double* next_vel_a[3], double* pos_a[3], double* vel_a[3], double* ctl_a[3],
+                           double* const prm_a[WS_PARAMS]) {
+    const uint32_t n = d->particle_count;
+    const real dt = (real)d->time_step;
+    const cbox box = collision_box(d, step);
+    grid_t g;
+    grid_build(&g, d, step, pos[0], pos[1], pos[2]);
+    const real *m = prm[WS_MASS], *rest = prm[WS_REST];
+    real* rho = alloc_real(n);
+    real* p = alloc_real(n);
+    vfield ap = alloc_vfield(n), av = alloc_vfield(n), as = alloc_vfield(n), total = alloc_vfield(n),
+           zero = alloc_vfield(n);
+    for (int c = 0; c < 3; ++c) memset(zero.c[c], 0, sizeof(real) * n);
+    density_kernel(&g, 0, pos, pos, m, rest, rho);
+    eos_kernel(n, rho, rest, prm[WS_SPEED], prm[WS_EXP], p);
+    pressure_kernel(&g, pos, m, rest, rho, p, ap.c);
+    artificial_viscosity_kernel(&g, pos, vel, m, rest, prm[WS_VISC], prm[WS_SPEED], rho, av.c);
+    surface_kernel(&g, pos, m, rest, prm[WS_TENSION], prm[WS_BTENSION], as.c);
+    add_kernel(n, CV(ap), CV(av), total.c);
+    add_kernel(n, CV(total), CV(as), total.c);
+    const v3 gravity = v3_make((real)d->gravity[0], (real)d->gravity[1], (real)d->gravity[2]);
+    for (uint32_t i = 0; i < n; ++i) v3_store(ap.c, i, v3_add(gravity, v3_load(ctl, i)));
+    add_kernel(n, CV(total), CV(ap), total.c);
+    /* sph.cpp:241-248 */
+    dfield total_a = dfield_zeros(n), scratch = dfield_zeros(n);
+    double* rho_a = dzeros(n);
+    double* p_a = dzeros(n);
+    predict_vjp_kernel(n, dt, box, pos, vel, CV(total), CV(zero), next_pos_a, next_vel_a, pos_a, vel_a, total_a.c,
+                       scratch.c);
+    for (int c = 0; c < 3; ++c)
+        for (uint32_t i = 0; i < n; ++i) ctl_a[c][i] += total_a.c[c][i];
+    surface_vjp_kernel(&g, pos, prm, DV(total_a), pos_a, prm_a);
+    artificial_viscosity_vjp_kernel(&g, pos, vel, prm, rho, DV(total_a), pos_a, vel_a, rho_a, prm_a);
+    pressure_vjp_kernel(&g, pos, m, rest, rho, p, DV(total_a), pos_a, rho_a, p_a, prm_a[WS_MASS], prm_a[WS_REST]);
+    eos_vjp_kernel(n, rho, rest, prm[WS_SPEED], prm[WS_EXP], p_a, rho_a, prm_a[WS_REST], prm_a[WS_SPEED],
+                   prm_a[WS_EXP]);
+    /* sph.ixx:364: the state densities */
+    density_vjp_kernel(&g, pos, pos, m, rest, rho_a, pos_a, prm_a[WS_MASS], prm_a[WS_REST]);
+    free(rho);
+    free(p);
+    free(rho_a);
+    free(p_a);
+    free_vfield(&ap);
+    free_vfield(&av);
+    free_vfield(&as);
+    free_vfield(&total);
+    free_vfield(&zero);
+    dfield_free(&total_a);
+    dfield_free(&scratch);
+    grid_free(&g);
+}
+
 #ifndef ORACLE_F64
 int oracle_threads(void) {
 #ifdef _OPENMP
