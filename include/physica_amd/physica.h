@@ -457,22 +457,23 @@ typedef struct phx_sph_parameter_adjoint {
     phx_sph_method_parameter_adjoint method;
 } phx_sph_parameter_adjoint;
 
-/* SPH<PCISPH|IISPH>::jvp_step (sph/sph.ixx:356-359; PCISPH::jvp
- * sph/sph.cpp:312-331, IISPH::jvp sph/sph.cpp:423-442). Needs a solver
- * created in PHX_MODE_DIFFERENTIABLE (PCISPH or IISPH; WCSPH derivatives
- * return PHX_ERROR_UNSUPPORTED) and the cache of that step's forward_step,
- * which records the step. FP32 tangents overwrite next_state_tangent; null
- * parameter-tangent arrays count as zero. */
+/* SPH<WCSPH|PCISPH|IISPH>::jvp_step (sph/sph.ixx:356-359; WCSPH::jvp
+ * sph/sph.cpp:228-237, PCISPH::jvp sph/sph.cpp:312-331, IISPH::jvp
+ * sph/sph.cpp:423-442). Needs a solver created in PHX_MODE_DIFFERENTIABLE and
+ * the cache of that step's forward_step, which records the step. FP32
+ * tangents overwrite next_state_tangent; null parameter-tangent arrays count
+ * as zero. */
 phx_status phx_sph_jvp_step(phx_sph* solver, const phx_sph_step_cache* cache,
                             const phx_sph_state_tangent* state_tangent, const phx_control_tangent* control_tangent,
                             const phx_sph_parameter_tangent* parameter_tangent,
                             phx_sph_state_tangent* next_state_tangent);
-/* SPH<PCISPH|IISPH>::vjp_step (sph/sph.ixx:361-365; PCISPH::vjp
- * sph/sph.cpp:333-363, IISPH::vjp sph/sph.cpp:444-476): FP64 adjoints of the
- * recorded step (neighbour graph, collision masks and the active sets of the
- * max(0, .) updates frozen), accumulated (+=) into the outputs; null outputs
- * are skipped. The prediction's VJP under a no-slip collision is the exact
- * transpose of its JVP (see phx_dfsph_vjp_step). */
+/* SPH<WCSPH|PCISPH|IISPH>::vjp_step (sph/sph.ixx:361-365; WCSPH::vjp
+ * sph/sph.cpp:239-249, PCISPH::vjp sph/sph.cpp:333-363, IISPH::vjp
+ * sph/sph.cpp:444-476): FP64 adjoints of the recorded step (neighbour graph,
+ * collision masks, the compression test of every artificial-viscosity pair
+ * and the active sets of the max(0, .) updates frozen), accumulated (+=) into
+ * the outputs; null outputs are skipped. The integration's VJP under a no-slip
+ * collision is the exact transpose of its JVP (see phx_dfsph_vjp_step). */
 phx_status phx_sph_vjp_step(phx_sph* solver, const phx_sph_step_cache* cache,
                             const phx_sph_state_adjoint* next_state_adjoint,
                             phx_sph_state_adjoint* previous_state_adjoint, phx_control_adjoint* control_adjoint,

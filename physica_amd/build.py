@@ -17,7 +17,8 @@ ROOT = PACKAGE.parent
 CSRC = PACKAGE / "csrc"
 LIBRARY = PACKAGE / "libphysica_amd.so"
 
-SOURCES = ["grid.hip", "masks.hip", "dfsph.hip", "dfsph_adjoint.hip", "sph.hip", "pbf.hip", "pbf_adjoint.hip", "capi.hip"]
+SOURCES = ["grid.hip", "masks.hip", "dfsph.hip", "dfsph_adjoint.hip", "sph.hip", "wcsph_adjoint.hip", "pbf.hip",
+           "pbf_adjoint.hip", "capi.hip"]
 
 # -ffp-contract=off keeps every multiply and add a separate IEEE operation
 # (the kernels reproduce the reference's expressions operation by operation);

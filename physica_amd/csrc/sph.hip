@@ -43,6 +43,7 @@ struct WcsphForceArgs {
     const uint32_t* flags;
     const uint32_t* perm;
     Vec next_x, next_v;
+    uint8_t* collisions;  // optional (recorded step): box-collision bits of the integration
 };
 
 // pressure_forward_kernel (common.cu:165-194), viscosity_forward_kernel
@@ -110,6 +111,7 @@ __global__ void __launch_bounds__(kBlock) wcsph_force_kernel(WcsphForceArgs a) {
     total = add(total, add(a.gravity, load(a.u, i)));
     Float3 nx, nv;
     predict_clamped(a.box, a.dt, p, vel, total, nx, nv);
+    if (a.collisions != nullptr) a.collisions[i] = static_cast<uint8_t>(predict_collisions(a.box, a.dt, p, vel, total));
     const uint32_t j = a.perm[i];
     store(a.next_x, j, nx);
     store(a.next_v, j, nv);
@@ -161,8 +163,6 @@ SphSolver::SphSolver(const phx_domain_configuration* domain, const phx_sph_confi
             PHX_ERROR_INVALID_ARGUMENT, "unknown SPH method");
     require(mode == PHX_MODE_FORWARD || mode == PHX_MODE_DIFFERENTIABLE, PHX_ERROR_INVALID_ARGUMENT,
             "unknown execution mode");
-    require(mode == PHX_MODE_FORWARD || config_.method != PHX_SPH_WCSPH, PHX_ERROR_UNSUPPORTED,
-            "WCSPH derivatives are not supported yet");
     if (config_.method != PHX_SPH_WCSPH) {
         require(config_.checkpoint_interval > 0u, PHX_ERROR_INVALID_ARGUMENT, "checkpoint_interval must be positive");
         require(config_.pressure_iterations < 4096u, PHX_ERROR_INVALID_ARGUMENT, "too many pressure iterations");
@@ -176,7 +176,7 @@ SphSolver::SphSolver(const phx_domain_configuration* domain, const phx_sph_confi
     u_.resize(n);
     for (DeviceBuffer<float>* b : {&rho0_, &relax_, &tait_, &t_}) b->resize(n);
     masks_.allocate(n, config_.pressure_iterations + 1u, domain_.config.support_radius);
-    if (mode == PHX_MODE_DIFFERENTIABLE) {
+    if (mode == PHX_MODE_DIFFERENTIABLE && config_.method != PHX_SPH_WCSPH) {
         // PCISPH::forward (sph.cpp:295-309) and IISPH::forward (sph.cpp:408-421)
         // are the DFSPH step with nd = 0, no warm start and their relaxation in
         // the density phase: the divergence phase then adds a zero
@@ -194,15 +194,24 @@ SphSolver::SphSolver(const phx_domain_configuration* domain, const phx_sph_confi
 }
 
 void SphSolver::require_derivatives(const SphCache& cache) const {
-    require(engine_ != nullptr, PHX_ERROR_UNSUPPORTED, "derivatives need a solver created in differentiable mode");
-    require(cache.record != nullptr && cache.method == config_.method, PHX_ERROR_INVALID_ARGUMENT,
+    require(mode_ == PHX_MODE_DIFFERENTIABLE, PHX_ERROR_UNSUPPORTED,
+            "derivatives need a solver created in differentiable mode");
+    const bool recorded = config_.method == PHX_SPH_WCSPH ? cache.wcsph != nullptr && cache.wcsph->recorded
+                                                          : cache.record != nullptr && cache.record->recorded;
+    require(recorded && cache.method == config_.method, PHX_ERROR_INVALID_ARGUMENT,
             "the step cache holds no step recorded by this solver");
+    require(cache.grid().particles == domain_.config.particle_count, PHX_ERROR_INVALID_ARGUMENT,
+            "the step cache belongs to another particle count");
 }
 
 void SphSolver::jvp_step(const SphCache& cache, const phx_sph_state_tangent& state_t,
                          const phx_control_tangent& control_t, const phx_sph_parameter_tangent& parameter_t,
                          phx_sph_state_tangent& next_t) {
     require_derivatives(cache);
+    if (config_.method == PHX_SPH_WCSPH) {
+        wcsph_jvp_step(cache, state_t, control_t, parameter_t, next_t);
+        return;
+    }
     const bool iisph = config_.method == PHX_SPH_IISPH;
     phx_sph_dfsph_state_tangent st{state_t.particles, {nullptr, nullptr}};
     phx_sph_dfsph_state_tangent nt{next_t.particles, {nullptr, nullptr}};
@@ -216,6 +225,10 @@ void SphSolver::vjp_step(const SphCache& cache, const phx_sph_state_adjoint& nex
                          phx_sph_state_adjoint& previous_a, phx_control_adjoint& control_a,
                          phx_sph_parameter_adjoint& parameter_a) {
     require_derivatives(cache);
+    if (config_.method == PHX_SPH_WCSPH) {
+        wcsph_vjp_step(cache, next_a, previous_a, control_a, parameter_a);
+        return;
+    }
     const bool iisph = config_.method == PHX_SPH_IISPH;
     phx_sph_dfsph_state_adjoint na{next_a.particles, {nullptr, nullptr}};
     phx_sph_dfsph_state_adjoint pa{previous_a.particles, {nullptr, nullptr}};
@@ -273,16 +286,35 @@ void SphSolver::forward_wcsph(const phx_sph_state& state, const phx_control& con
     const auto& mp = parameters.method;
     require(pp.surface_tensions && mp.speed_of_sound && mp.tait_exponent && mp.boundary_surface_tension,
             PHX_ERROR_INVALID_ARGUMENT, "null WCSPH parameters");
+    // a recorded step keeps its sorted inputs, masks and collision bits in the cache
+    const bool rec = mode_ == PHX_MODE_DIFFERENTIABLE;
+    WcsphRecord* r = nullptr;
+    if (rec) {
+        if (!cache.wcsph) cache.wcsph = std::make_unique<WcsphRecord>();
+        r = cache.wcsph.get();
+        r->recorded = false;
+        for (DeviceBuffer<float4>* b : {&r->xm, &r->vmu, &r->aux}) b->resize(n);
+        for (DeviceBuffer<float>* b : {&r->rho0, &r->tait}) b->resize(n);
+        r->collisions.resize(n);
+        r->exact.resize(static_cast<size_t>(kStencil) * n);
+        r->flags.resize(masks_.flag_count());
+    }
+    float4* xm = rec ? r->xm.data() : xm_.data();
+    float4* vmu = rec ? r->vmu.data() : vmu_.data();
+    float4* aux = rec ? r->aux.data() : aux_.data();
+    float* rho0 = rec ? r->rho0.data() : rho0_.data();
+    float* tait = rec ? r->tait.data() : tait_.data();
+    uint16_t* exact = rec ? r->exact.data() : masks_.exact();
     ChannelList gather;
-    gather.add(cvec(state.particles.positions), xm_.data());
-    gather.add(pp.masses, xm_.data(), 3);
-    gather.add(cvec(state.particles.velocities), vmu_.data());
-    gather.add(pp.viscosities, vmu_.data(), 3);
+    gather.add(cvec(state.particles.positions), xm);
+    gather.add(pp.masses, xm, 3);
+    gather.add(cvec(state.particles.velocities), vmu);
+    gather.add(pp.viscosities, vmu, 3);
     gather.add(cvec(control.external_accelerations), u_.view());
-    gather.add(pp.rest_densities, rho0_.data());
-    gather.add(mp.speed_of_sound, aux_.data(), 2);
-    gather.add(pp.surface_tensions, aux_.data(), 3);
-    gather.add(mp.tait_exponent, tait_.data());
+    gather.add(pp.rest_densities, rho0);
+    gather.add(mp.speed_of_sound, aux, 2);
+    gather.add(pp.surface_tensions, aux, 3);
+    gather.add(mp.tait_exponent, tait);
     {
         PassScope scope(profiler_, stream_, kSphGrid);
         cache.neighborhood.build(stream_, domain_, step, cvec(state.particles.positions), nullptr, gather.get());
@@ -296,15 +328,15 @@ void SphSolver::forward_wcsph(const phx_sph_state& state, const phx_control& con
     ma.kc = constants_;
     ma.t_skin = masks_.t_skin();
     ma.grid = grid;
-    ma.xm = xm_.data();
-    ma.exact = masks_.exact();
+    ma.xm = xm;
+    ma.exact = exact;
     ma.skin = masks_.skin_mask();
     ma.rho = cache.densities.data();
     ma.flags = masks_.flags();
-    ma.rest_density = rho0_.data();
-    ma.tait_exponent = tait_.data();
+    ma.rest_density = rho0;
+    ma.tait_exponent = tait;
     ma.pressures = cache.pressures.data();
-    ma.aux = aux_.data();
+    ma.aux = aux;
     launch_mask_density<true>(profiler_, stream_, blocks, ma, kSphDensity);
 
     WcsphForceArgs wa{};
@@ -315,19 +347,26 @@ void SphSolver::forward_wcsph(const phx_sph_state& state, const phx_control& con
     wa.grid = grid;
     wa.box = collision_box(c, step);
     wa.gravity = Float3{c.gravity.x, c.gravity.y, c.gravity.z};
-    wa.xm = xm_.data();
-    wa.vmu = vmu_.data();
-    wa.aux = aux_.data();
+    wa.xm = xm;
+    wa.vmu = vmu;
+    wa.aux = aux;
     wa.u = u_.cview();
-    wa.exact = masks_.exact();
+    wa.exact = exact;
     wa.flags = masks_.flags();
     wa.perm = cache.neighborhood.perm.data();
     wa.next_x = vec(next.particles.positions);
     wa.next_v = vec(next.particles.velocities);
+    wa.collisions = rec ? r->collisions.data() : nullptr;
     {
         PassScope scope(profiler_, stream_, kSphForces);
         wcsph_force_kernel<<<blocks, kBlock, 0, stream_>>>(wa);
         PHX_LAUNCH_CHECK();
+    }
+    if (rec) {
+        PHX_HIP(hipMemcpyAsync(r->flags.data(), masks_.flags(), sizeof(uint32_t) * masks_.flag_count(),
+                               hipMemcpyDeviceToDevice, stream_));
+        r->step_index = step;
+        r->recorded = true;
     }
     masks_.end_step(stream_);
 }

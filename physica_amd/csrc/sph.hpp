@@ -11,6 +11,30 @@
 
 namespace phx {
 
+// A WCSPH step recorded in differentiable mode, in sorted order: the inputs
+// the derivatives read, the exact masks with their flags and the
+// integration's collision bits (densities and pressures stay in SphCache).
+struct WcsphRecord {
+    bool recorded = false;
+    uint64_t step_index = 0;
+    DeviceBuffer<float4> xm, vmu, aux;  // aux: density, p / rho^2, speed of sound, surface tension
+    DeviceBuffer<float> rho0, tait;
+    DeviceBuffer<uint8_t> collisions;
+    DeviceBuffer<uint16_t> exact;
+    DeviceBuffer<uint32_t> flags;
+};
+
+// Sorted-order tangent and adjoint workspace of the WCSPH derivative passes.
+struct WcsphDerivativeWorkspace {
+    DeviceBuffer<float4> xt, vt, ut, pt;  // (x', m'), (v', mu'), (u', rho0'), (c', gamma', sigma')
+    DeviceBuffer<float> rho_t;
+    DeviceBuffer<float2> tt;  // (p / rho^2, its tangent)
+    DVecBuffer pos_a, vel_a, total_a;
+    DeviceBuffer<double> rho_a, m_a, rest_a, visc_a, tension_a, speed_a, tait_a;
+    void allocate_tangents(uint32_t n);
+    void allocate_adjoints(uint32_t n);
+};
+
 // StepCache of SPH<Method> (sph/sph.ixx:299-303) with the method caches used
 // here: densities, pressures (WCSPH::Cache::pressures, or the last iteration
 // of PCISPH/IISPH) and the non-pressure accelerations (PCISPH/IISPH), all in
@@ -25,6 +49,8 @@ struct SphCache {
     // differentiable mode (PCISPH, IISPH): the step recorded by the
     // projection engine, which then holds every field above
     std::unique_ptr<DfsphCache> record;
+    // differentiable mode (WCSPH)
+    std::unique_ptr<WcsphRecord> wcsph;
     void allocate(uint32_t n, uint32_t cells);
 
     const Neighborhood& grid() const { return record ? record->neighborhood : neighborhood; }
@@ -79,6 +105,11 @@ class SphSolver {
                           const phx_sph_parameters& parameters, phx_sph_state& next, SphCache& cache);
 
     void require_derivatives(const SphCache& cache) const;
+    void wcsph_jvp_step(const SphCache& cache, const phx_sph_state_tangent& state_t,
+                        const phx_control_tangent& control_t, const phx_sph_parameter_tangent& parameter_t,
+                        phx_sph_state_tangent& next_t);
+    void wcsph_vjp_step(const SphCache& cache, const phx_sph_state_adjoint& next_a, phx_sph_state_adjoint& previous_a,
+                        phx_control_adjoint& control_a, phx_sph_parameter_adjoint& parameter_a);
 
     Domain domain_;
     phx_sph_configuration config_{};
@@ -89,6 +120,7 @@ class SphSolver {
     // records and differentiates exactly these steps
     std::unique_ptr<DfsphSolver> engine_;
     DeviceBuffer<float> engine_warm_[2];  // its (unused) warm-start outputs
+    WcsphDerivativeWorkspace wdw_;
     float gradient_norm_ = 0.0F;
     KernelConstants constants_{};
     PassProfiler profiler_;

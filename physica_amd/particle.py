@@ -416,6 +416,38 @@ class WCSPH:
                 _ptr(self.speed_of_sound), _ptr(self.tait_exponent), _ptr(self.boundary_surface_tension), None, None
             )
 
+    @dataclass
+    class StateTangent:
+        """Empty (sph/sph.ixx:42)."""
+
+    @dataclass
+    class StateAdjoint:
+        """Empty (sph/sph.ixx:43)."""
+
+    @dataclass
+    class ParameterTangent:
+        """FP32 (sph/sph.ixx:51-55); None counts as zero."""
+
+        speed_of_sound: Optional[torch.Tensor]
+        tait_exponent: Optional[torch.Tensor]
+        boundary_surface_tension: Optional[torch.Tensor]
+
+        def to_c(self) -> _capi.SphMethodParameterTangent:
+            return _capi.SphMethodParameterTangent(_ptr(self.speed_of_sound), _ptr(self.tait_exponent),
+                                                   _ptr(self.boundary_surface_tension), None, None)
+
+    @dataclass
+    class ParameterAdjoint:
+        """FP64 (sph/sph.ixx:57-61); None is skipped."""
+
+        speed_of_sound: Optional[torch.Tensor]
+        tait_exponent: Optional[torch.Tensor]
+        boundary_surface_tension: Optional[torch.Tensor]
+
+        def to_c(self) -> _capi.SphMethodParameterAdjoint:
+            return _capi.SphMethodParameterAdjoint(_ptr(self.speed_of_sound), _ptr(self.tait_exponent),
+                                                   _ptr(self.boundary_surface_tension), None, None)
+
 
 class PCISPH:
     """PCISPH method types, sph/sph.ixx:103-147."""
@@ -721,11 +753,9 @@ class SPH:
         """Per-pass totals since profiling was enabled or last read."""
         return _read_profile(self._prefix + "_read_profile", self._handle)
 
-    # tangents and adjoints (sph/sph.ixx:313-348): DFSPH, PCISPH, IISPH;
-    # allocated zero
+    # tangents and adjoints (sph/sph.ixx:313-348), allocated zero
     def _require_derivatives(self) -> None:
-        if self.method is WCSPH:
-            raise NotImplementedError("derivatives of SPH<WCSPH> are not provided")
+        pass
 
     def allocate_state_tangent(self) -> "SPH.StateTangent":
         self._require_derivatives()
@@ -743,7 +773,12 @@ class SPH:
         self._require_derivatives()
         d = self.domain
         z = lambda: torch.zeros(d.configuration.particle_count, dtype=torch.float32, device=d.device)  # noqa: E731
-        method = DFSPH.ParameterTangent(z(), z()) if self.method is DFSPH else self.method.ParameterTangent(z())
+        if self.method is DFSPH:
+            method = DFSPH.ParameterTangent(z(), z())
+        elif self.method is WCSPH:
+            method = WCSPH.ParameterTangent(z(), z(), z())
+        else:
+            method = self.method.ParameterTangent(z())
         return SPH.ParameterTangent(ParticleParameterTangent(z(), z(), z(), z()), method)
 
     def allocate_state_adjoint(self) -> "SPH.StateAdjoint":
@@ -760,7 +795,12 @@ class SPH:
     def allocate_parameter_adjoint(self) -> "SPH.ParameterAdjoint":
         self._require_derivatives()
         z = self.domain.allocate_scalar_adjoint_field
-        method = DFSPH.ParameterAdjoint(z(), z()) if self.method is DFSPH else self.method.ParameterAdjoint(z())
+        if self.method is DFSPH:
+            method = DFSPH.ParameterAdjoint(z(), z())
+        elif self.method is WCSPH:
+            method = WCSPH.ParameterAdjoint(z(), z(), z())
+        else:
+            method = self.method.ParameterAdjoint(z())
         return SPH.ParameterAdjoint(ParticleParameterAdjoint(z(), z(), z(), z()), method)
 
     def copy_state_tangent(self, source: "SPH.StateTangent", destination: "SPH.StateTangent") -> None:
@@ -787,7 +827,7 @@ class SPH:
     def jvp_step(self, state, control, parameters, next_state, cache, state_tangent, control_tangent,
                  parameter_tangent, next_state_tangent) -> None:
         """SPH<Method>::jvp_step (sph/sph.ixx:356-359) of the step recorded in
-        `cache` (DFSPH, PCISPH, IISPH): state, control, parameters and
+        `cache`: state, control, parameters and
         next_state are those of that forward step, which the cache recorded."""
         self._require_derivatives()
         _capi.call(

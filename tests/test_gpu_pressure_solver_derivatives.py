@@ -13,7 +13,7 @@ import pytest
 import torch
 
 import oracle
-from physica_amd.particle import ExecutionMode, SPH, WCSPH
+from physica_amd.particle import ExecutionMode
 from helpers import bits_equal, sph_setup
 from test_oracle_pressure_solver_derivatives import PARAMS, STATE, directions, inputs, setup
 
@@ -148,11 +148,9 @@ def test_gpu_dot_product_identity_and_accumulation(method):
                        rtol=1e-12, atol=0)
 
 
-def test_wcsph_and_forward_mode_refuse_derivatives():
+def test_forward_mode_refuses_derivatives():
     dev = torch.device("cuda:0")
     cs = setup("pcisph")
-    with pytest.raises(RuntimeError, match="WCSPH derivatives"):
-        SPH(cs.sc.domain, WCSPH.Configuration(), mode=ExecutionMode.differentiable, device=dev)
     solver, state, nxt, control, params, cache = sph_setup(cs.sc, dev, "iisph", 2)
     solver.forward_step(state, control, params, nxt, cache)
     st, ct, pt, nt = (solver.allocate_state_tangent(), solver.allocate_control_tangent(),
