@@ -243,6 +243,36 @@ def dfsph_step(domain, step_index, positions, velocities, controls, params: Dict
             "densities": rho, "non_pressure_accelerations": anp}
 
 
+def dfsph_step_tolerance(domain, step_index, positions, velocities, controls, params: Dict[str, np.ndarray],
+                         warm_divergence, warm_density, tolerance=1e-3, max_iterations=100, warm_start=True,
+                         precision="f32"):
+    """SPH<DFSPH>::forward_step in tolerance mode (an extension of the
+    reference's fixed counts). Returns the next state and "iterations", the
+    executed (divergence, density) counts."""
+    n = domain.particle_count
+    pos, pp = _vec(positions, precision)
+    vel, vp = _vec(velocities, precision)
+    ctl, cp = _vec(controls, precision)
+    arrs = {k: _arr(params[k], precision) for k in
+            ("masses", "rest_densities", "viscosities", "divergence_relaxation", "density_relaxation")}
+    wd = _arr(warm_divergence, precision)
+    we = _arr(warm_density, precision)
+    nx, nxp = _out_vec(n, precision)
+    nv, nvp = _out_vec(n, precision)
+    nwd = np.zeros(n, dtype=_dtype(precision))
+    nwe = np.zeros(n, dtype=_dtype(precision))
+    its = (ctypes.c_uint32 * 2)()
+    getattr(library(), "oracle_dfsph_step_tolerance_" + precision)(
+        ctypes.byref(domain), ctypes.c_double(tolerance), c_uint32(max_iterations), c_int(1 if warm_start else 0),
+        c_uint64(step_index), pp, vp, cp,
+        *(c_void_p(arrs[k].ctypes.data) for k in ("masses", "rest_densities", "viscosities",
+                                                   "divergence_relaxation", "density_relaxation")),
+        c_void_p(wd.ctypes.data), c_void_p(we.ctypes.data), nxp, nvp, c_void_p(nwd.ctypes.data),
+        c_void_p(nwe.ctypes.data), its)
+    return {"positions": nx, "velocities": nv, "warm_divergence_pressure": nwd, "warm_density_pressure": nwe,
+            "iterations": (int(its[0]), int(its[1]))}
+
+
 def pressure_solver_step(domain, step_index, positions, velocities, controls, params: Dict[str, np.ndarray],
                          method="pcisph", pressure_iterations=6, precision="f32"):
     """One SPH<PCISPH> or SPH<IISPH>::forward_step (params["relaxation"] is the

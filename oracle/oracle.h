@@ -132,6 +132,18 @@ void oracle_set_threads(int threads);
                              double* rest_densities_adjoint, double* relaxation_adjoint, double* strength_adjoint,      \
                              double* exponent_adjoint, double* radius_adjoint, double* xsph_adjoint,                    \
                              double* confinement_adjoint);                                                              \
+    /* SPH<DFSPH>::forward_step in tolerance mode (an extension): each phase iterates until the FP64 mean          \
+     * compression error (1/N) sum max(0, rho* - target) / rho0 of its prediction is <= tolerance or max_iterations  \
+     * is reached; iterations_out receives the executed (divergence, density) counts */                               \
+    void oracle_dfsph_step_tolerance##suf(const oracle_domain* domain, double tolerance, uint32_t max_iterations,     \
+                                          int warm_start, uint64_t step_index, const real* positions[3],               \
+                                          const real* velocities[3], const real* controls[3], const real* masses,     \
+                                          const real* rest_densities, const real* viscosities,                         \
+                                          const real* divergence_relaxation, const real* density_relaxation,          \
+                                          const real* warm_divergence, const real* warm_density,                       \
+                                          real* next_positions[3], real* next_velocities[3],                           \
+                                          real* next_warm_divergence, real* next_warm_density,                         \
+                                          uint32_t iterations_out[2]);                                                  \
     /* SPH<DFSPH>::jvp_step (sph.ixx:356-359, sph.cpp:612-659). parameters: masses, rest densities, viscosities,      \
      * divergence relaxation, density relaxation. volume_rest = 1 takes V = m / rho0 in the pressure update: with     \
      * divergence_iterations = 0 and no warm start the step is PCISPH's (volume_rest = 1, sph.cpp:312-331) or        \

@@ -627,6 +627,84 @@ void SUF(oracle_dfsph_step)(const oracle_domain* d, uint32_t nd, uint32_t ne, in
     grid_free(&g);
 }
 
+/* Tolerance mode (an extension: the reference's DFSPH runs fixed iteration
+ * counts, sph.cpp:580, 597). Each phase iterates until the mean compression
+ * error of its prediction, (1/N) sum_i max(0, rho*_i - target_i) / rho0_i
+ * summed in FP64, is at most `tolerance`, or `max_iterations` is reached;
+ * the update of the iteration that meets it is still applied. The error is
+ * that of the pressures the projection starts from (the targets: rho in the
+ * divergence phase, rho0 in the density phase), and the pressures clamp at
+ * zero, so expansion is not counted. */
+static double compression_error(uint32_t n, const real* predicted, const real* targets, const real* rest) {
+    double sum = 0.0;
+    for (uint32_t i = 0; i < n; ++i) sum += fmax(0.0, (double)predicted[i] - (double)targets[i]) / (double)rest[i];
+    return n > 0 ? sum / n : 0.0;
+}
+
+/* SPH<DFSPH>::forward_step in tolerance mode; iterations_out receives the
+ * executed (divergence, density) counts. The result equals oracle_dfsph_step
+ * with those counts. */
+void SUF(oracle_dfsph_step_tolerance)(const oracle_domain* d, double tolerance, uint32_t max_iterations,
+                                      int warm_start, uint64_t step, const real* pos[3], const real* vel[3],
+                                      const real* ctl[3], const real* masses, const real* rest_densities,
+                                      const real* viscosities, const real* div_relax, const real* den_relax,
+                                      const real* warm_div, const real* warm_den, real* next_pos[3],
+                                      real* next_vel[3], real* next_warm_div, real* next_warm_den,
+                                      uint32_t iterations_out[2]) {
+    const uint32_t n = d->particle_count;
+    const real dt = (real)d->time_step;
+    const real G = SUF(oracle_reference_gradient_norm)(d);
+    const cbox box = collision_box(d, step);
+    grid_t g;
+    grid_build(&g, d, step, pos[0], pos[1], pos[2]);
+    real* rho = alloc_real(n);
+    real* rho_star = alloc_real(n);
+    real* p = alloc_real(n);
+    vfield anp = alloc_vfield(n), ap = alloc_vfield(n), adiv = alloc_vfield(n), total = alloc_vfield(n);
+    vfield xs = alloc_vfield(n), vs = alloc_vfield(n);
+    density_kernel(&g, 0, pos, pos, masses, rest_densities, rho);
+    non_pressure_kernel(&g, pos, vel, ctl, masses, viscosities, rho, anp.c);
+    iterations_out[0] = iterations_out[1] = 0u;
+    for (uint32_t i = 0; i < n; ++i) p[i] = warm_start ? warm_div[i] : K(0.0);
+    pressure_kernel(&g, pos, masses, rest_densities, rho, p, ap.c);
+    for (uint32_t it = 1; it <= max_iterations; ++it) {
+        sph_predict_kernel(n, dt, box, pos, vel, CV(anp), CV(ap), xs.c, vs.c);
+        density_kernel(&g, 0, pos, CV(xs), masses, rest_densities, rho_star);
+        const double error = compression_error(n, rho_star, rho, rest_densities);
+        projection_update_kernel(n, dt, G, masses, rho, rho, rho_star, div_relax, p);
+        pressure_kernel(&g, pos, masses, rest_densities, rho, p, ap.c);
+        iterations_out[0] = it;
+        if (error <= tolerance) break;
+    }
+    for (int c = 0; c < 3; ++c) memcpy(adiv.c[c], ap.c[c], sizeof(real) * n);
+    memcpy(next_warm_div, p, sizeof(real) * n);
+    for (uint32_t i = 0; i < n; ++i) p[i] = warm_start ? warm_den[i] : K(0.0);
+    pressure_kernel(&g, pos, masses, rest_densities, rho, p, ap.c);
+    for (uint32_t it = 1; it <= max_iterations; ++it) {
+        add_kernel(n, CV(adiv), CV(ap), total.c);
+        sph_predict_kernel(n, dt, box, pos, vel, CV(anp), CV(total), xs.c, vs.c);
+        density_kernel(&g, 0, pos, CV(xs), masses, rest_densities, rho_star);
+        const double error = compression_error(n, rho_star, rest_densities, rest_densities);
+        projection_update_kernel(n, dt, G, masses, rho, rest_densities, rho_star, den_relax, p);
+        pressure_kernel(&g, pos, masses, rest_densities, rho, p, ap.c);
+        iterations_out[1] = it;
+        if (error <= tolerance) break;
+    }
+    add_kernel(n, CV(adiv), CV(ap), total.c);
+    memcpy(next_warm_den, p, sizeof(real) * n);
+    sph_predict_kernel(n, dt, box, pos, vel, CV(anp), CV(total), next_pos, next_vel);
+    free(rho);
+    free(rho_star);
+    free(p);
+    free_vfield(&anp);
+    free_vfield(&ap);
+    free_vfield(&adiv);
+    free_vfield(&total);
+    free_vfield(&xs);
+    free_vfield(&vs);
+    grid_free(&g);
+}
+
 /* -------------------------------------------------------- PCISPH, IISPH */
 /* SPH<PCISPH|IISPH>::forward_step (sph.ixx:350-354) = build(step, x) ->
  * density(x, x) -> PCISPH::forward (sph.cpp:296-315) or IISPH::forward

@@ -104,7 +104,13 @@ DfsphSolver::DfsphSolver(const phx_domain_configuration* domain, const phx_dfsph
     require(config_.checkpoint_interval > 0u, PHX_ERROR_INVALID_ARGUMENT, "checkpoint_interval must be positive");
     require(mode == PHX_MODE_FORWARD || mode == PHX_MODE_DIFFERENTIABLE, PHX_ERROR_INVALID_ARGUMENT,
             "unknown execution mode");
-    require(config_.density_error_tolerance == 0.0F, PHX_ERROR_UNSUPPORTED, "tolerance mode is not supported yet");
+    const float tolerance = config_.density_error_tolerance;
+    require(tolerance >= 0.0F && std::isfinite(tolerance), PHX_ERROR_INVALID_ARGUMENT,
+            "density_error_tolerance must be finite and non-negative");
+    require(tolerance == 0.0F || mode == PHX_MODE_FORWARD, PHX_ERROR_UNSUPPORTED,
+            "tolerance mode is supported in forward mode only");
+    require(tolerance == 0.0F || (config_.max_iterations >= 1u && config_.max_iterations < 2048u),
+            PHX_ERROR_INVALID_ARGUMENT, "tolerance mode needs 1 <= max_iterations < 2048");
     require(config_.divergence_iterations + config_.density_iterations < 4096u, PHX_ERROR_INVALID_ARGUMENT,
             "too many pressure iterations");
     gradient_norm_ = reference_gradient_norm(domain_.config);
@@ -113,10 +119,26 @@ DfsphSolver::DfsphSolver(const phx_domain_configuration* domain, const phx_dfsph
     for (DeviceBuffer<float4>* b : {&xm_, &vmu_, &xs_, &adiv_}) b->resize(n);
     u_.resize(n);
     for (DeviceBuffer<float>* b : {&rho0_, &relax_div_, &relax_den_, &p_div_, &p_den_, &t_div_, &t_den_}) b->resize(n);
-    masks_.allocate(n, config_.divergence_iterations + config_.density_iterations + 1u, domain_.config.support_radius);
+    // one displacement slot per predicting pass: the first, then one per
+    // iteration of each phase
+    const uint32_t predictions = tolerance > 0.0F ? 2u * config_.max_iterations + 1u
+                                                  : config_.divergence_iterations + config_.density_iterations + 1u;
+    masks_.allocate(n, predictions, domain_.config.support_radius);
+    if (tolerance > 0.0F) {
+        const size_t per_phase = config_.max_iterations + 1u;
+        tol_error_.resize(2u * per_phase);
+        tol_arrived_.resize(2u * per_phase);
+        tol_done_.resize(2u);
+        PHX_HIP(hipHostMalloc(reinterpret_cast<void**>(&tol_host_), 4u * sizeof(uint32_t), hipHostMallocDefault));
+        for (hipEvent_t& e : tol_events_) PHX_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    }
 }
 
-DfsphSolver::~DfsphSolver() = default;
+DfsphSolver::~DfsphSolver() {
+    for (hipEvent_t e : tol_events_)
+        if (e != nullptr) (void)hipEventDestroy(e);
+    if (tol_host_ != nullptr) (void)hipHostFree(tol_host_);
+}
 
 void DfsphSolver::forward_step(const phx_sph_dfsph_state& state, const phx_control& control,
                                const phx_sph_dfsph_parameters& parameters, phx_sph_dfsph_state& next,
@@ -259,6 +281,82 @@ void DfsphSolver::forward_step(const phx_sph_dfsph_state& state, const phx_contr
     };
 
     uint32_t slot = 0;  // displacement slot written by the latest predicting pass
+
+    if (config_.density_error_tolerance > 0.0F) {
+        // Tolerance mode: each phase runs until its projection's mean
+        // compression error is within the tolerance (or max_iterations),
+        // which the projection's last workgroup records on the device; the
+        // force pass of that iteration then ends the phase by itself
+        // (kDivAuto, or the final state). The host launches one iteration
+        // ahead and waits only for the previous projection's done word, so
+        // the GPU is never idle; a launched pass past the end returns at once.
+        const uint32_t max_it = config_.max_iterations;
+        const double tolerance = static_cast<double>(config_.density_error_tolerance);
+        const size_t per_phase = max_it + 1u;
+        fill_u32(stream_, reinterpret_cast<uint32_t*>(tol_error_.data()), 2u * 2u * per_phase, 0u);
+        fill_u32(stream_, tol_arrived_.data(), 2u * per_phase, 0u);
+        fill_u32(stream_, tol_done_.data(), 2u, 0u);
+        // non-pressure + the warm divergence pressure + the first prediction
+        fa.t1 = t_div_.data();
+        fa.t2 = t_den_.data();
+        fa.final_step = 0u;
+        set_slot(slot);
+        launch_force<true, kDivPredict>(profiler_, stream_, blocks, fa, kDfsphForceInit);
+        uint32_t executed[2] = {0u, 0u};
+        for (uint32_t phase = 0; phase < 2u; ++phase) {
+            uint32_t* done = tol_done_.data() + phase;
+            double* error = tol_error_.data() + phase * per_phase;
+            uint32_t* arrived = tol_arrived_.data() + phase * per_phase;
+            if (phase == 0u) {
+                pa.target = cache.densities.data();
+                pa.relaxation = relax_div;
+                pa.t = t_div_.data();
+            } else {
+                pa.target = rho0;
+                pa.relaxation = relax_den;
+                pa.t = t_den_.data();
+                fa.t1 = t_den_.data();
+                fa.t2 = nullptr;
+                fa.warm_div_out = nullptr;
+            }
+            const uint32_t first_slot = slot;  // the prediction the phase's first projection reads
+            auto launch_iteration = [&](uint32_t it) {
+                set_update(phase, it, first_slot + it - 1u);
+                pa.cv = Convergence{error + it, arrived + it, done, rho0, it, max_it, tolerance};
+                launch_projection(profiler_, stream_, blocks, pa,
+                                  phase == 0u ? kDfsphProjectionDivergence : kDfsphProjectionDensity);
+                uint32_t* host = tol_host_ + 2u * phase + (it & 1u);
+                PHX_HIP(hipMemcpyAsync(host, done, sizeof(uint32_t), hipMemcpyDeviceToHost, stream_));
+                PHX_HIP(hipEventRecord(tol_events_[it & 1u], stream_));
+                set_slot(first_slot + it);
+                fa.done = done;
+                fa.iteration = it;
+                fa.final_step = 0u;
+                if (phase == 0u) {
+                    launch_force<false, kDivAuto>(profiler_, stream_, blocks, fa, kDfsphForceDivergence);
+                } else {
+                    launch_force<false, kDenPredict>(profiler_, stream_, blocks, fa, kDfsphForceDensity);
+                }
+            };
+            launch_iteration(1u);
+            uint32_t it = 1u;
+            for (;; ++it) {
+                if (it < max_it) launch_iteration(it + 1u);
+                PHX_HIP(hipEventSynchronize(tol_events_[it & 1u]));
+                // the projection of iteration max_it always ends the phase
+                if (tol_host_[2u * phase + (it & 1u)] != 0u || it >= max_it) break;
+            }
+            executed[phase] = it;
+            slot = first_slot + it;  // the prediction of the pass that ended the phase
+        }
+        pa.cv = Convergence{};
+        fa.done = nullptr;
+        masks_.end_step(stream_);
+        next.particles.step_index = step + 1u;
+        last_iterations_[0] = executed[0];
+        last_iterations_[1] = executed[1];
+        return;
+    }
 
     // divergence phase: initial pressure with the warm p_div (sph.cpp:574-578)
     fa.t1 = t_div_.data();

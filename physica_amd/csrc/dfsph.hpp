@@ -128,6 +128,13 @@ class DfsphSolver {
     float gradient_norm_ = 0.0F;
     KernelConstants constants_{};
     uint32_t last_iterations_[2]{};
+    // tolerance mode: per-iteration FP64 error sums and arrival counters of
+    // each phase, the phases' done words, and their host copies (pinned),
+    // read one iteration behind the launches through two events
+    DeviceBuffer<double> tol_error_;
+    DeviceBuffer<uint32_t> tol_arrived_, tol_done_;
+    uint32_t* tol_host_ = nullptr;
+    hipEvent_t tol_events_[2] = {nullptr, nullptr};
     PassProfiler profiler_;
     // sorted-order workspace
     DeviceBuffer<float4> xm_, vmu_, xs_, adiv_;

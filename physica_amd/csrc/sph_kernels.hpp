@@ -91,6 +91,52 @@ __global__ void __launch_bounds__(kBlock) mask_density_kernel(MaskDensityArgs a)
     }
 }
 
+// Tolerance mode: the convergence state of one iteration of a pressure
+// phase. The projection adds its particles' compression errors
+// max(0, rho* - target) / rho0 into `error` (FP64); the last workgroup to
+// arrive marks the phase done at this iteration when the mean is within the
+// tolerance or the iteration is the last allowed one. Every pass of the
+// phase reads `done`: passes of later iterations (launched ahead by the
+// host) then return at once, and the force pass of the marked iteration ends
+// the phase.
+struct Convergence {
+    double* error;      // null: fixed iteration counts
+    uint32_t* arrived;  // workgroups that have added their error
+    uint32_t* done;     // iteration that ended the phase; 0 while it runs
+    const float* rest;  // rho0, the scale of the error
+    uint32_t iteration;
+    uint32_t max_iterations;
+    double tolerance;
+};
+
+// Block-wide FP64 sum reaching thread 0 (other threads get partial sums).
+template <uint32_t kBlockSize>
+__device__ __forceinline__ double block_sum(double value) {
+    __shared__ double wave_sum[kBlockSize / 64];
+    double v = value;
+    for (int offset = 32; offset > 0; offset >>= 1) v += __shfl_xor(v, offset, 64);
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t wave = threadIdx.x >> 6;
+    if (lane == 0u) wave_sum[wave] = v;
+    __syncthreads();
+    double total = 0.0;
+    if (threadIdx.x == 0u)
+        for (uint32_t w = 0; w < kBlockSize / 64; ++w) total += wave_sum[w];
+    return total;
+}
+
+// Adds a workgroup's error sum; the last workgroup decides the phase.
+__device__ __forceinline__ void converge(const Convergence& c, double block_error, uint32_t n) {
+    if (threadIdx.x != 0u) return;
+    atomicAdd(c.error, block_error);
+    __threadfence();
+    if (atomicAdd(c.arrived, 1u) == gridDim.x - 1u) {
+        __threadfence();
+        const double mean = atomicAdd(c.error, 0.0) / static_cast<double>(n);
+        if (mean <= c.tolerance || c.iteration >= c.max_iterations) atomicExch(c.done, c.iteration);
+    }
+}
+
 struct ProjectionArgs {
     uint32_t n;
     KernelConstants kc;
@@ -111,6 +157,7 @@ struct ProjectionArgs {
     float* p;
     float* t;
     float* rs_out;  // optional (recorded step): the predicted densities
+    Convergence cv;
 };
 
 // density at predicted positions (density.cu:18-40 with topology = x,
@@ -124,36 +171,45 @@ __global__ void __launch_bounds__(kBlock) projection_kernel(ProjectionArgs a) {
     __shared__ int delta[kStencil];
     fill_slot_deltas(a.grid, delta);
     __syncthreads();
+    const bool tolerance = a.cv.error != nullptr;
+    if (tolerance && *a.cv.done != 0u) return;  // the phase ended at an earlier iteration
     const uint32_t i = xcd_block() * kBlock + threadIdx.x;
-    if (i >= a.n) return;
-    const bool masked = a.flags[kFlagOverflow] == 0u &&
-                        __uint_as_float(a.flags[kFlagDisplacement + a.slot]) <= a.limit2;
-    const float4 self = a.xs[i];
-    const Float3 ps = xyz(self);
-    int cx, cy, cz;
-    particle_cell(a.grid, xyz(a.xm[i]), cx, cy, cz);
-    float rs = 0.0F;
-    auto visit = [&](uint32_t s) {
-        const float4 q = a.xs[s];
-        rs += q.w * cubic_value(pair_distance(sub(ps, xyz(q))), a.kc);
-    };
-    if (masked) {
-        uint16_t* column = smask + threadIdx.x;
-        const uint32_t nonzero = load_lane_column(a.skin, a.n, i, column, kBlock);
-        walk_masked_prefetch(
-            a.grid.offsets, cell_of(a.grid, cx, cy, cz), delta, column, kBlock, nonzero, i,
-            [&](uint32_t s) { return a.xs[s]; },
-            [&](float4 q) { rs += q.w * cubic_value(pair_distance(sub(ps, xyz(q))), a.kc); });
-    } else {
-        for_each_candidate(a.grid, cx, cy, cz, visit);
+    double error = 0.0;
+    if (i < a.n) {
+        const bool masked = a.flags[kFlagOverflow] == 0u &&
+                            __uint_as_float(a.flags[kFlagDisplacement + a.slot]) <= a.limit2;
+        const float4 self = a.xs[i];
+        const Float3 ps = xyz(self);
+        int cx, cy, cz;
+        particle_cell(a.grid, xyz(a.xm[i]), cx, cy, cz);
+        float rs = 0.0F;
+        auto visit = [&](uint32_t s) {
+            const float4 q = a.xs[s];
+            rs += q.w * cubic_value(pair_distance(sub(ps, xyz(q))), a.kc);
+        };
+        if (masked) {
+            uint16_t* column = smask + threadIdx.x;
+            const uint32_t nonzero = load_lane_column(a.skin, a.n, i, column, kBlock);
+            walk_masked_prefetch(
+                a.grid.offsets, cell_of(a.grid, cx, cy, cz), delta, column, kBlock, nonzero, i,
+                [&](uint32_t s) { return a.xs[s]; },
+                [&](float4 q) { rs += q.w * cubic_value(pair_distance(sub(ps, xyz(q))), a.kc); });
+        } else {
+            for_each_candidate(a.grid, cx, cy, cz, visit);
+        }
+        const float rho = a.rho[i];
+        const float target = a.target[i];
+        const float volume = self.w / a.volume_density[i];
+        const float delta_p = 1.0F / (2.0F * a.dt * a.dt * volume * volume * a.gradient_norm);
+        const float pressure = fmaxf(0.0F, a.p_prev[i] + a.relaxation[i] * delta_p * (rs - target));
+        a.p[i] = pressure;
+        a.t[i] = pressure / (rho * rho);
+        if (a.rs_out != nullptr) a.rs_out[i] = rs;
+        if (tolerance)
+            error = fmax(0.0, static_cast<double>(rs) - static_cast<double>(target)) /
+                    static_cast<double>(a.cv.rest[i]);
     }
-    const float rho = a.rho[i];
-    const float volume = self.w / a.volume_density[i];
-    const float delta_p = 1.0F / (2.0F * a.dt * a.dt * volume * volume * a.gradient_norm);
-    const float pressure = fmaxf(0.0F, a.p_prev[i] + a.relaxation[i] * delta_p * (rs - a.target[i]));
-    a.p[i] = pressure;
-    a.t[i] = pressure / (rho * rho);
-    if (a.rs_out != nullptr) a.rs_out[i] = rs;
+    if (tolerance) converge(a.cv, block_sum<kBlock>(error), a.n);
 }
 
 enum Epilogue : int {
@@ -161,6 +217,7 @@ enum Epilogue : int {
     kDivEnd = 1,       // a_div = a_p(t1); total = a_div + a_p(t2)
     kDenPredict = 2,   // total = a_div + a_p(t1)
     kInitPredict = 3,  // total = 0 (cleared pressure acceleration), no pressure pass
+    kDivAuto = 4,      // tolerance mode: kDivEnd at the iteration that ends the phase, else kDivPredict
 };
 
 struct ForceArgs {
@@ -190,6 +247,10 @@ struct ForceArgs {
     float* warm_div_out;
     const float* p_den;
     float* warm_den_out;
+    // tolerance mode: the phase's done word and this pass's iteration; a
+    // pass of a later iteration than the one that ended the phase returns
+    const uint32_t* done;
+    uint32_t iteration;
 };
 
 // Pressure acceleration (pressure_forward_kernel, common.cu:165-194),
@@ -200,12 +261,19 @@ struct ForceArgs {
 // same operations as the reference's expression.
 template <bool kNonPressure, int kEpilogue>
 __global__ void __launch_bounds__(kBlock) force_kernel(ForceArgs a) {
-    constexpr bool kTwo = kEpilogue == kDivEnd;
+    constexpr bool kAuto = kEpilogue == kDivAuto;
     constexpr bool kPressure = kEpilogue != kInitPredict;
     __shared__ uint16_t smask[kStencil * kBlock];
     __shared__ int delta[kStencil];
     fill_slot_deltas(a.grid, delta);
     __syncthreads();
+    // the iteration that ends its phase (tolerance mode), uniform over the grid
+    const uint32_t ended = a.done != nullptr ? *a.done : 0u;
+    if (ended != 0u && ended < a.iteration) return;
+    const bool ends = ended != 0u && ended == a.iteration;
+    // a second pressure field: the divergence phase's end
+    const bool kTwo = kEpilogue == kDivEnd || (kAuto && ends);
+    const uint32_t final_step = a.final_step != 0u || (kEpilogue == kDenPredict && ends) ? 1u : 0u;
     const uint32_t i = xcd_block() * kBlock + threadIdx.x;
     const bool active = i < a.n;
     float moved2 = 0.0F;
@@ -219,7 +287,7 @@ __global__ void __launch_bounds__(kBlock) force_kernel(ForceArgs a) {
         if constexpr (kNonPressure) acc = add(a.gravity, load(a.u, i));
         const float t1_i = kPressure ? a.t1[i] : 0.0F;
         float t2_i = 0.0F;
-        if constexpr (kTwo) t2_i = a.t2[i];
+        if (kTwo) t2_i = a.t2[i];
         Float3 ap1{0.0F, 0.0F, 0.0F};
         Float3 ap2{0.0F, 0.0F, 0.0F};
         int cx, cy, cz;
@@ -235,7 +303,7 @@ __global__ void __launch_bounds__(kBlock) force_kernel(ForceArgs a) {
             if constexpr (kPressure) {
                 const Float3 gradient = scale(d, f);
                 ap1 = add(ap1, scale(gradient, -ms * (t1_i + t1_s)));
-                if constexpr (kTwo) ap2 = add(ap2, scale(gradient, -ms * (t2_i + t2_s)));
+                if (kTwo) ap2 = add(ap2, scale(gradient, -ms * (t2_i + t2_s)));
             }
         };
         if (masked) {
@@ -282,23 +350,21 @@ __global__ void __launch_bounds__(kBlock) force_kernel(ForceArgs a) {
         Float3 total;
         if constexpr (kEpilogue == kInitPredict) {
             total = Float3{0.0F, 0.0F, 0.0F};
-        } else if constexpr (kEpilogue == kDivPredict) {
-            total = ap1;
-        } else if constexpr (kTwo) {
+        } else if constexpr (kEpilogue == kDenPredict) {
+            total = add(xyz(a.adiv[i]), ap1);
+        } else if (kTwo) {
             a.adiv[i] = make_float4(ap1.x, ap1.y, ap1.z, 0.0F);
             total = add(ap1, ap2);
         } else {
-            total = add(xyz(a.adiv[i]), ap1);
+            total = ap1;
         }
         Float3 nx, nv;
         const Float3 a_total = add(acc, total);
         predict_clamped(a.box, a.dt, p, vel, a_total, nx, nv);
         if (a.collisions != nullptr)
             a.collisions[i] = static_cast<uint8_t>(predict_collisions(a.box, a.dt, p, vel, a_total));
-        if constexpr (kTwo) {
-            if (a.warm_div_out != nullptr) a.warm_div_out[a.perm[i]] = a.p_div[i];
-        }
-        if (a.final_step != 0u) {
+        if (kTwo && a.warm_div_out != nullptr) a.warm_div_out[a.perm[i]] = a.p_div[i];
+        if (final_step != 0u) {
             const uint32_t j = a.perm[i];
             store(a.next_x, j, nx);
             store(a.next_v, j, nv);
@@ -309,7 +375,7 @@ __global__ void __launch_bounds__(kBlock) force_kernel(ForceArgs a) {
             moved2 = dot(moved, moved);
         }
     }
-    if (a.final_step == 0u) block_max_to<kBlock>(moved2, &a.flags[kFlagDisplacement + a.slot]);
+    if (final_step == 0u) block_max_to<kBlock>(moved2, &a.flags[kFlagDisplacement + a.slot]);
 }
 
 template <bool kNonPressure, int kEpilogue>

@@ -50,10 +50,10 @@ def cfg(**kw):
     return DomainConfiguration(**base)
 
 
-def create_dfsph(domain, config=None):
+def create_dfsph(domain, config=None, mode=0):
     handle = ctypes.c_void_p()
     status = _capi.library().phx_dfsph_create(
-        ctypes.byref(domain.to_c()), ctypes.byref((config or DFSPH.Configuration()).to_c()), 0, None,
+        ctypes.byref(domain.to_c()), ctypes.byref((config or DFSPH.Configuration()).to_c()), mode, None,
         ctypes.byref(handle))
     return status, handle
 
@@ -82,8 +82,13 @@ def test_invalid_domain_rejected_before_device_work(domain, status, fragment):
 def test_invalid_method_configuration_rejected():
     got, _ = create_dfsph(cfg(), DFSPH.Configuration(checkpoint_interval=0))
     assert got == _capi.PHX_ERROR_INVALID_ARGUMENT
-    got, _ = create_dfsph(cfg(), DFSPH.Configuration(density_error_tolerance=1e-3))
+    # tolerance mode: forward only, with 1 <= max_iterations, a finite non-negative tolerance
+    got, _ = create_dfsph(cfg(), DFSPH.Configuration(density_error_tolerance=1e-3), mode=1)
     assert got == _capi.PHX_ERROR_UNSUPPORTED
+    got, _ = create_dfsph(cfg(), DFSPH.Configuration(density_error_tolerance=1e-3, max_iterations=0))
+    assert got == _capi.PHX_ERROR_INVALID_ARGUMENT
+    got, _ = create_dfsph(cfg(), DFSPH.Configuration(density_error_tolerance=-1e-3))
+    assert got == _capi.PHX_ERROR_INVALID_ARGUMENT
     handle = ctypes.c_void_p()
     got = _capi.library().phx_pbf_create(ctypes.byref(cfg().to_c()), None, 0, None, ctypes.byref(handle))
     assert got == _capi.PHX_ERROR_INVALID_ARGUMENT
