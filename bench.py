@@ -1,12 +1,19 @@
 """Benchmark: SPH<DFSPH> dam break, 1,048,576 particles, one MI355X.
 
 Prints one JSON line (rank 0). A step is one SPH<DFSPH>::forward_step of the
-whole scene (grid build, density, 4 divergence + 6 density iterations) with
-every input resident in HBM. `value` is particle-steps/s over the timed steps;
-`roofline` is the dominant pass, its algorithmic bytes per launch over its
-average launch time (HIP events on the solver stream during the timed steps);
+whole scene (grid build, density, the divergence and density solvers) with
+every input resident in HBM. The configured workload runs both solvers to a
+0.1 % mean compression error (FP64 residual, at most 100 iterations each):
+`value` is particle-steps/s over the timed steps of that scenario, which
+start from the initial state (the warm-up steps run on a copy), so the
+default 200 timed steps are the whole configured run, cold start included.
+`fixed_iterations` gives the reference's own fixed counts (4 divergence + 6
+density iterations, no tolerance) measured over steady steps; `roofline` is
+its dominant pass, the algorithmic bytes per launch over the average launch
+time (HIP events on the solver stream during those timed steps).
 `cpu_baseline` is the FP32 oracle (OpenMP over particles, the same
-reference-order arithmetic) timed on this host over one step of the same scene.
+reference-order arithmetic) timed on this host over one fixed-count step of
+the same scene.
 
 `fwd_adjoint_ratio` is the differentiable PBF rollout: 262,144 particles (64^3
 lattice, spacing 0.02, h 0.04), 120 steps at dt 1/120, 4 iterations, loss the
@@ -36,8 +43,12 @@ METRIC = "particle-steps/s (1M DFSPH, 1 GPU, %HBM peak); fwd+adjoint/fwd time ra
 def parse_args(argv=None):
     p = argparse.ArgumentParser(description=__doc__)
     p.add_argument("--gpus", type=int, default=1)
-    p.add_argument("--steps", type=int, default=20)
+    p.add_argument("--steps", type=int, default=200)
     p.add_argument("--warmup", type=int, default=3)
+    p.add_argument("--mode", choices=["tolerance", "fixed"], default="tolerance",
+                   help="DFSPH solver stopping: 0.1 %% tolerance (the configured workload) or fixed 4 + 6 only")
+    p.add_argument("--fixed-steps", type=int, default=20,
+                   help="timed steps of the fixed-count measurement (0: skip it, and the kernel roofline with it)")
     p.add_argument("--workload", choices=["dfsph", "pbf"], default="dfsph")
     p.add_argument("--no-cpu-baseline", action="store_true", help="skip the CPU oracle timing")
     p.add_argument("--no-pass-timing", action="store_true", help="time without per-pass events")
@@ -73,14 +84,21 @@ def traffic_for(kernel: str):
     return None if entry is None else float(entry["bytes_per_launch"])
 
 
-def setup_dfsph(device):
+TOLERANCE = 1e-3  # 0.1 % mean density error
+MAX_ITERATIONS = 100
+
+
+def setup_dfsph(device, tolerance=0.0):
     import torch
 
     from physica_amd import scenes
     from physica_amd.particle import DFSPH, SPH
 
     sc = scenes.dfsph_dam_break()
-    cfg = DFSPH.Configuration(divergence_iterations=4, density_iterations=6)
+    if tolerance > 0.0:
+        cfg = DFSPH.Configuration(density_error_tolerance=tolerance, max_iterations=MAX_ITERATIONS)
+    else:
+        cfg = DFSPH.Configuration(divergence_iterations=4, density_iterations=6)
     solver = SPH(sc.domain, cfg, device=device)
     a, b = solver.allocate_state(), solver.allocate_state()
     control = solver.allocate_control()
@@ -127,7 +145,8 @@ def cpu_baseline(workload, sc):
     if workload == "dfsph":
         z = np.zeros(sc.count, np.float32)
         oracle.dfsph_step(d, 0, sc.positions, sc.velocities, zeros, sc.parameters, z, z, 4, 6)
-        sample = "1 step of the 1,048,576-particle DFSPH dam break (FP32 oracle, OpenMP over particles)"
+        sample = ("1 step of the 1,048,576-particle DFSPH dam break, fixed 4 + 6 iterations "
+                  "(FP32 oracle, OpenMP over particles)")
     else:
         oracle.pbf_step(d, 0, sc.positions, sc.velocities, zeros, sc.parameters, 4)
         sample = "1 step of the 32,768-particle PBF dam break (FP32 oracle, OpenMP over particles)"
@@ -222,72 +241,129 @@ def main(argv=None) -> int:
         return 2
     device = torch.device("cuda:0")
     torch.cuda.set_device(device)
-    if args.workload == "dfsph":
-        sc, solver, states, control, params, cache = setup_dfsph(device)
-    else:
-        sc, solver, states, control, params, cache = setup_pbf(device)
-    n = sc.count
-    cells = math.prod(sc.domain.grid_resolution())
 
-    def step(k):
-        solver.forward_step(states[k % 2], control, params, states[(k + 1) % 2], cache)
+    def measure(setup, steps, warmup, from_start, profile):
+        """Times `steps` forward steps; with from_start the warm-up steps run
+        on a copy and the timed steps start from the initial state. Returns
+        (scene, elapsed ms, per-pass profile, iteration counts per step)."""
+        sc, solver, states, control, params, cache = setup()
+        initial = solver.allocate_state()
+        solver.copy_state(states[0], initial)
 
-    for k in range(args.warmup):
-        step(k)
-    torch.cuda.synchronize()
-    if not args.no_pass_timing:
-        solver.set_profiling(True)
-    stream = torch.cuda.current_stream(device)
-    start = torch.cuda.Event(enable_timing=True)
-    stop = torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    start.record(stream)
-    for k in range(args.warmup, args.warmup + args.steps):
-        step(k)
-    stop.record(stream)
-    torch.cuda.synchronize()
-    elapsed_ms = start.elapsed_time(stop)
-    passes = [] if args.no_pass_timing else solver.read_profile()
-    solver.set_profiling(False)
-    ms_per_step = elapsed_ms / args.steps
-    value = n * args.steps / (elapsed_ms * 1e-3)
+        def step(k):
+            solver.forward_step(states[k % 2], control, params, states[(k + 1) % 2], cache)
+            return solver.last_iterations() if hasattr(solver, "last_iterations") else None
 
-    final = states[(args.warmup + args.steps) % 2]
-    x = final.particles.positions.numpy()
-    if not np.isfinite(x).all():
-        print("non-finite positions after the timed steps", file=sys.stderr)
-        return 1
+        for k in range(warmup):
+            step(k)
+        if from_start:
+            solver.copy_state(initial, states[0])
+            states[0].particles.step_index = 0
+            first = 0
+        else:
+            first = warmup
+        torch.cuda.synchronize()
+        if profile:
+            solver.set_profiling(True)
+        stream = torch.cuda.current_stream(device)
+        start = torch.cuda.Event(enable_timing=True)
+        stop = torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        start.record(stream)
+        counts = [step(k) for k in range(first, first + steps)]
+        stop.record(stream)
+        torch.cuda.synchronize()
+        elapsed_ms = start.elapsed_time(stop)
+        passes = solver.read_profile() if profile else []
+        solver.set_profiling(False)
+        final = states[(first + steps) % 2]
+        if not np.isfinite(final.particles.positions.numpy()).all():
+            raise RuntimeError("non-finite positions after the timed steps")
+        return sc, elapsed_ms, passes, counts, solver, states, cache
 
-    if args.workload == "dfsph":
-        iterations = solver.last_iterations()
-        bytes_step = dfsph_bytes_per_step(iterations, n, cells)
-        workload = ("DFSPH 3D dam break: 1,048,576 particles (128x64x128 block, spacing 0.01, jitter 0.001), "
-                    "h 0.02 cubic spline, dt 1e-3, 4 divergence + 6 density iterations (fixed counts, the "
-                    "reference's DFSPH), viscosity 0.01, static box 2.56x1.28x1.28")
-        config = {"workload": workload, "particles": n, "grid_cells": cells,
-                  "divergence_iterations": iterations[0], "density_iterations": iterations[1]}
-    else:
-        bytes_step = pbf_bytes_per_step(4, n, cells)
-        workload = "PBF dam break: 32,768 particles (32^3, spacing 0.05), h 0.1, dt 1/120, 4 iterations"
-        config = {"workload": workload, "particles": n, "grid_cells": cells, "pressure_iterations": 4}
-
-    roofline = None
-    if passes:
+    def roofline_of(passes, n):
+        if not passes:
+            return None
         dom = max(passes, key=lambda p: p["total_ms"])
         avg_ms = dom["total_ms"] / dom["launches"]
         achieved = dom["bytes_per_particle"] * n / (avg_ms * 1e-3) / 1e9
-        roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic_for(dom["name"]),
-                    "kernel": dom["name"], "avg_launch_us": round(avg_ms * 1e3, 2),
-                    "algorithmic_bytes_per_launch": dom["bytes_per_particle"] * n}
-    step_achieved = bytes_step * n * args.steps / (elapsed_ms * 1e-3) / 1e9
+        return {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic_for(dom["name"]),
+                "kernel": dom["name"], "avg_launch_us": round(avg_ms * 1e3, 2),
+                "algorithmic_bytes_per_launch": dom["bytes_per_particle"] * n}
+
+    fixed = None
+    roofline = None
+    if args.workload == "dfsph":
+        if args.mode == "fixed" and args.fixed_steps < 1:
+            print("--mode fixed needs --fixed-steps >= 1", file=sys.stderr)
+            return 2
+        if args.fixed_steps > 0:
+            # the reference's fixed counts over steady steps: per-pass timing and the kernel roofline
+            sc, f_ms, f_passes, f_counts, solver, states, cache = measure(
+                lambda: setup_dfsph(device), args.fixed_steps, args.warmup, False, not args.no_pass_timing)
+            n = sc.count
+            cells = math.prod(sc.domain.grid_resolution())
+            f_bytes = dfsph_bytes_per_step((4, 6), n, cells)
+            fixed = {"iterations": [4, 6], "steps": args.fixed_steps, "warmup": args.warmup,
+                     "ms_per_step": round(f_ms / args.fixed_steps, 4),
+                     "value": round(n * args.fixed_steps / (f_ms * 1e-3), 1),
+                     "hbm_fraction_step": round(
+                         f_bytes * n * args.fixed_steps / (f_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 5),
+                     "algorithmic_bytes_per_particle_step": round(f_bytes, 2),
+                     "passes": [{k: (round(v, 4) if isinstance(v, float) else v) for k, v in p.items()}
+                                for p in f_passes]}
+            roofline = roofline_of(f_passes, n)
+            del solver, states, cache
+        if args.mode == "tolerance":
+            sc, elapsed_ms, passes, counts, solver, states, cache = measure(
+                lambda: setup_dfsph(device, TOLERANCE), args.steps, args.warmup, True, False)
+            n = sc.count
+            cells = math.prod(sc.domain.grid_resolution())
+            bytes_total = sum(dfsph_bytes_per_step(c, n, cells) for c in counts)
+            bytes_step = bytes_total / len(counts)
+            workload = ("DFSPH 3D dam break: 1,048,576 particles (128x64x128 block, spacing 0.01, jitter 0.001), "
+                        "h 0.02 cubic spline, dt 1e-3, divergence and density solvers to a 0.1 % mean density "
+                        "error (FP64 residual, max 100 iterations each), viscosity 0.01, static box 2.56x1.28x1.28; "
+                        f"timed steps 0..{args.steps - 1} of the scenario")
+            nd = [c[0] for c in counts]
+            ne = [c[1] for c in counts]
+            config = {"workload": workload, "particles": n, "grid_cells": cells, "tolerance": TOLERANCE,
+                      "max_iterations": MAX_ITERATIONS,
+                      "divergence_iterations": {"total": sum(nd), "first_step": nd[0], "max": max(nd),
+                                                "mean_after_first": round(sum(nd[1:]) / max(1, len(nd) - 1), 3)},
+                      "density_iterations": {"total": sum(ne), "first_step": ne[0], "max": max(ne),
+                                             "mean_after_first": round(sum(ne[1:]) / max(1, len(ne) - 1), 3)}}
+            steps_timed, warmup_used = args.steps, args.warmup
+        else:
+            elapsed_ms, passes, steps_timed, warmup_used = f_ms, f_passes, args.fixed_steps, args.warmup
+            bytes_step = f_bytes
+            workload = ("DFSPH 3D dam break: 1,048,576 particles (128x64x128 block, spacing 0.01, jitter 0.001), "
+                        "h 0.02 cubic spline, dt 1e-3, 4 divergence + 6 density iterations (fixed counts, the "
+                        "reference's DFSPH), viscosity 0.01, static box 2.56x1.28x1.28")
+            config = {"workload": workload, "particles": n, "grid_cells": cells, "divergence_iterations": 4,
+                      "density_iterations": 6}
+            fixed = None
+    else:
+        sc, elapsed_ms, passes, _, solver, states, cache = measure(
+            lambda: setup_pbf(device), args.steps, args.warmup, False, not args.no_pass_timing)
+        n = sc.count
+        cells = math.prod(sc.domain.grid_resolution())
+        bytes_step = pbf_bytes_per_step(4, n, cells)
+        workload = "PBF dam break: 32,768 particles (32^3, spacing 0.05), h 0.1, dt 1/120, 4 iterations"
+        config = {"workload": workload, "particles": n, "grid_cells": cells, "pressure_iterations": 4}
+        roofline = roofline_of(passes, n)
+        steps_timed, warmup_used = args.steps, args.warmup
+    ms_per_step = elapsed_ms / steps_timed
+    value = n * steps_timed / (elapsed_ms * 1e-3)
+    step_achieved = bytes_step * n * steps_timed / (elapsed_ms * 1e-3) / 1e9
 
     baseline = None
     if not args.no_cpu_baseline:
         baseline = cpu_baseline(args.workload, sc)
     adjoint = None
     if not args.no_adjoint:
-        del states, cache
+        del states, cache, solver
         adjoint = adjoint_ratio(device, steps=args.adjoint_steps)
 
     line = {
@@ -295,8 +371,8 @@ def main(argv=None) -> int:
         "value": round(value, 1),
         "unit": "particle-steps/s",
         "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
+        "steps": steps_timed,
+        "warmup": warmup_used,
         "ms_per_step": round(ms_per_step, 4),
         "higher_is_better": True,
         "scaling": "weak",
@@ -310,6 +386,7 @@ def main(argv=None) -> int:
         "algorithmic_bytes_per_particle_step": round(bytes_step, 2),
         "fwd_adjoint_ratio": None if adjoint is None else adjoint["ratio"],
         "fwd_adjoint": adjoint,
+        "fixed_iterations": fixed,
         "passes": [{k: (round(v, 4) if isinstance(v, float) else v) for k, v in p.items()} for p in passes],
     }
     print(json.dumps(line), flush=True)

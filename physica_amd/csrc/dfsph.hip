@@ -125,9 +125,7 @@ DfsphSolver::DfsphSolver(const phx_domain_configuration* domain, const phx_dfsph
                                                   : config_.divergence_iterations + config_.density_iterations + 1u;
     masks_.allocate(n, predictions, domain_.config.support_radius);
     if (tolerance > 0.0F) {
-        const size_t per_phase = config_.max_iterations + 1u;
-        tol_error_.resize(2u * per_phase);
-        tol_arrived_.resize(2u * per_phase);
+        tol_partials_.resize(pass_blocks(n));  // one per projection workgroup
         tol_done_.resize(2u);
         PHX_HIP(hipHostMalloc(reinterpret_cast<void**>(&tol_host_), 4u * sizeof(uint32_t), hipHostMallocDefault));
         for (hipEvent_t& e : tol_events_) PHX_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
@@ -284,17 +282,16 @@ void DfsphSolver::forward_step(const phx_sph_dfsph_state& state, const phx_contr
 
     if (config_.density_error_tolerance > 0.0F) {
         // Tolerance mode: each phase runs until its projection's mean
-        // compression error is within the tolerance (or max_iterations),
-        // which the projection's last workgroup records on the device; the
-        // force pass of that iteration then ends the phase by itself
-        // (kDivAuto, or the final state). The host launches one iteration
-        // ahead and waits only for the previous projection's done word, so
-        // the GPU is never idle; a launched pass past the end returns at once.
+        // compression error is within the tolerance (or max_iterations).
+        // The projection leaves one FP64 error sum per workgroup; a
+        // one-workgroup pass adds them in a fixed order and records the
+        // ending iteration on the device; the force pass of that iteration
+        // then ends the phase by itself (kDivAuto, or the final state). The
+        // host launches one iteration ahead and waits only for the previous
+        // iteration's done word, so the GPU is never idle; a launched pass
+        // past the end returns at once.
         const uint32_t max_it = config_.max_iterations;
         const double tolerance = static_cast<double>(config_.density_error_tolerance);
-        const size_t per_phase = max_it + 1u;
-        fill_u32(stream_, reinterpret_cast<uint32_t*>(tol_error_.data()), 2u * 2u * per_phase, 0u);
-        fill_u32(stream_, tol_arrived_.data(), 2u * per_phase, 0u);
         fill_u32(stream_, tol_done_.data(), 2u, 0u);
         // non-pressure + the warm divergence pressure + the first prediction
         fa.t1 = t_div_.data();
@@ -305,8 +302,6 @@ void DfsphSolver::forward_step(const phx_sph_dfsph_state& state, const phx_contr
         uint32_t executed[2] = {0u, 0u};
         for (uint32_t phase = 0; phase < 2u; ++phase) {
             uint32_t* done = tol_done_.data() + phase;
-            double* error = tol_error_.data() + phase * per_phase;
-            uint32_t* arrived = tol_arrived_.data() + phase * per_phase;
             if (phase == 0u) {
                 pa.target = cache.densities.data();
                 pa.relaxation = relax_div;
@@ -322,9 +317,12 @@ void DfsphSolver::forward_step(const phx_sph_dfsph_state& state, const phx_contr
             const uint32_t first_slot = slot;  // the prediction the phase's first projection reads
             auto launch_iteration = [&](uint32_t it) {
                 set_update(phase, it, first_slot + it - 1u);
-                pa.cv = Convergence{error + it, arrived + it, done, rho0, it, max_it, tolerance};
+                pa.cv = Convergence{tol_partials_.data(), done, rho0};
                 launch_projection(profiler_, stream_, blocks, pa,
                                   phase == 0u ? kDfsphProjectionDivergence : kDfsphProjectionDensity);
+                converge_kernel<<<1, kConvergeBlock, 0, stream_>>>(tol_partials_.data(), blocks, n, tolerance, it,
+                                                                   max_it, done);
+                PHX_HIP(hipGetLastError());
                 uint32_t* host = tol_host_ + 2u * phase + (it & 1u);
                 PHX_HIP(hipMemcpyAsync(host, done, sizeof(uint32_t), hipMemcpyDeviceToHost, stream_));
                 PHX_HIP(hipEventRecord(tol_events_[it & 1u], stream_));

@@ -131,8 +131,8 @@ class DfsphSolver {
     // tolerance mode: per-iteration FP64 error sums and arrival counters of
     // each phase, the phases' done words, and their host copies (pinned),
     // read one iteration behind the launches through two events
-    DeviceBuffer<double> tol_error_;
-    DeviceBuffer<uint32_t> tol_arrived_, tol_done_;
+    DeviceBuffer<double> tol_partials_;
+    DeviceBuffer<uint32_t> tol_done_;
     uint32_t* tol_host_ = nullptr;
     hipEvent_t tol_events_[2] = {nullptr, nullptr};
     PassProfiler profiler_;

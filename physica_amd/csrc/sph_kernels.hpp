@@ -100,16 +100,13 @@ __global__ void __launch_bounds__(kBlock) mask_density_kernel(MaskDensityArgs a)
 // host) then return at once, and the force pass of the marked iteration ends
 // the phase.
 struct Convergence {
-    double* error;      // null: fixed iteration counts
-    uint32_t* arrived;  // workgroups that have added their error
-    uint32_t* done;     // iteration that ended the phase; 0 while it runs
+    double* partials;   // null: fixed iteration counts; else one error sum per workgroup
+    const uint32_t* done;  // iteration that ended the phase; 0 while it runs
     const float* rest;  // rho0, the scale of the error
-    uint32_t iteration;
-    uint32_t max_iterations;
-    double tolerance;
 };
 
 // Block-wide FP64 sum reaching thread 0 (other threads get partial sums).
+// The order of the additions is fixed, so the result is reproducible.
 template <uint32_t kBlockSize>
 __device__ __forceinline__ double block_sum(double value) {
     __shared__ double wave_sum[kBlockSize / 64];
@@ -125,15 +122,24 @@ __device__ __forceinline__ double block_sum(double value) {
     return total;
 }
 
-// Adds a workgroup's error sum; the last workgroup decides the phase.
-__device__ __forceinline__ void converge(const Convergence& c, double block_error, uint32_t n) {
-    if (threadIdx.x != 0u) return;
-    atomicAdd(c.error, block_error);
-    __threadfence();
-    if (atomicAdd(c.arrived, 1u) == gridDim.x - 1u) {
-        __threadfence();
-        const double mean = atomicAdd(c.error, 0.0) / static_cast<double>(n);
-        if (mean <= c.tolerance || c.iteration >= c.max_iterations) atomicExch(c.done, c.iteration);
+constexpr uint32_t kConvergeBlock = 1024;
+
+// Ends a tolerance-mode phase: sums the projection's per-workgroup error
+// sums in a fixed order (thread t takes partials t, t + 1024, ...), and
+// records the iteration in *done when the mean error is within the
+// tolerance or the iteration is the last allowed. One workgroup; the kernel
+// boundaries order it after the projection and before the force pass, so
+// no atomics or fences are needed.
+[[maybe_unused]] __global__ void __launch_bounds__(kConvergeBlock)
+    converge_kernel(const double* partials, uint32_t count, uint32_t n, double tolerance, uint32_t iteration,
+                    uint32_t max_iterations, uint32_t* done) {
+    if (*done != 0u) return;  // the phase ended at an earlier iteration
+    double sum = 0.0;
+    for (uint32_t k = threadIdx.x; k < count; k += kConvergeBlock) sum += partials[k];
+    const double total = block_sum<kConvergeBlock>(sum);
+    if (threadIdx.x == 0u) {
+        const double mean = total / static_cast<double>(n);
+        if (mean <= tolerance || iteration >= max_iterations) *done = iteration;
     }
 }
 
@@ -171,7 +177,7 @@ __global__ void __launch_bounds__(kBlock) projection_kernel(ProjectionArgs a) {
     __shared__ int delta[kStencil];
     fill_slot_deltas(a.grid, delta);
     __syncthreads();
-    const bool tolerance = a.cv.error != nullptr;
+    const bool tolerance = a.cv.partials != nullptr;
     if (tolerance && *a.cv.done != 0u) return;  // the phase ended at an earlier iteration
     const uint32_t i = xcd_block() * kBlock + threadIdx.x;
     double error = 0.0;
@@ -209,7 +215,10 @@ __global__ void __launch_bounds__(kBlock) projection_kernel(ProjectionArgs a) {
             error = fmax(0.0, static_cast<double>(rs) - static_cast<double>(target)) /
                     static_cast<double>(a.cv.rest[i]);
     }
-    if (tolerance) converge(a.cv, block_sum<kBlock>(error), a.n);
+    if (tolerance) {
+        const double block_error = block_sum<kBlock>(error);
+        if (threadIdx.x == 0u) a.cv.partials[blockIdx.x] = block_error;
+    }
 }
 
 enum Epilogue : int {
