@@ -192,30 +192,53 @@ typedef struct phx_neighborhood_view {
     float cell_size;
 } phx_neighborhood_view;
 
+/* BoundaryView, neighborhood/device.h:26-36: the boundary samples as device
+ * arrays in original order; a sample's position at `time` is
+ * position + time * velocity (boundary_position, neighborhood/device.cuh:16-22). */
+typedef struct phx_boundary_view {
+    const float* position_x;
+    const float* position_y;
+    const float* position_z;
+    const float* velocity_x;
+    const float* velocity_y;
+    const float* velocity_z;
+    const float* volumes;
+    uint32_t count;
+    float time;
+} phx_boundary_view;
+
 /* Workspace bytes phx_build_neighborhood needs (radix_sort_storage_size,
- * neighborhood.cu:48-52). */
+ * neighborhood.cu:48-52), boundary samples included. */
 phx_status phx_neighborhood_workspace_size(const phx_domain_configuration* domain, size_t* bytes);
 
-/* NeighborhoodSearch::build(step_index, positions, neighborhood):
+/* NeighborhoodSearch::build(step_index, positions, neighborhood)
+ * (neighborhood/neighborhood.cpp:44-89, neighborhood.cu:54-70):
  * key = (cz*Cy + cy)*Cx + cx with cell = clamp(floor((x - min - t v_box)/h)),
  * t = step_index * time_step; stable ascending sort of (key, index);
  * cell_offsets[c] = first sorted position with key >= c for c in [0, C].
- * Outputs (device): sorted_keys[N], sorted_particle_indices[N],
- * cell_offsets[C+1], boundary_cell_offsets[C+1] (all zero when the domain
- * has no boundary particles). Writes the lookup origin min + t v_box into
- * *view if view is non-null. Boundary particles: PHX_ERROR_UNSUPPORTED. */
+ * The domain's boundary samples get the same keys at their positions
+ * x0 + t v and their own sort and offsets. Outputs (device; the sorted keys
+ * and boundary arrays may be null): sorted_keys[N], sorted_particle_indices[N],
+ * cell_offsets[C+1], sorted_boundary_keys[B], sorted_boundary_indices[B],
+ * boundary_cell_offsets[C+1] (all zero when B = 0). The boundary samples are
+ * copied from the domain's host array into the workspace, and the call waits
+ * for that copy. Writes the arrays, counts, resolution and the lookup origin
+ * min + t v_box into *view if view is non-null. */
 phx_status phx_build_neighborhood(phx_stream stream, const phx_domain_configuration* domain, uint64_t step_index,
                                   phx_const_vector_view positions, void* workspace, size_t workspace_bytes,
                                   uint64_t* sorted_keys, uint32_t* sorted_particle_indices, uint32_t* cell_offsets,
+                                  uint64_t* sorted_boundary_keys, uint32_t* sorted_boundary_indices,
                                   uint32_t* boundary_cell_offsets, phx_neighborhood_view* view);
 
 /* density_forward (density.cu:18-40): rho_i = sum_j m_j W(x_i - x_j) over the
- * 27 cells of topology_positions[i], j = i included; W = poly6 when
- * pbf_kernel != 0, else the cubic spline. */
+ * 27 cells of topology_positions[i], j = i included, each cell's boundary
+ * samples b after its particles adding rho0_i V_b W(x_i - x_b(t)); W = poly6
+ * when pbf_kernel != 0, else the cubic spline. boundary.count must equal
+ * neighborhood.boundary_count (0: no boundary arrays are read). */
 phx_status phx_density_forward(phx_stream stream, uint32_t particle_count, float support_radius, int pbf_kernel,
                                phx_const_vector_view topology_positions, phx_const_vector_view positions,
                                phx_particle_parameters parameters, phx_neighborhood_view neighborhood,
-                               float* densities);
+                               phx_boundary_view boundary, float* densities);
 
 /* ----------------------------------------------------------------- DFSPH */
 /* DFSPH::Configuration, sph/sph.ixx:196-201. density_error_tolerance > 0

@@ -121,19 +121,27 @@ def _out_vec(n, precision):
     return a, ptrs
 
 
-def build_neighborhood(domain: OracleDomain, step_index: int, positions, precision="f32"):
-    """(sorted_keys u64[N], sorted_indices u32[N], cell_offsets u32[C+1])."""
+def build_neighborhood(domain: OracleDomain, step_index: int, positions, precision="f32", boundary=False):
+    """(sorted_keys u64[N], sorted_indices u32[N], cell_offsets u32[C+1]);
+    with boundary=True also (sorted boundary keys u64[B], sorted boundary
+    indices u32[B], boundary cell offsets u32[C+1])."""
     n = domain.particle_count
+    b = domain.boundary_count
     cells = int(np.prod(grid_resolution(domain)))
     pos = _arr(positions, precision)
     keys = np.zeros(n, dtype=np.uint64)
     idx = np.zeros(n, dtype=np.uint32)
     off = np.zeros(cells + 1, dtype=np.uint32)
+    bkeys = np.zeros(max(b, 1), dtype=np.uint64)
+    bidx = np.zeros(max(b, 1), dtype=np.uint32)
     boff = np.zeros(cells + 1, dtype=np.uint32)
     getattr(library(), "oracle_build_neighborhood_" + precision)(
         ctypes.byref(domain), c_uint64(step_index), c_void_p(pos[0].ctypes.data), c_void_p(pos[1].ctypes.data),
         c_void_p(pos[2].ctypes.data), c_void_p(keys.ctypes.data), c_void_p(idx.ctypes.data),
-        c_void_p(off.ctypes.data), None, None, c_void_p(boff.ctypes.data))
+        c_void_p(off.ctypes.data), c_void_p(bkeys.ctypes.data), c_void_p(bidx.ctypes.data),
+        c_void_p(boff.ctypes.data))
+    if boundary:
+        return keys, idx, off, bkeys[:b], bidx[:b], boff
     return keys, idx, off
 
 

@@ -11,9 +11,11 @@ from .particle import (  # noqa: F401
     SPH,
     BoxBoundary,
     Control,
+    DeviceBoundary,
     Domain,
     DomainConfiguration,
     ExecutionMode,
+    Neighborhood,
     ParticleParameters,
     ParticleState,
     VectorField,

@@ -112,6 +112,20 @@ class NeighborhoodView(ctypes.Structure):
     ]
 
 
+class BoundaryView(ctypes.Structure):
+    _fields_ = [
+        ("position_x", c_void_p),
+        ("position_y", c_void_p),
+        ("position_z", c_void_p),
+        ("velocity_x", c_void_p),
+        ("velocity_y", c_void_p),
+        ("velocity_z", c_void_p),
+        ("volumes", c_void_p),
+        ("count", c_uint32),
+        ("time", c_float),
+    ]
+
+
 class DfsphConfiguration(ctypes.Structure):
     _fields_ = [
         ("divergence_iterations", c_uint32),
@@ -299,11 +313,12 @@ PROTOTYPES = {
     "phx_build_neighborhood": (
         c_int,
         [c_void_p, POINTER(DomainConfiguration), c_uint64, VectorView, c_void_p, c_size_t, c_void_p, c_void_p,
-         c_void_p, c_void_p, POINTER(NeighborhoodView)],
+         c_void_p, c_void_p, c_void_p, c_void_p, POINTER(NeighborhoodView)],
     ),
     "phx_density_forward": (
         c_int,
-        [c_void_p, c_uint32, c_float, c_int, VectorView, VectorView, ParticleParameters, NeighborhoodView, c_void_p],
+        [c_void_p, c_uint32, c_float, c_int, VectorView, VectorView, ParticleParameters, NeighborhoodView,
+         BoundaryView, c_void_p],
     ),
     "phx_dfsph_create": (
         c_int,

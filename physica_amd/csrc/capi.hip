@@ -73,15 +73,18 @@ constexpr uint32_t kBlock = 256;
 struct RefGrid {
     const uint32_t* indices;
     const uint32_t* offsets;
+    const uint32_t* boundary_indices;  // null when there are no boundary samples
+    const uint32_t* boundary_offsets;
     uint32_t cx, cy, cz;
     float ox, oy, oz, cell;
 };
 
 // density_forward_kernel (density/density.cu:18-40) in original particle order
-// over a reference-layout neighbourhood; boundary particles are not supported.
+// over a reference-layout neighbourhood, each cell's boundary samples after
+// its particles.
 __global__ void __launch_bounds__(kBlock)
     reference_density_kernel(uint32_t n, float h, int pbf_kernel, phx::CVec topo, phx::CVec pos, const float* masses,
-                             RefGrid g, float* out) {
+                             const float* rest_densities, phx_boundary_view boundary, RefGrid g, float* out) {
     using namespace phx;
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -105,8 +108,30 @@ __global__ void __launch_bounds__(kBlock)
                     const Float3 d = sub(p, load(pos, j));
                     result += masses[j] * (pbf_kernel != 0 ? poly6(d, h) : cubic(d, h));
                 }
+                if (g.boundary_indices == nullptr) continue;
+                for (uint32_t s = g.boundary_offsets[c]; s < g.boundary_offsets[c + 1u]; ++s) {
+                    const uint32_t b = g.boundary_indices[s];
+                    const Float3 xb{boundary.position_x[b] + boundary.time * boundary.velocity_x[b],
+                                    boundary.position_y[b] + boundary.time * boundary.velocity_y[b],
+                                    boundary.position_z[b] + boundary.time * boundary.velocity_z[b]};
+                    const Float3 d = sub(p, xb);
+                    result += rest_densities[i] * boundary.volumes[b] * (pbf_kernel != 0 ? poly6(d, h) : cubic(d, h));
+                }
             }
     out[i] = result;
+}
+
+// Workspace of phx_build_neighborhood: the fluid grid scratch, then for B
+// boundary samples their scratch, the packed samples (7 B floats) and the
+// boundary permutation.
+size_t round256(size_t b) { return (b + 255u) & ~static_cast<size_t>(255u); }
+size_t neighborhood_workspace_bytes(const phx::Domain& d) {
+    const size_t b = d.config.boundary_particle_count;
+    size_t bytes = round256(phx::grid_scratch_bytes(d.config.particle_count, d.cells));
+    if (b > 0u)
+        bytes += round256(phx::boundary_scratch_bytes(static_cast<uint32_t>(b))) + round256(7u * b * sizeof(float)) +
+                 round256(b * sizeof(uint32_t));
+    return bytes;
 }
 
 }  // namespace
@@ -128,17 +153,19 @@ phx_status phx_neighborhood_workspace_size(const phx_domain_configuration* domai
     return guarded([&] {
         phx::require(bytes != nullptr, PHX_ERROR_INVALID_ARGUMENT, "null bytes");
         const phx::Domain d = phx::make_domain(domain);
-        *bytes = phx::grid_scratch_bytes(d.config.particle_count, d.cells);
+        *bytes = neighborhood_workspace_bytes(d);
     });
 }
 
 phx_status phx_build_neighborhood(phx_stream stream, const phx_domain_configuration* domain, uint64_t step_index,
                                   phx_const_vector_view positions, void* workspace, size_t workspace_bytes,
                                   uint64_t* sorted_keys, uint32_t* sorted_particle_indices, uint32_t* cell_offsets,
+                                  uint64_t* sorted_boundary_keys, uint32_t* sorted_boundary_indices,
                                   uint32_t* boundary_cell_offsets, phx_neighborhood_view* view) {
     return guarded([&] {
         const phx::Domain d = phx::make_domain(domain);
         const uint32_t n = d.config.particle_count;
+        const uint32_t b = d.config.boundary_particle_count;
         const hipStream_t s = stream_of(stream);
         phx::require(cell_offsets != nullptr, PHX_ERROR_INVALID_ARGUMENT, "null cell_offsets");
         if (n > 0) {
@@ -146,23 +173,50 @@ phx_status phx_build_neighborhood(phx_stream stream, const phx_domain_configurat
             phx::require(sorted_particle_indices != nullptr, PHX_ERROR_INVALID_ARGUMENT, "null sorted_particle_indices");
         }
         phx::require(workspace != nullptr, PHX_ERROR_INVALID_ARGUMENT, "null workspace");
-        const phx::GridScratch scratch = phx::carve_grid_scratch(workspace, workspace_bytes, n, d.cells);
+        phx::require(workspace_bytes >= neighborhood_workspace_bytes(d), PHX_ERROR_WORKSPACE_TOO_SMALL,
+                     "neighbourhood workspace too small");
+        const std::vector<float> packed = phx::pack_boundary_samples(d.config);
+        auto* base = static_cast<uint8_t*>(workspace);
+        const size_t grid_bytes = round256(phx::grid_scratch_bytes(n, d.cells));
+        const phx::GridScratch scratch = phx::carve_grid_scratch(base, grid_bytes, n, d.cells);
         phx::ChannelList none;
         const phx::GridGeometry geo = phx::build_grid(s, d, step_index, phx::cvec(positions), nullptr, none.get(),
                                                       scratch, sorted_particle_indices, cell_offsets);
         if (sorted_keys != nullptr && n > 0)
             phx::export_sorted_keys(s, sorted_particle_indices, scratch.keys, n, sorted_keys);
-        if (boundary_cell_offsets != nullptr)
+        if (b > 0u) {
+            uint8_t* p = base + grid_bytes;
+            const size_t scratch_bytes = round256(phx::boundary_scratch_bytes(b));
+            const phx::BoundaryScratch bs = phx::carve_boundary_scratch(p, scratch_bytes, b);
+            p += scratch_bytes;
+            auto* samples = reinterpret_cast<float*>(p);
+            p += round256(7u * static_cast<size_t>(b) * sizeof(float));
+            uint32_t* perm = sorted_boundary_indices != nullptr ? sorted_boundary_indices : reinterpret_cast<uint32_t*>(p);
+            PHX_HIP(hipMemcpyAsync(samples, packed.data(), sizeof(float) * packed.size(), hipMemcpyHostToDevice, s));
+            // scratch offsets when the caller does not want them
+            uint32_t* offsets = boundary_cell_offsets;
+            phx::DeviceBuffer<uint32_t> own;
+            if (offsets == nullptr) {
+                own.resize(static_cast<size_t>(d.cells) + 1u);
+                offsets = own.data();
+            }
+            phx::build_boundary_grid(s, d, step_index, phx::boundary_source(samples, b), b, bs, scratch, perm, offsets,
+                                     nullptr, nullptr);
+            if (sorted_boundary_keys != nullptr) phx::export_sorted_keys(s, perm, bs.keys, b, sorted_boundary_keys);
+            // the host copy of the samples and any owned offsets end with this call
+            PHX_HIP(hipStreamSynchronize(s));
+        } else if (boundary_cell_offsets != nullptr) {
             phx::fill_u32(s, boundary_cell_offsets, static_cast<size_t>(d.cells) + 1u, 0u);
+        }
         if (view != nullptr) {
             view->sorted_keys = sorted_keys;
             view->sorted_particle_indices = sorted_particle_indices;
             view->cell_offsets = cell_offsets;
-            view->sorted_boundary_keys = nullptr;
-            view->sorted_boundary_indices = nullptr;
+            view->sorted_boundary_keys = sorted_boundary_keys;
+            view->sorted_boundary_indices = sorted_boundary_indices;
             view->boundary_cell_offsets = boundary_cell_offsets;
             view->particle_count = n;
-            view->boundary_count = 0u;
+            view->boundary_count = b;
             view->cells_x = geo.resolution[0];
             view->cells_y = geo.resolution[1];
             view->cells_z = geo.resolution[2];
@@ -177,7 +231,7 @@ phx_status phx_build_neighborhood(phx_stream stream, const phx_domain_configurat
 phx_status phx_density_forward(phx_stream stream, uint32_t particle_count, float support_radius, int pbf_kernel,
                                phx_const_vector_view topology_positions, phx_const_vector_view positions,
                                phx_particle_parameters parameters, phx_neighborhood_view neighborhood,
-                               float* densities) {
+                               phx_boundary_view boundary, float* densities) {
     return guarded([&] {
         if (particle_count == 0) return;
         phx::require_vec(topology_positions, "topology positions");
@@ -186,18 +240,37 @@ phx_status phx_density_forward(phx_stream stream, uint32_t particle_count, float
                      "null masses or densities");
         phx::require(neighborhood.sorted_particle_indices != nullptr && neighborhood.cell_offsets != nullptr,
                      PHX_ERROR_INVALID_ARGUMENT, "null neighbourhood arrays");
-        phx::require(neighborhood.boundary_count == 0u, PHX_ERROR_UNSUPPORTED, "boundary particles are not supported yet");
+        phx::require(neighborhood.boundary_count == boundary.count, PHX_ERROR_INVALID_ARGUMENT,
+                     "boundary count does not match the neighbourhood's");
+        if (boundary.count > 0u) {
+            phx::require(neighborhood.sorted_boundary_indices != nullptr && neighborhood.boundary_cell_offsets != nullptr,
+                         PHX_ERROR_INVALID_ARGUMENT, "null neighbourhood boundary arrays");
+            phx::require(boundary.position_x != nullptr && boundary.position_y != nullptr &&
+                             boundary.position_z != nullptr && boundary.velocity_x != nullptr &&
+                             boundary.velocity_y != nullptr && boundary.velocity_z != nullptr &&
+                             boundary.volumes != nullptr && parameters.rest_densities != nullptr,
+                         PHX_ERROR_INVALID_ARGUMENT, "null boundary arrays or rest densities");
+        }
         phx::require(neighborhood.particle_count == particle_count, PHX_ERROR_INVALID_ARGUMENT,
                      "neighbourhood particle_count does not match");
         phx::require(neighborhood.cells_x > 0 && neighborhood.cells_y > 0 && neighborhood.cells_z > 0 &&
                          neighborhood.cell_size > 0.0F,
                      PHX_ERROR_INVALID_ARGUMENT, "empty neighbourhood grid");
-        RefGrid g{neighborhood.sorted_particle_indices, neighborhood.cell_offsets, neighborhood.cells_x,
-                  neighborhood.cells_y, neighborhood.cells_z, neighborhood.origin_x, neighborhood.origin_y,
-                  neighborhood.origin_z, neighborhood.cell_size};
+        const bool with_boundary = boundary.count > 0u;
+        RefGrid g{neighborhood.sorted_particle_indices,
+                  neighborhood.cell_offsets,
+                  with_boundary ? neighborhood.sorted_boundary_indices : nullptr,
+                  with_boundary ? neighborhood.boundary_cell_offsets : nullptr,
+                  neighborhood.cells_x,
+                  neighborhood.cells_y,
+                  neighborhood.cells_z,
+                  neighborhood.origin_x,
+                  neighborhood.origin_y,
+                  neighborhood.origin_z,
+                  neighborhood.cell_size};
         reference_density_kernel<<<phx::blocks_for(particle_count, kBlock), kBlock, 0, stream_of(stream)>>>(
             particle_count, support_radius, pbf_kernel, phx::cvec(topology_positions), phx::cvec(positions),
-            parameters.masses, g, densities);
+            parameters.masses, parameters.rest_densities, boundary, g, densities);
         PHX_LAUNCH_CHECK();
     });
 }

@@ -274,11 +274,22 @@ __device__ __forceinline__ void particle_cell(const SortedGrid& g, Float3 p, int
     z = cell_clamp((p.z - g.oz) / g.cell, g.cz);
 }
 
-// Visits the sorted candidates of the 27 cells around (cx, cy, cz) in the
-// reference order: offset_x outer, offset_y, offset_z inner; ascending sorted
-// position (= ascending original index) within a cell.
-template <class Visit>
-__device__ __forceinline__ void for_each_candidate(const SortedGrid& g, int cx, int cy, int cz, Visit&& visit) {
+// Sorted boundary samples of a grid build, the boundary half of
+// NeighborhoodView and BoundaryView (neighborhood/device.h:8-36): per cell
+// the samples in ascending original index, each stored as its position at
+// the step's time t, x0 + t v (boundary_position, neighborhood/device.cuh:
+// 16-22), with its volume, and its velocity.
+struct BoundaryGrid {
+    const uint32_t* offsets;  // C + 1 cell starts; null when the domain has no boundary samples
+    const float4* x;          // position at t (xyz), volume (w)
+    const float4* v;          // velocity (xyz)
+};
+
+// Calls cell(c) for the in-grid cells of the 27 around (cx, cy, cz) in the
+// reference order: offset_x outer, offset_y, offset_z inner (cell_range,
+// neighborhood/device.cuh:32-42, skips cells outside the grid).
+template <class Cell>
+__device__ __forceinline__ void for_each_cell(const SortedGrid& g, int cx, int cy, int cz, Cell&& cell) {
     for (int ox = -1; ox <= 1; ++ox) {
         const int X = cx + ox;
         if (X < 0 || X >= static_cast<int>(g.cx)) continue;
@@ -289,13 +300,38 @@ __device__ __forceinline__ void for_each_candidate(const SortedGrid& g, int cx, 
             for (int oz = -1; oz <= 1; ++oz) {
                 const int Z = cz + oz;
                 if (Z < 0 || Z >= static_cast<int>(g.cz)) continue;
-                const uint32_t c = static_cast<uint32_t>(Z) * g.cy * g.cx + row;
-                const uint32_t first = g.offsets[c];
-                const uint32_t last = g.offsets[c + 1u];
-                for (uint32_t s = first; s < last; ++s) visit(s);
+                cell(static_cast<uint32_t>(Z) * g.cy * g.cx + row);
             }
         }
     }
+}
+
+// Visits the sorted candidates of the 27 cells around (cx, cy, cz) in the
+// reference order; ascending sorted position (= ascending original index)
+// within a cell.
+template <class Visit>
+__device__ __forceinline__ void for_each_candidate(const SortedGrid& g, int cx, int cy, int cz, Visit&& visit) {
+    for_each_cell(g, cx, cy, cz, [&](uint32_t c) {
+        const uint32_t first = g.offsets[c];
+        const uint32_t last = g.offsets[c + 1u];
+        for (uint32_t s = first; s < last; ++s) visit(s);
+    });
+}
+
+// The same walk with each cell's boundary samples visited after its
+// particles, the order of every reference neighbour loop (e.g.
+// density/density.cu:28-37).
+template <class Visit, class VisitBoundary>
+__device__ __forceinline__ void for_each_candidate(const SortedGrid& g, const BoundaryGrid& b, int cx, int cy, int cz,
+                                                   Visit&& visit, VisitBoundary&& visit_boundary) {
+    for_each_cell(g, cx, cy, cz, [&](uint32_t c) {
+        const uint32_t first = g.offsets[c];
+        const uint32_t last = g.offsets[c + 1u];
+        for (uint32_t s = first; s < last; ++s) visit(s);
+        const uint32_t bfirst = b.offsets[c];
+        const uint32_t blast = b.offsets[c + 1u];
+        for (uint32_t s = bfirst; s < blast; ++s) visit_boundary(s);
+    });
 }
 
 }  // namespace phx

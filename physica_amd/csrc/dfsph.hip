@@ -113,6 +113,10 @@ DfsphSolver::DfsphSolver(const phx_domain_configuration* domain, const phx_dfsph
             PHX_ERROR_INVALID_ARGUMENT, "tolerance mode needs 1 <= max_iterations < 2048");
     require(config_.divergence_iterations + config_.density_iterations < 4096u, PHX_ERROR_INVALID_ARGUMENT,
             "too many pressure iterations");
+    require(domain_.config.boundary_particle_count == 0u || mode == PHX_MODE_FORWARD, PHX_ERROR_UNSUPPORTED,
+            "boundary particles are supported in forward mode only");
+    boundary_.upload(domain_.config);
+    domain_.config.boundary_particles = nullptr;  // the caller's host array is not kept
     gradient_norm_ = reference_gradient_norm(domain_.config);
     constants_ = kernel_constants(domain_.config.support_radius);
     const uint32_t n = domain_.config.particle_count;
@@ -202,20 +206,24 @@ void DfsphSolver::forward_step(const phx_sph_dfsph_state& state, const phx_contr
     }
     {
         PassScope scope(profiler_, stream_, kDfsphGrid);
-        cache.neighborhood.build(stream_, domain_, step, cvec(state.particles.positions), nullptr, gather.get());
+        cache.neighborhood.build(stream_, domain_, step, cvec(state.particles.positions), nullptr, gather.get(),
+                                 &boundary_);
         if (!warm) {
             fill_f32(stream_, p_div0, n, 0.0F);
             fill_f32(stream_, p_den0, n, 0.0F);
         }
-        masks_.begin_step(stream_);
+        // boundary samples: every pass walks the full stencil
+        masks_.begin_step(stream_, boundary_.count() > 0u);
     }
     const SortedGrid grid = cache.neighborhood.grid();
+    const BoundaryGrid boundary = cache.neighborhood.boundary();
     const uint32_t blocks = pass_blocks(n);
 
     MaskDensityArgs ma{n,
                        constants_,
                        masks_.t_skin(),
                        grid,
+                       boundary,
                        xm,
                        exact,
                        skin,
@@ -224,7 +232,8 @@ void DfsphSolver::forward_step(const phx_sph_dfsph_state& state, const phx_contr
                        cache.densities.data(),
                        t_div_.data(),
                        t_den_.data(),
-                       masks_.flags()};
+                       masks_.flags(),
+                       rho0};
     launch_mask_density<false>(profiler_, stream_, blocks, ma, kDfsphDensity);
 
     ForceArgs fa{};
@@ -249,6 +258,8 @@ void DfsphSolver::forward_step(const phx_sph_dfsph_state& state, const phx_contr
     fa.warm_div_out = next.method.warm_divergence_pressure;
     fa.p_den = rec ? cache.pressures(1u, ne) : p_den_.data();
     fa.warm_den_out = next.method.warm_density_pressure;
+    fa.boundary = boundary;
+    fa.rest_density = rho0;
     // prediction `slot` writes x* (recorded per slot) and, recorded, its collision bits
     auto set_slot = [&](uint32_t slot) {
         fa.slot = slot;
@@ -268,6 +279,8 @@ void DfsphSolver::forward_step(const phx_sph_dfsph_state& state, const phx_contr
     pa.flags = masks_.flags();
     pa.rho = cache.densities.data();
     pa.volume_density = volume_rest_ ? rho0 : cache.densities.data();
+    pa.boundary = boundary;
+    pa.rest_density = rho0;
     // update `it` of phase `phase` reads the x* of prediction `slot`
     auto set_update = [&](uint32_t phase, uint32_t it, uint32_t slot) {
         pa.slot = slot;

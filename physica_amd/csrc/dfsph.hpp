@@ -141,6 +141,7 @@ class DfsphSolver {
     VecBuffer u_;
     DeviceBuffer<float> rho0_, relax_div_, relax_den_, p_div_, p_den_, t_div_, t_den_;
     MaskSet masks_;
+    BoundarySamples boundary_;  // the domain's boundary samples on the device
     DfsphDerivativeWorkspace dw_;
 };
 

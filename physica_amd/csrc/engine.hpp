@@ -8,6 +8,7 @@
 #include <cstdint>
 #include <stdexcept>
 #include <string>
+#include <vector>
 
 #include "device_math.hpp"
 #include "physica_amd/physica.h"
@@ -210,6 +211,52 @@ GridGeometry build_grid(hipStream_t stream, const Domain& domain, uint64_t step,
                         const PredictInput* predict, const Channels& gather, const GridScratch& scratch, uint32_t* perm,
                         uint32_t* offsets);
 
+// Device arrays of the boundary samples in original order (BoundaryView,
+// neighborhood/device.h:26-36, without the time).
+struct BoundarySource {
+    const float* x[3];
+    const float* v[3];
+    const float* volume;
+};
+
+// The boundary samples of a configuration as seven host arrays of B floats
+// (x, y, z, vx, vy, vz, volume), checked finite.
+std::vector<float> pack_boundary_samples(const phx_domain_configuration& config);
+// The seven arrays of a packed copy at `base` on the device.
+BoundarySource boundary_source(const float* base, uint32_t count);
+
+// Device copy of DomainConfiguration::boundary_particles (DeviceBoundary,
+// domain/domain.ixx:25-39 and domain/domain.cpp): SoA position, velocity,
+// volume, uploaded once from the host array at solver creation.
+class BoundarySamples {
+  public:
+    void upload(const phx_domain_configuration& config);
+    uint32_t count() const { return count_; }
+    BoundarySource source() const;
+
+  private:
+    DeviceBuffer<float> data_;  // x, y, z, vx, vy, vz, volume, each count_ long
+    uint32_t count_ = 0;
+};
+
+// Scratch and results of the boundary half of a grid build.
+struct BoundaryScratch {
+    uint32_t* keys;      // cell key per sample, original order
+    uint32_t* slots;     // arrival slot within the cell
+    uint32_t* unstable;  // samples grouped by cell, arbitrary order inside
+};
+size_t boundary_scratch_bytes(uint32_t samples);
+BoundaryScratch carve_boundary_scratch(void* base, size_t bytes, uint32_t samples);
+
+// The boundary half of NeighborhoodSearch::build (neighborhood.cu:63-69):
+// keys of the samples moved to x0 + t v, a stable counting sort by cell
+// (the fluid build's histogram and scan buffers are reused), perm[B] (sorted
+// -> original index), offsets[C+1], and, when x/v are non-null, the sorted
+// records of BoundaryGrid.
+void build_boundary_grid(hipStream_t stream, const Domain& domain, uint64_t step, const BoundarySource& samples,
+                         uint32_t count, const BoundaryScratch& scratch, const GridScratch& grid_scratch,
+                         uint32_t* perm, uint32_t* offsets, float4* x, float4* v);
+
 // Owning grid state used by the solvers: scratch plus the permutation and
 // offsets of the last build (kept in the step cache for replay).
 struct Neighborhood {
@@ -231,16 +278,43 @@ struct Neighborhood {
         scratch_bytes.resize(bytes);
         scratch = carve_grid_scratch(scratch_bytes.data(), bytes, n, c);
     }
+    // `boundary` (may be null or empty) adds the sorted boundary samples.
     void build(hipStream_t stream, const Domain& domain, uint64_t step, CVec positions, const PredictInput* predict,
-               const Channels& gather) {
+               const Channels& gather, const BoundarySamples* boundary = nullptr) {
         allocate(domain.config.particle_count, domain.cells);
         geometry = build_grid(stream, domain, step, positions, predict, gather, scratch, perm.data(), offsets.data());
+        boundary_count = boundary != nullptr ? boundary->count() : 0u;
+        if (boundary_count == 0u) return;
+        if (boundary_perm.size() != boundary_count || boundary_offsets.size() != static_cast<size_t>(cells) + 1u) {
+            boundary_perm.resize(boundary_count);
+            boundary_offsets.resize(static_cast<size_t>(cells) + 1u);
+            boundary_x.resize(boundary_count);
+            boundary_v.resize(boundary_count);
+            const size_t bytes = boundary_scratch_bytes(boundary_count);
+            boundary_scratch_bytes_.resize(bytes);
+            boundary_scratch = carve_boundary_scratch(boundary_scratch_bytes_.data(), bytes, boundary_count);
+        }
+        build_boundary_grid(stream, domain, step, boundary->source(), boundary_count, boundary_scratch, scratch,
+                            boundary_perm.data(), boundary_offsets.data(), boundary_x.data(), boundary_v.data());
     }
     SortedGrid grid() const {
         return SortedGrid{offsets.data(),      geometry.resolution[0], geometry.resolution[1],
                           geometry.resolution[2], geometry.origin[0],     geometry.origin[1],
                           geometry.origin[2],     geometry.cell_size};
     }
+    BoundaryGrid boundary() const {
+        if (boundary_count == 0u) return BoundaryGrid{nullptr, nullptr, nullptr};
+        return BoundaryGrid{boundary_offsets.data(), boundary_x.data(), boundary_v.data()};
+    }
+
+    uint32_t boundary_count = 0;
+    DeviceBuffer<uint32_t> boundary_perm;
+    DeviceBuffer<uint32_t> boundary_offsets;
+    DeviceBuffer<float4> boundary_x, boundary_v;
+    BoundaryScratch boundary_scratch{};
+
+  private:
+    DeviceBuffer<uint8_t> boundary_scratch_bytes_;
 };
 
 // Device-wide exclusive scan of count u32 values (in and out may alias).

@@ -7,6 +7,7 @@
 // gives the offsets directly, and a stable order within a cell comes from
 // ranking each particle by original index among its cell's members.
 #include <cmath>
+#include <vector>
 
 #include "engine.hpp"
 
@@ -35,6 +36,23 @@ struct KeyBox {
     uint32_t res[3];
 };
 
+// key = (cz Cy + cy) Cx + cx of the cell of p - (min + t v_box), each axis
+// clamped into the grid (key_kernel, neighborhood.cu:20-39).
+__device__ __forceinline__ uint32_t cell_key(const KeyBox& box, Float3 p) {
+    const int cx = cell_clamp((p.x - box.minimum[0] - box.time * box.velocity[0]) / box.h, box.res[0]);
+    const int cy = cell_clamp((p.y - box.minimum[1] - box.time * box.velocity[1]) / box.h, box.res[1]);
+    const int cz = cell_clamp((p.z - box.minimum[2] - box.time * box.velocity[2]) / box.h, box.res[2]);
+    return (static_cast<uint32_t>(cz) * box.res[1] + static_cast<uint32_t>(cy)) * box.res[0] +
+           static_cast<uint32_t>(cx);
+}
+
+// boundary_position (neighborhood/device.cuh:16-22): x0 + t v per axis, the
+// product rounded before the sum.
+__device__ __forceinline__ Float3 moved_sample(const BoundarySource& src, float time, uint32_t i) {
+    return Float3{src.x[0][i] + time * src.v[0][i], src.x[1][i] + time * src.v[1][i],
+                  src.x[2][i] + time * src.v[2][i]};
+}
+
 // key_kernel, neighborhood.cu:20-39, fused with the cell histogram.
 template <bool kPredict>
 __global__ void __launch_bounds__(kBlock)
@@ -48,13 +66,43 @@ __global__ void __launch_bounds__(kBlock)
         p = add(p, scale(pv, predict.dt));
         store(predict.predicted, i, p);
     }
-    const int cx = cell_clamp((p.x - box.minimum[0] - box.time * box.velocity[0]) / box.h, box.res[0]);
-    const int cy = cell_clamp((p.y - box.minimum[1] - box.time * box.velocity[1]) / box.h, box.res[1]);
-    const int cz = cell_clamp((p.z - box.minimum[2] - box.time * box.velocity[2]) / box.h, box.res[2]);
-    const uint32_t key = (static_cast<uint32_t>(cz) * box.res[1] + static_cast<uint32_t>(cy)) * box.res[0] +
-                         static_cast<uint32_t>(cx);
+    const uint32_t key = cell_key(box, p);
     keys[i] = key;
     slots[i] = atomicAdd(&counts[key], 1u);
+}
+
+// key_kernel with moving_positions (neighborhood.cu:20-39 as called at
+// :63-69): the sample is first moved to x0 + t v.
+__global__ void __launch_bounds__(kBlock)
+    boundary_key_count_kernel(uint32_t count, KeyBox box, BoundarySource src, uint32_t* keys, uint32_t* slots,
+                              uint32_t* counts) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const Float3 p = moved_sample(src, box.time, i);
+    const uint32_t key = cell_key(box, p);
+    keys[i] = key;
+    slots[i] = atomicAdd(&counts[key], 1u);
+}
+
+// Stable rank of each sample within its cell (as rank_gather_kernel), then
+// its sorted record: position at t with volume, and velocity.
+__global__ void __launch_bounds__(kBlock)
+    boundary_rank_kernel(uint32_t count, float time, const uint32_t* keys, const uint32_t* offsets,
+                         const uint32_t* unstable, BoundarySource src, uint32_t* perm, float4* x, float4* v) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const uint32_t key = keys[i];
+    const uint32_t first = offsets[key];
+    const uint32_t last = offsets[key + 1u];
+    uint32_t rank = 0;
+    for (uint32_t s = first; s < last; ++s) rank += unstable[s] < i ? 1u : 0u;
+    const uint32_t pos = first + rank;
+    perm[pos] = i;
+    if (x != nullptr) {
+        const Float3 p = moved_sample(src, time, i);
+        x[pos] = make_float4(p.x, p.y, p.z, src.volume[i]);
+        v[pos] = make_float4(src.v[0][i], src.v[1][i], src.v[2][i], 0.0F);
+    }
 }
 
 __global__ void __launch_bounds__(kBlock)
@@ -244,8 +292,8 @@ Domain make_domain(const phx_domain_configuration* config) {
     require(c.boundary.maximum.x > c.boundary.minimum.x && c.boundary.maximum.y > c.boundary.minimum.y &&
                 c.boundary.maximum.z > c.boundary.minimum.z,
             PHX_ERROR_INVALID_ARGUMENT, "boundary box must have positive extent");
-    require(c.boundary_particle_count == 0u, PHX_ERROR_UNSUPPORTED,
-            "boundary particles are not supported yet (boundary_particle_count must be 0)");
+    require(c.boundary_particle_count == 0u || c.boundary_particles != nullptr, PHX_ERROR_INVALID_ARGUMENT,
+            "null boundary_particles with a non-zero boundary_particle_count");
     grid_resolution(c, d.resolution);
     const uint64_t cells = static_cast<uint64_t>(d.resolution[0]) * d.resolution[1] * d.resolution[2];
     require(d.resolution[0] > 0 && d.resolution[1] > 0 && d.resolution[2] > 0, PHX_ERROR_INVALID_ARGUMENT,
@@ -327,6 +375,97 @@ GridScratch carve_grid_scratch(void* base, size_t bytes, uint32_t particles, uin
     return s;
 }
 
+namespace {
+KeyBox key_box(const Domain& domain, float time) {
+    const auto& c = domain.config;
+    KeyBox box{};
+    box.minimum[0] = c.boundary.minimum.x;
+    box.minimum[1] = c.boundary.minimum.y;
+    box.minimum[2] = c.boundary.minimum.z;
+    box.velocity[0] = c.boundary.velocity.x;
+    box.velocity[1] = c.boundary.velocity.y;
+    box.velocity[2] = c.boundary.velocity.z;
+    box.h = c.support_radius;
+    box.time = time;
+    for (int a = 0; a < 3; ++a) box.res[a] = domain.resolution[a];
+    return box;
+}
+}  // namespace
+
+std::vector<float> pack_boundary_samples(const phx_domain_configuration& config) {
+    const size_t b = config.boundary_particle_count;
+    std::vector<float> host(7u * b);
+    if (b == 0u) return host;
+    require(config.boundary_particles != nullptr, PHX_ERROR_INVALID_ARGUMENT, "null boundary_particles");
+    for (size_t i = 0; i < b; ++i) {
+        const phx_boundary_particle& q = config.boundary_particles[i];
+        const float values[7] = {q.position.x, q.position.y, q.position.z, q.velocity.x, q.velocity.y, q.velocity.z,
+                                 q.volume};
+        for (size_t k = 0; k < 7u; ++k) {
+            require(std::isfinite(values[k]), PHX_ERROR_INVALID_ARGUMENT, "non-finite boundary particle");
+            host[k * b + i] = values[k];
+        }
+    }
+    return host;
+}
+
+BoundarySource boundary_source(const float* base, uint32_t count) {
+    const size_t b = count;
+    return BoundarySource{{base, base + b, base + 2u * b}, {base + 3u * b, base + 4u * b, base + 5u * b},
+                          base + 6u * b};
+}
+
+void BoundarySamples::upload(const phx_domain_configuration& config) {
+    const std::vector<float> host = pack_boundary_samples(config);
+    count_ = config.boundary_particle_count;
+    if (count_ == 0u) return;
+    data_.resize(host.size());
+    PHX_HIP(hipMemcpy(data_.data(), host.data(), sizeof(float) * host.size(), hipMemcpyHostToDevice));
+}
+
+BoundarySource BoundarySamples::source() const { return boundary_source(data_.data(), count_); }
+
+size_t boundary_scratch_bytes(uint32_t samples) {
+    const size_t round = (static_cast<size_t>(samples) * 4u + 255u) & ~static_cast<size_t>(255u);
+    return round * 3u + 256u;
+}
+
+BoundaryScratch carve_boundary_scratch(void* base, size_t bytes, uint32_t samples) {
+    require(bytes >= boundary_scratch_bytes(samples), PHX_ERROR_WORKSPACE_TOO_SMALL,
+            "boundary grid workspace too small");
+    const size_t round = (static_cast<size_t>(samples) * 4u + 255u) & ~static_cast<size_t>(255u);
+    uintptr_t p = (reinterpret_cast<uintptr_t>(base) + 255u) & ~static_cast<uintptr_t>(255u);
+    BoundaryScratch s{};
+    s.keys = reinterpret_cast<uint32_t*>(p);
+    s.slots = reinterpret_cast<uint32_t*>(p + round);
+    s.unstable = reinterpret_cast<uint32_t*>(p + 2u * round);
+    return s;
+}
+
+void build_boundary_grid(hipStream_t stream, const Domain& domain, uint64_t step, const BoundarySource& samples,
+                         uint32_t count, const BoundaryScratch& scratch, const GridScratch& grid_scratch,
+                         uint32_t* perm, uint32_t* offsets, float4* x, float4* v) {
+    const uint32_t cells = domain.cells;
+    if (count == 0u) {
+        fill_u32(stream, offsets, static_cast<size_t>(cells) + 1u, 0u);
+        return;
+    }
+    const float time = static_cast<float>(step) * domain.config.time_step;
+    const KeyBox box = key_box(domain, time);
+    fill_u32(stream, grid_scratch.counts, static_cast<size_t>(cells) + 1u, 0u);
+    const uint32_t grid = blocks_for(count, kBlock);
+    boundary_key_count_kernel<<<grid, kBlock, 0, stream>>>(count, box, samples, scratch.keys, scratch.slots,
+                                                           grid_scratch.counts);
+    PHX_LAUNCH_CHECK();
+    exclusive_scan(stream, grid_scratch.counts, offsets, static_cast<size_t>(cells) + 1u, grid_scratch.partials);
+    scatter_unstable_kernel<<<grid, kBlock, 0, stream>>>(count, scratch.keys, scratch.slots, offsets,
+                                                         scratch.unstable);
+    PHX_LAUNCH_CHECK();
+    boundary_rank_kernel<<<grid, kBlock, 0, stream>>>(count, time, scratch.keys, offsets, scratch.unstable, samples,
+                                                      perm, x, v);
+    PHX_LAUNCH_CHECK();
+}
+
 GridGeometry build_grid(hipStream_t stream, const Domain& domain, uint64_t step, CVec positions,
                         const PredictInput* predict, const Channels& gather, const GridScratch& scratch, uint32_t* perm,
                         uint32_t* offsets) {
@@ -348,16 +487,7 @@ GridGeometry build_grid(hipStream_t stream, const Domain& domain, uint64_t step,
         fill_u32(stream, offsets, static_cast<size_t>(cells) + 1u, 0u);
         return geo;
     }
-    KeyBox box{};
-    box.minimum[0] = c.boundary.minimum.x;
-    box.minimum[1] = c.boundary.minimum.y;
-    box.minimum[2] = c.boundary.minimum.z;
-    box.velocity[0] = c.boundary.velocity.x;
-    box.velocity[1] = c.boundary.velocity.y;
-    box.velocity[2] = c.boundary.velocity.z;
-    box.h = c.support_radius;
-    box.time = time;
-    for (int a = 0; a < 3; ++a) box.res[a] = domain.resolution[a];
+    const KeyBox box = key_box(domain, time);
     const uint32_t grid = blocks_for(n, kBlock);
     if (predict != nullptr) {
         key_count_kernel<true><<<grid, kBlock, 0, stream>>>(n, box, positions, *predict, scratch.keys, scratch.slots,

@@ -41,7 +41,7 @@ void MaskSet::allocate(uint32_t n, uint32_t predicting_passes, float support_rad
 // Picks the skin from the largest predicted displacement of the step read
 // back last: twice the displacement (both particles of a pair move) with a
 // 30 % margin, in [0.02, 0.6] h.
-void MaskSet::begin_step(hipStream_t stream) {
+void MaskSet::begin_step(hipStream_t stream, bool full_stencil) {
     if (readback_pending_ && hipEventQuery(readback_) == hipSuccess) {
         readback_pending_ = false;
         const auto* flags = static_cast<const uint32_t*>(flags_host_.data());
@@ -55,6 +55,7 @@ void MaskSet::begin_step(hipStream_t stream) {
         skin_ = std::fmin(0.6F, std::fmax(0.02F, 2.6F * moved + 0.01F));
     }
     fill_u32(stream, flags_.data(), flag_count_, 0u);
+    if (full_stencil) fill_u32(stream, flags_.data() + kFlagOverflow, 1u, 1u);
 }
 
 void MaskSet::end_step(hipStream_t stream) {

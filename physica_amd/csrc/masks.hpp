@@ -50,8 +50,10 @@ class MaskSet {
     // Masks of n particles and flags for `predicting_passes` displacement
     // slots (one per pass that writes predicted positions).
     void allocate(uint32_t n, uint32_t predicting_passes, float support_radius);
-    // Adopts the skin of a completed read-back, if any, and clears the flags.
-    void begin_step(hipStream_t stream);
+    // Adopts the skin of a completed read-back, if any, and clears the flags;
+    // with full_stencil the step's masks are marked unusable (overflow), so
+    // every pass walks the whole 27-cell stencil.
+    void begin_step(hipStream_t stream, bool full_stencil = false);
     // Enqueues the read-back of this step's displacement flags.
     void end_step(hipStream_t stream);
 

@@ -342,6 +342,8 @@ PbfSolver::PbfSolver(const phx_domain_configuration* domain, const phx_pbf_confi
                      phx_execution_mode mode, hipStream_t stream)
     : domain_(make_domain(domain)), mode_(mode), stream_(stream) {
     require(configuration != nullptr, PHX_ERROR_INVALID_ARGUMENT, "null PBF configuration");
+    require(domain_.config.boundary_particle_count == 0u, PHX_ERROR_UNSUPPORTED,
+            "boundary particles are not supported by this solver yet");
     config_ = *configuration;
     require(config_.checkpoint_interval > 0u, PHX_ERROR_INVALID_ARGUMENT, "checkpoint_interval must be positive");
     require(mode == PHX_MODE_FORWARD || mode == PHX_MODE_DIFFERENTIABLE, PHX_ERROR_INVALID_ARGUMENT,

@@ -158,6 +158,8 @@ SphSolver::SphSolver(const phx_domain_configuration* domain, const phx_sph_confi
                      phx_execution_mode mode, hipStream_t stream)
     : domain_(make_domain(domain)), mode_(mode), stream_(stream) {
     require(configuration != nullptr, PHX_ERROR_INVALID_ARGUMENT, "null SPH configuration");
+    require(domain_.config.boundary_particle_count == 0u, PHX_ERROR_UNSUPPORTED,
+            "boundary particles are not supported by this solver yet");
     config_ = *configuration;
     require(config_.method == PHX_SPH_WCSPH || config_.method == PHX_SPH_PCISPH || config_.method == PHX_SPH_IISPH,
             PHX_ERROR_INVALID_ARGUMENT, "unknown SPH method");

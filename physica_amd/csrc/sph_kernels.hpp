@@ -25,6 +25,7 @@ struct MaskDensityArgs {
     KernelConstants kc;
     float t_skin;
     SortedGrid grid;
+    BoundaryGrid boundary;  // offsets null: no boundary samples
     const float4* xm;
     uint16_t* exact;
     uint16_t* skin;
@@ -34,7 +35,8 @@ struct MaskDensityArgs {
     float* t_div;
     float* t_den;
     uint32_t* flags;
-    // equation of state (WCSPH): p = EOS(rho), aux = (rho, p / rho^2, c, sigma)
+    // rho0 (boundary terms), and the equation of state (WCSPH): p = EOS(rho),
+    // aux = (rho, p / rho^2, c, sigma)
     const float* rest_density;
     const float* tait_exponent;
     float* pressures;
@@ -45,7 +47,9 @@ struct MaskDensityArgs {
 // topology = positions) over the exact mask and the first_term p/rho^2 of
 // pressure_forward_kernel (common.cu:171) for the given pressures, or with
 // kEos the Tait pressure of eos_forward_kernel (wcsph.cu:92-100) and its
-// first_term.
+// first_term. With boundary samples no masks are built (the host marks the
+// step's masks unusable) and the density walks the full stencil, each cell's
+// samples b after its particles adding rho0_i V_b W (density.cu:34-37).
 template <bool kEos>
 __global__ void __launch_bounds__(kBlock) mask_density_kernel(MaskDensityArgs a) {
     __shared__ uint16_t smask[kStencil * kBlock];
@@ -59,10 +63,21 @@ __global__ void __launch_bounds__(kBlock) mask_density_kernel(MaskDensityArgs a)
     int cx, cy, cz;
     particle_cell(a.grid, p, cx, cy, cz);
     uint32_t nonzero = 0u;
-    const bool complete = build_masks(a.grid, a.xm, a.n, i, p, cx, cy, cz, a.kc.sqrt_support, a.t_skin, a.exact,
-                                      a.skin, column, kBlock, &nonzero);
     float result = 0.0F;
-    if (complete) {
+    if (a.boundary.offsets != nullptr) {
+        const float rest = a.rest_density[i];
+        for_each_candidate(
+            a.grid, a.boundary, cx, cy, cz,
+            [&](uint32_t s) {
+                const float4 q = a.xm[s];
+                result += q.w * cubic_value(pair_distance(sub(p, xyz(q))), a.kc);
+            },
+            [&](uint32_t s) {
+                const float4 b = a.boundary.x[s];
+                result += rest * b.w * cubic_value(pair_distance(sub(p, xyz(b))), a.kc);
+            });
+    } else if (build_masks(a.grid, a.xm, a.n, i, p, cx, cy, cz, a.kc.sqrt_support, a.t_skin, a.exact, a.skin, column,
+                           kBlock, &nonzero)) {
         walk_masked(a.grid.offsets, cell_of(a.grid, cx, cy, cz), delta, column, kBlock, nonzero, [&](uint32_t s) {
             const float4 q = a.xm[s];
             const float dist = pair_distance(sub(p, xyz(q)));
@@ -91,16 +106,16 @@ __global__ void __launch_bounds__(kBlock) mask_density_kernel(MaskDensityArgs a)
     }
 }
 
-// Tolerance mode: the convergence state of one iteration of a pressure
-// phase. The projection adds its particles' compression errors
-// max(0, rho* - target) / rho0 into `error` (FP64); the last workgroup to
-// arrive marks the phase done at this iteration when the mean is within the
+// Tolerance mode: the convergence state of a pressure phase. The projection
+// adds its workgroup's compression errors max(0, rho* - target) / rho0 in
+// FP64 and stores the sum in partials[blockIdx.x]; converge_kernel then
+// marks the phase done at this iteration when the mean is within the
 // tolerance or the iteration is the last allowed one. Every pass of the
 // phase reads `done`: passes of later iterations (launched ahead by the
 // host) then return at once, and the force pass of the marked iteration ends
 // the phase.
 struct Convergence {
-    double* partials;   // null: fixed iteration counts; else one error sum per workgroup
+    double* partials;      // null: fixed iteration counts; else one error sum per workgroup
     const uint32_t* done;  // iteration that ended the phase; 0 while it runs
     const float* rest;  // rho0, the scale of the error
 };
@@ -164,6 +179,8 @@ struct ProjectionArgs {
     float* t;
     float* rs_out;  // optional (recorded step): the predicted densities
     Convergence cv;
+    BoundaryGrid boundary;     // offsets null: no boundary samples
+    const float* rest_density;  // rho0, for the boundary terms
 };
 
 // density at predicted positions (density.cu:18-40 with topology = x,
@@ -182,7 +199,7 @@ __global__ void __launch_bounds__(kBlock) projection_kernel(ProjectionArgs a) {
     const uint32_t i = xcd_block() * kBlock + threadIdx.x;
     double error = 0.0;
     if (i < a.n) {
-        const bool masked = a.flags[kFlagOverflow] == 0u &&
+        const bool masked = a.flags[kFlagOverflow] == 0u && a.boundary.offsets == nullptr &&
                             __uint_as_float(a.flags[kFlagDisplacement + a.slot]) <= a.limit2;
         const float4 self = a.xs[i];
         const Float3 ps = xyz(self);
@@ -200,6 +217,13 @@ __global__ void __launch_bounds__(kBlock) projection_kernel(ProjectionArgs a) {
                 a.grid.offsets, cell_of(a.grid, cx, cy, cz), delta, column, kBlock, nonzero, i,
                 [&](uint32_t s) { return a.xs[s]; },
                 [&](float4 q) { rs += q.w * cubic_value(pair_distance(sub(ps, xyz(q))), a.kc); });
+        } else if (a.boundary.offsets != nullptr) {
+            // the predicted density's boundary terms rho0_i V_b W(x*_i - x_b)
+            const float rest = a.rest_density[i];
+            for_each_candidate(a.grid, a.boundary, cx, cy, cz, visit, [&](uint32_t s) {
+                const float4 b = a.boundary.x[s];
+                rs += rest * b.w * cubic_value(pair_distance(sub(ps, xyz(b))), a.kc);
+            });
         } else {
             for_each_candidate(a.grid, cx, cy, cz, visit);
         }
@@ -260,6 +284,8 @@ struct ForceArgs {
     // pass of a later iteration than the one that ended the phase returns
     const uint32_t* done;
     uint32_t iteration;
+    BoundaryGrid boundary;      // offsets null: no boundary samples
+    const float* rest_density;  // rho0, for the boundary pressure terms
 };
 
 // Pressure acceleration (pressure_forward_kernel, common.cu:165-194),
@@ -267,7 +293,9 @@ struct ForceArgs {
 // (non_pressure_forward_kernel, common.cu:14-47), then the predict of the
 // next iteration (dfsph.cu:26-44) or, on the last pass, the next state.
 // cubic_gradient(d, h) is evaluated as f * d with f its radial factor, the
-// same operations as the reference's expression.
+// same operations as the reference's expression. Boundary samples add, after
+// each cell's particles, mu_i V_b lap W (v_b - v_i) (common.cu:38-43) and
+// grad W (-rho0_i V_b p_i / rho_i^2) (common.cu:185-190).
 template <bool kNonPressure, int kEpilogue>
 __global__ void __launch_bounds__(kBlock) force_kernel(ForceArgs a) {
     constexpr bool kAuto = kEpilogue == kDivAuto;
@@ -287,7 +315,7 @@ __global__ void __launch_bounds__(kBlock) force_kernel(ForceArgs a) {
     const bool active = i < a.n;
     float moved2 = 0.0F;
     if (active) {
-        const bool masked = a.flags[kFlagOverflow] == 0u;
+        const bool masked = a.flags[kFlagOverflow] == 0u && a.boundary.offsets == nullptr;
         const float4 self = a.xm[i];
         const Float3 p = xyz(self);
         const float4 vm = a.vmu[i];
@@ -342,14 +370,32 @@ __global__ void __launch_bounds__(kBlock) force_kernel(ForceArgs a) {
                     });
             }
         } else {
-            for_each_candidate(a.grid, cx, cy, cz, [&](uint32_t s) {
+            auto visit = [&](uint32_t s) {
                 if (s == i) return;
                 const float4 q = a.xm[s];
                 const float t1_s = kPressure ? a.t1[s] : 0.0F;
                 const float t2_s = kTwo ? a.t2[s] : 0.0F;
                 const Float3 d = sub(p, xyz(q));
                 pair(s, q, t1_s, t2_s, d, kPressure ? cubic_gradient_factor(pair_distance(d), a.kc) : 0.0F);
-            });
+            };
+            if (a.boundary.offsets != nullptr) {
+                const float rest = a.rest_density[i];
+                for_each_candidate(a.grid, a.boundary, cx, cy, cz, visit, [&](uint32_t s) {
+                    const float4 b = a.boundary.x[s];
+                    const Float3 d = sub(p, xyz(b));
+                    if constexpr (kNonPressure) {
+                        const float weight = vm.w * b.w * viscosity_laplacian_c(d, a.kc);
+                        acc = add(acc, scale(sub(xyz(a.boundary.v[s]), vel), weight));
+                    }
+                    if constexpr (kPressure) {
+                        const Float3 gradient = scale(d, cubic_gradient_factor(pair_distance(d), a.kc));
+                        ap1 = add(ap1, scale(gradient, -rest * b.w * t1_i));
+                        if (kTwo) ap2 = add(ap2, scale(gradient, -rest * b.w * t2_i));
+                    }
+                });
+            } else {
+                for_each_candidate(a.grid, cx, cy, cz, visit);
+            }
         }
         if constexpr (kNonPressure) {
             a.anp[i] = make_float4(acc.x, acc.y, acc.z, 0.0F);

@@ -13,7 +13,7 @@ import ctypes
 import math
 from dataclasses import dataclass, field
 from enum import IntEnum
-from typing import Optional, Sequence, Tuple
+from typing import NamedTuple, Optional, Sequence, Tuple
 
 import numpy as np
 import torch
@@ -1118,19 +1118,67 @@ def neighborhood_workspace_size(configuration: DomainConfiguration) -> int:
     return int(size.value)
 
 
-def build_neighborhood(configuration: DomainConfiguration, step_index: int, positions: VectorField, device="cuda"):
-    """NeighborhoodSearch::build in the reference layout.
+class DeviceBoundary:
+    """DeviceBoundary (domain/domain.ixx:124): the configuration's boundary
+    samples as device arrays in original order (positions and velocities
+    (3, B), volumes (B,), FP32)."""
 
-    Returns (sorted_keys int64[N], sorted_particle_indices int32[N],
-    cell_offsets int32[C+1], NeighborhoodView)."""
+    def __init__(self, configuration: DomainConfiguration, device="cuda"):
+        dev = torch.device(device)
+        _require_gpu(dev)
+        samples = list(configuration.boundary_particles)
+        b = len(samples)
+        pos = np.zeros((3, b), np.float32)
+        vel = np.zeros((3, b), np.float32)
+        vol = np.zeros(b, np.float32)
+        for k, (x, v, volume) in enumerate(samples):
+            pos[:, k] = x
+            vel[:, k] = v
+            vol[k] = volume
+        self.count = b
+        self.positions = torch.from_numpy(pos).to(dev)
+        self.velocities = torch.from_numpy(vel).to(dev)
+        self.volumes = torch.from_numpy(vol).to(dev)
+
+    def view(self, time: float) -> _capi.BoundaryView:
+        """BoundaryView at `time` (neighborhood/interop.h:29-41)."""
+        if self.count == 0:
+            return _capi.BoundaryView()
+        return _capi.BoundaryView(
+            _ptr(self.positions[0]), _ptr(self.positions[1]), _ptr(self.positions[2]), _ptr(self.velocities[0]),
+            _ptr(self.velocities[1]), _ptr(self.velocities[2]), _ptr(self.volumes), self.count, float(time))
+
+
+class Neighborhood(NamedTuple):
+    """Neighborhood (neighborhood/neighborhood.ixx:12-23) in the reference
+    layout, with its C view."""
+
+    keys: torch.Tensor
+    indices: torch.Tensor
+    offsets: torch.Tensor
+    view: _capi.NeighborhoodView
+    boundary_keys: torch.Tensor
+    boundary_indices: torch.Tensor
+    boundary_offsets: torch.Tensor
+    boundary_time: float
+
+
+def build_neighborhood(configuration: DomainConfiguration, step_index: int, positions: VectorField,
+                       device="cuda") -> Neighborhood:
+    """NeighborhoodSearch::build in the reference layout: sorted keys int64[N],
+    sorted particle indices int32[N], cell offsets int32[C+1], the same three
+    for the boundary samples (B, B, C+1) and the boundary time step * dt."""
     dev = torch.device(device)
     _require_gpu(dev)
     n = configuration.particle_count
+    b = len(configuration.boundary_particles)
     cells = math.prod(configuration.grid_resolution())
     workspace = torch.empty(neighborhood_workspace_size(configuration), dtype=torch.uint8, device=dev)
     keys = torch.empty(n, dtype=torch.int64, device=dev)
     idx = torch.empty(n, dtype=torch.int32, device=dev)
     off = torch.empty(cells + 1, dtype=torch.int32, device=dev)
+    bkeys = torch.empty(b, dtype=torch.int64, device=dev)
+    bidx = torch.empty(b, dtype=torch.int32, device=dev)
     boff = torch.empty(cells + 1, dtype=torch.int32, device=dev)
     view = _capi.NeighborhoodView()
     _capi.call(
@@ -1144,11 +1192,14 @@ def build_neighborhood(configuration: DomainConfiguration, step_index: int, posi
         _ptr(keys),
         _ptr(idx),
         _ptr(off),
+        _ptr(bkeys) if b else None,
+        _ptr(bidx) if b else None,
         _ptr(boff),
         ctypes.byref(view),
     )
-    view._keep = (workspace, keys, idx, off, boff)
-    return keys, idx, off, view
+    view._keep = (workspace, keys, idx, off, bkeys, bidx, boff)
+    time = float(np.float32(step_index) * np.float32(configuration.time_step))
+    return Neighborhood(keys, idx, off, view, bkeys, bidx, boff, time)
 
 
 def density_forward(
@@ -1159,9 +1210,13 @@ def density_forward(
     parameters: ParticleParameters,
     view: _capi.NeighborhoodView,
     densities: torch.Tensor,
+    boundary: Optional[DeviceBoundary] = None,
+    boundary_time: float = 0.0,
 ) -> None:
-    """density::sph_forward / pbf_forward (density/density.cpp:12-34)."""
+    """density::sph_forward / pbf_forward (density/density.cpp:12-34), with
+    the boundary samples at `boundary_time` when given."""
     dev = densities.device
+    bview = boundary.view(boundary_time) if boundary is not None else _capi.BoundaryView()
     _capi.call(
         "phx_density_forward",
         _stream_handle(dev),
@@ -1172,5 +1227,6 @@ def density_forward(
         positions.view(),
         parameters.to_c(),
         view,
+        bview,
         _ptr(densities),
     )

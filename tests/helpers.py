@@ -49,6 +49,33 @@ def dfsph_setup(scene, device, configuration=None, warm=None, mode=ExecutionMode
     return solver, state, nxt, control, params, cache
 
 
+def add_boundary_samples(scene, moving=False):
+    """Boundary samples on a lattice of the fluid spacing: a floor half a
+    spacing below the box floor under the block, and a wall half a spacing
+    beyond the block's +x face (moving toward the fluid at 0.3 m/s when
+    `moving`). Volumes are spacing^3. Sets the scene's
+    domain.boundary_particles and returns (positions, velocities, volumes) in
+    the oracle's layout."""
+    x = scene.positions
+    s = float(scene.domain.particle_radius) * 2.0
+    lo, hi = x.min(axis=1), x.max(axis=1)
+    gx = np.arange(lo[0] - 2 * s, hi[0] + 1.5 * s, s)
+    gz = np.arange(lo[2] - 2 * s, hi[2] + 2.5 * s, s)
+    fx, fz = np.meshgrid(gx, gz, indexing="ij")
+    floor = np.stack([fx.ravel(), np.full(fx.size, -0.5 * s), fz.ravel()])
+    gy = np.arange(0.0, hi[1] + 1.5 * s, s)
+    wy, wz = np.meshgrid(gy, gz, indexing="ij")
+    wall = np.stack([np.full(wy.size, hi[0] + 0.9 * s), wy.ravel(), wz.ravel()])
+    pos = np.concatenate([floor, wall], axis=1).astype(np.float32)
+    b = pos.shape[1]
+    vel = np.zeros((3, b), np.float32)
+    if moving:
+        vel[0, floor.shape[1]:] = -0.3
+    vol = np.full(b, s ** 3, np.float32)
+    scene.domain.boundary_particles = [(tuple(pos[:, k]), tuple(vel[:, k]), float(vol[k])) for k in range(b)]
+    return pos, vel, vol
+
+
 def dfsph_state_numpy(state):
     return {
         "positions": state.particles.positions.numpy(),

@@ -68,7 +68,8 @@ def test_grid_resolution_host_only():
         (cfg(support_radius=0.0), _capi.PHX_ERROR_INVALID_ARGUMENT, "support_radius"),
         (cfg(time_step=-1.0), _capi.PHX_ERROR_INVALID_ARGUMENT, "time_step"),
         (cfg(boundary=BoxBoundary((0, 0, 0), (0.0, 1, 1))), _capi.PHX_ERROR_INVALID_ARGUMENT, "extent"),
-        (cfg(boundary_particles=[((0.1, 0.1, 0.1), (0, 0, 0), 1e-6)]), _capi.PHX_ERROR_UNSUPPORTED, "boundary"),
+        (cfg(boundary_particles=[((0.1, float("nan"), 0.1), (0, 0, 0), 1e-6)]), _capi.PHX_ERROR_INVALID_ARGUMENT,
+         "non-finite boundary"),
         (cfg(support_radius=1e-5), _capi.PHX_ERROR_CAPACITY, "cells"),
     ],
 )
@@ -77,6 +78,34 @@ def test_invalid_domain_rejected_before_device_work(domain, status, fragment):
     assert got == status
     assert handle.value is None
     assert fragment in _capi.library().phx_last_error().decode()
+
+
+def test_boundary_particles_rejected_where_unsupported():
+    sample = [((0.1, 0.1, 0.1), (0, 0, 0), 1e-6)]
+    got, handle = create_dfsph(cfg(boundary_particles=sample), mode=1)
+    assert got == _capi.PHX_ERROR_UNSUPPORTED and handle.value is None
+    assert "forward mode only" in _capi.library().phx_last_error().decode()
+    h = ctypes.c_void_p()
+    got = _capi.library().phx_pbf_create(ctypes.byref(cfg(boundary_particles=sample).to_c()),
+                                         ctypes.byref(PBF.Configuration().to_c()), 0, None, ctypes.byref(h))
+    assert got == _capi.PHX_ERROR_UNSUPPORTED and h.value is None
+    # a non-zero count with a null host array
+    c = cfg().to_c()
+    c.boundary_particle_count = 3
+    size = ctypes.c_size_t()
+    assert _capi.library().phx_neighborhood_workspace_size(ctypes.byref(c), ctypes.byref(size)) == \
+        _capi.PHX_ERROR_INVALID_ARGUMENT
+
+
+def test_neighborhood_workspace_grows_with_boundary_samples():
+    lib = _capi.library()
+    plain, with_b = ctypes.c_size_t(), ctypes.c_size_t()
+    assert lib.phx_neighborhood_workspace_size(ctypes.byref(cfg().to_c()), ctypes.byref(plain)) == 0
+    samples = [((0.01 * k, 0.0, 0.1), (0, 0, 0), 1e-6) for k in range(500)]
+    assert lib.phx_neighborhood_workspace_size(ctypes.byref(cfg(boundary_particles=samples).to_c()),
+                                               ctypes.byref(with_b)) == 0
+    # keys, slots, ranks, the seven packed arrays and the permutation
+    assert with_b.value >= plain.value + 500 * 4 * 11
 
 
 def test_invalid_method_configuration_rejected():

@@ -1,11 +1,12 @@
 """GPU neighbour search and reference-layout density vs the oracle: keys,
-sorted order and cell offsets bit-exact, densities bit-exact."""
+sorted order and cell offsets bit-exact, for the particles and for static
+and moving boundary samples; densities bit-exact, boundary terms included."""
 import numpy as np
 import pytest
 import torch
 
 import oracle
-from physica_amd import build_neighborhood, density_forward
+from physica_amd import DeviceBoundary, build_neighborhood, density_forward
 from physica_amd.particle import BoxBoundary, DomainConfiguration, ParticleParameters, VectorField
 from helpers import bits_equal
 
@@ -41,7 +42,7 @@ def test_build_neighborhood_bit_exact(gpu_device, kind, n):
     pos = positions(kind, n, rng, (1.0, 0.6, 0.8))
     field = VectorField.zeros(n, gpu_device)
     field.upload(pos)
-    keys, idx, off, view = build_neighborhood(cfg, 3, field, gpu_device)
+    keys, idx, off, view = build_neighborhood(cfg, 3, field, gpu_device)[:4]
     torch.cuda.synchronize()
     ok, oi, oo = oracle.build_neighborhood(oracle.make_domain(cfg), 3, pos)
     assert np.array_equal(keys.cpu().numpy().astype(np.uint64), ok)
@@ -58,7 +59,7 @@ def test_moving_box(gpu_device):
     field = VectorField.zeros(n, gpu_device)
     field.upload(pos)
     for step in (0, 7):
-        keys, idx, off, view = build_neighborhood(cfg, step, field, gpu_device)
+        keys, idx, off, view = build_neighborhood(cfg, step, field, gpu_device)[:4]
         ok, oi, oo = oracle.build_neighborhood(oracle.make_domain(cfg), step, pos)
         assert np.array_equal(idx.cpu().numpy().astype(np.uint32), oi)
         assert np.array_equal(off.cpu().numpy().astype(np.uint32), oo)
@@ -78,8 +79,83 @@ def test_reference_layout_density_bit_exact(gpu_device, pbf_kernel):
     pf.upload(pos)
     z = torch.zeros(n, device=gpu_device)
     params = ParticleParameters(torch.from_numpy(m).to(gpu_device), z + 1000.0, z, z)
-    keys, idx, off, view = build_neighborhood(cfg, 0, tf, gpu_device)
+    keys, idx, off, view = build_neighborhood(cfg, 0, tf, gpu_device)[:4]
     rho = torch.empty(n, device=gpu_device)
     density_forward(cfg.support_radius, pbf_kernel, tf, pf, params, view, rho)
     expected = oracle.density(oracle.make_domain(cfg), 0, pbf_kernel, topo, pos, m, np.full(n, 1000.0, np.float32))
+    assert bits_equal(rho.cpu().numpy(), expected)
+
+
+def boundary_samples(rng, count, box=(1.0, 0.6, 0.8), moving=False):
+    """A floor plane and a wall plane of samples (ascending original index
+    mixed over cells), plus a few outside the box; volumes ~ spacing^3."""
+    k = count // 2
+    floor = np.stack([rng.uniform(0, box[0], k), np.zeros(k), rng.uniform(0, box[2], k)])
+    wall = np.stack([np.full(count - k, box[0]), rng.uniform(0, box[1], count - k), rng.uniform(0, box[2], count - k)])
+    pos = np.concatenate([floor, wall], axis=1)
+    outside = min(7, count)
+    pos[:, :outside] = rng.uniform(-0.2, 1.2, size=(3, outside)) * np.array(box)[:, None]
+    order = rng.permutation(count)
+    pos = pos[:, order].astype(np.float32)
+    vel = np.zeros((3, count), np.float32)
+    if moving:
+        vel[:] = rng.uniform(-0.5, 0.5, size=(3, count)).astype(np.float32)
+    vol = rng.uniform(0.5, 1.5, count).astype(np.float32) * np.float32(0.02 ** 3)
+    return pos, vel, vol
+
+
+def with_boundary(cfg, samples):
+    pos, vel, vol = samples
+    cfg.boundary_particles = [(tuple(pos[:, k]), tuple(vel[:, k]), float(vol[k])) for k in range(vol.shape[0])]
+    return cfg
+
+
+@pytest.mark.parametrize("moving", [False, True])
+@pytest.mark.parametrize("count", [1, 3000])
+def test_boundary_neighborhood_bit_exact(gpu_device, moving, count):
+    print(f"boundary neighbourhood moving={moving} count={count}", flush=True)
+    n = 4000
+    rng = np.random.default_rng(31 + count + moving)
+    samples = boundary_samples(rng, count, moving=moving)
+    cfg = with_boundary(domain(n, vel=(0.2, 0.0, -0.1) if moving else (0.0, 0.0, 0.0), dt=0.05), samples)
+    pos = positions("uniform", n, rng, (1.0, 0.6, 0.8))
+    field = VectorField.zeros(n, gpu_device)
+    field.upload(pos)
+    od = oracle.make_domain(cfg, samples)
+    for step in (0, 5):
+        nb = build_neighborhood(cfg, step, field, gpu_device)
+        torch.cuda.synchronize()
+        ok, oi, oo, bk, bi, bo = oracle.build_neighborhood(od, step, pos, boundary=True)
+        assert np.array_equal(nb.indices.cpu().numpy().astype(np.uint32), oi)
+        assert np.array_equal(nb.offsets.cpu().numpy().astype(np.uint32), oo)
+        assert np.array_equal(nb.boundary_keys.cpu().numpy().astype(np.uint64), bk)
+        assert np.array_equal(nb.boundary_indices.cpu().numpy().astype(np.uint32), bi)
+        assert np.array_equal(nb.boundary_offsets.cpu().numpy().astype(np.uint32), bo)
+        assert nb.view.boundary_count == count
+        assert np.float32(nb.boundary_time) == np.float32(step) * np.float32(cfg.time_step)
+
+
+@pytest.mark.parametrize("pbf_kernel", [False, True])
+def test_reference_layout_density_with_boundary_bit_exact(gpu_device, pbf_kernel):
+    n = 20000
+    rng = np.random.default_rng(19)
+    samples = boundary_samples(rng, 5000, moving=True)
+    cfg = with_boundary(domain(n, dt=0.02), samples)
+    topo = positions("uniform", n, rng, (1.0, 0.6, 0.8))
+    pos = (topo + rng.uniform(-0.003, 0.003, size=topo.shape)).astype(np.float32)
+    m = rng.uniform(0.5, 1.5, size=n).astype(np.float32)
+    r0 = rng.uniform(900.0, 1100.0, size=n).astype(np.float32)
+    tf, pf = VectorField.zeros(n, gpu_device), VectorField.zeros(n, gpu_device)
+    tf.upload(topo)
+    pf.upload(pos)
+    z = torch.zeros(n, device=gpu_device)
+    params = ParticleParameters(torch.from_numpy(m).to(gpu_device), torch.from_numpy(r0).to(gpu_device), z, z)
+    step = 4
+    nb = build_neighborhood(cfg, step, tf, gpu_device)
+    boundary = DeviceBoundary(cfg, gpu_device)
+    rho = torch.empty(n, device=gpu_device)
+    density_forward(cfg.support_radius, pbf_kernel, tf, pf, params, nb.view, rho, boundary, nb.boundary_time)
+    expected = oracle.density(oracle.make_domain(cfg, samples), step, pbf_kernel, topo, pos, m, r0)
+    without = oracle.density(oracle.make_domain(cfg), step, pbf_kernel, topo, pos, m, r0)
+    assert np.sum(expected != without) > 1000  # the boundary terms are exercised
     assert bits_equal(rho.cpu().numpy(), expected)
