@@ -44,13 +44,20 @@ struct WcsphForceArgs {
     const uint32_t* perm;
     Vec next_x, next_v;
     uint8_t* collisions;  // optional (recorded step): box-collision bits of the integration
+    BoundaryGrid boundary;          // offsets null: no boundary samples
+    const float* rest_density;      // rho0, for the boundary terms
+    const float* boundary_tension;  // WCSPH boundary_surface_tension, for the boundary cohesion
 };
 
 // pressure_forward_kernel (common.cu:165-194), viscosity_forward_kernel
 // (wcsph.cu:136-174), surface_forward_kernel (wcsph.cu:306-339) as three
 // accumulators over one neighbour walk, then the sums of WCSPH::forward
 // (sph.cpp:219-222), external_forward_kernel (wcsph.cu:86-90) and
-// integrate_forward_kernel (common.cu:302-320).
+// integrate_forward_kernel (common.cu:302-320). A boundary sample b, after
+// each cell's particles, adds the pressure term grad W (-rho0_i V_b p_i /
+// rho_i^2), on approach the artificial viscosity with
+// nu = mu_i h c_i / rho_i and mass rho0_i V_b, and within h the cohesion
+// -sigma_b,i rho0_i V_b W / m_i d (wcsph.cu:160-170, 326-335).
 __global__ void __launch_bounds__(kBlock) wcsph_force_kernel(WcsphForceArgs a) {
     __shared__ uint16_t smask[kStencil * kBlock];
     __shared__ int delta[kStencil];
@@ -99,7 +106,28 @@ __global__ void __launch_bounds__(kBlock) wcsph_force_kernel(WcsphForceArgs a) {
     };
     int cx, cy, cz;
     particle_cell(a.grid, p, cx, cy, cz);
-    if (a.flags[kFlagOverflow] == 0u) {
+    if (a.boundary.offsets != nullptr) {
+        const float rest = a.rest_density[i];
+        const float tension = a.boundary_tension[i];
+        for_each_candidate(a.grid, a.boundary, cx, cy, cz, visit, [&](uint32_t s) {
+            const float4 b = a.boundary.x[s];
+            const Float3 d = sub(p, xyz(b));
+            const float dist = pair_distance(d);
+            const Float3 gradient = scale(d, cubic_gradient_factor(dist, a.kc));
+            ap = add(ap, scale(gradient, -rest * b.w * t_i));
+            const float compression = dot(sub(vel, xyz(a.boundary.v[s])), d);
+            if (!(compression >= 0.0F)) {
+                const float nu = vm.w * h * c_i / rho_i;
+                const float pi_value = -nu * compression / (dot(d, d) + softening);
+                av = add(av, scale(gradient, -rest * b.w * pi_value));
+            }
+            if (!(dist >= h)) {
+                const float weight = dist > a.diameter ? cubic_value(dist, a.kc) : contact_weight;
+                const float factor = -tension * rest * b.w * weight / self.w;
+                as = add(as, scale(d, factor));
+            }
+        });
+    } else if (a.flags[kFlagOverflow] == 0u) {
         uint16_t* column = smask + threadIdx.x;
         const uint32_t nonzero = load_lane_column(a.exact, a.n, i, column, kBlock);
         walk_masked(a.grid.offsets, cell_of(a.grid, cx, cy, cz), delta, column, kBlock, nonzero, visit);
@@ -158,13 +186,16 @@ SphSolver::SphSolver(const phx_domain_configuration* domain, const phx_sph_confi
                      phx_execution_mode mode, hipStream_t stream)
     : domain_(make_domain(domain)), mode_(mode), stream_(stream) {
     require(configuration != nullptr, PHX_ERROR_INVALID_ARGUMENT, "null SPH configuration");
-    require(domain_.config.boundary_particle_count == 0u, PHX_ERROR_UNSUPPORTED,
-            "boundary particles are not supported by this solver yet");
+
     config_ = *configuration;
     require(config_.method == PHX_SPH_WCSPH || config_.method == PHX_SPH_PCISPH || config_.method == PHX_SPH_IISPH,
             PHX_ERROR_INVALID_ARGUMENT, "unknown SPH method");
     require(mode == PHX_MODE_FORWARD || mode == PHX_MODE_DIFFERENTIABLE, PHX_ERROR_INVALID_ARGUMENT,
             "unknown execution mode");
+    require(domain_.config.boundary_particle_count == 0u || mode == PHX_MODE_FORWARD, PHX_ERROR_UNSUPPORTED,
+            "boundary particles are supported in forward mode only");
+    boundary_.upload(domain_.config);
+    domain_.config.boundary_particles = nullptr;  // the caller's host array is not kept
     if (config_.method != PHX_SPH_WCSPH) {
         require(config_.checkpoint_interval > 0u, PHX_ERROR_INVALID_ARGUMENT, "checkpoint_interval must be positive");
         require(config_.pressure_iterations < 4096u, PHX_ERROR_INVALID_ARGUMENT, "too many pressure iterations");
@@ -177,6 +208,7 @@ SphSolver::SphSolver(const phx_domain_configuration* domain, const phx_sph_confi
     for (DeviceBuffer<float4>* b : {&xm_, &vmu_, &xs_, &aux_}) b->resize(n);
     u_.resize(n);
     for (DeviceBuffer<float>* b : {&rho0_, &relax_, &tait_, &t_}) b->resize(n);
+    if (boundary_.count() > 0u && config_.method == PHX_SPH_WCSPH) btension_.resize(n);
     masks_.allocate(n, config_.pressure_iterations + 1u, domain_.config.support_radius);
     if (mode == PHX_MODE_DIFFERENTIABLE && config_.method != PHX_SPH_WCSPH) {
         // PCISPH::forward (sph.cpp:295-309) and IISPH::forward (sph.cpp:408-421)
@@ -317,12 +349,16 @@ void SphSolver::forward_wcsph(const phx_sph_state& state, const phx_control& con
     gather.add(mp.speed_of_sound, aux, 2);
     gather.add(pp.surface_tensions, aux, 3);
     gather.add(mp.tait_exponent, tait);
+    const bool with_boundary = boundary_.count() > 0u;
+    if (with_boundary) gather.add(mp.boundary_surface_tension, btension_.data());
     {
         PassScope scope(profiler_, stream_, kSphGrid);
-        cache.neighborhood.build(stream_, domain_, step, cvec(state.particles.positions), nullptr, gather.get());
-        masks_.begin_step(stream_);
+        cache.neighborhood.build(stream_, domain_, step, cvec(state.particles.positions), nullptr, gather.get(),
+                                 &boundary_);
+        masks_.begin_step(stream_, with_boundary);
     }
     const SortedGrid grid = cache.neighborhood.grid();
+    const BoundaryGrid boundary = cache.neighborhood.boundary();
     const uint32_t blocks = pass_blocks(n);
 
     MaskDensityArgs ma{};
@@ -330,6 +366,7 @@ void SphSolver::forward_wcsph(const phx_sph_state& state, const phx_control& con
     ma.kc = constants_;
     ma.t_skin = masks_.t_skin();
     ma.grid = grid;
+    ma.boundary = boundary;
     ma.xm = xm;
     ma.exact = exact;
     ma.skin = masks_.skin_mask();
@@ -359,6 +396,9 @@ void SphSolver::forward_wcsph(const phx_sph_state& state, const phx_control& con
     wa.next_x = vec(next.particles.positions);
     wa.next_v = vec(next.particles.velocities);
     wa.collisions = rec ? r->collisions.data() : nullptr;
+    wa.boundary = boundary;
+    wa.rest_density = rho0;
+    wa.boundary_tension = btension_.data();
     {
         PassScope scope(profiler_, stream_, kSphForces);
         wcsph_force_kernel<<<blocks, kBlock, 0, stream_>>>(wa);
@@ -391,13 +431,16 @@ void SphSolver::forward_pressure(const phx_sph_state& state, const phx_control& 
     gather.add(cvec(control.external_accelerations), u_.view());
     gather.add(pp.rest_densities, rho0_.data());
     gather.add(relaxation, relax_.data());
+    const bool with_boundary = boundary_.count() > 0u;
     {
         PassScope scope(profiler_, stream_, kSphGrid);
-        cache.neighborhood.build(stream_, domain_, step, cvec(state.particles.positions), nullptr, gather.get());
-        masks_.begin_step(stream_);
+        cache.neighborhood.build(stream_, domain_, step, cvec(state.particles.positions), nullptr, gather.get(),
+                                 &boundary_);
+        masks_.begin_step(stream_, with_boundary);
         fill_f32(stream_, cache.pressures.data(), n, 0.0F);  // clear_iteration (sph.cpp:298)
     }
     const SortedGrid grid = cache.neighborhood.grid();
+    const BoundaryGrid boundary = cache.neighborhood.boundary();
     const uint32_t blocks = pass_blocks(n);
 
     MaskDensityArgs ma{};
@@ -405,6 +448,8 @@ void SphSolver::forward_pressure(const phx_sph_state& state, const phx_control& 
     ma.kc = constants_;
     ma.t_skin = masks_.t_skin();
     ma.grid = grid;
+    ma.boundary = boundary;
+    ma.rest_density = rho0_.data();
     ma.xm = xm_.data();
     ma.exact = masks_.exact();
     ma.skin = masks_.skin_mask();
@@ -431,6 +476,8 @@ void SphSolver::forward_pressure(const phx_sph_state& state, const phx_control& 
     fa.perm = cache.neighborhood.perm.data();
     fa.next_x = vec(next.particles.positions);
     fa.next_v = vec(next.particles.velocities);
+    fa.boundary = boundary;
+    fa.rest_density = rho0_.data();
 
     ProjectionArgs pa{};
     pa.n = n;
@@ -450,6 +497,8 @@ void SphSolver::forward_pressure(const phx_sph_state& state, const phx_control& 
     pa.p_prev = cache.pressures.data();
     pa.p = cache.pressures.data();
     pa.t = t_.data();
+    pa.boundary = boundary;
+    pa.rest_density = rho0_.data();
 
     const uint32_t iterations = config_.pressure_iterations;
     uint32_t slot = 0;  // displacement slot written by the latest predicting pass

@@ -129,6 +129,8 @@ class SphSolver {
     DeviceBuffer<float4> xm_, vmu_, xs_, aux_;
     VecBuffer u_;
     DeviceBuffer<float> rho0_, relax_, tait_, t_;
+    DeviceBuffer<float> btension_;  // WCSPH boundary_surface_tension in cell order (boundary samples only)
+    BoundarySamples boundary_;      // the domain's boundary samples on the device
 };
 
 }  // namespace phx

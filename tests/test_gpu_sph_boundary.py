@@ -1,0 +1,75 @@
+"""SPH<PCISPH>, SPH<IISPH> and SPH<WCSPH>::forward_step with boundary
+particles on the GPU against the FP32 oracle, bit for bit: a floor of
+samples under two falling cubes and a wall beside them moving toward the
+fluid. The samples enter the density, viscosity, pressure and predicted
+densities of the pressure solvers, and WCSPH's Tait pressure, artificial
+viscosity and boundary cohesion (unit Tait exponent, so no powf)."""
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from physica_amd import scenes
+from physica_amd.particle import ExecutionMode, PCISPH, SPH
+from helpers import add_boundary_samples, bits_equal, sph_setup
+
+pytestmark = pytest.mark.gpu
+
+
+def scene_for(method, seed=7):
+    sc = scenes.two_cubes(method, shape=(10, 8, 10), seed=seed)
+    n = sc.count
+    # the lower cube just above the floor samples, so floor and wall both act
+    s = 2.0 * sc.domain.particle_radius
+    sc.positions[1] -= np.float32(sc.positions[1].min() - 0.6 * s)
+    rng = np.random.default_rng(seed)
+    sc.velocities = rng.normal(0, 0.3, size=(3, n)).astype(np.float32)
+    sc.velocities[1] -= np.float32(1.0)  # toward the floor samples
+    sc.parameters["viscosities"] = rng.uniform(0.0, 0.05, n).astype(np.float32)
+    if method == "wcsph":
+        sc.parameters["surface_tensions"] = rng.uniform(0.0, 2.0, n).astype(np.float32)
+        sc.parameters["boundary_surface_tension"] = rng.uniform(0.5, 2.0, n).astype(np.float32)
+        sc.parameters["speed_of_sound"] = rng.uniform(2.0, 3.0, n).astype(np.float32)
+        sc.parameters["tait_exponent"] = np.ones(n, dtype=np.float32)
+        sc.parameters["rest_densities"] = np.full(n, 970.0, dtype=np.float32)
+    else:
+        sc.parameters["relaxation"] = rng.uniform(0.3, 0.7, n).astype(np.float32)
+    return sc
+
+
+@pytest.mark.parametrize("method", ["pcisph", "iisph", "wcsph"])
+def test_rollout_with_boundary_bit_exact(gpu_device, method):
+    print(f"sph boundary {method}", flush=True)
+    sc = scene_for(method)
+    samples = add_boundary_samples(sc, moving=True)
+    solver, state, nxt, control, params, cache = sph_setup(sc, gpu_device, method, 4)
+    d = oracle.make_domain(sc.domain, samples)
+    dn = oracle.make_domain(sc.domain)
+    x, v = sc.positions, sc.velocities
+    u = np.zeros_like(x)
+
+    def reference(domain, step):
+        if method == "wcsph":
+            return oracle.wcsph_step(domain, step, x, v, u, sc.parameters)
+        return oracle.pressure_solver_step(domain, step, x, v, u, sc.parameters, method, 4)
+
+    for step in range(4):
+        solver.forward_step(state, control, params, nxt, cache)
+        torch.cuda.synchronize()
+        exp = reference(d, step)
+        if step == 0:
+            free = reference(dn, step)
+            assert np.sum(free["velocities"] != exp["velocities"]) > sc.count // 20
+        got = {"positions": nxt.particles.positions.numpy(), "velocities": nxt.particles.velocities.numpy()}
+        for name in ("positions", "velocities"):
+            assert bits_equal(got[name], exp[name]), (step, name)
+        assert bits_equal(cache.densities().cpu().numpy(), exp["densities"]), step
+        x, v = got["positions"], got["velocities"]
+        solver.copy_state(nxt, state)
+
+
+def test_boundary_particles_are_forward_only(gpu_device):
+    sc = scene_for("pcisph")
+    add_boundary_samples(sc)
+    with pytest.raises(RuntimeError, match="forward mode only"):
+        SPH(sc.domain, PCISPH.Configuration(), mode=ExecutionMode.differentiable, device=gpu_device)
