@@ -72,10 +72,15 @@ struct LambdaArgs {
     float t_exact;
     uint16_t* exact_out;
     uint32_t* exact_flags;
+    BoundaryGrid boundary;  // offsets null: no boundary samples
 };
 
 // density_forward_kernel (density.cu:18-40, poly6) and lambda_forward_kernel
 // (pbf.cu:95-127); the walk also writes the exact masks of the positions.
+// With boundary samples the walk is the full stencil, each cell's samples b
+// after its particles adding rho0_i V_b W to the density and V_b grad W to
+// the gradient sum (pbf.cu:118-121); no exact masks are written (the host
+// marks those sets unusable).
 __global__ void __launch_bounds__(kBlock) pbf_lambda_kernel(LambdaArgs a) {
     PHX_MASK_LDS(a.grid);
     const uint32_t i = xcd_block() * kBlock + threadIdx.x;
@@ -88,20 +93,32 @@ __global__ void __launch_bounds__(kBlock) pbf_lambda_kernel(LambdaArgs a) {
     float density = 0.0F;
     Float3 gradient_sum{0.0F, 0.0F, 0.0F};
     float denominator = a.eps[i];
-    const bool complete = visit_and_mask(
-        a.grid, a.mv, masked, i, cx, cy, cz, delta, smask + threadIdx.x, kBlock, a.exact_out, a.n, [&](uint32_t s) {
-            const Float3 d = sub(p, load(a.pos, s));
-            const float ms = a.m[s];
-            density += ms * poly6(d, a.h);
-            if (s != i) {
-                const Float3 g = spiky_gradient(d, a.h, a.spiky);
-                const float coefficient = ms / rest;
-                gradient_sum = add(gradient_sum, scale(g, coefficient));
-                denominator += coefficient * coefficient * dot(g, g);
-            }
-            return dot(d, d) < a.t_exact;
-        });
-    if (!complete) atomicOr(&a.exact_flags[kFlagOverflow], 1u);
+    auto visit = [&](uint32_t s) {
+        const Float3 d = sub(p, load(a.pos, s));
+        const float ms = a.m[s];
+        density += ms * poly6(d, a.h);
+        if (s != i) {
+            const Float3 g = spiky_gradient(d, a.h, a.spiky);
+            const float coefficient = ms / rest;
+            gradient_sum = add(gradient_sum, scale(g, coefficient));
+            denominator += coefficient * coefficient * dot(g, g);
+        }
+        return dot(d, d) < a.t_exact;
+    };
+    if (a.boundary.offsets != nullptr) {
+        for_each_candidate(
+            a.grid, a.boundary, cx, cy, cz, [&](uint32_t s) { (void)visit(s); },
+            [&](uint32_t s) {
+                const float4 b = a.boundary.x[s];
+                const Float3 d = sub(p, Float3{b.x, b.y, b.z});
+                density += rest * b.w * poly6(d, a.h);
+                gradient_sum = add(gradient_sum, scale(spiky_gradient(d, a.h, a.spiky), b.w));
+            });
+    } else {
+        const bool complete = visit_and_mask(a.grid, a.mv, masked, i, cx, cy, cz, delta, smask + threadIdx.x, kBlock,
+                                             a.exact_out, a.n, visit);
+        if (!complete) atomicOr(&a.exact_flags[kFlagOverflow], 1u);
+    }
     denominator += dot(gradient_sum, gradient_sum);
     a.rho[i] = density;
     a.lambda[i] = -(density / rest - 1.0F) / denominator;
@@ -134,11 +151,14 @@ struct CorrectionArgs {
     MaskView mv;
     uint32_t slot;           // displacement slot of the positions read
     uint32_t* moved_to;      // largest squared displacement of the output from x*
+    BoundaryGrid boundary;   // offsets null: no boundary samples
 };
 
 // correction_forward_kernel (pbf.cu:254-288) and project_forward_kernel
 // (pbf.cu:417-431), plus the largest displacement of the projected positions
-// from x* for the mask checks of later passes.
+// from x* for the mask checks of later passes. A boundary sample b adds
+// V_b (lambda_i + s_corr(d)) grad W after each cell's particles
+// (pbf.cu:278-284).
 __global__ void __launch_bounds__(kBlock) pbf_correction_kernel(CorrectionArgs a) {
     PHX_MASK_LDS(a.grid);
     const uint32_t i = xcd_block() * kBlock + threadIdx.x;
@@ -156,15 +176,28 @@ __global__ void __launch_bounds__(kBlock) pbf_correction_kernel(CorrectionArgs a
         int cx, cy, cz;
         particle_cell(a.grid, load(a.topo, i), cx, cy, cz);
         Float3 correction{0.0F, 0.0F, 0.0F};
-        visit_candidates(a.grid, a.mv, masked, i, cx, cy, cz, delta, smask + threadIdx.x, kBlock, [&](uint32_t s) {
+        auto artificial_at = [&](Float3 d) {
+            return vanishes ? (poly6(d, a.h) == 0.0F ? 0.0F : -strength)
+                            : artificial_value(d, a.h, strength, exponent, reference);
+        };
+        auto visit = [&](uint32_t s) {
             if (s == i) return;
             const Float3 d = sub(p, load(a.pos, s));
-            const float artificial = vanishes ? (poly6(d, a.h) == 0.0F ? 0.0F : -strength)
-                                              : artificial_value(d, a.h, strength, exponent, reference);
+            const float artificial = artificial_at(d);
             const float coefficient = a.m[s] / rest;
             correction = add(correction, scale(spiky_gradient(d, a.h, a.spiky),
                                                coefficient * (lambda_i + a.lambda[s] + artificial)));
-        });
+        };
+        if (a.boundary.offsets != nullptr) {
+            for_each_candidate(a.grid, a.boundary, cx, cy, cz, visit, [&](uint32_t s) {
+                const float4 b = a.boundary.x[s];
+                const Float3 d = sub(p, Float3{b.x, b.y, b.z});
+                const float artificial = artificial_at(d);
+                correction = add(correction, scale(spiky_gradient(d, a.h, a.spiky), b.w * (lambda_i + artificial)));
+            });
+        } else {
+            visit_candidates(a.grid, a.mv, masked, i, cx, cy, cz, delta, smask + threadIdx.x, kBlock, visit);
+        }
         const Float3 candidate = add(p, correction);
         if (a.mask != nullptr) {
             uint32_t bits = 0u;
@@ -342,12 +375,15 @@ PbfSolver::PbfSolver(const phx_domain_configuration* domain, const phx_pbf_confi
                      phx_execution_mode mode, hipStream_t stream)
     : domain_(make_domain(domain)), mode_(mode), stream_(stream) {
     require(configuration != nullptr, PHX_ERROR_INVALID_ARGUMENT, "null PBF configuration");
-    require(domain_.config.boundary_particle_count == 0u, PHX_ERROR_UNSUPPORTED,
-            "boundary particles are not supported by this solver yet");
+
     config_ = *configuration;
     require(config_.checkpoint_interval > 0u, PHX_ERROR_INVALID_ARGUMENT, "checkpoint_interval must be positive");
     require(mode == PHX_MODE_FORWARD || mode == PHX_MODE_DIFFERENTIABLE, PHX_ERROR_INVALID_ARGUMENT,
             "unknown execution mode");
+    require(domain_.config.boundary_particle_count == 0u || mode == PHX_MODE_FORWARD, PHX_ERROR_UNSUPPORTED,
+            "boundary particles are supported in forward mode only");
+    boundary_.upload(domain_.config);
+    domain_.config.boundary_particles = nullptr;  // the caller's host array is not kept
     spiky_ = spiky_coefficient(domain_.config.support_radius);
     const uint32_t n = domain_.config.particle_count;
     for (VecBuffer* b : {&xs_, &x0_, &pa_, &pb_, &vel_, &vort_, &vc_}) b->resize(n);
@@ -411,11 +447,13 @@ void PbfSolver::forward_step(const phx_pbf_state& state, const phx_control& cont
     {
         PassScope scope(profiler_, stream_, kPbfGrid);
         cache.neighborhood.build(stream_, domain_, step + 1u, cvec(state.particles.positions), &predict,
-                                 gather.get());
+                                 gather.get(), &boundary_);
         masks_.begin_step(stream_);
     }
 
     const SortedGrid grid = cache.neighborhood.grid();
+    const BoundaryGrid boundary = cache.neighborhood.boundary();
+    const bool with_boundary = boundary.offsets != nullptr;
     // whole blocks of kBlock particles, padded to a multiple of the XCD count
     const uint32_t blocks = (blocks_for(n, kBlock) + 7u) / 8u * 8u;
     const float h = c.support_radius;
@@ -437,6 +475,10 @@ void PbfSolver::forward_step(const phx_pbf_state& state, const phx_control& cont
     const uint32_t sets = config_.pressure_iterations + 1u;
     uint32_t* exact_flags = rec ? cache.exact_flags.data() : exact_flags_.data();
     fill_u32(stream_, exact_flags, sets, 0u);
+    // with boundary samples the iterations' passes walk the full stencil and
+    // write no exact masks; the final positions' set (vorticity, normal,
+    // XSPH: no boundary terms) is built as usual
+    if (with_boundary && sets > 1u) fill_u32(stream_, exact_flags, sets - 1u, 1u);
     auto exact_set = [&](uint32_t k) {
         require(k < sets, PHX_ERROR_INVALID_ARGUMENT, "exact mask set out of range");
         return rec ? cache.exact_masks.data() + static_cast<size_t>(k) * kStencil * n : exact_.data();
@@ -446,6 +488,7 @@ void PbfSolver::forward_step(const phx_pbf_state& state, const phx_control& cont
     LambdaArgs la{n,           h,        spiky_,     grid, xs.cview(), xs.cview(), m_.data(), rho0_.data(),
                   eps_.data(), cache.densities.data(), cache.lambdas.data(), nullptr, Vec{}, mv, 0u};
     la.t_exact = t_exact;
+    la.boundary = boundary;
     CorrectionArgs ca{};
     ca.n = n;
     ca.h = h;
@@ -463,6 +506,7 @@ void PbfSolver::forward_step(const phx_pbf_state& state, const phx_control& cont
     ca.inv_dt = inv_dt;
     ca.vel = vel.view();
     ca.mv = mv;
+    ca.boundary = boundary;
 
     // positions start at x* (pbf.cpp:171) and ping-pong between two buffers
     // (a recorded step keeps every iterate)

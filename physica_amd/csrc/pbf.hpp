@@ -125,6 +125,7 @@ class PbfSolver {
     float spiky_ = 0.0F;
     PassProfiler profiler_;
     MaskSet masks_;
+    BoundarySamples boundary_;  // the domain's boundary samples on the device
     // sorted-order workspace
     VecBuffer xs_, x0_, pa_, pb_, vel_, vort_, vc_;
     DeviceBuffer<float> m_, rho0_, eps_, strength_, exponent_, radius_, xsph_, confinement_, magnitude_;

@@ -8,7 +8,7 @@ from pathlib import Path
 import pytest
 
 from physica_amd import _capi
-from physica_amd.particle import BoxBoundary, DFSPH, PBF, DomainConfiguration
+from physica_amd.particle import PCISPH, BoxBoundary, DFSPH, PBF, DomainConfiguration
 
 HEADER = Path(__file__).resolve().parent.parent / "include" / "physica_amd" / "physica.h"
 
@@ -85,9 +85,13 @@ def test_boundary_particles_rejected_where_unsupported():
     got, handle = create_dfsph(cfg(boundary_particles=sample), mode=1)
     assert got == _capi.PHX_ERROR_UNSUPPORTED and handle.value is None
     assert "forward mode only" in _capi.library().phx_last_error().decode()
+    # derivatives with boundary samples: refused by every solver before device work
     h = ctypes.c_void_p()
     got = _capi.library().phx_pbf_create(ctypes.byref(cfg(boundary_particles=sample).to_c()),
-                                         ctypes.byref(PBF.Configuration().to_c()), 0, None, ctypes.byref(h))
+                                         ctypes.byref(PBF.Configuration().to_c()), 1, None, ctypes.byref(h))
+    assert got == _capi.PHX_ERROR_UNSUPPORTED and h.value is None
+    got = _capi.library().phx_sph_create(ctypes.byref(cfg(boundary_particles=sample).to_c()),
+                                         ctypes.byref(PCISPH.Configuration().to_c()), 1, None, ctypes.byref(h))
     assert got == _capi.PHX_ERROR_UNSUPPORTED and h.value is None
     # a non-zero count with a null host array
     c = cfg().to_c()

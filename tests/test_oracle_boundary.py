@@ -130,3 +130,102 @@ def test_viscosity_term_equals_matched_fluid_neighbours():
     gravity = np.abs(without).max()
     assert np.abs(with_b - as_fluid).max() <= 1e-12 * gravity
     assert np.abs(with_b - without).max() > 1e-4 * gravity
+
+
+def test_pbf_lambda_boundary_terms():
+    """PBF's gradient sum gains V_b grad W per sample, exactly as a fluid
+    neighbour of mass rho0 V_b; the denominator gains only |sum grad W|^2
+    (the samples add nothing to the sum of c^2 |grad W|^2); the poly6
+    density follows the density equivalence."""
+    sc, n, samples, xb, step = setup(True)
+    x0, v, vol = samples
+    b = vol.shape[0]
+    rng = np.random.default_rng(13)
+    rho0 = 1000.0
+    m = sc.parameters["masses"].astype(np.float64)
+    x = sc.positions.astype(np.float64)
+    rho = rng.uniform(990.0, 1030.0, n)
+    eps = np.full(n, 1e-4)
+    d = oracle.make_domain(sc.domain, samples)
+    gs, den, lam = oracle.pbf_lambda(d, step, x, x, m, np.full(n, rho0), rho, eps, "f64")
+    gs0, den0, _ = oracle.pbf_lambda(oracle.make_domain(sc.domain), step, x, x, m, np.full(n, rho0), rho, eps, "f64")
+    assert np.abs(gs - gs0).max() > 1e-3 * np.abs(gs0).max()
+    sq = lambda g: np.sum(g * g, axis=0)
+    assert np.allclose(den - den0, sq(gs) - sq(gs0), rtol=1e-10, atol=1e-12 * np.abs(den0).max())
+    assert np.allclose(lam, -(rho / rho0 - 1.0) / den, rtol=1e-14, atol=0)
+    sc.domain.particle_count = n + b
+    da = oracle.make_domain(sc.domain)
+    xa = merged(sc, x, xb)
+    gsa, _, _ = oracle.pbf_lambda(da, step, xa, xa, merged(sc, m, rho0 * vol.astype(np.float64)),
+                                  np.full(n + b, rho0), merged(sc, rho, np.full(b, rho0)), np.full(n + b, 1e-4), "f64")
+    sc.domain.particle_count = n
+    assert np.abs(gs - gsa[:, :n]).max() <= 1e-12 * np.abs(gs).max()
+    with_b = oracle.density(d, step, True, x, x, m, np.full(n, rho0), "f64")
+    sc.domain.particle_count = n + b
+    as_fluid = oracle.density(da, step, True, xa, xa, merged(sc, m, rho0 * vol.astype(np.float64)),
+                              np.full(n + b, rho0), "f64")[:n]
+    sc.domain.particle_count = n
+    assert np.abs(with_b - as_fluid).max() <= 1e-13 * np.abs(as_fluid).max()
+
+
+def textbook_pbf_single(x, v, g, dt, h, m, rho0, eps, xb, vol, iterations):
+    """One PBF step of a lone particle among boundary samples, from Macklin &
+    Mueller (2013) with Akinci volumes: x* = x + dt (v + dt g); per iteration
+    rho = m W(0) + rho0 sum V_b W(x - x_b), lambda = -(rho / rho0 - 1) /
+    (eps + |sum V_b grad W|^2), dx = sum V_b lambda grad W (no artificial
+    pressure); poly6 W = 315 / (64 pi h^9) (h^2 - r^2)^3, spiky grad W =
+    -45 / (pi h^6) (h - r)^2 r / |r|. Returns the next position (no wall
+    is reached)."""
+    def poly6(r2):
+        return np.where(r2 < h * h, 315.0 / (64.0 * np.pi * h ** 9) * (h * h - r2) ** 3, 0.0)
+
+    def spiky_grad(dvec):
+        r = np.sqrt(np.sum(dvec * dvec, axis=0))
+        f = np.where((r > 0) & (r < h), -45.0 / (np.pi * h ** 6) * (h - r) ** 2 / np.where(r > 0, r, 1.0), 0.0)
+        return dvec * f
+
+    p = x + dt * (v + dt * g)
+    for _ in range(iterations):
+        dvec = p[:, None] - xb
+        rho = m * poly6(0.0) + rho0 * np.sum(vol * poly6(np.sum(dvec * dvec, axis=0)))
+        grads = spiky_grad(dvec)
+        sg = np.sum(vol * grads, axis=1)
+        lam = -(rho / rho0 - 1.0) / (eps + sg @ sg)
+        p = p + np.sum(vol * lam * grads, axis=1)
+    return p
+
+
+def test_pbf_correction_boundary_term_lone_particle():
+    """A lone particle pressed into a floor of samples: the oracle's FP64 PBF
+    step (grid, density, lambda, correction with their boundary terms) equals
+    the closed form of the textbook step."""
+    from physica_amd.particle import BoxBoundary, DomainConfiguration
+    h = 0.04
+    s = 0.02
+    dom = DomainConfiguration(1, 1.0 / 120.0, h, 0.5 * s, (0.0, -9.81, 0.0),
+                              BoxBoundary((0, 0, 0), (1.0, 1.0, 1.0), (0, 0, 0), True))
+    g = np.arange(-3, 4) * s
+    gx, gz = np.meshgrid(g + 0.5, g + 0.5, indexing="ij")
+    xb = np.stack([gx.ravel(), np.full(gx.size, 0.3), gz.ravel()])
+    vol = np.full(xb.shape[1], np.float32(0.8 * s ** 3))
+    samples = (xb.astype(np.float32), np.zeros_like(xb, dtype=np.float32), vol)
+    x = np.array([0.503, 0.3 + 0.6 * s, 0.497])
+    v = np.array([0.1, -0.5, 0.05])
+    m, rho0, eps = 1000.0 * s ** 3, 1000.0, 1e-3
+    params = {"masses": np.array([m]), "rest_densities": np.array([rho0]), "relaxation": np.array([eps]),
+              "artificial_pressure_strength": np.zeros(1), "artificial_pressure_exponent": np.full(1, 4.0),
+              "artificial_pressure_radius": np.full(1, 0.2 * h), "xsph_viscosity": np.zeros(1),
+              "vorticity_confinement": np.zeros(1)}
+    d = oracle.make_domain(dom, samples)
+    # the domain holds FP32 values, which the FP64 step widens
+    f32 = lambda value: np.float64(np.float32(value))
+    dt, h32 = f32(1.0 / 120.0), f32(h)
+    gravity = np.array([0.0, f32(-9.81), 0.0])
+    for iterations in (1, 3):
+        out = oracle.pbf_step(d, 0, x[:, None], v[:, None], np.zeros((3, 1)), params, iterations, "f64")
+        want = textbook_pbf_single(x, v, gravity, dt, h32, m, rho0, eps, xb.astype(np.float32).astype(np.float64),
+                                   vol.astype(np.float64), iterations)
+        got = out["positions"][:, 0]
+        free = x + dt * (v + dt * gravity)
+        assert np.abs(want - free).max() > 1e-4  # the samples push the particle
+        assert np.abs(got - want).max() <= 1e-13, (iterations, got - want)
