@@ -159,6 +159,11 @@ typedef struct phx_particle_state_adjoint {
     phx_adjoint_vector_view velocities;
 } phx_particle_state_adjoint;
 
+/* Domain::accumulate (domain/domain.cpp:189-200, cuda_detail::accumulate,
+ * domain/domain.cu:8-29): destination[k] += source[k] for k < count, FP64
+ * device arrays, enqueued on `stream`. */
+phx_status phx_accumulate(phx_stream stream, const double* source, double* destination, size_t count);
+
 /* Grid resolution ceil((max - min) / h) per axis (neighborhood.cpp:25-30). */
 phx_status phx_grid_resolution(const phx_domain_configuration* domain, uint32_t resolution[3]);
 

@@ -309,6 +309,7 @@ PROTOTYPES = {
     "phx_last_error": (c_char_p, []),
     "phx_version": (c_char_p, []),
     "phx_grid_resolution": (c_int, [POINTER(DomainConfiguration), POINTER(c_uint32)]),
+    "phx_accumulate": (c_int, [c_void_p, c_void_p, c_void_p, c_size_t]),
     "phx_neighborhood_workspace_size": (c_int, [POINTER(DomainConfiguration), POINTER(c_size_t)]),
     "phx_build_neighborhood": (
         c_int,

@@ -121,6 +121,12 @@ __global__ void __launch_bounds__(kBlock)
     out[i] = result;
 }
 
+// accumulate_scalar_kernel (domain/domain.cu:16-20): one element per lane.
+__global__ void __launch_bounds__(kBlock) accumulate_kernel(const double* source, double* destination, size_t count) {
+    const size_t k = static_cast<size_t>(blockIdx.x) * kBlock + threadIdx.x;
+    if (k < count) destination[k] += source[k];
+}
+
 // Workspace of phx_build_neighborhood: the fluid grid scratch, then for B
 // boundary samples their scratch, the packed samples (7 B floats) and the
 // boundary permutation.
@@ -141,6 +147,17 @@ extern "C" {
 const char* phx_last_error(void) { return g_last_error.c_str(); }
 
 const char* phx_version(void) { return "physica_amd 0.1.0 (gfx950)"; }
+
+phx_status phx_accumulate(phx_stream stream, const double* source, double* destination, size_t count) {
+    return guarded([&] {
+        if (count == 0) return;
+        phx::require(source != nullptr && destination != nullptr, PHX_ERROR_INVALID_ARGUMENT, "null array");
+        const size_t blocks = (count + kBlock - 1) / kBlock;
+        phx::require(blocks < (1ull << 31), PHX_ERROR_CAPACITY, "count too large");
+        accumulate_kernel<<<static_cast<uint32_t>(blocks), kBlock, 0, stream_of(stream)>>>(source, destination, count);
+        PHX_LAUNCH_CHECK();
+    });
+}
 
 phx_status phx_grid_resolution(const phx_domain_configuration* domain, uint32_t resolution[3]) {
     return guarded([&] {
