@@ -342,6 +342,10 @@ phx_status phx_dfsph_reference_gradient_norm(const phx_dfsph* solver, float* val
 /* Iterations executed by the last forward_step (divergence, density). */
 phx_status phx_dfsph_last_iterations(const phx_dfsph* solver, uint32_t iterations[2]);
 
+/* Diagnostics (no reference counterpart): the skin of the neighbour masks
+ * the next step will use, in units of the support radius (adapted from the
+ * predicted displacements of a completed step). */
+phx_status phx_dfsph_mask_skin(const phx_dfsph* solver, float* skin);
 phx_status phx_dfsph_set_profiling(phx_dfsph* solver, int enabled);
 /* Fills up to capacity entries (passes never launched are omitted), sets
  * *count, and resets the totals. Synchronises with the recorded events. */

@@ -335,6 +335,7 @@ PROTOTYPES = {
     ),
     "phx_dfsph_reference_gradient_norm": (c_int, [c_void_p, POINTER(c_float)]),
     "phx_dfsph_last_iterations": (c_int, [c_void_p, POINTER(c_uint32)]),
+    "phx_dfsph_mask_skin": (c_int, [c_void_p, POINTER(c_float)]),
     "phx_dfsph_set_profiling": (c_int, [c_void_p, c_int]),
     "phx_dfsph_read_profile": (c_int, [c_void_p, POINTER(PassProfile), c_uint32, POINTER(c_uint32)]),
     "phx_dfsph_cache_neighborhood": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),

@@ -420,6 +420,13 @@ phx_status phx_dfsph_vjp_step(phx_dfsph* solver, const phx_dfsph_step_cache* cac
     });
 }
 
+phx_status phx_dfsph_mask_skin(const phx_dfsph* solver, float* skin) {
+    return guarded([&] {
+        phx::require(solver != nullptr && skin != nullptr, PHX_ERROR_INVALID_ARGUMENT, "null argument");
+        *skin = solver->impl.skin();
+    });
+}
+
 phx_status phx_dfsph_set_profiling(phx_dfsph* solver, int enabled) {
     return guarded([&] {
         phx::require(solver != nullptr, PHX_ERROR_INVALID_ARGUMENT, "null solver");

@@ -129,7 +129,7 @@ DfsphSolver::DfsphSolver(const phx_domain_configuration* domain, const phx_dfsph
                                                   : config_.divergence_iterations + config_.density_iterations + 1u;
     masks_.allocate(n, predictions, domain_.config.support_radius);
     if (tolerance > 0.0F) {
-        tol_partials_.resize(pass_blocks(n));  // one per projection workgroup
+        tol_partials_.resize(static_cast<size_t>(pass_blocks(n)) * kWaves);  // one per projection wave
         tol_done_.resize(2u);
         PHX_HIP(hipHostMalloc(reinterpret_cast<void**>(&tol_host_), 4u * sizeof(uint32_t), hipHostMallocDefault));
         for (hipEvent_t& e : tol_events_) PHX_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
@@ -296,7 +296,7 @@ void DfsphSolver::forward_step(const phx_sph_dfsph_state& state, const phx_contr
     if (config_.density_error_tolerance > 0.0F) {
         // Tolerance mode: each phase runs until its projection's mean
         // compression error is within the tolerance (or max_iterations).
-        // The projection leaves one FP64 error sum per workgroup; a
+        // The projection leaves one FP64 error sum per wave; a
         // one-workgroup pass adds them in a fixed order and records the
         // ending iteration on the device; the force pass of that iteration
         // then ends the phase by itself (kDivAuto, or the final state). The
@@ -333,8 +333,8 @@ void DfsphSolver::forward_step(const phx_sph_dfsph_state& state, const phx_contr
                 pa.cv = Convergence{tol_partials_.data(), done, rho0};
                 launch_projection(profiler_, stream_, blocks, pa,
                                   phase == 0u ? kDfsphProjectionDivergence : kDfsphProjectionDensity);
-                converge_kernel<<<1, kConvergeBlock, 0, stream_>>>(tol_partials_.data(), blocks, n, tolerance, it,
-                                                                   max_it, done);
+                converge_kernel<<<1, kConvergeBlock, 0, stream_>>>(tol_partials_.data(), blocks * kWaves, n,
+                                                                   tolerance, it, max_it, done);
                 PHX_HIP(hipGetLastError());
                 uint32_t* host = tol_host_ + 2u * phase + (it & 1u);
                 PHX_HIP(hipMemcpyAsync(host, done, sizeof(uint32_t), hipMemcpyDeviceToHost, stream_));

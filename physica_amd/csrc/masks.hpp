@@ -50,7 +50,7 @@ class MaskSet {
     // Masks of n particles and flags for `predicting_passes` displacement
     // slots (one per pass that writes predicted positions).
     void allocate(uint32_t n, uint32_t predicting_passes, float support_radius);
-    // Adopts the skin of a completed read-back, if any, and clears the flags;
+    // Adopts the skin of the newest completed read-back, if any, and clears the flags;
     // with full_stencil the step's masks are marked unusable (overflow), so
     // every pass walks the whole 27-cell stencil.
     void begin_step(hipStream_t stream, bool full_stencil = false);
@@ -74,9 +74,14 @@ class MaskSet {
     uint32_t flag_count_ = 0;
     DeviceBuffer<uint16_t> exact_, skin_mask_;
     DeviceBuffer<uint32_t> flags_;
-    PinnedBuffer flags_host_;
-    hipEvent_t readback_ = nullptr;
-    bool readback_pending_ = false;
+    // A ring of read-backs: a step's copy may still be queued behind its
+    // last pass when the next step starts (always so when the host waits
+    // inside a step), so begin_step adopts the newest one that completed.
+    static constexpr uint32_t kReadbacks = 4;
+    PinnedBuffer flags_host_;  // kReadbacks copies of the flags
+    hipEvent_t readback_[kReadbacks] = {};
+    bool readback_pending_[kReadbacks] = {};
+    uint32_t readback_next_ = 0;  // slot of the next end_step
 };
 
 }  // namespace phx

@@ -19,6 +19,7 @@ namespace {
 __device__ __forceinline__ Float3 xyz(float4 q) { return Float3{q.x, q.y, q.z}; }
 
 constexpr uint32_t kBlock = 256;
+constexpr uint32_t kWaves = kBlock / 64;  // waves of a pass workgroup
 
 struct MaskDensityArgs {
     uint32_t n;
@@ -107,15 +108,17 @@ __global__ void __launch_bounds__(kBlock) mask_density_kernel(MaskDensityArgs a)
 }
 
 // Tolerance mode: the convergence state of a pressure phase. The projection
-// adds its workgroup's compression errors max(0, rho* - target) / rho0 in
-// FP64 and stores the sum in partials[blockIdx.x]; converge_kernel then
+// adds each wave's compression errors max(0, rho* - target) / rho0 in FP64
+// (shuffles, no block barrier, so every wave retires as soon as its own walk
+// ends) and stores the sum in partials[blockIdx.x * kWaves + wave];
+// converge_kernel then
 // marks the phase done at this iteration when the mean is within the
 // tolerance or the iteration is the last allowed one. Every pass of the
 // phase reads `done`: passes of later iterations (launched ahead by the
 // host) then return at once, and the force pass of the marked iteration ends
 // the phase.
 struct Convergence {
-    double* partials;      // null: fixed iteration counts; else one error sum per workgroup
+    double* partials;      // null: fixed iteration counts; else one error sum per wave
     const uint32_t* done;  // iteration that ended the phase; 0 while it runs
     const float* rest;  // rho0, the scale of the error
 };
@@ -139,8 +142,8 @@ __device__ __forceinline__ double block_sum(double value) {
 
 constexpr uint32_t kConvergeBlock = 1024;
 
-// Ends a tolerance-mode phase: sums the projection's per-workgroup error
-// sums in a fixed order (thread t takes partials t, t + 1024, ...), and
+// Ends a tolerance-mode phase: sums the projection's per-wave error sums in
+// a fixed order (thread t takes partials t, t + 1024, ...), and
 // records the iteration in *done when the mean error is within the
 // tolerance or the iteration is the last allowed. One workgroup; the kernel
 // boundaries order it after the projection and before the force pass, so
@@ -240,8 +243,10 @@ __global__ void __launch_bounds__(kBlock) projection_kernel(ProjectionArgs a) {
                     static_cast<double>(a.cv.rest[i]);
     }
     if (tolerance) {
-        const double block_error = block_sum<kBlock>(error);
-        if (threadIdx.x == 0u) a.cv.partials[blockIdx.x] = block_error;
+        // every lane of the wave is here (no early return above)
+        double wave_error = error;
+        for (int offset = 32; offset > 0; offset >>= 1) wave_error += __shfl_xor(wave_error, offset, 64);
+        if ((threadIdx.x & 63u) == 0u) a.cv.partials[blockIdx.x * kWaves + (threadIdx.x >> 6)] = wave_error;
     }
 }
 

@@ -864,6 +864,15 @@ class SPH:
         _capi.call("phx_dfsph_last_iterations", self._handle, v)
         return int(v[0]), int(v[1])
 
+    def mask_skin(self) -> float:
+        """DFSPH diagnostics: the neighbour-mask skin of the next step, in
+        units of the support radius."""
+        if self.method is not DFSPH:
+            raise RuntimeError("mask_skin is a DFSPH diagnostic")
+        v = ctypes.c_float()
+        _capi.call("phx_dfsph_mask_skin", self._handle, ctypes.byref(v))
+        return float(v.value)
+
 
 # ---------------------------------------------------------------------- PBF
 class PBF:
