@@ -607,6 +607,17 @@ phx_status phx_pbf_cache_predicted_positions(const phx_pbf_step_cache* cache, ph
 /* Densities and lambdas of the last constraint iteration (IterationPrimal,
  * pbf/pbf.ixx:98-105). */
 phx_status phx_pbf_cache_last_iteration(const phx_pbf_step_cache* cache, float* densities, float* lambdas);
+/* StepCache::vorticity_magnitudes (pbf/pbf.ixx:67): |omega| of the step's
+ * final positions, the field the reference's viewer provider publishes
+ * (examples/fluids/liquid/pbf-dam-break/spectra/provider.ixx:90-91). */
+phx_status phx_pbf_cache_vorticity_magnitudes(const phx_pbf_step_cache* cache, float* magnitudes);
+
+/* ---------------------------------------------------------------- viewer */
+/* spectra_cuda::write_vectors (examples/fluids/liquid/pbf-dam-break/spectra/
+ * kernels.cu:8-18): SoA positions and velocities to interleaved x, y, z
+ * (Float3) device arrays of 3 * count floats; either output may be null. */
+phx_status phx_write_vectors(phx_stream stream, uint32_t count, phx_const_vector_view positions,
+                             phx_const_vector_view velocities, float* positions_xyz, float* velocities_xyz);
 
 #ifdef __cplusplus
 }

@@ -341,11 +341,13 @@ def pbf_step(domain, step_index, positions, velocities, controls, params: Dict[s
     rho = np.zeros(n, dtype=_dtype(precision))
     lam = np.zeros(n, dtype=_dtype(precision))
     pred, predp = _out_vec(n, precision)
+    mag = np.zeros(n, dtype=_dtype(precision))
     getattr(library(), "oracle_pbf_step_" + precision)(
         ctypes.byref(domain), c_uint32(pressure_iterations), c_uint64(step_index), pp, vp, cp,
         *(c_void_p(arrs[k].ctypes.data) for k in names), nxp, nvp, c_void_p(rho.ctypes.data),
-        c_void_p(lam.ctypes.data), predp)
-    return {"positions": nx, "velocities": nv, "densities": rho, "lambdas": lam, "predicted_positions": pred}
+        c_void_p(lam.ctypes.data), predp, c_void_p(mag.ctypes.data))
+    return {"positions": nx, "velocities": nv, "densities": rho, "lambdas": lam, "predicted_positions": pred,
+            "vorticity_magnitudes": mag}
 
 
 PBF_PARAMETERS = ("masses", "rest_densities", "relaxation", "artificial_pressure_strength",

@@ -108,7 +108,8 @@ void oracle_set_threads(int threads);
                               const real* masses, const real* rest_densities, const real* relaxation,                  \
                               const real* strength, const real* exponent, const real* radius, const real* xsph,         \
                               const real* confinement, real* next_positions[3], real* next_velocities[3],               \
-                              real* densities_out, real* lambdas_out, real* predicted_out[3]);                           \
+                              real* densities_out, real* lambdas_out, real* predicted_out[3],                           \
+                              real* vorticity_magnitudes_out);                                                          \
     /* PBF::jvp_step (pbf.cpp:193-215): tangents in the primal precision */                                           \
     void oracle_pbf_jvp##suf(const oracle_domain* domain, uint32_t pressure_iterations, uint64_t step_index,          \
                              const real* positions[3], const real* velocities[3], const real* controls[3],             \

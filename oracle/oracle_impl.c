@@ -1049,7 +1049,8 @@ void SUF(oracle_pbf_step)(const oracle_domain* d, uint32_t iterations, uint64_t 
                           const real* vel[3], const real* ctl[3], const real* masses, const real* rest_densities,
                           const real* relaxation, const real* strength, const real* exponent, const real* radius,
                           const real* xsph, const real* confinement, real* next_pos[3], real* next_vel[3],
-                          real* densities_out, real* lambdas_out, real* predicted_out[3]) {
+                          real* densities_out, real* lambdas_out, real* predicted_out[3],
+                          real* vorticity_magnitudes_out) {
     const uint32_t n = d->particle_count;
     const real dt = (real)d->time_step;
     const v3 gravity = v3_make((real)d->gravity[0], (real)d->gravity[1], (real)d->gravity[2]);
@@ -1110,6 +1111,7 @@ void SUF(oracle_pbf_step)(const oracle_domain* d, uint32_t iterations, uint64_t 
     if (predicted_out)
         for (int c = 0; c < 3; ++c)
             if (predicted_out[c]) memcpy(predicted_out[c], xs.c[c], sizeof(real) * n);
+    if (vorticity_magnitudes_out) memcpy(vorticity_magnitudes_out, mag, sizeof(real) * n);
     free_vfield(&xs);
     free_vfield(&pc);
     free_vfield(&corr);

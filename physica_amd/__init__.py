@@ -22,6 +22,7 @@ from .particle import (  # noqa: F401
     build_neighborhood,
     density_forward,
     neighborhood_workspace_size,
+    write_vectors,
 )
 from ._capi import LIBRARY_PATH, PhxError, library  # noqa: F401
 

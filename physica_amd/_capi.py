@@ -310,6 +310,8 @@ PROTOTYPES = {
     "phx_version": (c_char_p, []),
     "phx_grid_resolution": (c_int, [POINTER(DomainConfiguration), POINTER(c_uint32)]),
     "phx_accumulate": (c_int, [c_void_p, c_void_p, c_void_p, c_size_t]),
+    "phx_write_vectors": (c_int, [c_void_p, c_uint32, VectorView, VectorView, c_void_p, c_void_p]),
+    "phx_pbf_cache_vorticity_magnitudes": (c_int, [c_void_p, c_void_p]),
     "phx_neighborhood_workspace_size": (c_int, [POINTER(DomainConfiguration), POINTER(c_size_t)]),
     "phx_build_neighborhood": (
         c_int,

@@ -127,6 +127,24 @@ __global__ void __launch_bounds__(kBlock) accumulate_kernel(const double* source
     if (k < count) destination[k] += source[k];
 }
 
+// write_vectors_kernel (spectra/kernels.cu:8-14) for a block of kBlock
+// particles: the three SoA components are read coalesced into LDS and the
+// interleaved x, y, z written back as consecutive floats, so every global
+// access is a coalesced dword.
+__global__ void __launch_bounds__(kBlock) interleave_kernel(uint32_t n, phx_const_vector_view in, float* out) {
+    __shared__ float staged[3u * kBlock];
+    const uint32_t base = blockIdx.x * kBlock;
+    const uint32_t i = base + threadIdx.x;
+    if (i < n) {
+        staged[3u * threadIdx.x + 0u] = in.x[i];
+        staged[3u * threadIdx.x + 1u] = in.y[i];
+        staged[3u * threadIdx.x + 2u] = in.z[i];
+    }
+    __syncthreads();
+    const uint32_t valid = 3u * (n - base < kBlock ? n - base : kBlock);
+    for (uint32_t k = threadIdx.x; k < valid; k += kBlock) out[3u * static_cast<size_t>(base) + k] = staged[k];
+}
+
 // Workspace of phx_build_neighborhood: the fluid grid scratch, then for B
 // boundary samples their scratch, the packed samples (7 B floats) and the
 // boundary permutation.
@@ -156,6 +174,24 @@ phx_status phx_accumulate(phx_stream stream, const double* source, double* desti
         phx::require(blocks < (1ull << 31), PHX_ERROR_CAPACITY, "count too large");
         accumulate_kernel<<<static_cast<uint32_t>(blocks), kBlock, 0, stream_of(stream)>>>(source, destination, count);
         PHX_LAUNCH_CHECK();
+    });
+}
+
+phx_status phx_write_vectors(phx_stream stream, uint32_t count, phx_const_vector_view positions,
+                             phx_const_vector_view velocities, float* positions_xyz, float* velocities_xyz) {
+    return guarded([&] {
+        if (count == 0u) return;
+        const uint32_t blocks = phx::blocks_for(count, kBlock);
+        if (positions_xyz != nullptr) {
+            phx::require_vec(positions, "positions");
+            interleave_kernel<<<blocks, kBlock, 0, stream_of(stream)>>>(count, positions, positions_xyz);
+            PHX_LAUNCH_CHECK();
+        }
+        if (velocities_xyz != nullptr) {
+            phx::require_vec(velocities, "velocities");
+            interleave_kernel<<<blocks, kBlock, 0, stream_of(stream)>>>(count, velocities, velocities_xyz);
+            PHX_LAUNCH_CHECK();
+        }
     });
 }
 
@@ -757,6 +793,17 @@ phx_status phx_pbf_cache_last_iteration(const phx_pbf_step_cache* cache, float* 
         phx::ChannelList ch;
         if (densities != nullptr) ch.add(cache->impl.densities.data(), densities);
         if (lambdas != nullptr) ch.add(cache->impl.lambdas.data(), lambdas);
+        phx::scatter_to_original(cache->owner->impl.stream(), cache->impl.neighborhood.perm.data(),
+                                 cache->impl.neighborhood.particles, ch.get());
+    });
+}
+
+phx_status phx_pbf_cache_vorticity_magnitudes(const phx_pbf_step_cache* cache, float* magnitudes) {
+    return guarded([&] {
+        phx::require(cache != nullptr && cache->valid, PHX_ERROR_INVALID_ARGUMENT, "cache holds no step");
+        phx::require(magnitudes != nullptr, PHX_ERROR_INVALID_ARGUMENT, "null magnitudes");
+        phx::ChannelList ch;
+        ch.add(cache->impl.magnitudes.data(), magnitudes);
         phx::scatter_to_original(cache->owner->impl.stream(), cache->impl.neighborhood.perm.data(),
                                  cache->impl.neighborhood.particles, ch.get());
     });

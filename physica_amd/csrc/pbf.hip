@@ -354,6 +354,7 @@ void PbfCache::allocate(uint32_t n, uint32_t cells, uint32_t pressure_iterations
     predicted.resize(n);
     densities.resize(n);
     lambdas.resize(n);
+    magnitudes.resize(n);  // StepCache::vorticity_magnitudes (pbf/pbf.ixx:67)
     if (!record) return;
     const size_t per = static_cast<size_t>(pressure_iterations) * n;
     for (VecBuffer* b : {&xs, &x0, &velocities, &vorticities, &normals, &confined}) b->resize(n);
@@ -363,7 +364,6 @@ void PbfCache::allocate(uint32_t n, uint32_t cells, uint32_t pressure_iterations
     lambda.resize(per);
     den.resize(per);
     masks.resize(per);
-    magnitudes.resize(n);
     normalizers.resize(n);
     masks_skin.resize(static_cast<size_t>(kStencil) * n);
     mask_flags.resize(kFlagDisplacement + pressure_iterations + 1u);
@@ -387,7 +387,7 @@ PbfSolver::PbfSolver(const phx_domain_configuration* domain, const phx_pbf_confi
     spiky_ = spiky_coefficient(domain_.config.support_radius);
     const uint32_t n = domain_.config.particle_count;
     for (VecBuffer* b : {&xs_, &x0_, &pa_, &pb_, &vel_, &vort_, &vc_}) b->resize(n);
-    for (DeviceBuffer<float>* b : {&m_, &rho0_, &eps_, &strength_, &exponent_, &radius_, &xsph_, &confinement_, &magnitude_})
+    for (DeviceBuffer<float>* b : {&m_, &rho0_, &eps_, &strength_, &exponent_, &radius_, &xsph_, &confinement_})
         b->resize(n);
     exact_.resize(static_cast<size_t>(kStencil) * n);
     exact_flags_.resize(config_.pressure_iterations + 1u);
@@ -424,7 +424,7 @@ void PbfSolver::forward_step(const phx_pbf_state& state, const phx_control& cont
     VecBuffer& vel = rec ? cache.velocities : vel_;
     VecBuffer& vort = rec ? cache.vorticities : vort_;
     VecBuffer& vc = rec ? cache.confined : vc_;
-    float* magnitude = rec ? cache.magnitudes.data() : magnitude_.data();
+    float* magnitude = cache.magnitudes.data();
 
     // predict (pbf.cu:73-78) fused into the key pass of build(step + 1, x*)
     PredictInput predict{};

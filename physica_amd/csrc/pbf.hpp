@@ -49,7 +49,8 @@ struct PbfCache {
     VecStack gradient_sums;                // sum_j (m_j / rho0_i) grad W_ij per iteration
     DeviceBuffer<uint8_t> masks;           // [iterations][n] box-collision bits of the projection
     VecBuffer velocities, vorticities, normals, confined;
-    DeviceBuffer<float> magnitudes, normalizers;
+    DeviceBuffer<float> magnitudes;  // vorticity magnitudes (cell order), every mode
+    DeviceBuffer<float> normalizers;
     // the step's neighbour masks and mask flags, so the derivative passes
     // walk the same candidates as the forward passes
     DeviceBuffer<uint16_t> masks_skin;
@@ -128,7 +129,7 @@ class PbfSolver {
     BoundarySamples boundary_;  // the domain's boundary samples on the device
     // sorted-order workspace
     VecBuffer xs_, x0_, pa_, pb_, vel_, vort_, vc_;
-    DeviceBuffer<float> m_, rho0_, eps_, strength_, exponent_, radius_, xsph_, confinement_, magnitude_;
+    DeviceBuffer<float> m_, rho0_, eps_, strength_, exponent_, radius_, xsph_, confinement_;
     DeviceBuffer<uint16_t> exact_;  // exact masks of the current set of positions (forward mode)
     DeviceBuffer<uint32_t> exact_flags_;
     PbfDerivativeWorkspace dw_;

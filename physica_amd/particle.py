@@ -989,6 +989,12 @@ class PBF:
             _capi.call("phx_pbf_cache_last_iteration", self._handle, _ptr(rho), _ptr(lam))
             return rho, lam
 
+        def vorticity_magnitudes(self) -> torch.Tensor:
+            """StepCache::vorticity_magnitudes (pbf/pbf.ixx:67), original order."""
+            out = self._solver.domain.allocate_scalar_field()
+            _capi.call("phx_pbf_cache_vorticity_magnitudes", self._handle, _ptr(out))
+            return out
+
     def __init__(
         self,
         domain_configuration: DomainConfiguration,
@@ -1125,6 +1131,19 @@ def neighborhood_workspace_size(configuration: DomainConfiguration) -> int:
     size = ctypes.c_size_t()
     _capi.call("phx_neighborhood_workspace_size", ctypes.byref(configuration.to_c()), ctypes.byref(size))
     return int(size.value)
+
+
+def write_vectors(positions: VectorField, velocities: VectorField) -> Tuple[torch.Tensor, torch.Tensor]:
+    """spectra_cuda::write_vectors (examples/fluids/liquid/pbf-dam-break/
+    spectra/kernels.cu:8-18): interleaved (N, 3) float32 copies of the SoA
+    positions and velocities, for a viewer."""
+    n = positions.x.numel()
+    dev = positions.x.device
+    _require_gpu(dev)
+    pos = torch.empty((n, 3), dtype=torch.float32, device=dev)
+    vel = torch.empty((n, 3), dtype=torch.float32, device=dev)
+    _capi.call("phx_write_vectors", _stream_handle(dev), n, positions.view(), velocities.view(), _ptr(pos), _ptr(vel))
+    return pos, vel
 
 
 class DeviceBoundary:
