@@ -564,9 +564,12 @@ struct WCSPH {
 };
 
 struct PCISPH {
+    // density_error_tolerance > 0: tolerance mode (an extension)
     struct Configuration {
         std::uint32_t pressure_iterations{6u};
         std::uint32_t checkpoint_interval{2u};
+        float density_error_tolerance{0.0F};
+        std::uint32_t max_iterations{100u};
     };
     struct State {};
     struct StateTangent {};
@@ -583,9 +586,12 @@ struct PCISPH {
 };
 
 struct IISPH {
+    // density_error_tolerance > 0: tolerance mode (an extension)
     struct Configuration {
         std::uint32_t pressure_iterations{6u};
         std::uint32_t checkpoint_interval{2u};
+        float density_error_tolerance{0.0F};
+        std::uint32_t max_iterations{100u};
     };
     struct State {};
     struct StateTangent {};
@@ -865,7 +871,9 @@ struct SphMethod;
 
 template <>
 struct SphMethod<WCSPH> {
-    static phx_sph_configuration configuration(const WCSPH::Configuration&) { return {PHX_SPH_WCSPH, 0u, 2u}; }
+    static phx_sph_configuration configuration(const WCSPH::Configuration&) {
+        return {PHX_SPH_WCSPH, 0u, 2u, 0.0F, 0u};
+    }
     template <class Buffers>
     static Buffers allocate(hipStream_t stream, std::size_t n) {
         using T = typename std::remove_reference_t<decltype(std::declval<Buffers>().speed_of_sound)>;
@@ -891,7 +899,7 @@ struct SphMethod<WCSPH> {
 template <>
 struct SphMethod<PCISPH> {
     static phx_sph_configuration configuration(const PCISPH::Configuration& c) {
-        return {PHX_SPH_PCISPH, c.pressure_iterations, c.checkpoint_interval};
+        return {PHX_SPH_PCISPH, c.pressure_iterations, c.checkpoint_interval, c.density_error_tolerance, c.max_iterations};
     }
     template <class Buffers>
     static Buffers allocate(hipStream_t stream, std::size_t n) {
@@ -916,7 +924,7 @@ struct SphMethod<PCISPH> {
 template <>
 struct SphMethod<IISPH> {
     static phx_sph_configuration configuration(const IISPH::Configuration& c) {
-        return {PHX_SPH_IISPH, c.pressure_iterations, c.checkpoint_interval};
+        return {PHX_SPH_IISPH, c.pressure_iterations, c.checkpoint_interval, c.density_error_tolerance, c.max_iterations};
     }
     template <class Buffers>
     static Buffers allocate(hipStream_t stream, std::size_t n) {

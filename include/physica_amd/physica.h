@@ -394,11 +394,18 @@ phx_status phx_dfsph_vjp_step(phx_dfsph* solver, const phx_dfsph_step_cache* cac
 typedef enum phx_sph_method { PHX_SPH_WCSPH = 0, PHX_SPH_PCISPH = 1, PHX_SPH_IISPH = 2 } phx_sph_method;
 
 /* WCSPH::Configuration is empty (sph/sph.ixx:40); PCISPH/IISPH::Configuration
- * {pressure_iterations, checkpoint_interval} (sph/sph.ixx:104-107, 150-153). */
+ * {pressure_iterations, checkpoint_interval} (sph/sph.ixx:104-107, 150-153).
+ * density_error_tolerance > 0 selects tolerance mode for PCISPH/IISPH (an
+ * extension: iterate until the mean compression error
+ * (1/N) sum max(0, rho* - rho0) / rho0 of the prediction, reduced in FP64,
+ * is within the tolerance, at most max_iterations times; forward mode only);
+ * 0 gives the reference's fixed pressure_iterations. */
 typedef struct phx_sph_configuration {
     phx_sph_method method;
-    uint32_t pressure_iterations; /* PCISPH, IISPH; reference default 6 */
-    uint32_t checkpoint_interval; /* PCISPH, IISPH; reference default 2 */
+    uint32_t pressure_iterations;  /* PCISPH, IISPH; reference default 6 */
+    uint32_t checkpoint_interval;  /* PCISPH, IISPH; reference default 2 */
+    float density_error_tolerance; /* PCISPH, IISPH; 0: fixed iterations */
+    uint32_t max_iterations;
 } phx_sph_configuration;
 
 /* Method parameters, one value per particle (sph/sph.ixx:45-49, 112, 158).
@@ -438,6 +445,9 @@ phx_status phx_sph_forward_step(phx_sph* solver, const phx_sph_state* state, con
                                 phx_sph_step_cache* cache);
 /* The reference_gradient_norm of PCISPH/IISPH (sph/sph.cpp:37-58, 252, 365). */
 phx_status phx_sph_reference_gradient_norm(const phx_sph* solver, float* value);
+/* Pressure iterations run by the last PCISPH/IISPH forward_step (the
+ * configured count, or the count tolerance mode needed; 0 for WCSPH). */
+phx_status phx_sph_last_iterations(const phx_sph* solver, uint32_t* iterations);
 phx_status phx_sph_set_profiling(phx_sph* solver, int enabled);
 phx_status phx_sph_read_profile(phx_sph* solver, phx_pass_profile* profile, uint32_t capacity, uint32_t* count);
 

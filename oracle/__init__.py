@@ -306,6 +306,33 @@ def pressure_solver_step(domain, step_index, positions, velocities, controls, pa
             "pressures": prs}
 
 
+def pressure_solver_step_tolerance(domain, step_index, positions, velocities, controls, params: Dict[str, np.ndarray],
+                                   method="iisph", tolerance=1e-3, max_iterations=100, precision="f32"):
+    """SPH<PCISPH|IISPH>::forward_step in tolerance mode (an extension: the
+    reference runs fixed counts, sph.cpp:302, 415): iterate until the mean
+    compression error (1/N) sum max(0, rho* - rho0) / rho0 of the prediction
+    (FP64) is within the tolerance, at most max_iterations; the update of the
+    iteration that meets it still applies. Adds "iterations" to the result."""
+    if method not in ("pcisph", "iisph"):
+        raise ValueError(method)
+    n = domain.particle_count
+    pos, pp = _vec(positions, precision)
+    vel, vp = _vec(velocities, precision)
+    ctl, cp = _vec(controls, precision)
+    names = ("masses", "rest_densities", "viscosities", "relaxation")
+    arrs = {k: _arr(params[k], precision) for k in names}
+    nx, nxp = _out_vec(n, precision)
+    nv, nvp = _out_vec(n, precision)
+    rho = np.zeros(n, dtype=_dtype(precision))
+    prs = np.zeros(n, dtype=_dtype(precision))
+    it = c_uint32(0)
+    getattr(library(), "oracle_pressure_solver_step_tolerance_" + precision)(
+        ctypes.byref(domain), c_int(1 if method == "iisph" else 0), ctypes.c_double(tolerance),
+        c_uint32(max_iterations), c_uint64(step_index), pp, vp, cp, *(c_void_p(arrs[k].ctypes.data) for k in names),
+        nxp, nvp, c_void_p(rho.ctypes.data), c_void_p(prs.ctypes.data), ctypes.byref(it))
+    return {"positions": nx, "velocities": nv, "densities": rho, "pressures": prs, "iterations": int(it.value)}
+
+
 def wcsph_step(domain, step_index, positions, velocities, controls, params: Dict[str, np.ndarray],
                precision="f32"):
     """One SPH<WCSPH>::forward_step. Returns next state, densities and pressures."""

@@ -95,6 +95,13 @@ void oracle_set_threads(int threads);
                                           const real* viscosities, const real* relaxation, real* next_positions[3],    \
                                           real* next_velocities[3], real* densities_out, real* non_pressure_out[3],    \
                                           real* pressures_out);                                                         \
+    /* PCISPH / IISPH tolerance mode (an extension): iterate until the mean                                          \
+     * compression error of the prediction is within the tolerance */                                                 \
+    void oracle_pressure_solver_step_tolerance##suf(                                                                  \
+        const oracle_domain* domain, int iisph, double tolerance, uint32_t max_iterations, uint64_t step_index,      \
+        const real* positions[3], const real* velocities[3], const real* controls[3], const real* masses,          \
+        const real* rest_densities, const real* viscosities, const real* relaxation, real* next_positions[3],     \
+        real* next_velocities[3], real* densities_out, real* pressures_out, uint32_t* iterations_out);                  \
     /* SPH<WCSPH>::forward_step: sph.cpp:214-226 */                                                                     \
     void oracle_wcsph_step##suf(const oracle_domain* domain, uint64_t step_index, const real* positions[3],           \
                                 const real* velocities[3], const real* controls[3], const real* masses,                \

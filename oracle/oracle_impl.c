@@ -712,11 +712,17 @@ void SUF(oracle_dfsph_step_tolerance)(const oracle_domain* d, double tolerance, 
  * (pcisph.cu:104-110, V = m / rho0) and jacobi_update_forward_kernel
  * (iisph.cu:104-110, V = m / rho) share projection_update_kernel with the
  * volume density passed in; both target rho0. */
-void SUF(oracle_pressure_solver_step)(const oracle_domain* d, int iisph, uint32_t iterations, uint64_t step,
-                                      const real* pos[3], const real* vel[3], const real* ctl[3], const real* masses,
-                                      const real* rest_densities, const real* viscosities, const real* relaxation,
-                                      real* next_pos[3], real* next_vel[3], real* densities_out,
-                                      real* non_pressure_out[3], real* pressures_out) {
+/* The loop of both: with tolerance < 0 exactly `iterations` iterations;
+ * else tolerance mode (an extension, like oracle_dfsph_step_tolerance): stop
+ * after the iteration whose prediction has a mean compression error
+ * (1/N) sum_i max(0, rho*_i - rho0_i) / rho0_i (FP64) within the tolerance,
+ * at most `iterations`; that iteration's update and pressure force still
+ * apply. *executed receives the iterations run. */
+static void pressure_solver(const oracle_domain* d, int iisph, uint32_t iterations, double tolerance, uint64_t step,
+                            const real* pos[3], const real* vel[3], const real* ctl[3], const real* masses,
+                            const real* rest_densities, const real* viscosities, const real* relaxation,
+                            real* next_pos[3], real* next_vel[3], real* densities_out, real* non_pressure_out[3],
+                            real* pressures_out, uint32_t* executed) {
     const uint32_t n = d->particle_count;
     const real dt = (real)d->time_step;
     const real G = SUF(oracle_reference_gradient_norm)(d);
@@ -734,13 +740,18 @@ void SUF(oracle_pressure_solver_step)(const oracle_domain* d, int iisph, uint32_
         p[i] = K(0.0);
         for (int c = 0; c < 3; ++c) ap.c[c][i] = K(0.0);
     }
+    uint32_t run = 0;
     for (uint32_t it = 1; it <= iterations; ++it) {
         sph_predict_kernel(n, dt, box, pos, vel, CV(anp), CV(ap), xs.c, vs.c);
         density_kernel(&g, 0, pos, CV(xs), masses, rest_densities, rho_star);
+        const double error = tolerance >= 0.0 ? compression_error(n, rho_star, rest_densities, rest_densities) : 0.0;
         projection_update_kernel(n, dt, G, masses, iisph ? rho : rest_densities, rest_densities, rho_star, relaxation,
                                  p);
         pressure_kernel(&g, pos, masses, rest_densities, rho, p, ap.c);
+        run = it;
+        if (tolerance >= 0.0 && error <= tolerance) break;
     }
+    if (executed) *executed = run;
     sph_predict_kernel(n, dt, box, pos, vel, CV(anp), CV(ap), next_pos, next_vel);
     if (densities_out) memcpy(densities_out, rho, sizeof(real) * n);
     if (pressures_out) memcpy(pressures_out, p, sizeof(real) * n);
@@ -755,6 +766,25 @@ void SUF(oracle_pressure_solver_step)(const oracle_domain* d, int iisph, uint32_
     free_vfield(&xs);
     free_vfield(&vs);
     grid_free(&g);
+}
+
+void SUF(oracle_pressure_solver_step)(const oracle_domain* d, int iisph, uint32_t iterations, uint64_t step,
+                                      const real* pos[3], const real* vel[3], const real* ctl[3], const real* masses,
+                                      const real* rest_densities, const real* viscosities, const real* relaxation,
+                                      real* next_pos[3], real* next_vel[3], real* densities_out,
+                                      real* non_pressure_out[3], real* pressures_out) {
+    pressure_solver(d, iisph, iterations, -1.0, step, pos, vel, ctl, masses, rest_densities, viscosities, relaxation,
+                    next_pos, next_vel, densities_out, non_pressure_out, pressures_out, NULL);
+}
+
+void SUF(oracle_pressure_solver_step_tolerance)(const oracle_domain* d, int iisph, double tolerance,
+                                                uint32_t max_iterations, uint64_t step, const real* pos[3],
+                                                const real* vel[3], const real* ctl[3], const real* masses,
+                                                const real* rest_densities, const real* viscosities,
+                                                const real* relaxation, real* next_pos[3], real* next_vel[3],
+                                                real* densities_out, real* pressures_out, uint32_t* iterations_out) {
+    pressure_solver(d, iisph, max_iterations, tolerance, step, pos, vel, ctl, masses, rest_densities, viscosities,
+                    relaxation, next_pos, next_vel, densities_out, NULL, pressures_out, iterations_out);
 }
 
 /* ---------------------------------------------------------------- WCSPH */

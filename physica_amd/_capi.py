@@ -219,7 +219,8 @@ PHX_SPH_WCSPH, PHX_SPH_PCISPH, PHX_SPH_IISPH = 0, 1, 2
 
 
 class SphConfiguration(ctypes.Structure):
-    _fields_ = [("method", c_int), ("pressure_iterations", c_uint32), ("checkpoint_interval", c_uint32)]
+    _fields_ = [("method", c_int), ("pressure_iterations", c_uint32), ("checkpoint_interval", c_uint32),
+                ("density_error_tolerance", c_float), ("max_iterations", c_uint32)]
 
 
 class SphMethodParameters(ctypes.Structure):
@@ -365,6 +366,7 @@ PROTOTYPES = {
         [c_void_p, POINTER(SphState), POINTER(Control), POINTER(SphParameters), POINTER(SphState), c_void_p],
     ),
     "phx_sph_reference_gradient_norm": (c_int, [c_void_p, POINTER(c_float)]),
+    "phx_sph_last_iterations": (c_int, [c_void_p, POINTER(c_uint32)]),
     "phx_sph_set_profiling": (c_int, [c_void_p, c_int]),
     "phx_sph_read_profile": (c_int, [c_void_p, POINTER(PassProfile), c_uint32, POINTER(c_uint32)]),
     "phx_sph_cache_neighborhood": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),

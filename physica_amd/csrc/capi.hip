@@ -550,6 +550,13 @@ phx_status phx_sph_reference_gradient_norm(const phx_sph* solver, float* value) 
     });
 }
 
+phx_status phx_sph_last_iterations(const phx_sph* solver, uint32_t* iterations) {
+    return guarded([&] {
+        phx::require(solver != nullptr && iterations != nullptr, PHX_ERROR_INVALID_ARGUMENT, "null argument");
+        *iterations = solver->impl.last_iterations();
+    });
+}
+
 phx_status phx_sph_set_profiling(phx_sph* solver, int enabled) {
     return guarded([&] {
         phx::require(solver != nullptr, PHX_ERROR_INVALID_ARGUMENT, "null solver");

@@ -196,10 +196,18 @@ SphSolver::SphSolver(const phx_domain_configuration* domain, const phx_sph_confi
             "boundary particles are supported in forward mode only");
     boundary_.upload(domain_.config);
     domain_.config.boundary_particles = nullptr;  // the caller's host array is not kept
+    const float tolerance = config_.density_error_tolerance;
+    require(tolerance >= 0.0F && std::isfinite(tolerance), PHX_ERROR_INVALID_ARGUMENT,
+            "density_error_tolerance must be finite and non-negative");
     if (config_.method != PHX_SPH_WCSPH) {
         require(config_.checkpoint_interval > 0u, PHX_ERROR_INVALID_ARGUMENT, "checkpoint_interval must be positive");
         require(config_.pressure_iterations < 4096u, PHX_ERROR_INVALID_ARGUMENT, "too many pressure iterations");
+        require(tolerance == 0.0F || mode == PHX_MODE_FORWARD, PHX_ERROR_UNSUPPORTED,
+                "tolerance mode is supported in forward mode only");
+        require(tolerance == 0.0F || (config_.max_iterations >= 1u && config_.max_iterations < 2048u),
+                PHX_ERROR_INVALID_ARGUMENT, "tolerance mode needs 1 <= max_iterations < 2048");
     } else {
+        require(tolerance == 0.0F, PHX_ERROR_INVALID_ARGUMENT, "tolerance mode applies to PCISPH and IISPH");
         config_.pressure_iterations = 0u;
     }
     gradient_norm_ = reference_gradient_norm(domain_.config);
@@ -209,7 +217,15 @@ SphSolver::SphSolver(const phx_domain_configuration* domain, const phx_sph_confi
     u_.resize(n);
     for (DeviceBuffer<float>* b : {&rho0_, &relax_, &tait_, &t_}) b->resize(n);
     if (boundary_.count() > 0u && config_.method == PHX_SPH_WCSPH) btension_.resize(n);
-    masks_.allocate(n, config_.pressure_iterations + 1u, domain_.config.support_radius);
+    // one displacement slot per predicting pass
+    const uint32_t predictions = (tolerance > 0.0F ? config_.max_iterations : config_.pressure_iterations) + 1u;
+    masks_.allocate(n, predictions, domain_.config.support_radius);
+    if (tolerance > 0.0F) {
+        tol_partials_.resize(static_cast<size_t>(pass_blocks(n)) * kWaves);  // one per projection wave
+        tol_done_.resize(1u);
+        tol_host_.resize(2u * sizeof(uint32_t));
+        for (hipEvent_t& e : tol_events_) PHX_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    }
     if (mode == PHX_MODE_DIFFERENTIABLE && config_.method != PHX_SPH_WCSPH) {
         // PCISPH::forward (sph.cpp:295-309) and IISPH::forward (sph.cpp:408-421)
         // are the DFSPH step with nd = 0, no warm start and their relaxation in
@@ -225,6 +241,11 @@ SphSolver::SphSolver(const phx_domain_configuration* domain, const phx_sph_confi
         engine_->set_volume_rest_density(config_.method == PHX_SPH_PCISPH);
         for (auto& b : engine_warm_) b.resize(n);
     }
+}
+
+SphSolver::~SphSolver() {
+    for (hipEvent_t e : tol_events_)
+        if (e != nullptr) (void)hipEventDestroy(e);
 }
 
 void SphSolver::require_derivatives(const SphCache& cache) const {
@@ -500,7 +521,51 @@ void SphSolver::forward_pressure(const phx_sph_state& state, const phx_control& 
     pa.boundary = boundary;
     pa.rest_density = rho0_.data();
 
+    if (config_.density_error_tolerance > 0.0F) {
+        // Tolerance mode: iterate until the projection's mean compression
+        // error is within the tolerance (or max_iterations); the force pass
+        // of that iteration writes the state. Launched one iteration ahead
+        // of the host, as DfsphSolver's tolerance mode (passes of an
+        // iteration past the end return at once).
+        const uint32_t max_it = config_.max_iterations;
+        const double tolerance = static_cast<double>(config_.density_error_tolerance);
+        uint32_t* done = tol_done_.data();
+        auto* host = static_cast<uint32_t*>(tol_host_.data());
+        fill_u32(stream_, done, 1u, 0u);
+        fa.slot = 0u;
+        fa.final_step = 0u;
+        launch_force<true, kInitPredict>(profiler_, stream_, blocks, fa, kSphForceInit);
+        auto launch_iteration = [&](uint32_t it) {
+            pa.slot = it - 1u;  // the prediction of the previous pass
+            pa.cv = Convergence{tol_partials_.data(), done, rho0_.data()};
+            launch_projection(profiler_, stream_, blocks, pa, kSphProjection);
+            converge_kernel<<<1, kConvergeBlock, 0, stream_>>>(tol_partials_.data(), blocks * kWaves, n, tolerance,
+                                                               it, max_it, done);
+            PHX_HIP(hipGetLastError());
+            PHX_HIP(hipMemcpyAsync(host + (it & 1u), done, sizeof(uint32_t), hipMemcpyDeviceToHost, stream_));
+            PHX_HIP(hipEventRecord(tol_events_[it & 1u], stream_));
+            fa.slot = it;
+            fa.done = done;
+            fa.iteration = it;
+            fa.final_step = 0u;
+            launch_force<false, kDivPredict>(profiler_, stream_, blocks, fa, kSphForce);
+        };
+        launch_iteration(1u);
+        uint32_t it = 1u;
+        for (;; ++it) {
+            if (it < max_it) launch_iteration(it + 1u);
+            PHX_HIP(hipEventSynchronize(tol_events_[it & 1u]));
+            // the projection of iteration max_it always ends the step
+            if (host[it & 1u] != 0u || it >= max_it) break;
+        }
+        last_iterations_ = it;
+        pa.cv = Convergence{};
+        fa.done = nullptr;
+        masks_.end_step(stream_);
+        return;
+    }
     const uint32_t iterations = config_.pressure_iterations;
+    last_iterations_ = iterations;
     uint32_t slot = 0;  // displacement slot written by the latest predicting pass
     fa.slot = slot;
     fa.final_step = iterations == 0u ? 1u : 0u;

@@ -80,6 +80,7 @@ class SphSolver {
   public:
     SphSolver(const phx_domain_configuration* domain, const phx_sph_configuration* configuration,
               phx_execution_mode mode, hipStream_t stream);
+    ~SphSolver();
     SphSolver(const SphSolver&) = delete;
     SphSolver& operator=(const SphSolver&) = delete;
 
@@ -97,6 +98,8 @@ class SphSolver {
     phx_sph_method method() const { return config_.method; }
     float gradient_norm() const { return gradient_norm_; }
     PassProfiler& profiler() { return profiler_; }
+    // pressure iterations run by the last PCISPH/IISPH step
+    uint32_t last_iterations() const { return last_iterations_; }
 
   private:
     void forward_wcsph(const phx_sph_state& state, const phx_control& control, const phx_sph_parameters& parameters,
@@ -130,6 +133,13 @@ class SphSolver {
     VecBuffer u_;
     DeviceBuffer<float> rho0_, relax_, tait_, t_;
     DeviceBuffer<float> btension_;  // WCSPH boundary_surface_tension in cell order (boundary samples only)
+    // tolerance mode (PCISPH, IISPH): per-wave FP64 error sums, the device
+    // done word, its pinned host copies and their events
+    DeviceBuffer<double> tol_partials_;
+    DeviceBuffer<uint32_t> tol_done_;
+    PinnedBuffer tol_host_;
+    hipEvent_t tol_events_[2] = {nullptr, nullptr};
+    uint32_t last_iterations_ = 0;
     BoundarySamples boundary_;      // the domain's boundary samples on the device
 };
 

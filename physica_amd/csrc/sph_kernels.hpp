@@ -315,7 +315,9 @@ __global__ void __launch_bounds__(kBlock) force_kernel(ForceArgs a) {
     const bool ends = ended != 0u && ended == a.iteration;
     // a second pressure field: the divergence phase's end
     const bool kTwo = kEpilogue == kDivEnd || (kAuto && ends);
-    const uint32_t final_step = a.final_step != 0u || (kEpilogue == kDenPredict && ends) ? 1u : 0u;
+    // tolerance mode: the iteration that ends the last phase writes the state
+    const uint32_t final_step =
+        a.final_step != 0u || ((kEpilogue == kDenPredict || kEpilogue == kDivPredict) && ends) ? 1u : 0u;
     const uint32_t i = xcd_block() * kBlock + threadIdx.x;
     const bool active = i < a.n;
     float moved2 = 0.0F;

@@ -456,10 +456,15 @@ class PCISPH:
     class Configuration:
         pressure_iterations: int = 6
         checkpoint_interval: int = 2
+        # tolerance mode (an extension): iterate to this mean compression
+        # error, at most max_iterations times; 0: fixed pressure_iterations
+        density_error_tolerance: float = 0.0
+        max_iterations: int = 100
 
         def to_c(self) -> _capi.SphConfiguration:
             return _capi.SphConfiguration(
-                _capi.PHX_SPH_PCISPH, int(self.pressure_iterations), int(self.checkpoint_interval)
+                _capi.PHX_SPH_PCISPH, int(self.pressure_iterations), int(self.checkpoint_interval),
+                float(self.density_error_tolerance), int(self.max_iterations)
             )
 
     @dataclass
@@ -507,9 +512,15 @@ class IISPH:
     class Configuration:
         pressure_iterations: int = 6
         checkpoint_interval: int = 2
+        # tolerance mode (an extension): iterate to this mean compression
+        # error, at most max_iterations times; 0: fixed pressure_iterations
+        density_error_tolerance: float = 0.0
+        max_iterations: int = 100
 
         def to_c(self) -> _capi.SphConfiguration:
-            return _capi.SphConfiguration(_capi.PHX_SPH_IISPH, int(self.pressure_iterations), int(self.checkpoint_interval))
+            return _capi.SphConfiguration(_capi.PHX_SPH_IISPH, int(self.pressure_iterations),
+                                          int(self.checkpoint_interval), float(self.density_error_tolerance),
+                                          int(self.max_iterations))
 
     @dataclass
     class State:
@@ -859,7 +870,9 @@ class SPH:
     def last_iterations(self) -> Tuple[int, int]:
         """Iterations of the last DFSPH step (divergence, density)."""
         if self.method is not DFSPH:
-            return (int(getattr(self.method_configuration, "pressure_iterations", 0)), 0)
+            it = ctypes.c_uint32()
+            _capi.call("phx_sph_last_iterations", self._handle, ctypes.byref(it))
+            return (int(it.value), 0)
         v = (ctypes.c_uint32 * 2)()
         _capi.call("phx_dfsph_last_iterations", self._handle, v)
         return int(v[0]), int(v[1])

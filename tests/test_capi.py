@@ -101,6 +101,27 @@ def test_boundary_particles_rejected_where_unsupported():
         _capi.PHX_ERROR_INVALID_ARGUMENT
 
 
+def test_sph_tolerance_configuration_rejected_where_invalid():
+    lib = _capi.library()
+    h = ctypes.c_void_p()
+
+    def create(configuration, mode=0):
+        return lib.phx_sph_create(ctypes.byref(cfg().to_c()), ctypes.byref(configuration), mode, None,
+                                  ctypes.byref(h))
+
+    c = PCISPH.Configuration(density_error_tolerance=1e-3).to_c()
+    assert create(c, mode=1) == _capi.PHX_ERROR_UNSUPPORTED
+    assert "forward mode only" in lib.phx_last_error().decode()
+    c = PCISPH.Configuration(density_error_tolerance=1e-3, max_iterations=0).to_c()
+    assert create(c) == _capi.PHX_ERROR_INVALID_ARGUMENT
+    c = PCISPH.Configuration(density_error_tolerance=float("nan")).to_c()
+    assert create(c) == _capi.PHX_ERROR_INVALID_ARGUMENT
+    c = _capi.SphConfiguration(_capi.PHX_SPH_WCSPH, 0, 2, 1e-3, 10)
+    assert create(c) == _capi.PHX_ERROR_INVALID_ARGUMENT
+    assert "PCISPH and IISPH" in lib.phx_last_error().decode()
+    assert h.value is None
+
+
 def test_neighborhood_workspace_grows_with_boundary_samples():
     lib = _capi.library()
     plain, with_b = ctypes.c_size_t(), ctypes.c_size_t()

@@ -140,3 +140,49 @@ def test_empty_domain(gpu_device, method):
     solver.forward_step(state, control, params, nxt, cache)
     torch.cuda.synchronize()
     assert nxt.particles.step_index == 10
+
+
+@pytest.mark.parametrize("method,tolerance,max_iterations", [("iisph", 1e-3, 100), ("pcisph", 1e-3, 100),
+                                                             ("iisph", 1e-4, 5), ("pcisph", 1e-2, 100)])
+def test_pressure_solver_tolerance_mode_bit_exact(gpu_device, method, tolerance, max_iterations):
+    """Tolerance mode of PCISPH/IISPH (the IISPH scale configuration iterates
+    to 0.1 % with an FP64 residual): iteration counts and states equal the
+    oracle's over a rollout, a capped run stops at its cap."""
+    print(f"sph tolerance {method} {tolerance} cap {max_iterations}", flush=True)
+    from physica_amd.particle import IISPH, PCISPH, SPH
+    sc = scene_for(method, shape=(12, 12, 12), seed=13)
+    cls = IISPH if method == "iisph" else PCISPH
+    solver, state, nxt, control, params, cache = sph_setup(sc, gpu_device, method)
+    solver = SPH(sc.domain, cls.Configuration(density_error_tolerance=tolerance, max_iterations=max_iterations),
+                 device=gpu_device)
+    state = solver.allocate_state()
+    nxt = solver.allocate_state()
+    control = solver.allocate_control()
+    cache = solver.allocate_step_cache()
+    state.particles.positions.upload(sc.positions)
+    state.particles.velocities.upload(sc.velocities)
+    p2 = solver.allocate_parameters()
+    for name in ("masses", "rest_densities", "viscosities", "surface_tensions"):
+        getattr(p2.particles, name).copy_(getattr(params.particles, name))
+    relax = p2.method.jacobi_relaxation if method == "iisph" else p2.method.pressure_relaxation
+    relax.copy_(torch.from_numpy(sc.parameters["relaxation"]).to(gpu_device))
+    d = oracle.make_domain(sc.domain)
+    x, v = sc.positions, sc.velocities
+    counts = []
+    for step in range(4):
+        solver.forward_step(state, control, p2, nxt, cache)
+        torch.cuda.synchronize()
+        exp = oracle.pressure_solver_step_tolerance(d, step, x, v, np.zeros_like(x), sc.parameters, method,
+                                                    tolerance, max_iterations)
+        assert solver.last_iterations()[0] == exp["iterations"], step
+        got = {"positions": nxt.particles.positions.numpy(), "velocities": nxt.particles.velocities.numpy()}
+        for name in ("positions", "velocities"):
+            assert bits_equal(got[name], exp[name]), (step, name)
+        assert bits_equal(cache.densities().cpu().numpy(), exp["densities"]), step
+        counts.append(exp["iterations"])
+        x, v = got["positions"], got["velocities"]
+        solver.copy_state(nxt, state)
+    print("iterations", counts, flush=True)
+    assert max(counts) <= max_iterations
+    if max_iterations == 5:
+        assert 5 in counts

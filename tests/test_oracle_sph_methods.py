@@ -221,3 +221,30 @@ def test_f32_step_tracks_f64(method):
     ulp = float(np.spacing(np.float32(np.abs(sc.positions).max())))  # FP32 position resolution
     assert np.abs(r32["positions"] - r64["positions"]).max() <= 1e-4 * span + 4 * ulp
     assert np.abs(r32["densities"] / r64["densities"] - 1.0).max() < 1e-5
+
+
+def test_pressure_solver_tolerance_equals_fixed_counts():
+    """Tolerance mode of PCISPH/IISPH (oracle): the step equals the fixed-count
+    step run for the iterations it executed, and a tighter tolerance never
+    needs fewer iterations; a cap of one iteration is the one-iteration step."""
+    from physica_amd import scenes
+    for method in ("pcisph", "iisph"):
+        sc = scenes.two_cubes(method, shape=(8, 8, 8), seed=3)
+        n = sc.count
+        rng = np.random.default_rng(4)
+        v = rng.normal(0, 0.3, size=(3, n)).astype(np.float32)
+        x = (sc.positions * np.float32(0.97)).astype(np.float32)  # compressed: pressure acts
+        d = oracle.make_domain(sc.domain)
+        u = np.zeros_like(x)
+        counts = []
+        for tol in (1e-2, 1e-3, 1e-4):
+            t = oracle.pressure_solver_step_tolerance(d, 0, x, v, u, sc.parameters, method, tol, 50)
+            f = oracle.pressure_solver_step(d, 0, x, v, u, sc.parameters, method, t["iterations"])
+            assert 1 <= t["iterations"] <= 50
+            assert np.array_equal(t["positions"], f["positions"]) and np.array_equal(t["pressures"], f["pressures"])
+            counts.append(t["iterations"])
+        assert counts == sorted(counts) and counts[-1] > counts[0], counts
+        one = oracle.pressure_solver_step_tolerance(d, 0, x, v, u, sc.parameters, method, 0.0, 1)
+        assert one["iterations"] == 1
+        assert np.array_equal(one["positions"], oracle.pressure_solver_step(d, 0, x, v, u, sc.parameters, method,
+                                                                            1)["positions"])
