@@ -7,7 +7,7 @@
 // buffers throw on failure as well); the C ABI below never throws.
 //
 // Covered: Domain (with the tangent and adjoint fields), NeighborhoodSearch,
-// density::{sph,pbf}_forward, SPH<WCSPH|PCISPH|IISPH|DFSPH> and PBF with
+// density::{sph,pbf}_{forward,jvp,vjp}, SPH<WCSPH|PCISPH|IISPH|DFSPH> and PBF with
 // forward_step, jvp_step and vjp_step. The solvers keep their own
 // neighbourhood, densities and per-method cache in the C ABI's step cache;
 // jvp_step/vjp_step read the step the cache recorded, so their state,
@@ -537,6 +537,58 @@ inline void pbf_forward(const Domain& domain, const VectorField& topology_positi
                         const ParticleParameters& parameters, const Neighborhood& neighborhood, ScalarField& densities,
                         const DeviceBoundary* boundary = nullptr) {
     forward(domain, true, topology_positions, positions, parameters, neighborhood, densities, boundary);
+}
+// density::{sph,pbf}_jvp (density.cu:42-63): overwrites density_tangent
+inline void jvp(const Domain& domain, bool pbf_kernel, const VectorField& topology_positions,
+                const VectorField& positions, const VectorField& position_tangent, const ParticleParameters& parameters,
+                const ParticleParameterTangent& parameter_tangent, const Neighborhood& neighborhood,
+                ScalarField& density_tangent, const DeviceBoundary* boundary) {
+    const phx_boundary_view b = boundary != nullptr ? boundary->view(neighborhood.boundary_time) : phx_boundary_view{};
+    check(phx_density_jvp(domain.stream, domain.configuration.particle_count, domain.configuration.support_radius,
+                          pbf_kernel ? 1 : 0, topology_positions.const_view(), positions.const_view(),
+                          position_tangent.const_view(), parameters.view(), parameter_tangent.view(),
+                          neighborhood.view(), b, density_tangent.values.data()),
+          "density::jvp");
+}
+// density::{sph,pbf}_vjp (density.cu:65-98): accumulates (+=) the adjoints
+inline void vjp(const Domain& domain, bool pbf_kernel, const VectorField& topology_positions,
+                const VectorField& positions, const ParticleParameters& parameters, const Neighborhood& neighborhood,
+                const ScalarAdjointField& density_adjoint, VectorAdjointField& position_adjoint,
+                ParticleParameterAdjoint& parameter_adjoint, const DeviceBoundary* boundary) {
+    const phx_boundary_view b = boundary != nullptr ? boundary->view(neighborhood.boundary_time) : phx_boundary_view{};
+    check(phx_density_vjp(domain.stream, domain.configuration.particle_count, domain.configuration.support_radius,
+                          pbf_kernel ? 1 : 0, topology_positions.const_view(), positions.const_view(), parameters.view(),
+                          neighborhood.view(), b, density_adjoint.values.data(), position_adjoint.view(),
+                          parameter_adjoint.view()),
+          "density::vjp");
+}
+inline void sph_jvp(const Domain& domain, const VectorField& topology_positions, const VectorField& positions,
+                    const VectorField& position_tangent, const ParticleParameters& parameters,
+                    const ParticleParameterTangent& parameter_tangent, const Neighborhood& neighborhood,
+                    ScalarField& density_tangent, const DeviceBoundary* boundary = nullptr) {
+    jvp(domain, false, topology_positions, positions, position_tangent, parameters, parameter_tangent, neighborhood,
+        density_tangent, boundary);
+}
+inline void pbf_jvp(const Domain& domain, const VectorField& topology_positions, const VectorField& positions,
+                    const VectorField& position_tangent, const ParticleParameters& parameters,
+                    const ParticleParameterTangent& parameter_tangent, const Neighborhood& neighborhood,
+                    ScalarField& density_tangent, const DeviceBoundary* boundary = nullptr) {
+    jvp(domain, true, topology_positions, positions, position_tangent, parameters, parameter_tangent, neighborhood,
+        density_tangent, boundary);
+}
+inline void sph_vjp(const Domain& domain, const VectorField& topology_positions, const VectorField& positions,
+                    const ParticleParameters& parameters, const Neighborhood& neighborhood,
+                    const ScalarAdjointField& density_adjoint, VectorAdjointField& position_adjoint,
+                    ParticleParameterAdjoint& parameter_adjoint, const DeviceBoundary* boundary = nullptr) {
+    vjp(domain, false, topology_positions, positions, parameters, neighborhood, density_adjoint, position_adjoint,
+        parameter_adjoint, boundary);
+}
+inline void pbf_vjp(const Domain& domain, const VectorField& topology_positions, const VectorField& positions,
+                    const ParticleParameters& parameters, const Neighborhood& neighborhood,
+                    const ScalarAdjointField& density_adjoint, VectorAdjointField& position_adjoint,
+                    ParticleParameterAdjoint& parameter_adjoint, const DeviceBoundary* boundary = nullptr) {
+    vjp(domain, true, topology_positions, positions, parameters, neighborhood, density_adjoint, position_adjoint,
+        parameter_adjoint, boundary);
 }
 }  // namespace density
 

@@ -245,6 +245,24 @@ phx_status phx_density_forward(phx_stream stream, uint32_t particle_count, float
                                phx_particle_parameters parameters, phx_neighborhood_view neighborhood,
                                phx_boundary_view boundary, float* densities);
 
+/* density_jvp (density.cu:42-63): the FP32 density tangent for position and
+ * parameter tangents (masses and rest densities are read; a null tangent
+ * array counts as zero). */
+phx_status phx_density_jvp(phx_stream stream, uint32_t particle_count, float support_radius, int pbf_kernel,
+                           phx_const_vector_view topology_positions, phx_const_vector_view positions,
+                           phx_const_vector_view position_tangent, phx_particle_parameters parameters,
+                           phx_particle_parameter_tangent parameter_tangent, phx_neighborhood_view neighborhood,
+                           phx_boundary_view boundary, float* density_tangent);
+
+/* density_vjp (density.cu:65-98): the FP64 VJP of the densities for
+ * density_adjoint, accumulated (+=) into position_adjoint and the mass and
+ * rest-density adjoints; null output arrays are skipped. */
+phx_status phx_density_vjp(phx_stream stream, uint32_t particle_count, float support_radius, int pbf_kernel,
+                           phx_const_vector_view topology_positions, phx_const_vector_view positions,
+                           phx_particle_parameters parameters, phx_neighborhood_view neighborhood,
+                           phx_boundary_view boundary, const double* density_adjoint,
+                           phx_adjoint_vector_view position_adjoint, phx_particle_parameter_adjoint parameter_adjoint);
+
 /* ----------------------------------------------------------------- DFSPH */
 /* DFSPH::Configuration, sph/sph.ixx:196-201. density_error_tolerance > 0
  * selects tolerance mode (an extension: iterate each phase until the mean

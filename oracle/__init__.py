@@ -171,6 +171,42 @@ def density(domain, grid_step, pbf_kernel, topology, positions, masses, rest_den
     return out
 
 
+def density_jvp(domain, grid_step, pbf_kernel, topology, positions, position_tangent, masses, mass_tangent,
+                rest_densities, rest_tangent, precision="f32"):
+    """density_jvp (density.cu:42-63): the density tangent, primal precision."""
+    n = domain.particle_count
+    topo, tp = _vec(topology, precision)
+    pos, pp = _vec(positions, precision)
+    tan, tnp = _vec(position_tangent, precision)
+    arrs = [_arr(a, precision) for a in (masses, mass_tangent, rest_densities, rest_tangent)]
+    out = np.zeros(n, dtype=_dtype(precision))
+    getattr(library(), "oracle_density_jvp_" + precision)(
+        ctypes.byref(domain), c_uint64(grid_step), c_int(1 if pbf_kernel else 0), tp, pp, tnp,
+        *(c_void_p(a.ctypes.data) for a in arrs), c_void_p(out.ctypes.data))
+    return out
+
+
+def density_vjp(domain, grid_step, pbf_kernel, topology, positions, masses, rest_densities, density_adjoint,
+                precision="f32"):
+    """density_vjp (density.cu:65-98) in FP64: (position adjoint [3, N],
+    mass adjoint [N], rest-density adjoint [N]) from zero."""
+    n = domain.particle_count
+    topo, tp = _vec(topology, precision)
+    pos, pp = _vec(positions, precision)
+    m = _arr(masses, precision)
+    r0 = _arr(rest_densities, precision)
+    rho_a = np.ascontiguousarray(density_adjoint, dtype=np.float64)
+    pos_a = np.zeros((3, n))
+    m_a = np.zeros(n)
+    r_a = np.zeros(n)
+    ptrs = (c_void_p * 3)(*(c_void_p(pos_a[k].ctypes.data) for k in range(3)))
+    getattr(library(), "oracle_density_vjp_" + precision)(
+        ctypes.byref(domain), c_uint64(grid_step), c_int(1 if pbf_kernel else 0), tp, pp, c_void_p(m.ctypes.data),
+        c_void_p(r0.ctypes.data), c_void_p(rho_a.ctypes.data), ptrs, c_void_p(m_a.ctypes.data),
+        c_void_p(r_a.ctypes.data))
+    return pos_a, m_a, r_a
+
+
 def reference_gradient_norm(domain, precision="f32") -> float:
     return float(getattr(library(), "oracle_reference_gradient_norm_" + precision)(ctypes.byref(domain)))
 

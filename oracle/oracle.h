@@ -77,6 +77,15 @@ void oracle_set_threads(int threads);
     void oracle_density##suf(const oracle_domain* domain, uint64_t grid_step, int pbf_kernel, const real* topology[3], \
                              const real* positions[3], const real* masses, const real* rest_densities,                 \
                              real* densities);                                                                          \
+    /* density_jvp / density_vjp (density.cu:42-98); the VJP accumulates in FP64 */                                   \
+    void oracle_density_jvp##suf(const oracle_domain* domain, uint64_t grid_step, int pbf_kernel,                    \
+                                 const real* topology[3], const real* positions[3], const real* position_tangent[3], \
+                                 const real* masses, const real* mass_tangent, const real* rest_densities,           \
+                                 const real* rest_tangent, real* out);                                               \
+    void oracle_density_vjp##suf(const oracle_domain* domain, uint64_t grid_step, int pbf_kernel,                    \
+                                 const real* topology[3], const real* positions[3], const real* masses,              \
+                                 const real* rest_densities, const double* density_adjoint,                          \
+                                 double* position_adjoint[3], double* mass_adjoint, double* rest_adjoint);           \
     /* sph.cpp:37-58 */                                                                                                 \
     real oracle_reference_gradient_norm##suf(const oracle_domain* domain);                                             \
     /* SPH<DFSPH>::forward_step: sph.ixx:350-354 + DFSPH::forward sph.cpp:572-610 (fixed iterations) */                 \

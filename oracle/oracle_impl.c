@@ -549,6 +549,39 @@ void SUF(oracle_density)(const oracle_domain* d, uint64_t grid_step, int pbf_ker
     grid_free(&g);
 }
 
+static void density_jvp_kernel(const grid_t* g, int pbf, const real* const topo[3], const real* const pos[3],
+                               const real* const pos_t[3], const real* masses, const real* masses_t,
+                               const real* rest, const real* rest_t, real* out);
+static void density_vjp_kernel(const grid_t* g, int pbf, const real* const topo[3], const real* const pos[3],
+                               const real* masses, const real* rest, const double* rho_a, double* const pos_a[3],
+                               double* masses_a, double* rest_a);
+
+/* density_jvp / density_vjp (density.cu:42-98): the tangent of the density
+ * (FP32 or FP64, the primal precision) and its VJP in FP64, accumulated
+ * (+=) into pos_a, masses_a and rest_a, in gather form; the grid is built
+ * from the topology positions at grid_step. */
+void SUF(oracle_density_jvp)(const oracle_domain* d, uint64_t grid_step, int pbf_kernel, const real* topology[3],
+                             const real* positions[3], const real* position_tangent[3], const real* masses,
+                             const real* mass_tangent, const real* rest_densities, const real* rest_tangent,
+                             real* out) {
+    grid_t g;
+    grid_build(&g, d, grid_step, topology[0], topology[1], topology[2]);
+    density_jvp_kernel(&g, pbf_kernel, topology, positions, position_tangent, masses, mass_tangent, rest_densities,
+                       rest_tangent, out);
+    grid_free(&g);
+}
+
+void SUF(oracle_density_vjp)(const oracle_domain* d, uint64_t grid_step, int pbf_kernel, const real* topology[3],
+                             const real* positions[3], const real* masses, const real* rest_densities,
+                             const double* density_adjoint, double* position_adjoint[3], double* mass_adjoint,
+                             double* rest_adjoint) {
+    grid_t g;
+    grid_build(&g, d, grid_step, topology[0], topology[1], topology[2]);
+    density_vjp_kernel(&g, pbf_kernel, topology, positions, masses, rest_densities, density_adjoint, position_adjoint,
+                       mass_adjoint, rest_adjoint);
+    grid_free(&g);
+}
+
 void SUF(oracle_non_pressure)(const oracle_domain* d, uint64_t grid_step, const real* positions[3],
                               const real* velocities[3], const real* controls[3], const real* masses,
                               const real* viscosities, const real* densities, real* out[3]) {
@@ -2014,7 +2047,12 @@ static real viscosity_laplacian_tangent(v3 d, v3 dd, real h) {
 }
 
 /* density_jvp_kernel, density/density.cu:42-67 (cubic kernel) */
-static void density_jvp_kernel(const grid_t* g, const real* const topo[3], const real* const pos[3],
+/* the density kernel and its gradient: poly6 (PBF) or the cubic spline,
+ * density_kernel / density_kernel_gradient of density.cu:9-16 */
+static real density_w(v3 d, real h, int pbf) { return pbf ? poly6(d, h) : cubic(d, h); }
+static v3 density_g(v3 d, real h, int pbf) { return pbf ? poly6_gradient(d, h) : cubic_gradient(d, h); }
+
+static void density_jvp_kernel(const grid_t* g, int pbf, const real* const topo[3], const real* const pos[3],
                                const real* const pos_t[3], const real* masses, const real* masses_t,
                                const real* rest, const real* rest_t, real* out) {
     const real h = (real)g->d->support_radius;
@@ -2030,14 +2068,14 @@ static void density_jvp_kernel(const grid_t* g, const real* const topo[3], const
             for (uint32_t s = cr[k].first; s < cr[k].last; ++s) {
                 const uint32_t j = g->idx[s];
                 const v3 d = v3_sub(p, v3_load(pos, j));
-                result += masses_t[j] * cubic(d, h) +
-                          masses[j] * v3_dot(cubic_gradient(d, h), v3_sub(t, v3_load(pos_t, j)));
+                result += masses_t[j] * density_w(d, h, pbf) +
+                          masses[j] * v3_dot(density_g(d, h, pbf), v3_sub(t, v3_load(pos_t, j)));
             }
             for (uint32_t s = cr[k].bfirst; s < cr[k].blast; ++s) {
                 const uint32_t b = g->bidx[s];
                 const v3 d = v3_sub(p, boundary_position(g, b));
                 result += boundary_volume(g, b) *
-                          (rest_t[i] * cubic(d, h) + rest[i] * v3_dot(cubic_gradient(d, h), t));
+                          (rest_t[i] * density_w(d, h, pbf) + rest[i] * v3_dot(density_g(d, h, pbf), t));
             }
         }
         out[i] = result;
@@ -2208,7 +2246,7 @@ void SUF(oracle_dfsph_jvp)(const oracle_domain* d, uint32_t nd, uint32_t ne, int
     const real *m = prm[DF_MASS], *rest = prm[DF_REST], *m_t = prm_t[DF_MASS], *rest_t = prm_t[DF_REST];
     /* sph.ixx:357: tangent of the state densities */
     real* rho_t = alloc_real(n);
-    density_jvp_kernel(g, pos, pos, pos_t, m, m_t, rest, rest_t, rho_t);
+    density_jvp_kernel(g, 0, pos, pos, pos_t, m, m_t, rest, rest_t, rho_t);
     vfield anp_t = alloc_vfield(n), ap_t = alloc_vfield(n), adiv_t = alloc_vfield(n), tot_t = alloc_vfield(n);
     vfield xs_t = alloc_vfield(n), acc = alloc_vfield(n);
     non_pressure_jvp_kernel(g, pos, vel, ctl_t, pos_t, vel_t, m, m_t, prm[DF_VISC], prm_t[DF_VISC], r.rho, rho_t,
@@ -2230,7 +2268,7 @@ void SUF(oracle_dfsph_jvp)(const oracle_domain* d, uint32_t nd, uint32_t ne, int
                 at = CV(tot_t);
             }
             predict_jvp_kernel(n, r.dt, r.box, pos, vel, CV(r.anp), a, pos_t, vel_t, CV(anp_t), at, xs_t.c, NULL);
-            density_jvp_kernel(g, pos, CV(r.xs[q][k]), CV(xs_t), m, m_t, rest, rest_t, rs_t);
+            density_jvp_kernel(g, 0, pos, CV(r.xs[q][k]), CV(xs_t), m, m_t, rest, rest_t, rs_t);
             projection_update_jvp_kernel(n, r.dt, r.G, m, m_t, volume_rest ? rest : r.rho,
                                          volume_rest ? rest_t : rho_t, q ? rest : r.rho, q ? rest_t : rho_t,
                                          r.p[q][k - 1u], r.rs[q][k], prm[q ? DF_DEN : DF_DIV], p_t, rs_t,
@@ -2409,7 +2447,7 @@ static void non_pressure_vjp_kernel(const grid_t* g, const real* const pos[3], c
 }
 
 /* density_vjp_kernel, density/density.cu:69-100 (cubic kernel) */
-static void density_vjp_kernel(const grid_t* g, const real* const topo[3], const real* const pos[3],
+static void density_vjp_kernel(const grid_t* g, int pbf, const real* const topo[3], const real* const pos[3],
                                const real* masses, const real* rest, const double* rho_a, double* const pos_a[3],
                                double* masses_a, double* rest_a) {
     const real h = (real)g->d->support_radius;
@@ -2426,15 +2464,15 @@ static void density_vjp_kernel(const grid_t* g, const real* const topo[3], const
             for (uint32_t s = cr[k].first; s < cr[k].last; ++s) {
                 const uint32_t j = g->idx[s];
                 const v3 d = v3_sub(x, v3_load(pos, j));
-                position = d3_add(position, d3_scale(d3_of(cubic_gradient(d, h)), out_a * masses[j] + rho_a[j] * masses[i]));
-                mass_c += rho_a[j] * cubic(d, h);
+                position = d3_add(position, d3_scale(d3_of(density_g(d, h, pbf)), out_a * masses[j] + rho_a[j] * masses[i]));
+                mass_c += rho_a[j] * density_w(d, h, pbf);
             }
             for (uint32_t s = cr[k].bfirst; s < cr[k].blast; ++s) {
                 const uint32_t b = g->bidx[s];
                 const v3 d = v3_sub(x, boundary_position(g, b));
                 const real vol = boundary_volume(g, b);
-                position = d3_add(position, d3_scale(d3_of(cubic_gradient(d, h)), out_a * rest[i] * vol));
-                rest_c += out_a * vol * cubic(d, h);
+                position = d3_add(position, d3_scale(d3_of(density_g(d, h, pbf)), out_a * rest[i] * vol));
+                rest_c += out_a * vol * density_w(d, h, pbf);
             }
         }
         d3_acc(pos_a, i, position);
@@ -2558,7 +2596,7 @@ void SUF(oracle_dfsph_vjp)(const oracle_domain* d, uint32_t nd, uint32_t ne, int
                                          r.p[q][k - 1u], r.rs[q][k], prm[q ? DF_DEN : DF_DIV], cur_p, m_a,
                                          volume_rest ? rest_a : rho_a, q ? rest_a : target_a, prev_p, rs_a,
                                          prm_a[q ? DF_DEN : DF_DIV]);
-            density_vjp_kernel(g, pos, CV(r.xs[q][k]), m, rest, rs_a, xs_a.c, m_a, rest_a);
+            density_vjp_kernel(g, 0, pos, CV(r.xs[q][k]), m, rest, rs_a, xs_a.c, m_a, rest_a);
             if (q == 0) {
                 predict_vjp_kernel(n, r.dt, r.box, pos, vel, CV(r.anp), CV(r.a[0][k - 1u]), DV(xs_a), NULL, pos_a,
                                    vel_a, anp_a.c, prev_ap.c);
@@ -2588,7 +2626,7 @@ void SUF(oracle_dfsph_vjp)(const oracle_domain* d, uint32_t nd, uint32_t ne, int
     non_pressure_vjp_kernel(g, pos, vel, m, prm[DF_VISC], r.rho, DV(anp_a), pos_a, vel_a, ctl_a, rho_a, m_a,
                             prm_a[DF_VISC]);
     /* sph.ixx:364: the state densities */
-    density_vjp_kernel(g, pos, pos, m, rest, rho_a, pos_a, m_a, rest_a);
+    density_vjp_kernel(g, 0, pos, pos, m, rest, rho_a, pos_a, m_a, rest_a);
     free(rho_a);
     free(target_a);
     free(cur_p);
@@ -2812,7 +2850,7 @@ void SUF(oracle_wcsph_jvp)(const oracle_domain* d, uint64_t step, const real* po
     vfield apt = alloc_vfield(n), avt = alloc_vfield(n), ast = alloc_vfield(n), tt = alloc_vfield(n),
            zero = alloc_vfield(n);
     for (int c = 0; c < 3; ++c) memset(zero.c[c], 0, sizeof(real) * n);
-    density_jvp_kernel(&g, pos, pos, pos_t, m, prm_t[WS_MASS], rest, prm_t[WS_REST], rho_t);
+    density_jvp_kernel(&g, 0, pos, pos, pos_t, m, prm_t[WS_MASS], rest, prm_t[WS_REST], rho_t);
     eos_jvp_kernel(n, rho, rho_t, rest, prm_t[WS_REST], prm[WS_SPEED], prm_t[WS_SPEED], prm[WS_EXP], prm_t[WS_EXP],
                    p_t);
     pressure_jvp_kernel(&g, pos, pos_t, m, prm_t[WS_MASS], rest, prm_t[WS_REST], rho, rho_t, p, p_t, apt.c);
@@ -3063,7 +3101,7 @@ void SUF(oracle_wcsph_vjp)(const oracle_domain* d, uint64_t step, const real* po
     eos_vjp_kernel(n, rho, rest, prm[WS_SPEED], prm[WS_EXP], p_a, rho_a, prm_a[WS_REST], prm_a[WS_SPEED],
                    prm_a[WS_EXP]);
     /* sph.ixx:364: the state densities */
-    density_vjp_kernel(&g, pos, pos, m, rest, rho_a, pos_a, prm_a[WS_MASS], prm_a[WS_REST]);
+    density_vjp_kernel(&g, 0, pos, pos, m, rest, rho_a, pos_a, prm_a[WS_MASS], prm_a[WS_REST]);
     free(rho);
     free(p);
     free(rho_a);

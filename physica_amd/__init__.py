@@ -21,6 +21,8 @@ from .particle import (  # noqa: F401
     VectorField,
     build_neighborhood,
     density_forward,
+    density_jvp,
+    density_vjp,
     neighborhood_workspace_size,
     write_vectors,
 )

@@ -324,6 +324,16 @@ PROTOTYPES = {
         [c_void_p, c_uint32, c_float, c_int, VectorView, VectorView, ParticleParameters, NeighborhoodView,
          BoundaryView, c_void_p],
     ),
+    "phx_density_jvp": (
+        c_int,
+        [c_void_p, c_uint32, c_float, c_int, VectorView, VectorView, VectorView, ParticleParameters,
+         ParticleParameterTangent, NeighborhoodView, BoundaryView, c_void_p],
+    ),
+    "phx_density_vjp": (
+        c_int,
+        [c_void_p, c_uint32, c_float, c_int, VectorView, VectorView, ParticleParameters, NeighborhoodView,
+         BoundaryView, c_void_p, AdjointVectorView, ParticleParameterAdjoint],
+    ),
     "phx_dfsph_create": (
         c_int,
         [POINTER(DomainConfiguration), POINTER(DfsphConfiguration), c_int, c_void_p, POINTER(c_void_p)],

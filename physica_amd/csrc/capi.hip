@@ -79,6 +79,45 @@ struct RefGrid {
     float ox, oy, oz, cell;
 };
 
+// The reference-layout walk of particle i (NeighborhoodView, the cell of
+// its topology position): per stencil cell (offset_x outer), the cell's
+// particles j in sorted order, then its boundary samples b at
+// x0 + t v (boundary_position, neighborhood/device.cuh:16-22).
+template <class Fluid, class Boundary>
+__device__ __forceinline__ void reference_walk(const RefGrid& g, const phx_boundary_view& boundary, phx::Float3 topo,
+                                               Fluid&& fluid, Boundary&& sample) {
+    using namespace phx;
+    const int cx = cell_clamp((topo.x - g.ox) / g.cell, g.cx);
+    const int cy = cell_clamp((topo.y - g.oy) / g.cell, g.cy);
+    const int cz = cell_clamp((topo.z - g.oz) / g.cell, g.cz);
+    for (int ox = -1; ox <= 1; ++ox)
+        for (int oy = -1; oy <= 1; ++oy)
+            for (int oz = -1; oz <= 1; ++oz) {
+                const int X = cx + ox, Y = cy + oy, Z = cz + oz;
+                if (X < 0 || X >= static_cast<int>(g.cx) || Y < 0 || Y >= static_cast<int>(g.cy) || Z < 0 ||
+                    Z >= static_cast<int>(g.cz))
+                    continue;
+                const uint32_t c = (static_cast<uint32_t>(Z) * g.cy + static_cast<uint32_t>(Y)) * g.cx +
+                                   static_cast<uint32_t>(X);
+                for (uint32_t s = g.offsets[c]; s < g.offsets[c + 1u]; ++s) fluid(g.indices[s]);
+                if (g.boundary_indices == nullptr) continue;
+                for (uint32_t s = g.boundary_offsets[c]; s < g.boundary_offsets[c + 1u]; ++s) {
+                    const uint32_t b = g.boundary_indices[s];
+                    sample(b, Float3{boundary.position_x[b] + boundary.time * boundary.velocity_x[b],
+                                     boundary.position_y[b] + boundary.time * boundary.velocity_y[b],
+                                     boundary.position_z[b] + boundary.time * boundary.velocity_z[b]});
+                }
+            }
+}
+
+// density_kernel / density_kernel_gradient (density.cu:9-16)
+__device__ __forceinline__ float density_w(phx::Float3 d, float h, int pbf) {
+    return pbf != 0 ? phx::poly6(d, h) : phx::cubic(d, h);
+}
+__device__ __forceinline__ phx::Float3 density_g(phx::Float3 d, float h, int pbf) {
+    return pbf != 0 ? phx::poly6_gradient(d, h) : phx::cubic_gradient(d, h);
+}
+
 // density_forward_kernel (density/density.cu:18-40) in original particle order
 // over a reference-layout neighbourhood, each cell's boundary samples after
 // its particles.
@@ -89,36 +128,90 @@ __global__ void __launch_bounds__(kBlock)
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const Float3 p = load(pos, i);
-    const Float3 t = load(topo, i);
-    const int cx = cell_clamp((t.x - g.ox) / g.cell, g.cx);
-    const int cy = cell_clamp((t.y - g.oy) / g.cell, g.cy);
-    const int cz = cell_clamp((t.z - g.oz) / g.cell, g.cz);
     float result = 0.0F;
-    for (int ox = -1; ox <= 1; ++ox)
-        for (int oy = -1; oy <= 1; ++oy)
-            for (int oz = -1; oz <= 1; ++oz) {
-                const int X = cx + ox, Y = cy + oy, Z = cz + oz;
-                if (X < 0 || X >= static_cast<int>(g.cx) || Y < 0 || Y >= static_cast<int>(g.cy) || Z < 0 ||
-                    Z >= static_cast<int>(g.cz))
-                    continue;
-                const uint32_t c = (static_cast<uint32_t>(Z) * g.cy + static_cast<uint32_t>(Y)) * g.cx +
-                                   static_cast<uint32_t>(X);
-                for (uint32_t s = g.offsets[c]; s < g.offsets[c + 1u]; ++s) {
-                    const uint32_t j = g.indices[s];
-                    const Float3 d = sub(p, load(pos, j));
-                    result += masses[j] * (pbf_kernel != 0 ? poly6(d, h) : cubic(d, h));
-                }
-                if (g.boundary_indices == nullptr) continue;
-                for (uint32_t s = g.boundary_offsets[c]; s < g.boundary_offsets[c + 1u]; ++s) {
-                    const uint32_t b = g.boundary_indices[s];
-                    const Float3 xb{boundary.position_x[b] + boundary.time * boundary.velocity_x[b],
-                                    boundary.position_y[b] + boundary.time * boundary.velocity_y[b],
-                                    boundary.position_z[b] + boundary.time * boundary.velocity_z[b]};
-                    const Float3 d = sub(p, xb);
-                    result += rest_densities[i] * boundary.volumes[b] * (pbf_kernel != 0 ? poly6(d, h) : cubic(d, h));
-                }
-            }
+    reference_walk(
+        g, boundary, load(topo, i), [&](uint32_t j) { result += masses[j] * density_w(sub(p, load(pos, j)), h, pbf_kernel); },
+        [&](uint32_t b, Float3 xb) {
+            result += rest_densities[i] * boundary.volumes[b] * density_w(sub(p, xb), h, pbf_kernel);
+        });
     out[i] = result;
+}
+
+// density_jvp_kernel (density/density.cu:42-63); null tangent arrays count
+// as zero.
+__global__ void __launch_bounds__(kBlock)
+    reference_density_jvp_kernel(uint32_t n, float h, int pbf_kernel, phx::CVec topo, phx::CVec pos,
+                                 phx::CVec pos_t, const float* masses, const float* mass_t,
+                                 const float* rest_densities, const float* rest_t, phx_boundary_view boundary,
+                                 RefGrid g, float* out) {
+    using namespace phx;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const Float3 p = load(pos, i);
+    const Float3 t = load(pos_t, i);
+    const float rest_i = rest_densities != nullptr ? rest_densities[i] : 0.0F;
+    const float rest_t_i = rest_t != nullptr ? rest_t[i] : 0.0F;
+    float result = 0.0F;
+    reference_walk(
+        g, boundary, load(topo, i),
+        [&](uint32_t j) {
+            const Float3 d = sub(p, load(pos, j));
+            const float mt = mass_t != nullptr ? mass_t[j] : 0.0F;
+            result += mt * density_w(d, h, pbf_kernel) + masses[j] * dot(density_g(d, h, pbf_kernel), sub(t, load(pos_t, j)));
+        },
+        [&](uint32_t b, Float3 xb) {
+            const Float3 d = sub(p, xb);
+            result += boundary.volumes[b] *
+                      (rest_t_i * density_w(d, h, pbf_kernel) + rest_i * dot(density_g(d, h, pbf_kernel), t));
+        });
+    out[i] = result;
+}
+
+// density_vjp_kernel (density/density.cu:65-98) in FP64, gather form: each
+// particle collects the pair terms of its own density and of its
+// neighbours' and accumulates (+=) into its outputs; null outputs are
+// skipped.
+__global__ void __launch_bounds__(kBlock)
+    reference_density_vjp_kernel(uint32_t n, float h, int pbf_kernel, phx::CVec topo, phx::CVec pos,
+                                 const float* masses, const float* rest_densities, phx_boundary_view boundary,
+                                 RefGrid g, const double* rho_a, phx_adjoint_vector_view pos_a, double* mass_a,
+                                 double* rest_a) {
+    using namespace phx;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const Float3 p = load(pos, i);
+    const double out_a = rho_a[i];
+    const float m_i = masses[i];
+    const float rest_i = rest_densities != nullptr ? rest_densities[i] : 0.0F;
+    double px = 0.0, py = 0.0, pz = 0.0, mass_c = 0.0, rest_c = 0.0;
+    reference_walk(
+        g, boundary, load(topo, i),
+        [&](uint32_t j) {
+            const Float3 d = sub(p, load(pos, j));
+            const Float3 gr = density_g(d, h, pbf_kernel);
+            const double weight = out_a * static_cast<double>(masses[j]) + rho_a[j] * static_cast<double>(m_i);
+            px += static_cast<double>(gr.x) * weight;
+            py += static_cast<double>(gr.y) * weight;
+            pz += static_cast<double>(gr.z) * weight;
+            mass_c += rho_a[j] * static_cast<double>(density_w(d, h, pbf_kernel));
+        },
+        [&](uint32_t b, Float3 xb) {
+            const Float3 d = sub(p, xb);
+            const double volume = static_cast<double>(boundary.volumes[b]);
+            const Float3 gr = density_g(d, h, pbf_kernel);
+            const double weight = out_a * static_cast<double>(rest_i) * volume;
+            px += static_cast<double>(gr.x) * weight;
+            py += static_cast<double>(gr.y) * weight;
+            pz += static_cast<double>(gr.z) * weight;
+            rest_c += out_a * volume * static_cast<double>(density_w(d, h, pbf_kernel));
+        });
+    if (pos_a.x != nullptr) {
+        pos_a.x[i] += px;
+        pos_a.y[i] += py;
+        pos_a.z[i] += pz;
+    }
+    if (mass_a != nullptr) mass_a[i] += mass_c;
+    if (rest_a != nullptr) rest_a[i] += rest_c;
 }
 
 // accumulate_scalar_kernel (domain/domain.cu:16-20): one element per lane.
@@ -281,6 +374,44 @@ phx_status phx_build_neighborhood(phx_stream stream, const phx_domain_configurat
     });
 }
 
+namespace {
+// Checks of a reference-layout neighbourhood and boundary view shared by the
+// density entry points; returns the device-side grid view.
+RefGrid checked_grid(uint32_t particle_count, const phx_neighborhood_view& neighborhood,
+                     const phx_boundary_view& boundary, const float* rest_densities) {
+    phx::require(neighborhood.sorted_particle_indices != nullptr && neighborhood.cell_offsets != nullptr,
+                 PHX_ERROR_INVALID_ARGUMENT, "null neighbourhood arrays");
+    phx::require(neighborhood.boundary_count == boundary.count, PHX_ERROR_INVALID_ARGUMENT,
+                 "boundary count does not match the neighbourhood's");
+    if (boundary.count > 0u) {
+        phx::require(neighborhood.sorted_boundary_indices != nullptr && neighborhood.boundary_cell_offsets != nullptr,
+                     PHX_ERROR_INVALID_ARGUMENT, "null neighbourhood boundary arrays");
+        phx::require(boundary.position_x != nullptr && boundary.position_y != nullptr &&
+                         boundary.position_z != nullptr && boundary.velocity_x != nullptr &&
+                         boundary.velocity_y != nullptr && boundary.velocity_z != nullptr &&
+                         boundary.volumes != nullptr && rest_densities != nullptr,
+                     PHX_ERROR_INVALID_ARGUMENT, "null boundary arrays or rest densities");
+    }
+    phx::require(neighborhood.particle_count == particle_count, PHX_ERROR_INVALID_ARGUMENT,
+                 "neighbourhood particle_count does not match");
+    phx::require(neighborhood.cells_x > 0 && neighborhood.cells_y > 0 && neighborhood.cells_z > 0 &&
+                     neighborhood.cell_size > 0.0F,
+                 PHX_ERROR_INVALID_ARGUMENT, "empty neighbourhood grid");
+    const bool with_boundary = boundary.count > 0u;
+    return RefGrid{neighborhood.sorted_particle_indices,
+                   neighborhood.cell_offsets,
+                   with_boundary ? neighborhood.sorted_boundary_indices : nullptr,
+                   with_boundary ? neighborhood.boundary_cell_offsets : nullptr,
+                   neighborhood.cells_x,
+                   neighborhood.cells_y,
+                   neighborhood.cells_z,
+                   neighborhood.origin_x,
+                   neighborhood.origin_y,
+                   neighborhood.origin_z,
+                   neighborhood.cell_size};
+}
+}  // namespace
+
 phx_status phx_density_forward(phx_stream stream, uint32_t particle_count, float support_radius, int pbf_kernel,
                                phx_const_vector_view topology_positions, phx_const_vector_view positions,
                                phx_particle_parameters parameters, phx_neighborhood_view neighborhood,
@@ -291,39 +422,54 @@ phx_status phx_density_forward(phx_stream stream, uint32_t particle_count, float
         phx::require_vec(positions, "positions");
         phx::require(parameters.masses != nullptr && densities != nullptr, PHX_ERROR_INVALID_ARGUMENT,
                      "null masses or densities");
-        phx::require(neighborhood.sorted_particle_indices != nullptr && neighborhood.cell_offsets != nullptr,
-                     PHX_ERROR_INVALID_ARGUMENT, "null neighbourhood arrays");
-        phx::require(neighborhood.boundary_count == boundary.count, PHX_ERROR_INVALID_ARGUMENT,
-                     "boundary count does not match the neighbourhood's");
-        if (boundary.count > 0u) {
-            phx::require(neighborhood.sorted_boundary_indices != nullptr && neighborhood.boundary_cell_offsets != nullptr,
-                         PHX_ERROR_INVALID_ARGUMENT, "null neighbourhood boundary arrays");
-            phx::require(boundary.position_x != nullptr && boundary.position_y != nullptr &&
-                             boundary.position_z != nullptr && boundary.velocity_x != nullptr &&
-                             boundary.velocity_y != nullptr && boundary.velocity_z != nullptr &&
-                             boundary.volumes != nullptr && parameters.rest_densities != nullptr,
-                         PHX_ERROR_INVALID_ARGUMENT, "null boundary arrays or rest densities");
-        }
-        phx::require(neighborhood.particle_count == particle_count, PHX_ERROR_INVALID_ARGUMENT,
-                     "neighbourhood particle_count does not match");
-        phx::require(neighborhood.cells_x > 0 && neighborhood.cells_y > 0 && neighborhood.cells_z > 0 &&
-                         neighborhood.cell_size > 0.0F,
-                     PHX_ERROR_INVALID_ARGUMENT, "empty neighbourhood grid");
-        const bool with_boundary = boundary.count > 0u;
-        RefGrid g{neighborhood.sorted_particle_indices,
-                  neighborhood.cell_offsets,
-                  with_boundary ? neighborhood.sorted_boundary_indices : nullptr,
-                  with_boundary ? neighborhood.boundary_cell_offsets : nullptr,
-                  neighborhood.cells_x,
-                  neighborhood.cells_y,
-                  neighborhood.cells_z,
-                  neighborhood.origin_x,
-                  neighborhood.origin_y,
-                  neighborhood.origin_z,
-                  neighborhood.cell_size};
+        const RefGrid g = checked_grid(particle_count, neighborhood, boundary, parameters.rest_densities);
         reference_density_kernel<<<phx::blocks_for(particle_count, kBlock), kBlock, 0, stream_of(stream)>>>(
             particle_count, support_radius, pbf_kernel, phx::cvec(topology_positions), phx::cvec(positions),
             parameters.masses, parameters.rest_densities, boundary, g, densities);
+        PHX_LAUNCH_CHECK();
+    });
+}
+
+phx_status phx_density_jvp(phx_stream stream, uint32_t particle_count, float support_radius, int pbf_kernel,
+                           phx_const_vector_view topology_positions, phx_const_vector_view positions,
+                           phx_const_vector_view position_tangent, phx_particle_parameters parameters,
+                           phx_particle_parameter_tangent parameter_tangent, phx_neighborhood_view neighborhood,
+                           phx_boundary_view boundary, float* density_tangent) {
+    return guarded([&] {
+        if (particle_count == 0) return;
+        phx::require_vec(topology_positions, "topology positions");
+        phx::require_vec(positions, "positions");
+        phx::require_vec(position_tangent, "position tangent");
+        phx::require(parameters.masses != nullptr && density_tangent != nullptr, PHX_ERROR_INVALID_ARGUMENT,
+                     "null masses or density tangent");
+        const RefGrid g = checked_grid(particle_count, neighborhood, boundary, parameters.rest_densities);
+        reference_density_jvp_kernel<<<phx::blocks_for(particle_count, kBlock), kBlock, 0, stream_of(stream)>>>(
+            particle_count, support_radius, pbf_kernel, phx::cvec(topology_positions), phx::cvec(positions),
+            phx::cvec(position_tangent), parameters.masses, parameter_tangent.masses, parameters.rest_densities,
+            parameter_tangent.rest_densities, boundary, g, density_tangent);
+        PHX_LAUNCH_CHECK();
+    });
+}
+
+phx_status phx_density_vjp(phx_stream stream, uint32_t particle_count, float support_radius, int pbf_kernel,
+                           phx_const_vector_view topology_positions, phx_const_vector_view positions,
+                           phx_particle_parameters parameters, phx_neighborhood_view neighborhood,
+                           phx_boundary_view boundary, const double* density_adjoint,
+                           phx_adjoint_vector_view position_adjoint, phx_particle_parameter_adjoint parameter_adjoint) {
+    return guarded([&] {
+        if (particle_count == 0) return;
+        phx::require_vec(topology_positions, "topology positions");
+        phx::require_vec(positions, "positions");
+        phx::require(parameters.masses != nullptr && density_adjoint != nullptr, PHX_ERROR_INVALID_ARGUMENT,
+                     "null masses or density adjoint");
+        phx::require((position_adjoint.x == nullptr) == (position_adjoint.y == nullptr) &&
+                         (position_adjoint.x == nullptr) == (position_adjoint.z == nullptr),
+                     PHX_ERROR_INVALID_ARGUMENT, "position adjoint must have all three or no components");
+        const RefGrid g = checked_grid(particle_count, neighborhood, boundary, parameters.rest_densities);
+        reference_density_vjp_kernel<<<phx::blocks_for(particle_count, kBlock), kBlock, 0, stream_of(stream)>>>(
+            particle_count, support_radius, pbf_kernel, phx::cvec(topology_positions), phx::cvec(positions),
+            parameters.masses, parameters.rest_densities, boundary, g, density_adjoint, position_adjoint,
+            parameter_adjoint.masses, parameter_adjoint.rest_densities);
         PHX_LAUNCH_CHECK();
     });
 }

@@ -1146,6 +1146,50 @@ def neighborhood_workspace_size(configuration: DomainConfiguration) -> int:
     return int(size.value)
 
 
+def density_jvp(
+    support_radius: float,
+    pbf_kernel: bool,
+    topology: VectorField,
+    positions: VectorField,
+    position_tangent: VectorField,
+    parameters: ParticleParameters,
+    parameter_tangent: ParticleParameterTangent,
+    view: _capi.NeighborhoodView,
+    density_tangent: torch.Tensor,
+    boundary: Optional[DeviceBoundary] = None,
+    boundary_time: float = 0.0,
+) -> None:
+    """density::sph_jvp / pbf_jvp (density/density.cpp, density.cu:42-63):
+    overwrites `density_tangent` (FP32)."""
+    dev = density_tangent.device
+    bview = boundary.view(boundary_time) if boundary is not None else _capi.BoundaryView()
+    _capi.call("phx_density_jvp", _stream_handle(dev), int(view.particle_count), float(support_radius),
+               1 if pbf_kernel else 0, topology.view(), positions.view(), position_tangent.view(), parameters.to_c(),
+               parameter_tangent.to_c(), view, bview, _ptr(density_tangent))
+
+
+def density_vjp(
+    support_radius: float,
+    pbf_kernel: bool,
+    topology: VectorField,
+    positions: VectorField,
+    parameters: ParticleParameters,
+    view: _capi.NeighborhoodView,
+    density_adjoint: torch.Tensor,
+    position_adjoint: VectorAdjointField,
+    parameter_adjoint: ParticleParameterAdjoint,
+    boundary: Optional[DeviceBoundary] = None,
+    boundary_time: float = 0.0,
+) -> None:
+    """density::sph_vjp / pbf_vjp (density.cu:65-98): accumulates (+=) the
+    FP64 adjoints of positions, masses and rest densities."""
+    dev = density_adjoint.device
+    bview = boundary.view(boundary_time) if boundary is not None else _capi.BoundaryView()
+    _capi.call("phx_density_vjp", _stream_handle(dev), int(view.particle_count), float(support_radius),
+               1 if pbf_kernel else 0, topology.view(), positions.view(), parameters.to_c(), view, bview,
+               _ptr(density_adjoint), position_adjoint.view(), parameter_adjoint.to_c())
+
+
 def write_vectors(positions: VectorField, velocities: VectorField) -> Tuple[torch.Tensor, torch.Tensor]:
     """spectra_cuda::write_vectors (examples/fluids/liquid/pbf-dam-break/
     spectra/kernels.cu:8-18): interleaved (N, 3) float32 copies of the SoA

@@ -7,7 +7,8 @@
 //     "<name> dot <left> <right>" for <J t, w> and <t, J^T w> (FP64 sums);
 //   - SPH<DFSPH> in tolerance mode, printing the executed iterations;
 //   - NeighborhoodSearch + density::sph_forward with boundary samples
-//     (densities written to <out>/density.bin);
+//     (densities written to <out>/density.bin), density::sph_jvp (tangent
+//     written to <out>/density_tangent.bin) and sph_vjp ("density dot");
 //   - Domain::accumulate (prints "accumulate ok").
 // Usage: api_coverage <out_dir>
 #include <cmath>
@@ -345,6 +346,32 @@ int main(int argc, char** argv) {
             const std::vector<float> h = download(rho.values);
             std::ofstream f(out + "/density.bin", std::ios::binary);
             f.write(reinterpret_cast<const char*>(h.data()), static_cast<std::streamsize>(h.size() * sizeof(float)));
+
+            // density::sph_jvp with a seeded position tangent, and sph_vjp:
+            // <J t, w> = <t, J^T w> (the mass and rest-density tangents are 0)
+            Random r{31};
+            pp::VectorField t = d.allocate_vector_field(kCount);
+            random_fill(d, t, r, 1.0F);
+            pp::ParticleParameterTangent pt = d.allocate_particle_parameter_tangent();
+            pp::ScalarField rho_t = d.allocate_scalar_field(kCount);
+            pp::density::sph_jvp(d, x, x, t, p, pt, nb, rho_t, &boundary);
+            pp::ScalarAdjointField w = d.allocate_scalar_adjoint_field(kCount);
+            std::vector<double> hw(kCount);
+            for (auto& e : hw) e = r.next();
+            physica_amd::check_hip(hipMemcpy(w.values.data(), hw.data(), kCount * sizeof(double), hipMemcpyHostToDevice),
+                                   "hipMemcpy");
+            pp::VectorAdjointField xa = d.allocate_vector_adjoint_field(kCount);
+            d.clear(xa);
+            pp::ParticleParameterAdjoint pa = d.allocate_particle_parameter_adjoint();
+            pp::density::sph_vjp(d, x, x, p, nb, w, xa, pa, &boundary);
+            physica_amd::check_hip(hipDeviceSynchronize(), "hipDeviceSynchronize");
+            const std::vector<float> ht = download(rho_t.values);
+            double left = 0.0;
+            for (std::uint32_t k = 0; k < kCount; ++k) left += static_cast<double>(ht[k]) * hw[k];
+            std::printf("density dot %.17g %.17g\n", left, dot(t, xa));
+            std::ofstream ft(out + "/density_tangent.bin", std::ios::binary);
+            ft.write(reinterpret_cast<const char*>(ht.data()), static_cast<std::streamsize>(ht.size() * sizeof(float)));
+            write_positions(out + "/density_position_tangent.bin", t);
         }
 
         {  // Domain::accumulate

@@ -140,4 +140,10 @@ def test_api_coverage_matches_oracle(tmp_path, gpu_device):
     free = oracle.density(oracle.make_domain(cfg), 3, False, x0, x0, w["masses"], w["rest_densities"])
     assert np.sum(expected != free) > 10
     assert bits_equal(rho, expected)
+    od = oracle.make_domain(cfg, (bpos, bvel, bvol))
+    tan = np.fromfile(tmp_path / "density_position_tangent.bin", dtype=np.float32).reshape(3, n)
+    got_t = np.fromfile(tmp_path / "density_tangent.bin", dtype=np.float32)
+    assert bits_equal(got_t, oracle.density_jvp(od, 3, False, x0, x0, tan, w["masses"], z, w["rest_densities"], z))
+    _, left, right = lines["density"].split()
+    assert abs(float(left) - float(right)) <= 1e-5 * (abs(float(left)) + abs(float(right)))
     assert lines["accumulate"] == "ok"

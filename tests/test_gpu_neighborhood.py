@@ -159,3 +159,50 @@ def test_reference_layout_density_with_boundary_bit_exact(gpu_device, pbf_kernel
     without = oracle.density(oracle.make_domain(cfg), step, pbf_kernel, topo, pos, m, r0)
     assert np.sum(expected != without) > 1000  # the boundary terms are exercised
     assert bits_equal(rho.cpu().numpy(), expected)
+
+
+@pytest.mark.parametrize("pbf_kernel", [False, True])
+def test_reference_layout_density_jvp_vjp_bit_exact(gpu_device, pbf_kernel):
+    """density::{sph,pbf}_{jvp,vjp} in the reference layout with moving
+    boundary samples: the FP32 tangent and the FP64 adjoints (accumulated
+    onto non-zero outputs) equal the oracle's bit for bit."""
+    from physica_amd import density_jvp, density_vjp
+    from physica_amd.particle import ParticleParameterAdjoint, ParticleParameterTangent, VectorAdjointField
+    n = 20000
+    rng = np.random.default_rng(23 + pbf_kernel)
+    samples = boundary_samples(rng, 3000, moving=True)
+    cfg = with_boundary(domain(n, dt=0.02), samples)
+    x = positions("uniform", n, rng, (1.0, 0.6, 0.8))
+    xt = rng.normal(size=(3, n)).astype(np.float32)
+    m = rng.uniform(0.5, 1.5, size=n).astype(np.float32)
+    r0 = rng.uniform(900.0, 1100.0, size=n).astype(np.float32)
+    mt = rng.normal(size=n).astype(np.float32)
+    rt = rng.normal(size=n).astype(np.float32)
+    w = rng.normal(size=n)
+    xf, tf = VectorField.zeros(n, gpu_device), VectorField.zeros(n, gpu_device)
+    xf.upload(x)
+    tf.upload(xt)
+    z = torch.zeros(n, device=gpu_device)
+    dev = lambda a, dt=torch.float32: torch.from_numpy(np.ascontiguousarray(a)).to(device=gpu_device, dtype=dt)
+    params = ParticleParameters(dev(m), dev(r0), z, z)
+    step = 3
+    nb = build_neighborhood(cfg, step, xf, gpu_device)
+    boundary = DeviceBoundary(cfg, gpu_device)
+    rho_t = torch.empty(n, device=gpu_device)
+    density_jvp(cfg.support_radius, pbf_kernel, xf, xf, tf, params, ParticleParameterTangent(dev(mt), dev(rt), z, z),
+                nb.view, rho_t, boundary, nb.boundary_time)
+    start = rng.normal(size=(5, n))
+    pos_a = VectorAdjointField(*(dev(start[k], torch.float64) for k in range(3)))
+    zd = torch.zeros(n, device=gpu_device, dtype=torch.float64)
+    par_a = ParticleParameterAdjoint(dev(start[3], torch.float64), dev(start[4], torch.float64), zd, zd)
+    density_vjp(cfg.support_radius, pbf_kernel, xf, xf, params, nb.view, dev(w, torch.float64), pos_a, par_a,
+                boundary, nb.boundary_time)
+    torch.cuda.synchronize()
+    od = oracle.make_domain(cfg, samples)
+    exp_t = oracle.density_jvp(od, step, pbf_kernel, x, x, xt, m, mt, r0, rt)
+    assert bits_equal(rho_t.cpu().numpy(), exp_t)
+    ea, em, er = oracle.density_vjp(od, step, pbf_kernel, x, x, m, r0, w)
+    assert np.abs(er).max() > 0
+    assert bits_equal(pos_a.numpy(), start[:3] + ea)
+    assert bits_equal(par_a.masses.cpu().numpy(), start[3] + em)
+    assert bits_equal(par_a.rest_densities.cpu().numpy(), start[4] + er)
