@@ -1,0 +1,100 @@
+"""State dump format for viewers and restarts (the caller-side counterpart
+of the reference example's Spectra provider, which publishes positions,
+velocities and per-particle fields each frame:
+examples/fluids/liquid/pbf-dam-break/spectra/provider.ixx:74-96).
+
+One `.npz` file per state, written and read without pickling:
+
+- ``positions``, ``velocities``: float32 [3, N] in original particle order;
+- ``step_index``: uint64 scalar;
+- ``method/<name>``: the method state's per-particle fields (DFSPH's warm
+  pressures), float32 [N];
+- ``field/<name>``: optional extra per-particle fields (e.g. PBF vorticity
+  magnitudes), float32 [N] or [3, N];
+- ``meta``: a JSON string with the format version and, if given, the
+  domain configuration (counts, time step, radii, gravity, box).
+"""
+from __future__ import annotations
+
+import dataclasses
+import json
+from pathlib import Path
+from typing import Dict, Optional
+
+import numpy as np
+import torch
+
+FORMAT = "physica_amd.state/1"
+
+
+def _vector(field) -> np.ndarray:
+    return np.stack([field.x.detach().cpu().numpy(), field.y.detach().cpu().numpy(),
+                     field.z.detach().cpu().numpy()]).astype(np.float32)
+
+
+def _method_fields(state) -> Dict[str, torch.Tensor]:
+    method = getattr(state, "method", None)
+    if method is None or not dataclasses.is_dataclass(method):
+        return {}
+    return {f.name: getattr(method, f.name) for f in dataclasses.fields(method)
+            if isinstance(getattr(method, f.name), torch.Tensor)}
+
+
+def _configuration_meta(configuration) -> dict:
+    box = configuration.boundary
+    return {"particle_count": int(configuration.particle_count), "time_step": float(configuration.time_step),
+            "support_radius": float(configuration.support_radius),
+            "particle_radius": float(configuration.particle_radius),
+            "gravity": [float(g) for g in configuration.gravity],
+            "box_minimum": [float(v) for v in box.minimum], "box_maximum": [float(v) for v in box.maximum],
+            "box_velocity": [float(v) for v in box.velocity], "no_slip": bool(box.no_slip),
+            "boundary_particle_count": len(configuration.boundary_particles)}
+
+
+def save_state(path, state, configuration=None, fields: Optional[Dict[str, torch.Tensor]] = None) -> None:
+    """Writes `state` (SPH.State or PBF.State) and optional extra fields."""
+    particles = state.particles
+    arrays = {"positions": _vector(particles.positions), "velocities": _vector(particles.velocities),
+              "step_index": np.uint64(particles.step_index)}
+    for name, tensor in _method_fields(state).items():
+        arrays["method/" + name] = tensor.detach().cpu().numpy().astype(np.float32)
+    for name, value in (fields or {}).items():
+        arrays["field/" + name] = (_vector(value) if hasattr(value, "x") else
+                                   value.detach().cpu().numpy().astype(np.float32))
+    meta = {"format": FORMAT}
+    if configuration is not None:
+        meta["domain"] = _configuration_meta(configuration)
+    arrays["meta"] = np.array(json.dumps(meta))
+    with open(Path(path), "wb") as f:
+        np.savez(f, **arrays)
+
+
+def read_state(path) -> Dict[str, np.ndarray]:
+    """The arrays of a state file (no pickled objects are accepted)."""
+    with np.load(Path(path), allow_pickle=False) as data:
+        out = {k: data[k] for k in data.files}
+    meta = json.loads(str(out["meta"]))
+    if meta.get("format") != FORMAT:
+        raise ValueError(f"not a {FORMAT} file")
+    out["meta"] = meta
+    return out
+
+
+def load_state(path, state) -> dict:
+    """Fills an allocated state (positions, velocities, step index, method
+    fields) from a file; returns the file's metadata."""
+    data = read_state(path)
+    particles = state.particles
+    n = particles.positions.x.numel()
+    if data["positions"].shape != (3, n):
+        raise ValueError(f"state file holds {data['positions'].shape[1]} particles, the state {n}")
+    for field, key in ((particles.positions, "positions"), (particles.velocities, "velocities")):
+        for axis, name in enumerate("xyz"):
+            getattr(field, name).copy_(torch.from_numpy(np.ascontiguousarray(data[key][axis])))
+    particles.step_index = int(data["step_index"])
+    for name, tensor in _method_fields(state).items():
+        key = "method/" + name
+        if key not in data:
+            raise ValueError(f"state file has no {key}")
+        tensor.copy_(torch.from_numpy(np.ascontiguousarray(data[key])))
+    return data["meta"]
