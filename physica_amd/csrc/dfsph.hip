@@ -113,8 +113,6 @@ DfsphSolver::DfsphSolver(const phx_domain_configuration* domain, const phx_dfsph
             PHX_ERROR_INVALID_ARGUMENT, "tolerance mode needs 1 <= max_iterations < 2048");
     require(config_.divergence_iterations + config_.density_iterations < 4096u, PHX_ERROR_INVALID_ARGUMENT,
             "too many pressure iterations");
-    require(domain_.config.boundary_particle_count == 0u || mode == PHX_MODE_FORWARD, PHX_ERROR_UNSUPPORTED,
-            "boundary particles are supported in forward mode only");
     boundary_.upload(domain_.config);
     domain_.config.boundary_particles = nullptr;  // the caller's host array is not kept
     gradient_norm_ = reference_gradient_norm(domain_.config);

@@ -68,11 +68,27 @@ struct Step {
     const uint16_t* skin;
     const uint32_t* flags;
     float limit2;
+    BoundaryGrid boundary;  // offsets null: no boundary samples
+    const float* rest;      // rho0 (the boundary terms)
 };
 
 __device__ __forceinline__ bool exact_masked(const Step& s) { return s.flags[kFlagOverflow] == 0u; }
 __device__ __forceinline__ MaskView exact_view(const Step& s) { return MaskView{s.exact, s.flags, s.n, s.limit2}; }
 __device__ __forceinline__ MaskView skin_view(const Step& s) { return MaskView{s.skin, s.flags, s.n, s.limit2}; }
+
+// Visits the fluid candidates of particle i (through the masks where they
+// apply) or, with boundary samples, every fluid particle and every sample of
+// the 27 cells in the reference order: each cell's fluid, then its samples.
+template <class Visit, class VisitBoundary>
+__device__ __forceinline__ void walk(const Step& s, const MaskView& mv, bool masked, uint32_t i, int cx, int cy,
+                                     int cz, const int* delta, uint16_t* column, Visit&& visit,
+                                     VisitBoundary&& visit_boundary) {
+    if (s.boundary.offsets != nullptr) {
+        for_each_candidate(s.grid, s.boundary, cx, cy, cz, visit, visit_boundary);
+    } else {
+        visit_candidates(s.grid, mv, masked, i, cx, cy, cz, delta, column, kBlock, visit);
+    }
+}
 
 // ----------------------------------------------------------------- JVP
 struct GatherTangentArgs {
@@ -116,6 +132,7 @@ struct DensityJvpArgs {
     float* rho_t;
     float2* t_div;
     float2* t_den;
+    const float* rest_t;  // rest-density tangents (the boundary terms)
 };
 
 // density_jvp_kernel (density.cu:42-67, topology = positions = x) and the
@@ -130,15 +147,25 @@ __global__ void __launch_bounds__(kBlock) dfsph_jvp_density_kernel(DensityJvpArg
     int cx, cy, cz;
     particle_cell(s.grid, p, cx, cy, cz);
     float result = 0.0F;
-    visit_candidates(s.grid, exact_view(s), exact_masked(s), i, cx, cy, cz, delta, smask + threadIdx.x, kBlock,
-                     [&](uint32_t j) {
-                         const float4 q = s.xm[j];
-                         const float4 qt = a.xt[j];
-                         const Float3 d = sub(p, xyz(q));
-                         const float dist = length(d);
-                         result += qt.w * cubic_value(dist, s.kc) +
-                                   q.w * dot(scale(d, cubic_gradient_factor(dist, s.kc)), sub(pt, xyz(qt)));
-                     });
+    const float rest_i = s.rest[i];
+    const float rest_t_i = a.rest_t[i];
+    walk(
+        s, exact_view(s), exact_masked(s), i, cx, cy, cz, delta, smask + threadIdx.x,
+        [&](uint32_t j) {
+            const float4 q = s.xm[j];
+            const float4 qt = a.xt[j];
+            const Float3 d = sub(p, xyz(q));
+            const float dist = length(d);
+            result += qt.w * cubic_value(dist, s.kc) +
+                      q.w * dot(scale(d, cubic_gradient_factor(dist, s.kc)), sub(pt, xyz(qt)));
+        },
+        [&](uint32_t b) {
+            const float4 q = s.boundary.x[b];
+            const Float3 d = sub(p, xyz(q));
+            const float dist = length(d);
+            result += q.w * (rest_t_i * cubic_value(dist, s.kc) +
+                             rest_i * dot(scale(d, cubic_gradient_factor(dist, s.kc)), pt));
+        });
     a.rho_t[i] = result;
     const float rho = s.rho[i];
     a.t_div[i] = first_term(a.p_div[i], a.p_t_div[i], rho, result);
@@ -196,9 +223,17 @@ __global__ void __launch_bounds__(kBlock) dfsph_jvp_force_kernel(ForceJvpArgs a)
     if constexpr (kTwo) t2_i = a.t2[i];
     Float3 ap1{0.0F, 0.0F, 0.0F};
     Float3 ap2{0.0F, 0.0F, 0.0F};
+    const float rest_i = s.rest[i];
+    const float rest_t_i = a.ut[i].w;
     int cx, cy, cz;
     particle_cell(s.grid, x, cx, cy, cz);
-    visit_candidates(s.grid, exact_view(s), exact_masked(s), i, cx, cy, cz, delta, smask + threadIdx.x, kBlock,
+    // the pressure terms of sample b (pressure_jvp_kernel's boundary half)
+    auto boundary_pressure = [&](const Float3& gradient, const Float3& gradient_t, float vol, float2 t, Float3& ap) {
+        const float factor = -rest_i * vol * t.x;
+        const float factor_t = -vol * (rest_t_i * t.x + rest_i * t.y);
+        ap = add(ap, add(scale(gradient, factor_t), scale(gradient_t, factor)));
+    };
+    walk(s, exact_view(s), exact_masked(s), i, cx, cy, cz, delta, smask + threadIdx.x,
                      [&](uint32_t j) {
                          if (j == i) return;
                          const float4 q = s.xm[j];
@@ -233,6 +268,23 @@ __global__ void __launch_bounds__(kBlock) dfsph_jvp_force_kernel(ForceJvpArgs a)
                              const float f2_t = -qt.w * (t2_i.x + t2_j.x) - mass * (t2_i.y + t2_j.y);
                              ap2 = add(ap2, add(scale(gradient, f2_t), scale(gradient_t, f2)));
                          }
+                     },
+                     [&](uint32_t b) {
+                         const float4 q = s.boundary.x[b];
+                         const float vol = q.w;
+                         const Float3 d = sub(x, xyz(q));
+                         if constexpr (kNonPressure) {
+                             // non_pressure_jvp_kernel's boundary half: mu_i V lap (v_b - v_i)
+                             const float lap = viscosity_laplacian_c(d, s.kc);
+                             const float w = vm.w * vol * lap;
+                             const float w_t = vol * (vt_i.w * lap + vm.w * viscosity_laplacian_tangent(d, xti, s.kc));
+                             const Float3 diff = sub(xyz(s.boundary.v[b]), xyz(vm));
+                             acc_t = add(acc_t, add(scale(diff, w_t), scale(xyz(vt_i), -w)));
+                         }
+                         const Float3 gradient = scale(d, cubic_gradient_factor(length(d), s.kc));
+                         const Float3 gradient_t = cubic_gradient_tangent(d, xti, s.kc);
+                         boundary_pressure(gradient, gradient_t, vol, t1_i, ap1);
+                         if constexpr (kTwo) boundary_pressure(gradient, gradient_t, vol, t2_i, ap2);
                      });
     if constexpr (kNonPressure) {
         a.anp_t[i] = make_float4(acc_t.x, acc_t.y, acc_t.z, 0.0F);
@@ -281,6 +333,7 @@ struct ProjectionJvpArgs {
     const float* volume_t;  // its tangent
     float* p_t;  // in: tangent of p_prev; out: tangent of p
     float2* t_out;
+    const float* rest_t;  // rest-density tangents (the boundary terms)
 };
 
 // density_jvp_kernel at the predicted positions (topology x) +
@@ -296,15 +349,25 @@ __global__ void __launch_bounds__(kBlock) dfsph_jvp_projection_kernel(Projection
     int cx, cy, cz;
     particle_cell(s.grid, xyz(s.xm[i]), cx, cy, cz);
     float rs_t = 0.0F;
+    const float rest_i = s.rest[i];
+    const float rest_t_i = a.rest_t[i];
     const MaskView mv = skin_view(s);
-    visit_candidates(s.grid, mv, masks_apply(mv, a.slot), i, cx, cy, cz, delta, smask + threadIdx.x, kBlock,
-                     [&](uint32_t j) {
-                         const float4 q = a.xs[j];
-                         const Float3 d = sub(ps, xyz(q));
-                         const float dist = length(d);
-                         rs_t += a.xt[j].w * cubic_value(dist, s.kc) +
-                                 q.w * dot(scale(d, cubic_gradient_factor(dist, s.kc)), sub(pst, xyz(a.xs_t[j])));
-                     });
+    walk(
+        s, mv, masks_apply(mv, a.slot), i, cx, cy, cz, delta, smask + threadIdx.x,
+        [&](uint32_t j) {
+            const float4 q = a.xs[j];
+            const Float3 d = sub(ps, xyz(q));
+            const float dist = length(d);
+            rs_t += a.xt[j].w * cubic_value(dist, s.kc) +
+                    q.w * dot(scale(d, cubic_gradient_factor(dist, s.kc)), sub(pst, xyz(a.xs_t[j])));
+        },
+        [&](uint32_t b) {
+            const float4 q = s.boundary.x[b];
+            const Float3 d = sub(ps, xyz(q));
+            const float dist = length(d);
+            rs_t += q.w * (rest_t_i * cubic_value(dist, s.kc) +
+                           rest_i * dot(scale(d, cubic_gradient_factor(dist, s.kc)), pst));
+        });
     const float m = self.w;
     const float m_t = a.xt[i].w;
     const float rho = s.rho[i];
@@ -392,6 +455,7 @@ struct PressureVjpArgs {
     DVec pos_a;
     double* rho_a;
     double* m_a;
+    double* rest_a;  // accumulated: the boundary terms' rest-density adjoint
 };
 
 // pressure_vjp_kernel (common.cu:239-294) + projection_update_vjp_kernel
@@ -420,9 +484,11 @@ __global__ void __launch_bounds__(kBlock) dfsph_vjp_pressure_kernel(PressureVjpA
     D3 velocity{0.0, 0.0, 0.0};
     double dens_c = 0.0;
     double visc_c = 0.0;
+    double rest_c = 0.0;
+    const float rest_i = s.rest[i];
     int cx, cy, cz;
     particle_cell(s.grid, x, cx, cy, cz);
-    visit_candidates(s.grid, exact_view(s), exact_masked(s), i, cx, cy, cz, delta, smask + threadIdx.x, kBlock,
+    walk(s, exact_view(s), exact_masked(s), i, cx, cy, cz, delta, smask + threadIdx.x,
                      [&](uint32_t j) {
                          if (j == i) return;
                          const float4 q = s.xm[j];
@@ -461,7 +527,30 @@ __global__ void __launch_bounds__(kBlock) dfsph_vjp_pressure_kernel(PressureVjpA
                          mass_c += sum * projection;
                          first_a += m_i * projection;
                          position = add(position, cubic_hessian_product(d, scale(cross, m_i * sum), h));
+                     },
+                     [&](uint32_t b) {
+                         const float4 q = s.boundary.x[b];
+                         const float vol = q.w;
+                         const Float3 d = sub(x, xyz(q));
+                         if constexpr (kNonPressure) {
+                             // non_pressure_vjp_kernel's boundary half
+                             const Float3 diff = sub(xyz(s.boundary.v[b]), vel);
+                             const float lap = viscosity_laplacian_c(d, s.kc);
+                             const float w = vm.w * vol * lap;
+                             const double w_a = dot(lnp, d3(diff));
+                             velocity = add(velocity, scale(lnp, static_cast<double>(-w)));
+                             visc_c += w_a * vol * lap;
+                             position = add(position, viscosity_laplacian_gradient(d, h, s.pi_h6, w_a * vm.w * vol));
+                         }
+                         // pressure_vjp_kernel's boundary half: -rho0_i V (p_i / rho_i^2) grad W
+                         const D3 gradient = d3(scale(d, cubic_gradient_factor(length(d), s.kc)));
+                         const double factor = static_cast<double>(-rest_i * vol) * first;
+                         const double factor_a = dot(local, gradient);
+                         rest_c -= factor_a * vol * first;
+                         first_a -= factor_a * rest_i * vol;
+                         position = add(position, cubic_hessian_product(d, scale(local, factor), h));
                      });
+    if (s.boundary.offsets != nullptr) a.rest_a[i] += rest_c;
     accumulate(a.pos_a, i, position);
     double m_a = a.m_a[i] + mass_c;
     double rho_a = a.rho_a[i] - 2.0 * first_a * a.p[i] / (rho_i * rho_i * rho_i);
@@ -514,6 +603,7 @@ struct DensityVjpArgs {
     DVec prev_ap;  // written: adjoint of the previous iterate's pressure accelerations
     DVec adiv_a;   // accumulated in the density phase, unused in the divergence phase
     double* m_a;
+    double* rest_a;  // accumulated: the boundary terms' rest-density adjoint
 };
 
 // density_vjp_kernel at the predicted positions (density.cu:69-100, topology
@@ -534,18 +624,29 @@ __global__ void __launch_bounds__(kBlock) dfsph_vjp_density_kernel(DensityVjpArg
     particle_cell(s.grid, xyz(s.xm[i]), cx, cy, cz);
     D3 position{0.0, 0.0, 0.0};
     double mass_c = 0.0;
+    double rest_c = 0.0;
+    const float rest_i = s.rest[i];
     const MaskView mv = skin_view(s);
-    visit_candidates(s.grid, mv, masks_apply(mv, a.slot), i, cx, cy, cz, delta, smask + threadIdx.x, kBlock,
-                     [&](uint32_t j) {
-                         const float4 q = a.xs[j];
-                         const Float3 d = sub(ps, xyz(q));
-                         const float dist = length(d);
-                         const double rs_a_j = a.rs_a[j];
-                         position = add(position, scale(d3(scale(d, cubic_gradient_factor(dist, s.kc))),
-                                                        out_a * q.w + rs_a_j * m_i));
-                         mass_c += rs_a_j * cubic_value(dist, s.kc);
-                     });
+    walk(
+        s, mv, masks_apply(mv, a.slot), i, cx, cy, cz, delta, smask + threadIdx.x,
+        [&](uint32_t j) {
+            const float4 q = a.xs[j];
+            const Float3 d = sub(ps, xyz(q));
+            const float dist = length(d);
+            const double rs_a_j = a.rs_a[j];
+            position = add(position, scale(d3(scale(d, cubic_gradient_factor(dist, s.kc))),
+                                           out_a * q.w + rs_a_j * m_i));
+            mass_c += rs_a_j * cubic_value(dist, s.kc);
+        },
+        [&](uint32_t b) {
+            const float4 q = s.boundary.x[b];
+            const Float3 d = sub(ps, xyz(q));
+            const float dist = length(d);
+            position = add(position, scale(d3(scale(d, cubic_gradient_factor(dist, s.kc))), out_a * rest_i * q.w));
+            rest_c += out_a * q.w * cubic_value(dist, s.kc);
+        });
     a.m_a[i] += mass_c;
+    if (s.boundary.offsets != nullptr) a.rest_a[i] += rest_c;
     D3 pos, velocity;
     predict_adjoint(a.collisions[i], s.box.no_slip, static_cast<double>(s.dt), position, D3{0.0, 0.0, 0.0}, pos,
                     velocity);
@@ -583,23 +684,33 @@ __global__ void __launch_bounds__(kBlock) dfsph_vjp_scatter_kernel(ScatterVjpArg
     particle_cell(s.grid, x, cx, cy, cz);
     D3 position{0.0, 0.0, 0.0};
     double mass_c = 0.0;
-    visit_candidates(s.grid, exact_view(s), exact_masked(s), i, cx, cy, cz, delta, smask + threadIdx.x, kBlock,
-                     [&](uint32_t j) {
-                         const float4 q = s.xm[j];
-                         const Float3 d = sub(x, xyz(q));
-                         const float dist = length(d);
-                         const double rho_a_j = a.rho_a[j];
-                         position = add(position, scale(d3(scale(d, cubic_gradient_factor(dist, s.kc))),
-                                                        out_a * q.w + rho_a_j * m_i));
-                         mass_c += rho_a_j * cubic_value(dist, s.kc);
-                     });
+    double rest_c = 0.0;
+    const float rest_i = s.rest[i];
+    walk(
+        s, exact_view(s), exact_masked(s), i, cx, cy, cz, delta, smask + threadIdx.x,
+        [&](uint32_t j) {
+            const float4 q = s.xm[j];
+            const Float3 d = sub(x, xyz(q));
+            const float dist = length(d);
+            const double rho_a_j = a.rho_a[j];
+            position = add(position, scale(d3(scale(d, cubic_gradient_factor(dist, s.kc))),
+                                           out_a * q.w + rho_a_j * m_i));
+            mass_c += rho_a_j * cubic_value(dist, s.kc);
+        },
+        [&](uint32_t b) {
+            const float4 q = s.boundary.x[b];
+            const Float3 d = sub(x, xyz(q));
+            const float dist = length(d);
+            position = add(position, scale(d3(scale(d, cubic_gradient_factor(dist, s.kc))), out_a * rest_i * q.w));
+            rest_c += out_a * q.w * cubic_value(dist, s.kc);
+        });
     const uint32_t j = a.perm[i];
     const D3 pos = add(load(a.pos_a, i), position);
     if (a.out_pos.x != nullptr) accumulate(a.out_pos, j, pos);
     if (a.out_vel.x != nullptr) accumulate(a.out_vel, j, load(a.vel_a, i));
     if (a.out_ctl.x != nullptr) accumulate(a.out_ctl, j, load(a.anp_a, i));
     accumulate_to(a.out_m, j, a.m_a[i] + mass_c);
-    accumulate_to(a.out_rest, j, a.rest_a[i]);
+    accumulate_to(a.out_rest, j, a.rest_a[i] + rest_c);
     accumulate_to(a.out_visc, j, a.visc_a[i]);
     accumulate_to(a.out_relax_div, j, a.relax_div_a[i]);
     accumulate_to(a.out_relax_den, j, a.relax_den_a[i]);
@@ -649,6 +760,8 @@ Step make_step(const Domain& domain, const DfsphCache& cache, const KernelConsta
     s.skin = cache.skin.data();
     s.flags = cache.mask_flags.data();
     s.limit2 = cache.mask_limit2;
+    s.boundary = cache.neighborhood.boundary();
+    s.rest = cache.rho0.data();
     return s;
 }
 
@@ -714,7 +827,8 @@ void DfsphSolver::jvp_step(const DfsphCache& cache, const phx_sph_dfsph_state_ta
                       dw_.p_t_den.data(),
                       dw_.rho_t.data(),
                       dw_.t_div.data(),
-                      dw_.t_den.data()};
+                      dw_.t_den.data(),
+                      dw_.rest_t.data()};
     dfsph_jvp_density_kernel<<<blocks, kBlock, 0, stream_>>>(da);
     PHX_LAUNCH_CHECK();
 
@@ -742,6 +856,7 @@ void DfsphSolver::jvp_step(const DfsphCache& cache, const phx_sph_dfsph_state_ta
     pa.rho_t = dw_.rho_t.data();
     pa.volume = cache.volume_rest ? cache.rho0.data() : cache.densities.data();
     pa.volume_t = cache.volume_rest ? dw_.rest_t.data() : dw_.rho_t.data();
+    pa.rest_t = dw_.rest_t.data();
     const size_t nn = n;
     uint32_t slot = 0;  // prediction index
     auto collisions = [&](uint32_t k) { return cache.collisions.data() + k * nn; };
@@ -865,6 +980,7 @@ void DfsphSolver::vjp_step(const DfsphCache& cache, const phx_sph_dfsph_state_ad
     pv.pos_a = dw_.pos_a.view();
     pv.rho_a = dw_.rho_a.data();
     pv.m_a = dw_.m_a.data();
+    pv.rest_a = dw_.rest_a.data();
     pv.rs_a = dw_.rs_a.data();
     pv.volume = cache.volume_rest ? cache.rho0.data() : nullptr;
     pv.volume_a = cache.volume_rest ? dw_.rest_a.data() : nullptr;
@@ -875,6 +991,7 @@ void DfsphSolver::vjp_step(const DfsphCache& cache, const phx_sph_dfsph_state_ad
     dv.anp_a = dw_.anp_a.view();
     dv.adiv_a = adiv_a;
     dv.m_a = dw_.m_a.data();
+    dv.rest_a = dw_.rest_a.data();
     dv.rs_a = dw_.rs_a.data();
     // the buffer after `cur`, skipping buffer 2 while it is still needed
     auto next_buffer = [](int cur, bool keep2) {

@@ -192,8 +192,9 @@ SphSolver::SphSolver(const phx_domain_configuration* domain, const phx_sph_confi
             PHX_ERROR_INVALID_ARGUMENT, "unknown SPH method");
     require(mode == PHX_MODE_FORWARD || mode == PHX_MODE_DIFFERENTIABLE, PHX_ERROR_INVALID_ARGUMENT,
             "unknown execution mode");
-    require(domain_.config.boundary_particle_count == 0u || mode == PHX_MODE_FORWARD, PHX_ERROR_UNSUPPORTED,
-            "boundary particles are supported in forward mode only");
+    require(domain_.config.boundary_particle_count == 0u || mode == PHX_MODE_FORWARD ||
+                config_.method != PHX_SPH_WCSPH,
+            PHX_ERROR_UNSUPPORTED, "WCSPH supports boundary particles in forward mode only");
     boundary_.upload(domain_.config);
     domain_.config.boundary_particles = nullptr;  // the caller's host array is not kept
     const float tolerance = config_.density_error_tolerance;

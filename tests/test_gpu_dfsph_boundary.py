@@ -58,8 +58,7 @@ def test_rollout_matches_oracle(gpu_device, moving, tolerance):
         solver.copy_state(nxt, state)
 
 
-def test_boundary_particles_are_forward_only(gpu_device):
+def test_differentiable_mode_accepts_boundary_particles(gpu_device):
     sc = scenes.dfsph_dam_break(shape=(8, 4, 8))
     add_boundary_samples(sc)
-    with pytest.raises(RuntimeError, match="forward mode only"):
-        SPH(sc.domain, DFSPH.Configuration(), mode=ExecutionMode.differentiable, device=gpu_device)
+    SPH(sc.domain, DFSPH.Configuration(), mode=ExecutionMode.differentiable, device=gpu_device)

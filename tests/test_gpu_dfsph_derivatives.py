@@ -6,9 +6,9 @@ The recorded forward step is bit-exact against the FP32 oracle. The VJP
 only the order in which a few accumulators add their parts differing. The
 JVP (FP32, as the reference's) runs the oracle's operations in the same
 order and agrees to 1e-6. The scenes are those of the oracle's own pins
-(a compressed swirling block, and the same block pushed into a wall so that
-every prediction clamps some particles), plus a 16,384-particle dam break
-at rest in its corner."""
+(a compressed swirling block, the same block pushed into a wall so that
+every prediction clamps some particles, and the block over a moving plane of
+boundary samples), plus a 16,384-particle dam break at rest in its corner."""
 import numpy as np
 import pytest
 import torch
@@ -84,13 +84,13 @@ def run_jvp(solver, state, nxt, control, params, cache, t, device):
 
 
 CASES = [(0, 0, True, "free"), (1, 1, False, "free"), (2, 3, True, "free"), (4, 6, True, "free"),
-         (2, 3, True, "wall")]
+         (2, 3, True, "wall"), (2, 2, True, "boundary")]
 
 
 @pytest.mark.parametrize("nd,ne,warm,variant", CASES)
 def test_vjp_matches_oracle(gpu_device, nd, ne, warm, variant):
     print(f"dfsph vjp nd={nd} ne={ne} warm={warm} {variant}", flush=True)
-    cs = case(wall=variant == "wall")
+    cs = case(wall=variant == "wall", boundary=variant == "boundary")
     run = recorded(cs, gpu_device, nd, ne, warm)
     rng = np.random.default_rng(70 + nd + 7 * ne)
     _, w = directions(cs, rng)
@@ -109,7 +109,7 @@ def test_vjp_matches_oracle(gpu_device, nd, ne, warm, variant):
 @pytest.mark.parametrize("nd,ne,warm,variant", CASES)
 def test_jvp_matches_oracle(gpu_device, nd, ne, warm, variant):
     print(f"dfsph jvp nd={nd} ne={ne} warm={warm} {variant}", flush=True)
-    cs = case(wall=variant == "wall")
+    cs = case(wall=variant == "wall", boundary=variant == "boundary")
     run = recorded(cs, gpu_device, nd, ne, warm)
     rng = np.random.default_rng(80 + nd + 7 * ne)
     t, _ = directions(cs, rng)
@@ -123,9 +123,11 @@ def test_jvp_matches_oracle(gpu_device, nd, ne, warm, variant):
         assert err <= 1e-6, (k, err)
 
 
-def test_dot_product_identity_on_gpu(gpu_device):
+@pytest.mark.parametrize("boundary", [False, True])
+def test_dot_product_identity_on_gpu(gpu_device, boundary):
     """<J t, w> from the FP32 JVP against <t, J^T w> from the FP64 VJP."""
-    cs = case(wall=True)
+    print(f"dfsph dot-product identity boundary={boundary}", flush=True)
+    cs = case(wall=True, boundary=boundary)
     run = recorded(cs, gpu_device, 2, 3, True)
     rng = np.random.default_rng(90)
     t, w = directions(cs, rng)

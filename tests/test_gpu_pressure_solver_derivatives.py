@@ -7,7 +7,7 @@ bit for bit, cache fields included. The JVP (FP32) agrees with the oracle's
 FP32 JVP to 1e-6 of the largest entry and the VJP (FP64 adjoints of the FP32
 primal) to 1e-10, as for DFSPH; the GPU's own JVP and VJP satisfy the
 dot-product identity. The scene is the compressed swirling block of the
-oracle's derivative pins."""
+oracle's derivative pins, also over a moving plane of boundary samples."""
 import numpy as np
 import pytest
 import torch
@@ -101,10 +101,11 @@ def test_differentiable_forward_equals_forward_mode_and_its_cache(method):
 
 
 @pytest.mark.parametrize("method", METHODS)
-@pytest.mark.parametrize("iterations", [1, 4])
-def test_jvp_and_vjp_match_the_oracle(method, iterations):
+@pytest.mark.parametrize("iterations,boundary", [(1, False), (4, False), (3, True)])
+def test_jvp_and_vjp_match_the_oracle(method, iterations, boundary):
+    print(f"{method} derivatives iterations={iterations} boundary={boundary}", flush=True)
     dev = torch.device("cuda:0")
-    cs = setup(method)
+    cs = setup(method, boundary=boundary)
     run = recorded(cs, method, iterations, dev)
     rng = np.random.default_rng(90 + iterations)
     t, w = directions(cs, rng)

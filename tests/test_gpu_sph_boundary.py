@@ -10,7 +10,7 @@ import torch
 
 import oracle
 from physica_amd import scenes
-from physica_amd.particle import ExecutionMode, PCISPH, SPH
+from physica_amd.particle import ExecutionMode, PCISPH, SPH, WCSPH
 from helpers import add_boundary_samples, bits_equal, sph_setup
 
 pytestmark = pytest.mark.gpu
@@ -68,8 +68,11 @@ def test_rollout_with_boundary_bit_exact(gpu_device, method):
         solver.copy_state(nxt, state)
 
 
-def test_boundary_particles_are_forward_only(gpu_device):
-    sc = scene_for("pcisph")
+def test_wcsph_boundary_particles_are_forward_only(gpu_device):
+    """PCISPH and IISPH differentiate their boundary terms (on the projection
+    engine); WCSPH's derivative passes have none and refuse samples."""
+    sc = scene_for("wcsph")
     add_boundary_samples(sc)
     with pytest.raises(RuntimeError, match="forward mode only"):
-        SPH(sc.domain, PCISPH.Configuration(), mode=ExecutionMode.differentiable, device=gpu_device)
+        SPH(sc.domain, WCSPH.Configuration(), mode=ExecutionMode.differentiable, device=gpu_device)
+    SPH(sc.domain, PCISPH.Configuration(), mode=ExecutionMode.differentiable, device=gpu_device)

@@ -60,6 +60,7 @@ def case(shape=(6, 6, 5), seed=3, wall=False, boundary=False):
         bvel = np.zeros((3, m), dtype=np.float32)
         bvel[0] = 0.05
         bp = (bpos, bvel, np.full(m, s ** 3, dtype=np.float32))
+        sc.domain.boundary_particles = [(tuple(bpos[:, k]), tuple(bvel[:, k]), s ** 3) for k in range(m)]
     d = oracle.make_domain(sc.domain, bp)
     # warm-start pressures from one step of the same scene
     z = np.zeros(n, dtype=np.float32)

@@ -21,8 +21,8 @@ STATE = ("positions", "velocities", "controls")
 METHODS = ("pcisph", "iisph")
 
 
-def setup(method, seed=3, shape=(6, 6, 5)):
-    cs = case(shape=shape, seed=seed)
+def setup(method, seed=3, shape=(6, 6, 5), boundary=False):
+    cs = case(shape=shape, seed=seed, boundary=boundary)
     p = cs.sc.parameters
     p["relaxation"] = p["density_relaxation"]
     return cs
@@ -77,9 +77,9 @@ def test_recorded_primal_is_the_forward_step(method, precision):
 
 
 @pytest.mark.parametrize("method", METHODS)
-@pytest.mark.parametrize("iterations", [1, 4])
-def test_dot_product_identity(method, iterations):
-    cs = setup(method)
+@pytest.mark.parametrize("iterations,boundary", [(1, False), (4, False), (3, True)])
+def test_dot_product_identity(method, iterations, boundary):
+    cs = setup(method, boundary=boundary)
     x, v, u, p = inputs(cs)
     rng = np.random.default_rng(40 + iterations)
     t, w = directions(cs, rng)
@@ -91,8 +91,9 @@ def test_dot_product_identity(method, iterations):
 
 
 @pytest.mark.parametrize("method", METHODS)
-def test_jvp_matches_central_differences(method):
-    cs = setup(method)
+@pytest.mark.parametrize("boundary", [False, True])
+def test_jvp_matches_central_differences(method, boundary):
+    cs = setup(method, boundary=boundary)
     iterations = 3
     x, v, u, p = inputs(cs)
     rng = np.random.default_rng(50)
