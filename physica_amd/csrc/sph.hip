@@ -192,9 +192,6 @@ SphSolver::SphSolver(const phx_domain_configuration* domain, const phx_sph_confi
             PHX_ERROR_INVALID_ARGUMENT, "unknown SPH method");
     require(mode == PHX_MODE_FORWARD || mode == PHX_MODE_DIFFERENTIABLE, PHX_ERROR_INVALID_ARGUMENT,
             "unknown execution mode");
-    require(domain_.config.boundary_particle_count == 0u || mode == PHX_MODE_FORWARD ||
-                config_.method != PHX_SPH_WCSPH,
-            PHX_ERROR_UNSUPPORTED, "WCSPH supports boundary particles in forward mode only");
     boundary_.upload(domain_.config);
     domain_.config.boundary_particles = nullptr;  // the caller's host array is not kept
     const float tolerance = config_.density_error_tolerance;
@@ -354,6 +351,7 @@ void SphSolver::forward_wcsph(const phx_sph_state& state, const phx_control& con
         r->collisions.resize(n);
         r->exact.resize(static_cast<size_t>(kStencil) * n);
         r->flags.resize(masks_.flag_count());
+        if (boundary_.count() > 0u) r->btension.resize(n);
     }
     float4* xm = rec ? r->xm.data() : xm_.data();
     float4* vmu = rec ? r->vmu.data() : vmu_.data();
@@ -372,7 +370,8 @@ void SphSolver::forward_wcsph(const phx_sph_state& state, const phx_control& con
     gather.add(pp.surface_tensions, aux, 3);
     gather.add(mp.tait_exponent, tait);
     const bool with_boundary = boundary_.count() > 0u;
-    if (with_boundary) gather.add(mp.boundary_surface_tension, btension_.data());
+    float* btension = rec && with_boundary ? r->btension.data() : btension_.data();
+    if (with_boundary) gather.add(mp.boundary_surface_tension, btension);
     {
         PassScope scope(profiler_, stream_, kSphGrid);
         cache.neighborhood.build(stream_, domain_, step, cvec(state.particles.positions), nullptr, gather.get(),
@@ -420,7 +419,7 @@ void SphSolver::forward_wcsph(const phx_sph_state& state, const phx_control& con
     wa.collisions = rec ? r->collisions.data() : nullptr;
     wa.boundary = boundary;
     wa.rest_density = rho0;
-    wa.boundary_tension = btension_.data();
+    wa.boundary_tension = btension;
     {
         PassScope scope(profiler_, stream_, kSphForces);
         wcsph_force_kernel<<<blocks, kBlock, 0, stream_>>>(wa);

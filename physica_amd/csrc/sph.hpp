@@ -19,6 +19,7 @@ struct WcsphRecord {
     uint64_t step_index = 0;
     DeviceBuffer<float4> xm, vmu, aux;  // aux: density, p / rho^2, speed of sound, surface tension
     DeviceBuffer<float> rho0, tait;
+    DeviceBuffer<float> btension;  // boundary surface tensions (boundary samples only)
     DeviceBuffer<uint8_t> collisions;
     DeviceBuffer<uint16_t> exact;
     DeviceBuffer<uint32_t> flags;
@@ -30,7 +31,7 @@ struct WcsphDerivativeWorkspace {
     DeviceBuffer<float> rho_t;
     DeviceBuffer<float2> tt;  // (p / rho^2, its tangent)
     DVecBuffer pos_a, vel_a, total_a;
-    DeviceBuffer<double> rho_a, m_a, rest_a, visc_a, tension_a, speed_a, tait_a;
+    DeviceBuffer<double> rho_a, m_a, rest_a, visc_a, tension_a, speed_a, tait_a, btension_a;
     void allocate_tangents(uint32_t n);
     void allocate_adjoints(uint32_t n);
 };

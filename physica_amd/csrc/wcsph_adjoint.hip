@@ -45,10 +45,25 @@ struct WcsphStep {
     const uint16_t* exact;
     const uint32_t* flags;
     const uint8_t* collisions;
+    BoundaryGrid boundary;   // offsets null: no boundary samples
+    const float* btension;   // boundary surface tensions (boundary samples only)
 };
 
 __device__ __forceinline__ MaskView exact_view(const WcsphStep& s) { return MaskView{s.exact, s.flags, s.n, 0.0F}; }
 __device__ __forceinline__ bool exact_masked(const WcsphStep& s) { return s.flags[kFlagOverflow] == 0u; }
+
+// Visits the fluid candidates of particle i through its exact masks or, with
+// boundary samples, every fluid particle and every sample of the 27 cells in
+// the reference order: each cell's fluid, then its samples.
+template <class Visit, class VisitBoundary>
+__device__ __forceinline__ void walk(const WcsphStep& s, uint32_t i, int cx, int cy, int cz, const int* delta,
+                                     uint16_t* column, Visit&& visit, VisitBoundary&& visit_boundary) {
+    if (s.boundary.offsets != nullptr) {
+        for_each_candidate(s.grid, s.boundary, cx, cy, cz, visit, visit_boundary);
+    } else {
+        visit_candidates(s.grid, exact_view(s), exact_masked(s), i, cx, cy, cz, delta, column, kBlock, visit);
+    }
+}
 
 // The cohesion weight of surface_forward_kernel (wcsph.cu:324): W at the
 // distance, clamped below at the particle diameter.
@@ -61,11 +76,11 @@ struct WcsphGatherArgs {
     uint32_t n;
     const uint32_t* perm;
     CVec x_t, v_t, u_t;
-    const float *m_t, *rest_t, *visc_t, *tension_t, *speed_t, *tait_t;  // null: zero
+    const float *m_t, *rest_t, *visc_t, *tension_t, *speed_t, *tait_t, *btension_t;  // null: zero
     float4* xt;  // .w: mass tangent
     float4* vt;  // .w: viscosity tangent
     float4* ut;  // .w: rest-density tangent
-    float4* pt;  // speed of sound, Tait exponent, surface tension tangents
+    float4* pt;  // speed of sound, Tait exponent, surface tension, boundary surface tension tangents
 };
 
 __global__ void __launch_bounds__(kBlock) wcsph_gather_tangent_kernel(WcsphGatherArgs a) {
@@ -75,7 +90,8 @@ __global__ void __launch_bounds__(kBlock) wcsph_gather_tangent_kernel(WcsphGathe
     a.xt[i] = make_float4(a.x_t.x[j], a.x_t.y[j], a.x_t.z[j], or_zero(a.m_t, j));
     a.vt[i] = make_float4(a.v_t.x[j], a.v_t.y[j], a.v_t.z[j], or_zero(a.visc_t, j));
     a.ut[i] = make_float4(a.u_t.x[j], a.u_t.y[j], a.u_t.z[j], or_zero(a.rest_t, j));
-    a.pt[i] = make_float4(or_zero(a.speed_t, j), or_zero(a.tait_t, j), or_zero(a.tension_t, j), 0.0F);
+    a.pt[i] = make_float4(or_zero(a.speed_t, j), or_zero(a.tait_t, j), or_zero(a.tension_t, j),
+                          or_zero(a.btension_t, j));
 }
 
 struct WcsphDensityJvpArgs {
@@ -99,19 +115,28 @@ __global__ void __launch_bounds__(kBlock) wcsph_jvp_density_kernel(WcsphDensityJ
     int cx, cy, cz;
     particle_cell(s.grid, p, cx, cy, cz);
     float rho_t = 0.0F;
-    visit_candidates(s.grid, exact_view(s), exact_masked(s), i, cx, cy, cz, delta, smask + threadIdx.x, kBlock,
-                     [&](uint32_t j) {
-                         const float4 q = s.xm[j];
-                         const float4 qt = a.xt[j];
-                         const Float3 d = sub(p, xyz(q));
-                         const float dist = length(d);
-                         rho_t += qt.w * cubic_value(dist, s.kc) +
-                                  q.w * dot(scale(d, cubic_gradient_factor(dist, s.kc)), sub(pti, xyz(qt)));
-                     });
+    const float rest = s.rho0[i];
+    const float rest_t = a.ut[i].w;
+    walk(
+        s, i, cx, cy, cz, delta, smask + threadIdx.x,
+        [&](uint32_t j) {
+            const float4 q = s.xm[j];
+            const float4 qt = a.xt[j];
+            const Float3 d = sub(p, xyz(q));
+            const float dist = length(d);
+            rho_t += qt.w * cubic_value(dist, s.kc) +
+                     q.w * dot(scale(d, cubic_gradient_factor(dist, s.kc)), sub(pti, xyz(qt)));
+        },
+        [&](uint32_t b) {
+            const float4 q = s.boundary.x[b];
+            const Float3 d = sub(p, xyz(q));
+            const float dist = length(d);
+            rho_t += q.w * (rest_t * cubic_value(dist, s.kc) +
+                            rest * dot(scale(d, cubic_gradient_factor(dist, s.kc)), pti));
+        });
     a.rho_t[i] = rho_t;
     const float4 own = s.aux[i];
     const float density = own.x;
-    const float rest = s.rho0[i];
     const float speed = own.z;
     const float exponent = s.tait[i];
     const float4 pt = a.pt[i];
@@ -122,7 +147,7 @@ __global__ void __launch_bounds__(kBlock) wcsph_jvp_density_kernel(WcsphDensityJ
     const float d_rest = speed * speed * difference / exponent - speed * speed * powered;
     const float d_speed = 2.0F * rest * speed * difference / exponent;
     const float d_exponent = rest * speed * speed * (exponent * powered * logf(ratio) - difference) / (exponent * exponent);
-    const float p_t = d_density * rho_t + d_rest * a.ut[i].w + d_speed * pt.x + d_exponent * pt.y;
+    const float p_t = d_density * rho_t + d_rest * rest_t + d_speed * pt.x + d_exponent * pt.y;
     a.tt[i] = first_term(s.p[i], p_t, density, rho_t);
 }
 
@@ -165,9 +190,11 @@ __global__ void __launch_bounds__(kBlock) wcsph_jvp_force_kernel(WcsphForceJvpAr
     Float3 ap{0.0F, 0.0F, 0.0F};
     Float3 av{0.0F, 0.0F, 0.0F};
     Float3 as{0.0F, 0.0F, 0.0F};
+    const float rest_i = s.rho0[i];
+    const float rest_t_i = a.ut[i].w;
     int cx, cy, cz;
     particle_cell(s.grid, x, cx, cy, cz);
-    visit_candidates(s.grid, exact_view(s), exact_masked(s), i, cx, cy, cz, delta, smask + threadIdx.x, kBlock,
+    walk(s, i, cx, cy, cz, delta, smask + threadIdx.x,
                      [&](uint32_t j) {
                          if (j == i) return;
                          const float4 q = s.xm[j];
@@ -220,6 +247,47 @@ __global__ void __launch_bounds__(kBlock) wcsph_jvp_force_kernel(WcsphForceJvpAr
                              const float f_t = -pt_i.z * mass * w / m_i - sigma * mass_t * w / m_i -
                                                sigma * mass * w_t / m_i + sigma * mass * w * m_t_i / (m_i * m_i);
                              as = add(as, add(scale(d, f_t), scale(dd, f)));
+                         }
+                     },
+                     [&](uint32_t b) {
+                         const float4 q = s.boundary.x[b];
+                         const float vol = q.w;
+                         const Float3 d = sub(x, xyz(q));
+                         const float dist = length(d);
+                         const Float3 gradient = scale(d, cubic_gradient_factor(dist, s.kc));
+                         const Float3 gradient_t = cubic_gradient_tangent(d, xti, s.kc);
+                         // pressure: -rho0_i V (p_i / rho_i^2) grad W
+                         const float factor = -rest_i * vol * tt_i.x;
+                         const float factor_t = -vol * (rest_t_i * tt_i.x + rest_i * tt_i.y);
+                         ap = add(ap, add(scale(gradient, factor_t), scale(gradient_t, factor)));
+                         // artificial viscosity: nu = mu_i h c_i / rho_i, mass rho0_i V
+                         const Float3 dv = sub(vel, xyz(s.boundary.v[b]));
+                         const float compression = dot(dv, d);
+                         if (!(compression >= 0.0F)) {
+                             const float nu = vm.w * h * own.z / own.x;
+                             const float nu_t = h * (vt_i.w * own.z + vm.w * pt_i.x) / own.x - nu * rho_t_i / own.x;
+                             const float mass = rest_i * vol;
+                             const float mass_t = rest_t_i * vol;
+                             const float compression_t = dot(vti, d) + dot(dv, xti);
+                             const float denominator = dot(d, d) + 0.01F * h * h;
+                             const float denominator_t = 2.0F * dot(d, xti);
+                             const float pi = -nu * compression / denominator;
+                             const float pi_t = -(nu_t * compression + nu * compression_t) / denominator +
+                                                nu * compression * denominator_t / (denominator * denominator);
+                             av = add(av, add(scale(gradient, -(mass_t * pi + mass * pi_t)),
+                                              scale(gradient_t, -mass * pi)));
+                         }
+                         // cohesion with the boundary surface tension, mass rho0_i V
+                         if (!(dist >= h)) {
+                             const float w = cohesion_weight(dist, s);
+                             const float w_t = dist > s.diameter ? dot(gradient, xti) : 0.0F;
+                             const float sigma = s.btension[i];
+                             const float mb = rest_i * vol;
+                             const float mb_t = rest_t_i * vol;
+                             const float f = -sigma * mb * w / m_i;
+                             const float f_t = -pt_i.w * mb * w / m_i - sigma * mb_t * w / m_i -
+                                               sigma * mb * w_t / m_i + sigma * mb * w * m_t_i / (m_i * m_i);
+                             as = add(as, add(scale(d, f_t), scale(xti, f)));
                          }
                      });
     Float3 total = add(ap, av);
@@ -319,7 +387,7 @@ struct WcsphForceVjpArgs {
     WcsphStep s;
     DVec total_a;  // acceleration adjoints (complete)
     DVec pos_a, vel_a;  // accumulated
-    double *rho_a, *m_a, *rest_a, *visc_a, *tension_a, *speed_a, *tait_a;  // written
+    double *rho_a, *m_a, *rest_a, *visc_a, *tension_a, *speed_a, *tait_a, *btension_a;  // written
 };
 
 // pressure_vjp_kernel (common.cu:239-294), viscosity_vjp_kernel
@@ -343,9 +411,11 @@ __global__ void __launch_bounds__(kBlock) wcsph_vjp_force_kernel(WcsphForceVjpAr
     D3 position{0.0, 0.0, 0.0};
     D3 velocity{0.0, 0.0, 0.0};
     double first_a = 0.0, mass_c = 0.0, rho_c = 0.0, mu_c = 0.0, speed_c = 0.0, tension_c = 0.0;
+    double rest_c = 0.0, btension_c = 0.0;
+    const float rest_i = s.rho0[i];
     int cx, cy, cz;
     particle_cell(s.grid, x, cx, cy, cz);
-    visit_candidates(s.grid, exact_view(s), exact_masked(s), i, cx, cy, cz, delta, smask + threadIdx.x, kBlock,
+    walk(s, i, cx, cy, cz, delta, smask + threadIdx.x,
                      [&](uint32_t j) {
                          if (j == i) return;
                          const float4 q = s.xm[j];
@@ -390,6 +460,35 @@ __global__ void __launch_bounds__(kBlock) wcsph_vjp_force_kernel(WcsphForceVjpAr
                          const SurfaceAdjoint stheirs = surface_reverse(scale(d, -1.0F), mass, m_i, other.w, s, cross);
                          position = add(position, scale(stheirs.displacement, -1.0));
                          mass_c += stheirs.mass_b;
+                     },
+                     [&](uint32_t b) {
+                         // a sample is not differentiated: its terms reach particle i only
+                         const float4 q = s.boundary.x[b];
+                         const float vol = q.w;
+                         const Float3 d = sub(x, xyz(q));
+                         // pressure: -rho0_i V (p_i / rho_i^2) grad W
+                         const D3 gradient = d3(scale(d, cubic_gradient_factor(length(d), s.kc)));
+                         const double factor = static_cast<double>(-rest_i * vol) * first;
+                         const double factor_a = dot(local, gradient);
+                         rest_c -= factor_a * vol * first;
+                         first_a -= factor_a * rest_i * vol;
+                         position = add(position, cubic_hessian_product(d, scale(local, factor), h));
+                         // artificial viscosity: the fluid form with both ends particle i, mass rho0_i V
+                         const Float3 dv = sub(vel, xyz(s.boundary.v[b]));
+                         const ViscosityAdjoint mine = viscosity_reverse(d, dv, rho_i, rho_i, vm.w, vm.w, own.z, own.z,
+                                                                         rest_i * vol, h, s.kc, local);
+                         position = add(position, mine.displacement);
+                         velocity = add(velocity, mine.velocity);
+                         rest_c += mine.mass_b * vol;
+                         rho_c += mine.rho + mine.rho;
+                         mu_c += mine.mu + mine.mu;
+                         speed_c += mine.speed + mine.speed;
+                         // cohesion with the boundary surface tension, mass rho0_i V
+                         const SurfaceAdjoint smine = surface_reverse(d, m_i, rest_i * vol, s.btension[i], s, local);
+                         position = add(position, smine.displacement);
+                         mass_c += smine.mass_a;
+                         rest_c += smine.mass_b * vol;
+                         btension_c += smine.tension;
                      });
     accumulate(a.pos_a, i, position);
     accumulate(a.vel_a, i, velocity);
@@ -403,13 +502,14 @@ __global__ void __launch_bounds__(kBlock) wcsph_vjp_force_kernel(WcsphForceVjpAr
     const double powered = pow(ratio, exponent);
     const double difference = powered - 1.0;
     rho_a += p_a * speed * speed * pow(ratio, exponent - 1.0);
-    a.rest_a[i] = p_a * (speed * speed * difference / exponent - speed * speed * powered);
+    a.rest_a[i] = rest_c + p_a * (speed * speed * difference / exponent - speed * speed * powered);
     a.speed_a[i] = speed_c + p_a * 2.0 * rest * speed * difference / exponent;
     a.tait_a[i] = p_a * rest * speed * speed * (exponent * powered * log(ratio) - difference) / (exponent * exponent);
     a.rho_a[i] = rho_a;
     a.m_a[i] = mass_c;
     a.visc_a[i] = mu_c;
     a.tension_a[i] = tension_c;
+    a.btension_a[i] = btension_c;
 }
 
 struct WcsphDensityVjpArgs {
@@ -417,6 +517,7 @@ struct WcsphDensityVjpArgs {
     const double* rho_a;
     DVec pos_a;
     double* m_a;
+    double* rest_a;  // accumulated: the boundary terms' rest-density adjoint
 };
 
 // density_vjp_kernel (density.cu:69-100, topology = positions = x), gather
@@ -432,29 +533,40 @@ __global__ void __launch_bounds__(kBlock) wcsph_vjp_density_kernel(WcsphDensityV
     const double out_a = a.rho_a[i];
     D3 position{0.0, 0.0, 0.0};
     double mass_c = 0.0;
+    double rest_c = 0.0;
+    const float rest_i = s.rho0[i];
     int cx, cy, cz;
     particle_cell(s.grid, x, cx, cy, cz);
-    visit_candidates(s.grid, exact_view(s), exact_masked(s), i, cx, cy, cz, delta, smask + threadIdx.x, kBlock,
-                     [&](uint32_t j) {
-                         const float4 q = s.xm[j];
-                         const Float3 d = sub(x, xyz(q));
-                         const float dist = length(d);
-                         const double rho_a_j = a.rho_a[j];
-                         position = add(position, scale(d3(scale(d, cubic_gradient_factor(dist, s.kc))),
-                                                        out_a * q.w + rho_a_j * self.w));
-                         mass_c += rho_a_j * cubic_value(dist, s.kc);
-                     });
+    walk(
+        s, i, cx, cy, cz, delta, smask + threadIdx.x,
+        [&](uint32_t j) {
+            const float4 q = s.xm[j];
+            const Float3 d = sub(x, xyz(q));
+            const float dist = length(d);
+            const double rho_a_j = a.rho_a[j];
+            position = add(position, scale(d3(scale(d, cubic_gradient_factor(dist, s.kc))),
+                                           out_a * q.w + rho_a_j * self.w));
+            mass_c += rho_a_j * cubic_value(dist, s.kc);
+        },
+        [&](uint32_t b) {
+            const float4 q = s.boundary.x[b];
+            const Float3 d = sub(x, xyz(q));
+            const float dist = length(d);
+            position = add(position, scale(d3(scale(d, cubic_gradient_factor(dist, s.kc))), out_a * rest_i * q.w));
+            rest_c += out_a * q.w * cubic_value(dist, s.kc);
+        });
     accumulate(a.pos_a, i, position);
     a.m_a[i] += mass_c;
+    if (s.boundary.offsets != nullptr) a.rest_a[i] += rest_c;
 }
 
 struct WcsphScatterArgs {
     uint32_t n;
     const uint32_t* perm;
     DVec pos_a, vel_a, total_a;
-    const double *m_a, *rest_a, *visc_a, *tension_a, *speed_a, *tait_a;
+    const double *m_a, *rest_a, *visc_a, *tension_a, *speed_a, *tait_a, *btension_a;
     phx_adjoint_vector_view out_pos, out_vel, out_ctl;  // null: skipped
-    double *out_m, *out_rest, *out_visc, *out_tension, *out_speed, *out_tait;
+    double *out_m, *out_rest, *out_visc, *out_tension, *out_speed, *out_tait, *out_btension;
 };
 
 __device__ __forceinline__ void accumulate_view(phx_adjoint_vector_view v, uint32_t j, D3 a) {
@@ -476,6 +588,7 @@ __global__ void __launch_bounds__(kBlock) wcsph_vjp_scatter_kernel(WcsphScatterA
     accumulate_to(a.out_rest, j, a.rest_a[i]);
     accumulate_to(a.out_visc, j, a.visc_a[i]);
     accumulate_to(a.out_tension, j, a.tension_a[i]);
+    accumulate_to(a.out_btension, j, a.btension_a[i]);
     accumulate_to(a.out_speed, j, a.speed_a[i]);
     accumulate_to(a.out_tait, j, a.tait_a[i]);
 }
@@ -492,7 +605,8 @@ void WcsphDerivativeWorkspace::allocate_tangents(uint32_t n) {
 
 void WcsphDerivativeWorkspace::allocate_adjoints(uint32_t n) {
     for (DVecBuffer* b : {&pos_a, &vel_a, &total_a}) b->resize(n);
-    for (DeviceBuffer<double>* b : {&rho_a, &m_a, &rest_a, &visc_a, &tension_a, &speed_a, &tait_a}) b->resize(n);
+    for (DeviceBuffer<double>* b : {&rho_a, &m_a, &rest_a, &visc_a, &tension_a, &speed_a, &tait_a, &btension_a})
+        b->resize(n);
 }
 
 namespace {
@@ -516,6 +630,8 @@ WcsphStep make_wcsph_step(const Domain& domain, const SphCache& cache, const Ker
     s.exact = r.exact.data();
     s.flags = r.flags.data();
     s.collisions = r.collisions.data();
+    s.boundary = cache.neighborhood.boundary();
+    s.btension = r.btension.data();
     return s;
 }
 
@@ -546,6 +662,7 @@ void SphSolver::wcsph_jvp_step(const SphCache& cache, const phx_sph_state_tangen
     ga.tension_t = parameter_t.particles.surface_tensions;
     ga.speed_t = parameter_t.method.speed_of_sound;
     ga.tait_t = parameter_t.method.tait_exponent;
+    ga.btension_t = parameter_t.method.boundary_surface_tension;
     ga.xt = wdw_.xt.data();
     ga.vt = wdw_.vt.data();
     ga.ut = wdw_.ut.data();
@@ -614,9 +731,10 @@ void SphSolver::wcsph_vjp_step(const SphCache& cache, const phx_sph_state_adjoin
     va.tension_a = wdw_.tension_a.data();
     va.speed_a = wdw_.speed_a.data();
     va.tait_a = wdw_.tait_a.data();
+    va.btension_a = wdw_.btension_a.data();
     wcsph_vjp_force_kernel<<<blocks, kBlock, 0, stream_>>>(va);
     PHX_LAUNCH_CHECK();
-    WcsphDensityVjpArgs dv{s, wdw_.rho_a.data(), wdw_.pos_a.view(), wdw_.m_a.data()};
+    WcsphDensityVjpArgs dv{s, wdw_.rho_a.data(), wdw_.pos_a.view(), wdw_.m_a.data(), wdw_.rest_a.data()};
     wcsph_vjp_density_kernel<<<blocks, kBlock, 0, stream_>>>(dv);
     PHX_LAUNCH_CHECK();
     WcsphScatterArgs sa{};
@@ -631,6 +749,7 @@ void SphSolver::wcsph_vjp_step(const SphCache& cache, const phx_sph_state_adjoin
     sa.tension_a = wdw_.tension_a.data();
     sa.speed_a = wdw_.speed_a.data();
     sa.tait_a = wdw_.tait_a.data();
+    sa.btension_a = wdw_.btension_a.data();
     sa.out_pos = previous_a.particles.positions;
     sa.out_vel = previous_a.particles.velocities;
     sa.out_ctl = control_a.external_accelerations;
@@ -640,6 +759,7 @@ void SphSolver::wcsph_vjp_step(const SphCache& cache, const phx_sph_state_adjoin
     sa.out_tension = parameter_a.particles.surface_tensions;
     sa.out_speed = parameter_a.method.speed_of_sound;
     sa.out_tait = parameter_a.method.tait_exponent;
+    sa.out_btension = parameter_a.method.boundary_surface_tension;
     wcsph_vjp_scatter_kernel<<<blocks_for(n, kBlock), kBlock, 0, stream_>>>(sa);
     PHX_LAUNCH_CHECK();
 }

@@ -8,7 +8,7 @@ from pathlib import Path
 import pytest
 
 from physica_amd import _capi
-from physica_amd.particle import PCISPH, WCSPH, BoxBoundary, DFSPH, PBF, DomainConfiguration
+from physica_amd.particle import PCISPH, BoxBoundary, DFSPH, PBF, DomainConfiguration
 
 HEADER = Path(__file__).resolve().parent.parent / "include" / "physica_amd" / "physica.h"
 
@@ -82,15 +82,11 @@ def test_invalid_domain_rejected_before_device_work(domain, status, fragment):
 
 def test_boundary_particles_rejected_where_unsupported():
     sample = [((0.1, 0.1, 0.1), (0, 0, 0), 1e-6)]
-    # derivatives with boundary samples exist for DFSPH, PCISPH and IISPH;
-    # PBF and WCSPH refuse them before device work
+    # derivatives with boundary samples exist for the four SPH methods; PBF
+    # refuses them before device work
     h = ctypes.c_void_p()
     got = _capi.library().phx_pbf_create(ctypes.byref(cfg(boundary_particles=sample).to_c()),
                                          ctypes.byref(PBF.Configuration().to_c()), 1, None, ctypes.byref(h))
-    assert got == _capi.PHX_ERROR_UNSUPPORTED and h.value is None
-    assert "forward mode only" in _capi.library().phx_last_error().decode()
-    got = _capi.library().phx_sph_create(ctypes.byref(cfg(boundary_particles=sample).to_c()),
-                                         ctypes.byref(WCSPH.Configuration().to_c()), 1, None, ctypes.byref(h))
     assert got == _capi.PHX_ERROR_UNSUPPORTED and h.value is None
     assert "forward mode only" in _capi.library().phx_last_error().decode()
     # a non-zero count with a null host array

@@ -68,11 +68,8 @@ def test_rollout_with_boundary_bit_exact(gpu_device, method):
         solver.copy_state(nxt, state)
 
 
-def test_wcsph_boundary_particles_are_forward_only(gpu_device):
-    """PCISPH and IISPH differentiate their boundary terms (on the projection
-    engine); WCSPH's derivative passes have none and refuse samples."""
+def test_differentiable_mode_accepts_boundary_particles(gpu_device):
     sc = scene_for("wcsph")
     add_boundary_samples(sc)
-    with pytest.raises(RuntimeError, match="forward mode only"):
-        SPH(sc.domain, WCSPH.Configuration(), mode=ExecutionMode.differentiable, device=gpu_device)
+    SPH(sc.domain, WCSPH.Configuration(), mode=ExecutionMode.differentiable, device=gpu_device)
     SPH(sc.domain, PCISPH.Configuration(), mode=ExecutionMode.differentiable, device=gpu_device)

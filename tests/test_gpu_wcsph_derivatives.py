@@ -7,7 +7,9 @@ the VJP (FP64 adjoints of the FP32 primal) agrees with the oracle's to 1e-10
 of the largest entry of each output and the JVP (FP32) to 1e-6. With Tait
 exponents drawn in [1.5, 7] the platform's powf enters the primal, and both
 agree to 1e-5. The GPU's own JVP and VJP satisfy the dot-product identity.
-The scenes are those of the oracle's WCSPH derivative pins."""
+The scenes are those of the oracle's WCSPH derivative pins, one of them over
+a moving plane of boundary samples (pressure, artificial viscosity and
+cohesion terms with the boundary surface tension)."""
 import numpy as np
 import pytest
 import torch
@@ -22,8 +24,8 @@ pytestmark = pytest.mark.gpu
 METHOD_PARAMS = ("speed_of_sound", "tait_exponent", "boundary_surface_tension")
 
 
-def scene(unit_exponent, wall=False):
-    cs = setup(wall=wall)
+def scene(unit_exponent, wall=False, boundary=False):
+    cs = setup(wall=wall, boundary=boundary)
     if unit_exponent:
         cs.sc.parameters["tait_exponent"] = np.ones(cs.sc.count, dtype=np.float32)
     return cs
@@ -102,10 +104,12 @@ def test_differentiable_forward_equals_forward_mode():
     assert bits_equal(a[5].pressures().cpu().numpy(), cache.pressures().cpu().numpy())
 
 
-@pytest.mark.parametrize("unit_exponent,wall", [(True, False), (True, True), (False, False)])
-def test_jvp_and_vjp_match_the_oracle(unit_exponent, wall):
+@pytest.mark.parametrize("unit_exponent,wall,boundary",
+                         [(True, False, False), (True, True, False), (False, False, False), (True, False, True)])
+def test_jvp_and_vjp_match_the_oracle(unit_exponent, wall, boundary):
+    print(f"wcsph derivatives unit_exponent={unit_exponent} wall={wall} boundary={boundary}", flush=True)
     dev = torch.device("cuda:0")
-    cs = scene(unit_exponent, wall)
+    cs = scene(unit_exponent, wall, boundary)
     run = recorded(cs, dev, exact=unit_exponent)
     rng = np.random.default_rng(7 + 2 * wall + unit_exponent)
     t, w = directions(cs, rng)
@@ -117,10 +121,12 @@ def test_jvp_and_vjp_match_the_oracle(unit_exponent, wall):
     jw = run_vjp(*run, w)
     rw = oracle.wcsph_vjp(cs.d, 0, x, v, u, p, w, "f32")
     for k in STATE + PARAMS:
-        if k == "boundary_surface_tension":
+        if k == "boundary_surface_tension" and not boundary:
             # no boundary particles: the adjoint stays zero
             assert not np.any(jw[k]) and not np.any(rw[k])
             continue
+        if boundary and k in ("rest_densities", "boundary_surface_tension"):
+            assert np.abs(rw[k]).max() > 0, k  # the samples reach these adjoints
         assert close(jw[k], rw[k], 1e-10 if unit_exponent else 1e-5), k
 
 
