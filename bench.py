@@ -22,6 +22,11 @@ gradient with respect to the initial velocities (no parameter gradients). Ratio 
 rollout + chained VJPs) / plain forward rollout, with every step cache kept in
 HBM; the same with a state checkpoint every 10 steps and recompute is reported
 beside it.
+
+`batched_pbf` is the uneven batch: 64 independent PBF dam breaks (particle
+counts log-uniform in [4,096, 200,000] from seed 20260005, 3.79 M in all),
+100 steps each, every scene its own solver on one of 8 streams; then forward
+plus adjoint on the first 16 scenes against their plain forward.
 """
 from __future__ import annotations
 
@@ -53,6 +58,8 @@ def parse_args(argv=None):
     p.add_argument("--no-cpu-baseline", action="store_true", help="skip the CPU oracle timing")
     p.add_argument("--no-pass-timing", action="store_true", help="time without per-pass events")
     p.add_argument("--no-adjoint", action="store_true", help="skip the differentiable PBF rollout ratio")
+    p.add_argument("--no-batched", action="store_true", help="skip the 64-scene uneven PBF batch")
+    p.add_argument("--batch-streams", type=int, default=8, help="HIP streams the PBF batch's scenes share")
     p.add_argument("--adjoint-steps", type=int, default=120,
                    help="steps of the differentiable PBF rollout (120: the configured rollout)")
     return p.parse_args(argv)
@@ -169,17 +176,7 @@ def adjoint_ratio(device, steps=120):
     target = (sc.positions + np.array([[0.1], [-0.05], [0.08]]) + rng.normal(0, 0.01, size=(3, n))).astype(np.float64)
 
     def make(mode):
-        solver = PBF(sc.domain, PBF.Configuration(pressure_iterations=4), mode=mode, device=device)
-        state = solver.allocate_state()
-        state.particles.positions.upload(sc.positions)
-        state.particles.velocities.upload(sc.velocities)
-        params = solver.allocate_parameters()
-        for name in ("masses", "rest_densities", "viscosities", "surface_tensions"):
-            getattr(params.particles, name).copy_(torch.from_numpy(sc.parameters[name]))
-        for name in ("relaxation", "artificial_pressure_strength", "artificial_pressure_exponent",
-                     "artificial_pressure_radius", "xsph_viscosity", "vorticity_confinement"):
-            getattr(params, name).copy_(torch.from_numpy(sc.parameters[name]))
-        return solver, state, solver.allocate_control(), params
+        return pbf_solver_for(sc, mode, device)
 
     def timed(fn):
         torch.cuda.synchronize()
@@ -225,6 +222,104 @@ def adjoint_ratio(device, steps=120):
         rollout.release()
         del rollout, final, initial
     record["ratio"] = record["store_all"]["ratio"]
+    return record
+
+
+def pbf_solver_for(scene, mode, device):
+    """A PBF solver with `scene` uploaded: (solver, state, control, parameters)."""
+    import torch
+
+    from physica_amd.particle import PBF
+
+    solver = PBF(scene.domain, PBF.Configuration(pressure_iterations=4), mode=mode, device=device)
+    state = solver.allocate_state()
+    state.particles.positions.upload(scene.positions)
+    state.particles.velocities.upload(scene.velocities)
+    params = solver.allocate_parameters()
+    for name in ("masses", "rest_densities", "viscosities", "surface_tensions"):
+        getattr(params.particles, name).copy_(torch.from_numpy(scene.parameters[name]))
+    for name in ("relaxation", "artificial_pressure_strength", "artificial_pressure_exponent",
+                 "artificial_pressure_radius", "xsph_viscosity", "vorticity_confinement"):
+        getattr(params, name).copy_(torch.from_numpy(scene.parameters[name]))
+    return solver, state, solver.allocate_control(), params
+
+
+def batched_pbf(device, steps=100, adjoint_scenes=16, streams=8):
+    """The uneven PBF batch: 64 independent dam breaks (4,100 to 192,450
+    particles, 3.79 M in all), each with its own solver; scene k runs on
+    stream k mod `streams`, and the host issues the batch step by step, so
+    small scenes overlap. Then the gradient of a final-position loss with
+    respect to v0 on the first `adjoint_scenes` scenes (store-all rollouts,
+    one scene after another) against their plain forward timed the same way
+    (scene by scene); their overlapped plain forward is reported beside it."""
+    import torch
+
+    from physica_amd import scenes
+    from physica_amd.particle import ExecutionMode
+    from physica_amd.rollout import PbfRollout
+
+    batch = scenes.batched_pbf_scenes()
+    pool = [torch.cuda.Stream(device) for _ in range(streams)]
+    runs = []
+    for k, sc in enumerate(batch):
+        with torch.cuda.stream(pool[k % streams]):
+            solver, state, control, params = pbf_solver_for(sc, ExecutionMode.forward, device)
+            runs.append((solver, state, [solver.allocate_state(), solver.allocate_state()], control, params,
+                         solver.allocate_step_cache()))
+    torch.cuda.synchronize()
+
+    def forward(selected):
+        for solver, initial, states, _, _, _ in selected:
+            solver.copy_state(initial, states[0])
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for k in range(steps):
+            for solver, _, states, control, params, cache in selected:
+                solver.forward_step(states[k % 2], control, params, states[(k + 1) % 2], cache)
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) * 1e3
+
+    forward(runs)  # warm-up: allocations, code loading
+    batch_ms = forward(runs)
+    total = sum(sc.count for sc in batch)
+    record = {"workload": f"{len(batch)} uneven PBF dam breaks, {total} particles "
+                          f"({min(sc.count for sc in batch)} to {max(sc.count for sc in batch)}), "
+                          f"{steps} steps, dt 1/120, 4 iterations, {streams} streams",
+              "particles": total, "steps": steps, "forward_ms": round(batch_ms, 2),
+              "particle_steps_per_s": round(total * steps / (batch_ms * 1e-3), 1)}
+    overlapped_ms = forward(runs[:adjoint_scenes])
+    # the rollouts below run one scene after another, so their plain forward is timed that way too
+    plain_ms = sum(forward([run]) for run in runs[:adjoint_scenes])
+    adjoint_particles = sum(sc.count for sc in batch[:adjoint_scenes])
+    del runs
+    fwd_ms = bwd_ms = 0.0
+    for sc in batch[:adjoint_scenes]:
+        solver, state, control, params = pbf_solver_for(sc, ExecutionMode.differentiable, device)
+        rollout = PbfRollout(solver, steps, checkpoint_every=1, parameter_gradients=False)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        final = rollout.forward(state, control, params)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        target = sc.positions.astype(np.float64) + np.array([[0.1], [-0.05], [0.08]])
+        seed = solver.allocate_state_adjoint()
+        seed.particles.positions.upload(2.0 * (final.particles.positions.numpy().astype(np.float64) - target)
+                                        / sc.count)
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        initial, _, _ = rollout.backward(seed)
+        torch.cuda.synchronize()
+        t3 = time.perf_counter()
+        if not np.isfinite(initial.particles.velocities.numpy()).all():
+            raise RuntimeError("non-finite batch gradient")
+        fwd_ms += (t1 - t0) * 1e3
+        bwd_ms += (t3 - t2) * 1e3
+        rollout.release()
+        del rollout, final, initial, solver
+    record["adjoint"] = {"scenes": adjoint_scenes, "particles": adjoint_particles,
+                         "plain_forward_overlapped_ms": round(overlapped_ms, 2),
+                         "plain_forward_ms": round(plain_ms, 2), "forward_ms": round(fwd_ms, 2),
+                         "backward_ms": round(bwd_ms, 2), "ratio": round((fwd_ms + bwd_ms) / plain_ms, 3)}
     return record
 
 
@@ -365,6 +460,9 @@ def main(argv=None) -> int:
     if not args.no_adjoint:
         del states, cache, solver
         adjoint = adjoint_ratio(device, steps=args.adjoint_steps)
+    batched = None
+    if not args.no_batched:
+        batched = batched_pbf(device, streams=args.batch_streams)
 
     line = {
         "metric": METRIC,
@@ -386,6 +484,7 @@ def main(argv=None) -> int:
         "algorithmic_bytes_per_particle_step": round(bytes_step, 2),
         "fwd_adjoint_ratio": None if adjoint is None else adjoint["ratio"],
         "fwd_adjoint": adjoint,
+        "batched_pbf": batched,
         "fixed_iterations": fixed,
         "passes": [{k: (round(v, 4) if isinstance(v, float) else v) for k, v in p.items()} for p in passes],
     }

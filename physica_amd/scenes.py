@@ -12,7 +12,7 @@ position origin + index * spacing in FP32, masses rho0 * spacing^3.
 from __future__ import annotations
 
 from dataclasses import dataclass, field
-from typing import Dict, Tuple
+from typing import Dict, List, Tuple
 
 import numpy as np
 
@@ -156,6 +156,33 @@ def pbf_dam_break(shape=(32, 32, 32), spacing=0.05, seed=20260001, box=(3.2, 2.0
         "vorticity_confinement": np.zeros(n, dtype=np.float32),
     }
     return Scene("pbf_dam_break", domain, pos, np.zeros_like(pos), params, {"pressure_iterations": 4})
+
+
+def batched_pbf_scenes(count=64, seed=20260005, low=4096, high=200000, spacing=0.02) -> List[Scene]:
+    """The uneven PBF batch: `count` independent dam breaks whose particle
+    counts are N_k = floor(exp(U(ln low, ln high))), U the k-th draw of the
+    SplitMix64 stream `seed`. Scene k is a cube lattice of side
+    ceil(N_k^(1/3)) at `spacing` (jitter stream seed + 1 + k), truncated to
+    its first N_k particles, in a box twice the block's extent, with h =
+    2 spacing and pbf_dam_break's parameters."""
+    u = uniform(seed, count).astype(np.float64)
+    counts = np.floor(np.exp(np.log(low) + u * (np.log(high) - np.log(low)))).astype(np.int64)
+    out = []
+    for k, n in enumerate(counts.tolist()):
+        side = max(1, int(round(n ** (1.0 / 3.0))))
+        while side ** 3 < n:
+            side += 1
+        while side > 1 and (side - 1) ** 3 >= n:
+            side -= 1
+        extent = 2.0 * side * spacing
+        sc = pbf_dam_break(shape=(side, side, side), spacing=spacing, seed=seed + 1 + k, box=(extent,) * 3)
+        sc.name = f"pbf_batch_{k}"
+        sc.positions = np.ascontiguousarray(sc.positions[:, :n])
+        sc.velocities = np.ascontiguousarray(sc.velocities[:, :n])
+        sc.parameters = {name: np.ascontiguousarray(v[:n]) for name, v in sc.parameters.items()}
+        sc.domain.particle_count = int(n)
+        out.append(sc)
+    return out
 
 
 def two_cubes(method="iisph", shape=(128, 128, 128), spacing=0.01, seed=20260003, box=(4.0, 4.0, 2.0),

@@ -39,3 +39,24 @@ def test_benchmark_scenes():
     p = scenes.pbf_dam_break()
     assert p.count == 32768
     assert p.domain.grid_resolution() == (32, 20, 16)
+
+
+def test_batched_pbf_scenes():
+    """64 uneven dam breaks: counts log-uniform in [4096, 200000] from the
+    seeded stream, cube lattices truncated to the count, boxes twice the block."""
+    batch = scenes.batched_pbf_scenes()
+    assert len(batch) == 64
+    counts = np.array([sc.count for sc in batch])
+    assert counts.min() >= 4096 and counts.max() < 200000
+    assert counts.sum() == 3789607  # fixed by the seed
+    again = scenes.batched_pbf_scenes()
+    for a, b in zip(batch, again):
+        assert np.array_equal(a.positions, b.positions)
+    for sc in batch[:8]:
+        n = sc.count
+        side = round(float(sc.domain.boundary.maximum[0]) / (2 * 0.02))
+        assert (side - 1) ** 3 < n <= side ** 3
+        assert sc.positions.shape == (3, n) and sc.velocities.shape == (3, n)
+        assert all(v.shape == (n,) for v in sc.parameters.values())
+        assert sc.domain.support_radius == 2 * 0.02
+        assert (sc.positions > 0).all() and (sc.positions < float(sc.domain.boundary.maximum[0])).all()
