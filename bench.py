@@ -25,7 +25,7 @@ beside it.
 
 `batched_pbf` is the uneven batch: 64 independent PBF dam breaks (particle
 counts log-uniform in [4,096, 200,000] from seed 20260005, 3.79 M in all),
-100 steps each, every scene its own solver on one of 8 streams; then forward
+100 steps each, every scene its own solver on one of 32 streams; then forward
 plus adjoint on the first 16 scenes against their plain forward.
 """
 from __future__ import annotations
@@ -59,7 +59,7 @@ def parse_args(argv=None):
     p.add_argument("--no-pass-timing", action="store_true", help="time without per-pass events")
     p.add_argument("--no-adjoint", action="store_true", help="skip the differentiable PBF rollout ratio")
     p.add_argument("--no-batched", action="store_true", help="skip the 64-scene uneven PBF batch")
-    p.add_argument("--batch-streams", type=int, default=8, help="HIP streams the PBF batch's scenes share")
+    p.add_argument("--batch-streams", type=int, default=32, help="HIP streams the PBF batch's scenes share")
     p.add_argument("--adjoint-steps", type=int, default=120,
                    help="steps of the differentiable PBF rollout (120: the configured rollout)")
     return p.parse_args(argv)
@@ -244,7 +244,7 @@ def pbf_solver_for(scene, mode, device):
     return solver, state, solver.allocate_control(), params
 
 
-def batched_pbf(device, steps=100, adjoint_scenes=16, streams=8):
+def batched_pbf(device, steps=100, adjoint_scenes=16, streams=32):
     """The uneven PBF batch: 64 independent dam breaks (4,100 to 192,450
     particles, 3.79 M in all), each with its own solver; scene k runs on
     stream k mod `streams`, and the host issues the batch step by step, so
