@@ -14,7 +14,7 @@
  *   phx_build_neighborhood     NeighborhoodSearch::build neighborhood/neighborhood.cpp:44-89
  *                              cuda_detail::build_neighborhood neighborhood/neighborhood.cu:54-70
  *   phx_density_forward        density::sph_forward/pbf_forward density/density.cpp:12-34,
- *                              cuda_detail::density_forward density/density.cu:196-198
+ *                              cuda_detail::density_forward density/density.cu:104-106
  *   phx_dfsph_*                SPH<DFSPH> sph/sph.ixx:273-372, DFSPH sph/sph.ixx:195-268,
  *                              DFSPH::forward sph/sph.cpp:572-610
  *   phx_pbf_*                  PBF pbf/pbf.ixx:14-141, PBF::forward_step pbf/pbf.cpp:167-191
