@@ -594,7 +594,11 @@ inline void pbf_vjp(const Domain& domain, const VectorField& topology_positions,
 
 // sph/sph.ixx:39-193: the method descriptions. States of the three are empty.
 struct WCSPH {
-    struct Configuration {};
+    struct Configuration {
+        // VJP of a no-slip box collision: false (default) the reference's
+        // (the whole velocity adjoint zeroed), true the exact transpose
+        bool exact_no_slip_adjoint{false};
+    };
     struct State {};
     struct StateTangent {};
     struct StateAdjoint {};
@@ -622,6 +626,9 @@ struct PCISPH {
         std::uint32_t checkpoint_interval{2u};
         float density_error_tolerance{0.0F};
         std::uint32_t max_iterations{100u};
+        // VJP of a no-slip box collision: false (default) the reference's
+        // (the whole velocity adjoint zeroed), true the exact transpose
+        bool exact_no_slip_adjoint{false};
     };
     struct State {};
     struct StateTangent {};
@@ -644,6 +651,9 @@ struct IISPH {
         std::uint32_t checkpoint_interval{2u};
         float density_error_tolerance{0.0F};
         std::uint32_t max_iterations{100u};
+        // VJP of a no-slip box collision: false (default) the reference's
+        // (the whole velocity adjoint zeroed), true the exact transpose
+        bool exact_no_slip_adjoint{false};
     };
     struct State {};
     struct StateTangent {};
@@ -670,6 +680,9 @@ struct DFSPH {
         bool pressure_warm_start{true};
         float density_error_tolerance{0.0F};
         std::uint32_t max_iterations{100u};
+        // VJP of a no-slip box collision: false (default) the reference's
+        // (the whole velocity adjoint zeroed), true the exact transpose
+        bool exact_no_slip_adjoint{false};
     };
     struct State {
         ScalarField warm_divergence_pressure;
@@ -775,6 +788,7 @@ class SPH<DFSPH> {
         c.pressure_warm_start = method_configuration.pressure_warm_start ? 1u : 0u;
         c.density_error_tolerance = method_configuration.density_error_tolerance;
         c.max_iterations = method_configuration.max_iterations;
+        c.exact_no_slip_adjoint = method_configuration.exact_no_slip_adjoint ? 1u : 0u;
         check(phx_dfsph_create(&d, &c, static_cast<phx_execution_mode>(mode), stream, &solver_), "SPH<DFSPH>");
     }
     SPH(const SPH&) = delete;
@@ -923,8 +937,8 @@ struct SphMethod;
 
 template <>
 struct SphMethod<WCSPH> {
-    static phx_sph_configuration configuration(const WCSPH::Configuration&) {
-        return {PHX_SPH_WCSPH, 0u, 2u, 0.0F, 0u};
+    static phx_sph_configuration configuration(const WCSPH::Configuration& c) {
+        return {PHX_SPH_WCSPH, 0u, 2u, 0.0F, 0u, c.exact_no_slip_adjoint ? 1u : 0u};
     }
     template <class Buffers>
     static Buffers allocate(hipStream_t stream, std::size_t n) {
@@ -951,7 +965,8 @@ struct SphMethod<WCSPH> {
 template <>
 struct SphMethod<PCISPH> {
     static phx_sph_configuration configuration(const PCISPH::Configuration& c) {
-        return {PHX_SPH_PCISPH, c.pressure_iterations, c.checkpoint_interval, c.density_error_tolerance, c.max_iterations};
+        return {PHX_SPH_PCISPH,          c.pressure_iterations, c.checkpoint_interval, c.density_error_tolerance,
+                c.max_iterations,        c.exact_no_slip_adjoint ? 1u : 0u};
     }
     template <class Buffers>
     static Buffers allocate(hipStream_t stream, std::size_t n) {
@@ -976,7 +991,8 @@ struct SphMethod<PCISPH> {
 template <>
 struct SphMethod<IISPH> {
     static phx_sph_configuration configuration(const IISPH::Configuration& c) {
-        return {PHX_SPH_IISPH, c.pressure_iterations, c.checkpoint_interval, c.density_error_tolerance, c.max_iterations};
+        return {PHX_SPH_IISPH,           c.pressure_iterations, c.checkpoint_interval, c.density_error_tolerance,
+                c.max_iterations,        c.exact_no_slip_adjoint ? 1u : 0u};
     }
     template <class Buffers>
     static Buffers allocate(hipStream_t stream, std::size_t n) {

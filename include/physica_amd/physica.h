@@ -275,6 +275,12 @@ typedef struct phx_dfsph_configuration {
     uint32_t pressure_warm_start;   /* reference default 1 */
     float density_error_tolerance;  /* 0: fixed iterations */
     uint32_t max_iterations;
+    /* VJP of a prediction whose no-slip box collision resets the velocity:
+     * 0 (default) the reference's predict_vjp_kernel (sph/dfsph.cu:68-102,
+     * line 81), which zeroes the whole velocity adjoint and with it the
+     * dt * position adjoint of the components that did not collide; 1 the
+     * exact transpose of the JVP (those terms kept). */
+    uint32_t exact_no_slip_adjoint;
 } phx_dfsph_configuration;
 
 /* DFSPH::Parameters and DFSPH::State, sph/sph.ixx:202-217 */
@@ -424,6 +430,11 @@ typedef struct phx_sph_configuration {
     uint32_t checkpoint_interval;  /* PCISPH, IISPH; reference default 2 */
     float density_error_tolerance; /* PCISPH, IISPH; 0: fixed iterations */
     uint32_t max_iterations;
+    /* as phx_dfsph_configuration: 0 (default) the reference's VJP of a
+     * no-slip collision (predict_vjp_kernel, sph/pcisph.cu:68-102 and
+     * sph/iisph.cu:68-102; integrate_vjp_kernel, sph/common.cu:344-366 for
+     * WCSPH), 1 the exact transpose */
+    uint32_t exact_no_slip_adjoint;
 } phx_sph_configuration;
 
 /* Method parameters, one value per particle (sph/sph.ixx:45-49, 112, 158).

@@ -134,6 +134,7 @@ class DfsphConfiguration(ctypes.Structure):
         ("pressure_warm_start", c_uint32),
         ("density_error_tolerance", c_float),
         ("max_iterations", c_uint32),
+        ("exact_no_slip_adjoint", c_uint32),
     ]
 
 
@@ -220,7 +221,8 @@ PHX_SPH_WCSPH, PHX_SPH_PCISPH, PHX_SPH_IISPH = 0, 1, 2
 
 class SphConfiguration(ctypes.Structure):
     _fields_ = [("method", c_int), ("pressure_iterations", c_uint32), ("checkpoint_interval", c_uint32),
-                ("density_error_tolerance", c_float), ("max_iterations", c_uint32)]
+                ("density_error_tolerance", c_float), ("max_iterations", c_uint32),
+                ("exact_no_slip_adjoint", c_uint32)]
 
 
 class SphMethodParameters(ctypes.Structure):

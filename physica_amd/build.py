@@ -6,6 +6,7 @@ repository snapshot.
 """
 from __future__ import annotations
 
+import concurrent.futures
 import os
 import shutil
 import subprocess
@@ -51,16 +52,23 @@ def build_library(verbose: bool = False) -> Path:
     """Compile every HIP source for gfx950 and link libphysica_amd.so."""
     objdir = ROOT / "_build" / "physica_amd"
     objdir.mkdir(parents=True, exist_ok=True)
-    objects = []
     include = ["-I", str(ROOT / "include"), "-I", str(CSRC)]
+    commands = []
     for name in SOURCES:
         src = CSRC / name
         obj = objdir / (src.stem + ".o")
-        cmd = [hipcc(), *HIPCC_FLAGS, *include, "-c", str(src), "-o", str(obj)]
+        commands.append([hipcc(), *HIPCC_FLAGS, *include, "-c", str(src), "-o", str(obj)])
+    objects = [cmd[-1] for cmd in commands]
+
+    def compile_one(cmd):
         if verbose:
             print(" ".join(cmd), flush=True)
         subprocess.run(cmd, check=True)
-        objects.append(str(obj))
+
+    workers = max(1, min(len(commands), int(os.environ.get("PHX_BUILD_JOBS", "4"))))
+    with concurrent.futures.ThreadPoolExecutor(max_workers=workers) as pool:
+        for future in [pool.submit(compile_one, cmd) for cmd in commands]:
+            future.result()
     tmp = LIBRARY.with_suffix(".so.tmp")
     cmd = [hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", *objects, "-o", str(tmp)]
     if verbose:

@@ -106,14 +106,17 @@ __device__ __forceinline__ void predict_tangent(uint32_t bits, uint32_t no_slip,
     out_v = vd;
 }
 
-// Transpose of predict_tangent: position and velocity adjoints from the
-// adjoints xa, va of the predicted position and velocity (dfsph.cu:68-102,
-// exact transpose under no-slip collisions).
-__device__ __forceinline__ void predict_adjoint(uint32_t bits, uint32_t no_slip, double dt, D3 xa, D3 va,
-                                                D3& position, D3& velocity) {
+// VJP of the prediction: position and velocity adjoints from the adjoints
+// xa, va of the predicted position and velocity (dfsph.cu:68-102). Under a
+// no-slip collision the reference zeroes the whole velocity adjoint
+// (dfsph.cu:81, common.cu:357); with `exact` the dt * xa terms of the
+// components that did not collide are kept, which makes it the transpose
+// of predict_tangent.
+__device__ __forceinline__ void predict_adjoint(uint32_t bits, uint32_t no_slip, uint32_t exact, double dt, D3 xa,
+                                                D3 va, D3& position, D3& velocity) {
     position = D3{bits & 1u ? 0.0 : xa.x, bits & 2u ? 0.0 : xa.y, bits & 4u ? 0.0 : xa.z};
     if (no_slip != 0u && bits != 0u) {
-        velocity = scale(position, dt);
+        velocity = exact != 0u ? scale(position, dt) : D3{0.0, 0.0, 0.0};
     } else {
         velocity = add(va, scale(position, dt));
         if (bits & 1u) velocity.x = 0.0;

@@ -251,6 +251,50 @@ size_t neighborhood_workspace_bytes(const phx::Domain& d) {
     return bytes;
 }
 
+
+// Shared by phx_build_neighborhood and its no-allocation variant.
+void check_neighborhood_outputs(const phx::Domain& d, const phx_const_vector_view& positions,
+                                const uint32_t* sorted_particle_indices, const uint32_t* cell_offsets) {
+    phx::require(cell_offsets != nullptr, PHX_ERROR_INVALID_ARGUMENT, "null cell_offsets");
+    if (d.config.particle_count > 0) {
+        phx::require_vec(positions, "positions");
+        phx::require(sorted_particle_indices != nullptr, PHX_ERROR_INVALID_ARGUMENT, "null sorted_particle_indices");
+    }
+}
+
+phx::GridGeometry build_fluid_neighborhood(hipStream_t s, const phx::Domain& d, uint64_t step_index,
+                                           const phx_const_vector_view& positions, const phx::GridScratch& scratch,
+                                           uint64_t* sorted_keys, uint32_t* sorted_particle_indices,
+                                           uint32_t* cell_offsets) {
+    phx::ChannelList none;
+    const phx::GridGeometry geo = phx::build_grid(s, d, step_index, phx::cvec(positions), nullptr, none.get(),
+                                                  scratch, sorted_particle_indices, cell_offsets);
+    if (sorted_keys != nullptr && d.config.particle_count > 0)
+        phx::export_sorted_keys(s, sorted_particle_indices, scratch.keys, d.config.particle_count, sorted_keys);
+    return geo;
+}
+
+void fill_neighborhood_view(const phx::Domain& d, const phx::GridGeometry& geo, uint64_t* sorted_keys,
+                            uint32_t* sorted_particle_indices, uint32_t* cell_offsets, uint64_t* sorted_boundary_keys,
+                            uint32_t* sorted_boundary_indices, uint32_t* boundary_cell_offsets,
+                            phx_neighborhood_view* view) {
+    view->sorted_keys = sorted_keys;
+    view->sorted_particle_indices = sorted_particle_indices;
+    view->cell_offsets = cell_offsets;
+    view->sorted_boundary_keys = sorted_boundary_keys;
+    view->sorted_boundary_indices = sorted_boundary_indices;
+    view->boundary_cell_offsets = boundary_cell_offsets;
+    view->particle_count = d.config.particle_count;
+    view->boundary_count = d.config.boundary_particle_count;
+    view->cells_x = geo.resolution[0];
+    view->cells_y = geo.resolution[1];
+    view->cells_z = geo.resolution[2];
+    view->origin_x = geo.origin[0];
+    view->origin_y = geo.origin[1];
+    view->origin_z = geo.origin[2];
+    view->cell_size = geo.cell_size;
+}
+
 }  // namespace
 
 extern "C" {
@@ -313,11 +357,7 @@ phx_status phx_build_neighborhood(phx_stream stream, const phx_domain_configurat
         const uint32_t n = d.config.particle_count;
         const uint32_t b = d.config.boundary_particle_count;
         const hipStream_t s = stream_of(stream);
-        phx::require(cell_offsets != nullptr, PHX_ERROR_INVALID_ARGUMENT, "null cell_offsets");
-        if (n > 0) {
-            phx::require_vec(positions, "positions");
-            phx::require(sorted_particle_indices != nullptr, PHX_ERROR_INVALID_ARGUMENT, "null sorted_particle_indices");
-        }
+        check_neighborhood_outputs(d, positions, sorted_particle_indices, cell_offsets);
         phx::require(workspace != nullptr, PHX_ERROR_INVALID_ARGUMENT, "null workspace");
         phx::require(workspace_bytes >= neighborhood_workspace_bytes(d), PHX_ERROR_WORKSPACE_TOO_SMALL,
                      "neighbourhood workspace too small");
@@ -325,11 +365,9 @@ phx_status phx_build_neighborhood(phx_stream stream, const phx_domain_configurat
         auto* base = static_cast<uint8_t*>(workspace);
         const size_t grid_bytes = round256(phx::grid_scratch_bytes(n, d.cells));
         const phx::GridScratch scratch = phx::carve_grid_scratch(base, grid_bytes, n, d.cells);
-        phx::ChannelList none;
-        const phx::GridGeometry geo = phx::build_grid(s, d, step_index, phx::cvec(positions), nullptr, none.get(),
-                                                      scratch, sorted_particle_indices, cell_offsets);
-        if (sorted_keys != nullptr && n > 0)
-            phx::export_sorted_keys(s, sorted_particle_indices, scratch.keys, n, sorted_keys);
+        const phx::GridGeometry geo =
+            build_fluid_neighborhood(s, d, step_index, positions, scratch, sorted_keys, sorted_particle_indices,
+                                     cell_offsets);
         if (b > 0u) {
             uint8_t* p = base + grid_bytes;
             const size_t scratch_bytes = round256(phx::boundary_scratch_bytes(b));
@@ -354,23 +392,9 @@ phx_status phx_build_neighborhood(phx_stream stream, const phx_domain_configurat
         } else if (boundary_cell_offsets != nullptr) {
             phx::fill_u32(s, boundary_cell_offsets, static_cast<size_t>(d.cells) + 1u, 0u);
         }
-        if (view != nullptr) {
-            view->sorted_keys = sorted_keys;
-            view->sorted_particle_indices = sorted_particle_indices;
-            view->cell_offsets = cell_offsets;
-            view->sorted_boundary_keys = sorted_boundary_keys;
-            view->sorted_boundary_indices = sorted_boundary_indices;
-            view->boundary_cell_offsets = boundary_cell_offsets;
-            view->particle_count = n;
-            view->boundary_count = b;
-            view->cells_x = geo.resolution[0];
-            view->cells_y = geo.resolution[1];
-            view->cells_z = geo.resolution[2];
-            view->origin_x = geo.origin[0];
-            view->origin_y = geo.origin[1];
-            view->origin_z = geo.origin[2];
-            view->cell_size = geo.cell_size;
-        }
+        if (view != nullptr)
+            fill_neighborhood_view(d, geo, sorted_keys, sorted_particle_indices, cell_offsets, sorted_boundary_keys,
+                                   sorted_boundary_indices, boundary_cell_offsets, view);
     });
 }
 

@@ -82,6 +82,7 @@ void DfsphCache::allocate(uint32_t n, uint32_t cells, uint32_t divergence_iterat
     collisions.resize((updates + 1u) * nn);
     exact.resize(static_cast<size_t>(kStencil) * nn);
     skin.resize(static_cast<size_t>(kStencil) * nn);
+    crowded.resize(nn);
     mask_flags.resize(kFlagDisplacement + updates + 1u);
 }
 
@@ -187,6 +188,7 @@ void DfsphSolver::forward_step(const phx_sph_dfsph_state& state, const phx_contr
     float* p_den0 = rec ? cache.pressures(1u, 0u) : p_den_.data();
     uint16_t* exact = rec ? cache.exact.data() : masks_.exact();
     uint16_t* skin = rec ? cache.skin.data() : masks_.skin_mask();
+    uint8_t* crowded = rec ? cache.crowded.data() : masks_.crowded();
 
     // neighborhood.build(step, x) with every input gathered into cell order
     ChannelList gather;
@@ -232,6 +234,7 @@ void DfsphSolver::forward_step(const phx_sph_dfsph_state& state, const phx_contr
                        t_den_.data(),
                        masks_.flags(),
                        rho0};
+    ma.crowded = crowded;
     launch_mask_density<false>(profiler_, stream_, blocks, ma, kDfsphDensity);
 
     ForceArgs fa{};
@@ -258,6 +261,7 @@ void DfsphSolver::forward_step(const phx_sph_dfsph_state& state, const phx_contr
     fa.warm_den_out = next.method.warm_density_pressure;
     fa.boundary = boundary;
     fa.rest_density = rho0;
+    fa.crowded = crowded;
     // prediction `slot` writes x* (recorded per slot) and, recorded, its collision bits
     auto set_slot = [&](uint32_t slot) {
         fa.slot = slot;
@@ -279,6 +283,7 @@ void DfsphSolver::forward_step(const phx_sph_dfsph_state& state, const phx_contr
     pa.volume_density = volume_rest_ ? rho0 : cache.densities.data();
     pa.boundary = boundary;
     pa.rest_density = rho0;
+    pa.crowded = crowded;
     // update `it` of phase `phase` reads the x* of prediction `slot`
     auto set_update = [&](uint32_t phase, uint32_t it, uint32_t slot) {
         pa.slot = slot;

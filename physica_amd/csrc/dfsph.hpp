@@ -37,6 +37,7 @@ struct DfsphCache {
     // next state)
     DeviceBuffer<uint8_t> collisions;
     DeviceBuffer<uint16_t> exact, skin;  // the step's neighbour masks and flags
+    DeviceBuffer<uint8_t> crowded;       // per particle: walks the full stencil
     DeviceBuffer<uint32_t> mask_flags;
     float mask_limit2 = -1.0F;
 

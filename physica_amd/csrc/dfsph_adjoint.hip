@@ -60,6 +60,7 @@ struct Step {
     float gradient_norm;
     double pi_h6;  // double(pi) * pow(double(h), 6)
     Box box;
+    uint32_t exact_no_slip;  // 1: exact transpose of a no-slip collision, 0: the reference's VJP
     SortedGrid grid;
     const float4* xm;
     const float4* vmu;
@@ -68,13 +69,21 @@ struct Step {
     const uint16_t* skin;
     const uint32_t* flags;
     float limit2;
-    BoundaryGrid boundary;  // offsets null: no boundary samples
-    const float* rest;      // rho0 (the boundary terms)
+    BoundaryGrid boundary;   // offsets null: no boundary samples
+    const float* rest;       // rho0 (the boundary terms)
+    const uint8_t* crowded;  // per particle: walks the full stencil
 };
 
-__device__ __forceinline__ bool exact_masked(const Step& s) { return s.flags[kFlagOverflow] == 0u; }
-__device__ __forceinline__ MaskView exact_view(const Step& s) { return MaskView{s.exact, s.flags, s.n, s.limit2}; }
-__device__ __forceinline__ MaskView skin_view(const Step& s) { return MaskView{s.skin, s.flags, s.n, s.limit2}; }
+// Whether particle i walks the step's exact masks (not crowded, no boundary samples).
+__device__ __forceinline__ bool exact_masked(const Step& s, uint32_t i) {
+    return s.flags[kFlagOverflow] == 0u && s.crowded[i] == 0u;
+}
+__device__ __forceinline__ MaskView exact_view(const Step& s) {
+    return MaskView{s.exact, s.flags, s.n, s.limit2, 0u, s.crowded};
+}
+__device__ __forceinline__ MaskView skin_view(const Step& s) {
+    return MaskView{s.skin, s.flags, s.n, s.limit2, 0u, s.crowded};
+}
 
 // Visits the fluid candidates of particle i (through the masks where they
 // apply) or, with boundary samples, every fluid particle and every sample of
@@ -150,7 +159,7 @@ __global__ void __launch_bounds__(kBlock) dfsph_jvp_density_kernel(DensityJvpArg
     const float rest_i = s.rest[i];
     const float rest_t_i = a.rest_t[i];
     walk(
-        s, exact_view(s), exact_masked(s), i, cx, cy, cz, delta, smask + threadIdx.x,
+        s, exact_view(s), exact_masked(s, i), i, cx, cy, cz, delta, smask + threadIdx.x,
         [&](uint32_t j) {
             const float4 q = s.xm[j];
             const float4 qt = a.xt[j];
@@ -233,7 +242,7 @@ __global__ void __launch_bounds__(kBlock) dfsph_jvp_force_kernel(ForceJvpArgs a)
         const float factor_t = -vol * (rest_t_i * t.x + rest_i * t.y);
         ap = add(ap, add(scale(gradient, factor_t), scale(gradient_t, factor)));
     };
-    walk(s, exact_view(s), exact_masked(s), i, cx, cy, cz, delta, smask + threadIdx.x,
+    walk(s, exact_view(s), exact_masked(s, i), i, cx, cy, cz, delta, smask + threadIdx.x,
                      [&](uint32_t j) {
                          if (j == i) return;
                          const float4 q = s.xm[j];
@@ -353,7 +362,7 @@ __global__ void __launch_bounds__(kBlock) dfsph_jvp_projection_kernel(Projection
     const float rest_t_i = a.rest_t[i];
     const MaskView mv = skin_view(s);
     walk(
-        s, mv, masks_apply(mv, a.slot), i, cx, cy, cz, delta, smask + threadIdx.x,
+        s, mv, masks_apply(mv, a.slot, i), i, cx, cy, cz, delta, smask + threadIdx.x,
         [&](uint32_t j) {
             const float4 q = a.xs[j];
             const Float3 d = sub(ps, xyz(q));
@@ -410,7 +419,8 @@ __global__ void __launch_bounds__(kBlock) dfsph_vjp_final_kernel(FinalVjpArgs a)
     const D3 xa{a.next_x_a.x[j], a.next_x_a.y[j], a.next_x_a.z[j]};
     const D3 va{a.next_v_a.x[j], a.next_v_a.y[j], a.next_v_a.z[j]};
     D3 position, velocity;
-    predict_adjoint(a.collisions[i], a.s.box.no_slip, static_cast<double>(a.s.dt), xa, va, position, velocity);
+    predict_adjoint(a.collisions[i], a.s.box.no_slip, a.s.exact_no_slip, static_cast<double>(a.s.dt), xa, va, position,
+                    velocity);
     const D3 acc = scale(velocity, static_cast<double>(a.s.dt));
     store(a.pos_a, i, position);
     store(a.vel_a, i, velocity);
@@ -488,7 +498,7 @@ __global__ void __launch_bounds__(kBlock) dfsph_vjp_pressure_kernel(PressureVjpA
     const float rest_i = s.rest[i];
     int cx, cy, cz;
     particle_cell(s.grid, x, cx, cy, cz);
-    walk(s, exact_view(s), exact_masked(s), i, cx, cy, cz, delta, smask + threadIdx.x,
+    walk(s, exact_view(s), exact_masked(s, i), i, cx, cy, cz, delta, smask + threadIdx.x,
                      [&](uint32_t j) {
                          if (j == i) return;
                          const float4 q = s.xm[j];
@@ -628,7 +638,7 @@ __global__ void __launch_bounds__(kBlock) dfsph_vjp_density_kernel(DensityVjpArg
     const float rest_i = s.rest[i];
     const MaskView mv = skin_view(s);
     walk(
-        s, mv, masks_apply(mv, a.slot), i, cx, cy, cz, delta, smask + threadIdx.x,
+        s, mv, masks_apply(mv, a.slot, i), i, cx, cy, cz, delta, smask + threadIdx.x,
         [&](uint32_t j) {
             const float4 q = a.xs[j];
             const Float3 d = sub(ps, xyz(q));
@@ -648,8 +658,8 @@ __global__ void __launch_bounds__(kBlock) dfsph_vjp_density_kernel(DensityVjpArg
     a.m_a[i] += mass_c;
     if (s.boundary.offsets != nullptr) a.rest_a[i] += rest_c;
     D3 pos, velocity;
-    predict_adjoint(a.collisions[i], s.box.no_slip, static_cast<double>(s.dt), position, D3{0.0, 0.0, 0.0}, pos,
-                    velocity);
+    predict_adjoint(a.collisions[i], s.box.no_slip, s.exact_no_slip, static_cast<double>(s.dt), position,
+                    D3{0.0, 0.0, 0.0}, pos, velocity);
     const D3 acc = scale(velocity, static_cast<double>(s.dt));
     accumulate(a.pos_a, i, pos);
     accumulate(a.vel_a, i, velocity);
@@ -687,7 +697,7 @@ __global__ void __launch_bounds__(kBlock) dfsph_vjp_scatter_kernel(ScatterVjpArg
     double rest_c = 0.0;
     const float rest_i = s.rest[i];
     walk(
-        s, exact_view(s), exact_masked(s), i, cx, cy, cz, delta, smask + threadIdx.x,
+        s, exact_view(s), exact_masked(s, i), i, cx, cy, cz, delta, smask + threadIdx.x,
         [&](uint32_t j) {
             const float4 q = s.xm[j];
             const Float3 d = sub(x, xyz(q));
@@ -743,7 +753,8 @@ void DfsphSolver::require_recorded(const DfsphCache& cache) const {
 
 namespace {
 
-Step make_step(const Domain& domain, const DfsphCache& cache, const KernelConstants& kc, float gradient_norm) {
+Step make_step(const Domain& domain, const DfsphCache& cache, const KernelConstants& kc, float gradient_norm,
+               bool exact_no_slip) {
     const auto& c = domain.config;
     Step s{};
     s.n = c.particle_count;
@@ -752,6 +763,7 @@ Step make_step(const Domain& domain, const DfsphCache& cache, const KernelConsta
     s.gradient_norm = gradient_norm;
     s.pi_h6 = static_cast<double>(kPi) * std::pow(static_cast<double>(c.support_radius), 6.0);
     s.box = collision_box(c, cache.step_index);
+    s.exact_no_slip = exact_no_slip ? 1u : 0u;
     s.grid = cache.neighborhood.grid();
     s.xm = cache.xm.data();
     s.vmu = cache.vmu.data();
@@ -762,6 +774,7 @@ Step make_step(const Domain& domain, const DfsphCache& cache, const KernelConsta
     s.limit2 = cache.mask_limit2;
     s.boundary = cache.neighborhood.boundary();
     s.rest = cache.rho0.data();
+    s.crowded = cache.crowded.data();
     return s;
 }
 
@@ -791,7 +804,7 @@ void DfsphSolver::jvp_step(const DfsphCache& cache, const phx_sph_dfsph_state_ta
     const bool warm = cache.warm_start;
     const uint32_t* perm = cache.neighborhood.perm.data();
     const uint32_t blocks = derivative_blocks(n);
-    const Step s = make_step(domain_, cache, constants_, gradient_norm_);
+    const Step s = make_step(domain_, cache, constants_, gradient_norm_, config_.exact_no_slip_adjoint != 0u);
 
     GatherTangentArgs ga{};
     ga.n = n;
@@ -939,7 +952,7 @@ void DfsphSolver::vjp_step(const DfsphCache& cache, const phx_sph_dfsph_state_ad
     const bool warm = cache.warm_start;
     const uint32_t* perm = cache.neighborhood.perm.data();
     const uint32_t blocks = derivative_blocks(n);
-    const Step s = make_step(domain_, cache, constants_, gradient_norm_);
+    const Step s = make_step(domain_, cache, constants_, gradient_norm_, config_.exact_no_slip_adjoint != 0u);
     const size_t nn = n;
     auto collisions = [&](uint32_t k) { return cache.collisions.data() + k * nn; };
 

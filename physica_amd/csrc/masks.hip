@@ -33,6 +33,7 @@ void MaskSet::allocate(uint32_t n, uint32_t predicting_passes, float support_rad
     h_ = support_radius;
     exact_.resize(static_cast<size_t>(kStencil) * n);
     skin_mask_.resize(static_cast<size_t>(kStencil) * n);
+    crowded_.resize(n);
     flag_count_ = kFlagDisplacement + predicting_passes;
     flags_.resize(flag_count_);
     flags_host_.resize(sizeof(uint32_t) * flag_count_ * kReadbacks);

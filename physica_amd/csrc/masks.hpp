@@ -67,12 +67,15 @@ class MaskSet {
     uint16_t* exact() { return exact_.data(); }
     uint16_t* skin_mask() { return skin_mask_.data(); }
     uint32_t* flags() { return flags_.data(); }
+    // per particle: walks the full stencil (written by the mask pass)
+    uint8_t* crowded() { return crowded_.data(); }
 
   private:
     float h_ = 0.0F;
     float skin_ = 0.3F;  // in units of h
     uint32_t flag_count_ = 0;
     DeviceBuffer<uint16_t> exact_, skin_mask_;
+    DeviceBuffer<uint8_t> crowded_;
     DeviceBuffer<uint32_t> flags_;
     // A ring of read-backs: a step's copy may still be queued behind its
     // last pass when the next step starts (always so when the host waits

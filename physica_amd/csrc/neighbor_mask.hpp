@@ -6,8 +6,9 @@
 // the marked candidates in exactly the reference's summation order (27 cells,
 // ascending original index), so a sum over the marked candidates equals the
 // full-stencil sum whenever every unmarked candidate contributes an exact
-// zero. Cells with more than kMaskBits particles cannot be encoded; the build
-// then raises a device flag and every pass takes the full-stencil path.
+// zero. Cells with more than kMaskBits particles cannot be encoded: a particle
+// with such a cell among its 27 is marked crowded by the pass that builds the
+// step's masks, and every pass walks the full stencil for that particle alone.
 #pragma once
 
 #include <cstdint>
@@ -19,8 +20,9 @@ namespace phx {
 constexpr int kStencil = 27;
 constexpr uint32_t kMaskBits = 16;
 
-// Device flags of a step: [0] mask overflow; [1 + k] largest squared
-// displacement |x* - x|^2 (float bits) written by the k-th predicting pass.
+// Device flags of a step: [0] masks unusable for every particle (boundary
+// samples); [1 + k] largest squared displacement |x* - x|^2 (float bits)
+// written by the k-th predicting pass.
 constexpr uint32_t kFlagOverflow = 0;
 constexpr uint32_t kFlagDisplacement = 1;
 
@@ -203,13 +205,18 @@ struct MaskView {
     const uint32_t* flags;
     uint32_t n;
     float limit2;
-    // exact masks of one set of positions: they apply unless their build
-    // overflowed (flags[kFlagOverflow]); no displacement slot is read
+    // exact masks of one set of positions: they apply unless flags[kFlagOverflow]
+    // is raised; no displacement slot is read
     uint32_t exact = 0u;
+    // per particle (sorted order): non-zero where a stencil cell holds more
+    // than kMaskBits particles; null: no particle is crowded
+    const uint8_t* crowded = nullptr;
 };
 
-__device__ __forceinline__ bool masks_apply(const MaskView& m, uint32_t slot) {
+// Whether particle i walks its masks in a pass reading displacement slot `slot`.
+__device__ __forceinline__ bool masks_apply(const MaskView& m, uint32_t slot, uint32_t i) {
     if (m.masks == nullptr || m.flags[kFlagOverflow] != 0u) return false;
+    if (m.crowded != nullptr && m.crowded[i] != 0u) return false;
     return m.exact != 0u || __uint_as_float(m.flags[kFlagDisplacement + slot]) <= m.limit2;
 }
 

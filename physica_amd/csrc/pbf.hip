@@ -38,16 +38,25 @@ struct MaskArgs {
     CVec xs;
     float t_skin;
     uint16_t* masks;
-    uint32_t* flags;
+    uint8_t* crowded;
 };
 
+// Skin masks at x*; a particle with a stencil cell of more than kMaskBits
+// particles is marked crowded and walks the full stencil in every pass of
+// the step (the others keep their masks).
 __global__ void __launch_bounds__(kBlock) pbf_mask_kernel(MaskArgs a) {
     const uint32_t i = xcd_block() * kBlock + threadIdx.x;
     if (i >= a.n) return;
     const Float3 p = load(a.xs, i);
     int cx, cy, cz;
     particle_cell(a.grid, p, cx, cy, cz);
-    if (!build_skin_mask(a.grid, a.xs, a.n, i, p, cx, cy, cz, a.t_skin, a.masks)) atomicOr(&a.flags[kFlagOverflow], 1u);
+    a.crowded[i] = build_skin_mask(a.grid, a.xs, a.n, i, p, cx, cy, cz, a.t_skin, a.masks) ? 0u : 1u;
+}
+
+// A particle that is not crowded always gets complete exact masks; the set's
+// overflow word is raised only if that ever fails.
+__device__ __forceinline__ void flag_incomplete(bool complete, const MaskView& mv, uint32_t i, uint32_t* set_flag) {
+    if (!complete && (mv.crowded == nullptr || mv.crowded[i] == 0u)) atomicOr(set_flag, 1u);
 }
 
 struct LambdaArgs {
@@ -67,8 +76,8 @@ struct LambdaArgs {
     MaskView mv;
     uint32_t slot;  // displacement slot of the positions read
     // exact masks of the positions read (pairs inside exact_mask_threshold),
-    // for the correction pass and the derivatives; flags[kFlagOverflow] is
-    // raised if a cell cannot be encoded
+    // for the correction pass and the derivatives (crowded particles walk the
+    // full stencil instead)
     float t_exact;
     uint16_t* exact_out;
     uint32_t* exact_flags;
@@ -85,7 +94,7 @@ __global__ void __launch_bounds__(kBlock) pbf_lambda_kernel(LambdaArgs a) {
     PHX_MASK_LDS(a.grid);
     const uint32_t i = xcd_block() * kBlock + threadIdx.x;
     if (i >= a.n) return;
-    const bool masked = masks_apply(a.mv, a.slot);
+    const bool masked = masks_apply(a.mv, a.slot, i);
     const Float3 p = load(a.pos, i);
     const float rest = a.rho0[i];
     int cx, cy, cz;
@@ -117,7 +126,7 @@ __global__ void __launch_bounds__(kBlock) pbf_lambda_kernel(LambdaArgs a) {
     } else {
         const bool complete = visit_and_mask(a.grid, a.mv, masked, i, cx, cy, cz, delta, smask + threadIdx.x, kBlock,
                                              a.exact_out, a.n, visit);
-        if (!complete) atomicOr(&a.exact_flags[kFlagOverflow], 1u);
+        flag_incomplete(complete, a.mv, i, &a.exact_flags[kFlagOverflow]);
     }
     denominator += dot(gradient_sum, gradient_sum);
     a.rho[i] = density;
@@ -164,7 +173,7 @@ __global__ void __launch_bounds__(kBlock) pbf_correction_kernel(CorrectionArgs a
     const uint32_t i = xcd_block() * kBlock + threadIdx.x;
     float moved2 = 0.0F;
     if (i < a.n) {
-        const bool masked = masks_apply(a.mv, a.slot);
+        const bool masked = masks_apply(a.mv, a.slot, i);
         const Float3 p = load(a.pos, i);
         const float rest = a.rho0[i];
         const float lambda_i = a.lambda[i];
@@ -254,7 +263,7 @@ __global__ void __launch_bounds__(kBlock) pbf_vorticity_kernel(PostArgs a) {
     PHX_MASK_LDS(a.grid);
     const uint32_t i = xcd_block() * kBlock + threadIdx.x;
     if (i >= a.n) return;
-    const bool masked = masks_apply(a.mv, a.slot);
+    const bool masked = masks_apply(a.mv, a.slot, i);
     const Float3 p = load(a.pos, i);
     const Float3 v = load(a.vel, i);
     int cx, cy, cz;
@@ -266,7 +275,7 @@ __global__ void __launch_bounds__(kBlock) pbf_vorticity_kernel(PostArgs a) {
             if (s != i) w = add(w, cross(sub(load(a.vel, s), v), spiky_gradient(d, a.h, a.spiky)));
             return dot(d, d) < a.t_exact;
         });
-    if (!complete) atomicOr(&a.exact_flags[kFlagOverflow], 1u);
+    flag_incomplete(complete, a.mv, i, &a.exact_flags[kFlagOverflow]);
     store(a.vort, i, w);
     a.magnitude[i] = length(w);
 }
@@ -275,7 +284,7 @@ __global__ void __launch_bounds__(kBlock) pbf_normal_kernel(PostArgs a) {
     PHX_MASK_LDS(a.grid);
     const uint32_t i = xcd_block() * kBlock + threadIdx.x;
     if (i >= a.n) return;
-    const bool masked = masks_apply(a.mv, a.slot);
+    const bool masked = masks_apply(a.mv, a.slot, i);
     const Float3 p = load(a.pos, i);
     const float mag_i = a.magnitude[i];
     int cx, cy, cz;
@@ -299,7 +308,7 @@ __global__ void __launch_bounds__(kBlock) pbf_xsph_kernel(PostArgs a) {
     PHX_MASK_LDS(a.grid);
     const uint32_t i = xcd_block() * kBlock + threadIdx.x;
     if (i >= a.n) return;
-    const bool masked = masks_apply(a.mv, a.slot);
+    const bool masked = masks_apply(a.mv, a.slot, i);
     const CVec vc{a.vc.x, a.vc.y, a.vc.z};
     const Float3 p = load(a.pos, i);
     const Float3 v = load(vc, i);
@@ -366,6 +375,7 @@ void PbfCache::allocate(uint32_t n, uint32_t cells, uint32_t pressure_iterations
     masks.resize(per);
     normalizers.resize(n);
     masks_skin.resize(static_cast<size_t>(kStencil) * n);
+    crowded.resize(n);
     mask_flags.resize(kFlagDisplacement + pressure_iterations + 1u);
     exact_masks.resize(static_cast<size_t>(pressure_iterations + 1u) * kStencil * n);
     exact_flags.resize(pressure_iterations + 1u);
@@ -390,6 +400,7 @@ PbfSolver::PbfSolver(const phx_domain_configuration* domain, const phx_pbf_confi
     for (DeviceBuffer<float>* b : {&m_, &rho0_, &eps_, &strength_, &exponent_, &radius_, &xsph_, &confinement_})
         b->resize(n);
     exact_.resize(static_cast<size_t>(kStencil) * n);
+    crowded_.resize(n);
     exact_flags_.resize(config_.pressure_iterations + 1u);
     // displacement slots: one per iterate (the first, x* itself, stays zero)
     masks_.allocate(n, config_.pressure_iterations + 1u, domain_.config.support_radius);
@@ -460,13 +471,15 @@ void PbfSolver::forward_step(const phx_pbf_state& state, const phx_control& cont
     const float inv_dt = 1.0F / c.time_step;
     // a recorded step keeps its masks and flags for its derivative passes
     uint16_t* skin = rec ? cache.masks_skin.data() : masks_.skin_mask();
+    uint8_t* crowded = rec ? cache.crowded.data() : crowded_.data();
     {
         PassScope scope(profiler_, stream_, kPbfMask);
-        MaskArgs ma{n, grid, xs.cview(), masks_.t_skin(), skin, masks_.flags()};
+        MaskArgs ma{n, grid, xs.cview(), masks_.t_skin(), skin, crowded};
         pbf_mask_kernel<<<blocks, kBlock, 0, stream_>>>(ma);
         PHX_LAUNCH_CHECK();
     }
-    const MaskView mv{skin, masks_.flags(), n, masks_.limit2()};
+    MaskView mv{skin, masks_.flags(), n, masks_.limit2()};
+    mv.crowded = crowded;
     // exact masks of position set k (iterate k, the final positions last):
     // written by the pass that first walks those positions, walked by the
     // later passes at them; a recorded step keeps every set. Each set has one
@@ -483,7 +496,7 @@ void PbfSolver::forward_step(const phx_pbf_state& state, const phx_control& cont
         require(k < sets, PHX_ERROR_INVALID_ARGUMENT, "exact mask set out of range");
         return rec ? cache.exact_masks.data() + static_cast<size_t>(k) * kStencil * n : exact_.data();
     };
-    auto exact_view = [&](uint32_t k) { return MaskView{exact_set(k), exact_flags + k, n, 0.0F, 1u}; };
+    auto exact_view = [&](uint32_t k) { return MaskView{exact_set(k), exact_flags + k, n, 0.0F, 1u, crowded}; };
 
     LambdaArgs la{n,           h,        spiky_,     grid, xs.cview(), xs.cview(), m_.data(), rho0_.data(),
                   eps_.data(), cache.densities.data(), cache.lambdas.data(), nullptr, Vec{}, mv, 0u};

@@ -54,6 +54,7 @@ struct PbfCache {
     // the step's neighbour masks and mask flags, so the derivative passes
     // walk the same candidates as the forward passes
     DeviceBuffer<uint16_t> masks_skin;
+    DeviceBuffer<uint8_t> crowded;  // per particle: walks the full stencil (a stencil cell overflows the masks)
     DeviceBuffer<uint32_t> mask_flags;
     float mask_limit2 = -1.0F;
     // exact masks of every set of positions (iterates 0 .. I - 1, then the
@@ -131,6 +132,7 @@ class PbfSolver {
     VecBuffer xs_, x0_, pa_, pb_, vel_, vort_, vc_;
     DeviceBuffer<float> m_, rho0_, eps_, strength_, exponent_, radius_, xsph_, confinement_;
     DeviceBuffer<uint16_t> exact_;  // exact masks of the current set of positions (forward mode)
+    DeviceBuffer<uint8_t> crowded_;  // per particle: walks the full stencil (forward mode)
     DeviceBuffer<uint32_t> exact_flags_;
     PbfDerivativeWorkspace dw_;
 };

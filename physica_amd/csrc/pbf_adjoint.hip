@@ -158,7 +158,7 @@ __global__ void __launch_bounds__(kBlock) pbf_vjp_post_kernel(PostVjpArgs a) {
     PHX_MASK_LDS(a.s.grid);
     const uint32_t i = xcd_block() * kBlock + threadIdx.x;
     if (i >= a.s.n) return;
-    const bool masked = masks_apply(a.s.mv, a.slot);
+    const bool masked = masks_apply(a.s.mv, a.slot, i);
     const Step& s = a.s;
     const Float3 pi = load(a.pos, i);
     const D3 out = load(a.next_v_a, i);
@@ -211,7 +211,7 @@ __global__ void __launch_bounds__(kBlock) pbf_vjp_normal_kernel(NormalVjpArgs a)
     PHX_MASK_LDS(a.s.grid);
     const uint32_t i = xcd_block() * kBlock + threadIdx.x;
     if (i >= a.s.n) return;
-    const bool masked = masks_apply(a.s.mv, a.slot);
+    const bool masked = masks_apply(a.s.mv, a.slot, i);
     const Step& s = a.s;
     const Float3 pi = load(a.pos, i);
     const D3 gi = load(a.g_a, i);
@@ -246,7 +246,7 @@ __global__ void __launch_bounds__(kBlock) pbf_vjp_vorticity_kernel(VorticityVjpA
     PHX_MASK_LDS(a.s.grid);
     const uint32_t i = xcd_block() * kBlock + threadIdx.x;
     if (i >= a.s.n) return;
-    const bool masked = masks_apply(a.s.mv, a.slot);
+    const bool masked = masks_apply(a.s.mv, a.slot, i);
     const Step& s = a.s;
     const Float3 pi = load(a.pos, i);
     const D3 vi = d3(load(a.vrec, i));
@@ -308,7 +308,7 @@ __global__ void __launch_bounds__(kBlock) pbf_vjp_correction_kernel(CorrectionVj
     PHX_MASK_LDS(a.s.grid);
     const uint32_t i = xcd_block() * kBlock + threadIdx.x;
     if (i >= a.s.n) return;
-    const bool masked = masks_apply(a.s.mv, a.slot);
+    const bool masked = masks_apply(a.s.mv, a.slot, i);
     const Step& s = a.s;
     const double h = static_cast<double>(s.h), k = static_cast<double>(s.spiky);
     const Float3 pi = load(a.pos, i);
@@ -386,7 +386,7 @@ __global__ void __launch_bounds__(kBlock) pbf_vjp_lambda_kernel(LambdaVjpArgs a)
     PHX_MASK_LDS(a.s.grid);
     const uint32_t i = xcd_block() * kBlock + threadIdx.x;
     if (i >= a.s.n) return;
-    const bool masked = masks_apply(a.s.mv, a.slot);
+    const bool masked = masks_apply(a.s.mv, a.slot, i);
     const Step& s = a.s;
     const double h = static_cast<double>(s.h), k = static_cast<double>(s.spiky);
     const Float3 pi = load(a.pos, i);
@@ -539,7 +539,7 @@ __global__ void __launch_bounds__(kBlock) pbf_jvp_lambda_kernel(LambdaJvpArgs a)
     PHX_MASK_LDS(a.s.grid);
     const uint32_t i = xcd_block() * kBlock + threadIdx.x;
     if (i >= a.s.n) return;
-    const bool masked = masks_apply(a.s.mv, a.slot);
+    const bool masked = masks_apply(a.s.mv, a.slot, i);
     const Step& s = a.s;
     const Float3 pi = load(a.pos, i), ti = load(a.p_t, i);
     const float rho0 = s.rho0[i], rho0_t = a.t.rho0[i];
@@ -586,7 +586,7 @@ __global__ void __launch_bounds__(kBlock) pbf_jvp_correction_kernel(CorrectionJv
     PHX_MASK_LDS(a.s.grid);
     const uint32_t i = xcd_block() * kBlock + threadIdx.x;
     if (i >= a.s.n) return;
-    const bool masked = masks_apply(a.s.mv, a.slot);
+    const bool masked = masks_apply(a.s.mv, a.slot, i);
     const Step& s = a.s;
     const Float3 pi = load(a.pos, i), ti = load(a.p_t, i);
     const float rho0 = s.rho0[i], rho0_t = a.t.rho0[i];
@@ -637,7 +637,7 @@ __global__ void __launch_bounds__(kBlock) pbf_jvp_vorticity_kernel(PostJvpArgs a
     PHX_MASK_LDS(a.s.grid);
     const uint32_t i = xcd_block() * kBlock + threadIdx.x;
     if (i >= a.s.n) return;
-    const bool masked = masks_apply(a.s.mv, a.slot);
+    const bool masked = masks_apply(a.s.mv, a.slot, i);
     const Step& s = a.s;
     const Float3 pi = load(a.pos, i), ti = load(a.p_t, i);
     const Float3 vi = load(a.vrec, i), vti = vrec_tangent(a, i);
@@ -663,7 +663,7 @@ __global__ void __launch_bounds__(kBlock) pbf_jvp_normal_kernel(PostJvpArgs a) {
     PHX_MASK_LDS(a.s.grid);
     const uint32_t i = xcd_block() * kBlock + threadIdx.x;
     if (i >= a.s.n) return;
-    const bool masked = masks_apply(a.s.mv, a.slot);
+    const bool masked = masks_apply(a.s.mv, a.slot, i);
     const Step& s = a.s;
     const Float3 pi = load(a.pos, i), ti = load(a.p_t, i);
     Float3 gt{0.0F, 0.0F, 0.0F};
@@ -693,7 +693,7 @@ __global__ void __launch_bounds__(kBlock) pbf_jvp_xsph_kernel(PostJvpArgs a) {
     PHX_MASK_LDS(a.s.grid);
     const uint32_t i = xcd_block() * kBlock + threadIdx.x;
     if (i >= a.s.n) return;
-    const bool masked = masks_apply(a.s.mv, a.slot);
+    const bool masked = masks_apply(a.s.mv, a.slot, i);
     const Step& s = a.s;
     const CVec vct{a.vc_t.x, a.vc_t.y, a.vc_t.z};
     const Float3 pi = load(a.pos, i), ti = load(a.p_t, i);
@@ -764,11 +764,18 @@ void require_recorded(const PbfCache& cache, phx_execution_mode mode, uint32_t n
 
 namespace {
 // The recorded exact masks of position set k (iterate k; the final positions
-// are set I), which apply unless their build overflowed.
+// are set I), which apply to every particle that is not crowded unless the
+// set's overflow word is raised.
 MaskView exact_view(const PbfCache& cache, uint32_t k, uint32_t n) {
     require(k <= cache.iterations, PHX_ERROR_INVALID_ARGUMENT, "exact mask set out of range");
     return MaskView{cache.exact_masks.data() + static_cast<size_t>(k) * kStencil * n, cache.exact_flags.data() + k, n,
-                    0.0F, 1u};
+                    0.0F, 1u, cache.crowded.data()};
+}
+
+// The recorded skin masks of the step and their displacement flags.
+MaskView skin_view(const PbfCache& cache, uint32_t n) {
+    return MaskView{cache.masks_skin.data(), cache.mask_flags.data(), n, cache.mask_limit2, 0u,
+                    cache.crowded.data()};
 }
 }  // namespace
 
@@ -822,7 +829,7 @@ void PbfSolver::jvp_step(const phx_pbf_parameters& parameters, const PbfCache& c
            dw_.radius.data(),
            dw_.xsph.data(),
            dw_.confinement.data(),
-           MaskView{cache.masks_skin.data(), cache.mask_flags.data(), n, cache.mask_limit2}};
+           skin_view(cache, n)};
     const TangentParams tp{dw_.m_t.data(),        dw_.rho0_t.data(),   dw_.eps_t.data(),  dw_.strength_t.data(),
                            dw_.exponent_t.data(), dw_.radius_t.data(), dw_.xsph_t.data(), dw_.confinement_t.data()};
     PredictJvpArgs pa{n,
@@ -918,7 +925,7 @@ void PbfSolver::vjp_step(const phx_pbf_parameters& parameters, const PbfCache& c
            dw_.radius.data(),
            dw_.xsph.data(),
            dw_.confinement.data(),
-           MaskView{cache.masks_skin.data(), cache.mask_flags.data(), n, cache.mask_limit2},
+           skin_view(cache, n),
            dw_.inv_rho0.data()};
     // sorted parameter adjoints start at zero (+0.0 is the all-zero bit pattern)
     for (DeviceBuffer<double>* b : {&dw_.m_a, &dw_.rho0_a, &dw_.eps_a, &dw_.strength_a, &dw_.exponent_a,

@@ -47,6 +47,7 @@ struct WcsphForceArgs {
     BoundaryGrid boundary;          // offsets null: no boundary samples
     const float* rest_density;      // rho0, for the boundary terms
     const float* boundary_tension;  // WCSPH boundary_surface_tension, for the boundary cohesion
+    const uint8_t* crowded;         // per particle: walks the full stencil
 };
 
 // pressure_forward_kernel (common.cu:165-194), viscosity_forward_kernel
@@ -127,7 +128,7 @@ __global__ void __launch_bounds__(kBlock) wcsph_force_kernel(WcsphForceArgs a) {
                 as = add(as, scale(d, factor));
             }
         });
-    } else if (a.flags[kFlagOverflow] == 0u) {
+    } else if (a.flags[kFlagOverflow] == 0u && a.crowded[i] == 0u) {
         uint16_t* column = smask + threadIdx.x;
         const uint32_t nonzero = load_lane_column(a.exact, a.n, i, column, kBlock);
         walk_masked(a.grid.offsets, cell_of(a.grid, cx, cy, cz), delta, column, kBlock, nonzero, visit);
@@ -235,6 +236,7 @@ SphSolver::SphSolver(const phx_domain_configuration* domain, const phx_sph_confi
         dc.density_iterations = config_.pressure_iterations;
         dc.checkpoint_interval = config_.checkpoint_interval;
         dc.pressure_warm_start = 0u;
+        dc.exact_no_slip_adjoint = config_.exact_no_slip_adjoint;
         engine_ = std::make_unique<DfsphSolver>(domain, &dc, PHX_MODE_DIFFERENTIABLE, stream);
         engine_->set_volume_rest_density(config_.method == PHX_SPH_PCISPH);
         for (auto& b : engine_warm_) b.resize(n);
@@ -350,6 +352,7 @@ void SphSolver::forward_wcsph(const phx_sph_state& state, const phx_control& con
         for (DeviceBuffer<float>* b : {&r->rho0, &r->tait}) b->resize(n);
         r->collisions.resize(n);
         r->exact.resize(static_cast<size_t>(kStencil) * n);
+        r->crowded.resize(n);
         r->flags.resize(masks_.flag_count());
         if (boundary_.count() > 0u) r->btension.resize(n);
     }
@@ -359,6 +362,7 @@ void SphSolver::forward_wcsph(const phx_sph_state& state, const phx_control& con
     float* rho0 = rec ? r->rho0.data() : rho0_.data();
     float* tait = rec ? r->tait.data() : tait_.data();
     uint16_t* exact = rec ? r->exact.data() : masks_.exact();
+    uint8_t* crowded = rec ? r->crowded.data() : masks_.crowded();
     ChannelList gather;
     gather.add(cvec(state.particles.positions), xm);
     gather.add(pp.masses, xm, 3);
@@ -397,6 +401,7 @@ void SphSolver::forward_wcsph(const phx_sph_state& state, const phx_control& con
     ma.tait_exponent = tait;
     ma.pressures = cache.pressures.data();
     ma.aux = aux;
+    ma.crowded = crowded;
     launch_mask_density<true>(profiler_, stream_, blocks, ma, kSphDensity);
 
     WcsphForceArgs wa{};
@@ -420,6 +425,7 @@ void SphSolver::forward_wcsph(const phx_sph_state& state, const phx_control& con
     wa.boundary = boundary;
     wa.rest_density = rho0;
     wa.boundary_tension = btension;
+    wa.crowded = crowded;
     {
         PassScope scope(profiler_, stream_, kSphForces);
         wcsph_force_kernel<<<blocks, kBlock, 0, stream_>>>(wa);
@@ -476,6 +482,7 @@ void SphSolver::forward_pressure(const phx_sph_state& state, const phx_control& 
     ma.skin = masks_.skin_mask();
     ma.rho = cache.densities.data();
     ma.flags = masks_.flags();
+    ma.crowded = masks_.crowded();
     launch_mask_density<false>(profiler_, stream_, blocks, ma, kSphDensity);
 
     ForceArgs fa{};
@@ -499,6 +506,7 @@ void SphSolver::forward_pressure(const phx_sph_state& state, const phx_control& 
     fa.next_v = vec(next.particles.velocities);
     fa.boundary = boundary;
     fa.rest_density = rho0_.data();
+    fa.crowded = masks_.crowded();
 
     ProjectionArgs pa{};
     pa.n = n;
@@ -520,6 +528,7 @@ void SphSolver::forward_pressure(const phx_sph_state& state, const phx_control& 
     pa.t = t_.data();
     pa.boundary = boundary;
     pa.rest_density = rho0_.data();
+    pa.crowded = masks_.crowded();
 
     if (config_.density_error_tolerance > 0.0F) {
         // Tolerance mode: iterate until the projection's mean compression

@@ -22,6 +22,7 @@ struct WcsphRecord {
     DeviceBuffer<float> btension;  // boundary surface tensions (boundary samples only)
     DeviceBuffer<uint8_t> collisions;
     DeviceBuffer<uint16_t> exact;
+    DeviceBuffer<uint8_t> crowded;  // per particle: walks the full stencil
     DeviceBuffer<uint32_t> flags;
 };
 

@@ -42,6 +42,9 @@ struct MaskDensityArgs {
     const float* tait_exponent;
     float* pressures;
     float4* aux;  // z (speed of sound) and w (surface tension) are inputs
+    // per particle: set where a stencil cell holds more than kMaskBits
+    // particles (that particle walks the full stencil in every pass)
+    uint8_t* crowded;
 };
 
 // Mask build, then density_forward_kernel (density.cu:18-40, cubic kernel,
@@ -50,7 +53,9 @@ struct MaskDensityArgs {
 // kEos the Tait pressure of eos_forward_kernel (wcsph.cu:92-100) and its
 // first_term. With boundary samples no masks are built (the host marks the
 // step's masks unusable) and the density walks the full stencil, each cell's
-// samples b after its particles adding rho0_i V_b W (density.cu:34-37).
+// samples b after its particles adding rho0_i V_b W (density.cu:34-37). A
+// particle with a stencil cell of more than kMaskBits particles is marked
+// crowded and walks the full stencil here and in every later pass.
 template <bool kEos>
 __global__ void __launch_bounds__(kBlock) mask_density_kernel(MaskDensityArgs a) {
     __shared__ uint16_t smask[kStencil * kBlock];
@@ -66,6 +71,7 @@ __global__ void __launch_bounds__(kBlock) mask_density_kernel(MaskDensityArgs a)
     uint32_t nonzero = 0u;
     float result = 0.0F;
     if (a.boundary.offsets != nullptr) {
+        a.crowded[i] = 1u;  // no masks are built with boundary samples
         const float rest = a.rest_density[i];
         for_each_candidate(
             a.grid, a.boundary, cx, cy, cz,
@@ -79,13 +85,14 @@ __global__ void __launch_bounds__(kBlock) mask_density_kernel(MaskDensityArgs a)
             });
     } else if (build_masks(a.grid, a.xm, a.n, i, p, cx, cy, cz, a.kc.sqrt_support, a.t_skin, a.exact, a.skin, column,
                            kBlock, &nonzero)) {
+        a.crowded[i] = 0u;
         walk_masked(a.grid.offsets, cell_of(a.grid, cx, cy, cz), delta, column, kBlock, nonzero, [&](uint32_t s) {
             const float4 q = a.xm[s];
             const float dist = pair_distance(sub(p, xyz(q)));
             result += q.w * cubic_value(dist, a.kc);
         });
     } else {
-        atomicOr(&a.flags[kFlagOverflow], 1u);
+        a.crowded[i] = 1u;
         for_each_candidate(a.grid, cx, cy, cz, [&](uint32_t s) {
             const float4 q = a.xm[s];
             result += q.w * cubic_value(pair_distance(sub(p, xyz(q))), a.kc);
@@ -184,6 +191,7 @@ struct ProjectionArgs {
     Convergence cv;
     BoundaryGrid boundary;     // offsets null: no boundary samples
     const float* rest_density;  // rho0, for the boundary terms
+    const uint8_t* crowded;     // per particle: walks the full stencil
 };
 
 // density at predicted positions (density.cu:18-40 with topology = x,
@@ -202,7 +210,7 @@ __global__ void __launch_bounds__(kBlock) projection_kernel(ProjectionArgs a) {
     const uint32_t i = xcd_block() * kBlock + threadIdx.x;
     double error = 0.0;
     if (i < a.n) {
-        const bool masked = a.flags[kFlagOverflow] == 0u && a.boundary.offsets == nullptr &&
+        const bool masked = a.flags[kFlagOverflow] == 0u && a.boundary.offsets == nullptr && a.crowded[i] == 0u &&
                             __uint_as_float(a.flags[kFlagDisplacement + a.slot]) <= a.limit2;
         const float4 self = a.xs[i];
         const Float3 ps = xyz(self);
@@ -291,6 +299,7 @@ struct ForceArgs {
     uint32_t iteration;
     BoundaryGrid boundary;      // offsets null: no boundary samples
     const float* rest_density;  // rho0, for the boundary pressure terms
+    const uint8_t* crowded;     // per particle: walks the full stencil
 };
 
 // Pressure acceleration (pressure_forward_kernel, common.cu:165-194),
@@ -322,7 +331,7 @@ __global__ void __launch_bounds__(kBlock) force_kernel(ForceArgs a) {
     const bool active = i < a.n;
     float moved2 = 0.0F;
     if (active) {
-        const bool masked = a.flags[kFlagOverflow] == 0u && a.boundary.offsets == nullptr;
+        const bool masked = a.flags[kFlagOverflow] == 0u && a.boundary.offsets == nullptr && a.crowded[i] == 0u;
         const float4 self = a.xm[i];
         const Float3 p = xyz(self);
         const float4 vm = a.vmu[i];

@@ -35,6 +35,7 @@ struct WcsphStep {
     float dt;
     float diameter;  // 2 r_p, below which the cohesion weight is clamped
     Box box;
+    uint32_t exact_no_slip;  // 1: exact transpose of a no-slip collision, 0: the reference's VJP
     SortedGrid grid;
     const float4* xm;   // position + mass
     const float4* vmu;  // velocity + viscosity
@@ -47,10 +48,16 @@ struct WcsphStep {
     const uint8_t* collisions;
     BoundaryGrid boundary;   // offsets null: no boundary samples
     const float* btension;   // boundary surface tensions (boundary samples only)
+    const uint8_t* crowded;  // per particle: walks the full stencil
 };
 
-__device__ __forceinline__ MaskView exact_view(const WcsphStep& s) { return MaskView{s.exact, s.flags, s.n, 0.0F}; }
-__device__ __forceinline__ bool exact_masked(const WcsphStep& s) { return s.flags[kFlagOverflow] == 0u; }
+__device__ __forceinline__ MaskView exact_view(const WcsphStep& s) {
+    return MaskView{s.exact, s.flags, s.n, 0.0F, 0u, s.crowded};
+}
+// Whether particle i walks the step's exact masks (not crowded, no boundary samples).
+__device__ __forceinline__ bool exact_masked(const WcsphStep& s, uint32_t i) {
+    return s.flags[kFlagOverflow] == 0u && s.crowded[i] == 0u;
+}
 
 // Visits the fluid candidates of particle i through its exact masks or, with
 // boundary samples, every fluid particle and every sample of the 27 cells in
@@ -61,7 +68,7 @@ __device__ __forceinline__ void walk(const WcsphStep& s, uint32_t i, int cx, int
     if (s.boundary.offsets != nullptr) {
         for_each_candidate(s.grid, s.boundary, cx, cy, cz, visit, visit_boundary);
     } else {
-        visit_candidates(s.grid, exact_view(s), exact_masked(s), i, cx, cy, cz, delta, column, kBlock, visit);
+        visit_candidates(s.grid, exact_view(s), exact_masked(s, i), i, cx, cy, cz, delta, column, kBlock, visit);
     }
 }
 
@@ -309,7 +316,8 @@ struct WcsphFinalVjpArgs {
 };
 
 // integrate_vjp_kernel (common.cu:344-366) with the recorded collision bits
-// (exact transpose of the tangent under no-slip collisions).
+// (under no-slip collisions the reference's zeroed velocity adjoint, or the
+// exact transpose of the tangent when the solver asks for it).
 __global__ void __launch_bounds__(kBlock) wcsph_vjp_final_kernel(WcsphFinalVjpArgs a) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= a.s.n) return;
@@ -317,7 +325,8 @@ __global__ void __launch_bounds__(kBlock) wcsph_vjp_final_kernel(WcsphFinalVjpAr
     const D3 xa{a.next_x_a.x[j], a.next_x_a.y[j], a.next_x_a.z[j]};
     const D3 va = a.next_v_a.x != nullptr ? D3{a.next_v_a.x[j], a.next_v_a.y[j], a.next_v_a.z[j]} : D3{0.0, 0.0, 0.0};
     D3 position, velocity;
-    predict_adjoint(a.s.collisions[i], a.s.box.no_slip, static_cast<double>(a.s.dt), xa, va, position, velocity);
+    predict_adjoint(a.s.collisions[i], a.s.box.no_slip, a.s.exact_no_slip, static_cast<double>(a.s.dt), xa, va, position,
+                    velocity);
     store(a.pos_a, i, position);
     store(a.vel_a, i, velocity);
     store(a.total_a, i, scale(velocity, static_cast<double>(a.s.dt)));
@@ -611,7 +620,7 @@ void WcsphDerivativeWorkspace::allocate_adjoints(uint32_t n) {
 
 namespace {
 
-WcsphStep make_wcsph_step(const Domain& domain, const SphCache& cache, const KernelConstants& kc) {
+WcsphStep make_wcsph_step(const Domain& domain, const SphCache& cache, const KernelConstants& kc, bool exact_no_slip) {
     const auto& c = domain.config;
     const WcsphRecord& r = *cache.wcsph;
     WcsphStep s{};
@@ -620,6 +629,7 @@ WcsphStep make_wcsph_step(const Domain& domain, const SphCache& cache, const Ker
     s.dt = c.time_step;
     s.diameter = 2.0F * c.particle_radius;
     s.box = collision_box(c, r.step_index);
+    s.exact_no_slip = exact_no_slip ? 1u : 0u;
     s.grid = cache.neighborhood.grid();
     s.xm = r.xm.data();
     s.vmu = r.vmu.data();
@@ -632,6 +642,7 @@ WcsphStep make_wcsph_step(const Domain& domain, const SphCache& cache, const Ker
     s.collisions = r.collisions.data();
     s.boundary = cache.neighborhood.boundary();
     s.btension = r.btension.data();
+    s.crowded = r.crowded.data();
     return s;
 }
 
@@ -648,7 +659,7 @@ void SphSolver::wcsph_jvp_step(const SphCache& cache, const phx_sph_state_tangen
     require_vec(next_t.particles.positions, "next position tangent");
     require_vec(next_t.particles.velocities, "next velocity tangent");
     wdw_.allocate_tangents(n);
-    const WcsphStep s = make_wcsph_step(domain_, cache, constants_);
+    const WcsphStep s = make_wcsph_step(domain_, cache, constants_, config_.exact_no_slip_adjoint != 0u);
     const uint32_t* perm = cache.neighborhood.perm.data();
     WcsphGatherArgs ga{};
     ga.n = n;
@@ -706,7 +717,7 @@ void SphSolver::wcsph_vjp_step(const SphCache& cache, const phx_sph_state_adjoin
     all_or_none(previous_a.particles.velocities, "velocity adjoint");
     all_or_none(control_a.external_accelerations, "control adjoint");
     wdw_.allocate_adjoints(n);
-    const WcsphStep s = make_wcsph_step(domain_, cache, constants_);
+    const WcsphStep s = make_wcsph_step(domain_, cache, constants_, config_.exact_no_slip_adjoint != 0u);
     const uint32_t* perm = cache.neighborhood.perm.data();
     WcsphFinalVjpArgs fa{};
     fa.s = s;

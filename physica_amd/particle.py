@@ -10,6 +10,7 @@ HIP library (libphysica_amd.so) and fails loudly if it is missing.
 from __future__ import annotations
 
 import ctypes
+import functools
 import math
 from dataclasses import dataclass, field
 from enum import IntEnum
@@ -311,6 +312,43 @@ def _stream_handle(device: torch.device) -> int:
     return torch.cuda.current_stream(device).cuda_stream
 
 
+class _SolverStream:
+    """A solver enqueues its work on the stream that was current when it was
+    created; the tensors a call reads or writes are filled, read and freed on
+    whatever stream is current at the call. Inside this block the solver's
+    stream first waits for the caller's current stream, and on leaving it the
+    current stream waits for the solver's stream, so work on the two streams
+    never overlaps on shared tensors (nothing is waited for when the two are
+    the same stream)."""
+
+    def __init__(self, solver):
+        self._own = solver._stream
+        self._current = torch.cuda.current_stream(solver.domain.device)
+        self._same = self._current.cuda_stream == self._own.cuda_stream
+
+    def __enter__(self):
+        if not self._same:
+            self._own.wait_stream(self._current)
+        return self
+
+    def __exit__(self, *exc):
+        if not self._same:
+            self._current.wait_stream(self._own)
+        return False
+
+
+def _stream_ordered(method):
+    """Runs a solver method inside _SolverStream (the method must enqueue no
+    work of its own on the current stream)."""
+
+    @functools.wraps(method)
+    def ordered(self, *args, **kwargs):
+        with _SolverStream(getattr(self, "_solver", self)):
+            return method(self, *args, **kwargs)
+
+    return ordered
+
+
 def _require_gpu(device: torch.device) -> None:
     if device.type != "cuda" or not torch.cuda.is_available():
         raise RuntimeError("physica_amd solvers run on the GPU only (no CPU fallback); no HIP device is available")
@@ -328,6 +366,10 @@ class DFSPH:
         pressure_warm_start: bool = True
         density_error_tolerance: float = 0.0
         max_iterations: int = 100
+        # VJP of a no-slip box collision: False (default) the reference's
+        # (sph/dfsph.cu:81 zeroes the whole velocity adjoint), True the exact
+        # transpose of the JVP
+        exact_no_slip_adjoint: bool = False
 
         def to_c(self) -> _capi.DfsphConfiguration:
             return _capi.DfsphConfiguration(
@@ -337,6 +379,7 @@ class DFSPH:
                 1 if self.pressure_warm_start else 0,
                 float(self.density_error_tolerance),
                 int(self.max_iterations),
+                1 if self.exact_no_slip_adjoint else 0,
             )
 
     @dataclass
@@ -398,8 +441,13 @@ class WCSPH:
 
     @dataclass
     class Configuration:
+        # VJP of a no-slip box collision: False (default) the reference's
+        # (sph/common.cu:357 zeroes the whole velocity adjoint), True the
+        # exact transpose of the JVP
+        exact_no_slip_adjoint: bool = False
+
         def to_c(self) -> _capi.SphConfiguration:
-            return _capi.SphConfiguration(_capi.PHX_SPH_WCSPH, 0, 2)
+            return _capi.SphConfiguration(_capi.PHX_SPH_WCSPH, 0, 2, 0.0, 0, 1 if self.exact_no_slip_adjoint else 0)
 
     @dataclass
     class State:
@@ -460,11 +508,15 @@ class PCISPH:
         # error, at most max_iterations times; 0: fixed pressure_iterations
         density_error_tolerance: float = 0.0
         max_iterations: int = 100
+        # VJP of a no-slip box collision: False (default) the reference's
+        # (sph/dfsph.cu:81 zeroes the whole velocity adjoint), True the exact
+        # transpose of the JVP
+        exact_no_slip_adjoint: bool = False
 
         def to_c(self) -> _capi.SphConfiguration:
             return _capi.SphConfiguration(
                 _capi.PHX_SPH_PCISPH, int(self.pressure_iterations), int(self.checkpoint_interval),
-                float(self.density_error_tolerance), int(self.max_iterations)
+                float(self.density_error_tolerance), int(self.max_iterations), 1 if self.exact_no_slip_adjoint else 0
             )
 
     @dataclass
@@ -516,11 +568,15 @@ class IISPH:
         # error, at most max_iterations times; 0: fixed pressure_iterations
         density_error_tolerance: float = 0.0
         max_iterations: int = 100
+        # VJP of a no-slip box collision: False (default) the reference's
+        # (sph/dfsph.cu:81 zeroes the whole velocity adjoint), True the exact
+        # transpose of the JVP
+        exact_no_slip_adjoint: bool = False
 
         def to_c(self) -> _capi.SphConfiguration:
             return _capi.SphConfiguration(_capi.PHX_SPH_IISPH, int(self.pressure_iterations),
                                           int(self.checkpoint_interval), float(self.density_error_tolerance),
-                                          int(self.max_iterations))
+                                          int(self.max_iterations), 1 if self.exact_no_slip_adjoint else 0)
 
     @dataclass
     class State:
@@ -652,12 +708,14 @@ class SPH:
             keys = torch.empty(n, dtype=torch.int64, device=dev)
             idx = torch.empty(n, dtype=torch.int32, device=dev)
             off = torch.empty(cells + 1, dtype=torch.int32, device=dev)
-            _capi.call(self._prefix + "_cache_neighborhood", self._handle, _ptr(keys), _ptr(idx), _ptr(off))
+            with _SolverStream(self._solver):
+                _capi.call(self._prefix + "_cache_neighborhood", self._handle, _ptr(keys), _ptr(idx), _ptr(off))
             return keys, idx, off
 
         def densities(self) -> torch.Tensor:
             out = self._solver.domain.allocate_scalar_field()
-            _capi.call(self._prefix + "_cache_densities", self._handle, _ptr(out))
+            with _SolverStream(self._solver):
+                _capi.call(self._prefix + "_cache_densities", self._handle, _ptr(out))
             return out
 
         def pressures(self) -> torch.Tensor:
@@ -665,12 +723,14 @@ class SPH:
             if self._prefix != "phx_sph":
                 raise AttributeError("the DFSPH step cache keeps its pressures in the method state")
             out = self._solver.domain.allocate_scalar_field()
-            _capi.call("phx_sph_cache_pressures", self._handle, _ptr(out))
+            with _SolverStream(self._solver):
+                _capi.call("phx_sph_cache_pressures", self._handle, _ptr(out))
             return out
 
         def non_pressure_accelerations(self) -> VectorField:
             out = self._solver.domain.allocate_vector_field()
-            _capi.call(self._prefix + "_cache_non_pressure_accelerations", self._handle, out.view())
+            with _SolverStream(self._solver):
+                _capi.call(self._prefix + "_cache_non_pressure_accelerations", self._handle, out.view())
             return out
 
     def __init__(
@@ -689,12 +749,13 @@ class SPH:
         _require_gpu(self.domain.device)
         handle = ctypes.c_void_p()
         self._domain_c = domain_configuration.to_c()
+        self._stream = torch.cuda.current_stream(self.domain.device)
         _capi.call(
             self._prefix + "_create",
             ctypes.byref(self._domain_c),
             ctypes.byref(self.method_configuration.to_c()),
             int(mode),
-            _stream_handle(self.domain.device),
+            self._stream.cuda_stream,
             ctypes.byref(handle),
         )
         self._handle = handle
@@ -736,6 +797,7 @@ class SPH:
             destination.method.warm_divergence_pressure.copy_(source.method.warm_divergence_pressure)
             destination.method.warm_density_pressure.copy_(source.method.warm_density_pressure)
 
+    @_stream_ordered
     def forward_step(self, state, control, parameters, next_state, cache) -> None:
         s = state.to_c()
         n = next_state.to_c()
@@ -835,6 +897,7 @@ class SPH:
             destination.method.warm_divergence_pressure.add_(source.method.warm_divergence_pressure)
             destination.method.warm_density_pressure.add_(source.method.warm_density_pressure)
 
+    @_stream_ordered
     def jvp_step(self, state, control, parameters, next_state, cache, state_tangent, control_tangent,
                  parameter_tangent, next_state_tangent) -> None:
         """SPH<Method>::jvp_step (sph/sph.ixx:356-359) of the step recorded in
@@ -851,6 +914,7 @@ class SPH:
             ctypes.byref(next_state_tangent.to_c()),
         )
 
+    @_stream_ordered
     def vjp_step(self, state, control, parameters, next_state, cache, next_state_adjoint, previous_state_adjoint,
                  control_adjoint, parameter_adjoint) -> None:
         """SPH<Method>::vjp_step (sph/sph.ixx:361-365): particle, control and
@@ -988,24 +1052,28 @@ class PBF:
             keys = torch.empty(n, dtype=torch.int64, device=dev)
             idx = torch.empty(n, dtype=torch.int32, device=dev)
             off = torch.empty(cells + 1, dtype=torch.int32, device=dev)
-            _capi.call("phx_pbf_cache_neighborhood", self._handle, _ptr(keys), _ptr(idx), _ptr(off))
+            with _SolverStream(self._solver):
+                _capi.call("phx_pbf_cache_neighborhood", self._handle, _ptr(keys), _ptr(idx), _ptr(off))
             return keys, idx, off
 
         def predicted_positions(self) -> VectorField:
             out = self._solver.domain.allocate_vector_field()
-            _capi.call("phx_pbf_cache_predicted_positions", self._handle, out.view())
+            with _SolverStream(self._solver):
+                _capi.call("phx_pbf_cache_predicted_positions", self._handle, out.view())
             return out
 
         def last_iteration(self):
             rho = self._solver.domain.allocate_scalar_field()
             lam = self._solver.domain.allocate_scalar_field()
-            _capi.call("phx_pbf_cache_last_iteration", self._handle, _ptr(rho), _ptr(lam))
+            with _SolverStream(self._solver):
+                _capi.call("phx_pbf_cache_last_iteration", self._handle, _ptr(rho), _ptr(lam))
             return rho, lam
 
         def vorticity_magnitudes(self) -> torch.Tensor:
             """StepCache::vorticity_magnitudes (pbf/pbf.ixx:67), original order."""
             out = self._solver.domain.allocate_scalar_field()
-            _capi.call("phx_pbf_cache_vorticity_magnitudes", self._handle, _ptr(out))
+            with _SolverStream(self._solver):
+                _capi.call("phx_pbf_cache_vorticity_magnitudes", self._handle, _ptr(out))
             return out
 
     def __init__(
@@ -1020,12 +1088,13 @@ class PBF:
         _require_gpu(self.domain.device)
         handle = ctypes.c_void_p()
         self._domain_c = domain_configuration.to_c()
+        self._stream = torch.cuda.current_stream(self.domain.device)
         _capi.call(
             "phx_pbf_create",
             ctypes.byref(self._domain_c),
             ctypes.byref(self.configuration.to_c()),
             int(mode),
-            _stream_handle(self.domain.device),
+            self._stream.cuda_stream,
             ctypes.byref(handle),
         )
         self._handle = handle
@@ -1089,6 +1158,7 @@ class PBF:
         destination.particles.positions.add_(source.particles.positions)
         destination.particles.velocities.add_(source.particles.velocities)
 
+    @_stream_ordered
     def jvp_step(self, state, control, parameters, next_state, cache, state_tangent, control_tangent,
                  parameter_tangent, next_state_tangent) -> None:
         """PBF::jvp_step (pbf/pbf.cpp:193-215) of the step recorded in `cache`
@@ -1104,6 +1174,7 @@ class PBF:
             ctypes.byref(next_state_tangent.to_c()),
         )
 
+    @_stream_ordered
     def vjp_step(self, state, control, parameters, next_state, cache, next_state_adjoint, previous_state_adjoint,
                  control_adjoint, parameter_adjoint) -> None:
         """PBF::vjp_step (pbf/pbf.cpp:217-258); every output is accumulated."""
@@ -1124,6 +1195,7 @@ class PBF:
     def read_profile(self) -> list:
         return _read_profile("phx_pbf_read_profile", self._handle)
 
+    @_stream_ordered
     def forward_step(self, state, control, parameters, next_state, cache) -> None:
         s = state.to_c()
         n = next_state.to_c()

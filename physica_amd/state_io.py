@@ -82,19 +82,32 @@ def read_state(path) -> Dict[str, np.ndarray]:
 
 def load_state(path, state) -> dict:
     """Fills an allocated state (positions, velocities, step index, method
-    fields) from a file; returns the file's metadata."""
+    fields) from a file; returns the file's metadata. Every array's shape is
+    checked (and the particle count of the file's domain metadata, if any)
+    before anything is copied, so a malformed file raises ValueError instead
+    of broadcasting into the state."""
     data = read_state(path)
     particles = state.particles
     n = particles.positions.x.numel()
-    if data["positions"].shape != (3, n):
-        raise ValueError(f"state file holds {data['positions'].shape[1]} particles, the state {n}")
+    for key in ("positions", "velocities"):
+        if key not in data or data[key].shape != (3, n):
+            shape = data[key].shape if key in data else None
+            raise ValueError(f"state file {key} have shape {shape}, the state needs (3, {n})")
+    if "step_index" not in data or data["step_index"].shape != ():
+        raise ValueError("state file has no scalar step_index")
+    domain = data["meta"].get("domain")
+    if domain is not None and int(domain.get("particle_count", n)) != n:
+        raise ValueError(f"state file was written for {domain['particle_count']} particles, the state holds {n}")
+    for name, tensor in _method_fields(state).items():
+        key = "method/" + name
+        if key not in data:
+            raise ValueError(f"state file has no {key}")
+        if data[key].shape != tuple(tensor.shape):
+            raise ValueError(f"state file {key} has shape {data[key].shape}, the state needs {tuple(tensor.shape)}")
     for field, key in ((particles.positions, "positions"), (particles.velocities, "velocities")):
         for axis, name in enumerate("xyz"):
             getattr(field, name).copy_(torch.from_numpy(np.ascontiguousarray(data[key][axis])))
     particles.step_index = int(data["step_index"])
     for name, tensor in _method_fields(state).items():
-        key = "method/" + name
-        if key not in data:
-            raise ValueError(f"state file has no {key}")
-        tensor.copy_(torch.from_numpy(np.ascontiguousarray(data[key])))
+        tensor.copy_(torch.from_numpy(np.ascontiguousarray(data["method/" + name])))
     return data["meta"]

@@ -145,7 +145,9 @@ void particle_tangent(const pp::Domain& d, pp::ParticleParameterTangent& t, Rand
     if (surface) random_fill(d, t.surface_tensions, r, 1e-3F);
 }
 
-// Forward step in forward mode, then the derivative identity in differentiable mode.
+// Forward step in forward mode, then the derivative identity in differentiable mode
+// (the SPH solvers with the exact transpose of a no-slip collision, for which
+// the identity holds).
 template <class Solver, class Configuration, class Setup, class Tangent, class MethodDot>
 void run(const std::string& name, const std::string& out_dir, float dt, Configuration configuration, Setup&& setup,
          Tangent&& tangent, MethodDot&& method_dot) {
@@ -224,7 +226,7 @@ int main(int argc, char** argv) {
     const std::string out = argv[1];
     try {
         run<pp::SPH<pp::DFSPH>>(
-            "dfsph", out, 1e-3F, pp::DFSPH::Configuration{3u, 4u, 2u, false},
+            "dfsph", out, 1e-3F, pp::DFSPH::Configuration{3u, 4u, 2u, false, 0.0F, 100u, true},
             [](const pp::Domain& d, auto& p) {
                 water(d, p.particles);
                 fill(d, p.method.divergence_relaxation, 0.5F);
@@ -240,7 +242,7 @@ int main(int argc, char** argv) {
                        dot(pt.method.density_relaxation, pa.method.density_relaxation);
             });
         run<pp::SPH<pp::PCISPH>>(
-            "pcisph", out, 1e-3F, pp::PCISPH::Configuration{4u, 2u},
+            "pcisph", out, 1e-3F, pp::PCISPH::Configuration{4u, 2u, 0.0F, 100u, true},
             [](const pp::Domain& d, auto& p) {
                 water(d, p.particles);
                 fill(d, p.method.pressure_relaxation, 0.5F);
@@ -253,7 +255,7 @@ int main(int argc, char** argv) {
                 return dot(pt.method.pressure_relaxation, pa.method.pressure_relaxation);
             });
         run<pp::SPH<pp::IISPH>>(
-            "iisph", out, 1e-3F, pp::IISPH::Configuration{4u, 2u},
+            "iisph", out, 1e-3F, pp::IISPH::Configuration{4u, 2u, 0.0F, 100u, true},
             [](const pp::Domain& d, auto& p) {
                 water(d, p.particles);
                 fill(d, p.method.jacobi_relaxation, 0.5F);
@@ -266,7 +268,7 @@ int main(int argc, char** argv) {
                 return dot(pt.method.jacobi_relaxation, pa.method.jacobi_relaxation);
             });
         run<pp::SPH<pp::WCSPH>>(
-            "wcsph", out, 2e-4F, pp::WCSPH::Configuration{},
+            "wcsph", out, 2e-4F, pp::WCSPH::Configuration{true},
             [](const pp::Domain& d, auto& p) {
                 water(d, p.particles);
                 fill(d, p.particles.surface_tensions, 0.01F);

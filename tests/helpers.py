@@ -1,7 +1,11 @@
 """Shared set-up for GPU tests: upload a scene into solver buffers and read
 results back as numpy."""
+import contextlib
+
 import numpy as np
 import torch
+
+import oracle
 
 from physica_amd.particle import DFSPH, IISPH, PBF, PCISPH, SPH, WCSPH, ExecutionMode
 
@@ -106,14 +110,18 @@ def pbf_setup(scene, device, configuration=None, mode=ExecutionMode.forward):
     return solver, state, nxt, control, params, cache
 
 
-def sph_setup(scene, device, method, pressure_iterations=None, mode=ExecutionMode.forward):
+def sph_setup(scene, device, method, pressure_iterations=None, mode=ExecutionMode.forward, exact_no_slip_adjoint=False):
     """SPH<WCSPH|PCISPH|IISPH> with the scene uploaded; `method` is "wcsph",
     "pcisph" or "iisph" (the scene's "relaxation" feeds PCISPH's pressure or
-    IISPH's Jacobi relaxation)."""
+    IISPH's Jacobi relaxation). exact_no_slip_adjoint picks the VJP of a
+    no-slip collision: the exact transpose, or the reference's (default)."""
     iterations = scene.solver.get("pressure_iterations", 6) if pressure_iterations is None else pressure_iterations
-    configuration = {"wcsph": lambda: WCSPH.Configuration(),
-                     "pcisph": lambda: PCISPH.Configuration(pressure_iterations=iterations),
-                     "iisph": lambda: IISPH.Configuration(pressure_iterations=iterations)}[method]()
+    exact = exact_no_slip_adjoint
+    configuration = {"wcsph": lambda: WCSPH.Configuration(exact_no_slip_adjoint=exact),
+                     "pcisph": lambda: PCISPH.Configuration(pressure_iterations=iterations,
+                                                            exact_no_slip_adjoint=exact),
+                     "iisph": lambda: IISPH.Configuration(pressure_iterations=iterations,
+                                                          exact_no_slip_adjoint=exact)}[method]()
     solver = SPH(scene.domain, configuration, mode=mode, device=device)
     state = solver.allocate_state()
     nxt = solver.allocate_state()
@@ -133,3 +141,15 @@ def sph_setup(scene, device, method, pressure_iterations=None, mode=ExecutionMod
     else:
         upload_scalar(m.jacobi_relaxation, scene.parameters["relaxation"])
     return solver, state, nxt, control, params, cache
+
+
+@contextlib.contextmanager
+def reference_no_slip_oracle():
+    """The oracle's SPH VJPs with the reference's no-slip collision adjoint
+    (sph/dfsph.cu:81, sph/common.cu:357) for the duration of the block; the
+    oracle's default is the exact transpose."""
+    oracle.set_reference_no_slip_adjoint(True)
+    try:
+        yield
+    finally:
+        oracle.set_reference_no_slip_adjoint(False)

@@ -5,7 +5,11 @@ The recorded forward step is bit-exact against the FP32 oracle. The VJP
 1e-10 of the largest entry of each output: the same per-particle sums, with
 only the order in which a few accumulators add their parts differing. The
 JVP (FP32, as the reference's) runs the oracle's operations in the same
-order and agrees to 1e-6. The scenes are those of the oracle's own pins
+order and agrees to 1e-6. The VJP of a no-slip collision is the exact
+transpose in most tests; with the solver's default (the reference's, which
+zeroes the whole velocity adjoint of a colliding particle) it is compared
+with the oracle's restatement of that variant. The scenes are those of the
+oracle's own pins
 (a compressed swirling block, the same block pushed into a wall so that
 every prediction clamps some particles, and the block over a moving plane of
 boundary samples), plus a 16,384-particle dam break at rest in its corner."""
@@ -16,14 +20,15 @@ import torch
 import oracle
 from physica_amd import scenes
 from physica_amd.particle import DFSPH, ExecutionMode
-from helpers import bits_equal, dfsph_setup, dfsph_state_numpy
+from helpers import bits_equal, dfsph_setup, dfsph_state_numpy, reference_no_slip_oracle
 from test_oracle_dfsph_derivatives import PARAMS, STATE, WARM, Case, case, directions, inputs
 
 pytestmark = pytest.mark.gpu
 
 
-def recorded(cs, device, nd, ne, warm):
-    cfg = DFSPH.Configuration(divergence_iterations=nd, density_iterations=ne, pressure_warm_start=warm)
+def recorded(cs, device, nd, ne, warm, exact=True):
+    cfg = DFSPH.Configuration(divergence_iterations=nd, density_iterations=ne, pressure_warm_start=warm,
+                              exact_no_slip_adjoint=exact)
     solver, state, nxt, control, params, cache = dfsph_setup(cs.sc, device, cfg, warm=cs.warm,
                                                              mode=ExecutionMode.differentiable)
     solver.forward_step(state, control, params, nxt, cache)
@@ -154,6 +159,37 @@ def test_dam_break_vjp_matches_oracle(gpu_device):
     got = run_vjp(*run, w, gpu_device)
     x, v, u, p, wd, we = inputs(cs, "f32")
     exp = oracle.dfsph_vjp(cs.d, 0, x, v, u, p, wd, we, w, 2, 3, True, "f32")
+    for k in STATE + PARAMS + WARM:
+        err = np.abs(got[k] - exp[k]).max() / max(np.abs(exp[k]).max(), 1e-300)
+        assert err < 1e-10, (k, err)
+
+
+@pytest.mark.parametrize("scene", ["wall", "dam_break"])
+def test_reference_no_slip_vjp_matches_oracle(gpu_device, scene):
+    """The default configuration: the reference's VJP of a no-slip collision
+    (sph/dfsph.cu:81) against the oracle's restatement of it, in scenes where
+    predictions collide with the box; it differs from the exact transpose
+    there."""
+    print(f"dfsph reference no-slip vjp {scene}", flush=True)
+    if scene == "wall":
+        cs = case(wall=True)
+    else:
+        sc = scenes.dfsph_dam_break(shape=(32, 16, 32))
+        d = oracle.make_domain(sc.domain)
+        z = np.zeros(sc.count, dtype=np.float32)
+        first = oracle.dfsph_step(d, 0, sc.positions, sc.velocities, np.zeros_like(sc.positions), sc.parameters, z,
+                                  z, 2, 3)
+        cs = Case(sc, d, (first[WARM[0]], first[WARM[1]]))
+    assert DFSPH.Configuration().exact_no_slip_adjoint is False
+    run = recorded(cs, gpu_device, 2, 3, True, exact=False)
+    rng = np.random.default_rng(110)
+    _, w = directions(cs, rng)
+    got = run_vjp(*run, w, gpu_device)
+    x, v, u, p, wd, we = inputs(cs, "f32")
+    with reference_no_slip_oracle():
+        exp = oracle.dfsph_vjp(cs.d, 0, x, v, u, p, wd, we, w, 2, 3, True, "f32")
+    exact = oracle.dfsph_vjp(cs.d, 0, x, v, u, p, wd, we, w, 2, 3, True, "f32")
+    assert not np.array_equal(exp["velocities"], exact["velocities"])
     for k in STATE + PARAMS + WARM:
         err = np.abs(got[k] - exp[k]).max() / max(np.abs(exp[k]).max(), 1e-300)
         assert err < 1e-10, (k, err)

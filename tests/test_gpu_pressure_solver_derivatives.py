@@ -7,14 +7,17 @@ bit for bit, cache fields included. The JVP (FP32) agrees with the oracle's
 FP32 JVP to 1e-6 of the largest entry and the VJP (FP64 adjoints of the FP32
 primal) to 1e-10, as for DFSPH; the GPU's own JVP and VJP satisfy the
 dot-product identity. The scene is the compressed swirling block of the
-oracle's derivative pins, also over a moving plane of boundary samples."""
+oracle's derivative pins, also over a moving plane of boundary samples, and
+pushed into a wall for the VJP of a no-slip collision: the exact transpose
+in most tests, the reference's (the solver's default) against the oracle's
+restatement of it."""
 import numpy as np
 import pytest
 import torch
 
 import oracle
 from physica_amd.particle import ExecutionMode
-from helpers import bits_equal, sph_setup
+from helpers import bits_equal, reference_no_slip_oracle, sph_setup
 from test_oracle_pressure_solver_derivatives import PARAMS, STATE, directions, inputs, setup
 
 pytestmark = pytest.mark.gpu
@@ -26,9 +29,10 @@ def method_field(method):
     return "pressure_relaxation" if method == "pcisph" else "jacobi_relaxation"
 
 
-def recorded(cs, method, iterations, device):
+def recorded(cs, method, iterations, device, exact=True):
     solver, state, nxt, control, params, cache = sph_setup(cs.sc, device, method, iterations,
-                                                           mode=ExecutionMode.differentiable)
+                                                           mode=ExecutionMode.differentiable,
+                                                           exact_no_slip_adjoint=exact)
     solver.forward_step(state, control, params, nxt, cache)
     torch.cuda.synchronize()
     x, v, u, p = inputs(cs, "f32")
@@ -147,6 +151,27 @@ def test_gpu_dot_product_identity_and_accumulation(method):
     assert np.allclose(prev_a.particles.positions.numpy(), 2 * jw["positions"], rtol=1e-12, atol=0)
     assert np.allclose(getattr(par_a.method, method_field(method)).cpu().numpy(), 2 * jw["relaxation"],
                        rtol=1e-12, atol=0)
+
+
+@pytest.mark.parametrize("method", METHODS)
+def test_reference_no_slip_vjp_matches_oracle(method):
+    """Default configuration: the reference's VJP of a no-slip collision
+    (sph/pcisph.cu:81, sph/iisph.cu:81) against the oracle's restatement, in
+    the wall scene where predictions collide."""
+    print(f"{method} reference no-slip vjp", flush=True)
+    dev = torch.device("cuda:0")
+    cs = setup(method, wall=True)
+    run = recorded(cs, method, 3, dev, exact=False)
+    rng = np.random.default_rng(97)
+    _, w = directions(cs, rng)
+    x, v, u, p = inputs(cs, "f32")
+    jw = run_vjp(method, *run, w)
+    with reference_no_slip_oracle():
+        rw = oracle.pressure_solver_vjp(cs.d, 0, x, v, u, p, w, method, 3, "f32")
+    exact = oracle.pressure_solver_vjp(cs.d, 0, x, v, u, p, w, method, 3, "f32")
+    assert not np.array_equal(rw["velocities"], exact["velocities"])
+    for k in STATE + PARAMS:
+        assert close(jw[k], rw[k], 1e-10), k
 
 
 def test_forward_mode_refuses_derivatives():

@@ -9,14 +9,17 @@ exponents drawn in [1.5, 7] the platform's powf enters the primal, and both
 agree to 1e-5. The GPU's own JVP and VJP satisfy the dot-product identity.
 The scenes are those of the oracle's WCSPH derivative pins, one of them over
 a moving plane of boundary samples (pressure, artificial viscosity and
-cohesion terms with the boundary surface tension)."""
+cohesion terms with the boundary surface tension). The VJP of a no-slip
+collision is the exact transpose in most tests; with the solver's default
+(the reference's integrate_vjp_kernel, which zeroes the whole velocity
+adjoint) it is compared with the oracle's restatement in the wall scene."""
 import numpy as np
 import pytest
 import torch
 
 import oracle
 from physica_amd.particle import ExecutionMode
-from helpers import bits_equal, sph_setup
+from helpers import bits_equal, reference_no_slip_oracle, sph_setup
 from test_oracle_wcsph_derivatives import PARAMS, STATE, directions, inputs, setup
 
 pytestmark = pytest.mark.gpu
@@ -31,8 +34,8 @@ def scene(unit_exponent, wall=False, boundary=False):
     return cs
 
 
-def recorded(cs, device, exact):
-    run = sph_setup(cs.sc, device, "wcsph", mode=ExecutionMode.differentiable)
+def recorded(cs, device, exact, exact_adjoint=True):
+    run = sph_setup(cs.sc, device, "wcsph", mode=ExecutionMode.differentiable, exact_no_slip_adjoint=exact_adjoint)
     solver, state, nxt, control, params, cache = run
     solver.forward_step(state, control, params, nxt, cache)
     torch.cuda.synchronize()
@@ -154,6 +157,25 @@ def test_gpu_dot_product_identity_and_accumulation():
     torch.cuda.synchronize()
     assert np.allclose(prev_a.particles.positions.numpy(), 2 * jw["positions"], rtol=1e-12, atol=0)
     assert np.allclose(par_a.method.speed_of_sound.cpu().numpy(), 2 * jw["speed_of_sound"], rtol=1e-12, atol=0)
+
+
+def test_reference_no_slip_vjp_matches_oracle():
+    """Default configuration: the reference's integrate VJP under a no-slip
+    collision (sph/common.cu:357) against the oracle's restatement."""
+    print("wcsph reference no-slip vjp", flush=True)
+    dev = torch.device("cuda:0")
+    cs = scene(unit_exponent=True, wall=True)
+    run = recorded(cs, dev, exact=True, exact_adjoint=False)
+    rng = np.random.default_rng(19)
+    _, w = directions(cs, rng)
+    x, v, u, p = inputs(cs, "f32")
+    jw = run_vjp(*run, w)
+    with reference_no_slip_oracle():
+        rw = oracle.wcsph_vjp(cs.d, 0, x, v, u, p, w, "f32")
+    exact = oracle.wcsph_vjp(cs.d, 0, x, v, u, p, w, "f32")
+    assert not np.array_equal(rw["velocities"], exact["velocities"])
+    for k in STATE + PARAMS:
+        assert close(jw[k], rw[k], 1e-10), k
 
 
 def test_forward_mode_refuses_derivatives():

@@ -21,8 +21,8 @@ STATE = ("positions", "velocities", "controls")
 METHODS = ("pcisph", "iisph")
 
 
-def setup(method, seed=3, shape=(6, 6, 5), boundary=False):
-    cs = case(shape=shape, seed=seed, boundary=boundary)
+def setup(method, seed=3, shape=(6, 6, 5), boundary=False, wall=False):
+    cs = case(shape=shape, seed=seed, boundary=boundary, wall=wall)
     p = cs.sc.parameters
     p["relaxation"] = p["density_relaxation"]
     return cs

@@ -46,8 +46,31 @@ def test_round_trip(tmp_path):
 
 def test_wrong_size_and_foreign_files_are_refused(tmp_path):
     save_state(tmp_path / "s.npz", state(10, 1))
-    with pytest.raises(ValueError, match="particles"):
+    with pytest.raises(ValueError, match="positions"):
         load_state(tmp_path / "s.npz", state(11, 1))
     np.savez(tmp_path / "other.npz", meta=np.array('{"format": "else"}'), positions=np.zeros((3, 2)))
     with pytest.raises(ValueError, match="physica_amd.state"):
         read_state(tmp_path / "other.npz")
+
+
+def test_malformed_fields_are_refused_before_anything_is_copied(tmp_path):
+    """A file whose velocities or method field would broadcast into the state
+    (a length-1 field, a (3, 1) vector) or whose domain metadata names another
+    particle count raises ValueError and leaves the state untouched."""
+    sc = scenes.dfsph_dam_break(shape=(4, 3, 2))
+    n = sc.count
+    good = state(n, 1)
+    save_state(tmp_path / "good.npz", good, sc.domain)
+    data = dict(np.load(tmp_path / "good.npz", allow_pickle=False))
+    cases = {"velocities": np.zeros((3, 1), np.float32),
+             "method/warm_density_pressure": np.zeros(1, np.float32),
+             "meta": np.array(str(data["meta"]).replace(f'"particle_count": {n}', '"particle_count": 5'))}
+    for key, bad in cases.items():
+        arrays = dict(data)
+        arrays[key] = bad
+        np.savez(tmp_path / "bad.npz", **arrays)
+        target = state(n, 2)
+        before = target.particles.positions.x.clone()
+        with pytest.raises(ValueError):
+            load_state(tmp_path / "bad.npz", target)
+        assert torch.equal(target.particles.positions.x, before), key
