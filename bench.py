@@ -4,24 +4,34 @@ Prints one JSON line (rank 0). A step is one SPH<DFSPH>::forward_step of the
 whole scene (grid build, density, the divergence and density solvers) with
 every input resident in HBM. The configured workload runs both solvers to a
 0.1 % mean compression error (FP64 residual, at most 100 iterations each):
-`value` is particle-steps/s over the timed steps of that scenario, which
-start from the initial state (the warm-up steps run on a copy), so the
-default 200 timed steps are the whole configured run, cold start included.
+the W warm-up steps are the scenario's first steps (untimed, their iteration
+counts reported), and `value` is particle-steps/s over the K steps that
+follow them. Timed steps that end a solver at the iteration cap without
+reaching the tolerance are counted in `config.unconverged_steps`.
 `fixed_iterations` gives the reference's own fixed counts (4 divergence + 6
 density iterations, no tolerance) measured over steady steps; `roofline` is
 its dominant pass, the algorithmic bytes per launch over the average launch
 time (HIP events on the solver stream during those timed steps).
 `cpu_baseline` is the FP32 oracle (OpenMP over particles, the same
 reference-order arithmetic) timed on this host over one fixed-count step of
-the same scene.
+the same scene with every host thread; `cpu_baseline_1core` the same on one
+thread over a 131,072-particle block of the same spacing and kernel.
+
+`pbf_32k` is the PBF dam break (32,768 particles, h 0.1, 4 iterations):
+its whole configured run of 200 forward steps at dt 1/120 (100 frames of
+two substeps). `iisph_4m` and `wcsph_4m` are the scale test: two falling
+128^3 cubes (4,194,304 particles, h 0.02) for 100 steps each from rest, IISPH
+with relaxed Jacobi (0.5) to a 0.1 % mean compression error (FP64 residual,
+at most 100 iterations) at dt 1e-3, WCSPH with the Tait equation (c = sqrt 7,
+gamma 7) at dt 2e-4.
 
 `fwd_adjoint_ratio` is the differentiable PBF rollout: 262,144 particles (64^3
 lattice, spacing 0.02, h 0.04), 120 steps at dt 1/120, 4 iterations, loss the
 mean squared distance of the final positions to a seeded target cloud,
-gradient with respect to the initial velocities (no parameter gradients). Ratio = (recording forward
-rollout + chained VJPs) / plain forward rollout, with every step cache kept in
-HBM; the same with a state checkpoint every 10 steps and recompute is reported
-beside it.
+gradient with respect to the initial velocities (no parameter gradients).
+Ratio = (forward rollout + chained VJPs) / plain forward rollout, with a state
+checkpoint every 10 steps and each segment recomputed before its VJPs; the
+same with every step cache kept in HBM (no recompute) is reported beside it.
 
 `batched_pbf` is the uneven batch: 64 independent PBF dam breaks (particle
 counts log-uniform in [4,096, 200,000] from seed 20260005, 3.79 M in all),
@@ -59,6 +69,8 @@ def parse_args(argv=None):
     p.add_argument("--no-pass-timing", action="store_true", help="time without per-pass events")
     p.add_argument("--no-adjoint", action="store_true", help="skip the differentiable PBF rollout ratio")
     p.add_argument("--no-batched", action="store_true", help="skip the 64-scene uneven PBF batch")
+    p.add_argument("--no-pbf", action="store_true", help="skip the 32,768-particle PBF dam break")
+    p.add_argument("--no-scale", action="store_true", help="skip the 4,194,304-particle IISPH and WCSPH scale test")
     p.add_argument("--batch-streams", type=int, default=32, help="HIP streams the PBF batch's scenes share")
     p.add_argument("--adjoint-steps", type=int, default=120,
                    help="steps of the differentiable PBF rollout (120: the configured rollout)")
@@ -75,6 +87,16 @@ def dfsph_bytes_per_step(iterations, particles, cells):
 def pbf_bytes_per_step(iterations, particles, cells):
     """PBF: 252 + 108 I + 8 C / N."""
     return 252.0 + 108.0 * iterations + 8.0 * cells / particles
+
+
+def iisph_bytes_per_step(iterations, particles, cells):
+    """IISPH: 188 + 128 I + 8 C / N."""
+    return 188.0 + 128.0 * iterations + 8.0 * cells / particles
+
+
+def wcsph_bytes_per_step(particles, cells):
+    """WCSPH: 172 + 8 C / N."""
+    return 172.0 + 8.0 * cells / particles
 
 
 def traffic_for(kernel: str):
@@ -142,24 +164,61 @@ def setup_pbf(device):
     return sc, solver, [a, b], control, params, cache
 
 
-def cpu_baseline(workload, sc):
-    """The FP32 oracle on this host: one step of the same scene, all threads."""
+def cpu_baseline(workload, sc, threads=None):
+    """The FP32 oracle on this host: one step of `sc`, on `threads` OpenMP
+    threads (all host threads when None)."""
     import oracle
 
-    d = oracle.make_domain(sc.domain)
-    zeros = np.zeros_like(sc.positions)
-    t0 = time.perf_counter()
-    if workload == "dfsph":
-        z = np.zeros(sc.count, np.float32)
-        oracle.dfsph_step(d, 0, sc.positions, sc.velocities, zeros, sc.parameters, z, z, 4, 6)
-        sample = ("1 step of the 1,048,576-particle DFSPH dam break, fixed 4 + 6 iterations "
-                  "(FP32 oracle, OpenMP over particles)")
-    else:
-        oracle.pbf_step(d, 0, sc.positions, sc.velocities, zeros, sc.parameters, 4)
-        sample = "1 step of the 32,768-particle PBF dam break (FP32 oracle, OpenMP over particles)"
-    elapsed = time.perf_counter() - t0
-    return {"value": sc.count / elapsed, "unit": "particle-steps/s", "cores": oracle.threads(), "kind": "port",
+    previous = oracle.threads()
+    if threads is not None:
+        oracle.set_threads(threads)
+    try:
+        used = oracle.threads()
+        d = oracle.make_domain(sc.domain)
+        zeros = np.zeros_like(sc.positions)
+        t0 = time.perf_counter()
+        if workload == "dfsph":
+            z = np.zeros(sc.count, np.float32)
+            oracle.dfsph_step(d, 0, sc.positions, sc.velocities, zeros, sc.parameters, z, z, 4, 6)
+            sample = (f"1 step of the {sc.count:,}-particle DFSPH dam break (spacing 0.01, h 0.02), fixed 4 + 6 "
+                      f"iterations (FP32 oracle, OpenMP over particles, {used} thread{'s' if used > 1 else ''})")
+        else:
+            oracle.pbf_step(d, 0, sc.positions, sc.velocities, zeros, sc.parameters, 4)
+            sample = (f"1 step of the {sc.count:,}-particle PBF dam break (FP32 oracle, OpenMP over particles, "
+                      f"{used} thread{'s' if used > 1 else ''})")
+        elapsed = time.perf_counter() - t0
+    finally:
+        oracle.set_threads(previous)
+    return {"value": sc.count / elapsed, "unit": "particle-steps/s", "cores": used, "kind": "port",
             "sample": sample, "seconds": elapsed}
+
+
+def setup_two_cubes(device, method):
+    """The scale test: two falling 128^3 cubes (4,194,304 particles), IISPH to
+    a 0.1 % mean compression error or WCSPH with the Tait equation."""
+    import torch
+
+    from physica_amd import scenes
+    from physica_amd.particle import IISPH, SPH, WCSPH
+
+    sc = scenes.two_cubes(method)
+    if method == "iisph":
+        cfg = IISPH.Configuration(density_error_tolerance=TOLERANCE, max_iterations=MAX_ITERATIONS)
+    else:
+        cfg = WCSPH.Configuration()
+    solver = SPH(sc.domain, cfg, device=device)
+    a, b = solver.allocate_state(), solver.allocate_state()
+    control, params, cache = solver.allocate_control(), solver.allocate_parameters(), solver.allocate_step_cache()
+    a.particles.positions.upload(sc.positions)
+    a.particles.velocities.upload(sc.velocities)
+    for name in ("masses", "rest_densities", "viscosities", "surface_tensions"):
+        getattr(params.particles, name).copy_(torch.from_numpy(sc.parameters[name]))
+    if method == "iisph":
+        params.method.jacobi_relaxation.copy_(torch.from_numpy(sc.parameters["relaxation"]))
+    else:
+        for name in ("speed_of_sound", "tait_exponent", "boundary_surface_tension"):
+            getattr(params.method, name).copy_(torch.from_numpy(sc.parameters[name]))
+    return sc, solver, [a, b], control, params, cache
 
 
 def adjoint_ratio(device, steps=120):
@@ -221,7 +280,8 @@ def adjoint_ratio(device, steps=120):
                        "ratio": round((fwd_ms + bwd_ms) / plain_ms, 3)}
         rollout.release()
         del rollout, final, initial
-    record["ratio"] = record["store_all"]["ratio"]
+    # BASELINE's definition: forward + backward with checkpoint-every-10 recompute
+    record["ratio"] = record["checkpoint_10"]["ratio"]
     return record
 
 
@@ -339,18 +399,20 @@ def main(argv=None) -> int:
 
     def measure(setup, steps, warmup, from_start, profile):
         """Times `steps` forward steps; with from_start the warm-up steps run
-        on a copy and the timed steps start from the initial state. Returns
-        (scene, elapsed ms, per-pass profile, iteration counts per step)."""
+        on a copy and the timed steps start from the initial state, otherwise
+        the timed steps continue the scenario after the warm-up steps.
+        Returns (scene, elapsed ms, per-pass profile, iteration counts per
+        timed step, solver, states, cache, iteration counts per warm-up step)."""
         sc, solver, states, control, params, cache = setup()
-        initial = solver.allocate_state()
-        solver.copy_state(states[0], initial)
+        initial = solver.allocate_state() if from_start else None
+        if from_start:
+            solver.copy_state(states[0], initial)
 
         def step(k):
             solver.forward_step(states[k % 2], control, params, states[(k + 1) % 2], cache)
             return solver.last_iterations() if hasattr(solver, "last_iterations") else None
 
-        for k in range(warmup):
-            step(k)
+        warm_counts = [step(k) for k in range(warmup)]
         if from_start:
             solver.copy_state(initial, states[0])
             states[0].particles.step_index = 0
@@ -374,7 +436,7 @@ def main(argv=None) -> int:
         final = states[(first + steps) % 2]
         if not np.isfinite(final.particles.positions.numpy()).all():
             raise RuntimeError("non-finite positions after the timed steps")
-        return sc, elapsed_ms, passes, counts, solver, states, cache
+        return sc, elapsed_ms, passes, counts, solver, states, cache, warm_counts
 
     def roofline_of(passes, n):
         if not passes:
@@ -395,7 +457,7 @@ def main(argv=None) -> int:
             return 2
         if args.fixed_steps > 0:
             # the reference's fixed counts over steady steps: per-pass timing and the kernel roofline
-            sc, f_ms, f_passes, f_counts, solver, states, cache = measure(
+            sc, f_ms, f_passes, f_counts, solver, states, cache, _ = measure(
                 lambda: setup_dfsph(device), args.fixed_steps, args.warmup, False, not args.no_pass_timing)
             n = sc.count
             cells = math.prod(sc.domain.grid_resolution())
@@ -411,24 +473,26 @@ def main(argv=None) -> int:
             roofline = roofline_of(f_passes, n)
             del solver, states, cache
         if args.mode == "tolerance":
-            sc, elapsed_ms, passes, counts, solver, states, cache = measure(
-                lambda: setup_dfsph(device, TOLERANCE), args.steps, args.warmup, True, False)
+            sc, elapsed_ms, passes, counts, solver, states, cache, warm_counts = measure(
+                lambda: setup_dfsph(device, TOLERANCE), args.steps, args.warmup, False, False)
             n = sc.count
             cells = math.prod(sc.domain.grid_resolution())
             bytes_total = sum(dfsph_bytes_per_step(c, n, cells) for c in counts)
             bytes_step = bytes_total / len(counts)
+            first, last = args.warmup, args.warmup + args.steps - 1
             workload = ("DFSPH 3D dam break: 1,048,576 particles (128x64x128 block, spacing 0.01, jitter 0.001), "
                         "h 0.02 cubic spline, dt 1e-3, divergence and density solvers to a 0.1 % mean density "
                         "error (FP64 residual, max 100 iterations each), viscosity 0.01, static box 2.56x1.28x1.28; "
-                        f"timed steps 0..{args.steps - 1} of the scenario")
+                        f"steps 0..{first - 1} untimed warm-up, timed steps {first}..{last} of the scenario")
             nd = [c[0] for c in counts]
             ne = [c[1] for c in counts]
+            unconverged = sum(1 for a, b in counts if a >= MAX_ITERATIONS or b >= MAX_ITERATIONS)
             config = {"workload": workload, "particles": n, "grid_cells": cells, "tolerance": TOLERANCE,
                       "max_iterations": MAX_ITERATIONS,
-                      "divergence_iterations": {"total": sum(nd), "first_step": nd[0], "max": max(nd),
-                                                "mean_after_first": round(sum(nd[1:]) / max(1, len(nd) - 1), 3)},
-                      "density_iterations": {"total": sum(ne), "first_step": ne[0], "max": max(ne),
-                                             "mean_after_first": round(sum(ne[1:]) / max(1, len(ne) - 1), 3)}}
+                      "divergence_iterations": {"total": sum(nd), "max": max(nd), "mean": round(sum(nd) / len(nd), 3)},
+                      "density_iterations": {"total": sum(ne), "max": max(ne), "mean": round(sum(ne) / len(ne), 3)},
+                      "unconverged_steps": unconverged,
+                      "warmup_iterations": [list(c) for c in warm_counts]}
             steps_timed, warmup_used = args.steps, args.warmup
         else:
             elapsed_ms, passes, steps_timed, warmup_used = f_ms, f_passes, args.fixed_steps, args.warmup
@@ -440,7 +504,7 @@ def main(argv=None) -> int:
                       "density_iterations": 6}
             fixed = None
     else:
-        sc, elapsed_ms, passes, _, solver, states, cache = measure(
+        sc, elapsed_ms, passes, _, solver, states, cache, _ = measure(
             lambda: setup_pbf(device), args.steps, args.warmup, False, not args.no_pass_timing)
         n = sc.count
         cells = math.prod(sc.domain.grid_resolution())
@@ -453,12 +517,60 @@ def main(argv=None) -> int:
     value = n * steps_timed / (elapsed_ms * 1e-3)
     step_achieved = bytes_step * n * steps_timed / (elapsed_ms * 1e-3) / 1e9
 
-    baseline = None
+    solver = states = cache = None  # release the headline solver's buffers
+    pbf_leg = None
+    if not args.no_pbf and args.workload == "dfsph":
+        # configuration 1: the whole 200-step PBF dam break from the initial state
+        psc, p_ms, _, _, psolver, pstates, pcache, _ = measure(lambda: setup_pbf(device), 200, 3, True, False)
+        pn = psc.count
+        pcells = math.prod(psc.domain.grid_resolution())
+        pbytes = pbf_bytes_per_step(4, pn, pcells)
+        pbf_leg = {"workload": "PBF dam break: 32,768 particles (32^3 lattice, spacing 0.05, jitter 0.005), h 0.1, "
+                               "rest density 1000, 4 iterations, dt 1/120 (two substeps of 1/60), box 3.2x2.0x1.6; "
+                               "forward steps 0..199 from the initial state",
+                   "particles": pn, "steps": 200, "ms_per_step": round(p_ms / 200, 4),
+                   "value": round(pn * 200 / (p_ms * 1e-3), 1),
+                   "hbm_fraction_step": round(pbytes * pn * 200 / (p_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 5),
+                   "algorithmic_bytes_per_particle_step": round(pbytes, 2)}
+        del psolver, pstates, pcache
+    scale = None
+    if not args.no_scale and args.workload == "dfsph":
+        scale = {}
+        for method in ("iisph", "wcsph"):
+            ssc, s_ms, _, s_counts, ssolver, sstates, scache, _ = measure(
+                lambda: setup_two_cubes(device, method), 100, 1, True, False)
+            sn = ssc.count
+            scells = math.prod(ssc.domain.grid_resolution())
+            rec = {"particles": sn, "grid_cells": scells, "steps": 100, "ms_per_step": round(s_ms / 100, 4),
+                   "value": round(sn * 100 / (s_ms * 1e-3), 1)}
+            if method == "iisph":
+                its = [c[0] for c in s_counts]
+                sbytes = sum(iisph_bytes_per_step(i, sn, scells) for i in its) / len(its)
+                rec["workload"] = ("IISPH scale test: two falling 128^3 cubes (4,194,304 particles, spacing 0.01, "
+                                   "jitter 0.001) in a 4x4x2 basin, h 0.02, relaxed Jacobi 0.5 to a 0.1 % mean "
+                                   "compression error (FP64 residual, max 100 iterations), dt 1e-3, steps 0..99")
+                rec["pressure_iterations"] = {"total": sum(its), "max": max(its), "first_step": its[0],
+                                              "mean": round(sum(its) / len(its), 3)}
+                rec["unconverged_steps"] = sum(1 for i in its if i >= MAX_ITERATIONS)
+            else:
+                sbytes = wcsph_bytes_per_step(sn, scells)
+                rec["workload"] = ("WCSPH scale test: two falling 128^3 cubes (4,194,304 particles, spacing 0.01, "
+                                   "jitter 0.001) in a 4x4x2 basin, h 0.02, Tait equation c = sqrt(7) gamma 7 "
+                                   "(stiffness 1000), no viscosity or surface tension, dt 2e-4, steps 0..99")
+            rec["hbm_fraction_step"] = round(sbytes * sn * 100 / (s_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 5)
+            rec["algorithmic_bytes_per_particle_step"] = round(sbytes, 2)
+            scale[f"{method}_4m"] = rec
+            del ssolver, sstates, scache
+
+    baseline = baseline_1core = None
     if not args.no_cpu_baseline:
         baseline = cpu_baseline(args.workload, sc)
+        if args.workload == "dfsph":
+            from physica_amd import scenes
+
+            baseline_1core = cpu_baseline("dfsph", scenes.dfsph_dam_break(shape=(64, 32, 64)), threads=1)
     adjoint = None
     if not args.no_adjoint:
-        del states, cache, solver
         adjoint = adjoint_ratio(device, steps=args.adjoint_steps)
     batched = None
     if not args.no_batched:
@@ -480,11 +592,15 @@ def main(argv=None) -> int:
         "config": config,
         "roofline": roofline,
         "cpu_baseline": baseline,
+        "cpu_baseline_1core": baseline_1core,
         "hbm_fraction_step": round(step_achieved / HBM_PEAK_GBS, 5),
         "algorithmic_bytes_per_particle_step": round(bytes_step, 2),
         "fwd_adjoint_ratio": None if adjoint is None else adjoint["ratio"],
         "fwd_adjoint": adjoint,
         "batched_pbf": batched,
+        "pbf_32k": pbf_leg,
+        "iisph_4m": None if scale is None else scale["iisph_4m"],
+        "wcsph_4m": None if scale is None else scale["wcsph_4m"],
         "fixed_iterations": fixed,
         "passes": [{k: (round(v, 4) if isinstance(v, float) else v) for k, v in p.items()} for p in passes],
     }
