@@ -306,81 +306,163 @@ def pbf_solver_for(scene, mode, device):
 
 def batched_pbf(device, steps=100, adjoint_scenes=16, streams=32):
     """The uneven PBF batch: 64 independent dam breaks (4,100 to 192,450
-    particles, 3.79 M in all), each with its own solver; scene k runs on
-    stream k mod `streams`, and the host issues the batch step by step, so
-    small scenes overlap. Then the gradient of a final-position loss with
-    respect to v0 on the first `adjoint_scenes` scenes (store-all rollouts,
-    one scene after another) against their plain forward timed the same way
-    (scene by scene); their overlapped plain forward is reported beside it."""
+    particles, 3.79 M in all), 100 steps. The batched engine runs every pass
+    of a substep as one launch over all scenes and replays a captured substep
+    (settings from the autotuner's JSON when present). Beside it: the same
+    batch as 64 solvers spread over `streams` streams, issued step by step.
+    Then forward + adjoint on the first `adjoint_scenes` scenes: store-all
+    rollouts (every step cache kept) of one solver per scene on its own
+    stream, issued step by step, forward and then the chained VJPs, against
+    their plain forward issued the same way."""
     import torch
 
     from physica_amd import scenes
-    from physica_amd.particle import ExecutionMode
-    from physica_amd.rollout import PbfRollout
+    from physica_amd.batch import PbfBatch
+    from physica_amd.particle import ExecutionMode, PBF
 
     batch = scenes.batched_pbf_scenes()
+    total = sum(sc.count for sc in batch)
+
+    def timed(fn):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) * 1e3
+
+    # batched engine
+    engine = PbfBatch([sc.domain for sc in batch], PBF.Configuration(pressure_iterations=4), device=device)
+    state, control, params = engine.allocate_state(), engine.allocate_control(), engine.allocate_parameters()
+    engine.upload(batch, state, params)
+    initial = engine.allocate_state()
+    initial.particles.positions.copy_(state.particles.positions)
+    initial.particles.velocities.copy_(state.particles.velocities)
+    engine.run(state, control, params, 2)  # warm-up: code loading and the graph capture
+    state.particles.positions.copy_(initial.particles.positions)
+    state.particles.velocities.copy_(initial.particles.velocities)
+    state.particles.step_index = 0
+    engine_ms = timed(lambda: engine.run(state, control, params, steps))
+    if not np.isfinite(state.particles.positions.numpy()).all():
+        raise RuntimeError("non-finite positions after the batched run")
+    record = {"workload": f"{len(batch)} uneven PBF dam breaks, {total} particles "
+                          f"({min(sc.count for sc in batch)} to {max(sc.count for sc in batch)}), "
+                          f"{steps} steps, dt 1/120, 4 iterations",
+              "particles": total, "steps": steps, "forward_ms": round(engine_ms, 2),
+              "particle_steps_per_s": round(total * steps / (engine_ms * 1e-3), 1),
+              "engine": {"launches_per_pass": 1, "kernels_per_step": engine.kernels_per_step,
+                         "graph_replay": engine.use_graph, "mask_skin": engine.mask_skin}}
+    del engine, state, control, params, initial
+
+    # the same batch as one solver per scene on a pool of streams
     pool = [torch.cuda.Stream(device) for _ in range(streams)]
     runs = []
     for k, sc in enumerate(batch):
         with torch.cuda.stream(pool[k % streams]):
             solver, state, control, params = pbf_solver_for(sc, ExecutionMode.forward, device)
-            runs.append((solver, state, [solver.allocate_state(), solver.allocate_state()], control, params,
-                         solver.allocate_step_cache()))
+            runs.append((pool[k % streams], solver, state, [solver.allocate_state(), solver.allocate_state()],
+                         control, params, solver.allocate_step_cache()))
     torch.cuda.synchronize()
 
     def forward(selected):
-        for solver, initial, states, _, _, _ in selected:
-            solver.copy_state(initial, states[0])
+        for stream, solver, initial, states, _, _, _ in selected:
+            with torch.cuda.stream(stream):
+                solver.copy_state(initial, states[0])
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for k in range(steps):
-            for solver, _, states, control, params, cache in selected:
-                solver.forward_step(states[k % 2], control, params, states[(k + 1) % 2], cache)
+            for stream, solver, _, states, control, params, cache in selected:
+                with torch.cuda.stream(stream):
+                    solver.forward_step(states[k % 2], control, params, states[(k + 1) % 2], cache)
         torch.cuda.synchronize()
         return (time.perf_counter() - t0) * 1e3
 
-    forward(runs)  # warm-up: allocations, code loading
-    batch_ms = forward(runs)
-    total = sum(sc.count for sc in batch)
-    record = {"workload": f"{len(batch)} uneven PBF dam breaks, {total} particles "
-                          f"({min(sc.count for sc in batch)} to {max(sc.count for sc in batch)}), "
-                          f"{steps} steps, dt 1/120, 4 iterations, {streams} streams",
-              "particles": total, "steps": steps, "forward_ms": round(batch_ms, 2),
-              "particle_steps_per_s": round(total * steps / (batch_ms * 1e-3), 1)}
-    overlapped_ms = forward(runs[:adjoint_scenes])
-    # the rollouts below run one scene after another, so their plain forward is timed that way too
-    plain_ms = sum(forward([run]) for run in runs[:adjoint_scenes])
-    adjoint_particles = sum(sc.count for sc in batch[:adjoint_scenes])
+    forward(runs)  # warm-up
+    streams_ms = forward(runs)
+    record["per_solver_streams"] = {"streams": streams, "forward_ms": round(streams_ms, 2),
+                                    "particle_steps_per_s": round(total * steps / (streams_ms * 1e-3), 1)}
     del runs
-    fwd_ms = bwd_ms = 0.0
-    for sc in batch[:adjoint_scenes]:
-        solver, state, control, params = pbf_solver_for(sc, ExecutionMode.differentiable, device)
-        rollout = PbfRollout(solver, steps, checkpoint_every=1, parameter_gradients=False)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        final = rollout.forward(state, control, params)
-        torch.cuda.synchronize()
-        t1 = time.perf_counter()
-        target = sc.positions.astype(np.float64) + np.array([[0.1], [-0.05], [0.08]])
-        seed = solver.allocate_state_adjoint()
-        seed.particles.positions.upload(2.0 * (final.particles.positions.numpy().astype(np.float64) - target)
-                                        / sc.count)
-        torch.cuda.synchronize()
-        t2 = time.perf_counter()
-        initial, _, _ = rollout.backward(seed)
-        torch.cuda.synchronize()
-        t3 = time.perf_counter()
-        if not np.isfinite(initial.particles.velocities.numpy()).all():
+
+    # forward + adjoint on the first scenes, one stream per scene
+    chosen = batch[:adjoint_scenes]
+    lanes = []
+    for k, sc in enumerate(chosen):
+        stream = torch.cuda.Stream(device)
+        with torch.cuda.stream(stream):
+            solver, state, control, params = pbf_solver_for(sc, ExecutionMode.differentiable, device)
+            plain, pstate, _, _ = pbf_solver_for(sc, ExecutionMode.forward, device)
+            lanes.append({"stream": stream, "solver": solver, "state": state, "control": control, "params": params,
+                          "plain": plain, "plain_states": [plain.allocate_state(), plain.allocate_state()],
+                          "plain_cache": plain.allocate_step_cache(), "scene": sc})
+    torch.cuda.synchronize()
+
+    def plain_forward():
+        for lane in lanes:
+            with torch.cuda.stream(lane["stream"]):
+                lane["plain"].copy_state(lane["state"], lane["plain_states"][0])
+        for k in range(steps):
+            for lane in lanes:
+                with torch.cuda.stream(lane["stream"]):
+                    st = lane["plain_states"]
+                    lane["plain"].forward_step(st[k % 2], lane["control"], lane["params"], st[(k + 1) % 2],
+                                               lane["plain_cache"])
+
+    def recorded_forward():
+        for lane in lanes:
+            with torch.cuda.stream(lane["stream"]):
+                lane["states"] = [lane["state"]]
+                lane["caches"] = []
+        for k in range(steps):
+            for lane in lanes:
+                with torch.cuda.stream(lane["stream"]):
+                    solver = lane["solver"]
+                    nxt, cache = solver.allocate_state(), solver.allocate_step_cache()
+                    solver.forward_step(lane["states"][-1], lane["control"], lane["params"], nxt, cache)
+                    lane["states"].append(nxt)
+                    lane["caches"].append(cache)
+
+    def backward():
+        from physica_amd.particle import ParticleParameterAdjoint
+
+        for lane in lanes:
+            with torch.cuda.stream(lane["stream"]):
+                solver, sc = lane["solver"], lane["scene"]
+                target = sc.positions.astype(np.float64) + np.array([[0.1], [-0.05], [0.08]])
+                seed = solver.allocate_state_adjoint()
+                final = lane["states"][-1].particles.positions.numpy().astype(np.float64)
+                seed.particles.positions.upload(2.0 * (final - target) / sc.count)
+                lane["adjoint"] = seed
+                lane["control_adjoint"] = solver.allocate_control_adjoint()
+                lane["parameter_adjoint"] = PBF.ParameterAdjoint(ParticleParameterAdjoint(None, None, None, None),
+                                                                 None, None, None, None, None, None)
+        for k in reversed(range(steps)):
+            for lane in lanes:
+                with torch.cuda.stream(lane["stream"]):
+                    solver = lane["solver"]
+                    previous = solver.allocate_state_adjoint()
+                    solver.vjp_step(lane["states"][k], lane["control"], lane["params"], lane["states"][k + 1],
+                                    lane["caches"][k], lane["adjoint"], previous, lane["control_adjoint"],
+                                    lane["parameter_adjoint"])
+                    lane["adjoint"] = previous
+
+    plain_forward()  # warm-up
+    plain_ms = timed(plain_forward)
+    fwd_ms = timed(recorded_forward)
+    bwd_ms = timed(backward)
+    for lane in lanes:
+        if not np.isfinite(lane["adjoint"].particles.velocities.numpy()).all():
             raise RuntimeError("non-finite batch gradient")
-        fwd_ms += (t1 - t0) * 1e3
-        bwd_ms += (t3 - t2) * 1e3
-        rollout.release()
-        del rollout, final, initial, solver
-    record["adjoint"] = {"scenes": adjoint_scenes, "particles": adjoint_particles,
-                         "plain_forward_overlapped_ms": round(overlapped_ms, 2),
-                         "plain_forward_ms": round(plain_ms, 2), "forward_ms": round(fwd_ms, 2),
-                         "backward_ms": round(bwd_ms, 2), "ratio": round((fwd_ms + bwd_ms) / plain_ms, 3)}
+    record["adjoint"] = {"scenes": adjoint_scenes, "particles": sum(sc.count for sc in chosen),
+                         "streams": adjoint_scenes, "plain_forward_ms": round(plain_ms, 2),
+                         "forward_ms": round(fwd_ms, 2), "backward_ms": round(bwd_ms, 2),
+                         "ratio": round((fwd_ms + bwd_ms) / plain_ms, 3)}
+    del lanes
+    torch.cuda.synchronize()
     return record
+
+
+def progress(message: str) -> None:
+    """One line per finished leg on stderr (the JSON line stays alone on stdout)."""
+    print(f"bench: {message}", file=sys.stderr, flush=True)
 
 
 def main(argv=None) -> int:
@@ -456,6 +538,7 @@ def main(argv=None) -> int:
             print("--mode fixed needs --fixed-steps >= 1", file=sys.stderr)
             return 2
         if args.fixed_steps > 0:
+            progress("DFSPH fixed 4 + 6 iterations")
             # the reference's fixed counts over steady steps: per-pass timing and the kernel roofline
             sc, f_ms, f_passes, f_counts, solver, states, cache, _ = measure(
                 lambda: setup_dfsph(device), args.fixed_steps, args.warmup, False, not args.no_pass_timing)
@@ -473,6 +556,7 @@ def main(argv=None) -> int:
             roofline = roofline_of(f_passes, n)
             del solver, states, cache
         if args.mode == "tolerance":
+            progress("DFSPH 0.1 % tolerance")
             sc, elapsed_ms, passes, counts, solver, states, cache, warm_counts = measure(
                 lambda: setup_dfsph(device, TOLERANCE), args.steps, args.warmup, False, False)
             n = sc.count
@@ -518,6 +602,7 @@ def main(argv=None) -> int:
     step_achieved = bytes_step * n * steps_timed / (elapsed_ms * 1e-3) / 1e9
 
     solver = states = cache = None  # release the headline solver's buffers
+    progress("headline done")
     pbf_leg = None
     if not args.no_pbf and args.workload == "dfsph":
         # configuration 1: the whole 200-step PBF dam break from the initial state
@@ -533,6 +618,7 @@ def main(argv=None) -> int:
                    "hbm_fraction_step": round(pbytes * pn * 200 / (p_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 5),
                    "algorithmic_bytes_per_particle_step": round(pbytes, 2)}
         del psolver, pstates, pcache
+    progress("PBF 32,768 done")
     scale = None
     if not args.no_scale and args.workload == "dfsph":
         scale = {}
@@ -562,6 +648,7 @@ def main(argv=None) -> int:
             scale[f"{method}_4m"] = rec
             del ssolver, sstates, scache
 
+    progress("scale test done")
     baseline = baseline_1core = None
     if not args.no_cpu_baseline:
         baseline = cpu_baseline(args.workload, sc)
@@ -569,9 +656,11 @@ def main(argv=None) -> int:
             from physica_amd import scenes
 
             baseline_1core = cpu_baseline("dfsph", scenes.dfsph_dam_break(shape=(64, 32, 64)), threads=1)
+    progress("CPU baselines done")
     adjoint = None
     if not args.no_adjoint:
         adjoint = adjoint_ratio(device, steps=args.adjoint_steps)
+    progress("differentiable rollout done")
     batched = None
     if not args.no_batched:
         batched = batched_pbf(device, streams=args.batch_streams)

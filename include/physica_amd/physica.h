@@ -658,6 +658,40 @@ phx_status phx_pbf_cache_vorticity_magnitudes(const phx_pbf_step_cache* cache, f
 phx_status phx_write_vectors(phx_stream stream, uint32_t count, phx_const_vector_view positions,
                              phx_const_vector_view velocities, float* positions_xyz, float* velocities_xyz);
 
+/* ------------------------------------------------------- batched PBF scenes */
+/* Independent PBF scenes of uneven particle counts stepped together. The
+ * reference runs one PBF solver per scene and stream (domain/domain.ixx:
+ * 121-124), each substep the fixed launch sequence of PBF::forward_step
+ * (pbf/pbf.cpp:167-191). A batch runs that sequence for every scene at once:
+ * each pass of a substep is one launch over all scenes, and with static
+ * boxes one substep is captured in a HIP graph and replayed. Forward mode,
+ * no boundary particles; the scenes share the support radius, the time step
+ * and the iteration count. The scenes' particles are concatenated in scene
+ * order: scene k holds particles offsets[k] .. offsets[k + 1] - 1 of every
+ * state, control and parameter array, and its results equal its own PBF
+ * solver's bit for bit. */
+typedef struct phx_pbf_batch phx_pbf_batch;
+
+typedef struct phx_pbf_batch_configuration {
+    float mask_skin;    /* neighbour-mask skin in units of h (0: 0.3); results do not depend on it */
+    uint32_t use_graph; /* replay a captured substep (only when every box is static) */
+} phx_pbf_batch_configuration;
+
+phx_status phx_pbf_batch_create(const phx_domain_configuration* domains, uint32_t scene_count,
+                                const phx_pbf_configuration* configuration,
+                                const phx_pbf_batch_configuration* batch_configuration, phx_stream stream,
+                                phx_pbf_batch** batch);
+phx_status phx_pbf_batch_destroy(phx_pbf_batch* batch);
+/* offsets[0 .. scene_count]: the first particle of each scene, then the total */
+phx_status phx_pbf_batch_particle_offsets(const phx_pbf_batch* batch, uint64_t* offsets);
+/* `steps` forward substeps of every scene, in place on `state`, enqueued on
+ * the batch's stream (ordered after and before the caller's work on it);
+ * state->particles.step_index advances by `steps`. */
+phx_status phx_pbf_batch_run(phx_pbf_batch* batch, phx_pbf_state* state, const phx_control* control,
+                             const phx_pbf_parameters* parameters, uint32_t steps);
+/* kernel launches of one substep over all scenes */
+phx_status phx_pbf_batch_kernels_per_step(const phx_pbf_batch* batch, uint32_t* kernels);
+
 #ifdef __cplusplus
 }
 #endif

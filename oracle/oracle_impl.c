@@ -1419,12 +1419,20 @@ void SUF(oracle_pbf_jvp)(const oracle_domain* d, uint32_t iterations, uint64_t s
             cell_range cr[27];
             const int nr = stencil(g, v3_load(CV(r.xs), i), cr);
             real acc = K(0.0);
-            for (int c = 0; c < nr; ++c)
+            for (int c = 0; c < nr; ++c) {
                 for (uint32_t s = cr[c].first; s < cr[c].last; ++s) {
                     const uint32_t j = g->idx[s];
                     const v3 dd = v3_sub(pi, v3_load(p, j));
                     acc += masses_t[j] * poly6(dd, h) + masses[j] * v3_dot(poly6_gradient(dd, h), v3_sub(ti, v3_load(CV(pt), j)));
                 }
+                /* boundary samples (density.cu:59-63): rho0_i V_b W, the sample fixed */
+                for (uint32_t s = cr[c].bfirst; s < cr[c].blast; ++s) {
+                    const uint32_t b = g->bidx[s];
+                    const v3 dd = v3_sub(pi, boundary_position(g, b));
+                    acc += boundary_volume(g, b) *
+                           (rest_densities_t[i] * poly6(dd, h) + rest_densities[i] * v3_dot(poly6_gradient(dd, h), ti));
+                }
+            }
             rho_t[i] = acc;
         }
         /* lambda_jvp_kernel (pbf.cu:129-176) */
@@ -1437,7 +1445,7 @@ void SUF(oracle_pbf_jvp)(const oracle_domain* d, uint32_t iterations, uint64_t s
             real den_t = relaxation_t[i];
             cell_range cr[27];
             const int nr = stencil(g, v3_load(CV(r.xs), i), cr);
-            for (int c = 0; c < nr; ++c)
+            for (int c = 0; c < nr; ++c) {
                 for (uint32_t s = cr[c].first; s < cr[c].last; ++s) {
                     const uint32_t j = g->idx[s];
                     if (j == i) continue;
@@ -1452,6 +1460,15 @@ void SUF(oracle_pbf_jvp)(const oracle_domain* d, uint32_t iterations, uint64_t s
                     sg_t = v3_add(sg_t, ng_t);
                     den_t += K(2.0) * v3_dot(ng, ng_t);
                 }
+                /* boundary samples (pbf.cu:164-168): V_b grad W in the gradient sum only */
+                for (uint32_t s = cr[c].bfirst; s < cr[c].blast; ++s) {
+                    const uint32_t b = g->bidx[s];
+                    const v3 dd = v3_sub(pi, boundary_position(g, b));
+                    const real vb = boundary_volume(g, b);
+                    sg = v3_add(sg, v3_scale(spiky_gradient(dd, h, k), vb));
+                    sg_t = v3_add(sg_t, v3_scale(spiky_hessian_r(dd, ti, h, k), vb));
+                }
+            }
             den_t += K(2.0) * v3_dot(sg, sg_t);
             const real den = r.den[it][i];
             const real con = r.rho[it][i] / rho0 - K(1.0);
@@ -1467,7 +1484,7 @@ void SUF(oracle_pbf_jvp)(const oracle_domain* d, uint32_t iterations, uint64_t s
             v3 res = v3_make(0, 0, 0);
             cell_range cr[27];
             const int nr = stencil(g, v3_load(CV(r.xs), i), cr);
-            for (int c = 0; c < nr; ++c)
+            for (int c = 0; c < nr; ++c) {
                 for (uint32_t s = cr[c].first; s < cr[c].last; ++s) {
                     const uint32_t j = g->idx[s];
                     if (j == i) continue;
@@ -1483,6 +1500,19 @@ void SUF(oracle_pbf_jvp)(const oracle_domain* d, uint32_t iterations, uint64_t s
                     res = v3_add(res, v3_add(v3_scale(spiky_gradient(dd, h, k), cf_t * mu + cf * mu_t),
                                              v3_scale(spiky_hessian_r(dd, dd_t, h, k), cf * mu)));
                 }
+                /* boundary samples (pbf.cu:321-329): V_b (lambda_i + s_corr) grad W */
+                for (uint32_t s = cr[c].bfirst; s < cr[c].blast; ++s) {
+                    const uint32_t b = g->bidx[s];
+                    const v3 dd = v3_sub(pi, boundary_position(g, b));
+                    const artificial_t a = artificial_full(dd, h, strength[i], exponent[i], radius[i]);
+                    const real a_t = v3_dot(a.gradient, ti) + a.d_strength * strength_t[i] +
+                                     a.d_exponent * exponent_t[i] + a.d_radius * radius_t[i];
+                    const real mu = r.lam[it][i] + a.value;
+                    res = v3_add(res, v3_scale(v3_add(v3_scale(spiky_gradient(dd, h, k), lam_t[i] + a_t),
+                                                      v3_scale(spiky_hessian_r(dd, ti, h, k), mu)),
+                                               boundary_volume(g, b)));
+                }
+            }
             v3 out = v3_add(ti, res);
             const uint8_t m = r.mask[it][i];
             if (m & 1u) out.x = K(0.0);
@@ -1764,7 +1794,7 @@ void SUF(oracle_pbf_vjp)(const oracle_domain* d, uint32_t iterations, uint64_t s
             double lcon = 0.0, mcon = 0.0, rcon = 0.0, scon = 0.0, econ = 0.0, qcon = 0.0;
             cell_range cr[27];
             const int nr = stencil(g, v3_load(CV(r.xs), a), cr);
-            for (int c = 0; c < nr; ++c)
+            for (int c = 0; c < nr; ++c) {
                 for (uint32_t s = cr[c].first; s < cr[c].last; ++s) {
                     const uint32_t j = g->idx[s];
                     if (j == a) continue;
@@ -1796,6 +1826,24 @@ void SUF(oracle_pbf_vjp)(const oracle_domain* d, uint32_t iterations, uint64_t s
                     lcon += rsa;
                     mcon += rmu / rrho0 * d3_dot(ob, rgr);
                 }
+                /* boundary samples (pbf.cu:390-397): the sample is fixed, so
+                 * the pair reaches only a's adjoints */
+                for (uint32_t s = cr[c].bfirst; s < cr[c].blast; ++s) {
+                    const uint32_t b = g->bidx[s];
+                    const v3 dd = v3_sub(pi, boundary_position(g, b));
+                    const d3 gr = d3_of(spiky_gradient(dd, hr, kr));
+                    const artificial_t art = artificial_full(dd, hr, strength[a], exponent[a], radius[a]);
+                    const double vol = (double)boundary_volume(g, b);
+                    const double mu = (double)lam[a] + (double)art.value;
+                    const double sa = vol * d3_dot(oa, gr);
+                    pcon = d3_add(pcon, spiky_hessian_d(dd, d3_scale(oa, vol * mu), h, k));
+                    pcon = d3_add(pcon, d3_scale(d3_of(art.gradient), sa));
+                    lcon += sa;
+                    scon += sa * (double)art.d_strength;
+                    econ += sa * (double)art.d_exponent;
+                    qcon += sa * (double)art.d_radius;
+                }
+            }
             d3_acc(pn.c, a, pcon);
             lama[a] += lcon;
             masses_a[a] += mcon;
@@ -1818,7 +1866,7 @@ void SUF(oracle_pbf_vjp)(const oracle_domain* d, uint32_t iterations, uint64_t s
             double mcon = 0.0, rcon = lama[a] * (double)r.rho[it][a] / (den * rho0 * rho0);
             cell_range cr[27];
             const int nr = stencil(g, v3_load(CV(r.xs), a), cr);
-            for (int c = 0; c < nr; ++c)
+            for (int c = 0; c < nr; ++c) {
                 for (uint32_t s = cr[c].first; s < cr[c].last; ++s) {
                     const uint32_t j = g->idx[s];
                     if (j == a) continue;
@@ -1844,6 +1892,14 @@ void SUF(oracle_pbf_vjp)(const oracle_domain* d, uint32_t iterations, uint64_t s
                     const double rcbar = 2.0 * jdbar * (rcf * d3_dot(rgr, rgr) + d3_dot(rgr, sgj));
                     mcon += rcbar / jrho0;
                 }
+                /* boundary samples (pbf.cu:226-229): V_b grad W enters S only, so
+                 * its position adjoint is H(d) (2 D_bar V_b S) */
+                for (uint32_t s = cr[c].bfirst; s < cr[c].blast; ++s) {
+                    const uint32_t b = g->bidx[s];
+                    const v3 dd = v3_sub(pi, boundary_position(g, b));
+                    pcon = d3_add(pcon, spiky_hessian_d(dd, d3_scale(sgi, 2.0 * dbar * (double)boundary_volume(g, b)), h, k));
+                }
+            }
             d3_acc(pn.c, a, pcon);
             rhoa[a] -= lama[a] / (den * rho0);
             masses_a[a] += mcon;
@@ -1856,10 +1912,10 @@ void SUF(oracle_pbf_vjp)(const oracle_domain* d, uint32_t iterations, uint64_t s
             const uint32_t a = (uint32_t)ii;
             const v3 pi = v3_load(p, a);
             d3 pcon = d3_make(0, 0, 0);
-            double mcon = 0.0;
+            double mcon = 0.0, rcon = 0.0;
             cell_range cr[27];
             const int nr = stencil(g, v3_load(CV(r.xs), a), cr);
-            for (int c = 0; c < nr; ++c)
+            for (int c = 0; c < nr; ++c) {
                 for (uint32_t s = cr[c].first; s < cr[c].last; ++s) {
                     const uint32_t j = g->idx[s];
                     const v3 dd = v3_sub(pi, v3_load(p, j));
@@ -1867,8 +1923,18 @@ void SUF(oracle_pbf_vjp)(const oracle_domain* d, uint32_t iterations, uint64_t s
                     pcon = d3_add(pcon, d3_scale(gw, rhoa[a] * (double)masses[j] + rhoa[j] * (double)masses[a]));
                     mcon += rhoa[j] * (double)poly6(dd, hr);
                 }
+                /* boundary samples (density.cu:88-93): rho0_a V_b W */
+                for (uint32_t s = cr[c].bfirst; s < cr[c].blast; ++s) {
+                    const uint32_t b = g->bidx[s];
+                    const v3 dd = v3_sub(pi, boundary_position(g, b));
+                    const double vol = (double)boundary_volume(g, b);
+                    pcon = d3_add(pcon, d3_scale(d3_of(poly6_gradient(dd, hr)), rhoa[a] * (double)rest_densities[a] * vol));
+                    rcon += rhoa[a] * vol * (double)poly6(dd, hr);
+                }
+            }
             d3_acc(pn.c, a, pcon);
             masses_a[a] += mcon;
+            rest_densities_a[a] += rcon;
         }
         dfield tmp = pa;
         pa = pn;

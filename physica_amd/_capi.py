@@ -138,6 +138,10 @@ class DfsphConfiguration(ctypes.Structure):
     ]
 
 
+class PbfBatchConfiguration(ctypes.Structure):
+    _fields_ = [("mask_skin", c_float), ("use_graph", c_uint32)]
+
+
 class DfsphParameters(ctypes.Structure):
     _fields_ = [("divergence_relaxation", c_void_p), ("density_relaxation", c_void_p)]
 
@@ -321,6 +325,15 @@ PROTOTYPES = {
         [c_void_p, POINTER(DomainConfiguration), c_uint64, VectorView, c_void_p, c_size_t, c_void_p, c_void_p,
          c_void_p, c_void_p, c_void_p, c_void_p, POINTER(NeighborhoodView)],
     ),
+    "phx_pbf_batch_create": (
+        c_int,
+        [POINTER(DomainConfiguration), c_uint32, POINTER(PbfConfiguration), POINTER(PbfBatchConfiguration), c_void_p,
+         POINTER(c_void_p)],
+    ),
+    "phx_pbf_batch_destroy": (c_int, [c_void_p]),
+    "phx_pbf_batch_particle_offsets": (c_int, [c_void_p, POINTER(c_uint64)]),
+    "phx_pbf_batch_run": (c_int, [c_void_p, POINTER(PbfState), POINTER(Control), POINTER(PbfParameters), c_uint32]),
+    "phx_pbf_batch_kernels_per_step": (c_int, [c_void_p, POINTER(c_uint32)]),
     "phx_density_forward": (
         c_int,
         [c_void_p, c_uint32, c_float, c_int, VectorView, VectorView, ParticleParameters, NeighborhoodView,

@@ -1,0 +1,82 @@
+"""Autotuner of the batched PBF engine.
+
+Times a batch of scenes for each candidate setting the engine exposes (the
+neighbour-mask skin, which trades candidates per walk against passes that
+fall back to the full stencil, and replay of a captured substep against
+eager launches) and writes every timing and the fastest setting to a JSON
+file, which PbfBatch reads for its defaults (physica_amd/tuning/
+pbf_batch.json). The engine's results do not depend on these settings, so
+the tuner checks that every candidate ends in the same state bit for bit.
+
+    python -m physica_amd.autotune --out gpurun_out/pbf_batch.json
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+import torch
+
+from . import scenes
+from .batch import TUNING, PbfBatch
+from .particle import PBF
+
+SKINS = (0.1, 0.2, 0.3, 0.45)
+
+
+def tune(batch_scenes, steps=20, skins=SKINS, graphs=(True, False), device="cuda"):
+    """Returns {"candidates": [...], "best": {...}} for the given scenes."""
+    candidates = []
+    reference = None
+    for skin in skins:
+        for graph in graphs:
+            engine = PbfBatch([sc.domain for sc in batch_scenes], PBF.Configuration(pressure_iterations=4),
+                              mask_skin=skin, use_graph=graph, device=device)
+            state, control, params = engine.allocate_state(), engine.allocate_control(), engine.allocate_parameters()
+            engine.upload(batch_scenes, state, params)
+            engine.run(state, control, params, 2)  # warm-up (graph capture)
+            engine.upload(batch_scenes, state, params)
+            state.particles.step_index = 0
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            engine.run(state, control, params, steps)
+            torch.cuda.synchronize()
+            ms = (time.perf_counter() - t0) * 1e3 / steps
+            final = state.particles.positions.numpy()
+            if reference is None:
+                reference = final
+            elif not np.array_equal(final.view(np.uint32), reference.view(np.uint32)):
+                raise RuntimeError(f"mask skin {skin}, graph {graph}: the state differs from the first candidate")
+            candidates.append({"mask_skin": skin, "use_graph": graph, "ms_per_step": round(ms, 4)})
+            print(f"autotune: skin {skin} graph {graph}: {ms:.3f} ms/step", file=sys.stderr, flush=True)
+            del engine, state, control, params
+    best = min(candidates, key=lambda c: c["ms_per_step"])
+    return {"candidates": candidates, "best": {k: best[k] for k in ("mask_skin", "use_graph")}}
+
+
+def main(argv=None) -> int:
+    p = argparse.ArgumentParser(description=__doc__)
+    p.add_argument("--out", type=Path, default=TUNING)
+    p.add_argument("--steps", type=int, default=20)
+    args = p.parse_args(argv)
+    if not torch.cuda.is_available():
+        print("autotune needs a HIP device", file=sys.stderr)
+        return 2
+    batch = scenes.batched_pbf_scenes()
+    result = tune(batch, steps=args.steps)
+    props = torch.cuda.get_device_properties(0)
+    result["device"] = {"name": props.name, "arch": getattr(props, "gcnArchName", "")}
+    result["workload"] = (f"{len(batch)} uneven PBF dam breaks, {sum(sc.count for sc in batch)} particles, "
+                          f"4 iterations, {args.steps} timed steps after 2 warm-up steps")
+    args.out.parent.mkdir(parents=True, exist_ok=True)
+    args.out.write_text(json.dumps(result, indent=1) + "\n")
+    print(json.dumps(result["best"]), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -34,6 +34,12 @@ struct phx_pbf {
     phx_pbf(const phx_domain_configuration* d, const phx_pbf_configuration* c, phx_execution_mode m, hipStream_t s)
         : impl(d, c, m, s) {}
 };
+struct phx_pbf_batch {
+    phx::PbfBatch impl;
+    phx_pbf_batch(const phx_domain_configuration* d, uint32_t count, const phx_pbf_configuration* c,
+                  const phx_pbf_batch_configuration* b, hipStream_t s)
+        : impl(d, count, c, b, s) {}
+};
 struct phx_pbf_step_cache {
     const phx_pbf* owner = nullptr;
     phx::PbfCache impl;
@@ -983,6 +989,45 @@ phx_status phx_pbf_cache_vorticity_magnitudes(const phx_pbf_step_cache* cache, f
         ch.add(cache->impl.magnitudes.data(), magnitudes);
         phx::scatter_to_original(cache->owner->impl.stream(), cache->impl.neighborhood.perm.data(),
                                  cache->impl.neighborhood.particles, ch.get());
+    });
+}
+
+phx_status phx_pbf_batch_create(const phx_domain_configuration* domains, uint32_t scene_count,
+                                const phx_pbf_configuration* configuration,
+                                const phx_pbf_batch_configuration* batch_configuration, phx_stream stream,
+                                phx_pbf_batch** batch) {
+    return guarded([&] {
+        phx::require(batch != nullptr, PHX_ERROR_INVALID_ARGUMENT, "null batch handle");
+        *batch = nullptr;
+        *batch = new phx_pbf_batch(domains, scene_count, configuration, batch_configuration, stream_of(stream));
+    });
+}
+
+phx_status phx_pbf_batch_destroy(phx_pbf_batch* batch) {
+    return guarded([&] { delete batch; });
+}
+
+phx_status phx_pbf_batch_particle_offsets(const phx_pbf_batch* batch, uint64_t* offsets) {
+    return guarded([&] {
+        phx::require(batch != nullptr && offsets != nullptr, PHX_ERROR_INVALID_ARGUMENT, "null argument");
+        const auto& o = batch->impl.offsets();
+        std::memcpy(offsets, o.data(), sizeof(uint64_t) * o.size());
+    });
+}
+
+phx_status phx_pbf_batch_run(phx_pbf_batch* batch, phx_pbf_state* state, const phx_control* control,
+                             const phx_pbf_parameters* parameters, uint32_t steps) {
+    return guarded([&] {
+        phx::require(batch != nullptr && state != nullptr && control != nullptr && parameters != nullptr,
+                     PHX_ERROR_INVALID_ARGUMENT, "null argument");
+        batch->impl.run(*state, *control, *parameters, steps);
+    });
+}
+
+phx_status phx_pbf_batch_kernels_per_step(const phx_pbf_batch* batch, uint32_t* kernels) {
+    return guarded([&] {
+        phx::require(batch != nullptr && kernels != nullptr, PHX_ERROR_INVALID_ARGUMENT, "null argument");
+        *kernels = batch->impl.kernels_per_step();
     });
 }
 

@@ -182,13 +182,43 @@ struct PredictInput {
     Vec predicted{};
 };
 
+// A batch of independent scenes stepped together: every scene's particles
+// occupy one contiguous range of the batch's arrays (original and sorted
+// order alike, in scene order), and every scene has its own grid, whose
+// cells occupy one contiguous range of the batch's cell offsets. Workgroups
+// are listed in a table, each serving consecutive particles of one scene, so
+// a workgroup's grid, collision box and stencil slot table are its scene's.
+struct BatchBlock {
+    uint32_t scene;
+    uint32_t first;  // first particle of the workgroup (batch index)
+    uint32_t count;  // particles of the workgroup (0: padding)
+    uint32_t pad;
+};
+struct BatchScene {
+    uint32_t cell_base;  // first cell of the scene's grid in the batch's offsets
+    uint32_t res[3];
+    float minimum[3];
+    float maximum[3];
+    float velocity[3];
+    float particle_radius;
+    float gravity[3];
+    uint32_t no_slip;
+};
+struct BatchView {
+    const BatchBlock* blocks;  // null: a single scene
+    const BatchScene* scenes;
+    float time;  // step time of the grid and collision box (index * dt)
+    float h;
+};
+
 // Scratch of one grid build, carved from a single byte buffer.
 struct GridScratch {
-    uint32_t* keys;      // cell key per particle, original order
-    uint32_t* slots;     // arrival slot within the cell (atomic)
-    uint32_t* unstable;  // particles grouped by cell, arbitrary order inside
-    uint32_t* counts;    // C + 1 histogram
-    uint32_t* partials;  // scan tile sums
+    uint32_t* keys;       // cell key per particle, original order
+    uint32_t* slots;      // arrival slot within the cell (atomic)
+    uint32_t* unstable;   // particles grouped by cell, arbitrary order inside
+    uint32_t* counts;     // C + 1 histogram, then the count of big cells
+    uint32_t* partials;   // scan tile sums
+    uint32_t* big_cells;  // cells whose members are radix-sorted
 };
 size_t grid_scratch_bytes(uint32_t particles, uint32_t cells);
 GridScratch carve_grid_scratch(void* base, size_t bytes, uint32_t particles, uint32_t cells);
@@ -256,6 +286,15 @@ BoundaryScratch carve_boundary_scratch(void* base, size_t bytes, uint32_t sample
 void build_boundary_grid(hipStream_t stream, const Domain& domain, uint64_t step, const BoundarySource& samples,
                          uint32_t count, const BoundaryScratch& scratch, const GridScratch& grid_scratch,
                          uint32_t* perm, uint32_t* offsets, float4* x, float4* v);
+
+// The grid build of a batch (build_grid per scene, one launch per kernel):
+// each particle predicts x* with its scene's gravity, takes the key of its
+// cell in its scene's grid (time batch.time) plus the scene's cell base, and
+// one counting sort over the batch's cells orders every scene at once, each
+// scene's particles stably by index exactly as its own build would.
+void build_batch_grid(hipStream_t stream, uint32_t particles, uint32_t cells, uint32_t block_count,
+                      const BatchView& batch, CVec positions, const PredictInput& predict, const Channels& gather,
+                      const GridScratch& scratch, uint32_t* perm, uint32_t* offsets);
 
 // Owning grid state used by the solvers: scratch plus the permutation and
 // offsets of the last build (kept in the step cache for replay).

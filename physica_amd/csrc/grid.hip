@@ -5,7 +5,11 @@
 // ascending sort of (key, index), cell_offsets[c] = lower_bound(keys, c).
 // The implementation differs: keys are < C, so a histogram + exclusive scan
 // gives the offsets directly, and a stable order within a cell comes from
-// ranking each particle by original index among its cell's members.
+// ranking each particle by original index among its cell's members: a scan
+// of the members for cells of at most kRankScan particles, and for larger
+// cells (any distribution, up to every particle in one cell) a workgroup per
+// cell that radix-sorts the cell's indices, so the build stays O(N).
+#include <algorithm>
 #include <cmath>
 #include <vector>
 
@@ -15,7 +19,15 @@ namespace phx {
 namespace {
 
 constexpr uint32_t kBlock = 256;
+constexpr uint32_t kWaves = kBlock / 64;
 constexpr uint32_t kScanItems = 8;
+// cells with more members than this are ordered by big_cell_sort_kernel
+constexpr uint32_t kRankScan = 128;
+constexpr uint32_t kBigCellGroups = 512;  // workgroups of big_cell_sort_kernel
+
+// Most cells a build can queue for big_cell_sort_kernel (each holds more
+// than kRankScan particles).
+size_t big_cell_capacity(uint32_t particles) { return static_cast<size_t>(particles) / (kRankScan + 1u) + 1u; }
 constexpr uint32_t kScanTile = kBlock * kScanItems;
 
 __global__ void __launch_bounds__(kBlock) fill_u32_kernel(uint32_t* data, size_t count, uint32_t value) {
@@ -71,6 +83,33 @@ __global__ void __launch_bounds__(kBlock)
     slots[i] = atomicAdd(&counts[key], 1u);
 }
 
+// key_count_kernel<true> of a batch: workgroup b serves particles
+// [first, first + count) of one scene, whose gravity predicts x* and whose
+// grid (box, resolution, cell base) keys it.
+__global__ void __launch_bounds__(kBlock)
+    batch_key_count_kernel(BatchView batch, CVec positions, PredictInput predict, uint32_t* keys, uint32_t* slots,
+                           uint32_t* counts) {
+    const BatchBlock blk = batch.blocks[blockIdx.x];
+    if (threadIdx.x >= blk.count) return;
+    const BatchScene sc = batch.scenes[blk.scene];
+    const uint32_t i = blk.first + threadIdx.x;
+    const Float3 gravity{sc.gravity[0], sc.gravity[1], sc.gravity[2]};
+    const Float3 pv = add(load(predict.velocities, i), scale(add(gravity, load(predict.controls, i)), predict.dt));
+    const Float3 p = add(load(positions, i), scale(pv, predict.dt));
+    store(predict.predicted, i, p);
+    KeyBox box{};
+    for (int a = 0; a < 3; ++a) {
+        box.minimum[a] = sc.minimum[a];
+        box.velocity[a] = sc.velocity[a];
+        box.res[a] = sc.res[a];
+    }
+    box.h = batch.h;
+    box.time = batch.time;
+    const uint32_t key = sc.cell_base + cell_key(box, p);
+    keys[i] = key;
+    slots[i] = atomicAdd(&counts[key], 1u);
+}
+
 // key_kernel with moving_positions (neighborhood.cu:20-39 as called at
 // :63-69): the sample is first moved to x0 + t v.
 __global__ void __launch_bounds__(kBlock)
@@ -116,14 +155,21 @@ __global__ void __launch_bounds__(kBlock)
 // Final position of particle i = cell start + number of cell members with a
 // smaller original index; equal keys therefore keep input order, which is the
 // stable-sort order of cub::DeviceRadixSort::SortPairs (neighborhood.cu:57).
+// A cell with more than kRankScan members is queued once (by its member that
+// arrived first) for big_cell_sort_kernel instead.
 __global__ void __launch_bounds__(kBlock)
-    rank_gather_kernel(uint32_t n, const uint32_t* keys, const uint32_t* offsets, const uint32_t* unstable,
-                       uint32_t* perm, Channels channels) {
+    rank_gather_kernel(uint32_t n, const uint32_t* keys, const uint32_t* slots, const uint32_t* offsets,
+                       const uint32_t* unstable, uint32_t* perm, Channels channels, uint32_t* big_cells,
+                       uint32_t* big_count) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const uint32_t key = keys[i];
     const uint32_t first = offsets[key];
     const uint32_t last = offsets[key + 1u];
+    if (last - first > kRankScan) {
+        if (slots[i] == 0u) big_cells[atomicAdd(big_count, 1u)] = key;
+        return;
+    }
     uint32_t rank = 0;
     for (uint32_t s = first; s < last; ++s) rank += unstable[s] < i ? 1u : 0u;
     const uint32_t pos = first + rank;
@@ -235,6 +281,94 @@ __global__ void __launch_bounds__(kBlock)
     }
 }
 
+// Lanes of this wave whose `digit` equals the lane's own, among the lanes in
+// `active` (eight ballots, one per digit bit).
+__device__ __forceinline__ uint64_t digit_peers(uint32_t digit, uint64_t active) {
+    uint64_t peers = active;
+#pragma unroll
+    for (uint32_t b = 0; b < 8u; ++b) {
+        const bool bit = ((digit >> b) & 1u) != 0u;
+        const uint64_t set = __ballot(bit);
+        peers &= bit ? set : ~set;
+    }
+    return peers;
+}
+
+// The cells queued by rank_gather_kernel, one workgroup per cell at a time:
+// the cell's member indices (unstable[first .. last), distinct) are sorted
+// ascending by an LSD radix sort of 8-bit digits over the index bits, each
+// pass a digit histogram then a stable scatter in rounds of one member per
+// lane (a member's place among equal digits is its wave rank plus the counts
+// of earlier waves and rounds). The passes alternate between perm and slots
+// (free after the scatter into `unstable`) and end in perm; then the
+// members' channels are gathered. Every workgroup loops until the queue is
+// empty, so the grid drains whatever the queue holds.
+__global__ void __launch_bounds__(kBlock)
+    big_cell_sort_kernel(const uint32_t* offsets, const uint32_t* unstable, uint32_t* slots, uint32_t* perm,
+                         const uint32_t* big_cells, const uint32_t* big_count, uint32_t index_bits,
+                         Channels channels) {
+    __shared__ uint32_t base[256];
+    __shared__ uint32_t wave_counts[kWaves][256];
+    const uint32_t t = threadIdx.x;
+    const uint32_t lane = t & 63u;
+    const uint32_t wave = t >> 6;
+    for (uint32_t w = 0; w < kWaves; ++w) wave_counts[w][t] = 0u;
+    const uint32_t cells = *big_count;
+    const uint32_t passes = (index_bits + 7u) / 8u;
+    for (uint32_t q = blockIdx.x; q < cells; q += gridDim.x) {
+        const uint32_t c = big_cells[q];
+        const uint32_t first = offsets[c];
+        const uint32_t k = offsets[c + 1u] - first;
+        const uint32_t* src = unstable + first;
+        uint32_t* dst = passes % 2u == 1u ? perm + first : slots + first;
+        for (uint32_t p = 0; p < passes; ++p) {
+            const uint32_t shift = 8u * p;
+            __syncthreads();
+            base[t] = 0u;
+            __syncthreads();
+            for (uint32_t e = t; e < k; e += kBlock) atomicAdd(&base[(src[e] >> shift) & 255u], 1u);
+            __syncthreads();
+            uint32_t total;
+            const uint32_t start = block_exclusive_scan(base[t], total);
+            base[t] = start;
+            __syncthreads();
+            for (uint32_t r = 0; r < k; r += kBlock) {
+                const uint32_t e = r + t;
+                const bool valid = e < k;
+                const uint32_t v = valid ? src[e] : 0u;
+                const uint32_t d = (v >> shift) & 255u;
+                const uint64_t peers = digit_peers(d, __ballot(valid));
+                const uint32_t rank = static_cast<uint32_t>(__popcll(peers & ((1ull << lane) - 1ull)));
+                if (valid && rank == 0u) wave_counts[wave][d] = static_cast<uint32_t>(__popcll(peers));
+                __syncthreads();
+                if (valid) {
+                    uint32_t place = base[d] + rank;
+                    for (uint32_t w = 0; w < wave; ++w) place += wave_counts[w][d];
+                    dst[place] = v;
+                }
+                __syncthreads();
+                uint32_t round = 0u;
+                for (uint32_t w = 0; w < kWaves; ++w) {
+                    round += wave_counts[w][t];
+                    wave_counts[w][t] = 0u;
+                }
+                base[t] += round;
+                __syncthreads();
+            }
+            // the next pass reads what this one wrote
+            src = dst;
+            dst = dst == perm + first ? slots + first : perm + first;
+        }
+        __syncthreads();
+        for (uint32_t e = t; e < k; e += kBlock) {
+            const uint32_t pos = first + e;
+            const uint32_t i = perm[pos];
+            for (uint32_t ch = 0; ch < channels.count; ++ch)
+                channels.dst[ch][pos * channels.stride[ch]] = channels.src[ch][i];
+        }
+    }
+}
+
 }  // namespace
 
 void fill_u32(hipStream_t stream, uint32_t* data, size_t count, uint32_t value) {
@@ -250,6 +384,24 @@ void fill_f32(hipStream_t stream, float* data, size_t count, float value) {
 }
 
 size_t exclusive_scan_partials(size_t count) { return (count + kScanTile - 1) / kScanTile; }
+
+namespace {
+// The stable in-cell order of a fluid build: rank_gather_kernel for small
+// cells, then big_cell_sort_kernel for the cells it queued (counts[C + 1]).
+void order_cells(hipStream_t stream, uint32_t n, uint32_t cells, const GridScratch& scratch, const uint32_t* offsets,
+                 uint32_t* perm, const Channels& gather) {
+    uint32_t* big_count = scratch.counts + static_cast<size_t>(cells) + 1u;
+    rank_gather_kernel<<<blocks_for(n, kBlock), kBlock, 0, stream>>>(n, scratch.keys, scratch.slots, offsets,
+                                                                     scratch.unstable, perm, gather,
+                                                                     scratch.big_cells, big_count);
+    PHX_LAUNCH_CHECK();
+    const uint32_t index_bits = n > 1u ? 32u - static_cast<uint32_t>(__builtin_clz(n - 1u)) : 1u;
+    const uint32_t groups = static_cast<uint32_t>(std::min<size_t>(kBigCellGroups, big_cell_capacity(n)));
+    big_cell_sort_kernel<<<groups, kBlock, 0, stream>>>(offsets, scratch.unstable, scratch.slots, perm,
+                                                        scratch.big_cells, big_count, index_bits, gather);
+    PHX_LAUNCH_CHECK();
+}
+}  // namespace
 
 void exclusive_scan(hipStream_t stream, const uint32_t* in, uint32_t* out, size_t count, uint32_t* partials) {
     if (count == 0) return;
@@ -353,7 +505,8 @@ size_t grid_scratch_bytes(uint32_t particles, uint32_t cells) {
     auto round = [](size_t b) { return (b + 255u) & ~static_cast<size_t>(255u); };
     const size_t n = particles;
     const size_t c = static_cast<size_t>(cells) + 1u;
-    return round(n * 4u) * 3u + round(c * 4u) + round(exclusive_scan_partials(c) * 4u) + 256u;
+    return round(n * 4u) * 3u + round((c + 1u) * 4u) + round(exclusive_scan_partials(c) * 4u) +
+           round(big_cell_capacity(particles) * 4u) + 256u;
 }
 
 GridScratch carve_grid_scratch(void* base, size_t bytes, uint32_t particles, uint32_t cells) {
@@ -370,8 +523,10 @@ GridScratch carve_grid_scratch(void* base, size_t bytes, uint32_t particles, uin
     s.unstable = reinterpret_cast<uint32_t*>(p);
     p += round(n * 4u);
     s.counts = reinterpret_cast<uint32_t*>(p);
-    p += round(c * 4u);
+    p += round((c + 1u) * 4u);
     s.partials = reinterpret_cast<uint32_t*>(p);
+    p += round(exclusive_scan_partials(c) * 4u);
+    s.big_cells = reinterpret_cast<uint32_t*>(p);
     return s;
 }
 
@@ -482,7 +637,7 @@ GridGeometry build_grid(hipStream_t stream, const Domain& domain, uint64_t step,
     geo.cell_size = c.support_radius;
     geo.time = time;
 
-    fill_u32(stream, scratch.counts, static_cast<size_t>(cells) + 1u, 0u);
+    fill_u32(stream, scratch.counts, static_cast<size_t>(cells) + 2u, 0u);  // histogram and big-cell count
     if (n == 0) {
         fill_u32(stream, offsets, static_cast<size_t>(cells) + 1u, 0u);
         return geo;
@@ -500,9 +655,27 @@ GridGeometry build_grid(hipStream_t stream, const Domain& domain, uint64_t step,
     exclusive_scan(stream, scratch.counts, offsets, static_cast<size_t>(cells) + 1u, scratch.partials);
     scatter_unstable_kernel<<<grid, kBlock, 0, stream>>>(n, scratch.keys, scratch.slots, offsets, scratch.unstable);
     PHX_LAUNCH_CHECK();
-    rank_gather_kernel<<<grid, kBlock, 0, stream>>>(n, scratch.keys, offsets, scratch.unstable, perm, gather);
-    PHX_LAUNCH_CHECK();
+    order_cells(stream, n, cells, scratch, offsets, perm, gather);
     return geo;
+}
+
+void build_batch_grid(hipStream_t stream, uint32_t particles, uint32_t cells, uint32_t block_count,
+                      const BatchView& batch, CVec positions, const PredictInput& predict, const Channels& gather,
+                      const GridScratch& scratch, uint32_t* perm, uint32_t* offsets) {
+    fill_u32(stream, scratch.counts, static_cast<size_t>(cells) + 2u, 0u);  // histogram and big-cell count
+    if (particles == 0) {
+        fill_u32(stream, offsets, static_cast<size_t>(cells) + 1u, 0u);
+        return;
+    }
+    batch_key_count_kernel<<<block_count, kBlock, 0, stream>>>(batch, positions, predict, scratch.keys, scratch.slots,
+                                                               scratch.counts);
+    PHX_LAUNCH_CHECK();
+    exclusive_scan(stream, scratch.counts, offsets, static_cast<size_t>(cells) + 1u, scratch.partials);
+    const uint32_t grid = blocks_for(particles, kBlock);
+    scatter_unstable_kernel<<<grid, kBlock, 0, stream>>>(particles, scratch.keys, scratch.slots, offsets,
+                                                         scratch.unstable);
+    PHX_LAUNCH_CHECK();
+    order_cells(stream, particles, cells, scratch, offsets, perm, gather);
 }
 
 void export_neighborhood(hipStream_t stream, const Neighborhood& nb, uint64_t* sorted_keys, uint32_t* sorted_indices,

@@ -149,15 +149,16 @@ __device__ __forceinline__ uint32_t xcd_block() {
 
 // Builds both masks of particle i at position p: `exact` marks d2 < t_exact,
 // `skin` marks d2 < t_skin, with d2 = dot(p - x_s, p - x_s) evaluated as the
-// kernels do. The exact mask is also copied to `column` and its non-empty
-// slots are added to the bitmap `nonzero`, if given. Returns false if a
-// stencil cell holds more than kMaskBits particles (the masks of that cell are
-// then truncated and must not be used).
+// kernels do. on_exact(q, d2) is called for every candidate the exact mask
+// keeps, in the reference order (slots, then ascending sorted position).
+// Returns false if a stencil cell holds more than kMaskBits particles (the
+// masks of that cell are then truncated and must not be used, and on_exact
+// has seen only the first kMaskBits of it).
+template <class OnExact>
 __device__ __forceinline__ bool build_masks(const SortedGrid& g, const float4* __restrict__ xm, uint32_t n, uint32_t i,
                                             Float3 p, int cx, int cy, int cz, float t_exact, float t_skin,
                                             uint16_t* __restrict__ exact, uint16_t* __restrict__ skin,
-                                            uint16_t* column = nullptr, uint32_t stride = 0u,
-                                            uint32_t* nonzero = nullptr) {
+                                            OnExact&& on_exact) {
     bool complete = true;
     int k = 0;
     for (int ox = -1; ox <= 1; ++ox) {
@@ -182,14 +183,14 @@ __device__ __forceinline__ bool build_masks(const SortedGrid& g, const float4* _
                         const Float3 d = sub(p, Float3{q.x, q.y, q.z});
                         const float d2 = dot(d, d);
                         const uint32_t bit = 1u << (s - first);
-                        me |= d2 < t_exact ? bit : 0u;
+                        const bool keep = d2 < t_exact;
+                        me |= keep ? bit : 0u;
                         ms |= d2 < t_skin ? bit : 0u;
+                        if (keep) on_exact(q, d2);
                     }
                 }
                 exact[static_cast<size_t>(k) * n + i] = static_cast<uint16_t>(me);
                 skin[static_cast<size_t>(k) * n + i] = static_cast<uint16_t>(ms);
-                if (column != nullptr) column[static_cast<uint32_t>(k) * stride] = static_cast<uint16_t>(me);
-                if (nonzero != nullptr && me != 0u) *nonzero |= 1u << k;
             }
         }
     }

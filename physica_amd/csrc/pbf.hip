@@ -32,6 +32,51 @@ constexpr uint32_t kBlock = 256;
     fill_slot_deltas(grid, delta);                  \
     __syncthreads()
 
+// The particle, grid and collision box of a lane: one scene (kBatch false),
+// or in a batch the scene of the workgroup's block, whose grid starts at
+// the scene's cell base in the batch's offsets.
+struct Lane {
+    uint32_t i;
+    bool valid;
+    SortedGrid grid;
+    Box box;
+};
+
+template <bool kBatch>
+__device__ __forceinline__ Lane lane_of(uint32_t n, const SortedGrid& grid, const Box& box, const BatchView& batch) {
+    Lane l;
+    if constexpr (kBatch) {
+        const BatchBlock blk = batch.blocks[xcd_block()];
+        const BatchScene& sc = batch.scenes[blk.scene];
+        l.i = blk.first + threadIdx.x;
+        l.valid = threadIdx.x < blk.count;
+        // the lookup origin min + t v_box and the collision box of the step
+        // (neighborhood.cpp:83-88, pbf.cpp:17-31), as the host computes them
+        const float t = batch.time;
+        l.grid = SortedGrid{grid.offsets + sc.cell_base,
+                            sc.res[0],
+                            sc.res[1],
+                            sc.res[2],
+                            sc.minimum[0] + t * sc.velocity[0],
+                            sc.minimum[1] + t * sc.velocity[1],
+                            sc.minimum[2] + t * sc.velocity[2],
+                            batch.h};
+        const float rp = sc.particle_radius;
+        for (int a = 0; a < 3; ++a) {
+            l.box.lo[a] = sc.minimum[a] + t * sc.velocity[a] + rp;
+            l.box.hi[a] = sc.maximum[a] + t * sc.velocity[a] - rp;
+            l.box.vel[a] = sc.velocity[a];
+        }
+        l.box.no_slip = sc.no_slip;
+    } else {
+        l.i = xcd_block() * kBlock + threadIdx.x;
+        l.valid = l.i < n;
+        l.grid = grid;
+        l.box = box;
+    }
+    return l;
+}
+
 struct MaskArgs {
     uint32_t n;
     SortedGrid grid;
@@ -44,13 +89,16 @@ struct MaskArgs {
 // Skin masks at x*; a particle with a stencil cell of more than kMaskBits
 // particles is marked crowded and walks the full stencil in every pass of
 // the step (the others keep their masks).
-__global__ void __launch_bounds__(kBlock) pbf_mask_kernel(MaskArgs a) {
-    const uint32_t i = xcd_block() * kBlock + threadIdx.x;
-    if (i >= a.n) return;
+template <bool kBatch>
+__global__ void __launch_bounds__(kBlock) pbf_mask_kernel(MaskArgs a, BatchView batch) {
+    const Lane lane = lane_of<kBatch>(a.n, a.grid, Box{}, batch);
+    const uint32_t i = lane.i;
+    if (!lane.valid) return;
+    const SortedGrid& grid = lane.grid;
     const Float3 p = load(a.xs, i);
     int cx, cy, cz;
-    particle_cell(a.grid, p, cx, cy, cz);
-    a.crowded[i] = build_skin_mask(a.grid, a.xs, a.n, i, p, cx, cy, cz, a.t_skin, a.masks) ? 0u : 1u;
+    particle_cell(grid, p, cx, cy, cz);
+    a.crowded[i] = build_skin_mask(grid, a.xs, a.n, i, p, cx, cy, cz, a.t_skin, a.masks) ? 0u : 1u;
 }
 
 // A particle that is not crowded always gets complete exact masks; the set's
@@ -90,15 +138,18 @@ struct LambdaArgs {
 // after its particles adding rho0_i V_b W to the density and V_b grad W to
 // the gradient sum (pbf.cu:118-121); no exact masks are written (the host
 // marks those sets unusable).
-__global__ void __launch_bounds__(kBlock) pbf_lambda_kernel(LambdaArgs a) {
-    PHX_MASK_LDS(a.grid);
-    const uint32_t i = xcd_block() * kBlock + threadIdx.x;
-    if (i >= a.n) return;
+template <bool kBatch>
+__global__ void __launch_bounds__(kBlock) pbf_lambda_kernel(LambdaArgs a, BatchView batch) {
+    const Lane lane = lane_of<kBatch>(a.n, a.grid, Box{}, batch);
+    PHX_MASK_LDS(lane.grid);
+    const uint32_t i = lane.i;
+    if (!lane.valid) return;
+    const SortedGrid& grid = lane.grid;
     const bool masked = masks_apply(a.mv, a.slot, i);
     const Float3 p = load(a.pos, i);
     const float rest = a.rho0[i];
     int cx, cy, cz;
-    particle_cell(a.grid, load(a.topo, i), cx, cy, cz);
+    particle_cell(grid, load(a.topo, i), cx, cy, cz);
     float density = 0.0F;
     Float3 gradient_sum{0.0F, 0.0F, 0.0F};
     float denominator = a.eps[i];
@@ -116,7 +167,7 @@ __global__ void __launch_bounds__(kBlock) pbf_lambda_kernel(LambdaArgs a) {
     };
     if (a.boundary.offsets != nullptr) {
         for_each_candidate(
-            a.grid, a.boundary, cx, cy, cz, [&](uint32_t s) { (void)visit(s); },
+            grid, a.boundary, cx, cy, cz, [&](uint32_t s) { (void)visit(s); },
             [&](uint32_t s) {
                 const float4 b = a.boundary.x[s];
                 const Float3 d = sub(p, Float3{b.x, b.y, b.z});
@@ -124,7 +175,7 @@ __global__ void __launch_bounds__(kBlock) pbf_lambda_kernel(LambdaArgs a) {
                 gradient_sum = add(gradient_sum, scale(spiky_gradient(d, a.h, a.spiky), b.w));
             });
     } else {
-        const bool complete = visit_and_mask(a.grid, a.mv, masked, i, cx, cy, cz, delta, smask + threadIdx.x, kBlock,
+        const bool complete = visit_and_mask(grid, a.mv, masked, i, cx, cy, cz, delta, smask + threadIdx.x, kBlock,
                                              a.exact_out, a.n, visit);
         flag_incomplete(complete, a.mv, i, &a.exact_flags[kFlagOverflow]);
     }
@@ -168,11 +219,15 @@ struct CorrectionArgs {
 // from x* for the mask checks of later passes. A boundary sample b adds
 // V_b (lambda_i + s_corr(d)) grad W after each cell's particles
 // (pbf.cu:278-284).
-__global__ void __launch_bounds__(kBlock) pbf_correction_kernel(CorrectionArgs a) {
-    PHX_MASK_LDS(a.grid);
-    const uint32_t i = xcd_block() * kBlock + threadIdx.x;
+template <bool kBatch>
+__global__ void __launch_bounds__(kBlock) pbf_correction_kernel(CorrectionArgs a, BatchView batch) {
+    const Lane lane = lane_of<kBatch>(a.n, a.grid, a.box, batch);
+    PHX_MASK_LDS(lane.grid);
+    const uint32_t i = lane.i;
+    const SortedGrid& grid = lane.grid;
+    const Box& box = lane.box;
     float moved2 = 0.0F;
-    if (i < a.n) {
+    if (lane.valid) {
         const bool masked = masks_apply(a.mv, a.slot, i);
         const Float3 p = load(a.pos, i);
         const float rest = a.rho0[i];
@@ -183,7 +238,7 @@ __global__ void __launch_bounds__(kBlock) pbf_correction_kernel(CorrectionArgs a
         // with k = +-0 the term is -k wherever W(d) > 0 (no powf per pair)
         const bool vanishes = artificial_vanishes(a.h, strength, exponent, reference);
         int cx, cy, cz;
-        particle_cell(a.grid, load(a.topo, i), cx, cy, cz);
+        particle_cell(grid, load(a.topo, i), cx, cy, cz);
         Float3 correction{0.0F, 0.0F, 0.0F};
         auto artificial_at = [&](Float3 d) {
             return vanishes ? (poly6(d, a.h) == 0.0F ? 0.0F : -strength)
@@ -198,26 +253,26 @@ __global__ void __launch_bounds__(kBlock) pbf_correction_kernel(CorrectionArgs a
                                                coefficient * (lambda_i + a.lambda[s] + artificial)));
         };
         if (a.boundary.offsets != nullptr) {
-            for_each_candidate(a.grid, a.boundary, cx, cy, cz, visit, [&](uint32_t s) {
+            for_each_candidate(grid, a.boundary, cx, cy, cz, visit, [&](uint32_t s) {
                 const float4 b = a.boundary.x[s];
                 const Float3 d = sub(p, Float3{b.x, b.y, b.z});
                 const float artificial = artificial_at(d);
                 correction = add(correction, scale(spiky_gradient(d, a.h, a.spiky), b.w * (lambda_i + artificial)));
             });
         } else {
-            visit_candidates(a.grid, a.mv, masked, i, cx, cy, cz, delta, smask + threadIdx.x, kBlock, visit);
+            visit_candidates(grid, a.mv, masked, i, cx, cy, cz, delta, smask + threadIdx.x, kBlock, visit);
         }
         const Float3 candidate = add(p, correction);
         if (a.mask != nullptr) {
             uint32_t bits = 0u;
-            if (candidate.x < a.box.lo[0] || candidate.x > a.box.hi[0]) bits |= 1u;
-            if (candidate.y < a.box.lo[1] || candidate.y > a.box.hi[1]) bits |= 2u;
-            if (candidate.z < a.box.lo[2] || candidate.z > a.box.hi[2]) bits |= 4u;
+            if (candidate.x < box.lo[0] || candidate.x > box.hi[0]) bits |= 1u;
+            if (candidate.y < box.lo[1] || candidate.y > box.hi[1]) bits |= 2u;
+            if (candidate.z < box.lo[2] || candidate.z > box.hi[2]) bits |= 4u;
             a.mask[i] = static_cast<uint8_t>(bits);
         }
-        const Float3 projected{fminf(a.box.hi[0], fmaxf(a.box.lo[0], candidate.x)),
-                               fminf(a.box.hi[1], fmaxf(a.box.lo[1], candidate.y)),
-                               fminf(a.box.hi[2], fmaxf(a.box.lo[2], candidate.z))};
+        const Float3 projected{fminf(box.hi[0], fmaxf(box.lo[0], candidate.x)),
+                               fminf(box.hi[1], fmaxf(box.lo[1], candidate.y)),
+                               fminf(box.hi[2], fmaxf(box.lo[2], candidate.z))};
         store(a.out, i, projected);
         if (a.reconstruct != 0u) store(a.vel, i, scale(sub(projected, load(a.x0, i)), a.inv_dt));
         const Float3 moved = sub(projected, load(a.topo, i));
@@ -259,18 +314,21 @@ struct PostArgs {
     uint32_t* exact_flags;
 };
 
-__global__ void __launch_bounds__(kBlock) pbf_vorticity_kernel(PostArgs a) {
-    PHX_MASK_LDS(a.grid);
-    const uint32_t i = xcd_block() * kBlock + threadIdx.x;
-    if (i >= a.n) return;
+template <bool kBatch>
+__global__ void __launch_bounds__(kBlock) pbf_vorticity_kernel(PostArgs a, BatchView batch) {
+    const Lane lane = lane_of<kBatch>(a.n, a.grid, Box{}, batch);
+    PHX_MASK_LDS(lane.grid);
+    const uint32_t i = lane.i;
+    if (!lane.valid) return;
+    const SortedGrid& grid = lane.grid;
     const bool masked = masks_apply(a.mv, a.slot, i);
     const Float3 p = load(a.pos, i);
     const Float3 v = load(a.vel, i);
     int cx, cy, cz;
-    particle_cell(a.grid, load(a.topo, i), cx, cy, cz);
+    particle_cell(grid, load(a.topo, i), cx, cy, cz);
     Float3 w{0.0F, 0.0F, 0.0F};
     const bool complete = visit_and_mask(
-        a.grid, a.mv, masked, i, cx, cy, cz, delta, smask + threadIdx.x, kBlock, a.exact_out, a.n, [&](uint32_t s) {
+        grid, a.mv, masked, i, cx, cy, cz, delta, smask + threadIdx.x, kBlock, a.exact_out, a.n, [&](uint32_t s) {
             const Float3 d = sub(p, load(a.pos, s));
             if (s != i) w = add(w, cross(sub(load(a.vel, s), v), spiky_gradient(d, a.h, a.spiky)));
             return dot(d, d) < a.t_exact;
@@ -280,17 +338,20 @@ __global__ void __launch_bounds__(kBlock) pbf_vorticity_kernel(PostArgs a) {
     a.magnitude[i] = length(w);
 }
 
-__global__ void __launch_bounds__(kBlock) pbf_normal_kernel(PostArgs a) {
-    PHX_MASK_LDS(a.grid);
-    const uint32_t i = xcd_block() * kBlock + threadIdx.x;
-    if (i >= a.n) return;
+template <bool kBatch>
+__global__ void __launch_bounds__(kBlock) pbf_normal_kernel(PostArgs a, BatchView batch) {
+    const Lane lane = lane_of<kBatch>(a.n, a.grid, Box{}, batch);
+    PHX_MASK_LDS(lane.grid);
+    const uint32_t i = lane.i;
+    if (!lane.valid) return;
+    const SortedGrid& grid = lane.grid;
     const bool masked = masks_apply(a.mv, a.slot, i);
     const Float3 p = load(a.pos, i);
     const float mag_i = a.magnitude[i];
     int cx, cy, cz;
-    particle_cell(a.grid, load(a.topo, i), cx, cy, cz);
+    particle_cell(grid, load(a.topo, i), cx, cy, cz);
     Float3 g{0.0F, 0.0F, 0.0F};
-    visit_candidates(a.grid, a.mv, masked, i, cx, cy, cz, delta, smask + threadIdx.x, kBlock, [&](uint32_t s) {
+    visit_candidates(grid, a.mv, masked, i, cx, cy, cz, delta, smask + threadIdx.x, kBlock, [&](uint32_t s) {
         if (s == i) return;
         g = add(g, scale(spiky_gradient(sub(p, load(a.pos, s)), a.h, a.spiky), a.magnitude[s] - mag_i));
     });
@@ -304,18 +365,21 @@ __global__ void __launch_bounds__(kBlock) pbf_normal_kernel(PostArgs a) {
     store(a.vc, i, add(load(a.vel, i), scale(cross(normal, w), a.dt * a.confinement[i])));
 }
 
-__global__ void __launch_bounds__(kBlock) pbf_xsph_kernel(PostArgs a) {
-    PHX_MASK_LDS(a.grid);
-    const uint32_t i = xcd_block() * kBlock + threadIdx.x;
-    if (i >= a.n) return;
+template <bool kBatch>
+__global__ void __launch_bounds__(kBlock) pbf_xsph_kernel(PostArgs a, BatchView batch) {
+    const Lane lane = lane_of<kBatch>(a.n, a.grid, Box{}, batch);
+    PHX_MASK_LDS(lane.grid);
+    const uint32_t i = lane.i;
+    if (!lane.valid) return;
+    const SortedGrid& grid = lane.grid;
     const bool masked = masks_apply(a.mv, a.slot, i);
     const CVec vc{a.vc.x, a.vc.y, a.vc.z};
     const Float3 p = load(a.pos, i);
     const Float3 v = load(vc, i);
     int cx, cy, cz;
-    particle_cell(a.grid, load(a.topo, i), cx, cy, cz);
+    particle_cell(grid, load(a.topo, i), cx, cy, cz);
     Float3 smoothing{0.0F, 0.0F, 0.0F};
-    visit_candidates(a.grid, a.mv, masked, i, cx, cy, cz, delta, smask + threadIdx.x, kBlock, [&](uint32_t s) {
+    visit_candidates(grid, a.mv, masked, i, cx, cy, cz, delta, smask + threadIdx.x, kBlock, [&](uint32_t s) {
         if (s == i) return;
         smoothing = add(smoothing, scale(sub(load(vc, s), v), poly6(sub(p, load(a.pos, s)), a.h)));
     });
@@ -390,8 +454,6 @@ PbfSolver::PbfSolver(const phx_domain_configuration* domain, const phx_pbf_confi
     require(config_.checkpoint_interval > 0u, PHX_ERROR_INVALID_ARGUMENT, "checkpoint_interval must be positive");
     require(mode == PHX_MODE_FORWARD || mode == PHX_MODE_DIFFERENTIABLE, PHX_ERROR_INVALID_ARGUMENT,
             "unknown execution mode");
-    require(domain_.config.boundary_particle_count == 0u || mode == PHX_MODE_FORWARD, PHX_ERROR_UNSUPPORTED,
-            "boundary particles are supported in forward mode only");
     boundary_.upload(domain_.config);
     domain_.config.boundary_particles = nullptr;  // the caller's host array is not kept
     spiky_ = spiky_coefficient(domain_.config.support_radius);
@@ -475,7 +537,7 @@ void PbfSolver::forward_step(const phx_pbf_state& state, const phx_control& cont
     {
         PassScope scope(profiler_, stream_, kPbfMask);
         MaskArgs ma{n, grid, xs.cview(), masks_.t_skin(), skin, crowded};
-        pbf_mask_kernel<<<blocks, kBlock, 0, stream_>>>(ma);
+        pbf_mask_kernel<false><<<blocks, kBlock, 0, stream_>>>(ma, BatchView{});
         PHX_LAUNCH_CHECK();
     }
     MaskView mv{skin, masks_.flags(), n, masks_.limit2()};
@@ -545,7 +607,7 @@ void PbfSolver::forward_step(const phx_pbf_state& state, const phx_control& cont
         }
         {
             PassScope scope(profiler_, stream_, kPbfLambda);
-            pbf_lambda_kernel<<<blocks, kBlock, 0, stream_>>>(la);
+            pbf_lambda_kernel<false><<<blocks, kBlock, 0, stream_>>>(la, BatchView{});
             PHX_LAUNCH_CHECK();
         }
         const Vec target = rec ? cache.positions.view(it) : targets[it & 1u]->view();
@@ -554,7 +616,7 @@ void PbfSolver::forward_step(const phx_pbf_state& state, const phx_control& cont
         ca.reconstruct = it + 1u == iterations ? 1u : 0u;
         {
             PassScope scope(profiler_, stream_, kPbfCorrection);
-            pbf_correction_kernel<<<blocks, kBlock, 0, stream_>>>(ca);
+            pbf_correction_kernel<false><<<blocks, kBlock, 0, stream_>>>(ca, BatchView{});
             PHX_LAUNCH_CHECK();
         }
         current = CVec{target.x, target.y, target.z};
@@ -604,17 +666,17 @@ void PbfSolver::forward_step(const phx_pbf_state& state, const phx_control& cont
     post_exact.slot = 0u;
     {
         PassScope scope(profiler_, stream_, kPbfVorticity);
-        pbf_vorticity_kernel<<<blocks, kBlock, 0, stream_>>>(post);
+        pbf_vorticity_kernel<false><<<blocks, kBlock, 0, stream_>>>(post, BatchView{});
         PHX_LAUNCH_CHECK();
     }
     {
         PassScope scope(profiler_, stream_, kPbfNormal);
-        pbf_normal_kernel<<<blocks, kBlock, 0, stream_>>>(post_exact);
+        pbf_normal_kernel<false><<<blocks, kBlock, 0, stream_>>>(post_exact, BatchView{});
         PHX_LAUNCH_CHECK();
     }
     {
         PassScope scope(profiler_, stream_, kPbfXsph);
-        pbf_xsph_kernel<<<blocks, kBlock, 0, stream_>>>(post_exact);
+        pbf_xsph_kernel<false><<<blocks, kBlock, 0, stream_>>>(post_exact, BatchView{});
         PHX_LAUNCH_CHECK();
     }
     if (rec) {
@@ -627,6 +689,342 @@ void PbfSolver::forward_step(const phx_pbf_state& state, const phx_control& cont
     }
     masks_.end_step(stream_);
     next.particles.step_index = step + 1u;
+}
+
+}  // namespace phx
+
+namespace phx {
+
+// ------------------------------------------------------------------ batch
+PbfBatch::PbfBatch(const phx_domain_configuration* domains, uint32_t scene_count,
+                   const phx_pbf_configuration* configuration, const phx_pbf_batch_configuration* batch_configuration,
+                   hipStream_t stream)
+    : caller_(stream) {
+    require(domains != nullptr && scene_count > 0u, PHX_ERROR_INVALID_ARGUMENT, "a batch needs at least one scene");
+    require(configuration != nullptr, PHX_ERROR_INVALID_ARGUMENT, "null PBF configuration");
+    config_ = *configuration;
+    require(config_.pressure_iterations < 1024u, PHX_ERROR_INVALID_ARGUMENT, "too many pressure iterations");
+    if (batch_configuration != nullptr) {
+        const float skin = batch_configuration->mask_skin;
+        require(skin == 0.0F || (skin >= 0.01F && skin <= 1.0F), PHX_ERROR_INVALID_ARGUMENT,
+                "mask_skin must be 0 (default) or in [0.01, 1]");
+        if (skin > 0.0F) skin_ = skin;
+        use_graph_ = batch_configuration->use_graph != 0u;
+    }
+    uint64_t particles = 0, cells = 0;
+    offsets_.push_back(0u);
+    for (uint32_t k = 0; k < scene_count; ++k) {
+        const Domain d = make_domain(&domains[k]);
+        const auto& c = d.config;
+        require(c.boundary_particle_count == 0u, PHX_ERROR_UNSUPPORTED, "batched scenes take no boundary particles");
+        if (k == 0u) {
+            h_ = c.support_radius;
+            dt_ = c.time_step;
+        }
+        require(c.support_radius == h_ && c.time_step == dt_, PHX_ERROR_INVALID_ARGUMENT,
+                "the scenes of a batch share the support radius and time step");
+        BatchScene s{};
+        s.cell_base = static_cast<uint32_t>(cells);
+        for (int a = 0; a < 3; ++a) s.res[a] = d.resolution[a];
+        s.minimum[0] = c.boundary.minimum.x;
+        s.minimum[1] = c.boundary.minimum.y;
+        s.minimum[2] = c.boundary.minimum.z;
+        s.maximum[0] = c.boundary.maximum.x;
+        s.maximum[1] = c.boundary.maximum.y;
+        s.maximum[2] = c.boundary.maximum.z;
+        s.velocity[0] = c.boundary.velocity.x;
+        s.velocity[1] = c.boundary.velocity.y;
+        s.velocity[2] = c.boundary.velocity.z;
+        s.particle_radius = c.particle_radius;
+        s.gravity[0] = c.gravity.x;
+        s.gravity[1] = c.gravity.y;
+        s.gravity[2] = c.gravity.z;
+        s.no_slip = c.boundary.no_slip != 0u ? 1u : 0u;
+        if (s.velocity[0] != 0.0F || s.velocity[1] != 0.0F || s.velocity[2] != 0.0F) static_boxes_ = false;
+        scenes_.push_back(s);
+        cells += d.cells;
+        particles += c.particle_count;
+        offsets_.push_back(particles);
+    }
+    require(particles < (1ull << 31) && cells < (1ull << 31), PHX_ERROR_CAPACITY, "batch too large");
+    n_ = static_cast<uint32_t>(particles);
+    cells_ = static_cast<uint32_t>(cells);
+    // a captured substep bakes in the step time, which moving boxes change
+    use_graph_ = use_graph_ && static_boxes_;
+    spiky_ = spiky_coefficient(h_);
+
+    // workgroup table: each scene's particles in blocks of kBlock, padded to
+    // a multiple of the XCD count
+    std::vector<BatchBlock> table;
+    for (uint32_t k = 0; k < scene_count; ++k) {
+        for (uint64_t first = offsets_[k]; first < offsets_[k + 1]; first += kBlock) {
+            const uint64_t count = std::min<uint64_t>(kBlock, offsets_[k + 1] - first);
+            table.push_back(BatchBlock{k, static_cast<uint32_t>(first), static_cast<uint32_t>(count), 0u});
+        }
+    }
+    while (table.empty() || table.size() % 8u != 0u) table.push_back(BatchBlock{0u, 0u, 0u, 0u});
+    blocks_ = static_cast<uint32_t>(table.size());
+    block_table_.resize(table.size());
+    scene_table_.resize(scenes_.size());
+    PHX_HIP(hipMemcpy(block_table_.data(), table.data(), sizeof(BatchBlock) * table.size(), hipMemcpyHostToDevice));
+    PHX_HIP(hipMemcpy(scene_table_.data(), scenes_.data(), sizeof(BatchScene) * scenes_.size(),
+                      hipMemcpyHostToDevice));
+
+    const uint32_t n = n_;
+    grid_.allocate(n, cells_);
+    for (VecBuffer* b : {&positions_[0], &positions_[1], &velocities_[0], &velocities_[1], &predicted_, &xs_, &x0_,
+                         &pa_, &pb_, &vel_, &vort_, &vc_})
+        b->resize(n);
+    for (DeviceBuffer<float>* b :
+         {&m_, &rho0_, &eps_, &strength_, &exponent_, &radius_, &xsph_, &confinement_, &rho_, &lambda_, &magnitude_})
+        b->resize(n);
+    skin_masks_.resize(static_cast<size_t>(kStencil) * n);
+    exact_.resize(static_cast<size_t>(kStencil) * n);
+    crowded_.resize(n);
+    flags_.resize(kFlagDisplacement + config_.pressure_iterations + 1u);
+    exact_flags_.resize(config_.pressure_iterations + 1u);
+    // grid build 8, flag fills 2, masks 1, two per iteration, three post passes
+    const uint32_t it = config_.pressure_iterations;
+    kernels_per_step_ = 8u + 2u + 1u + 2u * it + (it == 0u ? 1u : 0u) + 3u;
+    PHX_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    PHX_HIP(hipEventCreateWithFlags(&ready_, hipEventDisableTiming));
+    PHX_HIP(hipEventCreateWithFlags(&done_, hipEventDisableTiming));
+}
+
+PbfBatch::~PbfBatch() {
+    if (stream_ != nullptr) (void)hipStreamSynchronize(stream_);
+    release_graphs();
+    if (ready_ != nullptr) (void)hipEventDestroy(ready_);
+    if (done_ != nullptr) (void)hipEventDestroy(done_);
+    if (stream_ != nullptr) (void)hipStreamDestroy(stream_);
+}
+
+void PbfBatch::release_graphs() {
+    for (hipGraphExec_t& g : graph_) {
+        if (g != nullptr) (void)hipGraphExecDestroy(g);
+        g = nullptr;
+    }
+    captured_inputs_.clear();
+}
+
+// One forward step of every scene from state_[from] into state_[1 - from]:
+// the pass sequence of PbfSolver::forward_step in forward mode, each pass
+// one launch over the batch's workgroup table.
+void PbfBatch::substep(uint32_t from, uint64_t step, const phx_control& control,
+                       const phx_pbf_parameters& parameters) {
+    const uint32_t n = n_;
+    const hipStream_t s = stream_;
+    const VecBuffer& x = positions_[from];
+    const VecBuffer& v = velocities_[from];
+    VecBuffer& next_x = positions_[1u - from];
+    VecBuffer& next_v = velocities_[1u - from];
+    const auto& pp = parameters.particles;
+    BatchView bv{block_table_.data(), scene_table_.data(), static_cast<float>(step + 1u) * dt_, h_};
+    kernels_per_step_ = 0u;
+
+    PredictInput predict{};
+    predict.velocities = v.cview();
+    predict.controls = cvec(control.external_accelerations);
+    predict.dt = dt_;
+    predict.predicted = predicted_.view();
+    ChannelList gather;
+    gather.add(predicted_.cview(), xs_.view());
+    gather.add(x.cview(), x0_.view());
+    gather.add(pp.masses, m_.data());
+    gather.add(pp.rest_densities, rho0_.data());
+    gather.add(parameters.relaxation, eps_.data());
+    gather.add(parameters.artificial_pressure_strength, strength_.data());
+    gather.add(parameters.artificial_pressure_exponent, exponent_.data());
+    gather.add(parameters.artificial_pressure_radius, radius_.data());
+    gather.add(parameters.xsph_viscosity, xsph_.data());
+    gather.add(parameters.vorticity_confinement, confinement_.data());
+    build_batch_grid(s, n, cells_, blocks_, bv, x.cview(), predict, gather.get(), grid_.scratch, grid_.perm.data(),
+                     grid_.offsets.data());
+    kernels_per_step_ += 8u;  // counts fill, key, three scan kernels, scatter, rank + gather, big-cell sort
+    const uint32_t sets = config_.pressure_iterations + 1u;
+    fill_u32(s, flags_.data(), kFlagDisplacement + sets, 0u);
+    fill_u32(s, exact_flags_.data(), sets, 0u);
+    kernels_per_step_ += 2u;
+
+    // the batch's offsets; each lane's grid starts at its scene's cell base
+    const SortedGrid grid{grid_.offsets.data(), 0u, 0u, 0u, 0.0F, 0.0F, 0.0F, h_};
+    const float radius = h_ * (1.0F + skin_);
+    const float t_skin = radius * radius * (1.0F + 1e-5F);
+    const float limit = 0.5F * (radius - h_ * (1.0F + 2e-5F));
+    const float limit2 = limit > 0.0F ? limit * limit : -1.0F;
+    MaskArgs ma{n, grid, xs_.cview(), t_skin, skin_masks_.data(), crowded_.data()};
+    pbf_mask_kernel<true><<<blocks_, kBlock, 0, s>>>(ma, bv);
+    PHX_LAUNCH_CHECK();
+    ++kernels_per_step_;
+    MaskView mv{skin_masks_.data(), flags_.data(), n, limit2};
+    mv.crowded = crowded_.data();
+    const MaskView exact_view{exact_.data(), exact_flags_.data(), n, 0.0F, 1u, crowded_.data()};
+    const float t_exact = exact_mask_threshold(h_);
+
+    LambdaArgs la{n,           h_,         spiky_,        grid, xs_.cview(), xs_.cview(), m_.data(), rho0_.data(),
+                  eps_.data(), rho_.data(), lambda_.data(), nullptr, Vec{},       mv,          0u};
+    la.t_exact = t_exact;
+    la.exact_out = exact_.data();
+    CorrectionArgs ca{};
+    ca.n = n;
+    ca.h = h_;
+    ca.spiky = spiky_;
+    ca.grid = grid;
+    ca.topo = xs_.cview();
+    ca.m = m_.data();
+    ca.rho0 = rho0_.data();
+    ca.lambda = lambda_.data();
+    ca.strength = strength_.data();
+    ca.exponent = exponent_.data();
+    ca.radius = radius_.data();
+    ca.x0 = x0_.cview();
+    ca.inv_dt = 1.0F / dt_;
+    ca.vel = vel_.view();
+    ca.mv = exact_view;
+    ca.slot = 0u;
+    // the exact masks of each set of positions are rebuilt by the pass that
+    // first walks them, so one buffer serves every set in forward mode
+    CVec current = xs_.cview();
+    VecBuffer* targets[2] = {&pa_, &pb_};
+    const uint32_t iterations = config_.pressure_iterations;
+    for (uint32_t it = 0; it < iterations; ++it) {
+        la.pos = current;
+        la.slot = it;
+        la.exact_flags = exact_flags_.data() + it;
+        ca.mv.flags = exact_flags_.data() + it;
+        ca.moved_to = flags_.data() + kFlagDisplacement + it + 1u;
+        pbf_lambda_kernel<true><<<blocks_, kBlock, 0, s>>>(la, bv);
+        PHX_LAUNCH_CHECK();
+        const Vec target = targets[it & 1u]->view();
+        ca.pos = current;
+        ca.out = target;
+        ca.reconstruct = it + 1u == iterations ? 1u : 0u;
+        pbf_correction_kernel<true><<<blocks_, kBlock, 0, s>>>(ca, bv);
+        PHX_LAUNCH_CHECK();
+        kernels_per_step_ += 2u;
+        current = CVec{target.x, target.y, target.z};
+    }
+    if (iterations == 0u) {
+        pbf_reconstruct_kernel<<<blocks_for(n, kBlock), kBlock, 0, s>>>(n, current, x0_.cview(), 1.0F / dt_,
+                                                                      vel_.view());
+        PHX_LAUNCH_CHECK();
+        ++kernels_per_step_;
+    }
+    PostArgs post{};
+    post.n = n;
+    post.h = h_;
+    post.spiky = spiky_;
+    post.dt = dt_;
+    post.grid = grid;
+    post.topo = xs_.cview();
+    post.pos = current;
+    post.vel = vel_.cview();
+    post.vort = vort_.view();
+    post.magnitude = magnitude_.data();
+    post.confinement = confinement_.data();
+    post.vc = vc_.view();
+    post.xsph = xsph_.data();
+    post.perm = grid_.perm.data();
+    post.mv = mv;
+    post.slot = iterations;
+    post.t_exact = t_exact;
+    post.exact_out = exact_.data();
+    post.exact_flags = exact_flags_.data() + iterations;
+    post.next_x = next_x.view();
+    post.next_v = next_v.view();
+    PostArgs post_exact = post;
+    post_exact.mv = exact_view;
+    post_exact.mv.flags = exact_flags_.data() + iterations;
+    post_exact.slot = 0u;
+    pbf_vorticity_kernel<true><<<blocks_, kBlock, 0, s>>>(post, bv);
+    PHX_LAUNCH_CHECK();
+    pbf_normal_kernel<true><<<blocks_, kBlock, 0, s>>>(post_exact, bv);
+    PHX_LAUNCH_CHECK();
+    pbf_xsph_kernel<true><<<blocks_, kBlock, 0, s>>>(post_exact, bv);
+    PHX_LAUNCH_CHECK();
+    kernels_per_step_ += 3u;
+}
+}  // namespace phx
+
+namespace phx {
+
+void PbfBatch::capture(const phx_control& control, const phx_pbf_parameters& parameters) {
+    release_graphs();
+    // one eager substep first, so every kernel is loaded before the capture
+    // (it writes only the batch's own state buffers, which the replays overwrite)
+    substep(0u, 0u, control, parameters);
+    for (uint32_t from = 0; from < 2u; ++from) {
+        hipGraph_t graph = nullptr;
+        PHX_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+        try {
+            substep(from, 0u, control, parameters);
+        } catch (...) {
+            (void)hipStreamEndCapture(stream_, &graph);
+            if (graph != nullptr) (void)hipGraphDestroy(graph);
+            throw;
+        }
+        PHX_HIP(hipStreamEndCapture(stream_, &graph));
+        hipGraphExec_t exec = nullptr;
+        const hipError_t status = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        PHX_HIP(status);
+        graph_[from] = exec;
+    }
+}
+
+void PbfBatch::run(phx_pbf_state& state, const phx_control& control, const phx_pbf_parameters& parameters,
+                   uint32_t steps) {
+    require_vec(state.particles.positions, "state positions");
+    require_vec(state.particles.velocities, "state velocities");
+    require_vec(control.external_accelerations, "control");
+    const auto& pp = parameters.particles;
+    require(pp.masses && pp.rest_densities, PHX_ERROR_INVALID_ARGUMENT, "null particle parameters");
+    require(parameters.relaxation && parameters.artificial_pressure_strength && parameters.artificial_pressure_exponent &&
+                parameters.artificial_pressure_radius && parameters.xsph_viscosity && parameters.vorticity_confinement,
+            PHX_ERROR_INVALID_ARGUMENT, "null PBF parameters");
+    if (steps == 0u || n_ == 0u) {
+        state.particles.step_index += steps;
+        return;
+    }
+    const size_t bytes = sizeof(float) * n_;
+    // the batch's stream follows the caller's work on the state and inputs
+    PHX_HIP(hipEventRecord(ready_, caller_));
+    PHX_HIP(hipStreamWaitEvent(stream_, ready_, 0));
+    const float* in[6] = {state.particles.positions.x,  state.particles.positions.y,  state.particles.positions.z,
+                          state.particles.velocities.x, state.particles.velocities.y, state.particles.velocities.z};
+    float* first[6] = {positions_[0].x.data(),  positions_[0].y.data(),  positions_[0].z.data(),
+                       velocities_[0].x.data(), velocities_[0].y.data(), velocities_[0].z.data()};
+    for (int k = 0; k < 6; ++k) PHX_HIP(hipMemcpyAsync(first[k], in[k], bytes, hipMemcpyDeviceToDevice, stream_));
+    if (use_graph_) {
+        // a captured substep reads the control and parameter arrays it was captured with
+        const std::vector<const void*> inputs = {
+            control.external_accelerations.x, control.external_accelerations.y, control.external_accelerations.z,
+            pp.masses, pp.rest_densities, parameters.relaxation, parameters.artificial_pressure_strength,
+            parameters.artificial_pressure_exponent, parameters.artificial_pressure_radius,
+            parameters.xsph_viscosity, parameters.vorticity_confinement};
+        if (graph_[0] == nullptr || inputs != captured_inputs_) {
+            capture(control, parameters);
+            captured_inputs_ = inputs;
+        }
+    }
+    const uint64_t step = state.particles.step_index;
+    uint32_t from = 0u;
+    for (uint32_t k = 0; k < steps; ++k) {
+        if (use_graph_) {
+            PHX_HIP(hipGraphLaunch(graph_[from], stream_));
+        } else {
+            substep(from, step + k, control, parameters);
+        }
+        from = 1u - from;
+    }
+    float* out[6] = {state.particles.positions.x,  state.particles.positions.y,  state.particles.positions.z,
+                     state.particles.velocities.x, state.particles.velocities.y, state.particles.velocities.z};
+    const float* last[6] = {positions_[from].x.data(),  positions_[from].y.data(),  positions_[from].z.data(),
+                            velocities_[from].x.data(), velocities_[from].y.data(), velocities_[from].z.data()};
+    for (int k = 0; k < 6; ++k) PHX_HIP(hipMemcpyAsync(out[k], last[k], bytes, hipMemcpyDeviceToDevice, stream_));
+    PHX_HIP(hipEventRecord(done_, stream_));
+    PHX_HIP(hipStreamWaitEvent(caller_, done_, 0));
+    state.particles.step_index = step + steps;
 }
 
 }  // namespace phx

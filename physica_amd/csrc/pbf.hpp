@@ -137,4 +137,59 @@ class PbfSolver {
     PbfDerivativeWorkspace dw_;
 };
 
+// Independent PBF scenes stepped together, forward mode (the configured
+// uneven batch): the scenes share the support radius, time step and
+// iteration count, and each keeps its own box, gravity and particle count.
+// Every pass of a substep is one launch over all scenes (PBF::forward_step,
+// pbf/pbf.cpp:167-191, per scene), and with static boxes a substep is
+// captured once in a HIP graph and replayed.
+class PbfBatch {
+  public:
+    PbfBatch(const phx_domain_configuration* domains, uint32_t scene_count, const phx_pbf_configuration* configuration,
+             const phx_pbf_batch_configuration* batch_configuration, hipStream_t stream);
+    ~PbfBatch();
+    PbfBatch(const PbfBatch&) = delete;
+    PbfBatch& operator=(const PbfBatch&) = delete;
+
+    // `steps` substeps of every scene, in place on the concatenated state
+    // (scene k holds particles [offsets[k], offsets[k + 1])).
+    void run(phx_pbf_state& state, const phx_control& control, const phx_pbf_parameters& parameters, uint32_t steps);
+
+    uint32_t scene_count() const { return static_cast<uint32_t>(scenes_.size()); }
+    uint32_t particle_count() const { return n_; }
+    const std::vector<uint64_t>& offsets() const { return offsets_; }
+    uint32_t kernels_per_step() const { return kernels_per_step_; }
+    bool graphed() const { return use_graph_; }
+
+  private:
+    void substep(uint32_t from, uint64_t step, const phx_control& control, const phx_pbf_parameters& parameters);
+    void capture(const phx_control& control, const phx_pbf_parameters& parameters);
+    void release_graphs();
+
+    phx_pbf_configuration config_{};
+    hipStream_t caller_;
+    hipStream_t stream_ = nullptr;  // the batch's own stream (graphs cannot be captured on the null stream)
+    hipEvent_t ready_ = nullptr, done_ = nullptr;
+    std::vector<BatchScene> scenes_;
+    std::vector<uint64_t> offsets_;
+    uint32_t n_ = 0, cells_ = 0, blocks_ = 0;
+    float h_ = 0.0F, dt_ = 0.0F, spiky_ = 0.0F, skin_ = 0.3F;
+    bool static_boxes_ = true, use_graph_ = false;
+    uint32_t kernels_per_step_ = 0;
+    DeviceBuffer<BatchBlock> block_table_;
+    DeviceBuffer<BatchScene> scene_table_;
+    Neighborhood grid_;
+    VecBuffer positions_[2], velocities_[2];  // the state in original order, ping-pong
+    VecBuffer predicted_, xs_, x0_, pa_, pb_, vel_, vort_, vc_;
+    DeviceBuffer<float> m_, rho0_, eps_, strength_, exponent_, radius_, xsph_, confinement_;
+    DeviceBuffer<float> rho_, lambda_, magnitude_;
+    DeviceBuffer<uint16_t> skin_masks_, exact_;
+    DeviceBuffer<uint8_t> crowded_;
+    DeviceBuffer<uint32_t> flags_, exact_flags_;
+    // captured substeps 0 -> 1 and 1 -> 0, and the control and parameter
+    // arrays they read
+    hipGraphExec_t graph_[2] = {nullptr, nullptr};
+    std::vector<const void*> captured_inputs_;
+};
+
 }  // namespace phx

@@ -127,7 +127,29 @@ struct Step {
     const float *m, *rho0, *eps, *strength, *exponent, *radius, *xsph, *confinement;
     MaskView mv;  // the recorded step's masks and flags
     const double* inv_rho0;  // 1 / rho0 (reverse passes)
+    BoundaryGrid boundary;   // the step's sorted boundary samples; offsets null: none
 };
+
+// Visits particle i's fluid candidates through its masks where they apply or,
+// with boundary samples, every particle and every sample of the 27 cells,
+// each cell's samples after its particles (the order of the reference's
+// loops, e.g. pbf.cu:143-168). Samples are inputs: their pair terms reach
+// only particle i's values and adjoints.
+template <class Visit, class VisitBoundary>
+__device__ __forceinline__ void walk_with_samples(const Step& s, bool masked, uint32_t i, int cx, int cy, int cz,
+                                                  const int* delta, uint16_t* column, Visit&& visit,
+                                                  VisitBoundary&& visit_boundary) {
+    if (s.boundary.offsets != nullptr) {
+        for_each_candidate(s.grid, s.boundary, cx, cy, cz, visit, visit_boundary);
+    } else {
+        visit_candidates(s.grid, s.mv, masked, i, cx, cy, cz, delta, column, kBlock, visit);
+    }
+}
+
+__device__ __forceinline__ Float3 sample_position(const Step& s, uint32_t b) {
+    const float4 q = s.boundary.x[b];
+    return Float3{q.x, q.y, q.z};
+}
 
 __device__ __forceinline__ void cell_of_topology(const Step& s, uint32_t i, int& cx, int& cy, int& cz) {
     particle_cell(s.grid, load(s.xs, i), cx, cy, cz);
@@ -320,7 +342,7 @@ __global__ void __launch_bounds__(kBlock) pbf_vjp_correction_kernel(CorrectionVj
     double lcon = 0.0, mcon = 0.0, rcon = 0.0, scon = 0.0, econ = 0.0, qcon = 0.0;
     int cx, cy, cz;
     cell_of_topology(s, i, cx, cy, cz);
-    visit_candidates(s.grid, s.mv, masked, i, cx, cy, cz, delta, smask + threadIdx.x, kBlock, [&](uint32_t j) {
+    walk_with_samples(s, masked, i, cx, cy, cz, delta, smask + threadIdx.x, [&](uint32_t j) {
         if (j == i) return;
         const Float3 dd = sub(pi, load(a.pos, j));
         const D3 gr = d3(spiky_gradient(dd, s.h, s.spiky));
@@ -350,6 +372,21 @@ __global__ void __launch_bounds__(kBlock) pbf_vjp_correction_kernel(CorrectionVj
         mcon += rmu * inv_rrho0 * ob_rgr;
         pcon = add(pcon, spiky_hessian_one_division(dd, sub(scale(oa, cf * mu), scale(ob, rcf * rmu)), h, k));
         pcon = add(pcon, sub(scale(d3(art.gradient), sa), scale(d3(rart.gradient), rsa)));
+    }, [&](uint32_t b) {
+        // a boundary sample: V_b (lambda_i + s_corr) grad W (pbf.cu:390-397)
+        const Float3 dd = sub(pi, sample_position(s, b));
+        const D3 gr = d3(spiky_gradient(dd, s.h, s.spiky));
+        const Artificial art = artificial_full(dd, s.h, s.strength[i], s.exponent[i], s.radius[i], a.vanishes[i] != 0u,
+                                               a.strength_wanted != 0u);
+        const double vol = static_cast<double>(s.boundary.x[b].w);
+        const double mu = static_cast<double>(lam_i) + static_cast<double>(art.value);
+        const double sa = vol * dot(oa, gr);
+        lcon += sa;
+        scon += sa * static_cast<double>(art.d_strength);
+        econ += sa * static_cast<double>(art.d_exponent);
+        qcon += sa * static_cast<double>(art.d_radius);
+        pcon = add(pcon, spiky_hessian_one_division(dd, scale(oa, vol * mu), h, k));
+        pcon = add(pcon, scale(d3(art.gradient), sa));
     });
     store(a.p_out, i, add(oa, pcon));
     a.lambda_a[i] = lcon;
@@ -402,9 +439,10 @@ __global__ void __launch_bounds__(kBlock) pbf_vjp_lambda_kernel(LambdaVjpArgs a)
     D3 pl{0.0, 0.0, 0.0}, pd{0.0, 0.0, 0.0};
     double ml = 0.0, md = 0.0;
     double rcon = lam_a * static_cast<double>(a.rho[i]) / (den * rho0 * rho0);
+    double rd = 0.0;  // the density's rest-density adjoint (boundary samples)
     int cx, cy, cz;
     cell_of_topology(s, i, cx, cy, cz);
-    visit_candidates(s.grid, s.mv, masked, i, cx, cy, cz, delta, smask + threadIdx.x, kBlock, [&](uint32_t j) {
+    walk_with_samples(s, masked, i, cx, cy, cz, delta, smask + threadIdx.x, [&](uint32_t j) {
         const Float3 dd = sub(pi, load(a.pos, j));
         const double2 jterms = a.lambda_terms[j];
         // density term (the self term has a zero gradient)
@@ -427,11 +465,20 @@ __global__ void __launch_bounds__(kBlock) pbf_vjp_lambda_kernel(LambdaVjpArgs a)
         const double rcbar = 2.0 * jterms.y * (rcf * gr2 - dot(gr, sgj));
         ml += rcbar * inv_jrho0;
         pl = add(pl, spiky_hessian_one_division(dd, sub(gbar, rgbar), h, k));
+    }, [&](uint32_t b) {
+        // a boundary sample: the density term rho0_i V_b W (density.cu:88-93)
+        // and V_b grad W in the gradient sum only (pbf.cu:226-229)
+        const Float3 dd = sub(pi, sample_position(s, b));
+        const double vol = static_cast<double>(s.boundary.x[b].w);
+        pd = add(pd, scale(d3(poly6_gradient(dd, s.h)), rho_a * rho0 * vol));
+        rd += rho_a * vol * static_cast<double>(poly6(dd, s.h));
+        pl = add(pl, spiky_hessian_one_division(dd, scale(sgi, 2.0 * dbar * vol), h, k));
     });
     store(a.p_out, i, add(add(load(a.p_out, i), pl), pd));
     a.m_a[i] += ml;
     a.m_a[i] += md;
     a.rho0_a[i] += rcon;
+    a.rho0_a[i] += rd;
     a.eps_a[i] += dbar;
 }
 
@@ -548,7 +595,7 @@ __global__ void __launch_bounds__(kBlock) pbf_jvp_lambda_kernel(LambdaJvpArgs a)
     float den_t = a.t.eps[i];
     int cx, cy, cz;
     cell_of_topology(s, i, cx, cy, cz);
-    visit_candidates(s.grid, s.mv, masked, i, cx, cy, cz, delta, smask + threadIdx.x, kBlock, [&](uint32_t j) {
+    walk_with_samples(s, masked, i, cx, cy, cz, delta, smask + threadIdx.x, [&](uint32_t j) {
         const Float3 dd = sub(pi, load(a.pos, j));
         const Float3 dd_t = sub(ti, load(a.p_t, j));
         rho_t += a.t.m[j] * poly6(dd, s.h) + s.m[j] * dot(poly6_gradient(dd, s.h), dd_t);
@@ -562,6 +609,13 @@ __global__ void __launch_bounds__(kBlock) pbf_jvp_lambda_kernel(LambdaJvpArgs a)
         sg = add(sg, ng);
         sg_t = add(sg_t, ng_t);
         den_t += 2.0F * dot(ng, ng_t);
+    }, [&](uint32_t b) {
+        // a boundary sample (density.cu:59-63, pbf.cu:164-168), itself fixed
+        const Float3 dd = sub(pi, sample_position(s, b));
+        const float vol = s.boundary.x[b].w;
+        rho_t += vol * (rho0_t * poly6(dd, s.h) + rho0 * dot(poly6_gradient(dd, s.h), ti));
+        sg = add(sg, scale(spiky_gradient(dd, s.h, s.spiky), vol));
+        sg_t = add(sg_t, scale(spiky_hessian(dd, ti, s.h, s.spiky), vol));
     });
     den_t += 2.0F * dot(sg, sg_t);
     const float den = a.den[i];
@@ -593,7 +647,7 @@ __global__ void __launch_bounds__(kBlock) pbf_jvp_correction_kernel(CorrectionJv
     Float3 res{0.0F, 0.0F, 0.0F};
     int cx, cy, cz;
     cell_of_topology(s, i, cx, cy, cz);
-    visit_candidates(s.grid, s.mv, masked, i, cx, cy, cz, delta, smask + threadIdx.x, kBlock, [&](uint32_t j) {
+    walk_with_samples(s, masked, i, cx, cy, cz, delta, smask + threadIdx.x, [&](uint32_t j) {
         if (j == i) return;
         const Float3 dd = sub(pi, load(a.pos, j));
         const Float3 dd_t = sub(ti, load(a.p_t, j));
@@ -606,6 +660,16 @@ __global__ void __launch_bounds__(kBlock) pbf_jvp_correction_kernel(CorrectionJv
         const float mu_t = a.lambda_t[i] + a.lambda_t[j] + a_t;
         res = add(res, add(scale(spiky_gradient(dd, s.h, s.spiky), cf_t * mu + cf * mu_t),
                            scale(spiky_hessian(dd, dd_t, s.h, s.spiky), cf * mu)));
+    }, [&](uint32_t b) {
+        // a boundary sample: V_b (lambda_i + s_corr) grad W (pbf.cu:321-329)
+        const Float3 dd = sub(pi, sample_position(s, b));
+        const Artificial art = artificial_full(dd, s.h, s.strength[i], s.exponent[i], s.radius[i]);
+        const float a_t = dot(art.gradient, ti) + art.d_strength * a.t.strength[i] +
+                          art.d_exponent * a.t.exponent[i] + art.d_radius * a.t.radius[i];
+        const float mu = a.lambda[i] + art.value;
+        res = add(res, scale(add(scale(spiky_gradient(dd, s.h, s.spiky), a.lambda_t[i] + a_t),
+                                 scale(spiky_hessian(dd, ti, s.h, s.spiky), mu)),
+                             s.boundary.x[b].w));
     });
     Float3 out = add(ti, res);
     const uint8_t m = a.mask[i];
@@ -829,7 +893,9 @@ void PbfSolver::jvp_step(const phx_pbf_parameters& parameters, const PbfCache& c
            dw_.radius.data(),
            dw_.xsph.data(),
            dw_.confinement.data(),
-           skin_view(cache, n)};
+           skin_view(cache, n),
+           nullptr,
+           cache.neighborhood.boundary()};
     const TangentParams tp{dw_.m_t.data(),        dw_.rho0_t.data(),   dw_.eps_t.data(),  dw_.strength_t.data(),
                            dw_.exponent_t.data(), dw_.radius_t.data(), dw_.xsph_t.data(), dw_.confinement_t.data()};
     PredictJvpArgs pa{n,
@@ -926,7 +992,8 @@ void PbfSolver::vjp_step(const phx_pbf_parameters& parameters, const PbfCache& c
            dw_.xsph.data(),
            dw_.confinement.data(),
            skin_view(cache, n),
-           dw_.inv_rho0.data()};
+           dw_.inv_rho0.data(),
+           cache.neighborhood.boundary()};
     // sorted parameter adjoints start at zero (+0.0 is the all-zero bit pattern)
     for (DeviceBuffer<double>* b : {&dw_.m_a, &dw_.rho0_a, &dw_.eps_a, &dw_.strength_a, &dw_.exponent_a,
                                     &dw_.radius_a, &dw_.xsph_a, &dw_.confinement_a})

@@ -58,17 +58,11 @@ struct MaskDensityArgs {
 // crowded and walks the full stencil here and in every later pass.
 template <bool kEos>
 __global__ void __launch_bounds__(kBlock) mask_density_kernel(MaskDensityArgs a) {
-    __shared__ uint16_t smask[kStencil * kBlock];
-    __shared__ int delta[kStencil];
-    fill_slot_deltas(a.grid, delta);
-    __syncthreads();
     const uint32_t i = xcd_block() * kBlock + threadIdx.x;
     if (i >= a.n) return;
-    uint16_t* column = smask + threadIdx.x;
     const Float3 p = xyz(a.xm[i]);
     int cx, cy, cz;
     particle_cell(a.grid, p, cx, cy, cz);
-    uint32_t nonzero = 0u;
     float result = 0.0F;
     if (a.boundary.offsets != nullptr) {
         a.crowded[i] = 1u;  // no masks are built with boundary samples
@@ -83,16 +77,14 @@ __global__ void __launch_bounds__(kBlock) mask_density_kernel(MaskDensityArgs a)
                 const float4 b = a.boundary.x[s];
                 result += rest * b.w * cubic_value(pair_distance(sub(p, xyz(b))), a.kc);
             });
-    } else if (build_masks(a.grid, a.xm, a.n, i, p, cx, cy, cz, a.kc.sqrt_support, a.t_skin, a.exact, a.skin, column,
-                           kBlock, &nonzero)) {
+    } else if (build_masks(a.grid, a.xm, a.n, i, p, cx, cy, cz, a.kc.sqrt_support, a.t_skin, a.exact, a.skin,
+                           [&](float4 q, float d2) { result += q.w * cubic_value(sqrt_rn(d2), a.kc); })) {
+        // the density over the exact mask, summed while building it (the
+        // candidates it leaves out add exact zeros in the reference)
         a.crowded[i] = 0u;
-        walk_masked(a.grid.offsets, cell_of(a.grid, cx, cy, cz), delta, column, kBlock, nonzero, [&](uint32_t s) {
-            const float4 q = a.xm[s];
-            const float dist = pair_distance(sub(p, xyz(q)));
-            result += q.w * cubic_value(dist, a.kc);
-        });
     } else {
         a.crowded[i] = 1u;
+        result = 0.0F;  // the partial sum of the truncated masks is dropped
         for_each_candidate(a.grid, cx, cy, cz, [&](uint32_t s) {
             const float4 q = a.xm[s];
             result += q.w * cubic_value(pair_distance(sub(p, xyz(q))), a.kc);

@@ -80,15 +80,7 @@ def test_invalid_domain_rejected_before_device_work(domain, status, fragment):
     assert fragment in _capi.library().phx_last_error().decode()
 
 
-def test_boundary_particles_rejected_where_unsupported():
-    sample = [((0.1, 0.1, 0.1), (0, 0, 0), 1e-6)]
-    # derivatives with boundary samples exist for the four SPH methods; PBF
-    # refuses them before device work
-    h = ctypes.c_void_p()
-    got = _capi.library().phx_pbf_create(ctypes.byref(cfg(boundary_particles=sample).to_c()),
-                                         ctypes.byref(PBF.Configuration().to_c()), 1, None, ctypes.byref(h))
-    assert got == _capi.PHX_ERROR_UNSUPPORTED and h.value is None
-    assert "forward mode only" in _capi.library().phx_last_error().decode()
+def test_boundary_particles_with_a_null_array_rejected():
     # a non-zero count with a null host array
     c = cfg().to_c()
     c.boundary_particle_count = 3

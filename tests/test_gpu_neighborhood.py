@@ -35,8 +35,6 @@ def positions(kind, n, rng, box):
 @pytest.mark.parametrize("n", [0, 1, 777, 65536])
 def test_build_neighborhood_bit_exact(gpu_device, kind, n):
     print(f"neighbourhood case kind={kind} n={n}", flush=True)
-    if kind == "one_cell" and n > 4096:
-        n = 4096  # the in-cell rank is O(k) per particle
     cfg = domain(n)
     rng = np.random.default_rng(n + len(kind))
     pos = positions(kind, n, rng, (1.0, 0.6, 0.8))
@@ -49,6 +47,30 @@ def test_build_neighborhood_bit_exact(gpu_device, kind, n):
     assert np.array_equal(idx.cpu().numpy().astype(np.uint32), oi)
     assert np.array_equal(off.cpu().numpy().astype(np.uint32), oo)
     assert (view.cells_x, view.cells_y, view.cells_z) == cfg.grid_resolution()
+
+
+@pytest.mark.parametrize("n", [129, 5000, 1_000_000])
+def test_every_particle_in_one_cell(gpu_device, n):
+    """Cells with more than 128 particles are ordered by a radix sort of
+    their indices (one workgroup per cell): up to every particle of a
+    1,000,000-particle scene in one cell, next to ordinary cells, the order
+    is still the stable sort of the keys and the offsets count every cell."""
+    print(f"neighbourhood one cell n={n}", flush=True)
+    rng = np.random.default_rng(n)
+    pos = (np.full((3, n), 0.3) + rng.uniform(0, 0.01, size=(3, n))).astype(np.float32)
+    # a tenth spread over the box, so small and large cells mix
+    spread = n // 10
+    pos[:, :spread] = (rng.uniform(0.0, 1.0, size=(3, spread)) * np.array([[1.0], [0.6], [0.8]])).astype(np.float32)
+    cfg = domain(n)
+    field = VectorField.zeros(n, gpu_device)
+    field.upload(pos)
+    keys, idx, off, view = build_neighborhood(cfg, 0, field, gpu_device)[:4]
+    torch.cuda.synchronize()
+    ok, oi, oo = oracle.build_neighborhood(oracle.make_domain(cfg), 0, pos)
+    assert np.array_equal(keys.cpu().numpy().astype(np.uint64), ok)
+    assert np.array_equal(idx.cpu().numpy().astype(np.uint32), oi)
+    assert np.array_equal(off.cpu().numpy().astype(np.uint32), oo)
+    assert np.diff(oo).max() >= n - spread - 1
 
 
 def test_moving_box(gpu_device):

@@ -56,10 +56,3 @@ def test_artificial_pressure_with_boundary_within_tolerance(gpu_device):
                           np.zeros((3, n)), sc.parameters, 4)
     assert max_rel(nxt.particles.positions.numpy(), exp["positions"]) < 1e-5
     assert max_rel(nxt.particles.velocities.numpy(), exp["velocities"]) < 1e-5
-
-
-def test_boundary_particles_are_forward_only(gpu_device):
-    sc = scenes.pbf_dam_break(shape=(8, 8, 8))
-    add_boundary_samples(sc)
-    with pytest.raises(RuntimeError, match="forward mode only"):
-        PBF(sc.domain, PBF.Configuration(), mode=ExecutionMode.differentiable, device=gpu_device)

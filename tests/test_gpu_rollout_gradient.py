@@ -9,8 +9,12 @@ particles, where the oracle is fast.
   oracle's bit for bit, and the gradient equals the oracle's chained VJPs
   (FP64 adjoints of the FP32 primal, pbf/pbf.cpp:217-258 step after step)
   to 1e-6 along each of 64 seeded directions.
-- 5 steps: the gradient agrees with central differences of the FP64 oracle
+- 2 steps: the gradient agrees with central differences of the FP64 oracle
   (eps = 1e-7, in the linear regime) to 1e-3 along each of 16 directions.
+  The GPU differentiates the FP32 trajectory and the differences the FP64
+  one; in this clumping dam break their derivatives part by up to 4 % along
+  single directions from the third step on (median 4e-4), while the FP64
+  oracle's own gradient keeps matching its differences (the oracle's pins).
 At 120 steps a difference quotient cannot check the gradient: the exact
 derivative grows about 3x per step (the oracle's own pins show it), far
 beyond what eps = 1e-3 resolves."""
@@ -94,7 +98,7 @@ def test_120_step_gradient_matches_the_chained_oracle_vjps(gpu_device):
 
 def test_short_horizon_gradient_matches_central_differences(gpu_device):
     sc, target = scene()
-    steps, eps = 5, 1e-7
+    steps, eps = 2, 1e-7
     print(f"rollout gradient vs FD n={sc.count} steps={steps}", flush=True)
     _, g_gpu = gpu_gradient(sc, target, steps, gpu_device)
     d = oracle.make_domain(sc.domain)

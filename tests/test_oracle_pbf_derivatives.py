@@ -162,3 +162,87 @@ def test_vjp_accumulates_into_outputs():
                        2, "f64")
     for k in ("positions", "velocities", "controls") + PARAMS:
         np.testing.assert_allclose(b[k], 2 * a[k], rtol=1e-12, atol=1e-300)
+
+
+def scene_with_samples(moving=True, seed=3):
+    """The swirling block over a plane of boundary samples half a spacing
+    below its lowest particles (moving along x when `moving`, so the sample
+    positions at the step's time enter), close enough that every iteration's
+    density, lambda and correction carry sample terms."""
+    sc = scene(seed=seed)
+    s = float(sc.domain.particle_radius) * 2.0
+    lo, hi = sc.positions.min(axis=1), sc.positions.max(axis=1)
+    gx = np.arange(lo[0] - 1.5 * s, hi[0] + 1.5 * s, 0.8 * s)
+    gz = np.arange(lo[2] - 1.5 * s, hi[2] + 1.5 * s, 0.8 * s)
+    fx, fz = np.meshgrid(gx, gz, indexing="ij")
+    pos = np.stack([fx.ravel(), np.full(fx.size, lo[1] - 0.6 * s), fz.ravel()]).astype(np.float32)
+    b = pos.shape[1]
+    vel = np.zeros((3, b), np.float32)
+    if moving:
+        vel[0] = 0.2
+    vol = np.full(b, 0.8 * s ** 3, np.float32)
+    sc.domain.boundary_particles = [(tuple(pos[:, k]), tuple(vel[:, k]), float(vol[k])) for k in range(b)]
+    return sc, oracle.make_domain(sc.domain, (pos, vel, vol))
+
+
+@pytest.mark.parametrize("iterations", [1, 3])
+def test_boundary_samples_dot_product_identity(iterations):
+    """With boundary samples (pbf.cu:164-168, 226-229, 321-329, 390-397 and
+    the density's rho0_i V_b W): the JVP and VJP stay each other's transpose."""
+    sc, d = scene_with_samples()
+    rng = np.random.default_rng(60 + iterations)
+    t, w = directions(sc, rng)
+    u = np.zeros_like(sc.positions)
+    jt = oracle.pbf_jvp(d, 0, sc.positions, sc.velocities, u, sc.parameters, t, iterations, "f64")
+    jw = oracle.pbf_vjp(d, 0, sc.positions, sc.velocities, u, sc.parameters, w["positions"], w["velocities"],
+                        iterations, "f64")
+    left = np.sum(jt["positions"] * w["positions"]) + np.sum(jt["velocities"] * w["velocities"])
+    right = sum(np.sum(t[k] * jw[k]) for k in ("positions", "velocities", "controls") + PARAMS)
+    assert abs(left - right) <= 1e-11 * (abs(left) + abs(right)), (left, right)
+    # the samples reach the rest-density adjoint through the density and nothing else vanishes
+    assert np.abs(jw["rest_densities"]).max() > 0
+
+
+def test_boundary_samples_jvp_matches_central_differences():
+    """Richardson-extrapolated central differences of the FP64 forward with
+    samples, along random directions and along the rest densities alone
+    (which the samples' density term rho0_i V_b W makes matter)."""
+    sc, d = scene_with_samples()
+    rng = np.random.default_rng(70)
+    u = np.zeros_like(sc.positions, dtype=np.float64)
+    x = sc.positions.astype(np.float64)
+    v = sc.velocities.astype(np.float64)
+    base = {k: sc.parameters[k].astype(np.float64) for k in sc.parameters}
+    t_all, _ = directions(sc, rng)
+    only_rest = {k: np.zeros_like(val) for k, val in t_all.items()}
+    only_rest["rest_densities"] = t_all["rest_densities"]
+    for t in (t_all, only_rest):
+        jt = oracle.pbf_jvp(d, 0, x, v, u, base, t, 3, "f64")
+
+        def central(eps):
+            def shifted(sign):
+                p = {k: base[k] + sign * eps * t[k] if k in PARAMS else base[k] for k in base}
+                return forward(d, sc, x + sign * eps * t["positions"], v + sign * eps * t["velocities"],
+                               u + sign * eps * t["controls"], p, 3)
+
+            (xp, vp), (xm, vm) = shifted(1.0), shifted(-1.0)
+            return (xp - xm) / (2 * eps), (vp - vm) / (2 * eps)
+
+        (fx1, fv1), (fx2, fv2) = central(2e-6), central(1e-6)
+        for got, fd in ((jt["positions"], (4 * fx2 - fx1) / 3), (jt["velocities"], (4 * fv2 - fv1) / 3)):
+            err = np.abs(got - fd).max() / np.abs(fd).max()
+            assert err < 2e-6, err
+
+
+def test_boundary_terms_change_the_derivatives():
+    """The same step without the samples gives other derivatives: the terms
+    above are exercised, not vacuous."""
+    sc, d = scene_with_samples()
+    rng = np.random.default_rng(80)
+    _, w = directions(sc, rng)
+    u = np.zeros_like(sc.positions)
+    with_samples = oracle.pbf_vjp(d, 0, sc.positions, sc.velocities, u, sc.parameters, w["positions"],
+                                  w["velocities"], 2, "f64")
+    plain = oracle.pbf_vjp(oracle.make_domain(sc.domain), 0, sc.positions, sc.velocities, u, sc.parameters,
+                           w["positions"], w["velocities"], 2, "f64")
+    assert np.abs(with_samples["positions"] - plain["positions"]).max() > 1e-3 * np.abs(plain["positions"]).max()
