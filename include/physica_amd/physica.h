@@ -235,6 +235,27 @@ phx_status phx_build_neighborhood(phx_stream stream, const phx_domain_configurat
                                   uint64_t* sorted_boundary_keys, uint32_t* sorted_boundary_indices,
                                   uint32_t* boundary_cell_offsets, phx_neighborhood_view* view);
 
+/* Scratch bytes phx_build_neighborhood_noalloc holds per GPU (64 MiB): the
+ * grid scratch of phx_build_neighborhood without boundary samples,
+ * 3 round256(4N) + round256(4(C+2)) + round256(4 ceil((C+1)/2048))
+ * + round256(4 (N/129 + 1)) + 256 bytes, must fit (about 4.4M particles
+ * with 4M cells). */
+#define PHX_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+
+/* No counterpart in the reference. phx_build_neighborhood without a
+ * workspace argument and without any allocation, for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). Same outputs as phx_build_neighborhood;
+ * its scratch is a fixed device array of PHX_NOALLOC_SCRATCH_BYTES on each
+ * GPU, kept until the process ends and shared by every call on that GPU, so
+ * calls that overlap on one GPU race. Larger scenes return PHX_ERROR_CAPACITY;
+ * boundary samples return PHX_ERROR_UNSUPPORTED (they are uploaded from host
+ * memory); the stream's device must be the current device. */
+phx_status phx_build_neighborhood_noalloc(phx_stream stream, const phx_domain_configuration* domain,
+                                          uint64_t step_index, phx_const_vector_view positions,
+                                          uint64_t* sorted_keys, uint32_t* sorted_particle_indices,
+                                          uint32_t* cell_offsets, phx_neighborhood_view* view);
+
 /* density_forward (density.cu:18-40): rho_i = sum_j m_j W(x_i - x_j) over the
  * 27 cells of topology_positions[i], j = i included, each cell's boundary
  * samples b after its particles adding rho0_i V_b W(x_i - x_b(t)); W = poly6

@@ -325,6 +325,11 @@ PROTOTYPES = {
         [c_void_p, POINTER(DomainConfiguration), c_uint64, VectorView, c_void_p, c_size_t, c_void_p, c_void_p,
          c_void_p, c_void_p, c_void_p, c_void_p, POINTER(NeighborhoodView)],
     ),
+    "phx_build_neighborhood_noalloc": (
+        c_int,
+        [c_void_p, POINTER(DomainConfiguration), c_uint64, VectorView, c_void_p, c_void_p, c_void_p,
+         POINTER(NeighborhoodView)],
+    ),
     "phx_pbf_batch_create": (
         c_int,
         [POINTER(DomainConfiguration), c_uint32, POINTER(PbfConfiguration), POINTER(PbfBatchConfiguration), c_void_p,

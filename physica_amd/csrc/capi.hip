@@ -2,6 +2,7 @@
 // Exceptions never cross the ABI: each call maps them to a phx_status and a
 // per-thread message.
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 
@@ -301,6 +302,38 @@ void fill_neighborhood_view(const phx::Domain& d, const phx::GridGeometry& geo, 
     view->cell_size = geo.cell_size;
 }
 
+// Fixed device scratch of phx_build_neighborhood_noalloc. Every GPU the
+// library runs on holds its own copy until the process ends, and every call
+// on one GPU shares it: calls that overlap on one GPU race. In host code the
+// name `scratch` is only a host-side stand-in; kernels receive the per-device
+// address that hipGetSymbolAddress returns.
+constexpr size_t kNoAllocScratchBytes = PHX_NOALLOC_SCRATCH_BYTES;
+static __device__ __align__(256) uint8_t scratch[kNoAllocScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+void* g_scratch_address[kMaxDevices] = {};
+
+// The scratch address on the device of `stream`, looked up once per device
+// (that device current) on the first call there.
+void* noalloc_scratch(hipStream_t stream) {
+    int current = -1;
+    PHX_HIP(hipGetDevice(&current));
+    int device = current;
+    if (stream != nullptr) PHX_HIP(hipStreamGetDevice(stream, &device));
+    phx::require(device == current, PHX_ERROR_INVALID_ARGUMENT,
+                 "phx_build_neighborhood_noalloc: the stream's device must be the current device");
+    phx::require(device >= 0 && device < kMaxDevices, PHX_ERROR_UNSUPPORTED, "device ordinal above 63");
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (g_scratch_address[device] == nullptr) {
+        void* ptr = nullptr;
+        PHX_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        phx::require(ptr != nullptr, PHX_ERROR_DEVICE, "hipGetSymbolAddress returned no scratch address");
+        g_scratch_address[device] = ptr;
+    }
+    return g_scratch_address[device];
+}
+
 }  // namespace
 
 extern "C" {
@@ -401,6 +434,34 @@ phx_status phx_build_neighborhood(phx_stream stream, const phx_domain_configurat
         if (view != nullptr)
             fill_neighborhood_view(d, geo, sorted_keys, sorted_particle_indices, cell_offsets, sorted_boundary_keys,
                                    sorted_boundary_indices, boundary_cell_offsets, view);
+    });
+}
+
+phx_status phx_build_neighborhood_noalloc(phx_stream stream, const phx_domain_configuration* domain,
+                                          uint64_t step_index, phx_const_vector_view positions,
+                                          uint64_t* sorted_keys, uint32_t* sorted_particle_indices,
+                                          uint32_t* cell_offsets, phx_neighborhood_view* view) {
+    return guarded([&] {
+        const phx::Domain d = phx::make_domain(domain);
+        const uint32_t n = d.config.particle_count;
+        const hipStream_t s = stream_of(stream);
+        check_neighborhood_outputs(d, positions, sorted_particle_indices, cell_offsets);
+        phx::require(d.config.boundary_particle_count == 0u, PHX_ERROR_UNSUPPORTED,
+                     "phx_build_neighborhood_noalloc takes no boundary samples (they are uploaded from host memory "
+                     "on every call); use phx_build_neighborhood");
+        const size_t need = phx::grid_scratch_bytes(n, d.cells);
+        phx::require(need <= kNoAllocScratchBytes, PHX_ERROR_CAPACITY,
+                     "phx_build_neighborhood_noalloc: " + std::to_string(n) + " particles and " +
+                         std::to_string(d.cells) + " cells need " + std::to_string(need) +
+                         " bytes of scratch, above its fixed " + std::to_string(kNoAllocScratchBytes));
+        void* base = noalloc_scratch(s);
+        const phx::GridScratch scratch = phx::carve_grid_scratch(base, kNoAllocScratchBytes, n, d.cells);
+        const phx::GridGeometry geo =
+            build_fluid_neighborhood(s, d, step_index, positions, scratch, sorted_keys, sorted_particle_indices,
+                                     cell_offsets);
+        if (view != nullptr)
+            fill_neighborhood_view(d, geo, sorted_keys, sorted_particle_indices, cell_offsets, nullptr, nullptr,
+                                   nullptr, view);
     });
 }
 
