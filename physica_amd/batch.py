@@ -21,7 +21,7 @@ import numpy as np
 import torch
 
 from . import _capi
-from .particle import PBF, Control, Domain, DomainConfiguration, _require_gpu, _SolverStream
+from .particle import PBF, Control, Domain, DomainConfiguration, _release_handle, _require_gpu, _SolverStream
 
 TUNING = Path(__file__).resolve().parent / "tuning" / "pbf_batch.json"
 
@@ -61,6 +61,7 @@ class PbfBatch:
         _capi.call("phx_pbf_batch_create", self._domains_c, self.scene_count, ctypes.byref(self.configuration.to_c()),
                    ctypes.byref(batch_cfg), self._stream.cuda_stream, ctypes.byref(handle))
         self._handle = handle
+        self._release = _capi.library().phx_pbf_batch_destroy
         offsets = (ctypes.c_uint64 * (self.scene_count + 1))()
         _capi.call("phx_pbf_batch_particle_offsets", self._handle, offsets)
         self.offsets: List[int] = [int(v) for v in offsets]
@@ -70,10 +71,7 @@ class PbfBatch:
                                     first.particle_radius, first.gravity, first.boundary)
         self.domain = Domain(total, self.device)
 
-    def __del__(self):
-        if getattr(self, "_handle", None) and self._handle.value:
-            _capi.library().phx_pbf_batch_destroy(self._handle)
-            self._handle = ctypes.c_void_p()
+    __del__ = _release_handle
 
     def allocate_state(self) -> PBF.State:
         return PBF.State(self.domain.allocate_particle_state())

@@ -334,8 +334,11 @@ void DfsphSolver::forward_step(const phx_sph_dfsph_state& state, const phx_contr
             auto launch_iteration = [&](uint32_t it) {
                 set_update(phase, it, first_slot + it - 1u);
                 pa.cv = Convergence{tol_partials_.data(), done, rho0};
-                launch_projection(profiler_, stream_, blocks, pa,
-                                  phase == 0u ? kDfsphProjectionDivergence : kDfsphProjectionDensity);
+                if (phase == 0u) {
+                    launch_projection(profiler_, stream_, blocks, pa, kDfsphProjectionDivergence, kProjectDivergence);
+                } else {
+                    launch_projection(profiler_, stream_, blocks, pa, kDfsphProjectionDensity, kProjectDensity);
+                }
                 converge_kernel<<<1, kConvergeBlock, 0, stream_>>>(tol_partials_.data(), blocks * kWaves, n,
                                                                    tolerance, it, max_it, done);
                 PHX_HIP(hipGetLastError());
@@ -387,7 +390,7 @@ void DfsphSolver::forward_step(const phx_sph_dfsph_state& state, const phx_contr
         pa.t = t_div_.data();
         for (uint32_t it = 1; it <= nd; ++it) {  // sph.cpp:580-587
             set_update(0u, it, slot);
-            launch_projection(profiler_, stream_, blocks, pa, kDfsphProjectionDivergence);
+            launch_projection(profiler_, stream_, blocks, pa, kDfsphProjectionDivergence, kProjectDivergence);
             set_slot(++slot);
             if (it < nd) {
                 launch_force<false, kDivPredict>(profiler_, stream_, blocks, fa, kDfsphForceDivergence);
@@ -406,7 +409,7 @@ void DfsphSolver::forward_step(const phx_sph_dfsph_state& state, const phx_contr
     fa.warm_div_out = nullptr;
     for (uint32_t it = 1; it <= ne; ++it) {
         set_update(1u, it, slot);
-        launch_projection(profiler_, stream_, blocks, pa, kDfsphProjectionDensity);
+        launch_projection(profiler_, stream_, blocks, pa, kDfsphProjectionDensity, kProjectDensity);
         set_slot(++slot);
         fa.final_step = it == ne ? 1u : 0u;
         launch_force<false, kDenPredict>(profiler_, stream_, blocks, fa, kDfsphForceDensity);

@@ -81,24 +81,25 @@ __device__ __forceinline__ void walk_masked(const uint32_t* __restrict__ offsets
 }
 
 // walk_masked with the memory latency of each step taken off the critical
-// path: the base index of the next non-empty slot is fetched when the
-// current slot starts, and the data of the next candidate (load(s), which
-// returns a value) is fetched before the current one is processed
-// (process(value)), in the same order as walk_masked visits them. load()
-// must be safe to call with any s < n (lanes without a next candidate load
-// `fallback`, a valid index, and discard it).
-template <class Load, class Process>
-__device__ __forceinline__ void walk_masked_prefetch(const uint32_t* __restrict__ offsets, uint32_t center,
-                                                     const int* delta, const uint16_t* column, uint32_t stride,
-                                                     uint32_t nonzero, uint32_t fallback, Load&& load,
-                                                     Process&& process) {
+// path: the base index of the next non-empty slot (slot_base(k): the sorted
+// index of the first particle of stencil cell k, or its place in a staged
+// copy) is fetched when the current slot starts, and the data of the next
+// kDepth candidates (load(s), which returns a value) is fetched before the
+// current one is processed (process(value)), in the same order as
+// walk_masked visits them. load() must be safe to call with any index the
+// bases give (lanes without a next candidate load `fallback`, a valid index,
+// and discard it).
+template <int kDepth, class SlotBase, class Load, class Process>
+__device__ __forceinline__ void walk_slots_prefetch(SlotBase&& slot_base, const uint16_t* column, uint32_t stride,
+                                                    uint32_t nonzero, uint32_t fallback, Load&& load,
+                                                    Process&& process) {
     auto fetch = [&](uint32_t& mask, uint32_t& base) {
         mask = 0u;
         if (nonzero != 0u) {
             const uint32_t k = static_cast<uint32_t>(__builtin_ctz(nonzero));
             nonzero &= nonzero - 1u;
             mask = column[k * stride];
-            base = offsets[center + static_cast<uint32_t>(delta[k])];
+            base = slot_base(k);
         }
     };
     uint32_t m = 0u, base = 0u, m_next = 0u, base_next = 0u;
@@ -116,16 +117,35 @@ __device__ __forceinline__ void walk_masked_prefetch(const uint32_t* __restrict_
         m &= m - 1u;
         return have ? s : fallback;
     };
-    bool have;
-    auto data = load(next(have));
-    while (have) {
-        const auto current = data;
-        data = load(next(have));
-        // keeps the next candidate's loads issued ahead of this one's
-        // arithmetic (the compiler would otherwise sink them to their use)
-        asm volatile("" ::: "memory");
-        process(current);
+    using Data = decltype(load(0u));
+    Data data[kDepth];
+    bool have[kDepth];
+#pragma unroll
+    for (int d = 0; d < kDepth; ++d) data[d] = load(next(have[d]));
+    // candidates arrive in order, so a lane is done at its first empty slot;
+    // the ring is unrolled, so every data[d] stays in registers
+    for (;;) {
+#pragma unroll
+        for (int d = 0; d < kDepth; ++d) {
+            if (!have[d]) return;
+            const Data current = data[d];
+            data[d] = load(next(have[d]));
+            // keeps the loads issued ahead of this candidate's arithmetic
+            // (the compiler would otherwise sink them to their use)
+            asm volatile("" ::: "memory");
+            process(current);
+        }
     }
+}
+
+// walk_slots_prefetch over the grid's cell offsets.
+template <int kDepth = 1, class Load, class Process>
+__device__ __forceinline__ void walk_masked_prefetch(const uint32_t* __restrict__ offsets, uint32_t center,
+                                                     const int* delta, const uint16_t* column, uint32_t stride,
+                                                     uint32_t nonzero, uint32_t fallback, Load&& load,
+                                                     Process&& process) {
+    walk_slots_prefetch<kDepth>([&](uint32_t k) { return offsets[center + static_cast<uint32_t>(delta[k])]; },
+                                column, stride, nonzero, fallback, load, process);
 }
 
 // Clears particle i's own bit from its LDS column (slot 13, its own cell,
@@ -149,16 +169,15 @@ __device__ __forceinline__ uint32_t xcd_block() {
 
 // Builds both masks of particle i at position p: `exact` marks d2 < t_exact,
 // `skin` marks d2 < t_skin, with d2 = dot(p - x_s, p - x_s) evaluated as the
-// kernels do. on_exact(q, d2) is called for every candidate the exact mask
-// keeps, in the reference order (slots, then ascending sorted position).
-// Returns false if a stencil cell holds more than kMaskBits particles (the
-// masks of that cell are then truncated and must not be used, and on_exact
-// has seen only the first kMaskBits of it).
-template <class OnExact>
+// kernels do. The exact mask is also copied to `column` and its non-empty
+// slots are added to the bitmap `nonzero`, if given. Returns false if a
+// stencil cell holds more than kMaskBits particles (the masks of that cell are
+// then truncated and must not be used).
 __device__ __forceinline__ bool build_masks(const SortedGrid& g, const float4* __restrict__ xm, uint32_t n, uint32_t i,
                                             Float3 p, int cx, int cy, int cz, float t_exact, float t_skin,
                                             uint16_t* __restrict__ exact, uint16_t* __restrict__ skin,
-                                            OnExact&& on_exact) {
+                                            uint16_t* column = nullptr, uint32_t stride = 0u,
+                                            uint32_t* nonzero = nullptr) {
     bool complete = true;
     int k = 0;
     for (int ox = -1; ox <= 1; ++ox) {
@@ -183,14 +202,14 @@ __device__ __forceinline__ bool build_masks(const SortedGrid& g, const float4* _
                         const Float3 d = sub(p, Float3{q.x, q.y, q.z});
                         const float d2 = dot(d, d);
                         const uint32_t bit = 1u << (s - first);
-                        const bool keep = d2 < t_exact;
-                        me |= keep ? bit : 0u;
+                        me |= d2 < t_exact ? bit : 0u;
                         ms |= d2 < t_skin ? bit : 0u;
-                        if (keep) on_exact(q, d2);
                     }
                 }
                 exact[static_cast<size_t>(k) * n + i] = static_cast<uint16_t>(me);
                 skin[static_cast<size_t>(k) * n + i] = static_cast<uint16_t>(ms);
+                if (column != nullptr) column[static_cast<uint32_t>(k) * stride] = static_cast<uint16_t>(me);
+                if (nonzero != nullptr && me != 0u) *nonzero |= 1u << k;
             }
         }
     }

@@ -547,7 +547,7 @@ void SphSolver::forward_pressure(const phx_sph_state& state, const phx_control& 
         auto launch_iteration = [&](uint32_t it) {
             pa.slot = it - 1u;  // the prediction of the previous pass
             pa.cv = Convergence{tol_partials_.data(), done, rho0_.data()};
-            launch_projection(profiler_, stream_, blocks, pa, kSphProjection);
+            launch_projection(profiler_, stream_, blocks, pa, kSphProjection, kProjectDensity);
             converge_kernel<<<1, kConvergeBlock, 0, stream_>>>(tol_partials_.data(), blocks * kWaves, n, tolerance,
                                                                it, max_it, done);
             PHX_HIP(hipGetLastError());
@@ -581,7 +581,7 @@ void SphSolver::forward_pressure(const phx_sph_state& state, const phx_control& 
     launch_force<true, kInitPredict>(profiler_, stream_, blocks, fa, kSphForceInit);
     for (uint32_t it = 1; it <= iterations; ++it) {  // sph.cpp:300-307 / 413-420
         pa.slot = slot;
-        launch_projection(profiler_, stream_, blocks, pa, kSphProjection);
+        launch_projection(profiler_, stream_, blocks, pa, kSphProjection, kProjectDensity);
         fa.slot = ++slot;
         fa.final_step = it == iterations ? 1u : 0u;
         launch_force<false, kDivPredict>(profiler_, stream_, blocks, fa, kSphForce);

@@ -20,6 +20,8 @@ __device__ __forceinline__ Float3 xyz(float4 q) { return Float3{q.x, q.y, q.z}; 
 
 constexpr uint32_t kBlock = 256;
 constexpr uint32_t kWaves = kBlock / 64;  // waves of a pass workgroup
+// candidates a masked walk fetches ahead of the one it processes
+constexpr int kWalkDepth = 2;
 
 struct MaskDensityArgs {
     uint32_t n;
@@ -58,11 +60,17 @@ struct MaskDensityArgs {
 // crowded and walks the full stencil here and in every later pass.
 template <bool kEos>
 __global__ void __launch_bounds__(kBlock) mask_density_kernel(MaskDensityArgs a) {
+    __shared__ uint16_t smask[kStencil * kBlock];
+    __shared__ int delta[kStencil];
+    fill_slot_deltas(a.grid, delta);
+    __syncthreads();
     const uint32_t i = xcd_block() * kBlock + threadIdx.x;
     if (i >= a.n) return;
+    uint16_t* column = smask + threadIdx.x;
     const Float3 p = xyz(a.xm[i]);
     int cx, cy, cz;
     particle_cell(a.grid, p, cx, cy, cz);
+    uint32_t nonzero = 0u;
     float result = 0.0F;
     if (a.boundary.offsets != nullptr) {
         a.crowded[i] = 1u;  // no masks are built with boundary samples
@@ -77,14 +85,16 @@ __global__ void __launch_bounds__(kBlock) mask_density_kernel(MaskDensityArgs a)
                 const float4 b = a.boundary.x[s];
                 result += rest * b.w * cubic_value(pair_distance(sub(p, xyz(b))), a.kc);
             });
-    } else if (build_masks(a.grid, a.xm, a.n, i, p, cx, cy, cz, a.kc.sqrt_support, a.t_skin, a.exact, a.skin,
-                           [&](float4 q, float d2) { result += q.w * cubic_value(sqrt_rn(d2), a.kc); })) {
-        // the density over the exact mask, summed while building it (the
-        // candidates it leaves out add exact zeros in the reference)
+    } else if (build_masks(a.grid, a.xm, a.n, i, p, cx, cy, cz, a.kc.sqrt_support, a.t_skin, a.exact, a.skin, column,
+                           kBlock, &nonzero)) {
         a.crowded[i] = 0u;
+        walk_masked(a.grid.offsets, cell_of(a.grid, cx, cy, cz), delta, column, kBlock, nonzero, [&](uint32_t s) {
+            const float4 q = a.xm[s];
+            const float dist = pair_distance(sub(p, xyz(q)));
+            result += q.w * cubic_value(dist, a.kc);
+        });
     } else {
         a.crowded[i] = 1u;
-        result = 0.0F;  // the partial sum of the truncated masks is dropped
         for_each_candidate(a.grid, cx, cy, cz, [&](uint32_t s) {
             const float4 q = a.xm[s];
             result += q.w * cubic_value(pair_distance(sub(p, xyz(q))), a.kc);
@@ -186,12 +196,17 @@ struct ProjectionArgs {
     const uint8_t* crowded;     // per particle: walks the full stencil
 };
 
+// The pressure phase a projection serves. The kernel is the same for both;
+// the two instantiations only give profiles one kernel name per phase.
+enum ProjectionPhase : int { kProjectDivergence = 0, kProjectDensity = 1 };
+
 // density at predicted positions (density.cu:18-40 with topology = x,
 // positions = x*) fused with the pressure update and the next pressure
 // pass's first_term: projection_update_forward_kernel (dfsph.cu:104-110),
 // pressure_update_forward_kernel (pcisph.cu:104-110) or
 // jacobi_update_forward_kernel (iisph.cu:104-110), which differ only in the
 // density of the volume V = m / density.
+template <int kPhase>
 __global__ void __launch_bounds__(kBlock) projection_kernel(ProjectionArgs a) {
     __shared__ uint16_t smask[kStencil * kBlock];
     __shared__ int delta[kStencil];
@@ -216,7 +231,7 @@ __global__ void __launch_bounds__(kBlock) projection_kernel(ProjectionArgs a) {
         if (masked) {
             uint16_t* column = smask + threadIdx.x;
             const uint32_t nonzero = load_lane_column(a.skin, a.n, i, column, kBlock);
-            walk_masked_prefetch(
+            walk_masked_prefetch<kWalkDepth>(
                 a.grid.offsets, cell_of(a.grid, cx, cy, cz), delta, column, kBlock, nonzero, i,
                 [&](uint32_t s) { return a.xs[s]; },
                 [&](float4 q) { rs += q.w * cubic_value(pair_distance(sub(ps, xyz(q))), a.kc); });
@@ -369,7 +384,7 @@ __global__ void __launch_bounds__(kBlock) force_kernel(ForceArgs a) {
                     float4 q;
                     float t1, t2;
                 };
-                walk_masked_prefetch(
+                walk_masked_prefetch<kWalkDepth>(
                     a.grid.offsets, center, delta, column, kBlock, nonzero, i,
                     [&](uint32_t s) { return Candidate{a.xm[s], a.t1[s], kTwo ? a.t2[s] : 0.0F}; },
                     [&](const Candidate& c) {
@@ -449,9 +464,13 @@ void launch_force(PassProfiler& profiler, hipStream_t stream, uint32_t blocks, c
 }
 
 void launch_projection(PassProfiler& profiler, hipStream_t stream, uint32_t blocks, const ProjectionArgs& args,
-                       int pass) {
+                       int pass, ProjectionPhase phase) {
     PassScope scope(profiler, stream, pass);
-    projection_kernel<<<blocks, kBlock, 0, stream>>>(args);
+    if (phase == kProjectDivergence) {
+        projection_kernel<kProjectDivergence><<<blocks, kBlock, 0, stream>>>(args);
+    } else {
+        projection_kernel<kProjectDensity><<<blocks, kBlock, 0, stream>>>(args);
+    }
     PHX_LAUNCH_CHECK();
 }
 

@@ -349,6 +349,17 @@ def _stream_ordered(method):
     return ordered
 
 
+def _release_handle(owner) -> None:
+    """Frees owner._handle with the C entry point bound at creation
+    (owner._release), once. Used as the classes' __del__ itself, so it needs no
+    module global at interpreter exit, when those may already be cleared."""
+    handle = getattr(owner, "_handle", None)
+    release = getattr(owner, "_release", None)
+    if handle is not None and handle.value and release is not None:
+        owner._handle = None
+        release(handle)
+
+
 def _require_gpu(device: torch.device) -> None:
     if device.type != "cuda" or not torch.cuda.is_available():
         raise RuntimeError("physica_amd solvers run on the GPU only (no CPU fallback); no HIP device is available")
@@ -695,11 +706,10 @@ class SPH:
             handle = ctypes.c_void_p()
             _capi.call(self._prefix + "_allocate_step_cache", solver._handle, ctypes.byref(handle))
             self._handle = handle
+            # bound now: module globals may already be gone when __del__ runs at exit
+            self._release = getattr(_capi.library(), self._prefix + "_free_step_cache")
 
-        def __del__(self):
-            if getattr(self, "_handle", None) and self._handle.value:
-                getattr(_capi.library(), self._prefix + "_free_step_cache")(self._handle)
-                self._handle = ctypes.c_void_p()
+        __del__ = _release_handle
 
         def neighborhood(self):
             n = self._solver.domain.configuration.particle_count
@@ -759,11 +769,9 @@ class SPH:
             ctypes.byref(handle),
         )
         self._handle = handle
+        self._release = getattr(_capi.library(), self._prefix + "_destroy")
 
-    def __del__(self):
-        if getattr(self, "_handle", None) and self._handle.value:
-            getattr(_capi.library(), self._prefix + "_destroy")(self._handle)
-            self._handle = ctypes.c_void_p()
+    __del__ = _release_handle
 
     def allocate_state(self) -> "SPH.State":
         d = self.domain
@@ -1039,11 +1047,9 @@ class PBF:
             handle = ctypes.c_void_p()
             _capi.call("phx_pbf_allocate_step_cache", solver._handle, ctypes.byref(handle))
             self._handle = handle
+            self._release = _capi.library().phx_pbf_free_step_cache
 
-        def __del__(self):
-            if getattr(self, "_handle", None) and self._handle.value:
-                _capi.library().phx_pbf_free_step_cache(self._handle)
-                self._handle = ctypes.c_void_p()
+        __del__ = _release_handle
 
         def neighborhood(self):
             n = self._solver.domain.configuration.particle_count
@@ -1098,11 +1104,9 @@ class PBF:
             ctypes.byref(handle),
         )
         self._handle = handle
+        self._release = _capi.library().phx_pbf_destroy
 
-    def __del__(self):
-        if getattr(self, "_handle", None) and self._handle.value:
-            _capi.library().phx_pbf_destroy(self._handle)
-            self._handle = ctypes.c_void_p()
+    __del__ = _release_handle
 
     def allocate_state(self) -> "PBF.State":
         return PBF.State(self.domain.allocate_particle_state())
