@@ -290,13 +290,18 @@ struct BoundaryGrid {
 // neighborhood/device.cuh:32-42, skips cells outside the grid).
 template <class Cell>
 __device__ __forceinline__ void for_each_cell(const SortedGrid& g, int cx, int cy, int cz, Cell&& cell) {
+    // not unrolled: 27 inlined copies of a visitor cost registers (occupancy)
+    // and instruction cache for no gain, the visit loop being dynamic anyway
+#pragma unroll 1
     for (int ox = -1; ox <= 1; ++ox) {
         const int X = cx + ox;
         if (X < 0 || X >= static_cast<int>(g.cx)) continue;
+#pragma unroll 1
         for (int oy = -1; oy <= 1; ++oy) {
             const int Y = cy + oy;
             if (Y < 0 || Y >= static_cast<int>(g.cy)) continue;
             const uint32_t row = (static_cast<uint32_t>(Y)) * g.cx + static_cast<uint32_t>(X);
+#pragma unroll 1
             for (int oz = -1; oz <= 1; ++oz) {
                 const int Z = cz + oz;
                 if (Z < 0 || Z >= static_cast<int>(g.cz)) continue;

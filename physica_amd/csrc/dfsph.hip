@@ -21,7 +21,12 @@
 // masked-out candidate contributes an exact zero in the reference, so results
 // equal the reference expression by expression. An A pass uses the skin mask
 // only if no particle moved farther than half the skin (read on the device
-// from the preceding B pass); otherwise it scans the full stencil.
+// from the preceding B pass); otherwise it scans the full stencil. In forward
+// mode after the first step the skin is per cell instead: each cell's bound
+// comes from its particles' displacements in the previous step, a pair is
+// kept while it lies within h plus both cells' bounds, and a B pass marks
+// the 27 cells around every particle that moved past its cell's bound, whose
+// particles then scan the full stencil in the A pass that reads it.
 #include <cmath>
 #include <cstring>
 
@@ -119,7 +124,8 @@ DfsphSolver::DfsphSolver(const phx_domain_configuration* domain, const phx_dfsph
     gradient_norm_ = reference_gradient_norm(domain_.config);
     constants_ = kernel_constants(domain_.config.support_radius);
     const uint32_t n = domain_.config.particle_count;
-    for (DeviceBuffer<float4>* b : {&xm_, &vmu_, &xs_, &adiv_}) b->resize(n);
+    for (DeviceBuffer<float4>* b : {&xm_, &vmu_, &xs_, &adiv_, &xt_div_, &xt_den_}) b->resize(n);
+    mass_state_.resize(2u);
     u_.resize(n);
     for (DeviceBuffer<float>* b : {&rho0_, &relax_div_, &relax_den_, &p_div_, &p_den_, &t_div_, &t_den_}) b->resize(n);
     // one displacement slot per predicting pass: the first, then one per
@@ -127,6 +133,12 @@ DfsphSolver::DfsphSolver(const phx_domain_configuration* domain, const phx_dfsph
     const uint32_t predictions = tolerance > 0.0F ? 2u * config_.max_iterations + 1u
                                                   : config_.divergence_iterations + config_.density_iterations + 1u;
     masks_.allocate(n, predictions, domain_.config.support_radius);
+    if (mode == PHX_MODE_FORWARD) {
+        for (DeviceBuffer<float>* b : {&moved_, &moved_sorted_}) b->resize(n);
+        cell_bound_.resize(domain_.cells);
+        taint_.resize(domain_.cells);
+        fill_u32(stream_, taint_.data(), domain_.cells, 0u);
+    }
     if (tolerance > 0.0F) {
         tol_partials_.resize(static_cast<size_t>(pass_blocks(n)) * kWaves);  // one per projection wave
         tol_done_.resize(2u);
@@ -204,6 +216,16 @@ void DfsphSolver::forward_step(const phx_sph_dfsph_state& state, const phx_contr
         gather.add(state.method.warm_divergence_pressure, p_div0);
         gather.add(state.method.warm_density_pressure, p_den0);
     }
+    // per-cell skins need the previous step's displacements
+    const bool cell_skins = !rec && have_moved_ && boundary_.count() == 0u;
+    if (cell_skins) gather.add(moved_.data(), moved_sorted_.data());
+    // stamps of this step's predicting passes: stamp_base_ + slot
+    const uint32_t predictions = config_.density_error_tolerance > 0.0F ? 2u * config_.max_iterations + 1u
+                                                                         : nd + ne + 1u;
+    if (cell_skins && stamp_base_ > 0xFFFFFFFFu - predictions) {
+        fill_u32(stream_, taint_.data(), domain_.cells, 0u);
+        stamp_base_ = 1u;
+    }
     {
         PassScope scope(profiler_, stream_, kDfsphGrid);
         cache.neighborhood.build(stream_, domain_, step, cvec(state.particles.positions), nullptr, gather.get(),
@@ -214,6 +236,20 @@ void DfsphSolver::forward_step(const phx_sph_dfsph_state& state, const phx_contr
         }
         // boundary samples: every pass walks the full stencil
         masks_.begin_step(stream_, boundary_.count() > 0u);
+        fill_u32(stream_, mass_state_.data(), 1u, 1u);
+        mass_state_kernel<<<blocks_for(n, kBlock), kBlock, 0, stream_>>>(n, xm, mass_state_.data());
+        PHX_LAUNCH_CHECK();
+        if (cell_skins) {
+            // 1.2 x the previous step's largest displacement in the cell plus
+            // 0.01 h: on the dam break at step 100 this keeps 1.28 x the
+            // exact neighbours of the next predictions with no particle
+            // past its bound, where the global skin keeps 2.39 x
+            const uint32_t cells = domain_.cells;
+            cell_bound_kernel<<<blocks_for(cells, kBlock), kBlock, 0, stream_>>>(
+                cells, cache.neighborhood.grid().offsets, moved_sorted_.data(), 1.2F, 0.01F * c.support_radius,
+                cell_bound_.data());
+            PHX_LAUNCH_CHECK();
+        }
     }
     const SortedGrid grid = cache.neighborhood.grid();
     const BoundaryGrid boundary = cache.neighborhood.boundary();
@@ -235,6 +271,9 @@ void DfsphSolver::forward_step(const phx_sph_dfsph_state& state, const phx_contr
                        masks_.flags(),
                        rho0};
     ma.crowded = crowded;
+    if (cell_skins) ma.cell_bound = cell_bound_.data();
+    ma.xt_div = xt_div_.data();
+    ma.xt_den = xt_den_.data();
     launch_mask_density<false>(profiler_, stream_, blocks, ma, kDfsphDensity);
 
     ForceArgs fa{};
@@ -262,9 +301,16 @@ void DfsphSolver::forward_step(const phx_sph_dfsph_state& state, const phx_contr
     fa.boundary = boundary;
     fa.rest_density = rho0;
     fa.crowded = crowded;
+    if (cell_skins) {
+        fa.cell_bound = cell_bound_.data();
+        fa.taint = taint_.data();
+    }
+    if (!rec) fa.moved_out = moved_.data();
+    fa.mass_state = mass_state_.data();
     // prediction `slot` writes x* (recorded per slot) and, recorded, its collision bits
     auto set_slot = [&](uint32_t slot) {
         fa.slot = slot;
+        fa.stamp = stamp_base_ + slot;
         fa.xs = rec ? cache.xs.data() + slot * nn : xs_.data();
         fa.collisions = rec ? cache.collisions.data() + slot * nn : nullptr;
     };
@@ -284,9 +330,11 @@ void DfsphSolver::forward_step(const phx_sph_dfsph_state& state, const phx_contr
     pa.boundary = boundary;
     pa.rest_density = rho0;
     pa.crowded = crowded;
+    if (cell_skins) pa.taint = taint_.data();
     // update `it` of phase `phase` reads the x* of prediction `slot`
     auto set_update = [&](uint32_t phase, uint32_t it, uint32_t slot) {
         pa.slot = slot;
+        pa.stamp = stamp_base_ + slot;
         pa.xs = rec ? cache.xs.data() + slot * nn : xs_.data();
         pa.rs_out = rec ? cache.rs.data() + slot * nn : nullptr;
         float* base = phase == 0u ? p_div_.data() : p_den_.data();
@@ -311,6 +359,7 @@ void DfsphSolver::forward_step(const phx_sph_dfsph_state& state, const phx_contr
         fill_u32(stream_, tol_done_.data(), 2u, 0u);
         // non-pressure + the warm divergence pressure + the first prediction
         fa.t1 = t_div_.data();
+        fa.xt1 = xt_div_.data();
         fa.t2 = t_den_.data();
         fa.final_step = 0u;
         set_slot(slot);
@@ -322,11 +371,14 @@ void DfsphSolver::forward_step(const phx_sph_dfsph_state& state, const phx_contr
                 pa.target = cache.densities.data();
                 pa.relaxation = relax_div;
                 pa.t = t_div_.data();
+                pa.xt_out = xt_div_.data();
             } else {
                 pa.target = rho0;
                 pa.relaxation = relax_den;
                 pa.t = t_den_.data();
+                pa.xt_out = xt_den_.data();
                 fa.t1 = t_den_.data();
+                fa.xt1 = xt_den_.data();
                 fa.t2 = nullptr;
                 fa.warm_div_out = nullptr;
             }
@@ -369,6 +421,8 @@ void DfsphSolver::forward_step(const phx_sph_dfsph_state& state, const phx_contr
         pa.cv = Convergence{};
         fa.done = nullptr;
         masks_.end_step(stream_);
+        if (cell_skins) stamp_base_ += predictions;
+        have_moved_ = true;  // the final pass wrote every particle's displacement
         next.particles.step_index = step + 1u;
         last_iterations_[0] = executed[0];
         last_iterations_[1] = executed[1];
@@ -377,6 +431,7 @@ void DfsphSolver::forward_step(const phx_sph_dfsph_state& state, const phx_contr
 
     // divergence phase: initial pressure with the warm p_div (sph.cpp:574-578)
     fa.t1 = t_div_.data();
+    fa.xt1 = xt_div_.data();
     fa.t2 = t_den_.data();
     set_slot(slot);
     if (nd == 0u) {
@@ -388,6 +443,7 @@ void DfsphSolver::forward_step(const phx_sph_dfsph_state& state, const phx_contr
         pa.target = cache.densities.data();
         pa.relaxation = relax_div;
         pa.t = t_div_.data();
+        pa.xt_out = xt_div_.data();
         for (uint32_t it = 1; it <= nd; ++it) {  // sph.cpp:580-587
             set_update(0u, it, slot);
             launch_projection(profiler_, stream_, blocks, pa, kDfsphProjectionDivergence, kProjectDivergence);
@@ -404,7 +460,9 @@ void DfsphSolver::forward_step(const phx_sph_dfsph_state& state, const phx_contr
     pa.target = rho0;
     pa.relaxation = relax_den;
     pa.t = t_den_.data();
+    pa.xt_out = xt_den_.data();
     fa.t1 = t_den_.data();
+    fa.xt1 = xt_den_.data();
     fa.t2 = nullptr;
     fa.warm_div_out = nullptr;
     for (uint32_t it = 1; it <= ne; ++it) {
@@ -425,6 +483,8 @@ void DfsphSolver::forward_step(const phx_sph_dfsph_state& state, const phx_contr
     }
     // read the step's displacements back for the next step's skin
     masks_.end_step(stream_);
+    if (cell_skins) stamp_base_ += predictions;
+    if (!rec) have_moved_ = true;  // the final pass wrote every particle's displacement
     next.particles.step_index = step + 1u;
     last_iterations_[0] = nd;
     last_iterations_[1] = ne;

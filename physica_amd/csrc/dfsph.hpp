@@ -142,6 +142,18 @@ class DfsphSolver {
     VecBuffer u_;
     DeviceBuffer<float> rho0_, relax_div_, relax_den_, p_div_, p_den_, t_div_, t_den_;
     MaskSet masks_;
+    // Per-cell skins (forward mode, no boundary samples): each particle's
+    // displacement in the last forward step (original order), its sorted
+    // copy, the step's per-cell bounds, and the cells' taint stamps (one
+    // stamp per predicting pass, never reused until the counter wraps).
+    DeviceBuffer<float> moved_, moved_sorted_, cell_bound_;
+    DeviceBuffer<uint32_t> taint_;
+    bool have_moved_ = false;
+    uint32_t stamp_base_ = 1u;
+    // the force passes' candidate records (x, y, z, t) of each phase and the
+    // step's mass state (all masses equal, their bits)
+    DeviceBuffer<float4> xt_div_, xt_den_;
+    DeviceBuffer<uint32_t> mass_state_;
     BoundarySamples boundary_;  // the domain's boundary samples on the device
     DfsphDerivativeWorkspace dw_;
 };

@@ -177,13 +177,20 @@ __device__ __forceinline__ bool build_masks(const SortedGrid& g, const float4* _
                                             Float3 p, int cx, int cy, int cz, float t_exact, float t_skin,
                                             uint16_t* __restrict__ exact, uint16_t* __restrict__ skin,
                                             uint16_t* column = nullptr, uint32_t stride = 0u,
-                                            uint32_t* nonzero = nullptr) {
+                                            uint32_t* nonzero = nullptr, const float* cell_bound = nullptr,
+                                            float h = 0.0F) {
     bool complete = true;
     int k = 0;
+    const float bound_i = cell_bound != nullptr ? cell_bound[cell_of(g, cx, cy, cz)] : 0.0F;
+    // not unrolled: the unrolled 27 cells take 95 VGPRs (5 waves per SIMD)
+    // and the pass 20 % longer than the loops at 39 VGPRs
+#pragma unroll 1
     for (int ox = -1; ox <= 1; ++ox) {
         const int X = cx + ox;
+#pragma unroll 1
         for (int oy = -1; oy <= 1; ++oy) {
             const int Y = cy + oy;
+#pragma unroll 1
             for (int oz = -1; oz <= 1; ++oz, ++k) {
                 const int Z = cz + oz;
                 uint32_t me = 0u, ms = 0u;
@@ -197,13 +204,21 @@ __device__ __forceinline__ bool build_masks(const SortedGrid& g, const float4* _
                         complete = false;
                         last = first + kMaskBits;
                     }
+                    // with per-cell bounds: a pair within h after both moved
+                    // by at most their cells' bounds was within their sum
+                    // plus h before (the relative margin covers rounding)
+                    float t_cell = t_skin;
+                    if (cell_bound != nullptr) {
+                        const float radius = h + bound_i + cell_bound[c];
+                        t_cell = radius * radius * (1.0F + 2e-5F);
+                    }
                     for (uint32_t s = first; s < last; ++s) {
                         const float4 q = xm[s];
                         const Float3 d = sub(p, Float3{q.x, q.y, q.z});
                         const float d2 = dot(d, d);
                         const uint32_t bit = 1u << (s - first);
                         me |= d2 < t_exact ? bit : 0u;
-                        ms |= d2 < t_skin ? bit : 0u;
+                        ms |= d2 < t_cell ? bit : 0u;
                     }
                 }
                 exact[static_cast<size_t>(k) * n + i] = static_cast<uint16_t>(me);
@@ -290,10 +305,13 @@ __device__ __forceinline__ bool visit_and_mask(const SortedGrid& g, const MaskVi
         }
     } else {
         uint32_t k = 0u;
+#pragma unroll 1
         for (int ox = -1; ox <= 1; ++ox) {
             const int X = cx + ox;
+#pragma unroll 1
             for (int oy = -1; oy <= 1; ++oy) {
                 const int Y = cy + oy;
+#pragma unroll 1
                 for (int oz = -1; oz <= 1; ++oz, ++k) {
                     const int Z = cz + oz;
                     if (X < 0 || X >= static_cast<int>(g.cx) || Y < 0 || Y >= static_cast<int>(g.cy) || Z < 0 ||
@@ -328,10 +346,13 @@ __device__ __forceinline__ bool build_skin_mask(const SortedGrid& g, CVec pos, u
                                                 int cx, int cy, int cz, float t, uint16_t* __restrict__ masks) {
     bool complete = true;
     int k = 0;
+#pragma unroll 1
     for (int ox = -1; ox <= 1; ++ox) {
         const int X = cx + ox;
+#pragma unroll 1
         for (int oy = -1; oy <= 1; ++oy) {
             const int Y = cy + oy;
+#pragma unroll 1
             for (int oz = -1; oz <= 1; ++oz, ++k) {
                 const int Z = cz + oz;
                 uint32_t bits = 0u;

@@ -219,6 +219,14 @@ SphSolver::SphSolver(const phx_domain_configuration* domain, const phx_sph_confi
     // one displacement slot per predicting pass
     const uint32_t predictions = (tolerance > 0.0F ? config_.max_iterations : config_.pressure_iterations) + 1u;
     masks_.allocate(n, predictions, domain_.config.support_radius);
+    if (mode == PHX_MODE_FORWARD && config_.method != PHX_SPH_WCSPH) {
+        xt_.resize(n);
+        mass_state_.resize(2u);
+        for (DeviceBuffer<float>* b : {&moved_, &moved_sorted_}) b->resize(n);
+        cell_bound_.resize(domain_.cells);
+        taint_.resize(domain_.cells);
+        fill_u32(stream_, taint_.data(), domain_.cells, 0u);
+    }
     if (tolerance > 0.0F) {
         tol_partials_.resize(static_cast<size_t>(pass_blocks(n)) * kWaves);  // one per projection wave
         tol_done_.resize(1u);
@@ -459,12 +467,31 @@ void SphSolver::forward_pressure(const phx_sph_state& state, const phx_control& 
     gather.add(pp.rest_densities, rho0_.data());
     gather.add(relaxation, relax_.data());
     const bool with_boundary = boundary_.count() > 0u;
+    // per-cell skins (DfsphSolver::forward_step) from the previous step's displacements
+    const bool cell_skins = have_moved_ && !with_boundary;
+    if (cell_skins) gather.add(moved_.data(), moved_sorted_.data());
+    const uint32_t predictions =
+        (config_.density_error_tolerance > 0.0F ? config_.max_iterations : config_.pressure_iterations) + 1u;
+    if (cell_skins && stamp_base_ > 0xFFFFFFFFu - predictions) {
+        fill_u32(stream_, taint_.data(), domain_.cells, 0u);
+        stamp_base_ = 1u;
+    }
     {
         PassScope scope(profiler_, stream_, kSphGrid);
         cache.neighborhood.build(stream_, domain_, step, cvec(state.particles.positions), nullptr, gather.get(),
                                  &boundary_);
         masks_.begin_step(stream_, with_boundary);
         fill_f32(stream_, cache.pressures.data(), n, 0.0F);  // clear_iteration (sph.cpp:298)
+        fill_u32(stream_, mass_state_.data(), 1u, 1u);
+        mass_state_kernel<<<blocks_for(n, kBlock), kBlock, 0, stream_>>>(n, xm_.data(), mass_state_.data());
+        PHX_LAUNCH_CHECK();
+        if (cell_skins) {
+            const uint32_t cells = domain_.cells;
+            cell_bound_kernel<<<blocks_for(cells, kBlock), kBlock, 0, stream_>>>(
+                cells, cache.neighborhood.grid().offsets, moved_sorted_.data(), 1.2F, 0.01F * c.support_radius,
+                cell_bound_.data());
+            PHX_LAUNCH_CHECK();
+        }
     }
     const SortedGrid grid = cache.neighborhood.grid();
     const BoundaryGrid boundary = cache.neighborhood.boundary();
@@ -483,6 +510,7 @@ void SphSolver::forward_pressure(const phx_sph_state& state, const phx_control& 
     ma.rho = cache.densities.data();
     ma.flags = masks_.flags();
     ma.crowded = masks_.crowded();
+    if (cell_skins) ma.cell_bound = cell_bound_.data();
     launch_mask_density<false>(profiler_, stream_, blocks, ma, kSphDensity);
 
     ForceArgs fa{};
@@ -507,6 +535,13 @@ void SphSolver::forward_pressure(const phx_sph_state& state, const phx_control& 
     fa.boundary = boundary;
     fa.rest_density = rho0_.data();
     fa.crowded = masks_.crowded();
+    fa.moved_out = moved_.data();
+    fa.xt1 = xt_.data();
+    fa.mass_state = mass_state_.data();
+    if (cell_skins) {
+        fa.cell_bound = cell_bound_.data();
+        fa.taint = taint_.data();
+    }
 
     ProjectionArgs pa{};
     pa.n = n;
@@ -529,6 +564,22 @@ void SphSolver::forward_pressure(const phx_sph_state& state, const phx_control& 
     pa.boundary = boundary;
     pa.rest_density = rho0_.data();
     pa.crowded = masks_.crowded();
+    pa.xt_out = xt_.data();
+    if (cell_skins) pa.taint = taint_.data();
+    // the stamps of this step's predictions: stamp_base_ + slot
+    auto set_force_slot = [&](uint32_t slot) {
+        fa.slot = slot;
+        fa.stamp = stamp_base_ + slot;
+    };
+    auto set_projection_slot = [&](uint32_t slot) {
+        pa.slot = slot;
+        pa.stamp = stamp_base_ + slot;
+    };
+    auto end_step = [&] {
+        masks_.end_step(stream_);
+        if (cell_skins) stamp_base_ += predictions;
+        have_moved_ = true;  // the final pass wrote every particle's displacement
+    };
 
     if (config_.density_error_tolerance > 0.0F) {
         // Tolerance mode: iterate until the projection's mean compression
@@ -541,11 +592,11 @@ void SphSolver::forward_pressure(const phx_sph_state& state, const phx_control& 
         uint32_t* done = tol_done_.data();
         auto* host = static_cast<uint32_t*>(tol_host_.data());
         fill_u32(stream_, done, 1u, 0u);
-        fa.slot = 0u;
+        set_force_slot(0u);
         fa.final_step = 0u;
         launch_force<true, kInitPredict>(profiler_, stream_, blocks, fa, kSphForceInit);
         auto launch_iteration = [&](uint32_t it) {
-            pa.slot = it - 1u;  // the prediction of the previous pass
+            set_projection_slot(it - 1u);  // the prediction of the previous pass
             pa.cv = Convergence{tol_partials_.data(), done, rho0_.data()};
             launch_projection(profiler_, stream_, blocks, pa, kSphProjection, kProjectDensity);
             converge_kernel<<<1, kConvergeBlock, 0, stream_>>>(tol_partials_.data(), blocks * kWaves, n, tolerance,
@@ -553,7 +604,7 @@ void SphSolver::forward_pressure(const phx_sph_state& state, const phx_control& 
             PHX_HIP(hipGetLastError());
             PHX_HIP(hipMemcpyAsync(host + (it & 1u), done, sizeof(uint32_t), hipMemcpyDeviceToHost, stream_));
             PHX_HIP(hipEventRecord(tol_events_[it & 1u], stream_));
-            fa.slot = it;
+            set_force_slot(it);
             fa.done = done;
             fa.iteration = it;
             fa.final_step = 0u;
@@ -570,23 +621,23 @@ void SphSolver::forward_pressure(const phx_sph_state& state, const phx_control& 
         last_iterations_ = it;
         pa.cv = Convergence{};
         fa.done = nullptr;
-        masks_.end_step(stream_);
+        end_step();
         return;
     }
     const uint32_t iterations = config_.pressure_iterations;
     last_iterations_ = iterations;
     uint32_t slot = 0;  // displacement slot written by the latest predicting pass
-    fa.slot = slot;
+    set_force_slot(slot);
     fa.final_step = iterations == 0u ? 1u : 0u;
     launch_force<true, kInitPredict>(profiler_, stream_, blocks, fa, kSphForceInit);
     for (uint32_t it = 1; it <= iterations; ++it) {  // sph.cpp:300-307 / 413-420
-        pa.slot = slot;
+        set_projection_slot(slot);
         launch_projection(profiler_, stream_, blocks, pa, kSphProjection, kProjectDensity);
-        fa.slot = ++slot;
+        set_force_slot(++slot);
         fa.final_step = it == iterations ? 1u : 0u;
         launch_force<false, kDivPredict>(profiler_, stream_, blocks, fa, kSphForce);
     }
-    masks_.end_step(stream_);
+    end_step();
 }
 
 }  // namespace phx

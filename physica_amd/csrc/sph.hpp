@@ -130,6 +130,16 @@ class SphSolver {
     KernelConstants constants_{};
     PassProfiler profiler_;
     MaskSet masks_;
+    // per-cell skins of the forward PCISPH/IISPH step (as DfsphSolver's):
+    // last displacements (original order), their sorted copy, the per-cell
+    // bounds and the cells' taint stamps
+    DeviceBuffer<float> moved_, moved_sorted_, cell_bound_;
+    DeviceBuffer<uint32_t> taint_;
+    bool have_moved_ = false;
+    uint32_t stamp_base_ = 1u;
+    // the force pass's candidate records (x, y, z, t) and the step's mass state
+    DeviceBuffer<float4> xt_;
+    DeviceBuffer<uint32_t> mass_state_;
     // sorted-order workspace
     DeviceBuffer<float4> xm_, vmu_, xs_, aux_;
     VecBuffer u_;

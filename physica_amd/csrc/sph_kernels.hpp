@@ -23,6 +23,31 @@ constexpr uint32_t kWaves = kBlock / 64;  // waves of a pass workgroup
 // candidates a masked walk fetches ahead of the one it processes
 constexpr int kWalkDepth = 2;
 
+// Marks the 27 cells around (cx, cy, cz) with `stamp` (plain stores: every
+// writer writes the same value).
+__device__ __forceinline__ void taint_neighbour_cells(const SortedGrid& g, int cx, int cy, int cz, uint32_t* taint,
+                                                      uint32_t stamp) {
+    for (int z = max(cz - 1, 0); z <= min(cz + 1, static_cast<int>(g.cz) - 1); ++z)
+        for (int y = max(cy - 1, 0); y <= min(cy + 1, static_cast<int>(g.cy) - 1); ++y)
+            for (int x = max(cx - 1, 0); x <= min(cx + 1, static_cast<int>(g.cx) - 1); ++x)
+                taint[cell_of(g, x, y, z)] = stamp;
+}
+
+// Per-cell displacement bound of a step: scale * the largest displacement
+// of the previous step among the cell's particles (sorted order), plus
+// `allowance`; 0 for an empty cell.
+[[maybe_unused]] __global__ void __launch_bounds__(kBlock)
+    cell_bound_kernel(uint32_t cells, const uint32_t* __restrict__ offsets, const float* __restrict__ moved,
+                      float scale, float allowance, float* __restrict__ bound) {
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= cells) return;
+    const uint32_t first = offsets[c];
+    const uint32_t last = offsets[c + 1u];
+    float largest = 0.0F;
+    for (uint32_t s = first; s < last; ++s) largest = fmaxf(largest, moved[s]);
+    bound[c] = first == last ? 0.0F : scale * largest + allowance;
+}
+
 struct MaskDensityArgs {
     uint32_t n;
     KernelConstants kc;
@@ -47,6 +72,14 @@ struct MaskDensityArgs {
     // per particle: set where a stencil cell holds more than kMaskBits
     // particles (that particle walks the full stencil in every pass)
     uint8_t* crowded;
+    // per cell: a bound on how far its particles move this step; non-null:
+    // the skin mask keeps a pair while d2 < (h + B_ci + B_cj)^2 (plus a
+    // rounding margin) instead of d2 < t_skin
+    const float* cell_bound = nullptr;
+    // optional: (x, y, z, t_div) and (x, y, z, t_den) per particle, the
+    // candidate records of the force passes that read these t
+    float4* xt_div = nullptr;
+    float4* xt_den = nullptr;
 };
 
 // Mask build, then density_forward_kernel (density.cu:18-40, cubic kernel,
@@ -86,7 +119,7 @@ __global__ void __launch_bounds__(kBlock) mask_density_kernel(MaskDensityArgs a)
                 result += rest * b.w * cubic_value(pair_distance(sub(p, xyz(b))), a.kc);
             });
     } else if (build_masks(a.grid, a.xm, a.n, i, p, cx, cy, cz, a.kc.sqrt_support, a.t_skin, a.exact, a.skin, column,
-                           kBlock, &nonzero)) {
+                           kBlock, &nonzero, a.cell_bound, a.kc.h)) {
         a.crowded[i] = 0u;
         walk_masked(a.grid.offsets, cell_of(a.grid, cx, cy, cz), delta, column, kBlock, nonzero, [&](uint32_t s) {
             const float4 q = a.xm[s];
@@ -111,8 +144,16 @@ __global__ void __launch_bounds__(kBlock) mask_density_kernel(MaskDensityArgs a)
         a.pressures[i] = pressure;
         a.aux[i] = make_float4(result, pressure / (result * result), speed, aux.w);
     } else {
-        if (a.p_div != nullptr) a.t_div[i] = a.p_div[i] / (result * result);
-        if (a.p_den != nullptr) a.t_den[i] = a.p_den[i] / (result * result);
+        if (a.p_div != nullptr) {
+            const float t = a.p_div[i] / (result * result);
+            a.t_div[i] = t;
+            if (a.xt_div != nullptr) a.xt_div[i] = make_float4(p.x, p.y, p.z, t);
+        }
+        if (a.p_den != nullptr) {
+            const float t = a.p_den[i] / (result * result);
+            a.t_den[i] = t;
+            if (a.xt_den != nullptr) a.xt_den[i] = make_float4(p.x, p.y, p.z, t);
+        }
     }
 }
 
@@ -194,6 +235,13 @@ struct ProjectionArgs {
     BoundaryGrid boundary;     // offsets null: no boundary samples
     const float* rest_density;  // rho0, for the boundary terms
     const uint8_t* crowded;     // per particle: walks the full stencil
+    // per-cell skins: the skin mask applies to particle i unless its cell
+    // holds `stamp` (a particle of a neighbouring cell moved past its cell's
+    // bound in the prediction this pass reads); null: the global rule
+    // (every particle moved at most sqrt(limit2))
+    const uint32_t* taint = nullptr;
+    uint32_t stamp = 0u;
+    float4* xt_out = nullptr;  // optional: (x, y, z, t) per particle, the force pass's candidate record
 };
 
 // The pressure phase a projection serves. The kernel is the same for both;
@@ -217,12 +265,16 @@ __global__ void __launch_bounds__(kBlock) projection_kernel(ProjectionArgs a) {
     const uint32_t i = xcd_block() * kBlock + threadIdx.x;
     double error = 0.0;
     if (i < a.n) {
-        const bool masked = a.flags[kFlagOverflow] == 0u && a.boundary.offsets == nullptr && a.crowded[i] == 0u &&
-                            __uint_as_float(a.flags[kFlagDisplacement + a.slot]) <= a.limit2;
+        const float4 topology = a.xm[i];
+        int cx, cy, cz;
+        particle_cell(a.grid, xyz(topology), cx, cy, cz);
+        const bool moved_within = a.taint != nullptr
+                                      ? a.taint[cell_of(a.grid, cx, cy, cz)] != a.stamp
+                                      : __uint_as_float(a.flags[kFlagDisplacement + a.slot]) <= a.limit2;
+        const bool masked =
+            a.flags[kFlagOverflow] == 0u && a.boundary.offsets == nullptr && a.crowded[i] == 0u && moved_within;
         const float4 self = a.xs[i];
         const Float3 ps = xyz(self);
-        int cx, cy, cz;
-        particle_cell(a.grid, xyz(a.xm[i]), cx, cy, cz);
         float rs = 0.0F;
         auto visit = [&](uint32_t s) {
             const float4 q = a.xs[s];
@@ -251,7 +303,9 @@ __global__ void __launch_bounds__(kBlock) projection_kernel(ProjectionArgs a) {
         const float delta_p = 1.0F / (2.0F * a.dt * a.dt * volume * volume * a.gradient_norm);
         const float pressure = fmaxf(0.0F, a.p_prev[i] + a.relaxation[i] * delta_p * (rs - target));
         a.p[i] = pressure;
-        a.t[i] = pressure / (rho * rho);
+        const float t = pressure / (rho * rho);
+        a.t[i] = t;
+        if (a.xt_out != nullptr) a.xt_out[i] = make_float4(topology.x, topology.y, topology.z, t);
         if (a.rs_out != nullptr) a.rs_out[i] = rs;
         if (tolerance)
             error = fmax(0.0, static_cast<double>(rs) - static_cast<double>(target)) /
@@ -307,7 +361,30 @@ struct ForceArgs {
     BoundaryGrid boundary;      // offsets null: no boundary samples
     const float* rest_density;  // rho0, for the boundary pressure terms
     const uint8_t* crowded;     // per particle: walks the full stencil
+    // per-cell skins: a predicting pass marks the 27 cells around every
+    // particle that moved farther than its cell's bound with `stamp`
+    const float* cell_bound = nullptr;
+    uint32_t* taint = nullptr;
+    uint32_t stamp = 0u;
+    // optional: |x_next - x| per particle (original order), written with the next state
+    float* moved_out = nullptr;
+    // optional: (x, y, z, t1) per particle and the step's mass state {all
+    // masses equal (1/0), their value's bits}: with equal masses a candidate
+    // is one 16-byte record instead of its position, mass and t1
+    const float4* xt1 = nullptr;
+    const uint32_t* mass_state = nullptr;
 };
+
+// mass_state of a step: [0] = 1 while every sorted particle's mass has the
+// bits of particle 0's, [1] = those bits. [0] is set to 1 beforehand.
+[[maybe_unused]] __global__ void __launch_bounds__(kBlock)
+    mass_state_kernel(uint32_t n, const float4* __restrict__ xm, uint32_t* state) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t bits = __float_as_uint(xm[i].w);
+    if (i == 0u) state[1] = bits;
+    if (bits != __float_as_uint(xm[0].w)) state[0] = 0u;
+}
 
 // Pressure acceleration (pressure_forward_kernel, common.cu:165-194),
 // optionally fused with the non-pressure acceleration
@@ -352,6 +429,9 @@ __global__ void __launch_bounds__(kBlock) force_kernel(ForceArgs a) {
         Float3 ap2{0.0F, 0.0F, 0.0F};
         int cx, cy, cz;
         particle_cell(a.grid, p, cx, cy, cz);
+        // candidates as (x, y, z, t1) records with one mass (uniform over the launch)
+        const bool packed = kPressure && a.xt1 != nullptr && a.mass_state[0] != 0u;
+        const float mass = packed ? __uint_as_float(a.mass_state[1]) : 0.0F;
         auto pair = [&](uint32_t s, float4 q, float t1_s, float t2_s, Float3 d, float f) {
             const float ms = q.w;
             if constexpr (kNonPressure) {
@@ -371,7 +451,31 @@ __global__ void __launch_bounds__(kBlock) force_kernel(ForceArgs a) {
             const uint32_t center = cell_of(a.grid, cx, cy, cz);
             uint32_t nonzero = load_lane_column(a.exact, a.n, i, column, kBlock);
             nonzero = drop_self(i, a.grid.offsets[center], column, kBlock, nonzero);
-            if constexpr (kNonPressure) {
+            if (packed) {
+                // equal masses: (x, y, z, t1) is the whole candidate but t2
+                if constexpr (kNonPressure) {
+                    walk_masked(a.grid.offsets, center, delta, column, kBlock, nonzero, [&](uint32_t s) {
+                        const float4 r = a.xt1[s];
+                        const float t2_s = kTwo ? a.t2[s] : 0.0F;
+                        const Float3 d = sub(p, xyz(r));
+                        pair(s, make_float4(r.x, r.y, r.z, mass), r.w, t2_s, d,
+                             kPressure ? cubic_gradient_factor(pair_distance(d), a.kc) : 0.0F);
+                    });
+                } else {
+                    struct Record {
+                        float4 r;
+                        float t2;
+                    };
+                    walk_masked_prefetch<kWalkDepth>(
+                        a.grid.offsets, center, delta, column, kBlock, nonzero, i,
+                        [&](uint32_t s) { return Record{a.xt1[s], kTwo ? a.t2[s] : 0.0F}; },
+                        [&](const Record& c) {
+                            const Float3 d = sub(p, xyz(c.r));
+                            pair(0u, make_float4(c.r.x, c.r.y, c.r.z, mass), c.r.w, c.t2, d,
+                                 cubic_gradient_factor(pair_distance(d), a.kc));
+                        });
+                }
+            } else if constexpr (kNonPressure) {
                 walk_masked(a.grid.offsets, center, delta, column, kBlock, nonzero, [&](uint32_t s) {
                     const float4 q = a.xm[s];
                     const float t1_s = kPressure ? a.t1[s] : 0.0F;
@@ -442,15 +546,20 @@ __global__ void __launch_bounds__(kBlock) force_kernel(ForceArgs a) {
         if (a.collisions != nullptr)
             a.collisions[i] = static_cast<uint8_t>(predict_collisions(a.box, a.dt, p, vel, a_total));
         if (kTwo && a.warm_div_out != nullptr) a.warm_div_out[a.perm[i]] = a.p_div[i];
+        const Float3 moved = sub(nx, p);
         if (final_step != 0u) {
             const uint32_t j = a.perm[i];
             store(a.next_x, j, nx);
             store(a.next_v, j, nv);
             if (a.warm_den_out != nullptr) a.warm_den_out[j] = a.p_den[i];
+            if (a.moved_out != nullptr) a.moved_out[j] = sqrtf(dot(moved, moved));
         } else {
             a.xs[i] = make_float4(nx.x, nx.y, nx.z, self.w);
-            const Float3 moved = sub(nx, p);
             moved2 = dot(moved, moved);
+            if (a.taint != nullptr) {
+                const float bound = a.cell_bound[cell_of(a.grid, cx, cy, cz)];
+                if (moved2 > bound * bound) taint_neighbour_cells(a.grid, cx, cy, cz, a.taint, a.stamp);
+            }
         }
     }
     if (final_step == 0u) block_max_to<kBlock>(moved2, &a.flags[kFlagDisplacement + a.slot]);

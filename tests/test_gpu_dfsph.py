@@ -68,6 +68,52 @@ def test_rollout_bit_exact(gpu_device):
         solver.copy_state(nxt, state)
 
 
+@pytest.mark.parametrize("tolerance", [0.0, 1e-3])
+def test_per_cell_skins_stay_exact_when_particles_outrun_their_bounds(gpu_device, tolerance):
+    """From its second forward step on, the solver bounds each cell's
+    displacement by its particles' previous displacements and keeps only the
+    pairs within h plus both bounds in the skin mask. A predicting pass marks
+    the cells around every particle that moved past its bound, and particles
+    there walk the full stencil. Step 1 starts at rest (tiny bounds); step 2
+    gets a state in which a seeded tenth of the particles suddenly move at
+    up to 3 m/s (far past their bounds) and the rest keep their velocities.
+    Both steps must equal the FP32 oracle bit for bit."""
+    print(f"dfsph per-cell skins with outrunning particles tolerance={tolerance}", flush=True)
+    sc = scenes.dfsph_dam_break(shape=(16, 8, 16), seed=11)
+    n = sc.count
+    nd, ne = (4, 6)
+    cfg = DFSPH.Configuration(divergence_iterations=nd, density_iterations=ne, pressure_warm_start=True,
+                              density_error_tolerance=tolerance, max_iterations=100)
+    solver, state, nxt, control, params, cache = dfsph_setup(sc, gpu_device, cfg)
+    d = oracle.make_domain(sc.domain)
+    zero = np.zeros((3, n), np.float32)
+    warm = (np.zeros(n, np.float32), np.zeros(n, np.float32))
+
+    def oracle_next(x, v, warm, step):
+        if tolerance > 0.0:
+            return oracle.dfsph_step_tolerance(d, step, x, v, zero, sc.parameters, warm[0], warm[1], tolerance, 100)
+        return oracle.dfsph_step(d, step, x, v, zero, sc.parameters, warm[0], warm[1], nd, ne, True)
+
+    x, v = sc.positions, np.zeros((3, n), np.float32)
+    state.particles.velocities.upload(v)
+    got = run_gpu_step(solver, state, nxt, control, params, cache)
+    exp = oracle_next(x, v, warm, 0)
+    for name in FIELDS:
+        assert bits_equal(got[name], exp[name]), ("step 1", name)
+    rng = np.random.default_rng(12)
+    v = got["velocities"].copy()
+    kicked = rng.choice(n, size=n // 10, replace=False)
+    v[:, kicked] = rng.uniform(-3.0, 3.0, size=(3, kicked.size)).astype(np.float32)
+    x = got["positions"]
+    warm = (got["warm_divergence_pressure"], got["warm_density_pressure"])
+    solver.copy_state(nxt, state)
+    state.particles.velocities.upload(v)
+    got = run_gpu_step(solver, state, nxt, control, params, cache)
+    exp = oracle_next(x, v, warm, 1)
+    for name in FIELDS:
+        assert bits_equal(got[name], exp[name]), ("step 2", name)
+
+
 def test_fp64_drift_report(gpu_device):
     """100 free-running steps of a 24x12x24 dam break in FP32 on the GPU and in
     FP64 in the oracle. The reference algorithm has discontinuities (box clamp
