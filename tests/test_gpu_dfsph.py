@@ -50,6 +50,29 @@ def test_step_bit_exact_small(gpu_device, nd, ne, warm_start):
     assert bits_equal(cache.non_pressure_accelerations().numpy(), expected["non_pressure_accelerations"])
 
 
+@pytest.mark.parametrize("unequal", [None, "masses", "viscosities"])
+def test_step_bit_exact_with_equal_and_unequal_masses(gpu_device, unequal):
+    """With equal masses the force passes read each candidate as one
+    (x, y, z, p / rho^2) record and take the mass from the step's mass
+    state; with unequal masses they read position, mass and p / rho^2
+    separately (unequal viscosities leave the records as they are). Every
+    case must equal the FP32 oracle bit for bit."""
+    print(f"dfsph unequal={unequal}", flush=True)
+    sc = scenes.dfsph_dam_break(shape=(16, 8, 16), seed=13)
+    n = sc.count
+    rng = np.random.default_rng(14)
+    if unequal is not None:
+        sc.parameters[unequal] = (sc.parameters[unequal] * rng.uniform(0.9, 1.1, n)).astype(np.float32)
+    warm = (rng.uniform(0, 50, n).astype(np.float32), rng.uniform(0, 50, n).astype(np.float32))
+    v0 = rng.normal(0, 0.2, size=(3, n)).astype(np.float32)
+    sc.velocities = v0
+    solver, state, nxt, control, params, cache = dfsph_setup(sc, gpu_device, DFSPH.Configuration(), warm)
+    got = run_gpu_step(solver, state, nxt, control, params, cache)
+    expected = oracle_step(sc, sc.positions, v0, warm, 0, 4, 6)
+    for name in FIELDS:
+        assert bits_equal(got[name], expected[name]), name
+
+
 def test_rollout_bit_exact(gpu_device):
     """20 steps of a 32x16x32 dam break: every step bit-exact against the FP32
     oracle fed the same input (warm-start pressures carried across steps)."""
