@@ -387,19 +387,27 @@ size_t exclusive_scan_partials(size_t count) { return (count + kScanTile - 1) / 
 
 namespace {
 // The stable in-cell order of a fluid build: rank_gather_kernel for small
-// cells, then big_cell_sort_kernel for the cells it queued (counts[C + 1]).
+// cells, then big_cell_sort_kernel for the cells it queued (counts[C + 1]),
+// both writing the permutation only; then every input is gathered into cell
+// order, one sorted position per thread, so the stores are coalesced and
+// only the loads scatter.
 void order_cells(hipStream_t stream, uint32_t n, uint32_t cells, const GridScratch& scratch, const uint32_t* offsets,
                  uint32_t* perm, const Channels& gather) {
+    const Channels none{};
     uint32_t* big_count = scratch.counts + static_cast<size_t>(cells) + 1u;
     rank_gather_kernel<<<blocks_for(n, kBlock), kBlock, 0, stream>>>(n, scratch.keys, scratch.slots, offsets,
-                                                                     scratch.unstable, perm, gather,
+                                                                     scratch.unstable, perm, none,
                                                                      scratch.big_cells, big_count);
     PHX_LAUNCH_CHECK();
     const uint32_t index_bits = n > 1u ? 32u - static_cast<uint32_t>(__builtin_clz(n - 1u)) : 1u;
     const uint32_t groups = static_cast<uint32_t>(std::min<size_t>(kBigCellGroups, big_cell_capacity(n)));
     big_cell_sort_kernel<<<groups, kBlock, 0, stream>>>(offsets, scratch.unstable, scratch.slots, perm,
-                                                        scratch.big_cells, big_count, index_bits, gather);
+                                                        scratch.big_cells, big_count, index_bits, none);
     PHX_LAUNCH_CHECK();
+    if (gather.count > 0u) {
+        gather_channels_kernel<<<blocks_for(n, kBlock), kBlock, 0, stream>>>(n, perm, gather);
+        PHX_LAUNCH_CHECK();
+    }
 }
 }  // namespace
 

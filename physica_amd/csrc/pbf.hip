@@ -783,9 +783,9 @@ PbfBatch::PbfBatch(const phx_domain_configuration* domains, uint32_t scene_count
     crowded_.resize(n);
     flags_.resize(kFlagDisplacement + config_.pressure_iterations + 1u);
     exact_flags_.resize(config_.pressure_iterations + 1u);
-    // grid build 8, flag fills 2, masks 1, two per iteration, three post passes
+    // grid build 9, flag fills 2, masks 1, two per iteration, three post passes
     const uint32_t it = config_.pressure_iterations;
-    kernels_per_step_ = 8u + 2u + 1u + 2u * it + (it == 0u ? 1u : 0u) + 3u;
+    kernels_per_step_ = 9u + 2u + 1u + 2u * it + (it == 0u ? 1u : 0u) + 3u;
     PHX_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
     PHX_HIP(hipEventCreateWithFlags(&ready_, hipEventDisableTiming));
     PHX_HIP(hipEventCreateWithFlags(&done_, hipEventDisableTiming));
@@ -840,7 +840,7 @@ void PbfBatch::substep(uint32_t from, uint64_t step, const phx_control& control,
     gather.add(parameters.vorticity_confinement, confinement_.data());
     build_batch_grid(s, n, cells_, blocks_, bv, x.cview(), predict, gather.get(), grid_.scratch, grid_.perm.data(),
                      grid_.offsets.data());
-    kernels_per_step_ += 8u;  // counts fill, key, three scan kernels, scatter, rank + gather, big-cell sort
+    kernels_per_step_ += 9u;  // counts fill, key, three scan kernels, scatter, rank, big-cell sort, gather
     const uint32_t sets = config_.pressure_iterations + 1u;
     fill_u32(s, flags_.data(), kFlagDisplacement + sets, 0u);
     fill_u32(s, exact_flags_.data(), sets, 0u);

@@ -62,9 +62,9 @@ def test_batch_offsets_and_launches(gpu_device):
     engine, state, control, params = batch_engine(batch, gpu_device, True)
     assert engine.offsets == list(np.cumsum([0] + [sc.count for sc in batch]))
     engine.run(state, control, params, 1)
-    # counts fill, key, 3 scan kernels, scatter, rank + gather, big-cell sort,
+    # counts fill, key, 3 scan kernels, scatter, rank, big-cell sort, gather,
     # 2 flag fills, mask, 2 per iteration, vorticity, normal, XSPH
-    assert engine.kernels_per_step == 8 + 2 + 1 + 2 * ITERATIONS + 3
+    assert engine.kernels_per_step == 9 + 2 + 1 + 2 * ITERATIONS + 3
     assert state.particles.step_index == 1
 
 
