@@ -77,6 +77,35 @@ def test_pressure_solver_rollout_bit_exact(gpu_device, method):
         solver.copy_state(nxt, state)
 
 
+@pytest.mark.parametrize("method", ["pcisph", "iisph"])
+def test_per_cell_skins_stay_exact_when_particles_outrun_their_bounds(gpu_device, method):
+    """The forward PCISPH/IISPH step uses per-cell skins from its second step
+    on (as DFSPH's). Step 1 starts at rest, so the bounds are tiny; step 2
+    gets a state in which a seeded tenth of the particles move at up to
+    3 m/s, far past them. Both steps equal the FP32 oracle bit for bit."""
+    print(f"sph per-cell skins with outrunning particles {method}", flush=True)
+    sc = scene_for(method, shape=(12, 12, 12), seed=17)
+    n = sc.count
+    solver, state, nxt, control, params, cache = sph_setup(sc, gpu_device, method)
+    x, v = sc.positions, np.zeros((3, n), np.float32)
+    state.particles.velocities.upload(v)
+    got = gpu_step(solver, state, nxt, control, params, cache)
+    exp = oracle_step(sc, method, x, v, 0)
+    for name in ("positions", "velocities"):
+        assert bits_equal(got[name], exp[name]), ("step 1", name)
+    rng = np.random.default_rng(18)
+    v = got["velocities"].copy()
+    kicked = rng.choice(n, size=n // 10, replace=False)
+    v[:, kicked] = rng.uniform(-3.0, 3.0, size=(3, kicked.size)).astype(np.float32)
+    x = got["positions"]
+    solver.copy_state(nxt, state)
+    state.particles.velocities.upload(v)
+    got = gpu_step(solver, state, nxt, control, params, cache)
+    exp = oracle_step(sc, method, x, v, 1)
+    for name in ("positions", "velocities"):
+        assert bits_equal(got[name], exp[name]), ("step 2", name)
+
+
 def test_wcsph_unit_exponent_bit_exact(gpu_device):
     """Tait exponent 1: powf(x, 1) = x exactly, so the whole step is exact."""
     sc = scene_for("wcsph")
