@@ -30,8 +30,11 @@ lattice, spacing 0.02, h 0.04), 120 steps at dt 1/120, 4 iterations, loss the
 mean squared distance of the final positions to a seeded target cloud,
 gradient with respect to the initial velocities (no parameter gradients).
 Ratio = (forward rollout + chained VJPs) / plain forward rollout, with a state
-checkpoint every 10 steps and each segment recomputed before its VJPs; the
-same with every step cache kept in HBM (no recompute) is reported beside it.
+checkpoint every 10 steps: the forward records the last segment, and the
+backward pass recomputes each other segment on one of two solvers (two
+streams) while the other solver differentiates the segment after it. The same
+on one stream, and with every step cache kept in HBM (no recompute), are
+reported beside it.
 
 `batched_pbf` is the uneven batch: 64 independent PBF dam breaks (particle
 counts log-uniform in [4,096, 200,000] from seed 20260005, 3.79 M in all),
@@ -264,9 +267,9 @@ def adjoint_ratio(device, steps=120):
                           f"{steps} steps, dt 1/120, 4 iterations, gradient of the final-position loss w.r.t. v0",
               "plain_forward_ms": round(plain_ms, 2)}
     dsolver, dstate, dcontrol, dparams = make(ExecutionMode.differentiable)
-    for every in (1, 10):
+    for every, overlap in ((1, False), (10, False), (10, True)):
         rollout = PbfRollout(dsolver, steps, checkpoint_every=every, parameter_gradients=False,
-                             forward_solver=forward_solver if every > 1 else None)
+                             forward_solver=forward_solver if every > 1 else None, overlap=overlap)
         final, fwd_ms = timed(lambda: rollout.forward(dstate, dcontrol, dparams))
         xT = final.particles.positions.numpy().astype(np.float64)
         seed = dsolver.allocate_state_adjoint()
@@ -275,9 +278,14 @@ def adjoint_ratio(device, steps=120):
         grad = initial.particles.velocities.numpy()
         if not np.isfinite(grad).all():
             raise RuntimeError("non-finite rollout gradient")
-        key = "store_all" if every == 1 else f"checkpoint_{every}"
+        if every == 1:
+            store_all_grad = grad
+        key = "store_all" if every == 1 else f"checkpoint_{every}" + ("" if overlap else "_one_stream")
         record[key] = {"forward_ms": round(fwd_ms, 2), "backward_ms": round(bwd_ms, 2),
                        "ratio": round((fwd_ms + bwd_ms) / plain_ms, 3)}
+        if every > 1:
+            # the same gradient bit for bit as the store-all rollout
+            record[key]["gradient_equals_store_all"] = bool(np.array_equal(grad, store_all_grad))
         rollout.release()
         del rollout, final, initial
     # BASELINE's definition: forward + backward with checkpoint-every-10 recompute

@@ -21,10 +21,13 @@ def setup(device, mode=ExecutionMode.differentiable):
 
 
 def test_checkpointed_rollout_matches_store_all(gpu_device):
-    """Store-all, checkpoints every 4 steps (segments recomputed in one reused
-    workspace), and the same with the unkept steps on a forward-mode solver
-    give identical states and gradients."""
-    print("rollout checkpoint 1 vs 4", flush=True)
+    """Store-all, checkpoints every 4 or 3 steps (the last segment recorded by
+    the forward pass, the others recomputed), on one stream or overlapped over
+    two solvers on two streams, with the unkept steps on the differentiable
+    solver or on a forward-mode solver: identical states and gradients. A
+    second backward pass of the same rollout (whose recorded segment the first
+    pass overwrote) gives them again."""
+    print("rollout checkpoint 1 vs 4 and 3, one and two streams", flush=True)
     sc, solver, state, control, params = setup(gpu_device)
     forward_solver = setup(gpu_device, ExecutionMode.forward)[1]
     n = sc.count
@@ -32,18 +35,22 @@ def test_checkpointed_rollout_matches_store_all(gpu_device):
     w = rng.normal(0, 1, size=(3, n))
     grads = []
     finals = []
-    for every, stepper in ((1, None), (4, None), (4, forward_solver), (3, forward_solver)):
-        rollout = PbfRollout(solver, 10, checkpoint_every=every, forward_solver=stepper)
+    variants = ((1, None, True), (4, None, True), (4, None, False), (4, forward_solver, True),
+                (3, forward_solver, True), (3, forward_solver, False))
+    for every, stepper, overlap in variants:
+        rollout = PbfRollout(solver, 10, checkpoint_every=every, forward_solver=stepper, overlap=overlap)
         final = rollout.forward(state, control, params)
         finals.append(final.particles.positions.numpy())
-        seed = solver.allocate_state_adjoint()
-        seed.particles.positions.upload(w)
-        initial, control_adjoint, parameter_adjoint = rollout.backward(seed)
-        torch.cuda.synchronize()
-        grads.append((initial.particles.velocities.numpy(), parameter_adjoint.relaxation.cpu().numpy(),
-                      control_adjoint.external_accelerations.numpy()))
+        for _ in range(2 if every == 3 else 1):
+            seed = solver.allocate_state_adjoint()
+            seed.particles.positions.upload(w)
+            initial, control_adjoint, parameter_adjoint = rollout.backward(seed)
+            torch.cuda.synchronize()
+            grads.append((initial.particles.velocities.numpy(), parameter_adjoint.relaxation.cpu().numpy(),
+                          control_adjoint.external_accelerations.numpy()))
     for k in range(1, len(finals)):
         np.testing.assert_array_equal(finals[0], finals[k])
+    for k in range(1, len(grads)):
         for a, b in zip(grads[0], grads[k]):
             np.testing.assert_array_equal(a, b)
     # the state passed in is not modified
