@@ -138,6 +138,127 @@ __device__ __forceinline__ void walk_slots_prefetch(SlotBase&& slot_base, const 
     }
 }
 
+// walk_slots_prefetch that hands each candidate's place to process: its
+// sorted index s, stencil slot k and bit b within that cell's mask, as
+// process(value, s, k, b).
+template <int kDepth, class SlotBase, class Load, class Process>
+__device__ __forceinline__ void walk_slots_prefetch_tagged(SlotBase&& slot_base, const uint16_t* column,
+                                                           uint32_t stride, uint32_t nonzero, uint32_t fallback,
+                                                           Load&& load, Process&& process) {
+    auto fetch = [&](uint32_t& mask, uint32_t& base, uint32_t& slot) {
+        mask = 0u;
+        if (nonzero != 0u) {
+            slot = static_cast<uint32_t>(__builtin_ctz(nonzero));
+            nonzero &= nonzero - 1u;
+            mask = column[slot * stride];
+            base = slot_base(slot);
+        }
+    };
+    uint32_t m = 0u, base = 0u, slot = 0u, m_next = 0u, base_next = 0u, slot_next = 0u;
+    fetch(m, base, slot);
+    fetch(m_next, base_next, slot_next);
+    auto next = [&](bool& have, uint32_t& s, uint32_t& k, uint32_t& b) -> uint32_t {
+        if (m == 0u) {
+            m = m_next;
+            base = base_next;
+            slot = slot_next;
+            fetch(m_next, base_next, slot_next);
+        }
+        have = m != 0u;
+        b = static_cast<uint32_t>(__builtin_ctz(m));
+        s = base + b;
+        k = slot;
+        m &= m - 1u;
+        return have ? s : fallback;
+    };
+    using Data = decltype(load(0u));
+    Data data[kDepth];
+    bool have[kDepth];
+    uint32_t ss[kDepth], ks[kDepth], bs[kDepth];
+#pragma unroll
+    for (int d = 0; d < kDepth; ++d) data[d] = load(next(have[d], ss[d], ks[d], bs[d]));
+    for (;;) {
+#pragma unroll
+        for (int d = 0; d < kDepth; ++d) {
+            if (!have[d]) return;
+            const Data current = data[d];
+            const uint32_t s = ss[d], k = ks[d], b = bs[d];
+            data[d] = load(next(have[d], ss[d], ks[d], bs[d]));
+            asm volatile("" ::: "memory");
+            process(current, s, k, b);
+        }
+    }
+}
+
+// Every candidate of the 27 cells around (cx, cy, cz) in the reference order
+// (for_each_candidate) with the memory latency taken off the critical path as
+// in walk_slots_prefetch: the range of the next in-grid stencil cell is
+// fetched when a cell starts, and the records of the next kDepth candidates
+// before the current one is processed. process(value, s, k, first) gets the
+// candidate's sorted index s, stencil slot k and its cell's first index.
+// load() must accept `fallback`. It pays where a lane walks long lists with
+// few waves to hide its loads (crowded cells in a scene that does not fill
+// the GPU); with the GPU full, its bookkeeping costs more than it hides.
+template <int kDepth, class Load, class Process>
+__device__ __forceinline__ void walk_stencil_prefetch(const SortedGrid& g, int cx, int cy, int cz, uint32_t fallback,
+                                                      Load&& load, Process&& process) {
+    struct Range {
+        uint32_t s, last, first, k;
+    };
+    uint32_t slot = 0u;  // next stencil slot to fetch
+    // the next in-grid stencil cell (empty or not); slot 27 once none is left
+    auto fetch = [&](Range& r) {
+        r = Range{0u, 0u, 0u, static_cast<uint32_t>(kStencil)};
+        while (slot < static_cast<uint32_t>(kStencil)) {
+            const uint32_t k = slot++;
+            const int X = cx + static_cast<int>(k / 9u) - 1;
+            const int Y = cy + static_cast<int>((k / 3u) % 3u) - 1;
+            const int Z = cz + static_cast<int>(k % 3u) - 1;
+            if (X < 0 || X >= static_cast<int>(g.cx) || Y < 0 || Y >= static_cast<int>(g.cy) || Z < 0 ||
+                Z >= static_cast<int>(g.cz))
+                continue;
+            const uint32_t c = cell_of(g, X, Y, Z);
+            r.first = g.offsets[c];
+            r.s = r.first;
+            r.last = g.offsets[c + 1u];
+            r.k = k;
+            break;
+        }
+    };
+    Range cur, ahead;
+    fetch(cur);
+    fetch(ahead);
+    auto next = [&](bool& have, uint32_t& s, uint32_t& k, uint32_t& first) -> uint32_t {
+        while (cur.s == cur.last && cur.k < static_cast<uint32_t>(kStencil)) {
+            cur = ahead;
+            fetch(ahead);
+        }
+        have = cur.s != cur.last;
+        s = cur.s;
+        k = cur.k;
+        first = cur.first;
+        cur.s += have ? 1u : 0u;
+        return have ? s : fallback;
+    };
+    using Data = decltype(load(0u));
+    Data data[kDepth];
+    bool have[kDepth];
+    uint32_t ss[kDepth], ks[kDepth], fs[kDepth];
+#pragma unroll
+    for (int d = 0; d < kDepth; ++d) data[d] = load(next(have[d], ss[d], ks[d], fs[d]));
+    for (;;) {
+#pragma unroll
+        for (int d = 0; d < kDepth; ++d) {
+            if (!have[d]) return;
+            const Data current = data[d];
+            const uint32_t s = ss[d], k = ks[d], first = fs[d];
+            data[d] = load(next(have[d], ss[d], ks[d], fs[d]));
+            asm volatile("" ::: "memory");
+            process(current, s, k, first);
+        }
+    }
+}
+
 // walk_slots_prefetch over the grid's cell offsets.
 template <int kDepth = 1, class Load, class Process>
 __device__ __forceinline__ void walk_masked_prefetch(const uint32_t* __restrict__ offsets, uint32_t center,
@@ -158,6 +279,70 @@ __device__ __forceinline__ uint32_t drop_self(uint32_t i, uint32_t cell_first, u
     const uint32_t m = column[kCenter * stride] & ~(bit < kMaskBits ? 1u << bit : 0u);
     column[kCenter * stride] = static_cast<uint16_t>(m);
     return m == 0u ? nonzero & ~(1u << kCenter) : nonzero;
+}
+
+// ------------------------------------------------ crowded cells, wave-shared
+// A particle next to a crowded cell walks every candidate of its 27 cells.
+// Inside a crowded cell every particle has the same 27 cells, so a wave whose
+// crowded lanes all sit in one cell fetches each run of 64 candidates with
+// one coalesced load per lane and hands candidate k to every lane from scalar
+// registers (v_readlane): the walk has no memory access per candidate. Each
+// lane still adds its own terms in the reference order, so its sums are
+// unchanged bit for bit.
+
+// The value `value` holds on lane `lane`, on every lane (whole dwords).
+template <class T>
+__device__ __forceinline__ T broadcast_from(const T& value, uint32_t lane) {
+    static_assert(sizeof(T) % 4 == 0, "records are whole dwords");
+    constexpr int kWords = static_cast<int>(sizeof(T) / 4);
+    uint32_t words[kWords];
+    __builtin_memcpy(words, &value, sizeof(T));
+#pragma unroll
+    for (int w = 0; w < kWords; ++w)
+        words[w] = static_cast<uint32_t>(
+            __builtin_amdgcn_readlane(static_cast<int>(words[w]), static_cast<int>(lane)));
+    T out;
+    __builtin_memcpy(&out, words, sizeof(T));
+    return out;
+}
+
+// Whether the wave walks its crowded particles' stencil together: every lane
+// of the wave active, at least one crowded lane, and every crowded lane in
+// one cell. `leader` is then the first crowded lane. The result is uniform
+// over the wave.
+__device__ __forceinline__ bool shared_crowded_cell(bool crowded, uint32_t cell, uint32_t& leader) {
+    const uint64_t active = __ballot(1);
+    const uint64_t lanes = __ballot(crowded);
+    if (active != ~0ull || lanes == 0ull) return false;
+    leader = static_cast<uint32_t>(__builtin_ctzll(lanes));
+    const uint32_t shared = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(cell), leader));
+    return __ballot(!crowded || cell == shared) == ~0ull;
+}
+
+// The candidates of the 27 cells around the leader lane's cell (cx, cy, cz)
+// in the reference order, 64 at a time: lane l loads candidate base + l
+// (load(s) returns its record), then process(record, s) runs for each
+// candidate in turn on the lanes with `participate`. Every lane of the wave
+// must call it together, in uniform control flow.
+template <class Load, class Process>
+__device__ __forceinline__ void walk_stencil_shared(const SortedGrid& g, uint32_t leader, int cx, int cy, int cz,
+                                                    bool participate, Load&& load, Process&& process) {
+    const int x = __builtin_amdgcn_readlane(cx, static_cast<int>(leader));
+    const int y = __builtin_amdgcn_readlane(cy, static_cast<int>(leader));
+    const int z = __builtin_amdgcn_readlane(cz, static_cast<int>(leader));
+    const uint32_t lane = __lane_id();
+    for_each_cell(g, x, y, z, [&](uint32_t c) {
+        const uint32_t first = g.offsets[c];
+        const uint32_t last = g.offsets[c + 1u];
+        for (uint32_t base = first; base < last; base += 64u) {
+            const uint32_t count = min(64u, last - base);
+            const auto mine = load(base + min(lane, count - 1u));
+            for (uint32_t k = 0; k < count; ++k) {
+                const auto record = broadcast_from(mine, k);
+                if (participate) process(record, base + k);
+            }
+        }
+    });
 }
 
 // Workgroup index with consecutive ranges of indices on one XCD (MI355X
@@ -248,6 +433,27 @@ struct MaskView {
     const uint8_t* crowded = nullptr;
 };
 
+// Whether particle i is crowded in the step of these masks (it then walks the
+// full stencil in every pass).
+__device__ __forceinline__ bool is_crowded(const MaskView& m, uint32_t i) {
+    return m.crowded != nullptr && m.crowded[i] != 0u;
+}
+
+// Candidate records of the wave-shared walks (whole dwords).
+struct PointMass {
+    Float3 x;
+    float m;
+};
+struct PointMassLambda {
+    Float3 x;
+    float m;
+    float lambda;
+};
+struct PointVector {
+    Float3 x;
+    Float3 v;
+};
+
 // Whether particle i walks its masks in a pass reading displacement slot `slot`.
 __device__ __forceinline__ bool masks_apply(const MaskView& m, uint32_t slot, uint32_t i) {
     if (m.masks == nullptr || m.flags[kFlagOverflow] != 0u) return false;
@@ -336,6 +542,111 @@ __device__ __forceinline__ bool visit_and_mask(const SortedGrid& g, const MaskVi
         }
     }
     for (; written < static_cast<uint32_t>(kStencil); ++written) out[static_cast<size_t>(written) * n + i] = 0u;
+    return complete;
+}
+
+// A candidate's record and sorted index.
+template <class Record>
+struct Tagged {
+    Record r;
+    uint32_t s;
+};
+
+// Visits the candidates of particle i like visit_candidates, as
+// pair(load(s), s): load(s) fetches candidate s's record. With kDepth > 0 the
+// walks fetch kDepth records ahead of the one processed
+// (walk_masked_prefetch, walk_stencil_prefetch); with 0 they are the plain
+// walks, for launches that fill the GPU. A wave whose crowded particles all
+// sit in one cell walks their stencil together (walk_stencil_shared); its
+// other lanes walk their own candidates first. Every lane of the wave must
+// call it together, in uniform control flow.
+template <int kDepth, class Load, class Pair>
+__device__ __forceinline__ void visit_records(const SortedGrid& g, const MaskView& m, bool masked, uint32_t i, int cx,
+                                              int cy, int cz, const int* delta, uint16_t* column, uint32_t stride,
+                                              Load&& load, Pair&& pair) {
+    uint32_t leader = 0u;
+    const bool crowded = is_crowded(m, i);
+    const bool shared = shared_crowded_cell(crowded, cell_of(g, cx, cy, cz), leader);
+    if (!shared || !crowded) {
+        if constexpr (kDepth == 0) {
+            visit_candidates(g, m, masked, i, cx, cy, cz, delta, column, stride, [&](uint32_t s) { pair(load(s), s); });
+        } else if (masked) {
+            using Record = decltype(load(0u));
+            const uint32_t nonzero = load_lane_column(m.masks, m.n, i, column, stride);
+            walk_masked_prefetch<kDepth>(
+                g.offsets, cell_of(g, cx, cy, cz), delta, column, stride, nonzero, i,
+                [&](uint32_t s) { return Tagged<Record>{load(s), s}; },
+                [&](const Tagged<Record>& t) { pair(t.r, t.s); });
+        } else {
+            walk_stencil_prefetch<kDepth>(g, cx, cy, cz, i, load,
+                                          [&](const auto& r, uint32_t s, uint32_t, uint32_t) { pair(r, s); });
+        }
+    }
+    if (shared) walk_stencil_shared(g, leader, cx, cy, cz, crowded, load, pair);
+}
+
+// visit_records where keep(record, s) says whether candidate s is kept, and
+// the kept candidates of each stencil slot k are written as the mask
+// out[k * n + i] (every slot is written) like visit_and_mask. Crowded lanes
+// of a wave-shared cell write no masks (a crowded particle's masks are never
+// read). Returns false if a kept candidate lies beyond the first kMaskBits
+// particles of its cell.
+template <int kDepth, class Load, class Keep>
+__device__ __forceinline__ bool visit_records_and_mask(const SortedGrid& g, const MaskView& m, bool masked,
+                                                       uint32_t i, int cx, int cy, int cz, const int* delta,
+                                                       uint16_t* column, uint32_t stride,
+                                                       uint16_t* __restrict__ out, uint32_t n, Load&& load,
+                                                       Keep&& keep) {
+    uint32_t leader = 0u;
+    const bool crowded = is_crowded(m, i);
+    const bool shared = shared_crowded_cell(crowded, cell_of(g, cx, cy, cz), leader);
+    bool complete = true;
+    if (!shared || !crowded) {
+        if constexpr (kDepth == 0) {
+            complete = visit_and_mask(g, m, masked, i, cx, cy, cz, delta, column, stride, out, n,
+                                      [&](uint32_t s) { return keep(load(s), s); });
+        } else {
+            constexpr uint32_t kNone = static_cast<uint32_t>(kStencil);
+            uint32_t written = 0u, open = kNone, bits = 0u;
+            // closes the open slot (zero masks before it) and opens slot k
+            auto open_slot = [&](uint32_t k) {
+                if (open != kNone) {
+                    for (; written < open; ++written) out[static_cast<size_t>(written) * n + i] = 0u;
+                    out[static_cast<size_t>(open) * n + i] = static_cast<uint16_t>(bits);
+                    written = open + 1u;
+                }
+                open = k;
+                bits = 0u;
+            };
+            auto take = [&](const auto& r, uint32_t s, uint32_t k, uint32_t b) {
+                if (k != open) open_slot(k);
+                if (keep(r, s)) {
+                    if (b < kMaskBits) {
+                        bits |= 1u << b;
+                    } else {
+                        complete = false;
+                    }
+                }
+            };
+            if (masked) {
+                const uint32_t nonzero = load_lane_column(m.masks, m.n, i, column, stride);
+                const uint32_t center = cell_of(g, cx, cy, cz);
+                walk_slots_prefetch_tagged<kDepth>(
+                    [&](uint32_t k) { return g.offsets[center + static_cast<uint32_t>(delta[k])]; }, column, stride,
+                    nonzero, i, load, take);
+            } else {
+                walk_stencil_prefetch<kDepth>(
+                    g, cx, cy, cz, i, load,
+                    [&](const auto& r, uint32_t s, uint32_t k, uint32_t first) { take(r, s, k, s - first); });
+            }
+            open_slot(kNone);
+            for (; written < static_cast<uint32_t>(kStencil); ++written)
+                out[static_cast<size_t>(written) * n + i] = 0u;
+        }
+    }
+    if (shared) walk_stencil_shared(g, leader, cx, cy, cz, crowded, load, [&](const auto& r, uint32_t s) {
+        (void)keep(r, s);
+    });
     return complete;
 }
 

@@ -24,6 +24,11 @@ namespace phx {
 namespace {
 
 constexpr uint32_t kBlock = 256;
+// Records a neighbour walk fetches ahead: two in a single scene's launch
+// (too few waves to hide the loads of long crowded walks), none in a batch
+// launch, which fills the GPU.
+template <bool kBatch>
+constexpr int kPbfWalkDepth = kBatch ? 0 : 2;
 
 // LDS of a masked neighbour pass: the lanes' mask columns and the slot table.
 #define PHX_MASK_LDS(grid)                          \
@@ -153,9 +158,8 @@ __global__ void __launch_bounds__(kBlock) pbf_lambda_kernel(LambdaArgs a, BatchV
     float density = 0.0F;
     Float3 gradient_sum{0.0F, 0.0F, 0.0F};
     float denominator = a.eps[i];
-    auto visit = [&](uint32_t s) {
-        const Float3 d = sub(p, load(a.pos, s));
-        const float ms = a.m[s];
+    auto pair = [&](Float3 q, float ms, uint32_t s) {
+        const Float3 d = sub(p, q);
         density += ms * poly6(d, a.h);
         if (s != i) {
             const Float3 g = spiky_gradient(d, a.h, a.spiky);
@@ -165,6 +169,7 @@ __global__ void __launch_bounds__(kBlock) pbf_lambda_kernel(LambdaArgs a, BatchV
         }
         return dot(d, d) < a.t_exact;
     };
+    auto visit = [&](uint32_t s) { return pair(load(a.pos, s), a.m[s], s); };
     if (a.boundary.offsets != nullptr) {
         for_each_candidate(
             grid, a.boundary, cx, cy, cz, [&](uint32_t s) { (void)visit(s); },
@@ -175,8 +180,10 @@ __global__ void __launch_bounds__(kBlock) pbf_lambda_kernel(LambdaArgs a, BatchV
                 gradient_sum = add(gradient_sum, scale(spiky_gradient(d, a.h, a.spiky), b.w));
             });
     } else {
-        const bool complete = visit_and_mask(grid, a.mv, masked, i, cx, cy, cz, delta, smask + threadIdx.x, kBlock,
-                                             a.exact_out, a.n, visit);
+        const bool complete = visit_records_and_mask<kPbfWalkDepth<kBatch>>(
+            grid, a.mv, masked, i, cx, cy, cz, delta, smask + threadIdx.x, kBlock, a.exact_out, a.n,
+            [&](uint32_t s) { return PointMass{load(a.pos, s), a.m[s]}; },
+            [&](const PointMass& c, uint32_t s) { return pair(c.x, c.m, s); });
         flag_incomplete(complete, a.mv, i, &a.exact_flags[kFlagOverflow]);
     }
     denominator += dot(gradient_sum, gradient_sum);
@@ -244,14 +251,15 @@ __global__ void __launch_bounds__(kBlock) pbf_correction_kernel(CorrectionArgs a
             return vanishes ? (poly6(d, a.h) == 0.0F ? 0.0F : -strength)
                             : artificial_value(d, a.h, strength, exponent, reference);
         };
-        auto visit = [&](uint32_t s) {
+        auto pair = [&](Float3 q, float ms, float lambda_s, uint32_t s) {
             if (s == i) return;
-            const Float3 d = sub(p, load(a.pos, s));
+            const Float3 d = sub(p, q);
             const float artificial = artificial_at(d);
-            const float coefficient = a.m[s] / rest;
+            const float coefficient = ms / rest;
             correction = add(correction, scale(spiky_gradient(d, a.h, a.spiky),
-                                               coefficient * (lambda_i + a.lambda[s] + artificial)));
+                                               coefficient * (lambda_i + lambda_s + artificial)));
         };
+        auto visit = [&](uint32_t s) { pair(load(a.pos, s), a.m[s], a.lambda[s], s); };
         if (a.boundary.offsets != nullptr) {
             for_each_candidate(grid, a.boundary, cx, cy, cz, visit, [&](uint32_t s) {
                 const float4 b = a.boundary.x[s];
@@ -260,7 +268,10 @@ __global__ void __launch_bounds__(kBlock) pbf_correction_kernel(CorrectionArgs a
                 correction = add(correction, scale(spiky_gradient(d, a.h, a.spiky), b.w * (lambda_i + artificial)));
             });
         } else {
-            visit_candidates(grid, a.mv, masked, i, cx, cy, cz, delta, smask + threadIdx.x, kBlock, visit);
+            visit_records<kPbfWalkDepth<kBatch>>(
+                grid, a.mv, masked, i, cx, cy, cz, delta, smask + threadIdx.x, kBlock,
+                [&](uint32_t s) { return PointMassLambda{load(a.pos, s), a.m[s], a.lambda[s]}; },
+                [&](const PointMassLambda& c, uint32_t s) { pair(c.x, c.m, c.lambda, s); });
         }
         const Float3 candidate = add(p, correction);
         if (a.mask != nullptr) {
@@ -327,12 +338,15 @@ __global__ void __launch_bounds__(kBlock) pbf_vorticity_kernel(PostArgs a, Batch
     int cx, cy, cz;
     particle_cell(grid, load(a.topo, i), cx, cy, cz);
     Float3 w{0.0F, 0.0F, 0.0F};
-    const bool complete = visit_and_mask(
-        grid, a.mv, masked, i, cx, cy, cz, delta, smask + threadIdx.x, kBlock, a.exact_out, a.n, [&](uint32_t s) {
-            const Float3 d = sub(p, load(a.pos, s));
-            if (s != i) w = add(w, cross(sub(load(a.vel, s), v), spiky_gradient(d, a.h, a.spiky)));
-            return dot(d, d) < a.t_exact;
-        });
+    auto pair = [&](Float3 q, Float3 vq, uint32_t s) {
+        const Float3 d = sub(p, q);
+        if (s != i) w = add(w, cross(sub(vq, v), spiky_gradient(d, a.h, a.spiky)));
+        return dot(d, d) < a.t_exact;
+    };
+    const bool complete = visit_records_and_mask<kPbfWalkDepth<kBatch>>(
+        grid, a.mv, masked, i, cx, cy, cz, delta, smask + threadIdx.x, kBlock, a.exact_out, a.n,
+        [&](uint32_t s) { return PointVector{load(a.pos, s), load(a.vel, s)}; },
+        [&](const PointVector& c, uint32_t s) { return pair(c.x, c.v, s); });
     flag_incomplete(complete, a.mv, i, &a.exact_flags[kFlagOverflow]);
     store(a.vort, i, w);
     a.magnitude[i] = length(w);
@@ -351,10 +365,14 @@ __global__ void __launch_bounds__(kBlock) pbf_normal_kernel(PostArgs a, BatchVie
     int cx, cy, cz;
     particle_cell(grid, load(a.topo, i), cx, cy, cz);
     Float3 g{0.0F, 0.0F, 0.0F};
-    visit_candidates(grid, a.mv, masked, i, cx, cy, cz, delta, smask + threadIdx.x, kBlock, [&](uint32_t s) {
+    auto pair = [&](Float3 q, float mag_s, uint32_t s) {
         if (s == i) return;
-        g = add(g, scale(spiky_gradient(sub(p, load(a.pos, s)), a.h, a.spiky), a.magnitude[s] - mag_i));
-    });
+        g = add(g, scale(spiky_gradient(sub(p, q), a.h, a.spiky), mag_s - mag_i));
+    };
+    visit_records<kPbfWalkDepth<kBatch>>(
+        grid, a.mv, masked, i, cx, cy, cz, delta, smask + threadIdx.x, kBlock,
+        [&](uint32_t s) { return PointMass{load(a.pos, s), a.magnitude[s]}; },
+        [&](const PointMass& c, uint32_t s) { pair(c.x, c.m, s); });
     const float normalizer = length(g);
     const Float3 normal = normalizer > 0.0F ? scale(g, 1.0F / normalizer) : Float3{0.0F, 0.0F, 0.0F};
     if (a.normalizers != nullptr) {
@@ -379,10 +397,14 @@ __global__ void __launch_bounds__(kBlock) pbf_xsph_kernel(PostArgs a, BatchView 
     int cx, cy, cz;
     particle_cell(grid, load(a.topo, i), cx, cy, cz);
     Float3 smoothing{0.0F, 0.0F, 0.0F};
-    visit_candidates(grid, a.mv, masked, i, cx, cy, cz, delta, smask + threadIdx.x, kBlock, [&](uint32_t s) {
+    auto pair = [&](Float3 q, Float3 vq, uint32_t s) {
         if (s == i) return;
-        smoothing = add(smoothing, scale(sub(load(vc, s), v), poly6(sub(p, load(a.pos, s)), a.h)));
-    });
+        smoothing = add(smoothing, scale(sub(vq, v), poly6(sub(p, q), a.h)));
+    };
+    visit_records<kPbfWalkDepth<kBatch>>(
+        grid, a.mv, masked, i, cx, cy, cz, delta, smask + threadIdx.x, kBlock,
+        [&](uint32_t s) { return PointVector{load(a.pos, s), load(vc, s)}; },
+        [&](const PointVector& c, uint32_t s) { pair(c.x, c.v, s); });
     const uint32_t j = a.perm[i];
     store(a.next_v, j, add(v, scale(smoothing, a.xsph[i])));
     store(a.next_x, j, p);

@@ -31,6 +31,9 @@ namespace phx {
 namespace {
 
 constexpr uint32_t kBlock = 256;
+// records a derivative pass's walks fetch ahead (two were no faster: the
+// deeper ring costs occupancy)
+constexpr int kVjpDepth = 1;
 
 // LDS of a masked neighbour pass: the lanes' mask columns and the slot table
 // (every thread reaches the barrier before any returns).
@@ -146,6 +149,20 @@ __device__ __forceinline__ void walk_with_samples(const Step& s, bool masked, ui
     }
 }
 
+// The same with fluid candidate j visited as pair(load(j), j) (visit_records
+// without samples).
+template <class Load, class Pair, class VisitBoundary>
+__device__ __forceinline__ void walk_with_samples(const Step& s, bool masked, uint32_t i, int cx, int cy, int cz,
+                                                  const int* delta, uint16_t* column, Load&& load, Pair&& pair,
+                                                  VisitBoundary&& visit_boundary) {
+    if (s.boundary.offsets != nullptr) {
+        for_each_candidate(
+            s.grid, s.boundary, cx, cy, cz, [&](uint32_t j) { pair(load(j), j); }, visit_boundary);
+    } else {
+        visit_records<kVjpDepth>(s.grid, s.mv, masked, i, cx, cy, cz, delta, column, kBlock, load, pair);
+    }
+}
+
 __device__ __forceinline__ Float3 sample_position(const Step& s, uint32_t b) {
     const float4 q = s.boundary.x[b];
     return Float3{q.x, q.y, q.z};
@@ -191,14 +208,21 @@ __global__ void __launch_bounds__(kBlock) pbf_vjp_post_kernel(PostVjpArgs a) {
     double ccon = 0.0;
     int cx, cy, cz;
     cell_of_topology(s, i, cx, cy, cz);
-    visit_candidates(s.grid, s.mv, masked, i, cx, cy, cz, delta, smask + threadIdx.x, kBlock, [&](uint32_t j) {
+    struct Record {
+        Float3 x, vc;
+        D3 vb;
+        float c;
+    };
+    auto fetch = [&](uint32_t j) { return Record{load(a.pos, j), load(a.vc, j), load(a.next_v_a, j), s.xsph[j]}; };
+    visit_records<kVjpDepth>(s.grid, s.mv, masked, i, cx, cy, cz, delta, smask + threadIdx.x, kBlock, fetch,
+                  [&](const Record& r, uint32_t j) {
         if (j == i) return;
-        const Float3 dd = sub(pi, load(a.pos, j));
+        const Float3 dd = sub(pi, r.x);
         const double wgt = static_cast<double>(poly6(dd, s.h));
         const D3 gw = d3(poly6_gradient(dd, s.h));
-        const D3 diff = sub(d3(load(a.vc, j)), vci);
-        const D3 vb = load(a.next_v_a, j);
-        const double cj = static_cast<double>(s.xsph[j]);
+        const D3 diff = sub(d3(r.vc), vci);
+        const D3 vb = r.vb;
+        const double cj = static_cast<double>(r.c);
         vcon = add(vcon, scale(out, -ci * wgt));
         vcon = add(vcon, scale(vb, cj * wgt));
         pcon = add(pcon, scale(gw, ci * dot(out, diff) - cj * dot(vb, diff)));
@@ -242,12 +266,19 @@ __global__ void __launch_bounds__(kBlock) pbf_vjp_normal_kernel(NormalVjpArgs a)
     double mcon = 0.0;
     int cx, cy, cz;
     cell_of_topology(s, i, cx, cy, cz);
-    visit_candidates(s.grid, s.mv, masked, i, cx, cy, cz, delta, smask + threadIdx.x, kBlock, [&](uint32_t j) {
+    struct Record {
+        Float3 x;
+        float mag;
+        D3 g;
+    };
+    auto fetch = [&](uint32_t j) { return Record{load(a.pos, j), a.mag[j], load(a.g_a, j)}; };
+    visit_records<kVjpDepth>(s.grid, s.mv, masked, i, cx, cy, cz, delta, smask + threadIdx.x, kBlock, fetch,
+                  [&](const Record& r, uint32_t j) {
         if (j == i) return;
-        const Float3 dd = sub(pi, load(a.pos, j));
+        const Float3 dd = sub(pi, r.x);
         const D3 gr = d3(spiky_gradient(dd, s.h, s.spiky));
-        const D3 both = add(gi, load(a.g_a, j));
-        pcon = add(pcon, spiky_hessian(dd, scale(both, static_cast<double>(a.mag[j]) - mi), static_cast<double>(s.h),
+        const D3 both = add(gi, r.g);
+        pcon = add(pcon, spiky_hessian(dd, scale(both, static_cast<double>(r.mag) - mi), static_cast<double>(s.h),
                                        static_cast<double>(s.spiky)));
         mcon -= dot(gr, both);
     });
@@ -277,12 +308,18 @@ __global__ void __launch_bounds__(kBlock) pbf_vjp_vorticity_kernel(VorticityVjpA
     D3 vcon{0.0, 0.0, 0.0};
     int cx, cy, cz;
     cell_of_topology(s, i, cx, cy, cz);
-    visit_candidates(s.grid, s.mv, masked, i, cx, cy, cz, delta, smask + threadIdx.x, kBlock, [&](uint32_t j) {
+    struct Record {
+        Float3 x, v;
+        D3 w;
+    };
+    auto fetch = [&](uint32_t j) { return Record{load(a.pos, j), load(a.vrec, j), load(a.w_a, j)}; };
+    visit_records<kVjpDepth>(s.grid, s.mv, masked, i, cx, cy, cz, delta, smask + threadIdx.x, kBlock, fetch,
+                  [&](const Record& r, uint32_t j) {
         if (j == i) return;
-        const Float3 dd = sub(pi, load(a.pos, j));
+        const Float3 dd = sub(pi, r.x);
         const D3 gr = d3(spiky_gradient(dd, s.h, s.spiky));
-        const D3 both = add(wb, load(a.w_a, j));
-        const D3 diff = sub(d3(load(a.vrec, j)), vi);
+        const D3 both = add(wb, r.w);
+        const D3 diff = sub(d3(r.v), vi);
         vcon = sub(vcon, cross(gr, both));
         pcon = add(pcon, spiky_hessian(dd, cross(both, diff), static_cast<double>(s.h), static_cast<double>(s.spiky)));
     });
@@ -342,15 +379,31 @@ __global__ void __launch_bounds__(kBlock) pbf_vjp_correction_kernel(CorrectionVj
     double lcon = 0.0, mcon = 0.0, rcon = 0.0, scon = 0.0, econ = 0.0, qcon = 0.0;
     int cx, cy, cz;
     cell_of_topology(s, i, cx, cy, cz);
-    walk_with_samples(s, masked, i, cx, cy, cz, delta, smask + threadIdx.x, [&](uint32_t j) {
+    // candidate j: position, lambda, mass, artificial-pressure parameters and
+    // vanishing flag, 1 / rho0 and its unclamped position adjoint
+    struct Record {
+        Float3 x;
+        float lambda, m, strength, exponent, radius;
+        uint32_t vanishes;
+        double inv_rho0;
+        D3 ob;
+    };
+    auto fetch = [&](uint32_t j) {
+        return Record{load(a.pos, j),    a.lambda[j],  s.m[j],
+                      s.strength[j],     s.exponent[j], s.radius[j],
+                      a.vanishes[j],     s.inv_rho0[j], unclamped(load(a.p_in, j), a.mask[j])};
+    };
+    const float strength_i = s.strength[i], exponent_i = s.exponent[i], radius_i = s.radius[i];
+    const bool vanishes_i = a.vanishes[i] != 0u;
+    walk_with_samples(s, masked, i, cx, cy, cz, delta, smask + threadIdx.x, fetch, [&](const Record& r, uint32_t j) {
         if (j == i) return;
-        const Float3 dd = sub(pi, load(a.pos, j));
+        const Float3 dd = sub(pi, r.x);
         const D3 gr = d3(spiky_gradient(dd, s.h, s.spiky));
-        const float lam_j = a.lambda[j];
+        const float lam_j = r.lambda;
         // i as the centre
-        const Artificial art = artificial_full(dd, s.h, s.strength[i], s.exponent[i], s.radius[i], a.vanishes[i] != 0u,
-                                               a.strength_wanted != 0u);
-        const double cf = static_cast<double>(s.m[j]) * inv_rho0;
+        const Artificial art =
+            artificial_full(dd, s.h, strength_i, exponent_i, radius_i, vanishes_i, a.strength_wanted != 0u);
+        const double cf = static_cast<double>(r.m) * inv_rho0;
         const double mu = static_cast<double>(lam_i) + static_cast<double>(lam_j) + static_cast<double>(art.value);
         const double oa_gr = dot(oa, gr);
         const double sa = cf * oa_gr;
@@ -360,12 +413,12 @@ __global__ void __launch_bounds__(kBlock) pbf_vjp_correction_kernel(CorrectionVj
         econ += sa * static_cast<double>(art.d_exponent);
         qcon += sa * static_cast<double>(art.d_radius);
         // i as a neighbour of j: d_ji = -d_ij, gradient -gr
-        const Artificial rart = artificial_full(scale(dd, -1.0F), s.h, s.strength[j], s.exponent[j], s.radius[j],
-                                                a.vanishes[j] != 0u, a.strength_wanted != 0u);
-        const double inv_rrho0 = s.inv_rho0[j];
+        const Artificial rart = artificial_full(scale(dd, -1.0F), s.h, r.strength, r.exponent, r.radius,
+                                                r.vanishes != 0u, a.strength_wanted != 0u);
+        const double inv_rrho0 = r.inv_rho0;
         const double rcf = m_i * inv_rrho0;
         const double rmu = static_cast<double>(lam_j) + static_cast<double>(lam_i) + static_cast<double>(rart.value);
-        const D3 ob = unclamped(load(a.p_in, j), a.mask[j]);
+        const D3 ob = r.ob;
         const double ob_rgr = -dot(ob, gr);
         const double rsa = rcf * ob_rgr;
         lcon += rsa;
@@ -376,8 +429,8 @@ __global__ void __launch_bounds__(kBlock) pbf_vjp_correction_kernel(CorrectionVj
         // a boundary sample: V_b (lambda_i + s_corr) grad W (pbf.cu:390-397)
         const Float3 dd = sub(pi, sample_position(s, b));
         const D3 gr = d3(spiky_gradient(dd, s.h, s.spiky));
-        const Artificial art = artificial_full(dd, s.h, s.strength[i], s.exponent[i], s.radius[i], a.vanishes[i] != 0u,
-                                               a.strength_wanted != 0u);
+        const Artificial art =
+            artificial_full(dd, s.h, strength_i, exponent_i, radius_i, vanishes_i, a.strength_wanted != 0u);
         const double vol = static_cast<double>(s.boundary.x[b].w);
         const double mu = static_cast<double>(lam_i) + static_cast<double>(art.value);
         const double sa = vol * dot(oa, gr);
@@ -442,24 +495,36 @@ __global__ void __launch_bounds__(kBlock) pbf_vjp_lambda_kernel(LambdaVjpArgs a)
     double rd = 0.0;  // the density's rest-density adjoint (boundary samples)
     int cx, cy, cz;
     cell_of_topology(s, i, cx, cy, cz);
-    walk_with_samples(s, masked, i, cx, cy, cz, delta, smask + threadIdx.x, [&](uint32_t j) {
-        const Float3 dd = sub(pi, load(a.pos, j));
-        const double2 jterms = a.lambda_terms[j];
+    // candidate j: position, mass, per-particle lambda terms, 1 / rho0 and
+    // gradient sum
+    struct Record {
+        Float3 x;
+        float m;
+        double2 terms;
+        double inv_rho0;
+        Float3 gsum;
+    };
+    auto fetch = [&](uint32_t j) {
+        return Record{load(a.pos, j), s.m[j], a.lambda_terms[j], s.inv_rho0[j], load(a.gsum, j)};
+    };
+    walk_with_samples(s, masked, i, cx, cy, cz, delta, smask + threadIdx.x, fetch, [&](const Record& r, uint32_t j) {
+        const Float3 dd = sub(pi, r.x);
+        const double2 jterms = r.terms;
         // density term (the self term has a zero gradient)
         const D3 gw = d3(poly6_gradient(dd, s.h));
-        pd = add(pd, scale(gw, rho_a * static_cast<double>(s.m[j]) + jterms.x * mi));
+        pd = add(pd, scale(gw, rho_a * static_cast<double>(r.m) + jterms.x * mi));
         md += jterms.x * static_cast<double>(poly6(dd, s.h));
         if (j == i) return;
         const D3 gr = d3(spiky_gradient(dd, s.h, s.spiky));
         const double gr2 = dot(gr, gr);
         // i as the centre: g_bar = 2 D_bar c (c g + S), c_bar = 2 D_bar (c |g|^2 + g . S)
-        const double cf = static_cast<double>(s.m[j]) * inv_rho0;
+        const double cf = static_cast<double>(r.m) * inv_rho0;
         const D3 gbar = scale(add(scale(gr, cf), sgi), 2.0 * dbar * cf);
         const double cbar = 2.0 * dbar * (cf * gr2 + dot(gr, sgi));
         rcon -= cbar * cf * inv_rho0;
         // i as a neighbour of j: gradient -gr
-        const double inv_jrho0 = s.inv_rho0[j];
-        const D3 sgj = d3(load(a.gsum, j));
+        const double inv_jrho0 = r.inv_rho0;
+        const D3 sgj = d3(r.gsum);
         const double rcf = mi * inv_jrho0;
         const D3 rgbar = scale(sub(sgj, scale(gr, rcf)), 2.0 * jterms.y * rcf);
         const double rcbar = 2.0 * jterms.y * (rcf * gr2 - dot(gr, sgj));
