@@ -126,6 +126,7 @@ DfsphSolver::DfsphSolver(const phx_domain_configuration* domain, const phx_dfsph
     const uint32_t n = domain_.config.particle_count;
     for (DeviceBuffer<float4>* b : {&xm_, &vmu_, &xs_, &adiv_, &xt_div_, &xt_den_}) b->resize(n);
     mass_state_.resize(2u);
+    fill_u32(stream_, mass_state_.data(), 2u, 0u);
     u_.resize(n);
     for (DeviceBuffer<float>* b : {&rho0_, &relax_div_, &relax_den_, &p_div_, &p_den_, &t_div_, &t_den_}) b->resize(n);
     // one displacement slot per predicting pass: the first, then one per
@@ -143,6 +144,7 @@ DfsphSolver::DfsphSolver(const phx_domain_configuration* domain, const phx_dfsph
         tol_partials_.resize(static_cast<size_t>(pass_blocks(n)) * kWaves);  // one per projection wave
         tol_done_.resize(2u);
         PHX_HIP(hipHostMalloc(reinterpret_cast<void**>(&tol_host_), 4u * sizeof(uint32_t), hipHostMallocDefault));
+        PHX_HIP(hipHostGetDevicePointer(reinterpret_cast<void**>(&tol_host_device_), tol_host_, 0u));
         for (hipEvent_t& e : tol_events_) PHX_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     }
 }
@@ -236,9 +238,11 @@ void DfsphSolver::forward_step(const phx_sph_dfsph_state& state, const phx_contr
         }
         // boundary samples: every pass walks the full stencil
         masks_.begin_step(stream_, boundary_.count() > 0u);
-        fill_u32(stream_, mass_state_.data(), 1u, 1u);
-        mass_state_kernel<<<blocks_for(n, kBlock), kBlock, 0, stream_>>>(n, xm, mass_state_.data());
-        PHX_LAUNCH_CHECK();
+        // the mask pass marks unequal masses with this step's stamp
+        if (++mass_stamp_ == 0u) {
+            fill_u32(stream_, mass_state_.data(), 1u, 0u);
+            mass_stamp_ = 1u;
+        }
         if (cell_skins) {
             // 1.2 x the previous step's largest displacement in the cell plus
             // 0.01 h: on the dam break at step 100 this keeps 1.28 x the
@@ -274,6 +278,8 @@ void DfsphSolver::forward_step(const phx_sph_dfsph_state& state, const phx_contr
     if (cell_skins) ma.cell_bound = cell_bound_.data();
     ma.xt_div = xt_div_.data();
     ma.xt_den = xt_den_.data();
+    ma.mass_state = mass_state_.data();
+    ma.mass_stamp = mass_stamp_;
     launch_mask_density<false>(profiler_, stream_, blocks, ma, kDfsphDensity);
 
     ForceArgs fa{};
@@ -307,6 +313,7 @@ void DfsphSolver::forward_step(const phx_sph_dfsph_state& state, const phx_contr
     }
     if (!rec) fa.moved_out = moved_.data();
     fa.mass_state = mass_state_.data();
+    fa.mass_stamp = mass_stamp_;
     // prediction `slot` writes x* (recorded per slot) and, recorded, its collision bits
     auto set_slot = [&](uint32_t slot) {
         fa.slot = slot;
@@ -392,10 +399,9 @@ void DfsphSolver::forward_step(const phx_sph_dfsph_state& state, const phx_contr
                     launch_projection(profiler_, stream_, blocks, pa, kDfsphProjectionDensity, kProjectDensity);
                 }
                 converge_kernel<<<1, kConvergeBlock, 0, stream_>>>(tol_partials_.data(), blocks * kWaves, n,
-                                                                   tolerance, it, max_it, done);
+                                                                   tolerance, it, max_it, done,
+                                                                   tol_host_device_ + 2u * phase + (it & 1u));
                 PHX_HIP(hipGetLastError());
-                uint32_t* host = tol_host_ + 2u * phase + (it & 1u);
-                PHX_HIP(hipMemcpyAsync(host, done, sizeof(uint32_t), hipMemcpyDeviceToHost, stream_));
                 PHX_HIP(hipEventRecord(tol_events_[it & 1u], stream_));
                 set_slot(first_slot + it);
                 fa.done = done;
@@ -413,7 +419,8 @@ void DfsphSolver::forward_step(const phx_sph_dfsph_state& state, const phx_contr
                 if (it < max_it) launch_iteration(it + 1u);
                 PHX_HIP(hipEventSynchronize(tol_events_[it & 1u]));
                 // the projection of iteration max_it always ends the phase
-                if (tol_host_[2u * phase + (it & 1u)] != 0u || it >= max_it) break;
+                const volatile uint32_t* flag = tol_host_ + 2u * phase + (it & 1u);
+                if (*flag != 0u || it >= max_it) break;
             }
             executed[phase] = it;
             slot = first_slot + it;  // the prediction of the pass that ended the phase

@@ -135,6 +135,7 @@ class DfsphSolver {
     DeviceBuffer<double> tol_partials_;
     DeviceBuffer<uint32_t> tol_done_;
     uint32_t* tol_host_ = nullptr;
+    uint32_t* tol_host_device_ = nullptr;  // tol_host_ as the device addresses it
     hipEvent_t tol_events_[2] = {nullptr, nullptr};
     PassProfiler profiler_;
     // sorted-order workspace
@@ -154,6 +155,7 @@ class DfsphSolver {
     // step's mass state (all masses equal, their bits)
     DeviceBuffer<float4> xt_div_, xt_den_;
     DeviceBuffer<uint32_t> mass_state_;
+    uint32_t mass_stamp_ = 0u;  // this step's mark of unequal masses in mass_state_[0]
     BoundarySamples boundary_;  // the domain's boundary samples on the device
     DfsphDerivativeWorkspace dw_;
 };

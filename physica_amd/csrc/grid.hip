@@ -192,12 +192,21 @@ __global__ void __launch_bounds__(kBlock) copy_u32_kernel(const uint32_t* src, u
     if (i < count) dst[i] = src[i];
 }
 
+// Every channel is loaded before the first is stored: the stores could alias
+// the sources as far as the compiler knows, so a loop that stores each
+// channel after its load waits out each scattered load in turn.
 __global__ void __launch_bounds__(kBlock)
     scatter_channels_kernel(uint32_t n, const uint32_t* perm, Channels channels) {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
     const uint32_t i = perm[s];
-    for (uint32_t c = 0; c < channels.count; ++c) channels.dst[c][i] = channels.src[c][s * channels.stride[c]];
+    float v[kMaxChannels];
+#pragma unroll
+    for (int c = 0; c < kMaxChannels; ++c)
+        if (static_cast<uint32_t>(c) < channels.count) v[c] = channels.src[c][s * channels.stride[c]];
+#pragma unroll
+    for (int c = 0; c < kMaxChannels; ++c)
+        if (static_cast<uint32_t>(c) < channels.count) channels.dst[c][i] = v[c];
 }
 
 __global__ void __launch_bounds__(kBlock)
@@ -205,7 +214,13 @@ __global__ void __launch_bounds__(kBlock)
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
     const uint32_t i = perm[s];
-    for (uint32_t c = 0; c < channels.count; ++c) channels.dst[c][s * channels.stride[c]] = channels.src[c][i];
+    float v[kMaxChannels];
+#pragma unroll
+    for (int c = 0; c < kMaxChannels; ++c)
+        if (static_cast<uint32_t>(c) < channels.count) v[c] = channels.src[c][i];
+#pragma unroll
+    for (int c = 0; c < kMaxChannels; ++c)
+        if (static_cast<uint32_t>(c) < channels.count) channels.dst[c][s * channels.stride[c]] = v[c];
 }
 
 // ------------------------------------------------------------------- scan

@@ -31,12 +31,16 @@ class PinnedBuffer {
         if (ptr_ != nullptr) (void)hipHostFree(ptr_);
         ptr_ = nullptr;
         PHX_HIP(hipHostMalloc(&ptr_, bytes, hipHostMallocDefault));
+        PHX_HIP(hipHostGetDevicePointer(&device_, ptr_, 0u));
         bytes_ = bytes;
     }
     void* data() { return ptr_; }
+    // the same memory as the device addresses it
+    void* device() { return device_; }
 
   private:
     void* ptr_ = nullptr;
+    void* device_ = nullptr;
     size_t bytes_ = 0;
 };
 

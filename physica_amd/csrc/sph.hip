@@ -222,6 +222,7 @@ SphSolver::SphSolver(const phx_domain_configuration* domain, const phx_sph_confi
     if (mode == PHX_MODE_FORWARD && config_.method != PHX_SPH_WCSPH) {
         xt_.resize(n);
         mass_state_.resize(2u);
+        fill_u32(stream_, mass_state_.data(), 2u, 0u);
         for (DeviceBuffer<float>* b : {&moved_, &moved_sorted_}) b->resize(n);
         cell_bound_.resize(domain_.cells);
         taint_.resize(domain_.cells);
@@ -482,9 +483,11 @@ void SphSolver::forward_pressure(const phx_sph_state& state, const phx_control& 
                                  &boundary_);
         masks_.begin_step(stream_, with_boundary);
         fill_f32(stream_, cache.pressures.data(), n, 0.0F);  // clear_iteration (sph.cpp:298)
-        fill_u32(stream_, mass_state_.data(), 1u, 1u);
-        mass_state_kernel<<<blocks_for(n, kBlock), kBlock, 0, stream_>>>(n, xm_.data(), mass_state_.data());
-        PHX_LAUNCH_CHECK();
+        // the mask pass marks unequal masses with this step's stamp
+        if (++mass_stamp_ == 0u) {
+            fill_u32(stream_, mass_state_.data(), 1u, 0u);
+            mass_stamp_ = 1u;
+        }
         if (cell_skins) {
             const uint32_t cells = domain_.cells;
             cell_bound_kernel<<<blocks_for(cells, kBlock), kBlock, 0, stream_>>>(
@@ -511,6 +514,8 @@ void SphSolver::forward_pressure(const phx_sph_state& state, const phx_control& 
     ma.flags = masks_.flags();
     ma.crowded = masks_.crowded();
     if (cell_skins) ma.cell_bound = cell_bound_.data();
+    ma.mass_state = mass_state_.data();
+    ma.mass_stamp = mass_stamp_;
     launch_mask_density<false>(profiler_, stream_, blocks, ma, kSphDensity);
 
     ForceArgs fa{};
@@ -538,6 +543,7 @@ void SphSolver::forward_pressure(const phx_sph_state& state, const phx_control& 
     fa.moved_out = moved_.data();
     fa.xt1 = xt_.data();
     fa.mass_state = mass_state_.data();
+    fa.mass_stamp = mass_stamp_;
     if (cell_skins) {
         fa.cell_bound = cell_bound_.data();
         fa.taint = taint_.data();
@@ -590,7 +596,8 @@ void SphSolver::forward_pressure(const phx_sph_state& state, const phx_control& 
         const uint32_t max_it = config_.max_iterations;
         const double tolerance = static_cast<double>(config_.density_error_tolerance);
         uint32_t* done = tol_done_.data();
-        auto* host = static_cast<uint32_t*>(tol_host_.data());
+        const volatile uint32_t* host = static_cast<uint32_t*>(tol_host_.data());
+        auto* host_device = static_cast<uint32_t*>(tol_host_.device());
         fill_u32(stream_, done, 1u, 0u);
         set_force_slot(0u);
         fa.final_step = 0u;
@@ -600,9 +607,8 @@ void SphSolver::forward_pressure(const phx_sph_state& state, const phx_control& 
             pa.cv = Convergence{tol_partials_.data(), done, rho0_.data()};
             launch_projection(profiler_, stream_, blocks, pa, kSphProjection, kProjectDensity);
             converge_kernel<<<1, kConvergeBlock, 0, stream_>>>(tol_partials_.data(), blocks * kWaves, n, tolerance,
-                                                               it, max_it, done);
+                                                               it, max_it, done, host_device + (it & 1u));
             PHX_HIP(hipGetLastError());
-            PHX_HIP(hipMemcpyAsync(host + (it & 1u), done, sizeof(uint32_t), hipMemcpyDeviceToHost, stream_));
             PHX_HIP(hipEventRecord(tol_events_[it & 1u], stream_));
             set_force_slot(it);
             fa.done = done;

@@ -140,6 +140,7 @@ class SphSolver {
     // the force pass's candidate records (x, y, z, t) and the step's mass state
     DeviceBuffer<float4> xt_;
     DeviceBuffer<uint32_t> mass_state_;
+    uint32_t mass_stamp_ = 0u;  // this step's mark of unequal masses in mass_state_[0]
     // sorted-order workspace
     DeviceBuffer<float4> xm_, vmu_, xs_, aux_;
     VecBuffer u_;

@@ -80,6 +80,11 @@ struct MaskDensityArgs {
     // candidate records of the force passes that read these t
     float4* xt_div = nullptr;
     float4* xt_den = nullptr;
+    // optional: the step's mass state (see ForceArgs::mass_state): [1] the
+    // bits of sorted particle 0's mass; [0] set to mass_stamp by every
+    // particle whose mass has other bits
+    uint32_t* mass_state = nullptr;
+    uint32_t mass_stamp = 0u;
 };
 
 // Mask build, then density_forward_kernel (density.cu:18-40, cubic kernel,
@@ -100,7 +105,13 @@ __global__ void __launch_bounds__(kBlock) mask_density_kernel(MaskDensityArgs a)
     const uint32_t i = xcd_block() * kBlock + threadIdx.x;
     if (i >= a.n) return;
     uint16_t* column = smask + threadIdx.x;
-    const Float3 p = xyz(a.xm[i]);
+    const float4 self = a.xm[i];
+    const Float3 p = xyz(self);
+    if (a.mass_state != nullptr) {
+        const uint32_t bits = __float_as_uint(self.w);
+        if (i == 0u) a.mass_state[1] = bits;
+        if (bits != __float_as_uint(a.xm[0].w)) a.mass_state[0] = a.mass_stamp;  // one value for every writer
+    }
     int cx, cy, cz;
     particle_cell(a.grid, p, cx, cy, cz);
     uint32_t nonzero = 0u;
@@ -192,22 +203,37 @@ __device__ __forceinline__ double block_sum(double value) {
 
 constexpr uint32_t kConvergeBlock = 1024;
 
+// The value *done has after this iteration, for the host: a store to
+// host-visible (pinned) memory at system scope.
+__device__ __forceinline__ void publish_done(uint32_t* host_flag, uint32_t value) {
+    __hip_atomic_store(host_flag, value, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 // Ends a tolerance-mode phase: sums the projection's per-wave error sums in
 // a fixed order (thread t takes partials t, t + 1024, ...), and
 // records the iteration in *done when the mean error is within the
 // tolerance or the iteration is the last allowed. One workgroup; the kernel
 // boundaries order it after the projection and before the force pass, so
-// no atomics or fences are needed.
+// no atomics or fences are needed on the device. The resulting *done is also
+// stored to the host's flag of this iteration (pinned memory mapped for the
+// device), which the host reads once the iteration's event has completed,
+// with no copy in between.
 [[maybe_unused]] __global__ void __launch_bounds__(kConvergeBlock)
     converge_kernel(const double* partials, uint32_t count, uint32_t n, double tolerance, uint32_t iteration,
-                    uint32_t max_iterations, uint32_t* done) {
-    if (*done != 0u) return;  // the phase ended at an earlier iteration
+                    uint32_t max_iterations, uint32_t* done, uint32_t* host_flag) {
+    const uint32_t ended = *done;
+    if (ended != 0u) {  // the phase ended at an earlier iteration
+        if (threadIdx.x == 0u) publish_done(host_flag, ended);
+        return;
+    }
     double sum = 0.0;
     for (uint32_t k = threadIdx.x; k < count; k += kConvergeBlock) sum += partials[k];
     const double total = block_sum<kConvergeBlock>(sum);
     if (threadIdx.x == 0u) {
         const double mean = total / static_cast<double>(n);
-        if (mean <= tolerance || iteration >= max_iterations) *done = iteration;
+        const uint32_t now = mean <= tolerance || iteration >= max_iterations ? iteration : 0u;
+        if (now != 0u) *done = now;
+        publish_done(host_flag, now);
     }
 }
 
@@ -368,23 +394,14 @@ struct ForceArgs {
     uint32_t stamp = 0u;
     // optional: |x_next - x| per particle (original order), written with the next state
     float* moved_out = nullptr;
-    // optional: (x, y, z, t1) per particle and the step's mass state {all
-    // masses equal (1/0), their value's bits}: with equal masses a candidate
-    // is one 16-byte record instead of its position, mass and t1
+    // optional: (x, y, z, t1) per particle and the step's mass state {
+    // mass_stamp if some mass differs from particle 0's, particle 0's mass
+    // bits} (written by the mask pass): with equal masses a candidate is one
+    // 16-byte record instead of its position, mass and t1
     const float4* xt1 = nullptr;
     const uint32_t* mass_state = nullptr;
+    uint32_t mass_stamp = 0u;
 };
-
-// mass_state of a step: [0] = 1 while every sorted particle's mass has the
-// bits of particle 0's, [1] = those bits. [0] is set to 1 beforehand.
-[[maybe_unused]] __global__ void __launch_bounds__(kBlock)
-    mass_state_kernel(uint32_t n, const float4* __restrict__ xm, uint32_t* state) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const uint32_t bits = __float_as_uint(xm[i].w);
-    if (i == 0u) state[1] = bits;
-    if (bits != __float_as_uint(xm[0].w)) state[0] = 0u;
-}
 
 // Pressure acceleration (pressure_forward_kernel, common.cu:165-194),
 // optionally fused with the non-pressure acceleration
@@ -430,7 +447,7 @@ __global__ void __launch_bounds__(kBlock) force_kernel(ForceArgs a) {
         int cx, cy, cz;
         particle_cell(a.grid, p, cx, cy, cz);
         // candidates as (x, y, z, t1) records with one mass (uniform over the launch)
-        const bool packed = kPressure && a.xt1 != nullptr && a.mass_state[0] != 0u;
+        const bool packed = kPressure && a.xt1 != nullptr && a.mass_state[0] != a.mass_stamp;
         const float mass = packed ? __uint_as_float(a.mass_state[1]) : 0.0F;
         auto pair = [&](uint32_t s, float4 q, float t1_s, float t2_s, Float3 d, float f) {
             const float ms = q.w;
