@@ -67,13 +67,10 @@ __device__ __forceinline__ void store(DVec f, uint32_t i, D3 v) {
 // w: H(d) w = f w + (f'(r) / r) (d . w) d, f' = -k (h - r)(h + r) / r^2; zero
 // where G is zero (r = 0 or r >= h). H is symmetric and even in d
 // (spiky_hessian_product, spiky_gradient_tangent, pbf.cu:34-54).
+// The FP64 form (reverse passes) is spiky_hessian_one_division, below.
+__device__ __forceinline__ D3 spiky_hessian_one_division(Float3 d, D3 w, double h, double k);
 __device__ __forceinline__ D3 spiky_hessian(Float3 d, D3 w, double h, double k) {
-    const D3 dd = d3(d);
-    const double r = sqrt(dot(dd, dd));
-    if (r == 0.0 || r >= h) return {0.0, 0.0, 0.0};
-    const double f = k * (h - r) * (h - r) / r;
-    const double fp = -k * (h - r) * (h + r) / (r * r);
-    return add(scale(w, f), scale(dd, fp / r * dot(dd, w)));
+    return spiky_hessian_one_division(d, w, h, k);
 }
 
 __device__ __forceinline__ Float3 spiky_hessian(Float3 d, Float3 w, float h, float k) {
@@ -96,18 +93,19 @@ struct Artificial {
 // its gradient, exponent and radius derivatives are signed zeros that no sum
 // can notice; only the strength derivative -(W / W(dq))^n needs a powf, and
 // only when `strength_wanted`.
-__device__ __forceinline__ Artificial artificial_full(Float3 d, float h, float strength, float exponent, float radius,
-                                                      bool vanishes = false, bool strength_wanted = true) {
+// `kernel` is poly6(d, h), which a pair's two directions share (d and -d
+// give the same value).
+__device__ __forceinline__ Artificial artificial_at(float kernel, Float3 d, float h, float strength, float exponent,
+                                                    float radius, bool vanishes = false, bool strength_wanted = true) {
     Artificial a{0.0F, {0.0F, 0.0F, 0.0F}, 0.0F, 0.0F, 0.0F};
-    const float reference = poly6(Float3{radius, 0.0F, 0.0F}, h);
-    const float kernel = poly6(d, h);
     if (kernel == 0.0F) return a;
-    const float ratio = kernel / reference;
     if (vanishes) {
         a.value = -strength;
-        if (strength_wanted) a.d_strength = -powf(ratio, exponent);
+        if (strength_wanted) a.d_strength = -powf(kernel / poly6(Float3{radius, 0.0F, 0.0F}, h), exponent);
         return a;
     }
+    const float reference = poly6(Float3{radius, 0.0F, 0.0F}, h);
+    const float ratio = kernel / reference;
     const float power = powf(ratio, exponent);
     a.value = -strength * power;
     a.d_strength = -power;
@@ -119,6 +117,11 @@ __device__ __forceinline__ Artificial artificial_full(Float3 d, float h, float s
     a.d_radius = -a.value * exponent * radial / reference;
     a.gradient = scale(poly6_gradient(d, h), -strength * exponent * powf(ratio, exponent - 1.0F) / reference);
     return a;
+}
+
+__device__ __forceinline__ Artificial artificial_full(Float3 d, float h, float strength, float exponent, float radius,
+                                                      bool vanishes = false, bool strength_wanted = true) {
+    return artificial_at(poly6(d, h), d, h, strength, exponent, radius, vanishes, strength_wanted);
 }
 
 // Sorted-order views of a recorded step and the sorted parameters.
@@ -347,12 +350,29 @@ struct CorrectionVjpArgs {
     double2* lambda_terms;  // (density adjoint, D_bar) per particle (written)
 };
 
-// Derivative of G(d) = f(r) d (spiky gradient) along w, with one division.
+// 1 / sqrt(x) for a positive normal x: the hardware estimate refined by two
+// Newton steps, y <- y + y (1 - x y^2) / 2, to within a few units of the
+// last place (the derivative passes compare with the FP64 oracle to 1e-10;
+// the square root and the division they replace cost about 30 instructions).
+__device__ __forceinline__ double inverse_sqrt(double x) {
+    double y = __builtin_amdgcn_rsq(x);
+#pragma unroll
+    for (int step = 0; step < 2; ++step) {
+        const double e = fma(-x * y, y, 1.0);
+        y = fma(0.5 * y, e, y);
+    }
+    return y;
+}
+
+// Derivative of G(d) = f(r) d (spiky gradient) along w, with 1 / r from
+// inverse_sqrt instead of a square root and a division.
 __device__ __forceinline__ D3 spiky_hessian_one_division(Float3 d, D3 w, double h, double k) {
     const D3 dd = d3(d);
-    const double r = sqrt(dot(dd, dd));
-    if (r == 0.0 || r >= h) return {0.0, 0.0, 0.0};
-    const double inv_r = 1.0 / r;
+    const double r2 = dot(dd, dd);
+    if (r2 == 0.0) return {0.0, 0.0, 0.0};
+    const double inv_r = inverse_sqrt(r2);
+    const double r = r2 * inv_r;
+    if (r >= h) return {0.0, 0.0, 0.0};
     const double hr = h - r;
     const double f = k * hr * hr * inv_r;
     const double fpr = -k * hr * (h + r) * inv_r * inv_r * inv_r;  // f'(r) / r
@@ -401,8 +421,9 @@ __global__ void __launch_bounds__(kBlock) pbf_vjp_correction_kernel(CorrectionVj
         const D3 gr = d3(spiky_gradient(dd, s.h, s.spiky));
         const float lam_j = r.lambda;
         // i as the centre
+        const float kernel = poly6(dd, s.h);
         const Artificial art =
-            artificial_full(dd, s.h, strength_i, exponent_i, radius_i, vanishes_i, a.strength_wanted != 0u);
+            artificial_at(kernel, dd, s.h, strength_i, exponent_i, radius_i, vanishes_i, a.strength_wanted != 0u);
         const double cf = static_cast<double>(r.m) * inv_rho0;
         const double mu = static_cast<double>(lam_i) + static_cast<double>(lam_j) + static_cast<double>(art.value);
         const double oa_gr = dot(oa, gr);
@@ -413,8 +434,8 @@ __global__ void __launch_bounds__(kBlock) pbf_vjp_correction_kernel(CorrectionVj
         econ += sa * static_cast<double>(art.d_exponent);
         qcon += sa * static_cast<double>(art.d_radius);
         // i as a neighbour of j: d_ji = -d_ij, gradient -gr
-        const Artificial rart = artificial_full(scale(dd, -1.0F), s.h, r.strength, r.exponent, r.radius,
-                                                r.vanishes != 0u, a.strength_wanted != 0u);
+        const Artificial rart = artificial_at(kernel, scale(dd, -1.0F), s.h, r.strength, r.exponent, r.radius,
+                                              r.vanishes != 0u, a.strength_wanted != 0u);
         const double inv_rrho0 = r.inv_rho0;
         const double rcf = m_i * inv_rrho0;
         const double rmu = static_cast<double>(lam_j) + static_cast<double>(lam_i) + static_cast<double>(rart.value);
