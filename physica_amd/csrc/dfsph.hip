@@ -358,9 +358,9 @@ void DfsphSolver::forward_step(const phx_sph_dfsph_state& state, const phx_contr
         // one-workgroup pass adds them in a fixed order and records the
         // ending iteration on the device; the force pass of that iteration
         // then ends the phase by itself (kDivAuto, or the final state). The
-        // host launches one iteration ahead and waits only for the previous
-        // iteration's done word, so the GPU is never idle; a launched pass
-        // past the end returns at once.
+        // host reads each iteration's done word while that iteration's
+        // force pass runs and only then launches the next iteration, so
+        // the GPU stays busy and no pass runs past the end.
         const uint32_t max_it = config_.max_iterations;
         const double tolerance = static_cast<double>(config_.density_error_tolerance);
         fill_u32(stream_, tol_done_.data(), 2u, 0u);
@@ -416,11 +416,13 @@ void DfsphSolver::forward_step(const phx_sph_dfsph_state& state, const phx_contr
             launch_iteration(1u);
             uint32_t it = 1u;
             for (;; ++it) {
-                if (it < max_it) launch_iteration(it + 1u);
+                // the event follows the converge pass, so the force pass of
+                // this iteration runs while the host reads its done word
                 PHX_HIP(hipEventSynchronize(tol_events_[it & 1u]));
                 // the projection of iteration max_it always ends the phase
                 const volatile uint32_t* flag = tol_host_ + 2u * phase + (it & 1u);
                 if (*flag != 0u || it >= max_it) break;
+                launch_iteration(it + 1u);
             }
             executed[phase] = it;
             slot = first_slot + it;  // the prediction of the pass that ended the phase

@@ -590,9 +590,9 @@ void SphSolver::forward_pressure(const phx_sph_state& state, const phx_control& 
     if (config_.density_error_tolerance > 0.0F) {
         // Tolerance mode: iterate until the projection's mean compression
         // error is within the tolerance (or max_iterations); the force pass
-        // of that iteration writes the state. Launched one iteration ahead
-        // of the host, as DfsphSolver's tolerance mode (passes of an
-        // iteration past the end return at once).
+        // of that iteration writes the state. As DfsphSolver's tolerance
+        // mode, the host reads an iteration's done word while its force
+        // pass runs, then launches the next iteration.
         const uint32_t max_it = config_.max_iterations;
         const double tolerance = static_cast<double>(config_.density_error_tolerance);
         uint32_t* done = tol_done_.data();
@@ -619,10 +619,11 @@ void SphSolver::forward_pressure(const phx_sph_state& state, const phx_control& 
         launch_iteration(1u);
         uint32_t it = 1u;
         for (;; ++it) {
-            if (it < max_it) launch_iteration(it + 1u);
+            // the force pass of this iteration runs while the host reads its done word
             PHX_HIP(hipEventSynchronize(tol_events_[it & 1u]));
             // the projection of iteration max_it always ends the step
             if (host[it & 1u] != 0u || it >= max_it) break;
+            launch_iteration(it + 1u);
         }
         last_iterations_ = it;
         pa.cv = Convergence{};
