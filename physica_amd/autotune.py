@@ -25,10 +25,10 @@ from . import scenes
 from .batch import TUNING, PbfBatch
 from .particle import PBF
 
-SKINS = (0.1, 0.2, 0.3, 0.45)
+SKINS = (0.1, 0.2, 0.3, 0.45, 0.6)
 
 
-def tune(batch_scenes, steps=20, skins=SKINS, graphs=(True, False), device="cuda"):
+def tune(batch_scenes, steps=100, skins=SKINS, graphs=(True, False), device="cuda"):
     """Returns {"candidates": [...], "best": {...}} for the given scenes."""
     candidates = []
     reference = None
@@ -55,13 +55,17 @@ def tune(batch_scenes, steps=20, skins=SKINS, graphs=(True, False), device="cuda
             print(f"autotune: skin {skin} graph {graph}: {ms:.3f} ms/step", file=sys.stderr, flush=True)
             del engine, state, control, params
     best = min(candidates, key=lambda c: c["ms_per_step"])
+    # replay wins ties (within 1 %): it spares the host a launch per pass
+    replay = [c for c in candidates if c["mask_skin"] == best["mask_skin"] and c["use_graph"]]
+    if replay and replay[0]["ms_per_step"] <= 1.01 * best["ms_per_step"]:
+        best = replay[0]
     return {"candidates": candidates, "best": {k: best[k] for k in ("mask_skin", "use_graph")}}
 
 
 def main(argv=None) -> int:
     p = argparse.ArgumentParser(description=__doc__)
     p.add_argument("--out", type=Path, default=TUNING)
-    p.add_argument("--steps", type=int, default=20)
+    p.add_argument("--steps", type=int, default=100)
     args = p.parse_args(argv)
     if not torch.cuda.is_available():
         print("autotune needs a HIP device", file=sys.stderr)

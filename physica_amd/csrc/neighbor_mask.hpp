@@ -431,7 +431,38 @@ struct MaskView {
     // per particle (sorted order): non-zero where a stencil cell holds more
     // than kMaskBits particles; null: no particle is crowded
     const uint8_t* crowded = nullptr;
+    // per cell: the largest displacement (float bits, rounded up) of its
+    // particles from the positions the masks were built at, over the position
+    // sets written so far in the step; null: only the step-wide limit applies.
+    // Where that limit fails, a particle whose own displacement plus the
+    // largest of its stencil cells' stays within pair_limit keeps its masks.
+    const uint32_t* cell_moved = nullptr;
+    float pair_limit = -1.0F;
 };
+
+// The displacement |moved| rounded up for the per-cell bounds: the relative
+// margin covers the rounding of the difference, the dot product and the
+// square root.
+__device__ __forceinline__ float displacement_bound(Float3 moved) {
+    return sqrtf(dot(moved, moved)) * (1.0F + 4e-6F);
+}
+
+// Raises cell_moved[cell] to at least `moved` for each lane's cell. The lanes
+// of one cell are consecutive in sorted order, so a segmented maximum over the
+// wave leaves one atomic per run of equal cells. Every lane of the wave calls
+// this; a lane without a particle passes cell = ~0u.
+__device__ __forceinline__ void raise_cell_bound(uint32_t* cell_moved, uint32_t cell, float moved) {
+    const int lane = static_cast<int>(__lane_id());
+    float v = moved;
+#pragma unroll
+    for (int offset = 1; offset < 64; offset <<= 1) {
+        const float up = __shfl_up(v, offset);
+        const uint32_t up_cell = __shfl_up(cell, offset);
+        if (lane >= offset && up_cell == cell) v = fmaxf(v, up);
+    }
+    const uint32_t next_cell = __shfl_down(cell, 1);
+    if (cell != ~0u && (lane == 63 || next_cell != cell)) atomicMax(cell_moved + cell, __float_as_uint(v));
+}
 
 // Whether particle i is crowded in the step of these masks (it then walks the
 // full stencil in every pass).
@@ -459,6 +490,44 @@ __device__ __forceinline__ bool masks_apply(const MaskView& m, uint32_t slot, ui
     if (m.masks == nullptr || m.flags[kFlagOverflow] != 0u) return false;
     if (m.crowded != nullptr && m.crowded[i] != 0u) return false;
     return m.exact != 0u || __uint_as_float(m.flags[kFlagDisplacement + slot]) <= m.limit2;
+}
+
+// masks_apply, and where the step-wide displacement limit fails, the
+// per-cell rule: particle i may keep its masks if own() (its displacement
+// bound) plus the largest cell_moved of its stencil stays within pair_limit.
+// A pair within h at the current positions was then within h + pair_limit at
+// the positions the masks were built at. A wave whose lanes decided both ways
+// would run the masked and the full walk one after the other, so a wave walks
+// its masks only if every lane that is not crowded may. Every lane of the
+// wave with a particle calls this.
+template <class Own>
+__device__ __forceinline__ bool masks_apply(const MaskView& m, uint32_t slot, uint32_t i, const SortedGrid& g, int cx,
+                                            int cy, int cz, Own&& own) {
+    if (m.masks == nullptr || m.flags[kFlagOverflow] != 0u) return false;
+    const bool crowded = m.crowded != nullptr && m.crowded[i] != 0u;
+    if (m.exact != 0u || __uint_as_float(m.flags[kFlagDisplacement + slot]) <= m.limit2) return !crowded;
+    if (m.cell_moved == nullptr) return false;
+    bool may = crowded;  // crowded lanes walk the full stencil, whatever the wave does
+    if (!crowded) {
+        float largest = 0.0F;
+#pragma unroll 1
+        for (int oz = -1; oz <= 1; ++oz) {
+            const int Z = cz + oz;
+            if (Z < 0 || Z >= static_cast<int>(g.cz)) continue;
+#pragma unroll 1
+            for (int oy = -1; oy <= 1; ++oy) {
+                const int Y = cy + oy;
+                if (Y < 0 || Y >= static_cast<int>(g.cy)) continue;
+                for (int ox = -1; ox <= 1; ++ox) {
+                    const int X = cx + ox;
+                    if (X < 0 || X >= static_cast<int>(g.cx)) continue;
+                    largest = fmaxf(largest, __uint_as_float(m.cell_moved[cell_of(g, X, Y, Z)]));
+                }
+            }
+        }
+        may = own() + largest <= m.pair_limit;
+    }
+    return !crowded && __all(may) != 0;
 }
 
 // Visits the candidates of particle i in the reference order: the marked
@@ -494,21 +563,27 @@ __device__ __forceinline__ bool visit_and_mask(const SortedGrid& g, const MaskVi
     };
     bool complete = true;
     if (masked) {
+        // one candidate per loop trip, as in walk_masked (a loop per slot
+        // would run, slot after slot, as many trips as the wave's fullest
+        // lane in that slot); a lane writes a slot's mask as it moves on
         uint32_t nonzero = load_lane_column(m.masks, m.n, i, column, stride);
         const uint32_t center = cell_of(g, cx, cy, cz);
-        while (nonzero != 0u) {
-            const uint32_t k = static_cast<uint32_t>(__builtin_ctz(nonzero));
-            nonzero &= nonzero - 1u;
-            uint32_t candidates = column[k * stride];
-            const uint32_t base = g.offsets[center + static_cast<uint32_t>(delta[k])];
-            uint32_t bits = 0u;
-            while (candidates != 0u) {
-                const uint32_t b = static_cast<uint32_t>(__builtin_ctz(candidates));
-                candidates &= candidates - 1u;
-                if (visit(base + b)) bits |= 1u << b;
+        constexpr uint32_t kNone = static_cast<uint32_t>(kStencil);
+        uint32_t candidates = 0u, base = 0u, bits = 0u, k = kNone;
+        while ((candidates | nonzero) != 0u) {
+            if (candidates == 0u) {
+                if (k != kNone) emit(k, bits);
+                k = static_cast<uint32_t>(__builtin_ctz(nonzero));
+                nonzero &= nonzero - 1u;
+                candidates = column[k * stride];
+                base = g.offsets[center + static_cast<uint32_t>(delta[k])];
+                bits = 0u;
             }
-            emit(k, bits);
+            const uint32_t b = static_cast<uint32_t>(__builtin_ctz(candidates));
+            candidates &= candidates - 1u;
+            if (visit(base + b)) bits |= 1u << b;
         }
+        if (k != kNone) emit(k, bits);
     } else {
         uint32_t k = 0u;
 #pragma unroll 1

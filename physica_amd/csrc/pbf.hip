@@ -82,6 +82,14 @@ __device__ __forceinline__ Lane lane_of(uint32_t n, const SortedGrid& grid, cons
     return l;
 }
 
+// The lane's mask view: in a batch the per-cell displacement bounds of the
+// lane's scene start at its cell base, like its grid's offsets.
+__device__ __forceinline__ MaskView lane_masks(const MaskView& mv, const SortedGrid& launch, const SortedGrid& lane) {
+    MaskView v = mv;
+    if (v.cell_moved != nullptr) v.cell_moved += lane.offsets - launch.offsets;
+    return v;
+}
+
 struct MaskArgs {
     uint32_t n;
     SortedGrid grid;
@@ -150,11 +158,14 @@ __global__ void __launch_bounds__(kBlock) pbf_lambda_kernel(LambdaArgs a, BatchV
     const uint32_t i = lane.i;
     if (!lane.valid) return;
     const SortedGrid& grid = lane.grid;
-    const bool masked = masks_apply(a.mv, a.slot, i);
+    const MaskView mv = lane_masks(a.mv, a.grid, grid);
     const Float3 p = load(a.pos, i);
     const float rest = a.rho0[i];
+    const Float3 x_star = load(a.topo, i);
     int cx, cy, cz;
-    particle_cell(grid, load(a.topo, i), cx, cy, cz);
+    particle_cell(grid, x_star, cx, cy, cz);
+    const bool masked =
+        masks_apply(mv, a.slot, i, grid, cx, cy, cz, [&] { return displacement_bound(sub(p, x_star)); });
     float density = 0.0F;
     Float3 gradient_sum{0.0F, 0.0F, 0.0F};
     float denominator = a.eps[i];
@@ -181,10 +192,10 @@ __global__ void __launch_bounds__(kBlock) pbf_lambda_kernel(LambdaArgs a, BatchV
             });
     } else {
         const bool complete = visit_records_and_mask<kPbfWalkDepth<kBatch>>(
-            grid, a.mv, masked, i, cx, cy, cz, delta, smask + threadIdx.x, kBlock, a.exact_out, a.n,
+            grid, mv, masked, i, cx, cy, cz, delta, smask + threadIdx.x, kBlock, a.exact_out, a.n,
             [&](uint32_t s) { return PointMass{load(a.pos, s), a.m[s]}; },
             [&](const PointMass& c, uint32_t s) { return pair(c.x, c.m, s); });
-        flag_incomplete(complete, a.mv, i, &a.exact_flags[kFlagOverflow]);
+        flag_incomplete(complete, mv, i, &a.exact_flags[kFlagOverflow]);
     }
     denominator += dot(gradient_sum, gradient_sum);
     a.rho[i] = density;
@@ -218,6 +229,7 @@ struct CorrectionArgs {
     MaskView mv;
     uint32_t slot;           // displacement slot of the positions read
     uint32_t* moved_to;      // largest squared displacement of the output from x*
+    uint32_t* cell_moved;    // optional: per-cell displacement bounds of the output (MaskView::cell_moved)
     BoundaryGrid boundary;   // offsets null: no boundary samples
 };
 
@@ -234,6 +246,8 @@ __global__ void __launch_bounds__(kBlock) pbf_correction_kernel(CorrectionArgs a
     const SortedGrid& grid = lane.grid;
     const Box& box = lane.box;
     float moved2 = 0.0F;
+    uint32_t moved_cell = ~0u;
+    float moved_bound = 0.0F;
     if (lane.valid) {
         const bool masked = masks_apply(a.mv, a.slot, i);
         const Float3 p = load(a.pos, i);
@@ -288,7 +302,12 @@ __global__ void __launch_bounds__(kBlock) pbf_correction_kernel(CorrectionArgs a
         if (a.reconstruct != 0u) store(a.vel, i, scale(sub(projected, load(a.x0, i)), a.inv_dt));
         const Float3 moved = sub(projected, load(a.topo, i));
         moved2 = dot(moved, moved);
+        if (a.cell_moved != nullptr) {
+            moved_cell = static_cast<uint32_t>(grid.offsets - a.grid.offsets) + cell_of(grid, cx, cy, cz);
+            moved_bound = displacement_bound(moved);
+        }
     }
+    if (a.cell_moved != nullptr) raise_cell_bound(a.cell_moved, moved_cell, moved_bound);
     block_max_to<kBlock>(moved2, a.moved_to);
 }
 
@@ -332,11 +351,14 @@ __global__ void __launch_bounds__(kBlock) pbf_vorticity_kernel(PostArgs a, Batch
     const uint32_t i = lane.i;
     if (!lane.valid) return;
     const SortedGrid& grid = lane.grid;
-    const bool masked = masks_apply(a.mv, a.slot, i);
+    const MaskView mv = lane_masks(a.mv, a.grid, grid);
     const Float3 p = load(a.pos, i);
     const Float3 v = load(a.vel, i);
+    const Float3 x_star = load(a.topo, i);
     int cx, cy, cz;
-    particle_cell(grid, load(a.topo, i), cx, cy, cz);
+    particle_cell(grid, x_star, cx, cy, cz);
+    const bool masked =
+        masks_apply(mv, a.slot, i, grid, cx, cy, cz, [&] { return displacement_bound(sub(p, x_star)); });
     Float3 w{0.0F, 0.0F, 0.0F};
     auto pair = [&](Float3 q, Float3 vq, uint32_t s) {
         const Float3 d = sub(p, q);
@@ -344,10 +366,10 @@ __global__ void __launch_bounds__(kBlock) pbf_vorticity_kernel(PostArgs a, Batch
         return dot(d, d) < a.t_exact;
     };
     const bool complete = visit_records_and_mask<kPbfWalkDepth<kBatch>>(
-        grid, a.mv, masked, i, cx, cy, cz, delta, smask + threadIdx.x, kBlock, a.exact_out, a.n,
+        grid, mv, masked, i, cx, cy, cz, delta, smask + threadIdx.x, kBlock, a.exact_out, a.n,
         [&](uint32_t s) { return PointVector{load(a.pos, s), load(a.vel, s)}; },
         [&](const PointVector& c, uint32_t s) { return pair(c.x, c.v, s); });
-    flag_incomplete(complete, a.mv, i, &a.exact_flags[kFlagOverflow]);
+    flag_incomplete(complete, mv, i, &a.exact_flags[kFlagOverflow]);
     store(a.vort, i, w);
     a.magnitude[i] = length(w);
 }
@@ -564,6 +586,13 @@ void PbfSolver::forward_step(const phx_pbf_state& state, const phx_control& cont
     }
     MaskView mv{skin, masks_.flags(), n, masks_.limit2()};
     mv.crowded = crowded;
+    // per-cell displacement bounds of the iterates, for the passes that read
+    // skin masks where the step-wide limit fails
+    const size_t cells = static_cast<size_t>(grid.cx) * grid.cy * grid.cz;
+    if (cell_moved_.size() < cells) cell_moved_.resize(cells);
+    fill_u32(stream_, cell_moved_.data(), cells, 0u);
+    mv.cell_moved = cell_moved_.data();
+    mv.pair_limit = masks_.pair_limit();
     // exact masks of position set k (iterate k, the final positions last):
     // written by the pass that first walks those positions, walked by the
     // later passes at them; a recorded step keeps every set. Each set has one
@@ -604,6 +633,7 @@ void PbfSolver::forward_step(const phx_pbf_state& state, const phx_control& cont
     ca.vel = vel.view();
     ca.mv = mv;
     ca.boundary = boundary;
+    ca.cell_moved = cell_moved_.data();
 
     // positions start at x* (pbf.cpp:171) and ping-pong between two buffers
     // (a recorded step keeps every iterate)
@@ -794,6 +824,7 @@ PbfBatch::PbfBatch(const phx_domain_configuration* domains, uint32_t scene_count
 
     const uint32_t n = n_;
     grid_.allocate(n, cells_);
+    cell_moved_.resize(cells_);
     for (VecBuffer* b : {&positions_[0], &positions_[1], &velocities_[0], &velocities_[1], &predicted_, &xs_, &x0_,
                          &pa_, &pb_, &vel_, &vort_, &vc_})
         b->resize(n);
@@ -805,9 +836,9 @@ PbfBatch::PbfBatch(const phx_domain_configuration* domains, uint32_t scene_count
     crowded_.resize(n);
     flags_.resize(kFlagDisplacement + config_.pressure_iterations + 1u);
     exact_flags_.resize(config_.pressure_iterations + 1u);
-    // grid build 9, flag fills 2, masks 1, two per iteration, three post passes
+    // grid build 9, flag and cell-bound fills 3, masks 1, two per iteration, three post passes
     const uint32_t it = config_.pressure_iterations;
-    kernels_per_step_ = 9u + 2u + 1u + 2u * it + (it == 0u ? 1u : 0u) + 3u;
+    kernels_per_step_ = 9u + 3u + 1u + 2u * it + (it == 0u ? 1u : 0u) + 3u;
     PHX_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
     PHX_HIP(hipEventCreateWithFlags(&ready_, hipEventDisableTiming));
     PHX_HIP(hipEventCreateWithFlags(&done_, hipEventDisableTiming));
@@ -866,7 +897,8 @@ void PbfBatch::substep(uint32_t from, uint64_t step, const phx_control& control,
     const uint32_t sets = config_.pressure_iterations + 1u;
     fill_u32(s, flags_.data(), kFlagDisplacement + sets, 0u);
     fill_u32(s, exact_flags_.data(), sets, 0u);
-    kernels_per_step_ += 2u;
+    fill_u32(s, cell_moved_.data(), cells_, 0u);
+    kernels_per_step_ += 3u;
 
     // the batch's offsets; each lane's grid starts at its scene's cell base
     const SortedGrid grid{grid_.offsets.data(), 0u, 0u, 0u, 0.0F, 0.0F, 0.0F, h_};
@@ -880,6 +912,8 @@ void PbfBatch::substep(uint32_t from, uint64_t step, const phx_control& control,
     ++kernels_per_step_;
     MaskView mv{skin_masks_.data(), flags_.data(), n, limit2};
     mv.crowded = crowded_.data();
+    mv.cell_moved = cell_moved_.data();
+    mv.pair_limit = 2.0F * limit;
     const MaskView exact_view{exact_.data(), exact_flags_.data(), n, 0.0F, 1u, crowded_.data()};
     const float t_exact = exact_mask_threshold(h_);
 
@@ -904,6 +938,7 @@ void PbfBatch::substep(uint32_t from, uint64_t step, const phx_control& control,
     ca.vel = vel_.view();
     ca.mv = exact_view;
     ca.slot = 0u;
+    ca.cell_moved = cell_moved_.data();
     // the exact masks of each set of positions are rebuilt by the pass that
     // first walks them, so one buffer serves every set in forward mode
     CVec current = xs_.cview();

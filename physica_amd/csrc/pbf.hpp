@@ -134,6 +134,7 @@ class PbfSolver {
     DeviceBuffer<uint16_t> exact_;  // exact masks of the current set of positions (forward mode)
     DeviceBuffer<uint8_t> crowded_;  // per particle: walks the full stencil (forward mode)
     DeviceBuffer<uint32_t> exact_flags_;
+    DeviceBuffer<uint32_t> cell_moved_;  // per grid cell: displacement bound of the step (MaskView::cell_moved)
     PbfDerivativeWorkspace dw_;
 };
 
@@ -186,6 +187,7 @@ class PbfBatch {
     DeviceBuffer<uint16_t> skin_masks_, exact_;
     DeviceBuffer<uint8_t> crowded_;
     DeviceBuffer<uint32_t> flags_, exact_flags_;
+    DeviceBuffer<uint32_t> cell_moved_;  // per cell: displacement bound of the substep (MaskView::cell_moved)
     // captured substeps 0 -> 1 and 1 -> 0, and the control and parameter
     // arrays they read
     hipGraphExec_t graph_[2] = {nullptr, nullptr};

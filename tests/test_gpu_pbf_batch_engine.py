@@ -63,8 +63,9 @@ def test_batch_offsets_and_launches(gpu_device):
     assert engine.offsets == list(np.cumsum([0] + [sc.count for sc in batch]))
     engine.run(state, control, params, 1)
     # counts fill, key, 3 scan kernels, scatter, rank, big-cell sort, gather,
-    # 2 flag fills, mask, 2 per iteration, vorticity, normal, XSPH
-    assert engine.kernels_per_step == 9 + 2 + 1 + 2 * ITERATIONS + 3
+    # 2 flag fills, the cell-bound fill, mask, 2 per iteration, vorticity,
+    # normal, XSPH
+    assert engine.kernels_per_step == 9 + 3 + 1 + 2 * ITERATIONS + 3
     assert state.particles.step_index == 1
 
 
