@@ -92,10 +92,4 @@ float MaskSet::limit2() const {
     return limit > 0.0F ? limit * limit : -1.0F;
 }
 
-float MaskSet::pair_limit() const {
-    const float radius = h_ * (1.0F + skin_);
-    const float limit = 0.5F * (radius - h_ * (1.0F + 2e-5F));
-    return limit > 0.0F ? 2.0F * limit : -1.0F;
-}
-
 }  // namespace phx

@@ -65,9 +65,6 @@ class MaskSet {
     // moved at most sqrt(limit2) (negative: never).
     float t_skin() const;
     float limit2() const;
-    // the sum of two particles' displacements a pass may allow (twice the
-    // limit; negative: never)
-    float pair_limit() const;
     float skin() const { return skin_; }
     uint32_t flag_count() const { return flag_count_; }
 

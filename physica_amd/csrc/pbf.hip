@@ -584,15 +584,11 @@ void PbfSolver::forward_step(const phx_pbf_state& state, const phx_control& cont
         pbf_mask_kernel<false><<<blocks, kBlock, 0, stream_>>>(ma, BatchView{});
         PHX_LAUNCH_CHECK();
     }
+    // the skin adapts to the largest displacement of the previous step, so
+    // the step-wide limit holds nearly always: no per-cell bounds (the
+    // batch's fixed skin uses them)
     MaskView mv{skin, masks_.flags(), n, masks_.limit2()};
     mv.crowded = crowded;
-    // per-cell displacement bounds of the iterates, for the passes that read
-    // skin masks where the step-wide limit fails
-    const size_t cells = static_cast<size_t>(grid.cx) * grid.cy * grid.cz;
-    if (cell_moved_.size() < cells) cell_moved_.resize(cells);
-    fill_u32(stream_, cell_moved_.data(), cells, 0u);
-    mv.cell_moved = cell_moved_.data();
-    mv.pair_limit = masks_.pair_limit();
     // exact masks of position set k (iterate k, the final positions last):
     // written by the pass that first walks those positions, walked by the
     // later passes at them; a recorded step keeps every set. Each set has one
@@ -633,7 +629,6 @@ void PbfSolver::forward_step(const phx_pbf_state& state, const phx_control& cont
     ca.vel = vel.view();
     ca.mv = mv;
     ca.boundary = boundary;
-    ca.cell_moved = cell_moved_.data();
 
     // positions start at x* (pbf.cpp:171) and ping-pong between two buffers
     // (a recorded step keeps every iterate)

@@ -134,7 +134,6 @@ class PbfSolver {
     DeviceBuffer<uint16_t> exact_;  // exact masks of the current set of positions (forward mode)
     DeviceBuffer<uint8_t> crowded_;  // per particle: walks the full stencil (forward mode)
     DeviceBuffer<uint32_t> exact_flags_;
-    DeviceBuffer<uint32_t> cell_moved_;  // per grid cell: displacement bound of the step (MaskView::cell_moved)
     PbfDerivativeWorkspace dw_;
 };
 
