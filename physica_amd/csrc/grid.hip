@@ -65,22 +65,48 @@ __device__ __forceinline__ Float3 moved_sample(const BoundarySource& src, float 
                   src.x[2][i] + time * src.v[2][i]};
 }
 
+// Adds one to counts[key] for each lane with a particle and returns the
+// lane's slot: the cell's count before its own addition (unique within the
+// cell, in no particular order; the in-cell rank restores the reference's).
+// Consecutive lanes with equal keys, common where the particle order follows
+// space, take one atomic per run. Every lane of the wave calls this; a lane
+// without a particle passes key = ~0u, which no cell has.
+__device__ __forceinline__ uint32_t count_in_cell(uint32_t* counts, uint32_t key) {
+    const uint32_t lane = __lane_id();
+    const uint32_t previous = __shfl_up(key, 1);
+    const bool head = lane == 0u || previous != key;
+    const uint64_t heads = __ballot(head);
+    const uint64_t upto = lane == 63u ? ~0ull : (2ull << lane) - 1ull;
+    const uint32_t first = 63u - static_cast<uint32_t>(__builtin_clzll(heads & upto));  // lane 0 is a head
+    uint32_t base = 0u;
+    if (head && key != ~0u) {
+        const uint64_t above = lane == 63u ? 0ull : heads >> (lane + 1u);
+        const uint32_t run = above != 0ull ? static_cast<uint32_t>(__builtin_ctzll(above)) + 1u : 64u - lane;
+        base = atomicAdd(&counts[key], run);
+    }
+    return __shfl(base, static_cast<int>(first)) + (lane - first);
+}
+
 // key_kernel, neighborhood.cu:20-39, fused with the cell histogram.
 template <bool kPredict>
 __global__ void __launch_bounds__(kBlock)
     key_count_kernel(uint32_t n, KeyBox box, CVec positions, PredictInput predict, uint32_t* keys, uint32_t* slots,
                      uint32_t* counts) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    Float3 p = load(positions, i);
-    if constexpr (kPredict) {
-        const Float3 pv = add(load(predict.velocities, i), scale(add(predict.gravity, load(predict.controls, i)), predict.dt));
-        p = add(p, scale(pv, predict.dt));
-        store(predict.predicted, i, p);
+    uint32_t key = ~0u;
+    if (i < n) {
+        Float3 p = load(positions, i);
+        if constexpr (kPredict) {
+            const Float3 pv =
+                add(load(predict.velocities, i), scale(add(predict.gravity, load(predict.controls, i)), predict.dt));
+            p = add(p, scale(pv, predict.dt));
+            store(predict.predicted, i, p);
+        }
+        key = cell_key(box, p);
+        keys[i] = key;
     }
-    const uint32_t key = cell_key(box, p);
-    keys[i] = key;
-    slots[i] = atomicAdd(&counts[key], 1u);
+    const uint32_t slot = count_in_cell(counts, key);
+    if (i < n) slots[i] = slot;
 }
 
 // key_count_kernel<true> of a batch: workgroup b serves particles
@@ -90,24 +116,28 @@ __global__ void __launch_bounds__(kBlock)
     batch_key_count_kernel(BatchView batch, CVec positions, PredictInput predict, uint32_t* keys, uint32_t* slots,
                            uint32_t* counts) {
     const BatchBlock blk = batch.blocks[blockIdx.x];
-    if (threadIdx.x >= blk.count) return;
-    const BatchScene sc = batch.scenes[blk.scene];
+    const bool valid = threadIdx.x < blk.count;
     const uint32_t i = blk.first + threadIdx.x;
-    const Float3 gravity{sc.gravity[0], sc.gravity[1], sc.gravity[2]};
-    const Float3 pv = add(load(predict.velocities, i), scale(add(gravity, load(predict.controls, i)), predict.dt));
-    const Float3 p = add(load(positions, i), scale(pv, predict.dt));
-    store(predict.predicted, i, p);
-    KeyBox box{};
-    for (int a = 0; a < 3; ++a) {
-        box.minimum[a] = sc.minimum[a];
-        box.velocity[a] = sc.velocity[a];
-        box.res[a] = sc.res[a];
+    uint32_t key = ~0u;
+    if (valid) {
+        const BatchScene sc = batch.scenes[blk.scene];
+        const Float3 gravity{sc.gravity[0], sc.gravity[1], sc.gravity[2]};
+        const Float3 pv = add(load(predict.velocities, i), scale(add(gravity, load(predict.controls, i)), predict.dt));
+        const Float3 p = add(load(positions, i), scale(pv, predict.dt));
+        store(predict.predicted, i, p);
+        KeyBox box{};
+        for (int a = 0; a < 3; ++a) {
+            box.minimum[a] = sc.minimum[a];
+            box.velocity[a] = sc.velocity[a];
+            box.res[a] = sc.res[a];
+        }
+        box.h = batch.h;
+        box.time = batch.time;
+        key = sc.cell_base + cell_key(box, p);
+        keys[i] = key;
     }
-    box.h = batch.h;
-    box.time = batch.time;
-    const uint32_t key = sc.cell_base + cell_key(box, p);
-    keys[i] = key;
-    slots[i] = atomicAdd(&counts[key], 1u);
+    const uint32_t slot = count_in_cell(counts, key);
+    if (valid) slots[i] = slot;
 }
 
 // key_kernel with moving_positions (neighborhood.cu:20-39 as called at
