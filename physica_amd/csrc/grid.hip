@@ -146,11 +146,13 @@ __global__ void __launch_bounds__(kBlock)
     boundary_key_count_kernel(uint32_t count, KeyBox box, BoundarySource src, uint32_t* keys, uint32_t* slots,
                               uint32_t* counts) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= count) return;
-    const Float3 p = moved_sample(src, box.time, i);
-    const uint32_t key = cell_key(box, p);
-    keys[i] = key;
-    slots[i] = atomicAdd(&counts[key], 1u);
+    uint32_t key = ~0u;
+    if (i < count) {
+        key = cell_key(box, moved_sample(src, box.time, i));
+        keys[i] = key;
+    }
+    const uint32_t slot = count_in_cell(counts, key);
+    if (i < count) slots[i] = slot;
 }
 
 // Stable rank of each sample within its cell (as rank_gather_kernel), then
